@@ -1,0 +1,289 @@
+/*
+ * rhodst.h -- C-ABI of the MI355X block-coupled implicit step.
+ *
+ * The entry points below replace the run-time-selected pieces that
+ * rhoDST's Single::iterate calls (libraries/finiteVolume/iteration/Single/Single.C:80-104):
+ *
+ *   baseFluxDST::computeFluxDST + calcFields      -> rdst_flux_residual
+ *     (fluxSchemes/baseFluxDST/baseFluxDST.H:176, baseFluxDST.C:223-255,
+ *      selector newbaseFluxDST.C:49-85)
+ *   JacobianDST<tensor5>::computeJacobianDST       -> rdst_jacobian_assemble
+ *   + fvmDST::ddt + fluxDST::integrate
+ *     (jacobian/Roe/Roe.C:152-379, fvmDST/fvmDST.C:59-143,
+ *      timeIntegration/fluxDST/fluxDST/fluxDST.C:73-127)
+ *   fvBlockMatrixDST<vector5>::solve               -> rdst_block_solve
+ *     (rhoDST/fvBlockMatrixDST/fvBlockMatrixDST.C:98-155; the foam-extend
+ *      BlockLduSolver<vector5> GMRES/BiCGStab + ILUC0 it selects)
+ *   #include "updateFields.H" + BC updateCoeffs    -> rdst_update_fields
+ *     (libraries/include/updateFields.H:40-94,
+ *      boundaryConditions/farFieldDST/ * /updateCoeffs)
+ *   timeDST::setCourantDST / setSteadyStateDeltaT  -> rdst_set_courant
+ *     (rhoDST/timeDST/timeDST.C:218-273, :316-443)
+ *   the whole Single::iterate                     -> rdst_iterate
+ *
+ * Conventions: every call returns 0 on success and a nonzero RDST_ERR_*
+ * code otherwise; rdst_last_error() gives the message (the reference
+ * aborts through FatalError instead, e.g. newbaseFluxDST.C:72-82).
+ * Host arrays are caller-owned and only read during the call.  Device
+ * state lives behind the opaque handles.  Calls are synchronous on return.
+ * A handle is used from one host thread.
+ *
+ * Component order of the 5-vector W is (rho, rhoUx, rhoUy, rhoUz, rhoE)
+ * (rhoDST/geometricFieldDST/geometricFieldDST.C:156-158); 5x5 blocks are
+ * row-major, entry (r,c) at 5*r+c (rhoDST/DST/DST.C:240-242).
+ */
+#ifndef RHODST_H
+#define RHODST_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- codes */
+enum rdst_status {
+    RDST_OK = 0,
+    RDST_ERR_ARG = 1,          /* bad argument / size */
+    RDST_ERR_HIP = 2,          /* a HIP runtime call failed */
+    RDST_ERR_UNKNOWN_NAME = 3, /* unknown run-time-selection name */
+    RDST_ERR_STATE = 4,        /* call out of order (e.g. solve before assembly) */
+    RDST_ERR_NO_DEVICE = 5     /* no GPU visible / extension unusable */
+};
+
+/* fluxScheme names, the baseFluxDST RTS table (newbaseFluxDST.C:64-84) */
+enum rdst_flux_scheme {
+    RDST_FLUX_AUSMPLUS = 0,   /* "AUSM+"    AUSMPLDST.C:127-217   */
+    RDST_FLUX_AUSMPLUSUP = 1, /* "AUSM+up"  AUSMPLUPDST.C:209-320 */
+    RDST_FLUX_HLL = 2,        /* "HLL"      HLLDST.C:91-204       */
+    RDST_FLUX_HLLC = 3,       /* "HLLC"     HLLCDST.C:135-272     */
+    RDST_FLUX_HLLCP = 4,      /* "HLLCP"    HLLCPDST.C:166-317    */
+    RDST_FLUX_KURGANOV = 5    /* "Kurganov" KurganovDST.C:93-167  */
+};
+
+/* polyMesh patch geometric types (constant/polyMesh/boundary "type") */
+enum rdst_patch_type {
+    RDST_PATCH_GENERIC = 0,   /* patch */
+    RDST_PATCH_WALL = 1,      /* wall (timeDST.C:254 treats these specially) */
+    RDST_PATCH_SYMMETRY = 2,  /* symmetryPlane */
+    RDST_PATCH_EMPTY = 3      /* empty: fvPatch of size 0 */
+};
+
+/* boundary-condition types for p, T and U */
+enum rdst_bc_type {
+    RDST_BC_FIXED_VALUE = 0,  /* fixedValue (wall no-slip = fixedValue 0) */
+    RDST_BC_ZERO_GRADIENT = 1,
+    RDST_BC_INLET_OUTLET = 2, /* mixed, valueFraction 1-pos(phi) */
+    RDST_BC_SYMMETRY = 3,     /* symmetryPlane; for U also slip */
+    RDST_BC_FARFIELD = 4,     /* farField{Pressure,Temperature,Velocity}DST */
+    RDST_BC_EMPTY = 5
+};
+
+enum rdst_solver_kind { RDST_SOLVER_GMRES = 0, RDST_SOLVER_BICGSTAB = 1 };
+enum rdst_precond_kind { RDST_PRECOND_ILU0 = 0, RDST_PRECOND_NONE = 1, RDST_PRECOND_DIAG = 2 };
+enum rdst_dependent { RDST_DEP_W = 0, RDST_DEP_DELTAW = 1 };
+/* arithmetic of the linear solve: plain FP64, or FP32 Jacobian/ILU/Krylov
+ * inside an FP64 defect-correction loop */
+enum rdst_precision { RDST_PREC_FP64 = 0, RDST_PREC_MIXED = 1 };
+/* ordering of the block-ILU(0): natural cell order (the reference's
+ * ILUC0 factor, run level-scheduled) or multicolour (a different,
+ * more parallel preconditioner) */
+enum rdst_ilu_order { RDST_ILU_NATURAL = 0, RDST_ILU_COLOURED = 1 };
+
+typedef struct rdst_patch {
+    char name[64];
+    int32_t type;   /* rdst_patch_type */
+    int32_t start;  /* first polyMesh face of the patch */
+    int32_t size;   /* number of polyMesh faces */
+    int32_t pad;
+} rdst_patch;
+
+/* polyMesh primitives, as in constant/polyMesh/{points,faces,owner,neighbour,boundary} */
+typedef struct rdst_mesh_desc {
+    int32_t n_points;
+    int32_t n_faces;
+    int32_t n_internal_faces;
+    int32_t n_cells;
+    int32_t n_patches;
+    int32_t pad;
+    const double* points;        /* [n_points][3] */
+    const int32_t* face_offsets; /* [n_faces+1] */
+    const int32_t* face_points;  /* [face_offsets[n_faces]] */
+    const int32_t* owner;        /* [n_faces] */
+    const int32_t* neighbour;    /* [n_internal_faces] */
+    const rdst_patch* patches;   /* [n_patches] */
+} rdst_mesh_desc;
+
+/* per-patch boundary conditions for p, T and U (0/{p,T,U}) */
+typedef struct rdst_bc {
+    int32_t p_type, T_type, U_type, pad;
+    double p_value, T_value;      /* fixedValue value / inletOutlet inletValue */
+    double U_value[3];
+    double p_inf, T_inf;          /* farField*DST pInf, TInf (P.C:80-82) */
+    double U_inf[3];              /* farField*DST UInf */
+} rdst_bc;
+
+/* hPsiThermo<pureMixture<{const,sutherland}Transport<specieThermo<hConstThermo<perfectGas>>>>> */
+typedef struct rdst_thermo {
+    double Cp;      /* J/kg/K, constant */
+    double R;       /* specific gas constant RR/W */
+    int32_t transport; /* 0 = constTransport (mu), 1 = sutherlandTransport (As, Ts) */
+    int32_t pad;
+    double mu;      /* constTransport mu */
+    double Pr;      /* Prandtl number (alpha = mu/Pr) */
+    double As, Ts;  /* sutherland coefficients: mu = As*sqrt(T)/(1+Ts/T) */
+} rdst_thermo;
+
+/* system/rhoDSTDict, fvSolution.solvers.W and fvSolution.fieldBounds */
+typedef struct rdst_settings {
+    int32_t flux_scheme;      /* rdst_flux_scheme */
+    int32_t low_mach;         /* AUSM+up lowMach (AUSMPLUPDST.C:195) */
+    double Mainf;             /* AUSM+up Mainf, default 0.5 */
+    int32_t dependent;        /* rdst_dependent, default W (IOdictDST.C:147-150) */
+    int32_t neighbouring;     /* timeControls.neighbouring (IOdictDST.C:197-200) */
+    double beta;              /* timeIntegration.beta, default 1 */
+    double initial_co, min_co, max_co;
+    int32_t solver;           /* rdst_solver_kind */
+    int32_t precond;          /* rdst_precond_kind */
+    int32_t n_directions;     /* GMRES restart length */
+    int32_t min_iter;
+    int32_t max_iter;
+    int32_t precision;        /* rdst_precision */
+    double tolerance, rel_tol;
+    double rho_min, rho_max, p_min, p_max, T_min, T_max; /* fieldBounds */
+    int32_t ilu_order;        /* rdst_ilu_order */
+    int32_t viscous;          /* evaluate the viscous terms of calcFields */
+} rdst_settings;
+
+/* BlockSolverPerformance<vector5>: per-component initial/final residuals */
+typedef struct rdst_solver_perf {
+    double initial_residual[5];
+    double final_residual[5];
+    int32_t n_iterations;
+    int32_t converged;
+} rdst_solver_perf;
+
+/* fields addressable through rdst_case_get / rdst_case_put */
+enum rdst_field {
+    RDST_F_RHO = 0, RDST_F_RHOU = 1, RDST_F_RHOE = 2,   /* cells; rhoU is [n][3] */
+    RDST_F_P = 3, RDST_F_T = 4, RDST_F_U = 5, RDST_F_H = 6, RDST_F_PSI = 7,
+    RDST_F_R = 8,                 /* residual R_ [n_cells][5] */
+    RDST_F_W = 9,                 /* solution vector W [n_cells][5] */
+    RDST_F_RDELTAT = 10, RDST_F_COURANT = 11,
+    RDST_F_PHI = 12,              /* mass flux, internal then boundary faces */
+    RDST_F_PHIUP = 13,            /* [nf][3] */
+    RDST_F_PHIEP = 14,
+    RDST_F_UP = 15,               /* [nf][3] */
+    RDST_F_BRHO = 16, RDST_F_BU = 17, RDST_F_BP = 18, RDST_F_BT = 19, /* boundary faces */
+    RDST_F_BRHOU = 20, RDST_F_BRHOE = 21, RDST_F_BH = 22, RDST_F_BPSI = 23,
+    RDST_F_BCSR_VALUES = 24,      /* assembled FP64 blocks [nnzb][25] */
+    RDST_F_SOURCE = 25,           /* assembled source [n_cells][5] */
+    RDST_F_MU = 26,
+    RDST_F_RHO_OLD = 27, RDST_F_E_OLD = 28
+};
+
+/* integer arrays of a mesh */
+enum rdst_mesh_array {
+    RDST_M_OWNER = 0,       /* [n_internal + n_bfaces]: internal owners, then boundary face cells */
+    RDST_M_NEIGHBOUR = 1,   /* [n_internal] */
+    RDST_M_BCSR_ROWPTR = 2, /* [n_cells+1] */
+    RDST_M_BCSR_COLS = 3,   /* [nnzb] */
+    RDST_M_SLOT_UPPER = 4,  /* [n_internal] block slot of (owner row, neighbour col) */
+    RDST_M_SLOT_LOWER = 5,  /* [n_internal] block slot of (neighbour row, owner col) */
+    RDST_M_SLOT_DIAG = 6,   /* [n_cells] */
+    RDST_M_FACE_COLOUR = 7, /* [n_internal] */
+    RDST_M_ILU_LEVEL = 8,   /* [n_cells] level of the lower-triangular sweep */
+    RDST_M_BFACE_PATCH = 9, /* [n_bfaces] patch index of each non-empty boundary face */
+    RDST_M_CELL_COLOUR = 10 /* [n_cells] */
+};
+
+/* double arrays of a mesh */
+enum rdst_mesh_geom {
+    RDST_G_SF = 0,       /* [n_internal + n_bfaces][3] */
+    RDST_G_MAGSF = 1,
+    RDST_G_CF = 2,       /* [..][3] */
+    RDST_G_C = 3,        /* [n_cells][3] */
+    RDST_G_V = 4,        /* [n_cells] */
+    RDST_G_WEIGHTS = 5,  /* [n_internal + n_bfaces] (1 on boundary faces) */
+    RDST_G_DELTACOEFFS = 6
+};
+
+/* integer mesh sizes returned by rdst_mesh_info */
+enum rdst_mesh_info_idx {
+    RDST_I_NCELLS = 0, RDST_I_NINTERNAL = 1, RDST_I_NBFACES = 2, RDST_I_NNZB = 3,
+    RDST_I_NCOLOURS = 4, RDST_I_NLEVELS_L = 5, RDST_I_NLEVELS_U = 6, RDST_I_NCELL_COLOURS = 7,
+    RDST_I_COUNT = 8
+};
+
+typedef struct rdst_mesh rdst_mesh;
+typedef struct rdst_case rdst_case;
+
+/* ----------------------------------------------------------- library */
+const char* rdst_last_error(void);
+int rdst_version(void);
+/* select the HIP device used by later calls */
+int rdst_init(int device);
+
+/* ----------------------------------------------------------- mesh
+ * Copies the polyMesh primitives, computes the finite-volume geometry on
+ * the host (fvMesh: Sf, magSf, Cf, C, V, weights, deltaCoeffs), the
+ * owner/neighbour-sorted block-CSR with its face->slot maps, a face
+ * colouring and the ILU level sets (host only); the first rdst_case_create
+ * on the mesh uploads it to the GPU.  Replaces the fvMesh the reference's
+ * operators read through mesh_. */
+int rdst_mesh_create(const rdst_mesh_desc* desc, rdst_mesh** out);
+int rdst_mesh_destroy(rdst_mesh* mesh);
+int rdst_mesh_info(const rdst_mesh* mesh, int64_t* info /* [RDST_I_COUNT] */);
+int rdst_mesh_get_int(const rdst_mesh* mesh, int which, int32_t* out);
+int rdst_mesh_get_geom(const rdst_mesh* mesh, int which, double* out);
+
+/* ----------------------------------------------------------- case
+ * A case = mesh + thermo + per-patch BCs + rhoDSTDict/fvSolution settings
+ * + the device-resident state.  rdst_case_create validates the names the
+ * reference selects at run time (flux scheme, solver, preconditioner) and
+ * fails with RDST_ERR_UNKNOWN_NAME on an unknown id. */
+int rdst_case_create(rdst_mesh* mesh, const rdst_thermo* thermo,
+                     const rdst_bc* bcs /* [n_patches] */,
+                     const rdst_settings* settings, rdst_case** out);
+int rdst_case_destroy(rdst_case* c);
+/* createFields.H:21-72 + timeDST::initialize (timeDST.C:67-208):
+ * from cell p, T, U ([n][3]) build rho, rhoU, rhoE, h, psi, boundary
+ * values, phi = interpolate(rho*U)&Sf, W, courant and rDeltaTDST */
+int rdst_case_set_state(rdst_case* c, const double* p, const double* T, const double* U);
+int rdst_case_get(rdst_case* c, int field, double* out);
+/* overwrite a cell field (RDST_F_RHO, _RHOU, _RHOE, _W, _RDELTAT) */
+int rdst_case_put(rdst_case* c, int field, const double* in);
+
+/* computeFluxDST + calcFields: fills phi, phiUp, phiEp, Up and R_ */
+int rdst_flux_residual(rdst_case* c);
+/* computeJacobianDST + fvmDST::ddt + fluxDST::integrate into the BCSR
+ * (FP64, and the FP32 copy when precision is mixed) and the source */
+int rdst_jacobian_assemble(rdst_case* c);
+/* fvBlockMatrixDST::solve: W mode solves A.W = A.W + s; deltaW mode
+ * solves A.dW = s then W += dW */
+int rdst_block_solve(rdst_case* c, rdst_solver_perf* perf);
+/* updateFields.H: primitives from W, bounding, BC updates */
+int rdst_update_fields(rdst_case* c);
+/* timeDST::setCourantDST: ||R||_2, CFL ramp, setSteadyStateDeltaT;
+ * returns ||R||_2 (timeDST.C:328-331) in *res_norm */
+int rdst_set_courant(rdst_case* c, double* res_norm);
+/* Single::iterate, n_steps times; res_norm[n_steps] receives ||R||_2 of
+ * each step and perf[n_steps] (may be NULL) the solver performance */
+int rdst_iterate(rdst_case* c, int n_steps, double* res_norm, rdst_solver_perf* perf);
+
+/* forceCoeffs-style integral over one patch: force = sum p_b Sf (+ viscous
+ * part when settings.viscous), out[0]=Cd, out[1]=Cl, out[2..4]=force */
+int rdst_forces(rdst_case* c, int patch, const double* drag_dir, const double* lift_dir,
+                double p_ref, double rho_inf, double mag_u_inf, double a_ref, double* out);
+
+/* device time of the last rdst_flux_residual face kernel(s), in ms (HIP events) */
+int rdst_last_flux_ms(rdst_case* c, double* ms);
+/* run the face-flux + residual kernels n times back to back on the case's
+ * stream and report the mean device time per call (HIP events) */
+int rdst_time_flux(rdst_case* c, int n, double* ms_per_call);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RHODST_H */

@@ -1,0 +1,202 @@
+"""ctypes binding of the CPU FP64 oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, where it is the checker.  The product package
+never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from rhodst_amd._abi import (MeshDesc, dptr, rdst_solver_perf, rdst_mesh_desc, rdst_thermo,
+                             rdst_bc, rdst_settings)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        P = C.POINTER
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_geometry.argtypes = [P(rdst_mesh_desc)] + [P(C.c_double)] * 7 + [P(C.c_int32)]
+        L.orc_face_flux.argtypes = [C.c_int, P(C.c_double), P(C.c_double), P(C.c_double), C.c_double,
+                                    C.c_double, C.c_int, C.c_double, P(C.c_double)]
+        L.orc_euler_jacobian.argtypes = [P(C.c_double), C.c_double, C.c_double, C.c_double, P(C.c_double), P(C.c_double)]
+        L.orc_case_create.argtypes = [P(rdst_mesh_desc), P(rdst_thermo), P(rdst_bc), P(rdst_settings), P(C.c_void_p)]
+        L.orc_case_destroy.argtypes = [C.c_void_p]
+        L.orc_case_set_state.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_double)]
+        for name in ("orc_flux_residual", "orc_jacobian_assemble", "orc_update_fields"):
+            getattr(L, name).argtypes = [C.c_void_p]
+        L.orc_block_solve.argtypes = [C.c_void_p, P(rdst_solver_perf)]
+        L.orc_set_courant.argtypes = [C.c_void_p, P(C.c_double)]
+        L.orc_iterate.argtypes = [C.c_void_p, C.c_int, P(C.c_double), P(rdst_solver_perf)]
+        L.orc_get_matrix.argtypes = [C.c_void_p, C.c_int, P(C.c_double)]
+        L.orc_amul.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
+        L.orc_ilu_apply.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
+        L.orc_linear_solve.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), P(rdst_solver_perf)]
+        L.orc_set_matrix.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_double)]
+        L.orc_case_get.argtypes = [C.c_void_p, C.c_int, P(C.c_double)]
+        L.orc_case_put.argtypes = [C.c_void_p, C.c_int, P(C.c_double)]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError("oracle: " + lib().orc_last_error().decode())
+
+
+def geometry(mesh):
+    """fvMesh geometry of a PolyMesh: dict of Sf, magSf, Cf (internal then
+    non-empty boundary faces), C, V, weights, deltaCoeffs."""
+    md = MeshDesc(mesh)
+    nI = mesh.n_internal
+    nB = mesh.n_boundary_nonempty()
+    nf = nI + nB
+    out = {"Sf": np.zeros((nf, 3)), "magSf": np.zeros(nf), "Cf": np.zeros((nf, 3)),
+           "C": np.zeros((mesh.n_cells, 3)), "V": np.zeros(mesh.n_cells),
+           "weights": np.zeros(nf), "deltaCoeffs": np.zeros(nf)}
+    nb = C.c_int32()
+    _check(lib().orc_geometry(C.byref(md.desc), dptr(out["Sf"]), dptr(out["magSf"]), dptr(out["Cf"]),
+                              dptr(out["C"]), dptr(out["V"]), dptr(out["weights"]), dptr(out["deltaCoeffs"]),
+                              C.byref(nb)))
+    assert nb.value == nB
+    return out
+
+
+def face_flux(scheme: int, L, R, Sf, gamma=1.4, f_hllcp=1.0, low_mach=0, Mainf=0.5):
+    """One face of a scheme; L/R = [rho, rhoUx, rhoUy, rhoUz, e, rPsi, aStar, c].
+    Returns [phi, phiUx, phiUy, phiUz, phiE, Upx, Upy, Upz]."""
+    Lb = np.ascontiguousarray(L, dtype=np.float64)
+    Rb = np.ascontiguousarray(R, dtype=np.float64)
+    Sb = np.ascontiguousarray(Sf, dtype=np.float64)
+    out = np.zeros(8)
+    _check(lib().orc_face_flux(scheme, dptr(Lb), dptr(Rb), dptr(Sb), gamma, f_hllcp, low_mach, Mainf, dptr(out)))
+    return out
+
+
+def euler_jacobian(U, gamma, rhoE, rho, n):
+    Ub = np.ascontiguousarray(U, dtype=np.float64)
+    nb = np.ascontiguousarray(n, dtype=np.float64)
+    J = np.zeros(25)
+    _check(lib().orc_euler_jacobian(dptr(Ub), gamma, rhoE, rho, dptr(nb), dptr(J)))
+    return J.reshape(5, 5)
+
+
+_FIELD_SHAPES = {1: 3, 5: 3, 8: 5, 9: 5, 13: 3, 15: 3, 17: 3, 20: 3, 25: 5}
+
+
+class OracleCase:
+    """The oracle's Single::iterate state machine on one case."""
+
+    def __init__(self, spec):
+        self.spec = spec
+        self.md = MeshDesc(spec.mesh)
+        self.n = spec.mesh.n_cells
+        self.nI = spec.mesh.n_internal
+        self.nB = spec.mesh.n_boundary_nonempty()
+        h = C.c_void_p()
+        _check(lib().orc_case_create(C.byref(self.md.desc), C.byref(spec.thermo), spec.bcs,
+                                     C.byref(spec.settings), C.byref(h)))
+        self.h = h
+        self.set_state(spec.p0, spec.T0, spec.U0)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_case_destroy(self.h)
+            self.h = None
+
+    def set_state(self, p, T, U):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        _check(lib().orc_case_set_state(self.h, dptr(p), dptr(T), dptr(U)))
+
+    def _size(self, field):
+        if field in (12, 13, 14, 15):
+            return self.nI + self.nB
+        if 16 <= field <= 23:
+            return self.nB
+        return self.n
+
+    def get(self, field):
+        k = _FIELD_SHAPES.get(field, 1)
+        out = np.zeros((self._size(field), k))
+        _check(lib().orc_case_get(self.h, field, dptr(out)))
+        return out if k > 1 else out[:, 0]
+
+    def put(self, field, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        _check(lib().orc_case_put(self.h, field, dptr(a)))
+
+    def flux_residual(self):
+        _check(lib().orc_flux_residual(self.h))
+
+    def jacobian_assemble(self):
+        _check(lib().orc_jacobian_assemble(self.h))
+
+    def block_solve(self):
+        perf = rdst_solver_perf()
+        _check(lib().orc_block_solve(self.h, C.byref(perf)))
+        return perf
+
+    def update_fields(self):
+        _check(lib().orc_update_fields(self.h))
+
+    def set_courant(self):
+        r = C.c_double()
+        _check(lib().orc_set_courant(self.h, C.byref(r)))
+        return r.value
+
+    def iterate(self, n_steps=1, with_perf=False):
+        res = np.zeros(n_steps)
+        perfs = (rdst_solver_perf * n_steps)()
+        _check(lib().orc_iterate(self.h, n_steps, dptr(res), perfs))
+        return (res, perfs) if with_perf else res
+
+    def matrix(self, which):
+        sizes = {0: (self.n, 25), 1: (self.nI, 25), 2: (self.nI, 25), 3: (self.n, 5), 4: (self.nI, 1),
+                 5: (self.nI, 25), 6: (self.nI, 25), 7: (self.nB, 25)}
+        out = np.zeros(sizes[which])
+        _check(lib().orc_get_matrix(self.h, which, dptr(out)))
+        return out
+
+    def amul(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        _check(lib().orc_amul(self.h, dptr(x), dptr(y)))
+        return y
+
+    def ilu_apply(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        x = np.zeros_like(r)
+        _check(lib().orc_ilu_apply(self.h, dptr(r), dptr(x)))
+        return x
+
+    def linear_solve(self, b, x0=None):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros_like(b) if x0 is None else np.ascontiguousarray(x0, dtype=np.float64).copy()
+        perf = rdst_solver_perf()
+        _check(lib().orc_linear_solve(self.h, dptr(b), dptr(x), C.byref(perf)))
+        return x, perf
+
+    def set_matrix(self, diag, upper, lower):
+        d = np.ascontiguousarray(diag, dtype=np.float64)
+        u = np.ascontiguousarray(upper, dtype=np.float64)
+        l_ = np.ascontiguousarray(lower, dtype=np.float64)
+        _check(lib().orc_set_matrix(self.h, dptr(d), dptr(u), dptr(l_)))

@@ -1,0 +1,1753 @@
+/*
+ * oracle.cpp -- CPU FP64 restatement of rhoDST's block-coupled implicit step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity oracle: it is built
+ * into oracle/_build/liboracle.so and is loaded only by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg, always as
+ * the checker.  The product path (rhodst_amd/) never links or calls it.
+ *
+ * It follows the reference field-at-a-time, the way the foam-extend
+ * operators evaluate (one whole-field expression after another, branch
+ * masks multiplied in rather than branched on), citing each reference
+ * file:line it restates.  Paths are relative to the rhoDST tree.
+ *
+ * Pinning (see tests/test_oracle_*.py): flux consistency F(W,W)=F_phys(W),
+ * supersonic upwinding, mirror antisymmetry, discrete conservation,
+ * Euler-Jacobian homogeneity A(W)W = F(W).n and central finite
+ * differences, closed-form geometry on uniform grids, exact block LU on
+ * 1-D meshes against scipy, GMRES/BiCGStab against numpy.linalg.solve,
+ * far-field regime closed forms, LTS closed forms.
+ *
+ * Parity unpinned (no reference run is possible: foam-extend 4.1 is absent):
+ *  - thermo T(h) is taken as h/Cp exactly (hConstThermo Newton not restated);
+ *  - boundMinMax is restated as a plain clamp;
+ *  - the foam-extend BlockGMRES/BlockBiCGStab/ILUC0 internals (normFactor,
+ *    convergence test on |residual vector5|, modified Gram-Schmidt,
+ *    left-preconditioned GMRES, right-preconditioned BiCGStab) are restated
+ *    from the published foam-extend algorithms, not pinned by a run.
+ */
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <cstdio>
+#include <vector>
+#include <string>
+#include <algorithm>
+#include <stdexcept>
+
+#include "../include/rhodst.h"
+
+namespace orc {
+
+static const double SMALL = 1.0e-15;   // foam doubleScalar SMALL
+static const double VSMALL = 1.0e-300; // foam doubleScalar VSMALL
+static const double ROOTVSMALL = 1.0e-150;
+static const double BLOCK_SMALL = 1.0e-20; // BlockLduMatrix::small_
+static const double BLOCK_GREAT = 1.0e20;  // BlockLduMatrix::great_
+
+static std::string g_err;
+
+// ------------------------------------------------------------------ helpers
+struct V3 { double x, y, z; };
+static inline V3 v3(double x, double y, double z) { V3 r = {x, y, z}; return r; }
+static inline V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline V3 operator*(double s, V3 a) { return v3(s * a.x, s * a.y, s * a.z); }
+static inline V3 operator*(V3 a, double s) { return v3(a.x * s, a.y * s, a.z * s); }
+static inline V3 operator/(V3 a, double s) { return v3(a.x / s, a.y / s, a.z / s); }
+static inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline V3 cross(V3 a, V3 b) {
+    return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline double magSqr(V3 a) { return dot(a, a); }
+static inline double mag(V3 a) { return std::sqrt(magSqr(a)); }
+static inline double cmp(V3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+
+// foam scalar functions
+static inline double fsign(double s) { return (s >= 0) ? 1.0 : -1.0; } // foam sign: sign(0)=+1
+static inline double sqr(double s) { return s * s; }
+// DST::pos/pos0/neg/neg0 (DST.C:323-341)
+static inline double pos(double s) { return (s > 0) ? 1.0 : 0.0; }
+static inline double pos0(double s) { return (s >= 0) ? 1.0 : 0.0; }
+static inline double neg(double s) { return (s < 0) ? 1.0 : 0.0; }
+static inline double neg0(double s) { return (s <= 0) ? 1.0 : 0.0; }
+// foam-extend's own pos() used by inletOutlet (s >= 0)
+static inline double fe_pos(double s) { return (s >= 0) ? 1.0 : 0.0; }
+
+typedef double T25[25];
+
+// ------------------------------------------------------------------ mesh
+struct Mesh {
+    int nPoints = 0, nFaces = 0, nInternal = 0, nCells = 0;
+    std::vector<V3> points;
+    std::vector<int> faceOff, facePts, owner, neighbour;
+    std::vector<rdst_patch> patches;
+    // polyMesh-face geometry
+    std::vector<V3> Sf, Cf;
+    std::vector<double> magSf;
+    std::vector<V3> C;
+    std::vector<double> V;
+    // fvMesh surface fields: internal faces, then compacted boundary faces
+    int nB = 0;
+    std::vector<int> bFace;   // polyMesh face of each boundary face
+    std::vector<int> bPatch;  // patch of each boundary face
+    std::vector<int> bCell;   // faceCells
+    std::vector<double> w;    // weights, size nInternal + nB
+    std::vector<double> dc;   // deltaCoeffs
+    std::vector<std::vector<int>> cellCells;
+};
+
+// primitiveMesh::makeFaceCentresAndAreas (foam-extend primitiveMeshFaceCentresAndAreas.C)
+static void faceCentresAndAreas(Mesh& m) {
+    m.Sf.assign(m.nFaces, v3(0, 0, 0));
+    m.Cf.assign(m.nFaces, v3(0, 0, 0));
+    m.magSf.assign(m.nFaces, 0.0);
+    for (int f = 0; f < m.nFaces; ++f) {
+        const int s = m.faceOff[f], n = m.faceOff[f + 1] - s;
+        const int* fp = &m.facePts[s];
+        if (n == 3) {
+            V3 p0 = m.points[fp[0]], p1 = m.points[fp[1]], p2 = m.points[fp[2]];
+            m.Cf[f] = (1.0 / 3.0) * (p0 + p1 + p2);
+            m.Sf[f] = 0.5 * cross(p1 - p0, p2 - p0);
+        } else {
+            V3 sumN = v3(0, 0, 0), sumAc = v3(0, 0, 0);
+            double sumA = 0.0;
+            V3 fCentre = m.points[fp[0]];
+            for (int pi = 1; pi < n; ++pi) fCentre = fCentre + m.points[fp[pi]];
+            fCentre = fCentre / double(n);
+            for (int pi = 0; pi < n; ++pi) {
+                V3 p = m.points[fp[pi]];
+                V3 nextPoint = m.points[fp[(pi + 1) % n]];
+                V3 c = p + nextPoint + fCentre;
+                V3 nv = cross(nextPoint - p, fCentre - p);
+                double a = mag(nv);
+                sumN = sumN + nv;
+                sumA += a;
+                sumAc = sumAc + a * c;
+            }
+            if (sumA < ROOTVSMALL) {
+                m.Cf[f] = fCentre;
+                m.Sf[f] = v3(0, 0, 0);
+            } else {
+                m.Cf[f] = (1.0 / 3.0) * sumAc / sumA;
+                m.Sf[f] = 0.5 * sumN;
+            }
+        }
+        m.magSf[f] = mag(m.Sf[f]);
+    }
+}
+
+// primitiveMesh::makeCellCentresAndVols (foam-extend primitiveMeshCellCentresAndVols.C)
+static void cellCentresAndVols(Mesh& m) {
+    std::vector<V3> cEst(m.nCells, v3(0, 0, 0));
+    std::vector<int> nCellFaces(m.nCells, 0);
+    for (int f = 0; f < m.nFaces; ++f) {
+        cEst[m.owner[f]] = cEst[m.owner[f]] + m.Cf[f];
+        nCellFaces[m.owner[f]]++;
+    }
+    for (int f = 0; f < m.nInternal; ++f) {
+        cEst[m.neighbour[f]] = cEst[m.neighbour[f]] + m.Cf[f];
+        nCellFaces[m.neighbour[f]]++;
+    }
+    for (int c = 0; c < m.nCells; ++c) cEst[c] = cEst[c] / double(nCellFaces[c]);
+    m.C.assign(m.nCells, v3(0, 0, 0));
+    m.V.assign(m.nCells, 0.0);
+    for (int f = 0; f < m.nFaces; ++f) {
+        int o = m.owner[f];
+        double pyr3Vol = std::max(dot(m.Sf[f], m.Cf[f] - cEst[o]), VSMALL);
+        V3 pc = (3.0 / 4.0) * m.Cf[f] + (1.0 / 4.0) * cEst[o];
+        m.C[o] = m.C[o] + pyr3Vol * pc;
+        m.V[o] += pyr3Vol;
+    }
+    for (int f = 0; f < m.nInternal; ++f) {
+        int nb = m.neighbour[f];
+        double pyr3Vol = std::max(dot(m.Sf[f], cEst[nb] - m.Cf[f]), VSMALL);
+        V3 pc = (3.0 / 4.0) * m.Cf[f] + (1.0 / 4.0) * cEst[nb];
+        m.C[nb] = m.C[nb] + pyr3Vol * pc;
+        m.V[nb] += pyr3Vol;
+    }
+    for (int c = 0; c < m.nCells; ++c) {
+        m.C[c] = m.C[c] / m.V[c];
+        m.V[c] *= (1.0 / 3.0);
+    }
+}
+
+static void buildMesh(Mesh& m, const rdst_mesh_desc* d) {
+    m.nPoints = d->n_points; m.nFaces = d->n_faces; m.nInternal = d->n_internal_faces;
+    m.nCells = d->n_cells;
+    m.points.resize(m.nPoints);
+    for (int i = 0; i < m.nPoints; ++i)
+        m.points[i] = v3(d->points[3 * i], d->points[3 * i + 1], d->points[3 * i + 2]);
+    m.faceOff.assign(d->face_offsets, d->face_offsets + m.nFaces + 1);
+    m.facePts.assign(d->face_points, d->face_points + m.faceOff[m.nFaces]);
+    m.owner.assign(d->owner, d->owner + m.nFaces);
+    m.neighbour.assign(d->neighbour, d->neighbour + m.nInternal);
+    m.patches.assign(d->patches, d->patches + d->n_patches);
+    faceCentresAndAreas(m);
+    cellCentresAndVols(m);
+    // compacted boundary faces: fvPatch of an empty patch has size 0
+    m.bFace.clear(); m.bPatch.clear(); m.bCell.clear();
+    for (int p = 0; p < (int)m.patches.size(); ++p) {
+        if (m.patches[p].type == RDST_PATCH_EMPTY) continue;
+        for (int i = 0; i < m.patches[p].size; ++i) {
+            int f = m.patches[p].start + i;
+            m.bFace.push_back(f); m.bPatch.push_back(p); m.bCell.push_back(m.owner[f]);
+        }
+    }
+    m.nB = (int)m.bFace.size();
+    // surfaceInterpolation::makeWeights / makeDeltaCoeffs (foam-extend 4.1)
+    m.w.assign(m.nInternal + m.nB, 1.0);
+    m.dc.assign(m.nInternal + m.nB, 0.0);
+    for (int f = 0; f < m.nInternal; ++f) {
+        double dOwn = std::fabs(dot(m.Sf[f], m.Cf[f] - m.C[m.owner[f]]));
+        double dNei = std::fabs(dot(m.Sf[f], m.C[m.neighbour[f]] - m.Cf[f]));
+        m.w[f] = dNei / (dOwn + dNei);
+        V3 delta = m.C[m.neighbour[f]] - m.C[m.owner[f]];
+        V3 unitArea = m.Sf[f] / m.magSf[f];
+        // stabilised form, comment at timeDST.C:237
+        m.dc[f] = 1.0 / std::max(dot(unitArea, delta), 0.05 * mag(delta));
+    }
+    for (int b = 0; b < m.nB; ++b) {
+        int f = m.bFace[b];
+        V3 delta = m.Cf[f] - m.C[m.bCell[b]];
+        m.dc[m.nInternal + b] = 1.0 / mag(delta);
+    }
+    // mesh.cellCells(): neighbours across internal faces
+    m.cellCells.assign(m.nCells, std::vector<int>());
+    for (int f = 0; f < m.nInternal; ++f) {
+        m.cellCells[m.owner[f]].push_back(m.neighbour[f]);
+        m.cellCells[m.neighbour[f]].push_back(m.owner[f]);
+    }
+}
+
+// surface-field accessors: face k < nInternal internal, else boundary k-nInternal
+static inline V3 SfOf(const Mesh& m, int k) { return k < m.nInternal ? m.Sf[k] : m.Sf[m.bFace[k - m.nInternal]]; }
+static inline double magSfOf(const Mesh& m, int k) { return k < m.nInternal ? m.magSf[k] : m.magSf[m.bFace[k - m.nInternal]]; }
+
+// ------------------------------------------------------------------ case
+struct BCState {
+    // per boundary face mixed-BC data (valueFraction) kept between steps
+    std::vector<double> vf_p, vf_T, vf_U;
+};
+
+struct Case {
+    Mesh m;
+    rdst_thermo th;
+    std::vector<rdst_bc> bc;
+    rdst_settings st;
+    double Cv = 0, gamma = 0;
+
+    // volume fields (cells)
+    std::vector<double> rho, rhoE, p, T, h, psi, mu, alpha;
+    std::vector<V3> rhoU, U;
+    // boundary values, per compacted boundary face
+    std::vector<double> brho, brhoE, bp, bT, bh, bpsi, bmu, balpha;
+    std::vector<V3> brhoU, bU;
+    BCState bcs;
+    // surface fields, nInternal + nB
+    std::vector<double> phi, phiEp;
+    std::vector<V3> phiUp, Up;
+    // residual and solution vector
+    std::vector<double> R;  // [n][5]
+    std::vector<double> W;  // [n][5]
+    // Jacobian
+    std::vector<double> ownC, neiC;  // [nInternal][25]
+    std::vector<double> lambda;      // [nInternal] spectral radius
+    std::vector<double> patchIC;     // [nB][25]
+    // assembled LDU
+    std::vector<double> diag, upper, lower;  // [..][25]
+    std::vector<double> source;              // [n][5]
+    // time control
+    std::vector<double> rDeltaT, courant, rhoOld, eOld, dS;  // dS [n][3]
+    double residual = -1, residualRatio = 1, courantScalar = 1;
+    bool hasState = false, hasMatrix = false;
+};
+
+// ------------------------------------------------------------------ thermo
+static inline double muOf(const Case& c, double T) {
+    if (c.th.transport == 1) return c.th.As * std::sqrt(T) / (1.0 + c.th.Ts / T);
+    return c.th.mu;
+}
+static inline double alphaOf(const Case& c, double T) {
+    // constTransport alpha reduces to mu/Pr for hConstThermo; sutherland uses
+    // the modified Eucken conductivity kappa = mu*Cv*(1.32 + 1.77*R/Cv), alpha = kappa/Cp
+    if (c.th.transport == 1) {
+        double mu = muOf(c, T);
+        return mu * c.Cv * (1.32 + 1.77 * c.th.R / c.Cv) / c.th.Cp;
+    }
+    return c.th.mu / c.th.Pr;
+}
+
+// boundary value of a vector field on a symmetryPlane/slip patch:
+// (transform(I - 2 n n, U_I) + U_I)/2   (basicSymmetryFvPatchField::evaluate)
+static inline V3 symmetricValue(V3 n, V3 UI) {
+    double nU = dot(n, UI);
+    V3 t = v3(UI.x - 2.0 * n.x * nU, UI.y - 2.0 * n.y * nU, UI.z - 2.0 * n.z * nU);
+    return (t + UI) / 2.0;
+}
+
+// valueInternalCoeffs of each BC (Roe.C:363-374 reads them)
+static inline double scalarVIC(int type, double vf) {
+    switch (type) {
+        case RDST_BC_FIXED_VALUE: return 0.0;
+        case RDST_BC_ZERO_GRADIENT: return 1.0;
+        case RDST_BC_SYMMETRY: return 1.0;
+        case RDST_BC_INLET_OUTLET:
+        case RDST_BC_FARFIELD: return 1.0 - vf;
+        default: return 0.0;
+    }
+}
+static inline V3 vectorVIC(int type, double vf, V3 n) {
+    switch (type) {
+        case RDST_BC_FIXED_VALUE: return v3(0, 0, 0);
+        case RDST_BC_ZERO_GRADIENT: return v3(1, 1, 1);
+        case RDST_BC_SYMMETRY: return v3(1.0 - std::fabs(n.x), 1.0 - std::fabs(n.y), 1.0 - std::fabs(n.z));
+        case RDST_BC_INLET_OUTLET:
+        case RDST_BC_FARFIELD: return v3(1.0 - vf, 1.0 - vf, 1.0 - vf);
+        default: return v3(0, 0, 0);
+    }
+}
+
+// ------------------------------------------------------------------ set state
+// createFields.H:21-72 and compressibleCreatePhi; timeDST::initialize (timeDST.C:67-208)
+static void setSteadyStateDeltaT(Case& c);
+
+static void initialBoundary(Case& c) {
+    const Mesh& m = c.m;
+    int nB = m.nB;
+    c.bp.assign(nB, 0); c.bT.assign(nB, 0); c.bU.assign(nB, v3(0, 0, 0));
+    c.bcs.vf_p.assign(nB, 0.0); c.bcs.vf_T.assign(nB, 0.0); c.bcs.vf_U.assign(nB, 0.0);
+    for (int b = 0; b < nB; ++b) {
+        const rdst_bc& bc = c.bc[m.bPatch[b]];
+        int ci = m.bCell[b];
+        V3 n = m.Sf[m.bFace[b]] / m.magSf[m.bFace[b]];
+        switch (bc.p_type) {
+            case RDST_BC_FIXED_VALUE: case RDST_BC_INLET_OUTLET: c.bp[b] = bc.p_value; break;
+            case RDST_BC_FARFIELD: c.bp[b] = bc.p_inf; break;
+            default: c.bp[b] = c.p[ci];
+        }
+        switch (bc.T_type) {
+            case RDST_BC_FIXED_VALUE: case RDST_BC_INLET_OUTLET: c.bT[b] = bc.T_value; break;
+            case RDST_BC_FARFIELD: c.bT[b] = bc.T_inf; break;
+            default: c.bT[b] = c.T[ci];
+        }
+        switch (bc.U_type) {
+            case RDST_BC_FIXED_VALUE: case RDST_BC_INLET_OUTLET:
+                c.bU[b] = v3(bc.U_value[0], bc.U_value[1], bc.U_value[2]); break;
+            case RDST_BC_FARFIELD: c.bU[b] = v3(bc.U_inf[0], bc.U_inf[1], bc.U_inf[2]); break;
+            case RDST_BC_SYMMETRY: c.bU[b] = symmetricValue(n, c.U[ci]); break;
+            default: c.bU[b] = c.U[ci];
+        }
+    }
+}
+
+static void setState(Case& c, const double* p, const double* T, const double* U) {
+    const Mesh& m = c.m;
+    int n = m.nCells, nB = m.nB;
+    c.p.assign(p, p + n); c.T.assign(T, T + n);
+    c.U.resize(n);
+    for (int i = 0; i < n; ++i) c.U[i] = v3(U[3 * i], U[3 * i + 1], U[3 * i + 2]);
+    c.psi.resize(n); c.h.resize(n); c.mu.resize(n); c.alpha.resize(n);
+    for (int i = 0; i < n; ++i) {
+        c.psi[i] = 1.0 / (c.th.R * c.T[i]);
+        c.h[i] = c.th.Cp * c.T[i];
+        c.mu[i] = muOf(c, c.T[i]);
+        c.alpha[i] = alphaOf(c, c.T[i]);
+    }
+    initialBoundary(c);
+    c.bpsi.resize(nB); c.bh.resize(nB); c.bmu.resize(nB); c.balpha.resize(nB);
+    for (int b = 0; b < nB; ++b) {
+        c.bpsi[b] = 1.0 / (c.th.R * c.bT[b]);
+        c.bh[b] = c.th.Cp * c.bT[b];
+        c.bmu[b] = muOf(c, c.bT[b]);
+        c.balpha[b] = alphaOf(c, c.bT[b]);
+    }
+    // rho = thermo.rho() = psi*p; rhoU = rho*U; rhoE = rho*(h*Cv/Cp + 0.5|U|^2)
+    c.rho.resize(n); c.rhoU.resize(n); c.rhoE.resize(n);
+    for (int i = 0; i < n; ++i) {
+        c.rho[i] = c.psi[i] * c.p[i];
+        c.rhoU[i] = c.rho[i] * c.U[i];
+        c.rhoE[i] = c.rho[i] * (c.h[i] * c.Cv / c.th.Cp + 0.5 * magSqr(c.U[i]));
+    }
+    c.brho.resize(nB); c.brhoU.resize(nB); c.brhoE.resize(nB);
+    for (int b = 0; b < nB; ++b) {
+        c.brho[b] = c.bpsi[b] * c.bp[b];
+        c.brhoU[b] = c.brho[b] * c.bU[b];
+        c.brhoE[b] = c.brho[b] * (c.bh[b] * c.Cv / c.th.Cp + 0.5 * magSqr(c.bU[b]));
+    }
+    // phi = linearInterpolate(rho*U) & Sf
+    c.phi.assign(m.nInternal + nB, 0.0);
+    for (int f = 0; f < m.nInternal; ++f) {
+        int o = m.owner[f], nb = m.neighbour[f];
+        double w = m.w[f];
+        V3 rUo = c.rho[o] * c.U[o], rUn = c.rho[nb] * c.U[nb];
+        V3 rUf = v3(w * (rUo.x - rUn.x) + rUn.x, w * (rUo.y - rUn.y) + rUn.y, w * (rUo.z - rUn.z) + rUn.z);
+        c.phi[f] = dot(rUf, m.Sf[f]);
+    }
+    for (int b = 0; b < nB; ++b) c.phi[m.nInternal + b] = dot(c.brho[b] * c.bU[b], m.Sf[m.bFace[b]]);
+    c.phiUp.assign(m.nInternal + nB, v3(0, 0, 0));
+    c.phiEp.assign(m.nInternal + nB, 0.0);
+    c.Up.assign(m.nInternal + nB, v3(0, 0, 0));
+    c.R.assign(5 * n, 0.0);
+    // W.set(rho, rhoU, rhoE)
+    c.W.resize(5 * n);
+    for (int i = 0; i < n; ++i) {
+        c.W[5 * i] = c.rho[i];
+        c.W[5 * i + 1] = c.rhoU[i].x; c.W[5 * i + 2] = c.rhoU[i].y; c.W[5 * i + 3] = c.rhoU[i].z;
+        c.W[5 * i + 4] = c.rhoE[i];
+    }
+    // timeDST::initialize
+    c.rhoOld = c.rho;   // thermo rho
+    c.eOld.resize(n);
+    for (int i = 0; i < n; ++i) c.eOld[i] = c.h[i] * c.Cv / c.th.Cp;
+    if (!c.st.neighbouring) {
+        c.dS.assign(3 * n, 0.0);
+        for (int f = 0; f < m.nInternal; ++f) {
+            V3 a = v3(std::fabs(m.Sf[f].x), std::fabs(m.Sf[f].y), std::fabs(m.Sf[f].z));
+            int o = m.owner[f], nb = m.neighbour[f];
+            c.dS[3 * o] += a.x; c.dS[3 * o + 1] += a.y; c.dS[3 * o + 2] += a.z;
+            c.dS[3 * nb] += a.x; c.dS[3 * nb + 1] += a.y; c.dS[3 * nb + 2] += a.z;
+        }
+        for (int b = 0; b < nB; ++b) {
+            V3 sf = m.Sf[m.bFace[b]];
+            int ci = m.bCell[b];
+            c.dS[3 * ci] += std::fabs(sf.x); c.dS[3 * ci + 1] += std::fabs(sf.y); c.dS[3 * ci + 2] += std::fabs(sf.z);
+        }
+        for (auto& v : c.dS) v *= 0.5;
+    }
+    c.residual = -1; c.residualRatio = 1;
+    c.courantScalar = c.st.initial_co;
+    c.courant.assign(n, c.st.initial_co);
+    c.rDeltaT.assign(n, 0.0);
+    setSteadyStateDeltaT(c);
+    c.hasState = true; c.hasMatrix = false;
+}
+
+// ------------------------------------------------------------------ flux
+// Face states used by every scheme: upwind reconstruction with own_=+1 and
+// nei_=-1 (baseFluxDST.C:144-171) gives the owner cell value on the owner
+// side and the neighbour cell value on the neighbour side; boundary faces
+// take the patch value on both sides.
+struct FaceStates {
+    int nf;
+    std::vector<double> rho_o, rho_n, rPsi_o, rPsi_n, e_o, e_n, a_o, a_n, gam_f, p_cellf;
+    std::vector<V3> rhoU_o, rhoU_n, Sf;
+    std::vector<double> magSf;
+};
+
+// cell-centred auxiliary fields with boundary values
+struct CellAux {
+    std::vector<double> rPsi, e, aStar, c, gam;  // cells
+    std::vector<double> brPsi, be, baStar, bc, bgam;
+};
+
+static void makeAux(const Case& c, CellAux& a) {
+    const Mesh& m = c.m;
+    int n = m.nCells, nB = m.nB;
+    double Cp = c.th.Cp, Cv = c.Cv;
+    a.rPsi.resize(n); a.e.resize(n); a.aStar.resize(n); a.c.resize(n); a.gam.resize(n);
+    for (int i = 0; i < n; ++i) {
+        a.rPsi[i] = 1.0 / c.psi[i];
+        a.e[i] = c.h[i] - c.p[i] / c.rho[i];                       // AUSMPLDST.C:145
+        double H = std::max(c.rhoE[i] / c.rho[i], SMALL) + std::max(c.p[i] / c.rho[i], SMALL);  // :156-160
+        a.gam[i] = Cp / Cv;
+        a.aStar[i] = std::sqrt(2.0 * (a.gam[i] - 1.0) / (a.gam[i] + 1.0) * H);  // :161
+        a.c[i] = std::sqrt(a.gam[i] * a.rPsi[i]);                   // KurganovDST.C: sqrt(Cp/Cv*rPsi)
+    }
+    a.brPsi.resize(nB); a.be.resize(nB); a.baStar.resize(nB); a.bc.resize(nB); a.bgam.resize(nB);
+    for (int b = 0; b < nB; ++b) {
+        a.brPsi[b] = 1.0 / c.bpsi[b];
+        a.be[b] = c.bh[b] - c.bp[b] / c.brho[b];
+        double H = std::max(c.brhoE[b] / c.brho[b], SMALL) + std::max(c.bp[b] / c.brho[b], SMALL);
+        a.bgam[b] = Cp / Cv;
+        a.baStar[b] = std::sqrt(2.0 * (a.bgam[b] - 1.0) / (a.bgam[b] + 1.0) * H);
+        a.bc[b] = std::sqrt(a.bgam[b] * a.brPsi[b]);
+    }
+}
+
+// upwind-interpolated face value of a cell field (own side / nei side)
+template <class T>
+static void reconstruct(const Mesh& m, const std::vector<T>& cell, const std::vector<T>& bnd,
+                        std::vector<T>& own, std::vector<T>& nei) {
+    int nf = m.nInternal + m.nB;
+    own.resize(nf); nei.resize(nf);
+    for (int f = 0; f < m.nInternal; ++f) { own[f] = cell[m.owner[f]]; nei[f] = cell[m.neighbour[f]]; }
+    for (int b = 0; b < m.nB; ++b) { own[m.nInternal + b] = bnd[b]; nei[m.nInternal + b] = bnd[b]; }
+}
+// linear interpolation w*(P-N)+N; boundary value on boundary faces
+static void linearInterp(const Mesh& m, const std::vector<double>& cell, const std::vector<double>& bnd,
+                         std::vector<double>& out) {
+    int nf = m.nInternal + m.nB;
+    out.resize(nf);
+    for (int f = 0; f < m.nInternal; ++f) {
+        double P = cell[m.owner[f]], N = cell[m.neighbour[f]];
+        out[f] = m.w[f] * (P - N) + N;
+    }
+    for (int b = 0; b < m.nB; ++b) out[m.nInternal + b] = bnd[b];
+}
+static void linearInterpV(const Mesh& m, const std::vector<V3>& cell, const std::vector<V3>& bnd,
+                          std::vector<V3>& out) {
+    int nf = m.nInternal + m.nB;
+    out.resize(nf);
+    for (int f = 0; f < m.nInternal; ++f) {
+        V3 P = cell[m.owner[f]], N = cell[m.neighbour[f]];
+        double w = m.w[f];
+        out[f] = v3(w * (P.x - N.x) + N.x, w * (P.y - N.y) + N.y, w * (P.z - N.z) + N.z);
+    }
+    for (int b = 0; b < m.nB; ++b) out[m.nInternal + b] = bnd[b];
+}
+
+// The six schemes, evaluated face by face from the reconstructed states.
+// Each writes phi, phiUp, phiEp, Up for face k.
+struct FaceIn {
+    double rho_o, rho_n, rPsi_o, rPsi_n, e_o, e_n, a_o, a_n, c_o, c_n, gam_f, fHLLCP;
+    V3 rhoU_o, rhoU_n, Sf;
+    double magSf;
+};
+struct FaceOut { double phi, phiE; V3 phiU, Up; };
+
+// AUSMPLDST::computeFluxDST (AUSMPLDST.C:127-217), alpha=3/16, beta=1/8 (AUSMPLDST.H:78-79)
+static FaceOut fluxAUSMplus(const FaceIn& in) {
+    const double alpha_ = 3.0 / 16.0, beta_ = 0.125;
+    V3 U_own = in.rhoU_o / in.rho_o, U_nei = in.rhoU_n / in.rho_n;
+    V3 nrm = in.Sf / in.magSf;
+    double Uv_own = dot(U_own, nrm), Uv_nei = dot(U_nei, nrm);
+    double rhoE_own = in.rho_o * (in.e_o + 0.5 * magSqr(U_own));
+    double rhoE_nei = in.rho_n * (in.e_n + 0.5 * magSqr(U_nei));
+    double p_own = in.rho_o * in.rPsi_o, p_nei = in.rho_n * in.rPsi_n;
+    double a12 = 0.5 * (in.a_o + in.a_n);
+    double Ma_own = Uv_own / a12, Ma_nei = Uv_nei / a12;
+    double s1o = fsign(1.0 - std::fabs(Ma_own)), s2o = fsign(-1.0 + std::fabs(Ma_own));
+    double s1n = fsign(1.0 - std::fabs(Ma_nei)), s2n = fsign(-1.0 + std::fabs(Ma_nei));
+    double m1o = 0.5 * (s1o + std::fabs(s1o)), m2o = 0.5 * (s2o + std::fabs(s2o));
+    double m1n = 0.5 * (s1n + std::fabs(s1n)), m2n = 0.5 * (s2n + std::fabs(s2n));
+    double Ma4_own = m1o * (0.25 * sqr(Ma_own + 1.0)) * (1.0 - 16.0 * beta_ * (-0.25) * sqr(Ma_own - 1.0))
+                   + m2o * std::max(Ma_own, 0.0);
+    double P5_own = m1o * 0.25 * sqr(Ma_own + 1.0) * (2.0 - Ma_own - 16.0 * alpha_ * Ma_own * (-0.25) * sqr(Ma_own - 1.0))
+                  + m2o * pos(Ma_own);
+    double Ma4_nei = m1n * (-0.25) * sqr(Ma_nei - 1.0) * (1.0 + 16.0 * beta_ * 0.25 * sqr(Ma_nei + 1.0))
+                   + m2n * std::min(Ma_nei, 0.0);
+    double P5_nei = m1n * (-0.25) * sqr(Ma_nei - 1.0) * (-2.0 - Ma_nei + 16.0 * alpha_ * Ma_nei * 0.25 * sqr(Ma_nei + 1.0))
+                  + m2n * neg(Ma_nei);
+    double Ma12 = Ma4_own + Ma4_nei;
+    double P12 = P5_own * p_own + P5_nei * p_nei;
+    FaceOut o;
+    double ps = pos(Ma12), ng = neg0(Ma12);
+    o.Up = ps * U_own + ng * U_nei;
+    o.phi = Ma12 * a12 * (ps * in.rho_o + ng * in.rho_n) * in.magSf;
+    o.phiU = o.phi * o.Up + P12 * in.Sf;
+    double H_own = (rhoE_own + p_own) / in.rho_o, H_nei = (rhoE_nei + p_nei) / in.rho_n;
+    o.phiE = o.phi * (ps * H_own + ng * H_nei);
+    return o;
+}
+
+// AUSMPLUPDST::computeFluxDST (AUSMPLUPDST.C:209-320); Kp .25, Ku .75, sigma 1 (AUSMPLUPDST.H:84-87)
+static FaceOut fluxAUSMplusUp(const FaceIn& in, bool lowMach, double Mainf) {
+    const double beta_ = 0.125, Kp_ = 0.25, Ku_ = 0.75, sigma_ = 1.0;
+    V3 U_own = in.rhoU_o / in.rho_o, U_nei = in.rhoU_n / in.rho_n;
+    V3 nrm = in.Sf / in.magSf;
+    double Uv_own = dot(U_own, nrm), Uv_nei = dot(U_nei, nrm);
+    double rhoE_own = in.rho_o * (in.e_o + 0.5 * magSqr(U_own));
+    double rhoE_nei = in.rho_n * (in.e_n + 0.5 * magSqr(U_nei));
+    double p_own = in.rho_o * in.rPsi_o, p_nei = in.rho_n * in.rPsi_n;
+    double a_own = sqr(in.a_o) / std::max(in.a_o, Uv_own);      // :249
+    double a_nei = sqr(in.a_n) / std::max(in.a_n, -Uv_nei);     // :250
+    double a12 = std::min(a_own, a_nei);                         // :252
+    double Ma_own = Uv_own / a12, Ma_nei = Uv_nei / a12;
+    double alpha = 0.0, Mmean2 = 0.0, fa = 0.0;                  // alpha stays 0 without lowMach (:118-131)
+    if (lowMach) {
+        Mmean2 = 0.5 * (sqr(Uv_own) + sqr(Uv_nei)) / sqr(a12);
+        double M0 = std::sqrt(std::min(1.0, std::max(Mmean2, sqr(Mainf))));
+        fa = M0 * (2.0 - M0);
+        alpha = 3.0 / 16.0 * (-4.0 + 5.0 * sqr(fa));
+    }
+    double s1o = fsign(1.0 - std::fabs(Ma_own)), s2o = fsign(-1.0 + std::fabs(Ma_own));
+    double s1n = fsign(1.0 - std::fabs(Ma_nei)), s2n = fsign(-1.0 + std::fabs(Ma_nei));
+    double m1o = 0.5 * (s1o + std::fabs(s1o)), m2o = 0.5 * (s2o + std::fabs(s2o));
+    double m1n = 0.5 * (s1n + std::fabs(s1n)), m2n = 0.5 * (s2n + std::fabs(s2n));
+    double Ma4_own = m1o * (0.25 * sqr(Ma_own + 1.0)) * (1.0 - 16.0 * beta_ * (-0.25) * sqr(Ma_own - 1.0))
+                   + m2o * std::max(Ma_own, 0.0);
+    double P5_own = m1o * 0.25 * sqr(Ma_own + 1.0) * (2.0 - Ma_own - 16.0 * alpha * Ma_own * (-0.25) * sqr(Ma_own - 1.0))
+                  + m2o * pos(Ma_own);
+    double Ma4_nei = m1n * (-0.25) * sqr(Ma_nei - 1.0) * (1.0 + 16.0 * beta_ * 0.25 * sqr(Ma_nei + 1.0))
+                   + m2n * std::min(Ma_nei, 0.0);
+    double P5_nei = m1n * (-0.25) * sqr(Ma_nei - 1.0) * (-2.0 - Ma_nei + 16.0 * alpha * Ma_nei * 0.25 * sqr(Ma_nei + 1.0))
+                  + m2n * neg(Ma_nei);
+    double Ma12 = Ma4_own + Ma4_nei;
+    double P12 = P5_own * p_own + P5_nei * p_nei;
+    if (lowMach) {
+        double Mp = -Kp_ / fa * std::max(1.0 - sigma_ * Mmean2, 0.0) * (p_nei - p_own) / (0.5 * (in.rho_o + in.rho_n)) / sqr(a12);
+        double pu = -Ku_ * P5_own * P5_nei * (in.rho_o + in.rho_n) * a12 * fa * (Uv_own - Uv_nei);
+        Ma12 += Mp;
+        P12 += pu;
+    }
+    FaceOut o;
+    double ps = pos(Ma12), ng = neg0(Ma12);
+    o.Up = ps * U_own + ng * U_nei;
+    o.phi = Ma12 * a12 * (ps * in.rho_o + ng * in.rho_n) * in.magSf;
+    o.phiU = o.phi * o.Up + P12 * in.Sf;
+    double H_own = (rhoE_own + p_own) / in.rho_o, H_nei = (rhoE_nei + p_nei) / in.rho_n;
+    o.phiE = o.phi * (ps * H_own + ng * H_nei);
+    return o;
+}
+
+// HLLDST::computeFluxDST (HLLDST.C:91-204)
+static FaceOut fluxHLL(const FaceIn& in) {
+    V3 U_own = in.rhoU_o / in.rho_o, U_nei = in.rhoU_n / in.rho_n;
+    double E_own = in.e_o + 0.5 * magSqr(U_own), E_nei = in.e_n + 0.5 * magSqr(U_nei);
+    double p_own = in.rho_o * in.rPsi_o, p_nei = in.rho_n * in.rPsi_n;
+    double H_own = E_own + p_own / in.rho_o, H_nei = E_nei + p_nei / in.rho_n;
+    double coefR = std::sqrt(std::max(VSMALL, in.rho_n) / std::max(VSMALL, in.rho_o));
+    double HTilde = (coefR * H_nei + H_own) / (coefR + 1.0);
+    V3 UTilde = (coefR * U_nei + U_own) / (coefR + 1.0);
+    double aTilde = std::sqrt((in.gam_f - 1.0) * (HTilde - 0.5 * magSqr(UTilde)));
+    V3 nrm = in.Sf / in.magSf;
+    double Uv_own = dot(U_own, nrm), Uv_nei = dot(U_nei, nrm), UvTilde = dot(UTilde, nrm);
+    double S_own = std::min(Uv_own - in.a_o, UvTilde - aTilde);
+    double S_nei = std::max(Uv_nei + in.a_n, UvTilde + aTilde);
+    double cf_own = pos0(S_own), cf_m = neg(S_own) * pos0(S_nei), cf_nei = neg(S_nei) * neg(S_own);
+    V3 rhoUPhi_own = in.rho_o * U_own * Uv_own + p_own * nrm;
+    V3 rhoUPhi_nei = in.rho_n * U_nei * Uv_nei + p_nei * nrm;
+    double dS = S_nei - S_own;
+    FaceOut o;
+    o.phi = (cf_own * in.rho_o * Uv_own
+           + cf_m * (S_nei * in.rho_o * Uv_own - S_own * in.rho_n * Uv_nei + S_own * S_nei * (in.rho_n - in.rho_o)) / dS
+           + cf_nei * in.rho_n * Uv_nei) * in.magSf;
+    V3 mid = (S_nei * rhoUPhi_own - S_own * rhoUPhi_nei + S_own * S_nei * (in.rho_n * U_nei - in.rho_o * U_own)) / dS;
+    o.phiU = (cf_own * rhoUPhi_own + cf_m * mid + cf_nei * rhoUPhi_nei) * in.magSf;
+    o.phiE = (cf_own * in.rho_o * Uv_own * H_own
+            + cf_m * (S_nei * in.rho_o * Uv_own * H_own - S_own * in.rho_n * Uv_nei * H_nei + S_own * S_nei * (in.rho_n * E_nei - in.rho_o * E_own)) / dS
+            + cf_nei * in.rho_n * Uv_nei * H_nei) * in.magSf;
+    V3 upm = (S_nei * U_own - S_own * U_nei + Uv_own * U_own - Uv_nei * U_nei) / dS;
+    o.Up = cf_own * U_own + cf_m * upm + cf_nei * U_nei;
+    return o;
+}
+
+// HLLCDST::computeFluxDST (HLLCDST.C:135-272), directWaveSpeed = true
+// HLLCPDST::computeFluxDST (HLLCPDST.C:166-317) when hllcp
+static FaceOut fluxHLLC(const FaceIn& in, bool hllcp) {
+    V3 U_own = in.rhoU_o / in.rho_o, U_nei = in.rhoU_n / in.rho_n;
+    double E_own = in.e_o + 0.5 * magSqr(U_own), E_nei = in.e_n + 0.5 * magSqr(U_nei);
+    double p_own = in.rho_o * in.rPsi_o, p_nei = in.rho_n * in.rPsi_n;
+    double H_own = E_own + p_own / in.rho_o, H_nei = E_nei + p_nei / in.rho_n;
+    V3 normal = in.Sf / in.magSf;
+    double Uv_own = dot(U_own, normal), Uv_nei = dot(U_nei, normal);
+    double coefR = std::sqrt(std::max(VSMALL, in.rho_n) / std::max(VSMALL, in.rho_o));
+    double HTilde = (coefR * H_nei + H_own) / (coefR + 1.0);
+    V3 UTilde = (coefR * U_nei + U_own) / (coefR + 1.0);
+    double aTilde = std::sqrt((in.gam_f - 1.0) * (HTilde - 0.5 * magSqr(UTilde)));
+    double UvTilde = dot(UTilde, normal);
+    double S_own = std::min(Uv_own - in.a_o, UvTilde - aTilde);
+    double S_nei = std::max(Uv_nei + in.a_n, UvTilde + aTilde);
+    double S_star = (p_nei - p_own + in.rho_o * Uv_own * (S_own - Uv_own) - in.rho_n * Uv_nei * (S_nei - Uv_nei))
+                  / (in.rho_o * (S_own - Uv_own) - in.rho_n * (S_nei - Uv_nei));
+    double cf_own = pos(S_own);
+    double cf_m1 = neg0(S_own) * pos(S_star);
+    double cf_m2 = neg0(S_own) * neg0(S_star) * pos0(S_nei);
+    double cf_nei = neg0(S_own) * neg0(S_star) * neg(S_nei);
+    double rhoE_own = in.rho_o * E_own, rhoE_nei = in.rho_n * E_nei;
+    V3 rhoUPhi_own = in.rhoU_o * Uv_own + p_own * normal;
+    V3 rhoUPhi_nei = in.rhoU_n * Uv_nei + p_nei * normal;
+    double rhoEPhi_own = (rhoE_own + p_own) * Uv_own, rhoEPhi_nei = (rhoE_nei + p_nei) * Uv_nei;
+    FaceOut o;
+    if (!hllcp) {
+        double pstar_own = p_own + in.rho_o * (S_own - Uv_own) * (S_star - Uv_own);
+        double pstar_nei = p_nei + in.rho_n * (S_nei - Uv_nei) * (S_star - Uv_nei);
+        o.phi = (cf_own * in.rho_o * Uv_own
+               + cf_m1 * S_star * in.rho_o * (S_own - Uv_own) / (S_own - S_star)
+               + cf_m2 * S_star * in.rho_n * (S_nei - Uv_nei) / (S_nei - S_star)
+               + cf_nei * in.rho_n * Uv_nei) * in.magSf;
+        V3 m1 = (S_star * (S_own * in.rhoU_o - rhoUPhi_own) + S_own * pstar_own * normal) / (S_own - S_star);
+        V3 m2 = (S_star * (S_nei * in.rhoU_n - rhoUPhi_nei) + S_nei * pstar_nei * normal) / (S_nei - S_star);
+        o.phiU = (cf_own * rhoUPhi_own + cf_m1 * m1 + cf_m2 * m2 + cf_nei * rhoUPhi_nei) * in.magSf;
+        o.phiE = (cf_own * rhoEPhi_own
+                + cf_m1 * S_star * (S_own * rhoE_own - rhoEPhi_own + S_own * pstar_own) / (S_own - S_star)
+                + cf_m2 * S_star * (S_nei * rhoE_nei - rhoEPhi_nei + S_nei * pstar_nei) / (S_nei - S_star)
+                + cf_nei * rhoEPhi_nei) * in.magSf;
+    } else {
+        double Ma_own = Uv_own / in.a_o, Ma_nei = Uv_nei / in.a_n;
+        double MaTilde = (coefR * Ma_nei + Ma_own) / (coefR + 1.0);
+        double c_own = in.rho_o * (S_own - Uv_own), c_nei = in.rho_n * (S_nei - Uv_nei);
+        double p_star = (c_nei * p_own - c_own * p_nei - c_own * c_nei * (Uv_own - Uv_nei)) / (c_nei - c_own);
+        double theta = std::min(std::max(std::fabs(Ma_own), std::fabs(Ma_nei)), 1.0);
+        double p_av = 0.5 * (p_own + p_nei);
+        double p_ss = p_star * theta + (1.0 - theta) * p_av;
+        double f = in.fHLLCP;
+        double p_sss = f * p_ss + (1.0 - f) * p_star;
+        double phip = (f - 1.0) * S_own * S_nei / (S_nei - S_own) / (1.0 + std::fabs(MaTilde)) * (p_nei - p_own) / sqr(aTilde);
+        o.phi = (cf_own * in.rho_o * Uv_own
+               + cf_m1 * (S_star * in.rho_o * (S_own - Uv_own) / (S_own - S_star) + phip)
+               + cf_m2 * (S_star * in.rho_n * (S_nei - Uv_nei) / (S_nei - S_star) + phip)
+               + cf_nei * in.rho_n * Uv_nei) * in.magSf;
+        V3 m1 = (S_star * (S_own * in.rhoU_o - rhoUPhi_own) + S_own * p_sss * normal) / (S_own - S_star) + phip * UTilde;
+        V3 m2 = (S_star * (S_nei * in.rhoU_n - rhoUPhi_nei) + S_nei * p_sss * normal) / (S_nei - S_star) + phip * UTilde;
+        o.phiU = (cf_own * rhoUPhi_own + cf_m1 * m1 + cf_m2 * m2 + cf_nei * rhoUPhi_nei) * in.magSf;
+        double UT2 = magSqr(UTilde);
+        o.phiE = (cf_own * rhoEPhi_own
+                + cf_m1 * (S_star * (S_own * rhoE_own - rhoEPhi_own + S_own * p_star) / (S_own - S_star) + 0.5 * phip * UT2)
+                + cf_m2 * (S_star * (S_nei * rhoE_nei - rhoEPhi_nei + S_nei * p_star) / (S_nei - S_star) + 0.5 * phip * UT2)
+                + cf_nei * rhoEPhi_nei) * in.magSf;
+    }
+    o.Up = cf_own * U_own + cf_m1 * (S_own - Uv_own) * U_own / (S_own - S_star)
+         + cf_m2 * (S_nei - Uv_nei) * U_nei / (S_nei - S_star) + cf_nei * U_nei;
+    return o;
+}
+
+// KurganovDST::computeFluxDST (KurganovDST.C:93-167)
+static FaceOut fluxKurganov(const FaceIn& in) {
+    V3 U_own = in.rhoU_o / in.rho_o, U_nei = in.rhoU_n / in.rho_n;
+    double rhoE_own = in.rho_o * (in.e_o + 0.5 * magSqr(U_own));
+    double rhoE_nei = in.rho_n * (in.e_n + 0.5 * magSqr(U_nei));
+    double p_own = in.rho_o * in.rPsi_o, p_nei = in.rho_n * in.rPsi_n;
+    double phiv_own = dot(U_own, in.Sf), phiv_nei = dot(U_nei, in.Sf);
+    double cSf_own = in.c_o * in.magSf, cSf_nei = in.c_n * in.magSf;
+    double ap = std::max(std::max(phiv_own + cSf_own, phiv_nei + cSf_nei), 0.0);
+    double am = std::min(std::min(phiv_own - cSf_own, phiv_nei - cSf_nei), 0.0);
+    double a_own = ap / (ap - am);
+    double aSf = am * a_own;
+    double a_nei = 1.0 - a_own;
+    phiv_own *= a_own;
+    phiv_nei *= a_nei;
+    double aphiv_own = phiv_own - aSf, aphiv_nei = phiv_nei + aSf;
+    FaceOut o;
+    o.phi = aphiv_own * in.rho_o + aphiv_nei * in.rho_n;
+    o.phiU = (aphiv_own * in.rhoU_o + aphiv_nei * in.rhoU_n) + (a_own * p_own + a_nei * p_nei) * in.Sf;
+    o.phiE = aphiv_own * (rhoE_own + p_own) + aphiv_nei * (rhoE_nei + p_nei) + aSf * p_own - aSf * p_nei;
+    o.Up = a_own * U_own + a_nei * U_nei;
+    return o;
+}
+
+static FaceOut faceFlux(int scheme, const FaceIn& in, const rdst_settings& st) {
+    switch (scheme) {
+        case RDST_FLUX_AUSMPLUS: return fluxAUSMplus(in);
+        case RDST_FLUX_AUSMPLUSUP: return fluxAUSMplusUp(in, st.low_mach != 0, st.Mainf);
+        case RDST_FLUX_HLL: return fluxHLL(in);
+        case RDST_FLUX_HLLC: return fluxHLLC(in, false);
+        case RDST_FLUX_HLLCP: return fluxHLLC(in, true);
+        case RDST_FLUX_KURGANOV: return fluxKurganov(in);
+    }
+    throw std::runtime_error("unknown flux scheme");
+}
+
+// HLLCPDST::computef (HLLCPDST.C:131-164): f = interpolate(pow3(fCells)), fCells boundary = 1
+static void computeHLLCPf(const Case& c, std::vector<double>& f) {
+    const Mesh& m = c.m;
+    std::vector<double> fc(m.nCells, 1.0);
+    for (int i = 0; i < m.nCells; ++i)
+        for (int j : m.cellCells[i])
+            fc[i] = std::min(fc[i], std::min(c.p[i] / c.p[j], c.p[j] / c.p[i]));
+    std::vector<double> f3(m.nCells), b1(m.nB, 1.0);
+    for (int i = 0; i < m.nCells; ++i) f3[i] = fc[i] * fc[i] * fc[i];
+    linearInterp(m, f3, b1, f);
+}
+
+// computeFluxDST + calcFields (baseFluxDST.C:223-255), inviscid part
+static void computeFlux(Case& c) {
+    const Mesh& m = c.m;
+    CellAux a;
+    makeAux(c, a);
+    std::vector<double> rho_o, rho_n, rPsi_o, rPsi_n, e_o, e_n, a_o, a_n, c_o, c_n, gam_f, fH;
+    std::vector<V3> rhoU_o, rhoU_n;
+    reconstruct(m, c.rho, c.brho, rho_o, rho_n);
+    reconstruct(m, c.rhoU, c.brhoU, rhoU_o, rhoU_n);
+    reconstruct(m, a.rPsi, a.brPsi, rPsi_o, rPsi_n);
+    reconstruct(m, a.e, a.be, e_o, e_n);
+    reconstruct(m, a.aStar, a.baStar, a_o, a_n);
+    reconstruct(m, a.c, a.bc, c_o, c_n);
+    linearInterp(m, a.gam, a.bgam, gam_f);
+    if (c.st.flux_scheme == RDST_FLUX_HLLCP) computeHLLCPf(c, fH);
+    int nf = m.nInternal + m.nB;
+    for (int k = 0; k < nf; ++k) {
+        FaceIn in;
+        in.rho_o = rho_o[k]; in.rho_n = rho_n[k]; in.rPsi_o = rPsi_o[k]; in.rPsi_n = rPsi_n[k];
+        in.e_o = e_o[k]; in.e_n = e_n[k]; in.a_o = a_o[k]; in.a_n = a_n[k];
+        in.c_o = c_o[k]; in.c_n = c_n[k]; in.gam_f = gam_f[k];
+        in.fHLLCP = fH.empty() ? 1.0 : fH[k];
+        in.rhoU_o = rhoU_o[k]; in.rhoU_n = rhoU_n[k];
+        in.Sf = SfOf(m, k); in.magSf = magSfOf(m, k);
+        FaceOut o = faceFlux(c.st.flux_scheme, in, c.st);
+        c.phi[k] = o.phi; c.phiUp[k] = o.phiU; c.phiEp[k] = o.phiE; c.Up[k] = o.Up;
+    }
+    // calcFields: fvc::div = surfaceIntegrate (sum owner +, neighbour -, boundary +) / V
+    int n = m.nCells;
+    std::vector<double> d0(n, 0.0), d4(n, 0.0);
+    std::vector<V3> d1(n, v3(0, 0, 0));
+    for (int f = 0; f < m.nInternal; ++f) {
+        int o = m.owner[f], nb = m.neighbour[f];
+        d0[o] += c.phi[f]; d0[nb] -= c.phi[f];
+        d1[o] = d1[o] + c.phiUp[f]; d1[nb] = d1[nb] - c.phiUp[f];
+        d4[o] += c.phiEp[f]; d4[nb] -= c.phiEp[f];
+    }
+    for (int b = 0; b < m.nB; ++b) {
+        int ci = m.bCell[b], k = m.nInternal + b;
+        d0[ci] += c.phi[k]; d1[ci] = d1[ci] + c.phiUp[k]; d4[ci] += c.phiEp[k];
+    }
+    for (int i = 0; i < n; ++i) {
+        c.R[5 * i] = d0[i] / m.V[i];
+        c.R[5 * i + 1] = d1[i].x / m.V[i]; c.R[5 * i + 2] = d1[i].y / m.V[i]; c.R[5 * i + 3] = d1[i].z / m.V[i];
+        c.R[5 * i + 4] = d4[i] / m.V[i];
+    }
+}
+
+// ------------------------------------------------------------------ Jacobian
+// Euler flux Jacobian A(W).n in the Blazek A.7 form, Roe::calcJ (Roe.C:247-272)
+static void eulerJacobian(V3 U, double gamma, double rhoE, double rho, V3 n, double* J) {
+    double phiJ = 0.5 * (gamma - 1) * magSqr(U);
+    double a1 = gamma * rhoE / rho - phiJ;
+    for (int i = 0; i < 25; ++i) J[i] = 0.0;
+    double Un = dot(U, n);
+    // insertScalarEqn(nSf, 0, 1): row 0, cols 1..3
+    J[1] = n.x; J[2] = n.y; J[3] = n.z;
+    // insertVectorEqn(phiJ*n - (U*U)&n, 1, 0): column 0, rows 1..3
+    for (int i = 0; i < 3; ++i) J[5 * (1 + i)] = phiJ * cmp(n, i) - cmp(U, i) * Un;
+    // insertVectorEqn(U*n + (U&n) I - (gamma-1)(n*U), 1, 1)
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            J[5 * (1 + i) + 1 + j] = cmp(U, i) * cmp(n, j) + (i == j ? Un : 0.0) - (gamma - 1) * (cmp(n, i) * cmp(U, j));
+    // insertVectorEqn((gamma-1) n, 1, 4): column 4
+    for (int i = 0; i < 3; ++i) J[5 * (1 + i) + 4] = (gamma - 1) * cmp(n, i);
+    // insertScalarEqn((U&n)(phiJ - a1), 4, 0)
+    J[20] = Un * (phiJ - a1);
+    // insertScalarEqn(a1 n - (U&n)(gamma-1) U, 4, 1)
+    for (int j = 0; j < 3; ++j) J[21 + j] = a1 * cmp(n, j) - Un * (gamma - 1) * cmp(U, j);
+    // insertScalarEqn(gamma (U&n), 4, 4)
+    J[24] = gamma * Un;
+}
+
+static void matmul5(const double* A, const double* B, double* C) {
+    for (int i = 0; i < 5; ++i)
+        for (int j = 0; j < 5; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < 5; ++k) s += A[5 * i + k] * B[5 * k + j];
+            C[5 * i + j] = s;
+        }
+}
+
+// Roe::computeJacobianDST (Roe.C:152-214) + computeBoundaryJacobians (:274-379)
+static void computeJacobian(Case& c) {
+    const Mesh& m = c.m;
+    int n = m.nCells;
+    std::vector<V3> Uc(n);
+    std::vector<double> gam(n);
+    for (int i = 0; i < n; ++i) { Uc[i] = c.rhoU[i] / c.rho[i]; gam[i] = c.th.Cp / c.Cv; }
+    c.ownC.assign(25 * m.nInternal, 0.0);
+    c.neiC.assign(25 * m.nInternal, 0.0);
+    for (int f = 0; f < m.nInternal; ++f) {
+        V3 nSf = m.Sf[f] / m.magSf[f];
+        int o = m.owner[f], nb = m.neighbour[f];
+        eulerJacobian(Uc[o], gam[o], c.rhoE[o], c.rho[o], nSf, &c.ownC[25 * f]);
+        eulerJacobian(Uc[nb], gam[nb], c.rhoE[nb], c.rho[nb], nSf, &c.neiC[25 * f]);
+        for (int k = 0; k < 25; ++k) {
+            c.ownC[25 * f + k] *= m.w[f];
+            c.neiC[25 * f + k] *= (1 - m.w[f]);
+        }
+    }
+    // spectral radius (Roe.C:181-201)
+    std::vector<V3> bUc(m.nB);
+    for (int b = 0; b < m.nB; ++b) bUc[b] = c.brhoU[b] / c.brho[b];
+    // interpolate(U_) with U_ = rhoU/rho (boundary values rhoU_b/rho_b)
+    std::vector<V3> Uf;
+    linearInterpV(m, Uc, bUc, Uf);
+    std::vector<double> cc(n), bcc(m.nB), cf;
+    for (int i = 0; i < n; ++i) cc[i] = std::sqrt(c.th.Cp / c.Cv / c.psi[i]);
+    for (int b = 0; b < m.nB; ++b) bcc[b] = std::sqrt(c.th.Cp / c.Cv / c.bpsi[b]);
+    linearInterp(m, cc, bcc, cf);
+    std::vector<double> muEffSf, alphaEffSf, rhoSf, trho(n), btrho(m.nB);
+    linearInterp(m, c.mu, c.bmu, muEffSf);
+    linearInterp(m, c.alpha, c.balpha, alphaEffSf);
+    for (int i = 0; i < n; ++i) trho[i] = c.psi[i] * c.p[i];       // thermo.rho() = psi*p
+    for (int b = 0; b < m.nB; ++b) btrho[b] = c.bpsi[b] * c.bp[b];
+    linearInterp(m, trho, btrho, rhoSf);
+    c.lambda.assign(m.nInternal, 0.0);
+    for (int f = 0; f < m.nInternal; ++f) {
+        double amaxSf = std::fabs(dot(Uf[f], m.Sf[f])) + m.magSf[f] * cf[f];
+        c.lambda[f] = 0.5 * (amaxSf + m.magSf[f] * (muEffSf[f] + alphaEffSf[f]) / rhoSf[f] * m.dc[f]);
+    }
+    // boundary Jacobians on non-coupled patches (Roe.C:306-377)
+    c.patchIC.assign(25 * m.nB, 0.0);
+    for (int b = 0; b < m.nB; ++b) {
+        int f = m.bFace[b], ci = m.bCell[b];
+        const rdst_bc& bc = c.bc[m.bPatch[b]];
+        V3 pSf = m.Sf[f];
+        V3 UB = c.brhoU[b] / c.brho[b];
+        double pB = c.bp[b], TB = c.bT[b], rhoB = c.brho[b], CvB = c.Cv;
+        double EB = c.brhoE[b] / c.brho[b];
+        double rRT = rhoB / pB;
+        double USf = dot(UB, pSf);
+        double dR[25] = {0};
+        dR[0] = USf * rRT;
+        dR[1] = rhoB * pSf.x; dR[2] = rhoB * pSf.y; dR[3] = rhoB * pSf.z;
+        dR[4] = -rhoB / TB * USf;
+        for (int i = 0; i < 3; ++i) {
+            dR[5 * (1 + i)] = rRT * USf * cmp(UB, i) + cmp(pSf, i);
+            for (int j = 0; j < 3; ++j)
+                dR[5 * (1 + i) + 1 + j] = rhoB * cmp(UB, i) * cmp(pSf, j) + (i == j ? rhoB * USf : 0.0);
+            dR[5 * (1 + i) + 4] = -rhoB / TB * USf * cmp(UB, i);
+        }
+        dR[20] = (EB * rRT + 1) * USf;
+        for (int j = 0; j < 3; ++j) dR[21 + j] = rhoB * USf * cmp(UB, j) + cmp(pSf, j) * (rhoB * EB + pB);
+        dR[24] = rhoB * CvB * USf;
+        // dX/dW at the patch-internal cell
+        V3 UI = Uc[ci];
+        double rhoEI = c.rhoE[ci], pI = c.p[ci], rRho = 1.0 / c.rho[ci], Cv = c.Cv;
+        double rRhoCv = rRho / Cv;
+        double e = rhoEI * rRho - 0.5 * magSqr(UI);
+        double RT = pI * rRho;
+        double RCv = RT / e;
+        double dX[25] = {0};
+        dX[0] = RT - RCv * (e - 0.5 * magSqr(UI));
+        dX[1] = -RCv * UI.x; dX[2] = -RCv * UI.y; dX[3] = -RCv * UI.z;
+        dX[4] = RCv;
+        for (int i = 0; i < 3; ++i) {
+            dX[5 * (1 + i)] = -rRho * cmp(UI, i);
+            dX[5 * (1 + i) + 1 + i] = rRho;
+        }
+        dX[20] = rRhoCv * (0.5 * magSqr(UI) - e);
+        dX[21] = -rRhoCv * UI.x; dX[22] = -rRhoCv * UI.y; dX[23] = -rRhoCv * UI.z;
+        dX[24] = rRhoCv;
+        // dXb/dXi = diag(valueInternalCoeffs)
+        V3 nf = pSf / m.magSf[f];
+        V3 uvic = vectorVIC(bc.U_type, c.bcs.vf_U[b], nf);
+        double D[25] = {0};
+        D[0] = scalarVIC(bc.p_type, c.bcs.vf_p[b]);
+        D[6] = uvic.x; D[12] = uvic.y; D[18] = uvic.z;
+        D[24] = scalarVIC(bc.T_type, c.bcs.vf_T[b]);
+        double tmp[25];
+        matmul5(dR, D, tmp);
+        matmul5(tmp, dX, &c.patchIC[25 * b]);
+    }
+}
+
+// ------------------------------------------------------------------ assembly
+// fvmDST::ddt (fvmDST.C:77-143) with steadyStateDST::fvmDdt (steadyStateDST.C:173-198)
+// + fluxDST::integrate (fluxDST.C:73-127), summed by operator+ (fvBlockMatrixDST.C:201-213)
+static void assemble(Case& c) {
+    const Mesh& m = c.m;
+    int n = m.nCells, F = m.nInternal;
+    double beta = c.st.beta;
+    std::vector<double> dd(25 * n, 0.0), sd(5 * n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        double diagS = c.rDeltaT[i] * m.V[i];
+        for (int r = 0; r < 5; ++r) dd[25 * i + 6 * r] = diagS;  // I * rhoEqn.diag()
+        // source - diag*psi with source = rDeltaT*psi_old*V and psi_old = psi
+        double x[5] = {c.rho[i], c.rhoU[i].x, c.rhoU[i].y, c.rhoU[i].z, c.rhoE[i]};
+        for (int k = 0; k < 5; ++k) sd[5 * i + k] = c.rDeltaT[i] * x[k] * m.V[i] - diagS * x[k];
+    }
+    // flux matrix
+    std::vector<double> fd(25 * n, 0.0);
+    c.lower.assign(25 * F, 0.0); c.upper.assign(25 * F, 0.0);
+    for (int f = 0; f < F; ++f) {
+        for (int k = 0; k < 25; ++k) {
+            double lam = (k % 6 == 0) ? c.lambda[f] : 0.0;
+            c.lower[25 * f + k] = -beta * (c.ownC[25 * f + k] * m.magSf[f] + lam);
+            c.upper[25 * f + k] = beta * (c.neiC[25 * f + k] * m.magSf[f] - lam);
+        }
+    }
+    // negSumDiag
+    for (int f = 0; f < F; ++f) {
+        int o = m.owner[f], nb = m.neighbour[f];
+        for (int k = 0; k < 25; ++k) {
+            fd[25 * o + k] -= c.lower[25 * f + k];
+            fd[25 * nb + k] -= c.upper[25 * f + k];
+        }
+    }
+    for (int b = 0; b < m.nB; ++b) {
+        int ci = m.bCell[b];
+        for (int k = 0; k < 25; ++k) fd[25 * ci + k] += beta * (c.patchIC[25 * b + k] + 0.0);  // spectPatch = 0
+    }
+    std::vector<double> fs(5 * n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < 5; ++k) fs[5 * i + k] -= c.R[5 * i + k] * m.V[i];
+    c.diag.resize(25 * n);
+    c.source.resize(5 * n);
+    for (int i = 0; i < 25 * n; ++i) c.diag[i] = dd[i] + fd[i];
+    for (int i = 0; i < 5 * n; ++i) c.source[i] = sd[i] + fs[i];
+    c.hasMatrix = true;
+}
+
+// ------------------------------------------------------------------ linear algebra on LDU
+static void Amul(const Case& c, const std::vector<double>& x, std::vector<double>& y) {
+    const Mesh& m = c.m;
+    int n = m.nCells;
+    y.assign(5 * n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int r = 0; r < 5; ++r) {
+            double s = 0;
+            for (int k = 0; k < 5; ++k) s += c.diag[25 * i + 5 * r + k] * x[5 * i + k];
+            y[5 * i + r] = s;
+        }
+    for (int f = 0; f < m.nInternal; ++f) {
+        int l = m.owner[f], u = m.neighbour[f];
+        for (int r = 0; r < 5; ++r) {
+            double su = 0, sl = 0;
+            for (int k = 0; k < 5; ++k) {
+                sl += c.lower[25 * f + 5 * r + k] * x[5 * l + k];
+                su += c.upper[25 * f + 5 * r + k] * x[5 * u + k];
+            }
+            y[5 * u + r] += sl;
+            y[5 * l + r] += su;
+        }
+    }
+}
+
+// 5x5 inverse, Gauss-Jordan with partial pivoting
+static bool inv5(const double* A, double* Ainv) {
+    double a[5][10];
+    for (int i = 0; i < 5; ++i)
+        for (int j = 0; j < 5; ++j) { a[i][j] = A[5 * i + j]; a[i][5 + j] = (i == j) ? 1.0 : 0.0; }
+    for (int col = 0; col < 5; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < 5; ++r) if (std::fabs(a[r][col]) > std::fabs(a[piv][col])) piv = r;
+        if (a[piv][col] == 0.0) return false;
+        if (piv != col) for (int j = 0; j < 10; ++j) std::swap(a[piv][j], a[col][j]);
+        double d = 1.0 / a[col][col];
+        for (int j = 0; j < 10; ++j) a[col][j] *= d;
+        for (int r = 0; r < 5; ++r) {
+            if (r == col) continue;
+            double fct = a[r][col];
+            if (fct == 0.0) continue;
+            for (int j = 0; j < 10; ++j) a[r][j] -= fct * a[col][j];
+        }
+    }
+    for (int i = 0; i < 5; ++i) for (int j = 0; j < 5; ++j) Ainv[5 * i + j] = a[i][5 + j];
+    return true;
+}
+
+// Block ILU(0) on the cell-ordered sparsity of the LDU matrix.  ILU(0)
+// factors are unique for a fixed ordering and pattern, so this is the
+// factor of foam-extend's ILUC0 (natural order) up to rounding.
+struct ILU {
+    int n = 0;
+    std::vector<int> rowPtr, col;     // full pattern, sorted columns per row
+    std::vector<int> diagPos;
+    std::vector<double> val;          // [nnz][25]; L strictly lower (unit diag), U upper, diag inverted stored separately
+    std::vector<double> dinv;         // [n][25]
+};
+
+static void buildILU(const Case& c, ILU& il) {
+    const Mesh& m = c.m;
+    int n = m.nCells;
+    il.n = n;
+    std::vector<std::vector<std::pair<int, const double*>>> rows(n);
+    for (int i = 0; i < n; ++i) rows[i].push_back({i, &c.diag[25 * i]});
+    for (int f = 0; f < m.nInternal; ++f) {
+        int o = m.owner[f], nb = m.neighbour[f];
+        rows[o].push_back({nb, &c.upper[25 * f]});   // owner row, neighbour column
+        rows[nb].push_back({o, &c.lower[25 * f]});   // neighbour row, owner column
+    }
+    il.rowPtr.assign(n + 1, 0);
+    for (int i = 0; i < n; ++i) {
+        std::sort(rows[i].begin(), rows[i].end(), [](const std::pair<int, const double*>& a, const std::pair<int, const double*>& b) { return a.first < b.first; });
+        il.rowPtr[i + 1] = il.rowPtr[i] + (int)rows[i].size();
+    }
+    il.col.resize(il.rowPtr[n]);
+    il.val.resize(25 * (size_t)il.rowPtr[n]);
+    il.diagPos.resize(n);
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < (int)rows[i].size(); ++k) {
+            int pos = il.rowPtr[i] + k;
+            il.col[pos] = rows[i][k].first;
+            if (rows[i][k].first == i) il.diagPos[i] = pos;
+            std::memcpy(&il.val[25 * (size_t)pos], rows[i][k].second, 25 * sizeof(double));
+        }
+    // IKJ ILU(0)
+    il.dinv.assign(25 * (size_t)n, 0.0);
+    std::vector<int> where(n, -1);
+    for (int i = 0; i < n; ++i) {
+        for (int p = il.rowPtr[i]; p < il.rowPtr[i + 1]; ++p) where[il.col[p]] = p;
+        for (int p = il.rowPtr[i]; p < il.rowPtr[i + 1]; ++p) {
+            int k = il.col[p];
+            if (k >= i) break;
+            // L_ik = A_ik * inv(U_kk)
+            double t[25];
+            matmul5(&il.val[25 * (size_t)p], &il.dinv[25 * (size_t)k], t);
+            std::memcpy(&il.val[25 * (size_t)p], t, sizeof t);
+            for (int q = il.diagPos[k] + 1; q < il.rowPtr[k + 1]; ++q) {
+                int j = il.col[q];
+                int pj = where[j];
+                if (pj < 0) continue;
+                double prod[25];
+                matmul5(t, &il.val[25 * (size_t)q], prod);
+                for (int e = 0; e < 25; ++e) il.val[25 * (size_t)pj + e] -= prod[e];
+            }
+        }
+        if (!inv5(&il.val[25 * (size_t)il.diagPos[i]], &il.dinv[25 * (size_t)i]))
+            throw std::runtime_error("singular ILU pivot block");
+        for (int p = il.rowPtr[i]; p < il.rowPtr[i + 1]; ++p) where[il.col[p]] = -1;
+    }
+}
+
+static void mv5(const double* A, const double* x, double* y) {
+    for (int r = 0; r < 5; ++r) {
+        double s = 0;
+        for (int k = 0; k < 5; ++k) s += A[5 * r + k] * x[k];
+        y[r] = s;
+    }
+}
+
+static void iluApply(const ILU& il, const std::vector<double>& r, std::vector<double>& x) {
+    int n = il.n;
+    std::vector<double> y(r);
+    for (int i = 0; i < n; ++i) {
+        for (int p = il.rowPtr[i]; p < il.diagPos[i]; ++p) {
+            double t[5];
+            mv5(&il.val[25 * (size_t)p], &y[5 * (size_t)il.col[p]], t);
+            for (int k = 0; k < 5; ++k) y[5 * i + k] -= t[k];
+        }
+    }
+    x.assign(5 * (size_t)n, 0.0);
+    for (int i = n - 1; i >= 0; --i) {
+        double s[5];
+        for (int k = 0; k < 5; ++k) s[k] = y[5 * i + k];
+        for (int p = il.diagPos[i] + 1; p < il.rowPtr[i + 1]; ++p) {
+            double t[5];
+            mv5(&il.val[25 * (size_t)p], &x[5 * (size_t)il.col[p]], t);
+            for (int k = 0; k < 5; ++k) s[k] -= t[k];
+        }
+        mv5(&il.dinv[25 * (size_t)i], s, &x[5 * i]);
+    }
+}
+
+struct Precond {
+    int kind;
+    const Case* c;
+    ILU il;
+    std::vector<double> dinv;
+    void init(const Case& cc, int k) {
+        c = &cc; kind = k;
+        if (k == RDST_PRECOND_ILU0) buildILU(cc, il);
+        else if (k == RDST_PRECOND_DIAG) {
+            dinv.resize(cc.diag.size());
+            for (int i = 0; i < cc.m.nCells; ++i) inv5(&cc.diag[25 * i], &dinv[25 * i]);
+        }
+    }
+    void apply(std::vector<double>& out, const std::vector<double>& in) const {
+        if (kind == RDST_PRECOND_ILU0) iluApply(il, in, out);
+        else if (kind == RDST_PRECOND_DIAG) {
+            out.resize(in.size());
+            for (int i = 0; i < c->m.nCells; ++i) mv5(&dinv[25 * i], &in[5 * i], &out[5 * i]);
+        } else out = in;
+    }
+};
+
+static double sumProd(const std::vector<double>& a, const std::vector<double>& b) {
+    double s = 0;
+    for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+    return s;
+}
+// gSum(cmptMag(r)) per component
+static void cmptMagSum(const std::vector<double>& r, double* out) {
+    for (int k = 0; k < 5; ++k) out[k] = 0;
+    for (size_t i = 0; i < r.size() / 5; ++i)
+        for (int k = 0; k < 5; ++k) out[k] += std::fabs(r[5 * i + k]);
+}
+static double mag5(const double* v) {
+    double s = 0;
+    for (int k = 0; k < 5; ++k) s += v[k] * v[k];
+    return std::sqrt(s);
+}
+
+// BlockLduSolver::normFactor
+static double normFactor(const Case& c, const std::vector<double>& x, const std::vector<double>& b) {
+    int n = c.m.nCells;
+    double xRef[5] = {0};
+    for (int i = 0; i < n; ++i) for (int k = 0; k < 5; ++k) xRef[k] += x[5 * i + k];
+    for (int k = 0; k < 5; ++k) xRef[k] /= n;
+    std::vector<double> wA, pA, xr(5 * (size_t)n);
+    Amul(c, x, wA);
+    for (int i = 0; i < n; ++i) for (int k = 0; k < 5; ++k) xr[5 * i + k] = xRef[k];
+    Amul(c, xr, pA);
+    double s = 0;
+    for (int i = 0; i < n; ++i) {
+        double d1[5], d2[5];
+        for (int k = 0; k < 5; ++k) { d1[k] = wA[5 * i + k] - pA[5 * i + k]; d2[k] = b[5 * i + k] - pA[5 * i + k]; }
+        s += mag5(d1) + mag5(d2);
+    }
+    return s + BLOCK_SMALL;
+}
+
+struct Perf { double init[5], fin[5]; int nIter; bool conv; };
+
+static bool checkConvergence(const rdst_settings& st, Perf& p) {
+    double fm = mag5(p.fin), im = mag5(p.init);
+    p.conv = (fm < st.tolerance) || (st.rel_tol > SMALL && fm <= st.rel_tol * im);
+    return p.conv;
+}
+static bool converged(const rdst_settings& st, Perf& p) {
+    return p.nIter >= st.min_iter && (p.nIter >= st.max_iter || checkConvergence(st, p));
+}
+
+// Givens rotation as in foam-extend's GMRES
+static void givens(double h, double beta, double& cs, double& sn) {
+    if (beta == 0) { cs = 1; sn = 0; }
+    else if (std::fabs(beta) > std::fabs(h)) { double tau = -h / beta; sn = 1.0 / std::sqrt(1.0 + tau * tau); cs = sn * tau; }
+    else { double tau = -beta / h; cs = 1.0 / std::sqrt(1.0 + tau * tau); sn = cs * tau; }
+}
+
+// foam-extend BlockGMRESSolver<vector5>::solve: left-preconditioned
+// restarted GMRES(nDirections), modified Gram-Schmidt, one "iteration" per restart
+static Perf gmres(const Case& c, const Precond& M, std::vector<double>& x, const std::vector<double>& b) {
+    const rdst_settings& st = c.st;
+    size_t N5 = x.size();
+    int nDirs = st.n_directions;
+    Perf perf = {};
+    double norm = normFactor(c, x, b);
+    std::vector<double> wA, rA(N5);
+    Amul(c, x, wA);
+    for (size_t i = 0; i < N5; ++i) rA[i] = b[i] - wA[i];
+    cmptMagSum(rA, perf.init);
+    for (int k = 0; k < 5; ++k) { perf.init[k] /= norm; perf.fin[k] = perf.init[k]; }
+    if (!converged(st, perf)) {
+        std::vector<double> H((size_t)nDirs * nDirs, 0.0), yh(nDirs, 0.0), bh(nDirs + 1, 0.0), cs(nDirs, 0.0), sn(nDirs, 0.0);
+        std::vector<std::vector<double>> Vk(nDirs + 1, std::vector<double>(N5, 0.0));
+        do {
+            M.apply(wA, rA);
+            double beta = std::sqrt(sumProd(wA, wA));
+            std::fill(bh.begin(), bh.end(), 0.0);
+            bh[0] = beta;
+            for (int i = 0; i < nDirs; ++i) {
+                for (size_t q = 0; q < N5; ++q) Vk[i][q] = wA[q] / beta;
+                Amul(c, Vk[i], rA);
+                M.apply(wA, rA);
+                for (int j = 0; j <= i; ++j) {
+                    beta = sumProd(wA, Vk[j]);
+                    H[(size_t)j * nDirs + i] = beta;
+                    for (size_t q = 0; q < N5; ++q) wA[q] -= beta * Vk[j][q];
+                }
+                beta = std::sqrt(sumProd(wA, wA));
+                for (int j = 0; j < i; ++j) {
+                    double Hji = H[(size_t)j * nDirs + i];
+                    H[(size_t)j * nDirs + i] = cs[j] * Hji - sn[j] * H[(size_t)(j + 1) * nDirs + i];
+                    H[(size_t)(j + 1) * nDirs + i] = sn[j] * Hji + cs[j] * H[(size_t)(j + 1) * nDirs + i];
+                }
+                givens(H[(size_t)i * nDirs + i], beta, cs[i], sn[i]);
+                double bhi = bh[i];
+                bh[i] = cs[i] * bhi - sn[i] * bh[i + 1];
+                bh[i + 1] = sn[i] * bhi + cs[i] * bh[i + 1];
+                H[(size_t)i * nDirs + i] = cs[i] * H[(size_t)i * nDirs + i] - sn[i] * beta;
+            }
+            for (int i = nDirs - 1; i >= 0; --i) {
+                double s = bh[i];
+                for (int j = i + 1; j < nDirs; ++j) s -= H[(size_t)i * nDirs + j] * yh[j];
+                yh[i] = s / H[(size_t)i * nDirs + i];
+            }
+            for (int i = 0; i < nDirs; ++i)
+                for (size_t q = 0; q < N5; ++q) x[q] += yh[i] * Vk[i][q];
+            Amul(c, x, wA);
+            for (size_t q = 0; q < N5; ++q) rA[q] = b[q] - wA[q];
+            cmptMagSum(rA, perf.fin);
+            for (int k = 0; k < 5; ++k) perf.fin[k] /= norm;
+            perf.nIter++;
+        } while (!converged(st, perf));
+    }
+    return perf;
+}
+
+// foam-extend BlockBiCGStabSolver<vector5>::solve (right-preconditioned)
+static Perf bicgstab(const Case& c, const Precond& M, std::vector<double>& x, const std::vector<double>& b) {
+    const rdst_settings& st = c.st;
+    size_t N5 = x.size();
+    Perf perf = {};
+    double norm = normFactor(c, x, b);
+    std::vector<double> p, r(N5);
+    Amul(c, x, p);
+    for (size_t i = 0; i < N5; ++i) r[i] = b[i] - p[i];
+    cmptMagSum(r, perf.init);
+    for (int k = 0; k < 5; ++k) { perf.init[k] /= norm; perf.fin[k] = perf.init[k]; }
+    if (!converged(st, perf)) {
+        double rho = BLOCK_GREAT, rhoOld = rho, alpha = 0, omega = BLOCK_GREAT, beta;
+        p.assign(N5, 0.0);
+        std::vector<double> ph(N5, 0.0), v(N5, 0.0), t(N5, 0.0), th(N5, 0.0), rw(r);
+        do {
+            rhoOld = rho;
+            rho = sumProd(rw, r);
+            beta = rho / rhoOld * (alpha / omega);
+            for (size_t i = 0; i < N5; ++i) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+            M.apply(ph, p);
+            Amul(c, ph, v);
+            alpha = rho / sumProd(rw, v);
+            for (size_t i = 0; i < N5; ++i) r[i] -= alpha * v[i];
+            M.apply(th, r);
+            Amul(c, th, t);
+            omega = sumProd(t, r) / sumProd(t, t);
+            for (size_t i = 0; i < N5; ++i) x[i] += alpha * ph[i] + omega * th[i];
+            for (size_t i = 0; i < N5; ++i) r[i] -= omega * t[i];
+            cmptMagSum(r, perf.fin);
+            for (int k = 0; k < 5; ++k) perf.fin[k] /= norm;
+            perf.nIter++;
+        } while (!converged(st, perf));
+    }
+    return perf;
+}
+
+// fvBlockMatrixDST::solve (fvBlockMatrixDST.C:98-155)
+static Perf solve(Case& c) {
+    int n = c.m.nCells;
+    Precond M;
+    M.init(c, c.st.precond);
+    Perf perf;
+    if (c.st.dependent == RDST_DEP_W) {
+        std::vector<double> sourceW;
+        Amul(c, c.W, sourceW);
+        for (int i = 0; i < 5 * n; ++i) sourceW[i] += c.source[i];
+        perf = (c.st.solver == RDST_SOLVER_GMRES) ? gmres(c, M, c.W, sourceW) : bicgstab(c, M, c.W, sourceW);
+    } else {
+        std::vector<double> dW(5 * (size_t)n, 0.0);
+        perf = (c.st.solver == RDST_SOLVER_GMRES) ? gmres(c, M, dW, c.source) : bicgstab(c, M, dW, c.source);
+        for (int i = 0; i < 5 * n; ++i) c.W[i] += dW[i];
+    }
+    return perf;
+}
+
+// ------------------------------------------------------------------ updateFields.H
+static inline double clampv(double v, double lo, double hi) { return std::min(std::max(v, lo), hi); }
+
+static void updateFields(Case& c) {
+    Mesh& m = c.m;
+    int n = m.nCells, nB = m.nB;
+    double Cp = c.th.Cp, Cv = c.Cv, gamma = Cp / Cv;
+    // old values the BCs read before thermo.correct()
+    std::vector<double> pOld = c.p, TOld = c.T, psiOld = c.psi;
+    std::vector<double> bpsiOld = c.bpsi, brhoOld = c.brho;
+    // DST::retrieve (updateFields.H:40-42), boundMinMax(rho) as a clamp
+    for (int i = 0; i < n; ++i) {
+        c.rho[i] = clampv(c.W[5 * i], c.st.rho_min, c.st.rho_max);
+        c.rhoU[i] = v3(c.W[5 * i + 1], c.W[5 * i + 2], c.W[5 * i + 3]);
+        c.rhoE[i] = c.W[5 * i + 4];
+        c.U[i] = c.rhoU[i] / c.rho[i];
+    }
+    // U.correctBoundaryConditions()
+    std::vector<double> TrefH(nB, 0.0);
+    for (int b = 0; b < nB; ++b) {
+        const rdst_bc& bc = c.bc[m.bPatch[b]];
+        int ci = m.bCell[b], f = m.bFace[b];
+        V3 nrm = m.Sf[f] / m.magSf[f];
+        double phib = c.phi[m.nInternal + b];
+        V3 UI = c.U[ci];
+        switch (bc.U_type) {
+            case RDST_BC_FIXED_VALUE: c.bU[b] = v3(bc.U_value[0], bc.U_value[1], bc.U_value[2]); break;
+            case RDST_BC_ZERO_GRADIENT: c.bU[b] = UI; break;
+            case RDST_BC_SYMMETRY: c.bU[b] = symmetricValue(nrm, UI); break;
+            case RDST_BC_INLET_OUTLET: {
+                double vf = 1.0 - fe_pos(phib);
+                c.bcs.vf_U[b] = vf;
+                V3 ref = v3(bc.U_value[0], bc.U_value[1], bc.U_value[2]);
+                c.bU[b] = vf * ref + (1.0 - vf) * UI;
+                break;
+            }
+            case RDST_BC_FARFIELD: {
+                // farFieldVelocityDST::updateCoeffs (U.C:144-217)
+                double cb = std::sqrt(Cp / Cv / bpsiOld[b]);
+                double cI = std::sqrt(Cp / Cv / psiOld[ci]);
+                double mach = phib / brhoOld[b] / m.magSf[f] / cb;
+                double supIn = neg0(mach + 1), supOut = pos0(mach - 1);
+                double subIn = neg(mach) - supIn, subOut = pos0(mach) - supOut;
+                double rho0c0 = c.rho[ci] * cI;
+                V3 Uinf = v3(bc.U_inf[0], bc.U_inf[1], bc.U_inf[2]);
+                double pb = 0.5 * (bc.p_inf + pOld[ci] - rho0c0 * dot(nrm, Uinf - UI));
+                V3 subsInValue = Uinf - nrm * (bc.p_inf - pb) / rho0c0;
+                V3 subsOutValue = nrm * (pOld[ci] - pb) / rho0c0;
+                V3 ref = supIn * Uinf + subIn * subsInValue + subOut * subsOutValue;
+                double vf = 1 - supOut - subOut;
+                c.bcs.vf_U[b] = vf;
+                c.bU[b] = vf * ref + (1.0 - vf) * UI;
+                break;
+            }
+            default: break;
+        }
+        // T boundary coefficients evaluated inside h.correctBoundaryConditions()
+        // (mixedEnthalpy calls Tw.evaluate() with the not-yet-corrected T, p)
+        if (bc.T_type == RDST_BC_INLET_OUTLET) {
+            c.bcs.vf_T[b] = 1.0 - fe_pos(phib);
+            TrefH[b] = bc.T_value;
+        } else if (bc.T_type == RDST_BC_FARFIELD) {
+            // farFieldTemperatureDST::updateCoeffs (T.C:144-221)
+            double cb = std::sqrt(Cp / Cv / bpsiOld[b]);
+            double cI = std::sqrt(Cp / Cv / psiOld[ci]);
+            double mach = phib / brhoOld[b] / m.magSf[f] / cb;
+            double supIn = neg0(mach + 1), supOut = pos0(mach - 1);
+            double subIn = neg(mach) - supIn, subOut = pos0(mach) - supOut;
+            double rho0c0 = c.rho[ci] * cI;
+            V3 Uinf = v3(bc.U_inf[0], bc.U_inf[1], bc.U_inf[2]);
+            double pI = pOld[ci], TI = TOld[ci];
+            double pb = 0.5 * (bc.p_inf + pI - rho0c0 * dot(nrm, Uinf - UI));
+            double RI = pI / c.rho[ci] / TI;
+            double rhoa = bc.p_inf / bc.T_inf / RI;
+            double subsInValue = pb / (rhoa + (pb - bc.p_inf) / sqr(cI)) / RI;
+            double subsOutVF = pb / (pI + RI * TI * (pb - pI) / sqr(cI));
+            TrefH[b] = supIn * bc.T_inf + subIn * subsInValue;
+            c.bcs.vf_T[b] = 1 - supOut - subsOutVF * subOut;
+        }
+    }
+    // h = Cp/Cv*(rhoE/rho - 0.5|U|^2)  (updateFields.H:60)
+    for (int i = 0; i < n; ++i) c.h[i] = gamma * (c.rhoE[i] / c.rho[i] - 0.5 * magSqr(c.U[i]));
+    // h.correctBoundaryConditions() via the enthalpy BCs derived from T's
+    for (int b = 0; b < nB; ++b) {
+        const rdst_bc& bc = c.bc[m.bPatch[b]];
+        int ci = m.bCell[b];
+        switch (bc.T_type) {
+            case RDST_BC_FIXED_VALUE: c.bh[b] = Cp * bc.T_value; break;
+            case RDST_BC_INLET_OUTLET: case RDST_BC_FARFIELD: {
+                double vf = c.bcs.vf_T[b];
+                c.bh[b] = vf * (Cp * TrefH[b]) + (1.0 - vf) * c.h[ci];
+                break;
+            }
+            default: c.bh[b] = c.h[ci];
+        }
+    }
+    // boundMinMax(h, CpMin*TMin, CpMax*TMax)
+    double hMin = Cp * c.st.T_min, hMax = Cp * c.st.T_max;
+    for (int i = 0; i < n; ++i) c.h[i] = clampv(c.h[i], hMin, hMax);
+    // thermo.correct(): T = h/Cp, psi = 1/(R T), mu
+    for (int i = 0; i < n; ++i) {
+        c.T[i] = c.h[i] / Cp;
+        c.psi[i] = 1.0 / (c.th.R * c.T[i]);
+        c.mu[i] = muOf(c, c.T[i]);
+        c.alpha[i] = alphaOf(c, c.T[i]);
+    }
+    for (int b = 0; b < nB; ++b) {
+        const rdst_bc& bc = c.bc[m.bPatch[b]];
+        if (bc.T_type == RDST_BC_FIXED_VALUE) {
+            c.bT[b] = bc.T_value;
+            c.bh[b] = Cp * c.bT[b];
+        } else {
+            c.bT[b] = c.bh[b] / Cp;
+        }
+        c.bpsi[b] = 1.0 / (c.th.R * c.bT[b]);
+        c.bmu[b] = muOf(c, c.bT[b]);
+        c.balpha[b] = alphaOf(c, c.bT[b]);
+    }
+    // p = rho/psi; bound; p.correctBoundaryConditions()
+    for (int i = 0; i < n; ++i) c.p[i] = clampv(c.rho[i] / c.psi[i], c.st.p_min, c.st.p_max);
+    for (int b = 0; b < nB; ++b) {
+        const rdst_bc& bc = c.bc[m.bPatch[b]];
+        int ci = m.bCell[b], f = m.bFace[b];
+        double phib = c.phi[m.nInternal + b];
+        switch (bc.p_type) {
+            case RDST_BC_FIXED_VALUE: c.bp[b] = bc.p_value; break;
+            case RDST_BC_INLET_OUTLET: {
+                double vf = 1.0 - fe_pos(phib);
+                c.bcs.vf_p[b] = vf;
+                c.bp[b] = vf * bc.p_value + (1.0 - vf) * c.p[ci];
+                break;
+            }
+            case RDST_BC_FARFIELD: {
+                // farFieldPressureDST::updateCoeffs (P.C:144-215)
+                V3 nrm = m.Sf[f] / m.magSf[f];
+                double cb = std::sqrt(Cp / Cv / c.bpsi[b]);
+                double cI = std::sqrt(Cp / Cv / c.psi[ci]);
+                double mach = phib / brhoOld[b] / m.magSf[f] / cb;
+                double supIn = neg0(mach + 1), supOut = pos0(mach - 1);
+                double subIn = neg(mach) - supIn, subOut = pos0(mach) - supOut;
+                double rho0c0 = c.rho[ci] * cI;
+                V3 Uinf = v3(bc.U_inf[0], bc.U_inf[1], bc.U_inf[2]);
+                double pb = bc.p_inf - rho0c0 * dot(nrm, Uinf - c.U[ci]);
+                double ref = supIn * bc.p_inf + subIn * pb + subOut * bc.p_inf;
+                double vf = 1 - supOut - 0.5 * subIn;
+                c.bcs.vf_p[b] = vf;
+                c.bp[b] = vf * ref + (1.0 - vf) * c.p[ci];
+                break;
+            }
+            default: c.bp[b] = c.p[ci];
+        }
+        c.bp[b] = clampv(c.bp[b], c.st.p_min, c.st.p_max);
+        // rho_b = p_b psi_b, rhoU_b = rho_b U_b, rhoE_b = rho_b (h_b + 0.5|U_b|^2) - p_b
+        c.brho[b] = c.bp[b] * c.bpsi[b];
+        c.brhoU[b] = c.brho[b] * c.bU[b];
+        c.brhoE[b] = c.brho[b] * (c.bh[b] + 0.5 * magSqr(c.bU[b])) - c.bp[b];
+    }
+}
+
+// ------------------------------------------------------------------ timeDST
+// setSteadyStateDeltaT (timeDST.C:218-273)
+static void setSteadyStateDeltaT(Case& c) {
+    const Mesh& m = c.m;
+    int n = m.nCells;
+    double gam = c.th.Cp / c.Cv;
+    if (!c.st.neighbouring) {
+        for (int i = 0; i < n; ++i) {
+            double cc = std::sqrt(c.th.Cp / c.Cv / c.psi[i]);
+            double lam = (std::fabs(c.U[i].x) + cc) * c.dS[3 * i] + (std::fabs(c.U[i].y) + cc) * c.dS[3 * i + 1]
+                       + (std::fabs(c.U[i].z) + cc) * c.dS[3 * i + 2];
+            c.rDeltaT[i] = lam / (c.courantScalar * m.V[i]);
+        }
+        return;
+    }
+    std::vector<V3> Uf;
+    linearInterpV(m, c.U, c.bU, Uf);
+    std::vector<double> cc(n), bcc(m.nB), cf;
+    for (int i = 0; i < n; ++i) cc[i] = std::sqrt(gam / c.psi[i]);
+    for (int b = 0; b < m.nB; ++b) bcc[b] = std::sqrt(gam / c.bpsi[b]);
+    linearInterp(m, cc, bcc, cf);
+    for (int i = 0; i < n; ++i) c.rDeltaT[i] = 0.0;
+    for (int f = 0; f < m.nInternal; ++f) {
+        double dtSf = (mag(Uf[f]) + cf[f]) * m.dc[f];
+        int o = m.owner[f], nb = m.neighbour[f];
+        c.rDeltaT[o] = std::max(c.rDeltaT[o], dtSf);
+        c.rDeltaT[nb] = std::max(c.rDeltaT[nb], dtSf);
+    }
+    for (int b = 0; b < m.nB; ++b) {
+        if (m.patches[m.bPatch[b]].type != RDST_PATCH_WALL) continue;
+        int ci = m.bCell[b];
+        double pLambda = 0.5 * m.dc[m.nInternal + b] * (std::sqrt(gam / c.psi[ci])) + mag(c.U[ci]);
+        c.rDeltaT[ci] = std::max(c.rDeltaT[ci], pLambda);
+    }
+    for (int i = 0; i < n; ++i) c.rDeltaT[i] /= c.courant[i];
+}
+
+// setCourantDST (timeDST.C:316-443), steady branch
+static double setCourant(Case& c) {
+    const Mesh& m = c.m;
+    int n = m.nCells;
+    double ss = 0;
+    for (int i = 0; i < n; ++i) {
+        double mg = mag5(&c.R[5 * i]);
+        ss += mg * mg;
+    }
+    double residualR = std::sqrt(ss);
+    if (c.residual != -1) c.residualRatio = std::min(std::max(c.residual / residualR, 0.1), 2.0);
+    c.residual = residualR;
+    std::vector<double> eNew(n), factor(n, 1.0);
+    for (int i = 0; i < n; ++i) eNew[i] = c.rhoE[i] / c.rho[i] - 0.5 * magSqr(c.rhoU[i] / c.rho[i]);
+    for (int f = 0; f < m.nInternal; ++f) {
+        int o = m.owner[f], nb = m.neighbour[f];
+        if (c.rho[o] < 0.95 * c.rhoOld[o] || eNew[o] < 0.95 * c.eOld[o] || eNew[o] < SMALL) {
+            factor[o] = std::min(factor[o], 0.5);
+            factor[nb] = std::min(factor[nb], 0.75);
+        }
+        if (c.rho[nb] < 0.95 * c.rhoOld[nb] || eNew[nb] < 0.95 * c.eOld[nb] || eNew[nb] < SMALL) {
+            factor[nb] = std::min(factor[nb], 0.5);
+            factor[o] = std::min(factor[o], 0.75);
+        }
+    }
+    c.rhoOld = c.rho;
+    c.eOld = eNew;
+    for (int i = 0; i < n; ++i) {
+        c.courant[i] *= c.residualRatio * factor[i];
+        c.courant[i] = std::min(std::max(c.courant[i], c.st.min_co), c.st.max_co);
+    }
+    setSteadyStateDeltaT(c);
+    return residualR;
+}
+
+// Single::iterate (Single.C:80-104)
+static double iterate(Case& c, Perf* perfOut) {
+    computeFlux(c);
+    computeJacobian(c);
+    assemble(c);
+    Perf perf = solve(c);
+    if (perfOut) *perfOut = perf;
+    updateFields(c);
+    return setCourant(c);
+}
+
+}  // namespace orc
+
+// ======================================================================= C API
+using namespace orc;
+
+extern "C" {
+
+const char* orc_last_error(void) { return g_err.c_str(); }
+
+#define ORC_TRY(...) try { __VA_ARGS__; return 0; } catch (const std::exception& e) { g_err = e.what(); return 1; }
+
+// geometry of a polyMesh, the fvMesh quantities the operators read
+int orc_geometry(const rdst_mesh_desc* d, double* Sf, double* magSf, double* Cf, double* C, double* V,
+                 double* w, double* dc, int32_t* nB_out) {
+    ORC_TRY({
+        Mesh m;
+        buildMesh(m, d);
+        int nf = m.nInternal + m.nB;
+        for (int k = 0; k < nf; ++k) {
+            int f = k < m.nInternal ? k : m.bFace[k - m.nInternal];
+            if (Sf) { Sf[3 * k] = m.Sf[f].x; Sf[3 * k + 1] = m.Sf[f].y; Sf[3 * k + 2] = m.Sf[f].z; }
+            if (magSf) magSf[k] = m.magSf[f];
+            if (Cf) { Cf[3 * k] = m.Cf[f].x; Cf[3 * k + 1] = m.Cf[f].y; Cf[3 * k + 2] = m.Cf[f].z; }
+            if (w) w[k] = m.w[k];
+            if (dc) dc[k] = m.dc[k];
+        }
+        for (int i = 0; i < m.nCells; ++i) {
+            if (C) { C[3 * i] = m.C[i].x; C[3 * i + 1] = m.C[i].y; C[3 * i + 2] = m.C[i].z; }
+            if (V) V[i] = m.V[i];
+        }
+        if (nB_out) *nB_out = m.nB;
+    })
+}
+
+// single-face flux of a scheme (golden vectors and invariant tests)
+// state arrays: [rho, rhoUx, rhoUy, rhoUz, e, rPsi, aStar, c] for each side
+int orc_face_flux(int scheme, const double* L, const double* R, const double* Sf, double gamma,
+                  double fHLLCP, int lowMach, double Mainf, double* out /* phi, phiU[3], phiE, Up[3] */) {
+    ORC_TRY({
+        FaceIn in;
+        in.rho_o = L[0]; in.rhoU_o = v3(L[1], L[2], L[3]); in.e_o = L[4]; in.rPsi_o = L[5]; in.a_o = L[6]; in.c_o = L[7];
+        in.rho_n = R[0]; in.rhoU_n = v3(R[1], R[2], R[3]); in.e_n = R[4]; in.rPsi_n = R[5]; in.a_n = R[6]; in.c_n = R[7];
+        in.Sf = v3(Sf[0], Sf[1], Sf[2]); in.magSf = mag(in.Sf); in.gam_f = gamma; in.fHLLCP = fHLLCP;
+        rdst_settings st = {};
+        st.low_mach = lowMach; st.Mainf = Mainf;
+        FaceOut o = faceFlux(scheme, in, st);
+        out[0] = o.phi; out[1] = o.phiU.x; out[2] = o.phiU.y; out[3] = o.phiU.z; out[4] = o.phiE;
+        out[5] = o.Up.x; out[6] = o.Up.y; out[7] = o.Up.z;
+    })
+}
+
+int orc_euler_jacobian(const double* U, double gamma, double rhoE, double rho, const double* n, double* J) {
+    ORC_TRY({ eulerJacobian(v3(U[0], U[1], U[2]), gamma, rhoE, rho, v3(n[0], n[1], n[2]), J); })
+}
+
+int orc_case_create(const rdst_mesh_desc* d, const rdst_thermo* th, const rdst_bc* bcs,
+                    const rdst_settings* st, void** out) {
+    ORC_TRY({
+        Case* c = new Case();
+        buildMesh(c->m, d);
+        c->th = *th;
+        c->bc.assign(bcs, bcs + d->n_patches);
+        c->st = *st;
+        c->Cv = th->Cp - th->R;
+        c->gamma = th->Cp / c->Cv;
+        *out = c;
+    })
+}
+int orc_case_destroy(void* h) { delete (Case*)h; return 0; }
+int orc_case_set_state(void* h, const double* p, const double* T, const double* U) {
+    ORC_TRY({ setState(*(Case*)h, p, T, U); })
+}
+int orc_flux_residual(void* h) { ORC_TRY({ computeFlux(*(Case*)h); }) }
+int orc_jacobian_assemble(void* h) {
+    ORC_TRY({ Case& c = *(Case*)h; computeJacobian(c); assemble(c); })
+}
+int orc_block_solve(void* h, rdst_solver_perf* perf) {
+    ORC_TRY({
+        Perf p = solve(*(Case*)h);
+        if (perf) {
+            for (int k = 0; k < 5; ++k) { perf->initial_residual[k] = p.init[k]; perf->final_residual[k] = p.fin[k]; }
+            perf->n_iterations = p.nIter; perf->converged = p.conv;
+        }
+    })
+}
+int orc_update_fields(void* h) { ORC_TRY({ updateFields(*(Case*)h); }) }
+int orc_set_courant(void* h, double* res) { ORC_TRY({ double r = setCourant(*(Case*)h); if (res) *res = r; }) }
+int orc_iterate(void* h, int nSteps, double* res, rdst_solver_perf* perfs) {
+    ORC_TRY({
+        Case& c = *(Case*)h;
+        for (int s = 0; s < nSteps; ++s) {
+            Perf p;
+            double r = iterate(c, &p);
+            if (res) res[s] = r;
+            if (perfs) {
+                for (int k = 0; k < 5; ++k) { perfs[s].initial_residual[k] = p.init[k]; perfs[s].final_residual[k] = p.fin[k]; }
+                perfs[s].n_iterations = p.nIter; perfs[s].converged = p.conv;
+            }
+        }
+    })
+}
+
+// LDU access: which = 0 diag [n][25], 1 upper [F][25], 2 lower [F][25], 3 source [n][5],
+// 4 lambda [F], 5 ownerCoeffs [F][25], 6 neighbourCoeffs [F][25], 7 patchInternalCoeffs [nB][25]
+int orc_get_matrix(void* h, int which, double* out) {
+    ORC_TRY({
+        Case& c = *(Case*)h;
+        const std::vector<double>* v = nullptr;
+        switch (which) {
+            case 0: v = &c.diag; break; case 1: v = &c.upper; break; case 2: v = &c.lower; break;
+            case 3: v = &c.source; break; case 4: v = &c.lambda; break; case 5: v = &c.ownC; break;
+            case 6: v = &c.neiC; break; case 7: v = &c.patchIC; break;
+            default: throw std::runtime_error("bad matrix id");
+        }
+        std::memcpy(out, v->data(), v->size() * sizeof(double));
+    })
+}
+
+// apply the assembled matrix / the preconditioner (for solver tests)
+int orc_amul(void* h, const double* x, double* y) {
+    ORC_TRY({
+        Case& c = *(Case*)h;
+        std::vector<double> xv(x, x + 5 * (size_t)c.m.nCells), yv;
+        Amul(c, xv, yv);
+        std::memcpy(y, yv.data(), yv.size() * sizeof(double));
+    })
+}
+int orc_ilu_apply(void* h, const double* r, double* x) {
+    ORC_TRY({
+        Case& c = *(Case*)h;
+        ILU il;
+        buildILU(c, il);
+        std::vector<double> rv(r, r + 5 * (size_t)c.m.nCells), xv;
+        iluApply(il, rv, xv);
+        std::memcpy(x, xv.data(), xv.size() * sizeof(double));
+    })
+}
+// solve A x = b with the configured solver from x0 (for solver tests)
+int orc_linear_solve(void* h, const double* b, double* x, rdst_solver_perf* perf) {
+    ORC_TRY({
+        Case& c = *(Case*)h;
+        Precond M;
+        M.init(c, c.st.precond);
+        std::vector<double> bv(b, b + 5 * (size_t)c.m.nCells), xv(x, x + 5 * (size_t)c.m.nCells);
+        Perf p = (c.st.solver == RDST_SOLVER_GMRES) ? gmres(c, M, xv, bv) : bicgstab(c, M, xv, bv);
+        std::memcpy(x, xv.data(), xv.size() * sizeof(double));
+        if (perf) {
+            for (int k = 0; k < 5; ++k) { perf->initial_residual[k] = p.init[k]; perf->final_residual[k] = p.fin[k]; }
+            perf->n_iterations = p.nIter; perf->converged = p.conv;
+        }
+    })
+}
+// overwrite the assembled matrix (diag/upper/lower) for solver tests
+int orc_set_matrix(void* h, const double* diag, const double* upper, const double* lower) {
+    ORC_TRY({
+        Case& c = *(Case*)h;
+        int n = c.m.nCells, F = c.m.nInternal;
+        c.diag.assign(diag, diag + 25 * (size_t)n);
+        c.upper.assign(upper, upper + 25 * (size_t)F);
+        c.lower.assign(lower, lower + 25 * (size_t)F);
+        c.source.assign(5 * (size_t)n, 0.0);
+        c.hasMatrix = true;
+    })
+}
+
+int orc_case_get(void* h, int field, double* out) {
+    ORC_TRY({
+        Case& c = *(Case*)h;
+        const Mesh& m = c.m;
+        auto putS = [&](const std::vector<double>& v) { std::memcpy(out, v.data(), v.size() * sizeof(double)); };
+        auto putV = [&](const std::vector<V3>& v) {
+            for (size_t i = 0; i < v.size(); ++i) { out[3 * i] = v[i].x; out[3 * i + 1] = v[i].y; out[3 * i + 2] = v[i].z; }
+        };
+        switch (field) {
+            case RDST_F_RHO: putS(c.rho); break; case RDST_F_RHOU: putV(c.rhoU); break;
+            case RDST_F_RHOE: putS(c.rhoE); break; case RDST_F_P: putS(c.p); break;
+            case RDST_F_T: putS(c.T); break; case RDST_F_U: putV(c.U); break;
+            case RDST_F_H: putS(c.h); break; case RDST_F_PSI: putS(c.psi); break;
+            case RDST_F_R: putS(c.R); break; case RDST_F_W: putS(c.W); break;
+            case RDST_F_RDELTAT: putS(c.rDeltaT); break; case RDST_F_COURANT: putS(c.courant); break;
+            case RDST_F_PHI: putS(c.phi); break; case RDST_F_PHIUP: putV(c.phiUp); break;
+            case RDST_F_PHIEP: putS(c.phiEp); break; case RDST_F_UP: putV(c.Up); break;
+            case RDST_F_BRHO: putS(c.brho); break; case RDST_F_BU: putV(c.bU); break;
+            case RDST_F_BP: putS(c.bp); break; case RDST_F_BT: putS(c.bT); break;
+            case RDST_F_BRHOU: putV(c.brhoU); break; case RDST_F_BRHOE: putS(c.brhoE); break;
+            case RDST_F_BH: putS(c.bh); break; case RDST_F_BPSI: putS(c.bpsi); break;
+            case RDST_F_SOURCE: putS(c.source); break; case RDST_F_MU: putS(c.mu); break;
+            case RDST_F_RHO_OLD: putS(c.rhoOld); break; case RDST_F_E_OLD: putS(c.eOld); break;
+            default: throw std::runtime_error("bad field id");
+        }
+        (void)m;
+    })
+}
+
+int orc_case_put(void* h, int field, const double* in) {
+    ORC_TRY({
+        Case& c = *(Case*)h;
+        int n = c.m.nCells;
+        switch (field) {
+            case RDST_F_RHO: c.rho.assign(in, in + n); break;
+            case RDST_F_RHOE: c.rhoE.assign(in, in + n); break;
+            case RDST_F_RHOU: for (int i = 0; i < n; ++i) c.rhoU[i] = v3(in[3 * i], in[3 * i + 1], in[3 * i + 2]); break;
+            case RDST_F_W: c.W.assign(in, in + 5 * (size_t)n); break;
+            case RDST_F_RDELTAT: c.rDeltaT.assign(in, in + n); break;
+            default: throw std::runtime_error("field not writable");
+        }
+    })
+}
+
+}  // extern "C"

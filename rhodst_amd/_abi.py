@@ -1,0 +1,154 @@
+"""ctypes mirror of include/rhodst.h (structs and enums) and helpers that
+turn a PolyMesh + case description into the C descriptors."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+# rdst_flux_scheme, in the order of the baseFluxDST RTS table (newbaseFluxDST.C:64-84)
+FLUX_SCHEMES = {"AUSM+": 0, "AUSM+up": 1, "HLL": 2, "HLLC": 3, "HLLCP": 4, "Kurganov": 5}
+BC = {"fixedValue": 0, "zeroGradient": 1, "inletOutlet": 2, "symmetryPlane": 3, "slip": 3,
+      "symmetry": 3, "farField": 4, "empty": 5}
+SOLVERS = {"GMRES": 0, "BiCGStab": 1}
+PRECONDS = {"ILUC0": 0, "ILU0": 0, "none": 1, "diagonal": 2, "Diagonal": 2}
+DEPENDENT = {"W": 0, "deltaW": 1}
+PRECISION = {"fp64": 0, "mixed": 1}
+
+F_RHO, F_RHOU, F_RHOE, F_P, F_T, F_U, F_H, F_PSI, F_R, F_W, F_RDELTAT, F_COURANT = range(12)
+F_PHI, F_PHIUP, F_PHIEP, F_UP = 12, 13, 14, 15
+F_BRHO, F_BU, F_BP, F_BT, F_BRHOU, F_BRHOE, F_BH, F_BPSI = 16, 17, 18, 19, 20, 21, 22, 23
+F_BCSR_VALUES, F_SOURCE, F_MU, F_RHO_OLD, F_E_OLD = 24, 25, 26, 27, 28
+
+
+class rdst_patch(C.Structure):
+    _fields_ = [("name", C.c_char * 64), ("type", C.c_int32), ("start", C.c_int32),
+                ("size", C.c_int32), ("pad", C.c_int32)]
+
+
+class rdst_mesh_desc(C.Structure):
+    _fields_ = [("n_points", C.c_int32), ("n_faces", C.c_int32), ("n_internal_faces", C.c_int32),
+                ("n_cells", C.c_int32), ("n_patches", C.c_int32), ("pad", C.c_int32),
+                ("points", C.POINTER(C.c_double)), ("face_offsets", C.POINTER(C.c_int32)),
+                ("face_points", C.POINTER(C.c_int32)), ("owner", C.POINTER(C.c_int32)),
+                ("neighbour", C.POINTER(C.c_int32)), ("patches", C.POINTER(rdst_patch))]
+
+
+class rdst_bc(C.Structure):
+    _fields_ = [("p_type", C.c_int32), ("T_type", C.c_int32), ("U_type", C.c_int32), ("pad", C.c_int32),
+                ("p_value", C.c_double), ("T_value", C.c_double), ("U_value", C.c_double * 3),
+                ("p_inf", C.c_double), ("T_inf", C.c_double), ("U_inf", C.c_double * 3)]
+
+
+class rdst_thermo(C.Structure):
+    _fields_ = [("Cp", C.c_double), ("R", C.c_double), ("transport", C.c_int32), ("pad", C.c_int32),
+                ("mu", C.c_double), ("Pr", C.c_double), ("As", C.c_double), ("Ts", C.c_double)]
+
+
+class rdst_settings(C.Structure):
+    _fields_ = [("flux_scheme", C.c_int32), ("low_mach", C.c_int32), ("Mainf", C.c_double),
+                ("dependent", C.c_int32), ("neighbouring", C.c_int32), ("beta", C.c_double),
+                ("initial_co", C.c_double), ("min_co", C.c_double), ("max_co", C.c_double),
+                ("solver", C.c_int32), ("precond", C.c_int32), ("n_directions", C.c_int32),
+                ("min_iter", C.c_int32), ("max_iter", C.c_int32), ("precision", C.c_int32),
+                ("tolerance", C.c_double), ("rel_tol", C.c_double),
+                ("rho_min", C.c_double), ("rho_max", C.c_double), ("p_min", C.c_double),
+                ("p_max", C.c_double), ("T_min", C.c_double), ("T_max", C.c_double),
+                ("ilu_order", C.c_int32), ("viscous", C.c_int32)]
+
+
+class rdst_solver_perf(C.Structure):
+    _fields_ = [("initial_residual", C.c_double * 5), ("final_residual", C.c_double * 5),
+                ("n_iterations", C.c_int32), ("converged", C.c_int32)]
+
+
+def dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def iptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+class MeshDesc:
+    """Owns the contiguous arrays behind an rdst_mesh_desc."""
+
+    def __init__(self, mesh):
+        self.points = np.ascontiguousarray(mesh.points, dtype=np.float64)
+        self.face_offsets = np.ascontiguousarray(mesh.face_offsets, dtype=np.int32)
+        self.face_points = np.ascontiguousarray(mesh.face_points, dtype=np.int32)
+        self.owner = np.ascontiguousarray(mesh.owner, dtype=np.int32)
+        self.neighbour = np.ascontiguousarray(mesh.neighbour, dtype=np.int32)
+        self.patches = (rdst_patch * len(mesh.patches))()
+        for i, (name, t, start, size) in enumerate(mesh.patches):
+            self.patches[i].name = name.encode()[:63]
+            self.patches[i].type = t
+            self.patches[i].start = start
+            self.patches[i].size = size
+        self.desc = rdst_mesh_desc(
+            n_points=len(self.points), n_faces=len(self.owner), n_internal_faces=len(self.neighbour),
+            n_cells=mesh.n_cells, n_patches=len(mesh.patches), pad=0,
+            points=dptr(self.points), face_offsets=iptr(self.face_offsets),
+            face_points=iptr(self.face_points), owner=iptr(self.owner),
+            neighbour=iptr(self.neighbour), patches=C.cast(self.patches, C.POINTER(rdst_patch)))
+
+
+def make_bcs(mesh, spec: dict):
+    """spec: patch name -> dict(p=(type, value), T=(type, value), U=(type, (x,y,z)),
+    inf=dict(p=, T=, U=)).  Empty patches default to empty BCs."""
+    arr = (rdst_bc * len(mesh.patches))()
+    for i, (name, t, _s, _n) in enumerate(mesh.patches):
+        b = arr[i]
+        if t == 3:
+            b.p_type = b.T_type = b.U_type = BC["empty"]
+            continue
+        d = spec[name]
+        pt, pv = d["p"] if isinstance(d["p"], tuple) else (d["p"], 0.0)
+        tt, tv = d["T"] if isinstance(d["T"], tuple) else (d["T"], 0.0)
+        ut, uv = d["U"] if isinstance(d["U"], tuple) else (d["U"], (0.0, 0.0, 0.0))
+        b.p_type, b.T_type, b.U_type = BC[pt], BC[tt], BC[ut]
+        b.p_value, b.T_value = pv, tv
+        for k in range(3):
+            b.U_value[k] = uv[k]
+        inf = d.get("inf")
+        if inf:
+            b.p_inf, b.T_inf = inf["p"], inf["T"]
+            for k in range(3):
+                b.U_inf[k] = inf["U"][k]
+    return arr
+
+
+def make_settings(**kw) -> rdst_settings:
+    s = rdst_settings()
+    s.flux_scheme = FLUX_SCHEMES[kw.get("flux_scheme", "AUSM+")] if isinstance(kw.get("flux_scheme", "AUSM+"), str) else kw["flux_scheme"]
+    s.low_mach = int(kw.get("low_mach", 0))
+    s.Mainf = kw.get("Mainf", 0.5)
+    s.dependent = DEPENDENT[kw.get("dependent", "W")]
+    s.neighbouring = int(kw.get("neighbouring", 1))
+    s.beta = kw.get("beta", 1.0)
+    s.initial_co = kw.get("initial_co", 5.0)
+    s.min_co = kw.get("min_co", 5.0)
+    s.max_co = kw.get("max_co", 5.0)
+    s.solver = SOLVERS[kw.get("solver", "GMRES")]
+    s.precond = PRECONDS[kw.get("precond", "ILUC0")]
+    s.n_directions = kw.get("n_directions", 20)
+    s.min_iter = kw.get("min_iter", 1)
+    s.max_iter = kw.get("max_iter", 1)
+    s.precision = PRECISION[kw.get("precision", "fp64")]
+    s.tolerance = kw.get("tolerance", 1e-15)
+    s.rel_tol = kw.get("rel_tol", 1e-4)
+    s.rho_min, s.rho_max = kw.get("rho_bounds", (1e-3, 1e4))
+    s.p_min, s.p_max = kw.get("p_bounds", (1e-3, 1e4))
+    s.T_min, s.T_max = kw.get("T_bounds", (1e-3, 1e4))
+    s.ilu_order = kw.get("ilu_order", 0)
+    s.viscous = int(kw.get("viscous", 0))
+    return s
+
+
+def make_thermo(Cp=2.5, R=1.0 / 1.4, mu=0.0, Pr=1.0, sutherland=None) -> rdst_thermo:
+    t = rdst_thermo()
+    t.Cp, t.R, t.mu, t.Pr = Cp, R, mu, Pr
+    if sutherland:
+        t.transport = 1
+        t.As, t.Ts = sutherland
+    return t

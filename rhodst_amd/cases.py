@@ -1,0 +1,88 @@
+"""Case descriptions (mesh + thermo + BCs + settings + initial state) for
+the workloads the block-coupled implicit step is run on."""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import meshgen
+from ._abi import make_bcs, make_settings, make_thermo
+
+
+@dataclass
+class CaseSpec:
+    name: str
+    mesh: meshgen.PolyMesh
+    thermo: object
+    bcs: object
+    settings: object
+    p0: np.ndarray
+    T0: np.ndarray
+    U0: np.ndarray
+
+
+def forward_step_case(nx: int = 120, ny: int = 40, scheme: str = "AUSM+", perturb: float = 0.0,
+                      seed: int = 2, **settings) -> CaseSpec:
+    """Mach-3 forward step (tutorials/forwardStep/0/{p,T,U}, thermo
+    constant/thermophysicalProperties 'normalisedGas', Cp 2.5, mu 0), with R
+    set to 1/1.4 so that c = 1 at T = 1: inflow p = 1, T = 1, U = (3,0,0).
+    inlet fixedValue; outlet p zeroGradient, T/U inletOutlet; bottom/top
+    symmetryPlane; obstacle p/T zeroGradient, U slip.  Uniform initial state,
+    optionally multiplied by a seeded +-perturb factor (seed 2)."""
+    mesh = meshgen.forward_step(nx, ny)
+    spec = {
+        "inlet": {"p": ("fixedValue", 1.0), "T": ("fixedValue", 1.0), "U": ("fixedValue", (3.0, 0.0, 0.0))},
+        "outlet": {"p": "zeroGradient", "T": ("inletOutlet", 1.0), "U": ("inletOutlet", (3.0, 0.0, 0.0))},
+        "bottom": {"p": "symmetryPlane", "T": "symmetryPlane", "U": "symmetryPlane"},
+        "top": {"p": "symmetryPlane", "T": "symmetryPlane", "U": "symmetryPlane"},
+        "obstacle": {"p": "zeroGradient", "T": "zeroGradient", "U": "slip"},
+    }
+    n = mesh.n_cells
+    p0 = np.ones(n)
+    T0 = np.ones(n)
+    U0 = np.zeros((n, 3))
+    U0[:, 0] = 3.0
+    if perturb > 0:
+        rng = np.random.default_rng(seed)
+        p0 = p0 * (1 + perturb * rng.uniform(-1, 1, n))
+        T0 = T0 * (1 + perturb * rng.uniform(-1, 1, n))
+        U0[:, 0] *= 1 + perturb * rng.uniform(-1, 1, n)
+        U0[:, 1] = 3.0 * perturb * rng.uniform(-1, 1, n)
+    kw = dict(flux_scheme=scheme, neighbouring=1, initial_co=5.0, min_co=5.0, max_co=5.0,
+              solver="GMRES", precond="ILUC0", n_directions=20, min_iter=1, max_iter=1,
+              tolerance=1e-15, rel_tol=1e-4)
+    kw.update(settings)
+    return CaseSpec("forwardStep", mesh, make_thermo(Cp=2.5, R=1.0 / 1.4, mu=0.0, Pr=1.0),
+                    make_bcs(mesh, spec), make_settings(**kw), p0, T0, np.ascontiguousarray(U0))
+
+
+def box_farfield_case(nx: int = 24, ny: int = 16, scheme: str = "HLLC", mach: float = 0.5, aoa_deg: float = 5.0,
+                      jitter: float = 0.2, seed: int = 1, perturb: float = 0.05, **settings) -> CaseSpec:
+    """A jittered 2-D box with the characteristic far-field conditions
+    (farField{Pressure,Temperature,Velocity}DST) on all four sides and a
+    seeded perturbed subsonic initial state; exercises every far-field regime
+    the flow produces.  Air-like thermo, inviscid."""
+    mesh = meshgen.box_2d(nx, ny, jitter=jitter, seed=seed)
+    gamma, R = 1.4, 287.0
+    Cp = gamma * R / (gamma - 1)
+    Tinf, pinf = 300.0, 1.0e5
+    c = np.sqrt(gamma * R * Tinf)
+    a = np.deg2rad(aoa_deg)
+    Uinf = (mach * c * np.cos(a), mach * c * np.sin(a), 0.0)
+    ff = {"p": ("farField", pinf), "T": ("farField", Tinf), "U": ("farField", Uinf),
+          "inf": {"p": pinf, "T": Tinf, "U": Uinf}}
+    spec = {k: ff for k in ("left", "right", "bottom", "top")}
+    n = mesh.n_cells
+    rng = np.random.default_rng(2)
+    p0 = pinf * (1 + perturb * rng.uniform(-1, 1, n))
+    T0 = Tinf * (1 + perturb * rng.uniform(-1, 1, n))
+    U0 = np.tile(np.array(Uinf), (n, 1)) * (1 + perturb * rng.uniform(-1, 1, (n, 1)))
+    U0[:, 2] = 0.0
+    kw = dict(flux_scheme=scheme, neighbouring=1, initial_co=2.0, min_co=0.5, max_co=20.0,
+              solver="GMRES", precond="ILUC0", n_directions=10, min_iter=1, max_iter=2,
+              tolerance=1e-15, rel_tol=1e-3, p_bounds=(1.0, 1e7), T_bounds=(1.0, 1e4),
+              rho_bounds=(1e-4, 1e3))
+    kw.update(settings)
+    return CaseSpec("boxFarField", mesh, make_thermo(Cp=Cp, R=R, mu=0.0, Pr=0.72),
+                    make_bcs(mesh, spec), make_settings(**kw), p0, T0, np.ascontiguousarray(U0))

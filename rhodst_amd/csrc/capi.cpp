@@ -1,0 +1,456 @@
+// capi.cpp -- the extern "C" boundary declared in include/rhodst.h.
+//
+// Every entry point converts exceptions to an RDST_ERR_* code and keeps the
+// message for rdst_last_error(); the reference aborts through FatalError
+// instead (e.g. newbaseFluxDST.C:72-82 lists the valid fluxScheme names).
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rdst_internal.hpp"
+
+namespace rdst {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+MeshDev Case::meshDev() const {
+    const Mesh& m = *mesh;
+    MeshDev d;
+    d.nC = m.nC; d.nI = m.nI; d.nB = m.nB; d.nF = m.nF; d.ellW = m.ellW;
+    d.owner = m.d_owner.p; d.neigh = m.d_neigh.p; d.bcell = m.d_bcell.p; d.bpatch = m.d_bpatch.p; d.bwall = m.d_bWall.p;
+    d.Sf = m.d_Sf.p; d.magSf = m.d_magSf.p; d.w = m.d_w.p; d.dc = m.d_dc.p; d.V = m.d_V.p;
+    d.ellCode = m.d_ellCode.p; d.ellFace = m.d_ellFace.p; d.ellSlot = m.d_ellSlot.p; d.slotDiag = m.d_slotDiag.p;
+    d.ellSf = m.d_ellSf.p; d.ellMagSf = m.d_ellMagSf.p; d.ellWt = m.d_ellWt.p; d.ellDc = m.d_ellDc.p;
+    return d;
+}
+
+static void alloc_fields(Case& c) {
+    const Mesh& m = *c.mesh;
+    const size_t n = m.nC, nB = std::max(m.nB, 1), nF = m.nF;
+    c.cellBuf.alloc(14 * n);
+    c.bndBuf.alloc(18 * nB);
+    c.faceBuf.alloc(8 * nF);
+    c.RBuf.alloc(5 * n);
+    c.WBuf.alloc(5 * n);
+    c.timeBuf.alloc(7 * n + 4096);
+    c.fHBuf.alloc(n);
+    c.src.alloc(5 * n);
+    c.cellBuf.zero(); c.bndBuf.zero(); c.faceBuf.zero(); c.RBuf.zero(); c.WBuf.zero(); c.timeBuf.zero(); c.fHBuf.zero();
+    RDST_HIP(hipDeviceSynchronize());
+    Fields& f = c.f;
+    double* p = c.cellBuf.p;
+    double** cells[14] = {&f.rho, &f.rhoUx, &f.rhoUy, &f.rhoUz, &f.rhoE, &f.p, &f.T, &f.h, &f.psi, &f.mu, &f.alpha, &f.Ux, &f.Uy, &f.Uz};
+    for (int i = 0; i < 14; ++i) *cells[i] = p + i * n;
+    p = c.bndBuf.p;
+    double** bnd[18] = {&f.brho, &f.bUx, &f.bUy, &f.bUz, &f.bp, &f.bT, &f.bh, &f.bpsi, &f.bmu, &f.balpha,
+                        &f.brhoUx, &f.brhoUy, &f.brhoUz, &f.brhoE, &f.vf_p, &f.vf_T, &f.vf_U, &f.TrefH};
+    for (int i = 0; i < 18; ++i) *bnd[i] = p + i * nB;
+    p = c.faceBuf.p;
+    double** face[8] = {&f.phi, &f.phiUx, &f.phiUy, &f.phiUz, &f.phiE, &f.Upx, &f.Upy, &f.Upz};
+    for (int i = 0; i < 8; ++i) *face[i] = p + i * nF;
+    f.R = c.RBuf.p;
+    f.W = c.WBuf.p;
+    p = c.timeBuf.p;
+    f.rDeltaT = p; f.courant = p + n; f.rhoOld = p + 2 * n; f.eOld = p + 3 * n; f.dS = p + 4 * n;
+    f.fH = c.fHBuf.p;
+}
+
+static void download_soa(const double* const* comps, int k, size_t n, double* out) {
+    std::vector<double> tmp(n);
+    for (int d = 0; d < k; ++d) {
+        RDST_HIP(hipMemcpy(tmp.data(), comps[d], n * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) out[k * i + d] = tmp[i];
+    }
+}
+
+}  // namespace rdst
+
+using namespace rdst;
+
+#define RDST_API_BODY(...)                                    \
+    try {                                                     \
+        __VA_ARGS__;                                          \
+        return RDST_OK;                                       \
+    } catch (const ArgError& e) {                             \
+        set_error(e.msg);                                     \
+        return e.code;                                        \
+    } catch (const HipError& e) {                             \
+        set_error(e.msg);                                     \
+        return RDST_ERR_HIP;                                  \
+    } catch (const std::exception& e) {                       \
+        set_error(e.what());                                  \
+        return RDST_ERR_ARG;                                  \
+    }
+
+extern "C" {
+
+const char* rdst_last_error(void) { return g_err.c_str(); }
+int rdst_version(void) { return 1; }
+
+int rdst_init(int device) {
+    RDST_API_BODY({
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw ArgError("no HIP device visible", RDST_ERR_NO_DEVICE);
+        if (device < 0 || device >= n) throw ArgError("rdst_init: device index out of range");
+        RDST_HIP(hipSetDevice(device));
+    })
+}
+
+int rdst_mesh_create(const rdst_mesh_desc* desc, rdst_mesh** out) {
+    RDST_API_BODY({
+        if (!out) throw ArgError("rdst_mesh_create: null out");
+        Mesh* m = new Mesh();
+        try {
+            mesh_build(*m, desc);
+        } catch (...) {
+            delete m;
+            throw;
+        }
+        *out = reinterpret_cast<rdst_mesh*>(m);
+    })
+}
+
+int rdst_mesh_destroy(rdst_mesh* mesh) {
+    RDST_API_BODY({ delete reinterpret_cast<Mesh*>(mesh); })
+}
+
+int rdst_mesh_info(const rdst_mesh* mesh, int64_t* info) {
+    RDST_API_BODY({
+        const Mesh* m = reinterpret_cast<const Mesh*>(mesh);
+        if (!m || !info) throw ArgError("rdst_mesh_info: null argument");
+        info[RDST_I_NCELLS] = m->nC;
+        info[RDST_I_NINTERNAL] = m->nI;
+        info[RDST_I_NBFACES] = m->nB;
+        info[RDST_I_NNZB] = m->nnzb;
+        info[RDST_I_NCOLOURS] = m->nColours;
+        info[RDST_I_NLEVELS_L] = m->nLevL;
+        info[RDST_I_NLEVELS_U] = m->nLevU;
+        info[RDST_I_NCELL_COLOURS] = m->nCellColours;
+    })
+}
+
+int rdst_mesh_get_int(const rdst_mesh* mesh, int which, int32_t* out) {
+    RDST_API_BODY({
+        const Mesh* m = reinterpret_cast<const Mesh*>(mesh);
+        if (!m || !out) throw ArgError("rdst_mesh_get_int: null argument");
+        const std::vector<int32_t>* v = nullptr;
+        std::vector<int32_t> tmp;
+        switch (which) {
+            case RDST_M_OWNER: v = &m->owner; break;
+            case RDST_M_NEIGHBOUR: v = &m->neigh; break;
+            case RDST_M_BCSR_ROWPTR: v = &m->rowPtr; break;
+            case RDST_M_BCSR_COLS: v = &m->cols; break;
+            case RDST_M_SLOT_UPPER: v = &m->slotUpper; break;
+            case RDST_M_SLOT_LOWER: v = &m->slotLower; break;
+            case RDST_M_SLOT_DIAG: v = &m->slotDiag; break;
+            case RDST_M_FACE_COLOUR: v = &m->faceColour; break;
+            case RDST_M_ILU_LEVEL: v = &m->levL; break;
+            case RDST_M_BFACE_PATCH: v = &m->bPatch; break;
+            case RDST_M_CELL_COLOUR: v = &m->cellColour; break;
+            default: throw ArgError("rdst_mesh_get_int: unknown array");
+        }
+        std::memcpy(out, v->data(), v->size() * sizeof(int32_t));
+    })
+}
+
+int rdst_mesh_get_geom(const rdst_mesh* mesh, int which, double* out) {
+    RDST_API_BODY({
+        const Mesh* m = reinterpret_cast<const Mesh*>(mesh);
+        if (!m || !out) throw ArgError("rdst_mesh_get_geom: null argument");
+        const std::vector<double>* v = nullptr;
+        switch (which) {
+            case RDST_G_SF: v = &m->Sf; break;
+            case RDST_G_MAGSF: v = &m->magSf; break;
+            case RDST_G_CF: v = &m->Cf; break;
+            case RDST_G_C: v = &m->C; break;
+            case RDST_G_V: v = &m->V; break;
+            case RDST_G_WEIGHTS: v = &m->w; break;
+            case RDST_G_DELTACOEFFS: v = &m->dc; break;
+            default: throw ArgError("rdst_mesh_get_geom: unknown array");
+        }
+        std::memcpy(out, v->data(), v->size() * sizeof(double));
+    })
+}
+
+int rdst_case_create(rdst_mesh* mesh, const rdst_thermo* thermo, const rdst_bc* bcs, const rdst_settings* settings,
+                     rdst_case** out) {
+    RDST_API_BODY({
+        Mesh* m = reinterpret_cast<Mesh*>(mesh);
+        if (!m || !thermo || !bcs || !settings || !out) throw ArgError("rdst_case_create: null argument");
+        const rdst_settings& s = *settings;
+        if (s.flux_scheme < 0 || s.flux_scheme > RDST_FLUX_KURGANOV)
+            throw ArgError("Unknown fluxScheme; valid fluxSchemes are: AUSM+ AUSM+up HLL HLLC HLLCP Kurganov",
+                           RDST_ERR_UNKNOWN_NAME);
+        if (s.solver != RDST_SOLVER_GMRES && s.solver != RDST_SOLVER_BICGSTAB)
+            throw ArgError("Unknown solver; valid solvers are: GMRES BiCGStab", RDST_ERR_UNKNOWN_NAME);
+        if (s.precond < RDST_PRECOND_ILU0 || s.precond > RDST_PRECOND_DIAG)
+            throw ArgError("Unknown preconditioner; valid preconditioners are: ILUC0 none diagonal", RDST_ERR_UNKNOWN_NAME);
+        if (s.dependent != RDST_DEP_W && s.dependent != RDST_DEP_DELTAW)
+            throw ArgError("Unknown dependentVariable; valid: W deltaW", RDST_ERR_UNKNOWN_NAME);
+        if (s.precision != RDST_PREC_FP64 && s.precision != RDST_PREC_MIXED) throw ArgError("Unknown precision");
+        if (s.n_directions < 1 || s.max_iter < 0 || s.min_iter < 0) throw ArgError("bad solver controls");
+        if (s.viscous) throw ArgError("the viscous terms of calcFields are not provided by this build", RDST_ERR_UNKNOWN_NAME);
+        if (s.ilu_order != RDST_ILU_NATURAL) throw ArgError("only the natural-order ILU(0) is provided by this build");
+        if (!(thermo->Cp > thermo->R) || !(thermo->R > 0)) throw ArgError("rdst_case_create: need Cp > R > 0");
+        if (!m->uploaded) {
+            mesh_upload(*m);
+            m->uploaded = true;
+        }
+        Case* c = new Case();
+        try {
+            c->mesh = m;
+            c->st = s;
+            c->th.Cp = thermo->Cp;
+            c->th.R = thermo->R;
+            c->th.Cv = thermo->Cp - thermo->R;
+            c->th.gamma = thermo->Cp / c->th.Cv;
+            c->th.transport = thermo->transport;
+            c->th.mu = thermo->mu;
+            c->th.Pr = thermo->Pr;
+            c->th.As = thermo->As;
+            c->th.Ts = thermo->Ts;
+            c->bcs.assign(bcs, bcs + m->nPatches);
+            std::vector<BcPatchDev> hb(std::max(m->nPatches, 1));
+            for (int p = 0; p < m->nPatches; ++p) {
+                const rdst_bc& b = bcs[p];
+                for (int t : {b.p_type, b.T_type, b.U_type})
+                    if (t < RDST_BC_FIXED_VALUE || t > RDST_BC_EMPTY) throw ArgError("unknown boundary condition type");
+                if (m->patches[p].type != RDST_PATCH_EMPTY && (b.p_type == RDST_BC_EMPTY || b.T_type == RDST_BC_EMPTY || b.U_type == RDST_BC_EMPTY))
+                    throw ArgError(std::string("empty BC on non-empty patch ") + m->patches[p].name);
+                BcPatchDev& d = hb[p];
+                d.p_type = b.p_type; d.T_type = b.T_type; d.U_type = b.U_type;
+                d.is_wall = m->patches[p].type == RDST_PATCH_WALL;
+                d.p_value = b.p_value; d.T_value = b.T_value;
+                d.p_inf = b.p_inf; d.T_inf = b.T_inf;
+                for (int k = 0; k < 3; ++k) { d.U_value[k] = b.U_value[k]; d.U_inf[k] = b.U_inf[k]; }
+            }
+            c->d_bc.upload(hb);
+            RDST_HIP(hipStreamCreate(&c->stream));
+            RDST_HIP(hipEventCreate(&c->ev0));
+            RDST_HIP(hipEventCreate(&c->ev1));
+            alloc_fields(*c);
+        } catch (...) {
+            delete c;
+            throw;
+        }
+        *out = reinterpret_cast<rdst_case*>(c);
+    })
+}
+
+int rdst_case_destroy(rdst_case* cc) {
+    RDST_API_BODY({
+        Case* c = reinterpret_cast<Case*>(cc);
+        if (!c) return RDST_OK;
+        solver_free(*c);
+        if (c->stream) (void)hipStreamSynchronize(c->stream);
+        if (c->ev0) (void)hipEventDestroy(c->ev0);
+        if (c->ev1) (void)hipEventDestroy(c->ev1);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        delete c;
+    })
+}
+
+int rdst_case_set_state(rdst_case* cc, const double* p, const double* T, const double* U) {
+    RDST_API_BODY({
+        Case* c = reinterpret_cast<Case*>(cc);
+        if (!c || !p || !T || !U) throw ArgError("rdst_case_set_state: null argument");
+        for (int i = 0; i < c->mesh->nC; ++i)
+            if (!(p[i] > 0) || !(T[i] > 0)) throw ArgError("rdst_case_set_state: p and T must be positive");
+        set_state(*c, p, T, U);
+    })
+}
+
+int rdst_case_get(rdst_case* cc, int field, double* out) {
+    RDST_API_BODY({
+        Case* c = reinterpret_cast<Case*>(cc);
+        if (!c || !out) throw ArgError("rdst_case_get: null argument");
+        RDST_HIP(hipStreamSynchronize(c->stream));
+        const Mesh& m = *c->mesh;
+        const size_t n = m.nC, nB = m.nB, nF = m.nF;
+        const Fields& f = c->f;
+        auto one = [&](const double* d, size_t cnt) { RDST_HIP(hipMemcpy(out, d, cnt * sizeof(double), hipMemcpyDeviceToHost)); };
+        switch (field) {
+            case RDST_F_RHO: one(f.rho, n); break;
+            case RDST_F_RHOU: { const double* v[3] = {f.rhoUx, f.rhoUy, f.rhoUz}; download_soa(v, 3, n, out); break; }
+            case RDST_F_RHOE: one(f.rhoE, n); break;
+            case RDST_F_P: one(f.p, n); break;
+            case RDST_F_T: one(f.T, n); break;
+            case RDST_F_U: { const double* v[3] = {f.Ux, f.Uy, f.Uz}; download_soa(v, 3, n, out); break; }
+            case RDST_F_H: one(f.h, n); break;
+            case RDST_F_PSI: one(f.psi, n); break;
+            case RDST_F_R: { const double* v[5] = {f.R, f.R + n, f.R + 2 * n, f.R + 3 * n, f.R + 4 * n}; download_soa(v, 5, n, out); break; }
+            case RDST_F_W: one(f.W, 5 * n); break;
+            case RDST_F_RDELTAT: one(f.rDeltaT, n); break;
+            case RDST_F_COURANT: one(f.courant, n); break;
+            case RDST_F_PHI: one(f.phi, nF); break;
+            case RDST_F_PHIUP: { const double* v[3] = {f.phiUx, f.phiUy, f.phiUz}; download_soa(v, 3, nF, out); break; }
+            case RDST_F_PHIEP: one(f.phiE, nF); break;
+            case RDST_F_UP: { const double* v[3] = {f.Upx, f.Upy, f.Upz}; download_soa(v, 3, nF, out); break; }
+            case RDST_F_BRHO: one(f.brho, nB); break;
+            case RDST_F_BU: { const double* v[3] = {f.bUx, f.bUy, f.bUz}; download_soa(v, 3, nB, out); break; }
+            case RDST_F_BP: one(f.bp, nB); break;
+            case RDST_F_BT: one(f.bT, nB); break;
+            case RDST_F_BRHOU: { const double* v[3] = {f.brhoUx, f.brhoUy, f.brhoUz}; download_soa(v, 3, nB, out); break; }
+            case RDST_F_BRHOE: one(f.brhoE, nB); break;
+            case RDST_F_BH: one(f.bh, nB); break;
+            case RDST_F_BPSI: one(f.bpsi, nB); break;
+            case RDST_F_BCSR_VALUES:
+                if (!c->hasMatrix || !c->A64.p) throw ArgError("no FP64 matrix assembled", RDST_ERR_STATE);
+                one(c->A64.p, 25 * (size_t)m.nnzb);
+                break;
+            case RDST_F_SOURCE: one(c->src.p, 5 * n); break;
+            case RDST_F_MU: one(f.mu, n); break;
+            case RDST_F_RHO_OLD: one(f.rhoOld, n); break;
+            case RDST_F_E_OLD: one(f.eOld, n); break;
+            default: throw ArgError("rdst_case_get: unknown field");
+        }
+    })
+}
+
+int rdst_case_put(rdst_case* cc, int field, const double* in) {
+    RDST_API_BODY({
+        Case* c = reinterpret_cast<Case*>(cc);
+        if (!c || !in) throw ArgError("rdst_case_put: null argument");
+        RDST_HIP(hipStreamSynchronize(c->stream));
+        const size_t n = c->mesh->nC;
+        Fields& f = c->f;
+        auto one = [&](double* d, size_t cnt) { RDST_HIP(hipMemcpy(d, in, cnt * sizeof(double), hipMemcpyHostToDevice)); };
+        switch (field) {
+            case RDST_F_RHO: one(f.rho, n); break;
+            case RDST_F_RHOE: one(f.rhoE, n); break;
+            case RDST_F_W: one(f.W, 5 * n); break;
+            case RDST_F_RDELTAT: one(f.rDeltaT, n); break;
+            case RDST_F_RHOU: {
+                std::vector<double> t(n);
+                double* d[3] = {f.rhoUx, f.rhoUy, f.rhoUz};
+                for (int k = 0; k < 3; ++k) {
+                    for (size_t i = 0; i < n; ++i) t[i] = in[3 * i + k];
+                    RDST_HIP(hipMemcpy(d[k], t.data(), n * sizeof(double), hipMemcpyHostToDevice));
+                }
+                break;
+            }
+            default: throw ArgError("rdst_case_put: field not writable");
+        }
+    })
+}
+
+int rdst_flux_residual(rdst_case* cc) {
+    RDST_API_BODY({
+        Case* c = reinterpret_cast<Case*>(cc);
+        if (!c) throw ArgError("null case");
+        flux_residual(*c, true);
+    })
+}
+
+int rdst_jacobian_assemble(rdst_case* cc) {
+    RDST_API_BODY({
+        Case* c = reinterpret_cast<Case*>(cc);
+        if (!c) throw ArgError("null case");
+        jacobian_assemble(*c);
+        RDST_HIP(hipStreamSynchronize(c->stream));
+    })
+}
+
+int rdst_block_solve(rdst_case* cc, rdst_solver_perf* perf) {
+    RDST_API_BODY({
+        Case* c = reinterpret_cast<Case*>(cc);
+        if (!c) throw ArgError("null case");
+        block_solve(*c, perf);
+        RDST_HIP(hipStreamSynchronize(c->stream));
+    })
+}
+
+int rdst_update_fields(rdst_case* cc) {
+    RDST_API_BODY({
+        Case* c = reinterpret_cast<Case*>(cc);
+        if (!c) throw ArgError("null case");
+        if (!c->hasState) throw ArgError("rdst_update_fields: no state", RDST_ERR_STATE);
+        update_fields(*c);
+        RDST_HIP(hipStreamSynchronize(c->stream));
+    })
+}
+
+int rdst_set_courant(rdst_case* cc, double* res_norm) {
+    RDST_API_BODY({
+        Case* c = reinterpret_cast<Case*>(cc);
+        if (!c) throw ArgError("null case");
+        if (!c->hasState) throw ArgError("rdst_set_courant: no state", RDST_ERR_STATE);
+        double r = set_courant(*c);
+        RDST_HIP(hipStreamSynchronize(c->stream));
+        if (res_norm) *res_norm = r;
+    })
+}
+
+int rdst_iterate(rdst_case* cc, int n_steps, double* res_norm, rdst_solver_perf* perf) {
+    RDST_API_BODY({
+        Case* c = reinterpret_cast<Case*>(cc);
+        if (!c) throw ArgError("null case");
+        if (!c->hasState) throw ArgError("rdst_iterate: no state", RDST_ERR_STATE);
+        for (int s = 0; s < n_steps; ++s) {
+            flux_launch(*c, false);
+            jacobian_assemble(*c);
+            rdst_solver_perf p{};
+            block_solve(*c, &p);
+            update_fields(*c);
+            double r = set_courant(*c);
+            if (res_norm) res_norm[s] = r;
+            if (perf) perf[s] = p;
+        }
+        RDST_HIP(hipStreamSynchronize(c->stream));
+    })
+}
+
+int rdst_forces(rdst_case* cc, int patch, const double* drag_dir, const double* lift_dir, double p_ref, double rho_inf,
+                double mag_u_inf, double a_ref, double* out) {
+    RDST_API_BODY({
+        Case* c = reinterpret_cast<Case*>(cc);
+        if (!c || !drag_dir || !lift_dir || !out) throw ArgError("rdst_forces: null argument");
+        const Mesh& m = *c->mesh;
+        if (patch < 0 || patch >= m.nPatches) throw ArgError("rdst_forces: patch index out of range");
+        forces(*c, patch, drag_dir, lift_dir, p_ref, rho_inf, mag_u_inf, a_ref, out);
+    })
+}
+
+int rdst_last_flux_ms(rdst_case* cc, double* ms) {
+    RDST_API_BODY({
+        Case* c = reinterpret_cast<Case*>(cc);
+        if (!c || !ms) throw ArgError("null argument");
+        *ms = c->lastFluxMs;
+    })
+}
+
+int rdst_time_flux(rdst_case* cc, int n, double* ms_per_call) {
+    RDST_API_BODY({
+        Case* c = reinterpret_cast<Case*>(cc);
+        if (!c || !ms_per_call || n < 1) throw ArgError("rdst_time_flux: bad argument");
+        *ms_per_call = flux_time(*c, n);
+    })
+}
+
+}  // extern "C"
+
+namespace rdst {
+
+// forceCoeffs restated for the pressure part: F = sum_patch (p_b - pRef) Sf,
+// Cd = F.dragDir / (0.5 rhoInf |UInf|^2 Aref), Cl likewise (p in Pa).
+void forces(Case& c, int patch, const double* dragDir, const double* liftDir, double pRef, double rhoInf, double magUInf,
+            double aRef, double* out) {
+    const Mesh& m = *c.mesh;
+    RDST_HIP(hipStreamSynchronize(c.stream));
+    std::vector<double> bp(m.nB);
+    if (m.nB) RDST_HIP(hipMemcpy(bp.data(), c.f.bp, m.nB * sizeof(double), hipMemcpyDeviceToHost));
+    double F[3] = {0, 0, 0};
+    for (int b = 0; b < m.nB; ++b) {
+        if (m.bPatch[b] != patch) continue;
+        const int k = m.nI + b;
+        for (int d = 0; d < 3; ++d) F[d] += (bp[b] - pRef) * m.Sf[3 * (size_t)k + d];
+    }
+    const double q = 0.5 * rhoInf * magUInf * magUInf * aRef;
+    out[0] = (F[0] * dragDir[0] + F[1] * dragDir[1] + F[2] * dragDir[2]) / q;
+    out[1] = (F[0] * liftDir[0] + F[1] * liftDir[1] + F[2] * liftDir[2]) / q;
+    out[2] = F[0]; out[3] = F[1]; out[4] = F[2];
+}
+
+}  // namespace rdst

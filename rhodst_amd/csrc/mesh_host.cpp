@@ -1,0 +1,330 @@
+// mesh_host.cpp -- fvMesh geometry and addressing for the device path.
+//
+// The reference reads Sf, magSf, Cf, C, V, weights and deltaCoeffs from
+// foam-extend's fvMesh (e.g. Roe.C:79, :195-197; timeDST.C:237); this file
+// computes the same quantities from the polyMesh primitives on the host
+// (face centres/areas by triangle decomposition about the point average,
+// cell centres/volumes by face pyramids about the face-centre average,
+// linear weights from the face-normal distances, stabilised deltaCoeffs
+// 1/max(n.d, 0.05|d|) as the comment at timeDST.C:237 states) and derives
+// the device addressing: the ELL cell->face map the cell-centric face loop
+// walks, the owner/neighbour-sorted block-CSR with its face->slot maps, a
+// greedy face colouring, and the level sets of the natural-order ILU sweeps.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "rdst_internal.hpp"
+
+namespace rdst {
+
+namespace {
+struct P3 { double x, y, z; };
+inline P3 add(P3 a, P3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline P3 sub(P3 a, P3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline P3 scl(double s, P3 a) { return {s * a.x, s * a.y, s * a.z}; }
+inline double dotp(P3 a, P3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline P3 crossp(P3 a, P3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline double norm(P3 a) { return std::sqrt(dotp(a, a)); }
+}  // namespace
+
+void mesh_build(Mesh& m, const rdst_mesh_desc* d) {
+    if (!d || d->n_cells <= 0 || d->n_faces <= 0 || d->n_internal_faces < 0 || d->n_internal_faces > d->n_faces)
+        throw ArgError("rdst_mesh_create: bad mesh sizes");
+    if (!d->points || !d->face_offsets || !d->face_points || !d->owner || (d->n_internal_faces && !d->neighbour))
+        throw ArgError("rdst_mesh_create: null array");
+    const int nFaces = d->n_faces, nInt = d->n_internal_faces, nCells = d->n_cells;
+    for (int f = 0; f < nFaces; ++f) {
+        if (d->owner[f] < 0 || d->owner[f] >= nCells) throw ArgError("rdst_mesh_create: owner out of range");
+        if (d->face_offsets[f + 1] - d->face_offsets[f] < 3) throw ArgError("rdst_mesh_create: face with < 3 points");
+    }
+    for (int f = 0; f < nInt; ++f) {
+        if (d->neighbour[f] < 0 || d->neighbour[f] >= nCells) throw ArgError("rdst_mesh_create: neighbour out of range");
+        if (d->neighbour[f] <= d->owner[f]) throw ArgError("rdst_mesh_create: internal faces must have owner < neighbour");
+    }
+    m.patches.assign(d->patches, d->patches + d->n_patches);
+    m.nPatches = d->n_patches;
+    m.nC = nCells;
+    m.nI = nInt;
+
+    // --- face centres and areas
+    std::vector<P3> fSf(nFaces), fCf(nFaces);
+    std::vector<double> fmag(nFaces);
+    auto pt = [&](int i) { return P3{d->points[3 * i], d->points[3 * i + 1], d->points[3 * i + 2]}; };
+    for (int f = 0; f < nFaces; ++f) {
+        const int s = d->face_offsets[f], n = d->face_offsets[f + 1] - s;
+        const int32_t* fp = d->face_points + s;
+        for (int k = 0; k < n; ++k)
+            if (fp[k] < 0 || fp[k] >= d->n_points) throw ArgError("rdst_mesh_create: face point out of range");
+        if (n == 3) {
+            P3 p0 = pt(fp[0]), p1 = pt(fp[1]), p2 = pt(fp[2]);
+            fCf[f] = scl(1.0 / 3.0, add(add(p0, p1), p2));
+            fSf[f] = scl(0.5, crossp(sub(p1, p0), sub(p2, p0)));
+        } else {
+            P3 c = pt(fp[0]);
+            for (int k = 1; k < n; ++k) c = add(c, pt(fp[k]));
+            c = {c.x / n, c.y / n, c.z / n};
+            P3 sN = {0, 0, 0}, sAc = {0, 0, 0};
+            double sA = 0;
+            for (int k = 0; k < n; ++k) {
+                P3 a = pt(fp[k]), b = pt(fp[(k + 1) % n]);
+                P3 cc = add(add(a, b), c);
+                P3 nv = crossp(sub(b, a), sub(c, a));
+                double ar = norm(nv);
+                sN = add(sN, nv);
+                sA += ar;
+                sAc = add(sAc, scl(ar, cc));
+            }
+            if (sA < 1e-150) {
+                fCf[f] = c;
+                fSf[f] = {0, 0, 0};
+            } else {
+                P3 t = scl(1.0 / 3.0, sAc);  // (1/3)*sumAc/sumA, left to right
+                fCf[f] = {t.x / sA, t.y / sA, t.z / sA};
+                fSf[f] = scl(0.5, sN);
+            }
+        }
+        fmag[f] = norm(fSf[f]);
+    }
+    // --- cell centres and volumes
+    std::vector<P3> cEst(nCells, P3{0, 0, 0}), cC(nCells, P3{0, 0, 0});
+    std::vector<int> nCF(nCells, 0);
+    std::vector<double> vol(nCells, 0.0);
+    for (int f = 0; f < nFaces; ++f) { cEst[d->owner[f]] = add(cEst[d->owner[f]], fCf[f]); nCF[d->owner[f]]++; }
+    for (int f = 0; f < nInt; ++f) { cEst[d->neighbour[f]] = add(cEst[d->neighbour[f]], fCf[f]); nCF[d->neighbour[f]]++; }
+    for (int c = 0; c < nCells; ++c) cEst[c] = {cEst[c].x / nCF[c], cEst[c].y / nCF[c], cEst[c].z / nCF[c]};
+    for (int f = 0; f < nFaces; ++f) {
+        int o = d->owner[f];
+        double v3 = std::max(dotp(fSf[f], sub(fCf[f], cEst[o])), 1e-300);
+        P3 pc = add(scl(0.75, fCf[f]), scl(0.25, cEst[o]));
+        cC[o] = add(cC[o], scl(v3, pc));
+        vol[o] += v3;
+    }
+    for (int f = 0; f < nInt; ++f) {
+        int nb = d->neighbour[f];
+        double v3 = std::max(dotp(fSf[f], sub(cEst[nb], fCf[f])), 1e-300);
+        P3 pc = add(scl(0.75, fCf[f]), scl(0.25, cEst[nb]));
+        cC[nb] = add(cC[nb], scl(v3, pc));
+        vol[nb] += v3;
+    }
+    for (int c = 0; c < nCells; ++c) {
+        cC[c] = {cC[c].x / vol[c], cC[c].y / vol[c], cC[c].z / vol[c]};
+        vol[c] *= (1.0 / 3.0);
+        if (!(vol[c] > 0)) throw ArgError("rdst_mesh_create: non-positive cell volume");
+    }
+    // --- compacted boundary faces (empty patches have fvPatch size 0)
+    m.bPatch.clear();
+    m.bPolyFace.clear();
+    for (int p = 0; p < m.nPatches; ++p) {
+        const rdst_patch& pp = m.patches[p];
+        if (pp.start < nInt || pp.start + pp.size > nFaces) throw ArgError("rdst_mesh_create: patch range");
+        if (pp.type == RDST_PATCH_EMPTY) continue;
+        for (int i = 0; i < pp.size; ++i) { m.bPatch.push_back(p); m.bPolyFace.push_back(pp.start + i); }
+    }
+    m.nB = (int)m.bPatch.size();
+    m.nF = m.nI + m.nB;
+    const int nF = m.nF;
+    m.Sf.resize(3 * (size_t)nF); m.Cf.resize(3 * (size_t)nF); m.magSf.resize(nF); m.w.assign(nF, 1.0); m.dc.resize(nF);
+    m.owner.resize(nF); m.neigh.assign(d->neighbour, d->neighbour + nInt);
+    for (int k = 0; k < nF; ++k) {
+        int f = k < nInt ? k : m.bPolyFace[k - nInt];
+        m.Sf[3 * k] = fSf[f].x; m.Sf[3 * k + 1] = fSf[f].y; m.Sf[3 * k + 2] = fSf[f].z;
+        m.Cf[3 * k] = fCf[f].x; m.Cf[3 * k + 1] = fCf[f].y; m.Cf[3 * k + 2] = fCf[f].z;
+        m.magSf[k] = fmag[f];
+        m.owner[k] = d->owner[f];
+    }
+    m.C.resize(3 * (size_t)nCells); m.V = vol;
+    for (int c = 0; c < nCells; ++c) { m.C[3 * c] = cC[c].x; m.C[3 * c + 1] = cC[c].y; m.C[3 * c + 2] = cC[c].z; }
+    for (int f = 0; f < nInt; ++f) {
+        P3 sf = fSf[f], cf = fCf[f], co = cC[d->owner[f]], cn = cC[d->neighbour[f]];
+        double dOwn = std::fabs(dotp(sf, sub(cf, co)));
+        double dNei = std::fabs(dotp(sf, sub(cn, cf)));
+        m.w[f] = dNei / (dOwn + dNei);
+        P3 delta = sub(cn, co);
+        P3 ua = {sf.x / fmag[f], sf.y / fmag[f], sf.z / fmag[f]};
+        m.dc[f] = 1.0 / std::max(dotp(ua, delta), 0.05 * norm(delta));
+    }
+    for (int b = 0; b < m.nB; ++b) {
+        int f = m.bPolyFace[b];
+        P3 delta = sub(fCf[f], cC[d->owner[f]]);
+        m.dc[nInt + b] = 1.0 / norm(delta);
+    }
+
+    // --- ELL cell->face map, faces of a cell in surface-field order
+    std::vector<int> deg(nCells, 0);
+    for (int f = 0; f < nInt; ++f) { deg[m.owner[f]]++; deg[m.neigh[f]]++; }
+    for (int b = 0; b < m.nB; ++b) deg[m.owner[nInt + b]]++;
+    m.ellW = nCells ? *std::max_element(deg.begin(), deg.end()) : 0;
+    const size_t E = (size_t)m.ellW * nCells;
+    m.ellCode.assign(E, (ELL_NONE << 30));
+    m.ellFace.assign(E, -1);
+    std::vector<int> fill(nCells, 0);
+    // faces in increasing surface-field index: internal faces then boundary
+    for (int f = 0; f < nInt; ++f) {
+        int o = m.owner[f], nb = m.neigh[f];
+        size_t eo = (size_t)fill[o]++ * nCells + o;
+        m.ellCode[eo] = (ELL_OWNER << 30) | (uint32_t)nb;
+        m.ellFace[eo] = f;
+        size_t en = (size_t)fill[nb]++ * nCells + nb;
+        m.ellCode[en] = (ELL_NEIGH << 30) | (uint32_t)o;
+        m.ellFace[en] = f;
+    }
+    for (int b = 0; b < m.nB; ++b) {
+        int c = m.owner[nInt + b];
+        size_t e = (size_t)fill[c]++ * nCells + c;
+        m.ellCode[e] = (ELL_BOUNDARY << 30) | (uint32_t)b;
+        m.ellFace[e] = nInt + b;
+    }
+
+    // --- block-CSR (owner/neighbour-sorted): row i holds the diagonal, the
+    // upper coefficients of faces it owns (column = neighbour) and the lower
+    // coefficients of faces it neighbours (column = owner), columns ascending
+    std::vector<std::vector<std::pair<int, int>>> rows(nCells);  // (col, tag) tag: -1 diag, 2f upper, 2f+1 lower
+    for (int c = 0; c < nCells; ++c) rows[c].push_back({c, -1});
+    for (int f = 0; f < nInt; ++f) {
+        rows[m.owner[f]].push_back({m.neigh[f], 2 * f});
+        rows[m.neigh[f]].push_back({m.owner[f], 2 * f + 1});
+    }
+    m.rowPtr.assign(nCells + 1, 0);
+    for (int c = 0; c < nCells; ++c) {
+        std::sort(rows[c].begin(), rows[c].end());
+        m.rowPtr[c + 1] = m.rowPtr[c] + (int)rows[c].size();
+    }
+    m.nnzb = m.rowPtr[nCells];
+    m.cols.resize(m.nnzb);
+    m.slotUpper.resize(nInt); m.slotLower.resize(nInt); m.slotDiag.resize(nCells);
+    for (int c = 0; c < nCells; ++c)
+        for (size_t k = 0; k < rows[c].size(); ++k) {
+            int s = m.rowPtr[c] + (int)k;
+            m.cols[s] = rows[c][k].first;
+            int tag = rows[c][k].second;
+            if (tag == -1) m.slotDiag[c] = s;
+            else if ((tag & 1) == 0) m.slotUpper[tag >> 1] = s;
+            else m.slotLower[tag >> 1] = s;
+        }
+    for (int c = 0; c < nCells; ++c)
+        for (int s = m.rowPtr[c] + 1; s < m.rowPtr[c + 1]; ++s)
+            if (m.cols[s] == m.cols[s - 1]) throw ArgError("rdst_mesh_create: duplicate internal face between two cells");
+
+    // --- ELL slot of the off-diagonal block each cell writes: the owner
+    // writes the lower coefficient (row neighbour, column owner), the
+    // neighbour writes the upper one (row owner, column neighbour)
+    m.ellSlot.assign(E, -1);
+    for (size_t e = 0; e < E; ++e) {
+        uint32_t kind = m.ellCode[e] >> 30;
+        int f = m.ellFace[e];
+        if (kind == ELL_OWNER) m.ellSlot[e] = m.slotLower[f];
+        else if (kind == ELL_NEIGH) m.ellSlot[e] = m.slotUpper[f];
+    }
+
+    // --- greedy face colouring in face order (deterministic)
+    m.faceColour.assign(nInt, -1);
+    {
+        std::vector<uint64_t> used(nCells, 0);
+        int maxc = 0;
+        for (int f = 0; f < nInt; ++f) {
+            uint64_t u = used[m.owner[f]] | used[m.neigh[f]];
+            int col = 0;
+            while (col < 64 && (u >> col & 1ull)) ++col;
+            if (col >= 64) throw ArgError("rdst_mesh_create: face colouring needs more than 64 colours");
+            m.faceColour[f] = col;
+            used[m.owner[f]] |= 1ull << col;
+            used[m.neigh[f]] |= 1ull << col;
+            maxc = std::max(maxc, col + 1);
+        }
+        m.nColours = maxc;
+    }
+    // --- greedy cell colouring (distance-1) in cell order
+    m.cellColour.assign(nCells, -1);
+    {
+        int maxc = 0;
+        for (int c = 0; c < nCells; ++c) {
+            uint64_t u = 0;
+            for (int s = m.rowPtr[c]; s < m.rowPtr[c + 1]; ++s) {
+                int j = m.cols[s];
+                if (j != c && m.cellColour[j] >= 0) u |= 1ull << m.cellColour[j];
+            }
+            int col = 0;
+            while (col < 64 && (u >> col & 1ull)) ++col;
+            if (col >= 64) throw ArgError("rdst_mesh_create: cell colouring needs more than 64 colours");
+            m.cellColour[c] = col;
+            maxc = std::max(maxc, col + 1);
+        }
+        m.nCellColours = maxc;
+    }
+    // --- level sets of the natural-order triangular sweeps
+    m.levL.assign(nCells, 0);
+    for (int c = 0; c < nCells; ++c) {
+        int lv = 0;
+        for (int s = m.rowPtr[c]; s < m.rowPtr[c + 1]; ++s)
+            if (m.cols[s] < c) lv = std::max(lv, m.levL[m.cols[s]] + 1);
+        m.levL[c] = lv;
+    }
+    m.levU.assign(nCells, 0);
+    for (int c = nCells - 1; c >= 0; --c) {
+        int lv = 0;
+        for (int s = m.rowPtr[c]; s < m.rowPtr[c + 1]; ++s)
+            if (m.cols[s] > c) lv = std::max(lv, m.levU[m.cols[s]] + 1);
+        m.levU[c] = lv;
+    }
+    auto bucket = [&](const std::vector<int32_t>& lev, int& nLev, std::vector<int32_t>& ptr, std::vector<int32_t>& rowsOut) {
+        nLev = nCells ? *std::max_element(lev.begin(), lev.end()) + 1 : 0;
+        ptr.assign(nLev + 1, 0);
+        for (int c = 0; c < nCells; ++c) ptr[lev[c] + 1]++;
+        for (int l = 0; l < nLev; ++l) ptr[l + 1] += ptr[l];
+        rowsOut.resize(nCells);
+        std::vector<int> pos(ptr.begin(), ptr.end() - 1);
+        for (int c = 0; c < nCells; ++c) rowsOut[pos[lev[c]]++] = c;
+    };
+    bucket(m.levL, m.nLevL, m.levLPtr, m.levLRows);
+    bucket(m.levU, m.nLevU, m.levUPtr, m.levURows);
+}
+
+void mesh_upload(Mesh& m) {
+    const int nC = m.nC, nF = m.nF, W = m.ellW;
+    m.d_owner.upload(m.owner.data(), m.nI);
+    m.d_neigh.upload(m.neigh);
+    std::vector<int32_t> bcell(m.nB), bwall(m.nB);
+    for (int b = 0; b < m.nB; ++b) {
+        bcell[b] = m.owner[m.nI + b];
+        bwall[b] = m.patches[m.bPatch[b]].type == RDST_PATCH_WALL ? 1 : 0;
+    }
+    m.d_bcell.upload(bcell);
+    m.d_bpatch.upload(m.bPatch);
+    m.d_bWall.upload(bwall);
+    std::vector<double> soa(3 * (size_t)nF);
+    for (int k = 0; k < nF; ++k)
+        for (int d = 0; d < 3; ++d) soa[(size_t)d * nF + k] = m.Sf[3 * (size_t)k + d];
+    m.d_Sf.upload(soa);
+    m.d_magSf.upload(m.magSf);
+    m.d_w.upload(m.w);
+    m.d_dc.upload(m.dc);
+    m.d_V.upload(m.V);
+    m.d_ellCode.upload(m.ellCode);
+    m.d_ellFace.upload(m.ellFace);
+    m.d_ellSlot.upload(m.ellSlot);
+    const size_t E = (size_t)W * nC;
+    std::vector<double> esf(3 * E, 0.0), emag(E, 0.0), ew(E, 0.0), edc(E, 0.0);
+    for (size_t e = 0; e < E; ++e) {
+        int k = m.ellFace[e];
+        if (k < 0) continue;
+        for (int d = 0; d < 3; ++d) esf[(size_t)d * E + e] = m.Sf[3 * (size_t)k + d];
+        emag[e] = m.magSf[k];
+        ew[e] = m.w[k];
+        edc[e] = m.dc[k];
+    }
+    m.d_ellSf.upload(esf);
+    m.d_ellMagSf.upload(emag);
+    m.d_ellWt.upload(ew);
+    m.d_ellDc.upload(edc);
+    m.d_rowPtr.upload(m.rowPtr);
+    m.d_cols.upload(m.cols);
+    m.d_slotUpper.upload(m.slotUpper);
+    m.d_slotLower.upload(m.slotLower);
+    m.d_slotDiag.upload(m.slotDiag);
+    m.d_levLRows.upload(m.levLRows);
+    m.d_levURows.upload(m.levURows);
+}
+
+}  // namespace rdst

@@ -1,0 +1,203 @@
+// rdst_internal.hpp -- host-side structures of the MI355X block-coupled
+// implicit step: the uploaded mesh, the device-resident case state and the
+// kernel launch helpers shared by the .hip translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/rhodst.h"
+
+namespace rdst {
+
+void set_error(const std::string& msg);
+
+#define RDST_HIP(call)                                                                  \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            throw ::rdst::HipError(std::string(#call) + ": " + hipGetErrorString(e_)); \
+        }                                                                               \
+    } while (0)
+
+struct HipError : std::exception {
+    std::string msg;
+    explicit HipError(std::string m) : msg(std::move(m)) {}
+    const char* what() const noexcept override { return msg.c_str(); }
+};
+struct ArgError : std::exception {
+    std::string msg;
+    int code;
+    ArgError(std::string m, int c = RDST_ERR_ARG) : msg(std::move(m)), code(c) {}
+    const char* what() const noexcept override { return msg.c_str(); }
+};
+
+// A device buffer with RAII.
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) RDST_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+    }
+    void upload(const T* h, size_t count) {
+        alloc(count);
+        if (count) RDST_HIP(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice));
+    }
+    void upload(const std::vector<T>& h) { upload(h.data(), h.size()); }
+    void zero(hipStream_t s = 0) {
+        if (n) RDST_HIP(hipMemsetAsync(p, 0, n * sizeof(T), s));
+    }
+    void download(T* h, size_t count) const {
+        if (count) RDST_HIP(hipMemcpy(h, p, count * sizeof(T), hipMemcpyDeviceToHost));
+    }
+};
+
+// ELL cell->face entry kinds (bits 30-31 of the code word)
+enum : uint32_t { ELL_OWNER = 0u, ELL_NEIGH = 1u, ELL_BOUNDARY = 2u, ELL_NONE = 3u };
+static constexpr uint32_t ELL_IDX_MASK = 0x3fffffffu;
+
+// ---------------------------------------------------------------------------
+// Mesh: polyMesh primitives, fvMesh geometry (host), device copies.
+struct Mesh {
+    int nC = 0, nI = 0, nB = 0, nF = 0;  // nF = nI + nB surface-field length
+    int nPatches = 0;
+    std::vector<rdst_patch> patches;
+    // host geometry (surface fields: internal faces then compacted boundary faces)
+    std::vector<double> Sf, Cf, magSf, w, dc;  // Sf/Cf [nF][3]
+    std::vector<double> C, V;                  // C [nC][3]
+    std::vector<int32_t> owner, neigh;         // owner [nF] (boundary: face cell), neigh [nI]
+    std::vector<int32_t> bPatch, bPolyFace;    // [nB]
+    // ELL cell->face map (faces of each cell in face order: internal ascending, then boundary)
+    int ellW = 0;
+    std::vector<uint32_t> ellCode;   // [ellW][nC]
+    std::vector<int32_t> ellFace;    // [ellW][nC]  surface-field index or -1
+    std::vector<int32_t> ellSlot;    // [ellW][nC]  BCSR slot of the off-diagonal block this cell writes, or -1
+    // block-CSR, rows = cells, columns sorted
+    int nnzb = 0;
+    std::vector<int32_t> rowPtr, cols, slotUpper, slotLower, slotDiag;
+    // face colouring (no two faces of one colour share a cell)
+    int nColours = 0;
+    std::vector<int32_t> faceColour;
+    std::vector<int32_t> cellColour;
+    int nCellColours = 0;
+    // level sets of the natural-order lower/upper triangular sweeps
+    std::vector<int32_t> levL, levU;           // level of each row
+    std::vector<int32_t> levLPtr, levLRows, levUPtr, levURows;
+    int nLevL = 0, nLevU = 0;
+    bool uploaded = false;  // device copies are made by the first rdst_case_create
+
+    // ---- device copies
+    DBuf<int32_t> d_owner, d_neigh, d_bcell, d_bpatch;
+    DBuf<double> d_Sf, d_magSf, d_w, d_dc, d_V;  // Sf SoA [3][nF]
+    DBuf<uint32_t> d_ellCode;
+    DBuf<int32_t> d_ellFace, d_ellSlot;
+    DBuf<double> d_ellSf, d_ellMagSf, d_ellWt, d_ellDc;  // [3][ellW][nC], [ellW][nC]
+    DBuf<int32_t> d_rowPtr, d_cols, d_slotUpper, d_slotLower, d_slotDiag;
+    DBuf<int32_t> d_levLRows, d_levURows;
+    DBuf<int32_t> d_bWall;  // [nB] 1 where the patch is a wall
+};
+
+void mesh_build(Mesh& m, const rdst_mesh_desc* d);
+void mesh_upload(Mesh& m);
+
+// ---------------------------------------------------------------------------
+// Device-side views passed to kernels by value.
+struct BcPatchDev {
+    int p_type, T_type, U_type, is_wall;
+    double p_value, T_value, U_value[3];
+    double p_inf, T_inf, U_inf[3];
+};
+
+struct Thermo {
+    double Cp, R, Cv, gamma;
+    int transport;
+    double mu, Pr, As, Ts;
+};
+
+struct Fields {
+    // cells (SoA)
+    double *rho, *rhoUx, *rhoUy, *rhoUz, *rhoE, *p, *T, *h, *psi, *mu, *alpha;
+    double *Ux, *Uy, *Uz;
+    // boundary faces
+    double *brho, *bUx, *bUy, *bUz, *bp, *bT, *bh, *bpsi, *bmu, *balpha, *brhoUx, *brhoUy, *brhoUz, *brhoE;
+    double *vf_p, *vf_T, *vf_U, *TrefH;
+    // surface fields (internal then boundary)
+    double *phi, *phiUx, *phiUy, *phiUz, *phiE, *Upx, *Upy, *Upz;
+    // residual R_ [5][nC]
+    double* R;
+    // W [nC][5] (AoS, the linear-solve vector)
+    double* W;
+    // time control
+    double *rDeltaT, *courant, *rhoOld, *eOld, *dS;  // dS [3][nC]
+    double* fH;      // HLLCP pressure-sensor cell values pow3(f) [nC]
+};
+
+struct MeshDev {
+    int nC, nI, nB, nF, ellW;
+    const int32_t *owner, *neigh, *bcell, *bpatch, *bwall;
+    const double *Sf, *magSf, *w, *dc, *V;
+    const uint32_t* ellCode;
+    const int32_t *ellFace, *ellSlot;
+    const int32_t* slotDiag;
+    const double *ellSf, *ellMagSf, *ellWt, *ellDc;
+};
+
+// ---------------------------------------------------------------------------
+struct Case {
+    Mesh* mesh = nullptr;
+    Thermo th{};
+    rdst_settings st{};
+    std::vector<rdst_bc> bcs;
+    DBuf<BcPatchDev> d_bc;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float lastFluxMs = 0.f;
+
+    // field storage
+    DBuf<double> cellBuf, bndBuf, faceBuf, RBuf, WBuf, timeBuf, fHBuf;
+    Fields f{};
+    // matrix
+    DBuf<double> A64, src;          // [nnzb][25], [nC][5]
+    DBuf<float> A32;                // FP32 copy for mixed precision
+    bool hasState = false, hasMatrix = false;
+    // solver scratch owned by the solver module
+    struct SolverWork* work = nullptr;
+    // time-control scalars
+    double residual = -1, residualRatio = 1, courantScalar = 1;
+    std::vector<double> hostScratch;
+
+    MeshDev meshDev() const;
+};
+
+// kernels / steps (defined in the .hip units)
+void flux_residual(Case& c, bool storeFaceFluxes);
+void flux_launch(Case& c, bool storeFaceFluxes);
+void jacobian_assemble(Case& c);
+void block_solve(Case& c, rdst_solver_perf* perf);
+void update_fields(Case& c);
+double set_courant(Case& c);
+void set_steady_state_deltaT(Case& c);
+void set_state(Case& c, const double* p, const double* T, const double* U);
+void solver_free(Case& c);
+double flux_time(Case& c, int n);
+void forces(Case& c, int patch, const double* dragDir, const double* liftDir, double pRef, double rhoInf,
+            double magUInf, double aRef, double* out);
+
+inline int blocks_for(long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+}  // namespace rdst

@@ -1,0 +1,752 @@
+// solve.hip -- the block linear solve A.x = b behind fvBlockMatrixDST::solve
+// (rhoDST/fvBlockMatrixDST/fvBlockMatrixDST.C:98-155), i.e. the foam-extend
+// BlockLduSolver<vector5> GMRES / BiCGStab with the ILUC0 preconditioner
+// selected from fvSolution.solvers.W (Vector5typedefs.C:1596-1711).
+//
+// Matrix: owner/neighbour-sorted block-CSR of 5x5 row-major blocks.
+// Preconditioner: block ILU(0) in the natural cell order, computed and
+// applied level by level (rows of one level are independent), which gives
+// the same factors as a sequential natural-order ILU(0) -- ILU(0) factors
+// are unique for a given ordering and sparsity.
+// GMRES: restarted, left-preconditioned, modified Gram-Schmidt; each MGS
+// step is one fused kernel (w -= h_j V_j and the partial dot with the next
+// vector), reductions are deterministic two-level (wavefront shuffles, then
+// every block re-reduces the previous kernel's block partials in the same
+// order), and the Hessenberg/Givens/back-substitution run on the device in
+// FP64, so a restart cycle needs no host round trip.
+// Mixed precision: FP32 matrix, FP32 ILU factors and FP32 Krylov basis
+// inside an FP64 defect-correction loop (r = b - A x, norms, Hessenberg and
+// Givens in FP64).
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "rdst_internal.hpp"
+
+namespace rdst {
+
+constexpr int RED_BS = 256;     // threads per reduction block
+constexpr int RED_MAXG = 1024;  // max blocks of a reduction kernel
+
+struct SolverWork {
+    int n = 0, n5 = 0, nDirs = 0, G = 0;
+    bool mixed = false;
+    DBuf<double> LU, Dinv;     // FP64 factors
+    DBuf<float> LU32, Dinv32;  // FP32 factors
+    DBuf<double> b, r, wA, tmp, x0;
+    DBuf<double> V;            // (nDirs+1) x n5 FP64 basis
+    DBuf<float> V32, w32, r32, t32;
+    DBuf<double> scal;         // device scalars
+    DBuf<double> partA, partB; // reduction partials
+    std::vector<double> hscal;
+    // BiCGStab
+    DBuf<double> p, ph, v, t, th, rw;
+};
+
+// scalar slots in SolverWork::scal
+enum {
+    S_BETA = 0, S_NORM = 1, S_INIT = 2 /*5*/, S_FIN = 7 /*5*/, S_XREF = 12 /*5*/, S_TMP = 17 /*8*/,
+    S_H = 32  // then H[nDirs*nDirs], cs[nDirs], sn[nDirs], bh[nDirs+1], yh[nDirs]
+};
+
+namespace {
+
+// ---------------------------------------------------------------- reductions
+__device__ inline double block_sum(double v, double* sh) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) sh[wv] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += sh[k];
+    __syncthreads();
+    return t;  // valid in thread 0
+}
+
+// every block sums the G partials of the previous kernel in the same order
+__device__ inline double reduce_partials(const double* part, int G, double* sh) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < G; i += blockDim.x) v += part[i];
+    double t = block_sum(v, sh);
+    if (threadIdx.x == 0) sh[8] = t;
+    __syncthreads();
+    double r = sh[8];
+    __syncthreads();
+    return r;
+}
+
+template <class TA, class TB>
+__global__ __launch_bounds__(RED_BS) void k_dot_partial(long n, const TA* a, const TB* b, double* part) {
+    __shared__ double sh[16];
+    double s = 0.0;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        s += (double)a[i] * (double)b[i];
+    double t = block_sum(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// out = sum(part) (op 0) or sqrt(sum) (op 1); single block
+__global__ __launch_bounds__(RED_BS) void k_finalize(const double* part, int G, double* out, int op) {
+    __shared__ double sh[16];
+    double t = reduce_partials(part, G, sh);
+    if (threadIdx.x == 0) out[0] = op == 1 ? sqrt(t) : t;
+}
+
+// per-component sum of |r| over an AoS [n][5] vector: partials [5][G]
+__global__ __launch_bounds__(RED_BS) void k_cmptmag_partial(int n, const double* r, double* part) {
+    __shared__ double sh[16];
+    double s[5] = {0, 0, 0, 0, 0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        for (int k = 0; k < 5; ++k) s[k] += fabs(r[5 * i + k]);
+    for (int k = 0; k < 5; ++k) {
+        double t = block_sum(s[k], sh);
+        if (threadIdx.x == 0) part[k * gridDim.x + blockIdx.x] = t;
+    }
+}
+// out[k] = sum_k / norm
+__global__ __launch_bounds__(RED_BS) void k_cmptmag_final(const double* part, int G, const double* norm, double* out) {
+    __shared__ double sh[16];
+    for (int k = 0; k < 5; ++k) {
+        double t = reduce_partials(part + k * G, G, sh);
+        if (threadIdx.x == 0) out[k] = t / norm[0];
+    }
+}
+// per-component sums of x (for xRef): partials [5][G]
+__global__ __launch_bounds__(RED_BS) void k_cmptsum_partial(int n, const double* x, double* part) {
+    __shared__ double sh[16];
+    double s[5] = {0, 0, 0, 0, 0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        for (int k = 0; k < 5; ++k) s[k] += x[5 * i + k];
+    for (int k = 0; k < 5; ++k) {
+        double t = block_sum(s[k], sh);
+        if (threadIdx.x == 0) part[k * gridDim.x + blockIdx.x] = t;
+    }
+}
+__global__ __launch_bounds__(RED_BS) void k_xref_final(const double* part, int G, int n, double* xref) {
+    __shared__ double sh[16];
+    for (int k = 0; k < 5; ++k) {
+        double t = reduce_partials(part + k * G, G, sh);
+        if (threadIdx.x == 0) xref[k] = t / n;
+    }
+}
+// normFactor partials: sum |wA - pA| + |b - pA| (5-vector magnitudes)
+__global__ __launch_bounds__(RED_BS) void k_norm_partial(int n, const double* wA, const double* pA, const double* b,
+                                                          double* part) {
+    __shared__ double sh[16];
+    double s = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double a = 0, c = 0;
+        for (int k = 0; k < 5; ++k) {
+            double d1 = wA[5 * i + k] - pA[5 * i + k], d2 = b[5 * i + k] - pA[5 * i + k];
+            a += d1 * d1;
+            c += d2 * d2;
+        }
+        s += sqrt(a) + sqrt(c);
+    }
+    double t = block_sum(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+__global__ __launch_bounds__(RED_BS) void k_norm_final(const double* part, int G, double* norm) {
+    __shared__ double sh[16];
+    double t = reduce_partials(part, G, sh);
+    if (threadIdx.x == 0) norm[0] = t + 1.0e-20;  // + BlockLduMatrix::small_
+}
+
+// ---------------------------------------------------------------- vectors
+__global__ void k_fill_xref(int n, const double* xref, double* x) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) for (int k = 0; k < 5; ++k) x[5 * i + k] = xref[k];
+}
+template <class TY, class TX>
+__global__ void k_copy(long n, TY* y, const TX* x) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] = (TY)x[i];
+}
+__global__ void k_sub(long n, double* r, const double* b, const double* a) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) r[i] = b[i] - a[i];
+}
+__global__ void k_add(long n, double* y, const double* a) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] += a[i];
+}
+// y = x / s[0]
+template <class T>
+__global__ void k_scale_div(long n, T* y, const T* x, const double* s) {
+    const double d = s[0];
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        y[i] = (T)((double)x[i] / d);
+}
+// fused MGS step: h = reduce(partIn); w -= h Vj; partial dot(w, Vnext) -> partOut; block 0 stores h
+template <class T>
+__global__ __launch_bounds__(RED_BS) void k_mgs_step(long n, T* w, const T* Vj, const T* Vnext, const double* partIn,
+                                                      int G, double* partOut, double* hOut) {
+    __shared__ double sh[16];
+    const double h = reduce_partials(partIn, G, sh);
+    if (blockIdx.x == 0 && threadIdx.x == 0) hOut[0] = h;
+    double s = 0.0;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        T wi = (T)((double)w[i] - h * (double)Vj[i]);
+        w[i] = wi;
+        s += (double)wi * (double)Vnext[i];
+    }
+    double t = block_sum(s, sh);
+    if (threadIdx.x == 0) partOut[blockIdx.x] = t;
+}
+// x += sum_i yh[i] V_i
+template <class TV, class TX>
+__global__ void k_update_x(long n, TX* x, const TV* V, const double* yh, int m) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        double xi = (double)x[i];
+        for (int k = 0; k < m; ++k) xi += yh[k] * (double)V[(size_t)k * n + i];
+        x[i] = (TX)xi;
+    }
+}
+
+// ---------------------------------------------------------------- GMRES scalar work (one thread, FP64)
+__device__ inline void givens(double h, double beta, double& c, double& s) {
+    if (beta == 0) { c = 1; s = 0; }
+    else if (fabs(beta) > fabs(h)) { double tau = -h / beta; s = 1.0 / sqrt(1.0 + tau * tau); c = s * tau; }
+    else { double tau = -beta / h; c = 1.0 / sqrt(1.0 + tau * tau); s = c * tau; }
+}
+// beta = sqrt(reduce(part)); rotate column i, new rotation, update bh
+__global__ __launch_bounds__(RED_BS) void k_gmres_column(double* sc, const double* part, int G, int i, int m) {
+    __shared__ double sh[16];
+    const double beta = sqrt(reduce_partials(part, G, sh));
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double* H = sc + S_H;
+    double* cs = H + m * m;
+    double* sn = cs + m;
+    double* bh = sn + m;
+    for (int j = 0; j < i; ++j) {
+        const double Hji = H[j * m + i];
+        H[j * m + i] = cs[j] * Hji - sn[j] * H[(j + 1) * m + i];
+        H[(j + 1) * m + i] = sn[j] * Hji + cs[j] * H[(j + 1) * m + i];
+    }
+    givens(H[i * m + i], beta, cs[i], sn[i]);
+    const double bhi = bh[i];
+    bh[i] = cs[i] * bhi - sn[i] * bh[i + 1];
+    bh[i + 1] = sn[i] * bhi + cs[i] * bh[i + 1];
+    H[i * m + i] = cs[i] * H[i * m + i] - sn[i] * beta;
+}
+__global__ void k_gmres_start(double* sc, int m) {
+    double* H = sc + S_H;
+    double* bh = H + m * m + 2 * m;
+    for (int k = 0; k <= m; ++k) bh[k] = 0.0;
+    bh[0] = sc[S_BETA];
+}
+__global__ void k_gmres_backsub(double* sc, int m) {
+    double* H = sc + S_H;
+    double* bh = H + m * m + 2 * m;
+    double* yh = bh + m + 1;
+    for (int i = m - 1; i >= 0; --i) {
+        double s = bh[i];
+        for (int j = i + 1; j < m; ++j) s -= H[i * m + j] * yh[j];
+        yh[i] = s / H[i * m + i];
+    }
+}
+
+// ---------------------------------------------------------------- block SpMV
+template <class TA, class TX, class TY>
+__global__ __launch_bounds__(256) void k_bspmv(int n, const int32_t* rowPtr, const int32_t* cols, const TA* A, const TX* x,
+                                               TY* y) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int p = rowPtr[i]; p < rowPtr[i + 1]; ++p) {
+        const TA* B = A + 25 * (size_t)p;
+        const TX* xv = x + 5 * (size_t)cols[p];
+        double x0 = xv[0], x1 = xv[1], x2 = xv[2], x3 = xv[3], x4 = xv[4];
+        for (int r = 0; r < 5; ++r)
+            acc[r] += (double)B[5 * r] * x0 + (double)B[5 * r + 1] * x1 + (double)B[5 * r + 2] * x2 +
+                      (double)B[5 * r + 3] * x3 + (double)B[5 * r + 4] * x4;
+    }
+    for (int r = 0; r < 5; ++r) y[5 * (size_t)i + r] = (TY)acc[r];
+}
+
+// ---------------------------------------------------------------- block ILU(0)
+template <class T>
+__device__ inline void mm5(const T* A, const T* B, T* C) {
+    for (int i = 0; i < 5; ++i)
+        for (int j = 0; j < 5; ++j) {
+            T s = 0;
+            for (int k = 0; k < 5; ++k) s += A[5 * i + k] * B[5 * k + j];
+            C[5 * i + j] = s;
+        }
+}
+// Gauss-Jordan inverse with partial pivoting
+template <class T>
+__device__ inline bool inv5(const T* A, T* Ainv) {
+    T a[5][10];
+    for (int i = 0; i < 5; ++i)
+        for (int j = 0; j < 5; ++j) { a[i][j] = A[5 * i + j]; a[i][5 + j] = (i == j) ? T(1) : T(0); }
+    for (int col = 0; col < 5; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < 5; ++r) if (fabs((double)a[r][col]) > fabs((double)a[piv][col])) piv = r;
+        if (a[piv][col] == T(0)) return false;
+        if (piv != col) for (int j = 0; j < 10; ++j) { T t = a[piv][j]; a[piv][j] = a[col][j]; a[col][j] = t; }
+        T d = T(1) / a[col][col];
+        for (int j = 0; j < 10; ++j) a[col][j] *= d;
+        for (int r = 0; r < 5; ++r) {
+            if (r == col) continue;
+            T f = a[r][col];
+            if (f == T(0)) continue;
+            for (int j = 0; j < 10; ++j) a[r][j] -= f * a[col][j];
+        }
+    }
+    for (int i = 0; i < 5; ++i) for (int j = 0; j < 5; ++j) Ainv[5 * i + j] = a[i][5 + j];
+    return true;
+}
+
+// factor the rows of one level (IKJ, row i reads only rows of earlier levels)
+template <class T>
+__global__ __launch_bounds__(64) void k_ilu_factor(int cnt, const int32_t* rows, const int32_t* rowPtr, const int32_t* cols,
+                                                    const int32_t* diagPos, T* LU, T* Dinv, int* fail) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    const int i = rows[t];
+    const int rb = rowPtr[i], re = rowPtr[i + 1], di = diagPos[i];
+    for (int p = rb; p < di; ++p) {
+        const int k = cols[p];
+        T L[25];
+        mm5(LU + 25 * (size_t)p, Dinv + 25 * (size_t)k, L);
+        for (int e = 0; e < 25; ++e) LU[25 * (size_t)p + e] = L[e];
+        // A_ij -= L_ik U_kj for j > k present in row i (both rows sorted)
+        int qi = p + 1;
+        for (int q = diagPos[k] + 1; q < rowPtr[k + 1]; ++q) {
+            const int j = cols[q];
+            while (qi < re && cols[qi] < j) ++qi;
+            if (qi >= re) break;
+            if (cols[qi] != j) continue;
+            T P[25];
+            mm5(L, LU + 25 * (size_t)q, P);
+            for (int e = 0; e < 25; ++e) LU[25 * (size_t)qi + e] -= P[e];
+        }
+    }
+    if (!inv5(LU + 25 * (size_t)di, Dinv + 25 * (size_t)i)) atomicExch(fail, 1);
+}
+
+// forward sweep of one level: y_i = r_i - sum_{k<i} L_ik y_k
+template <class TM, class TV>
+__global__ __launch_bounds__(64) void k_ilu_fwd(int cnt, const int32_t* rows, const int32_t* rowPtr, const int32_t* cols,
+                                                const int32_t* diagPos, const TM* LU, const TV* r, TV* y) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    const int i = rows[t];
+    TV s[5];
+    for (int k = 0; k < 5; ++k) s[k] = r[5 * (size_t)i + k];
+    for (int p = rowPtr[i]; p < diagPos[i]; ++p) {
+        const TM* B = LU + 25 * (size_t)p;
+        const TV* yv = y + 5 * (size_t)cols[p];
+        TV y0 = yv[0], y1 = yv[1], y2 = yv[2], y3 = yv[3], y4 = yv[4];
+        for (int rr = 0; rr < 5; ++rr)
+            s[rr] -= (TV)(B[5 * rr] * y0 + B[5 * rr + 1] * y1 + B[5 * rr + 2] * y2 + B[5 * rr + 3] * y3 + B[5 * rr + 4] * y4);
+    }
+    for (int k = 0; k < 5; ++k) y[5 * (size_t)i + k] = s[k];
+}
+// backward sweep of one level: x_i = Dinv_i (y_i - sum_{j>i} U_ij x_j)
+template <class TM, class TV>
+__global__ __launch_bounds__(64) void k_ilu_bwd(int cnt, const int32_t* rows, const int32_t* rowPtr, const int32_t* cols,
+                                                const int32_t* diagPos, const TM* LU, const TM* Dinv, const TV* y, TV* x) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    const int i = rows[t];
+    TV s[5];
+    for (int k = 0; k < 5; ++k) s[k] = y[5 * (size_t)i + k];
+    for (int p = diagPos[i] + 1; p < rowPtr[i + 1]; ++p) {
+        const TM* B = LU + 25 * (size_t)p;
+        const TV* xv = x + 5 * (size_t)cols[p];
+        TV x0 = xv[0], x1 = xv[1], x2 = xv[2], x3 = xv[3], x4 = xv[4];
+        for (int rr = 0; rr < 5; ++rr)
+            s[rr] -= (TV)(B[5 * rr] * x0 + B[5 * rr + 1] * x1 + B[5 * rr + 2] * x2 + B[5 * rr + 3] * x3 + B[5 * rr + 4] * x4);
+    }
+    const TM* D = Dinv + 25 * (size_t)i;
+    for (int rr = 0; rr < 5; ++rr)
+        x[5 * (size_t)i + rr] = (TV)(D[5 * rr] * s[0] + D[5 * rr + 1] * s[1] + D[5 * rr + 2] * s[2] + D[5 * rr + 3] * s[3] +
+                                     D[5 * rr + 4] * s[4]);
+}
+// diagonal (block-Jacobi) preconditioner
+template <class TM, class TV>
+__global__ void k_diag_apply(int n, const TM* Dinv, const TV* r, TV* x) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const TM* D = Dinv + 25 * (size_t)i;
+    for (int rr = 0; rr < 5; ++rr) {
+        double s = 0;
+        for (int k = 0; k < 5; ++k) s += (double)D[5 * rr + k] * (double)r[5 * (size_t)i + k];
+        x[5 * (size_t)i + rr] = (TV)s;
+    }
+}
+template <class T>
+__global__ void k_diag_inv(int n, const int32_t* diagPos, const T* A, T* Dinv, int* fail) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (!inv5(A + 25 * (size_t)diagPos[i], Dinv + 25 * (size_t)i)) atomicExch(fail, 1);
+}
+
+// BiCGStab vector updates
+__global__ void k_bicg_p(long n, double* p, const double* r, const double* v, const double* sc) {
+    const double beta = sc[0], omega = sc[1];
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        p[i] = r[i] + beta * (p[i] - omega * v[i]);
+}
+__global__ void k_axpy_s(long n, double* y, const double* x, const double* s, double sign) {
+    const double a = sign * s[0];
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] += a * x[i];
+}
+__global__ void k_bicg_x(long n, double* x, const double* ph, const double* th, const double* sc) {
+    const double alpha = sc[0], omega = sc[1];
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        x[i] += alpha * ph[i] + omega * th[i];
+}
+// BiCGStab scalar updates, single thread: mode 0: rho/beta, 1: alpha, 2: omega
+__global__ void k_bicg_scalars(double* sc, int mode) {
+    // sc: [0]=rho [1]=rhoOld [2]=alpha [3]=omega [4]=beta [5]=dot1 [6]=dot2 [8]=(beta,omega) [10]=(alpha,omega)
+    if (mode == 0) {
+        sc[1] = sc[0];
+        sc[0] = sc[5];
+        sc[4] = sc[0] / sc[1] * (sc[2] / sc[3]);
+        sc[8] = sc[4]; sc[9] = sc[3];
+    } else if (mode == 1) {
+        sc[2] = sc[0] / sc[5];
+        sc[12] = sc[2];
+    } else {
+        sc[3] = sc[5] / sc[6];
+        sc[10] = sc[2]; sc[11] = sc[3];
+        sc[13] = sc[3];
+    }
+}
+
+}  // namespace
+
+// ======================================================================= host side
+
+static int grid_for(long n) { return std::max(1, std::min(RED_MAXG, blocks_for(n, RED_BS))); }
+
+static void ensure_work(Case& c) {
+    Mesh& m = *c.mesh;
+    const bool mixed = c.st.precision == RDST_PREC_MIXED;
+    SolverWork* w = c.work;
+    if (w && w->n == m.nC && w->nDirs == c.st.n_directions && w->mixed == mixed) return;
+    if (w) delete w;
+    w = new SolverWork();
+    c.work = w;
+    w->n = m.nC;
+    w->n5 = 5 * m.nC;
+    w->nDirs = std::max(1, c.st.n_directions);
+    w->mixed = mixed;
+    w->G = grid_for(w->n5);
+    const size_t n5 = w->n5, nn = (size_t)m.nnzb * 25;
+    if (mixed) {
+        w->LU32.alloc(nn);
+        w->Dinv32.alloc(25 * (size_t)m.nC);
+        w->V32.alloc((size_t)(w->nDirs + 1) * n5);
+        w->w32.alloc(n5);
+        w->r32.alloc(n5);
+        w->t32.alloc(n5);
+    } else {
+        w->LU.alloc(nn);
+        w->Dinv.alloc(25 * (size_t)m.nC);
+        w->V.alloc((size_t)(w->nDirs + 1) * n5);
+    }
+    w->b.alloc(n5);
+    w->r.alloc(n5);
+    w->wA.alloc(n5);
+    w->tmp.alloc(n5);
+    w->x0.alloc(n5);
+    const int m2 = w->nDirs;
+    w->scal.alloc(S_H + m2 * m2 + 4 * m2 + 8);
+    w->hscal.assign(w->scal.n, 0.0);
+    w->partA.alloc(5 * RED_MAXG);
+    w->partB.alloc(5 * RED_MAXG);
+}
+
+void solver_free(Case& c) {
+    delete c.work;
+    c.work = nullptr;
+}
+
+namespace {
+
+struct Ctx {
+    Case& c;
+    Mesh& m;
+    SolverWork& w;
+    hipStream_t s;
+    int G;
+};
+
+// y = A x (FP64 matrix)
+template <class TA, class TX, class TY>
+void spmv(Ctx& k, const TA* A, const TX* x, TY* y) {
+    hipLaunchKernelGGL((k_bspmv<TA, TX, TY>), dim3(blocks_for(k.m.nC, 256)), dim3(256), 0, k.s, k.m.nC, k.m.d_rowPtr.p,
+                       k.m.d_cols.p, A, x, y);
+}
+
+template <class T>
+void ilu_factor(Ctx& k, const T* A, T* LU, T* Dinv) {
+    Mesh& m = k.m;
+    RDST_HIP(hipMemcpyAsync(LU, A, sizeof(T) * 25 * (size_t)m.nnzb, hipMemcpyDeviceToDevice, k.s));
+    int* fail = (int*)(k.w.partB.p);  // scratch int
+    RDST_HIP(hipMemsetAsync(fail, 0, sizeof(int), k.s));
+    if (k.c.st.precond == RDST_PRECOND_ILU0) {
+        for (int L = 0; L < m.nLevL; ++L) {
+            int cnt = m.levLPtr[L + 1] - m.levLPtr[L];
+            hipLaunchKernelGGL((k_ilu_factor<T>), dim3(blocks_for(cnt, 64)), dim3(64), 0, k.s, cnt,
+                               m.d_levLRows.p + m.levLPtr[L], m.d_rowPtr.p, m.d_cols.p, m.d_slotDiag.p, LU, Dinv, fail);
+        }
+    } else if (k.c.st.precond == RDST_PRECOND_DIAG) {
+        hipLaunchKernelGGL((k_diag_inv<T>), dim3(blocks_for(m.nC, 256)), dim3(256), 0, k.s, m.nC, m.d_slotDiag.p, A, Dinv,
+                           fail);
+    }
+    RDST_HIP(hipGetLastError());
+    int hf = 0;
+    RDST_HIP(hipMemcpyAsync(&hf, fail, sizeof(int), hipMemcpyDeviceToHost, k.s));
+    RDST_HIP(hipStreamSynchronize(k.s));
+    if (hf) throw ArgError("block ILU(0): singular pivot block", RDST_ERR_STATE);
+}
+
+// x = M^{-1} r
+template <class TM, class TV>
+void precondition(Ctx& k, const TM* LU, const TM* Dinv, const TV* r, TV* x, TV* ytmp) {
+    Mesh& m = k.m;
+    const int pc = k.c.st.precond;
+    if (pc == RDST_PRECOND_NONE) {
+        RDST_HIP(hipMemcpyAsync(x, r, sizeof(TV) * 5 * (size_t)m.nC, hipMemcpyDeviceToDevice, k.s));
+        return;
+    }
+    if (pc == RDST_PRECOND_DIAG) {
+        hipLaunchKernelGGL((k_diag_apply<TM, TV>), dim3(blocks_for(m.nC, 256)), dim3(256), 0, k.s, m.nC, Dinv, r, x);
+        return;
+    }
+    for (int L = 0; L < m.nLevL; ++L) {
+        int cnt = m.levLPtr[L + 1] - m.levLPtr[L];
+        hipLaunchKernelGGL((k_ilu_fwd<TM, TV>), dim3(blocks_for(cnt, 64)), dim3(64), 0, k.s, cnt,
+                           m.d_levLRows.p + m.levLPtr[L], m.d_rowPtr.p, m.d_cols.p, m.d_slotDiag.p, LU, r, ytmp);
+    }
+    for (int L = 0; L < m.nLevU; ++L) {
+        int cnt = m.levUPtr[L + 1] - m.levUPtr[L];
+        hipLaunchKernelGGL((k_ilu_bwd<TM, TV>), dim3(blocks_for(cnt, 64)), dim3(64), 0, k.s, cnt,
+                           m.d_levURows.p + m.levUPtr[L], m.d_rowPtr.p, m.d_cols.p, m.d_slotDiag.p, LU, Dinv, ytmp, x);
+    }
+}
+
+template <class TA, class TB>
+void dot(Ctx& k, long n, const TA* a, const TB* b, double* out, int op) {
+    hipLaunchKernelGGL((k_dot_partial<TA, TB>), dim3(k.G), dim3(RED_BS), 0, k.s, n, a, b, k.w.partA.p);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(RED_BS), 0, k.s, k.w.partA.p, k.G, out, op);
+}
+
+void cmptmag(Ctx& k, const double* r, double* out /*5, device*/, const double* norm) {
+    const int g = std::max(1, std::min(RED_MAXG, blocks_for(k.m.nC, RED_BS)));
+    hipLaunchKernelGGL(k_cmptmag_partial, dim3(g), dim3(RED_BS), 0, k.s, k.m.nC, r, k.w.partA.p);
+    hipLaunchKernelGGL(k_cmptmag_final, dim3(1), dim3(RED_BS), 0, k.s, k.w.partA.p, g, norm, out);
+}
+
+// BlockLduSolver::normFactor: sum(|A x - A xRef| + |b - A xRef|) + small
+template <class TA>
+void norm_factor(Ctx& k, const TA* A, const double* x, const double* b) {
+    const int n = k.m.nC;
+    const int g = std::max(1, std::min(RED_MAXG, blocks_for(n, RED_BS)));
+    double* sc = k.w.scal.p;
+    hipLaunchKernelGGL(k_cmptsum_partial, dim3(g), dim3(RED_BS), 0, k.s, n, x, k.w.partA.p);
+    hipLaunchKernelGGL(k_xref_final, dim3(1), dim3(RED_BS), 0, k.s, k.w.partA.p, g, n, sc + S_XREF);
+    spmv(k, A, x, k.w.wA.p);
+    hipLaunchKernelGGL(k_fill_xref, dim3(blocks_for(n, 256)), dim3(256), 0, k.s, n, sc + S_XREF, k.w.tmp.p);
+    spmv(k, A, k.w.tmp.p, k.w.r.p);  // pA in r
+    hipLaunchKernelGGL(k_norm_partial, dim3(g), dim3(RED_BS), 0, k.s, n, k.w.wA.p, k.w.r.p, b, k.w.partA.p);
+    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(RED_BS), 0, k.s, k.w.partA.p, g, sc + S_NORM);
+}
+
+bool converged(const rdst_settings& st, rdst_solver_perf& p) {
+    auto mag5 = [](const double* v) {
+        double s = 0;
+        for (int k = 0; k < 5; ++k) s += v[k] * v[k];
+        return std::sqrt(s);
+    };
+    bool conv = false;
+    if (p.n_iterations >= st.min_iter) {
+        double fm = mag5(p.final_residual), im = mag5(p.initial_residual);
+        bool chk = (fm < st.tolerance) || (st.rel_tol > 1e-15 && fm <= st.rel_tol * im);
+        p.converged = chk;
+        conv = p.n_iterations >= st.max_iter || chk;
+    }
+    return conv;
+}
+
+void read_scalars(Ctx& k, int from, int count, double* dst) {
+    RDST_HIP(hipMemcpyAsync(dst, k.w.scal.p + from, sizeof(double) * count, hipMemcpyDeviceToHost, k.s));
+    RDST_HIP(hipStreamSynchronize(k.s));
+}
+
+// one left-preconditioned GMRES(m) restart cycle on A z = rhs from z
+// (z updated in place); TV/TM = basis/matrix type
+template <class TM, class TV>
+void gmres_cycle(Ctx& k, const TM* A, const TM* LU, const TM* Dinv, TV* z, const TV* rres, TV* wv, TV* rv, TV* V, TV* ytmp) {
+    const long n5 = k.w.n5;
+    const int m = k.w.nDirs;
+    const int G = k.G;
+    double* sc = k.w.scal.p;
+    double* H = sc + S_H;
+    double* yh = H + m * m + 3 * m + 1;
+    // wA = M^-1 rA; beta = |wA|
+    precondition(k, LU, Dinv, rres, wv, ytmp);
+    dot(k, n5, wv, wv, sc + S_BETA, 1);
+    hipLaunchKernelGGL(k_gmres_start, dim3(1), dim3(1), 0, k.s, sc, m);
+    for (int i = 0; i < m; ++i) {
+        TV* Vi = V + (size_t)i * n5;
+        hipLaunchKernelGGL((k_scale_div<TV>), dim3(G), dim3(RED_BS), 0, k.s, n5, Vi, wv, sc + S_BETA);
+        spmv(k, A, Vi, rv);
+        precondition(k, LU, Dinv, rv, wv, ytmp);
+        // partial dot(w, V0)
+        hipLaunchKernelGGL((k_dot_partial<TV, TV>), dim3(G), dim3(RED_BS), 0, k.s, n5, wv, V, k.w.partA.p);
+        double* pin = k.w.partA.p;
+        double* pout = k.w.partB.p;
+        for (int j = 0; j <= i; ++j) {
+            const TV* Vj = V + (size_t)j * n5;
+            const TV* Vn = (j < i) ? V + (size_t)(j + 1) * n5 : wv;
+            hipLaunchKernelGGL((k_mgs_step<TV>), dim3(G), dim3(RED_BS), 0, k.s, n5, wv, Vj, Vn, pin, G, pout, H + j * m + i);
+            std::swap(pin, pout);
+        }
+        // pin holds partials of |w|^2; rotations, and beta = |w| for the next V
+        hipLaunchKernelGGL(k_gmres_column, dim3(1), dim3(RED_BS), 0, k.s, sc, pin, G, i, m);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(RED_BS), 0, k.s, pin, G, sc + S_BETA, 1);
+    }
+    hipLaunchKernelGGL(k_gmres_backsub, dim3(1), dim3(1), 0, k.s, sc, m);
+    hipLaunchKernelGGL((k_update_x<TV, TV>), dim3(G), dim3(RED_BS), 0, k.s, n5, z, V, yh, m);
+}
+
+rdst_solver_perf gmres_fp64(Ctx& k, double* x, const double* b) {
+    SolverWork& w = k.w;
+    const long n5 = w.n5;
+    const double* A = k.c.A64.p;
+    rdst_solver_perf perf{};
+    norm_factor(k, A, x, b);
+    spmv(k, A, x, w.wA.p);
+    hipLaunchKernelGGL(k_sub, dim3(k.G), dim3(RED_BS), 0, k.s, n5, w.r.p, b, w.wA.p);
+    cmptmag(k, w.r.p, w.scal.p + S_INIT, w.scal.p + S_NORM);
+    read_scalars(k, S_INIT, 5, perf.initial_residual);
+    for (int q = 0; q < 5; ++q) perf.final_residual[q] = perf.initial_residual[q];
+    if (converged(k.c.st, perf)) return perf;
+    do {
+        gmres_cycle<double, double>(k, A, w.LU.p, w.Dinv.p, x, w.r.p, w.wA.p, w.tmp.p, w.V.p, w.x0.p);
+        spmv(k, A, x, w.wA.p);
+        hipLaunchKernelGGL(k_sub, dim3(k.G), dim3(RED_BS), 0, k.s, n5, w.r.p, b, w.wA.p);
+        cmptmag(k, w.r.p, w.scal.p + S_FIN, w.scal.p + S_NORM);
+        read_scalars(k, S_FIN, 5, perf.final_residual);
+        perf.n_iterations++;
+    } while (!converged(k.c.st, perf));
+    return perf;
+}
+
+// mixed precision: FP64 outer defect correction r = b - A x (A held in FP32,
+// promoted), inner FP32 GMRES cycle on A e = r with FP32 ILU and basis
+rdst_solver_perf gmres_mixed(Ctx& k, double* x, const double* b) {
+    SolverWork& w = k.w;
+    const long n5 = w.n5;
+    const float* A = k.c.A32.p;
+    rdst_solver_perf perf{};
+    norm_factor(k, A, x, b);
+    spmv(k, A, x, w.wA.p);
+    hipLaunchKernelGGL(k_sub, dim3(k.G), dim3(RED_BS), 0, k.s, n5, w.r.p, b, w.wA.p);
+    cmptmag(k, w.r.p, w.scal.p + S_INIT, w.scal.p + S_NORM);
+    read_scalars(k, S_INIT, 5, perf.initial_residual);
+    for (int q = 0; q < 5; ++q) perf.final_residual[q] = perf.initial_residual[q];
+    if (converged(k.c.st, perf)) return perf;
+    do {
+        hipLaunchKernelGGL((k_copy<float, double>), dim3(k.G), dim3(RED_BS), 0, k.s, n5, w.r32.p, w.r.p);
+        RDST_HIP(hipMemsetAsync(w.t32.p, 0, sizeof(float) * n5, k.s));
+        // correction e (t32) from zero
+        gmres_cycle<float, float>(k, A, w.LU32.p, w.Dinv32.p, w.t32.p, w.r32.p, w.w32.p, w.r32.p, w.V32.p,
+                                  (float*)w.tmp.p);
+        // x += e in FP64
+        hipLaunchKernelGGL((k_copy<double, float>), dim3(k.G), dim3(RED_BS), 0, k.s, n5, w.wA.p, w.t32.p);
+        hipLaunchKernelGGL(k_add, dim3(k.G), dim3(RED_BS), 0, k.s, n5, x, w.wA.p);
+        spmv(k, A, x, w.wA.p);
+        hipLaunchKernelGGL(k_sub, dim3(k.G), dim3(RED_BS), 0, k.s, n5, w.r.p, b, w.wA.p);
+        cmptmag(k, w.r.p, w.scal.p + S_FIN, w.scal.p + S_NORM);
+        read_scalars(k, S_FIN, 5, perf.final_residual);
+        perf.n_iterations++;
+    } while (!converged(k.c.st, perf));
+    return perf;
+}
+
+// foam-extend BlockBiCGStabSolver (right-preconditioned), FP64
+rdst_solver_perf bicgstab_fp64(Ctx& k, double* x, const double* b) {
+    SolverWork& w = k.w;
+    const long n5 = w.n5;
+    const double* A = k.c.A64.p;
+    if (!w.p.p) { w.p.alloc(n5); w.ph.alloc(n5); w.v.alloc(n5); w.t.alloc(n5); w.th.alloc(n5); w.rw.alloc(n5); }
+    rdst_solver_perf perf{};
+    norm_factor(k, A, x, b);
+    spmv(k, A, x, w.p.p);
+    hipLaunchKernelGGL(k_sub, dim3(k.G), dim3(RED_BS), 0, k.s, n5, w.r.p, b, w.p.p);
+    cmptmag(k, w.r.p, w.scal.p + S_INIT, w.scal.p + S_NORM);
+    read_scalars(k, S_INIT, 5, perf.initial_residual);
+    for (int q = 0; q < 5; ++q) perf.final_residual[q] = perf.initial_residual[q];
+    if (converged(k.c.st, perf)) return perf;
+    double* bs = w.scal.p + S_TMP;  // rho, rhoOld, alpha, omega, beta, dot1, dot2, -, (beta,omega), (alpha,omega), alpha, omega
+    double init[14] = {1e20, 1e20, 0.0, 1e20, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    RDST_HIP(hipMemcpyAsync(bs, init, sizeof(init), hipMemcpyHostToDevice, k.s));
+    RDST_HIP(hipMemsetAsync(w.p.p, 0, sizeof(double) * n5, k.s));
+    RDST_HIP(hipMemsetAsync(w.v.p, 0, sizeof(double) * n5, k.s));
+    RDST_HIP(hipMemcpyAsync(w.rw.p, w.r.p, sizeof(double) * n5, hipMemcpyDeviceToDevice, k.s));
+    do {
+        dot(k, n5, w.rw.p, w.r.p, bs + 5, 0);
+        hipLaunchKernelGGL(k_bicg_scalars, dim3(1), dim3(1), 0, k.s, bs, 0);
+        hipLaunchKernelGGL(k_bicg_p, dim3(k.G), dim3(RED_BS), 0, k.s, n5, w.p.p, w.r.p, w.v.p, bs + 8);
+        precondition(k, (const double*)w.LU.p, (const double*)w.Dinv.p, (const double*)w.p.p, w.ph.p, w.x0.p);
+        spmv(k, A, w.ph.p, w.v.p);
+        dot(k, n5, w.rw.p, w.v.p, bs + 5, 0);
+        hipLaunchKernelGGL(k_bicg_scalars, dim3(1), dim3(1), 0, k.s, bs, 1);
+        hipLaunchKernelGGL(k_axpy_s, dim3(k.G), dim3(RED_BS), 0, k.s, n5, w.r.p, w.v.p, bs + 12, -1.0);
+        precondition(k, (const double*)w.LU.p, (const double*)w.Dinv.p, (const double*)w.r.p, w.th.p, w.x0.p);
+        spmv(k, A, w.th.p, w.t.p);
+        dot(k, n5, w.t.p, w.r.p, bs + 5, 0);
+        dot(k, n5, w.t.p, w.t.p, bs + 6, 0);
+        hipLaunchKernelGGL(k_bicg_scalars, dim3(1), dim3(1), 0, k.s, bs, 2);
+        hipLaunchKernelGGL(k_bicg_x, dim3(k.G), dim3(RED_BS), 0, k.s, n5, x, w.ph.p, w.th.p, bs + 10);
+        hipLaunchKernelGGL(k_axpy_s, dim3(k.G), dim3(RED_BS), 0, k.s, n5, w.r.p, w.t.p, bs + 13, -1.0);
+        cmptmag(k, w.r.p, w.scal.p + S_FIN, w.scal.p + S_NORM);
+        read_scalars(k, S_FIN, 5, perf.final_residual);
+        perf.n_iterations++;
+    } while (!converged(k.c.st, perf));
+    return perf;
+}
+
+}  // namespace
+
+void block_solve(Case& c, rdst_solver_perf* perfOut) {
+    if (!c.hasMatrix) throw ArgError("rdst_block_solve: no assembled matrix (call rdst_jacobian_assemble)", RDST_ERR_STATE);
+    ensure_work(c);
+    Mesh& m = *c.mesh;
+    SolverWork& w = *c.work;
+    Ctx k{c, m, w, c.stream, w.G};
+    const long n5 = w.n5;
+    const bool mixed = c.st.precision == RDST_PREC_MIXED;
+    if (mixed && c.st.solver != RDST_SOLVER_GMRES)
+        throw ArgError("mixed precision is provided for the GMRES solver", RDST_ERR_UNKNOWN_NAME);
+    // preconditioner factors
+    if (mixed) ilu_factor<float>(k, c.A32.p, w.LU32.p, w.Dinv32.p);
+    else ilu_factor<double>(k, c.A64.p, w.LU.p, w.Dinv.p);
+    rdst_solver_perf perf{};
+    double* W = c.f.W;
+    if (!mixed && c.st.dependent == RDST_DEP_W) {
+        // sourceW = A.W + s; solve A.W = sourceW from W (fvBlockMatrixDST.C:112-123)
+        spmv(k, (const double*)c.A64.p, (const double*)W, w.b.p);
+        hipLaunchKernelGGL(k_add, dim3(k.G), dim3(RED_BS), 0, k.s, n5, w.b.p, c.src.p);
+        perf = (c.st.solver == RDST_SOLVER_GMRES) ? gmres_fp64(k, W, w.b.p) : bicgstab_fp64(k, W, w.b.p);
+    } else {
+        // A.dW = s from 0, W += dW (deltaW mode; the mixed form always solves for the correction)
+        RDST_HIP(hipMemsetAsync(w.x0.p, 0, sizeof(double) * n5, k.s));
+        DBuf<double> dW;
+        dW.alloc(n5);
+        RDST_HIP(hipMemsetAsync(dW.p, 0, sizeof(double) * n5, k.s));
+        if (mixed) perf = gmres_mixed(k, dW.p, c.src.p);
+        else perf = (c.st.solver == RDST_SOLVER_GMRES) ? gmres_fp64(k, dW.p, c.src.p) : bicgstab_fp64(k, dW.p, c.src.p);
+        hipLaunchKernelGGL(k_add, dim3(k.G), dim3(RED_BS), 0, k.s, n5, W, dW.p);
+        RDST_HIP(hipStreamSynchronize(k.s));
+    }
+    RDST_HIP(hipGetLastError());
+    if (perfOut) *perfOut = perf;
+}
+
+}  // namespace rdst

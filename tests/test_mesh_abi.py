@@ -1,0 +1,154 @@
+"""Mesh addressing, geometry and the C-ABI surface (host-side, no GPU)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from rhodst_amd import _capi, cases, meshgen
+from rhodst_amd._abi import MeshDesc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+RAE = "/root/reference/tutorials/RAE2822"
+
+
+def test_forward_step_counts_and_order():
+    m = meshgen.forward_step()
+    assert m.n_cells == 4032 and m.n_internal == 7904
+    sizes = {p[0]: p[3] for p in m.patches}
+    assert sizes == {"inlet": 40, "outlet": 32, "bottom": 24, "top": 120, "obstacle": 104, "defaultFaces": 8064}
+    own, nei = m.owner[:m.n_internal], m.neighbour
+    assert np.all(own < nei)
+    key = own.astype(np.int64) * m.n_cells + nei
+    assert np.all(np.diff(key) > 0), "internal faces not in upper-triangular order"
+
+
+def test_geometry_closed_forms_and_oracle(oracle):
+    m = meshgen.forward_step()
+    g = oracle.geometry(m)
+    nI = m.n_internal
+    assert np.allclose(g["V"], 0.025 ** 2 * 0.1, rtol=1e-12)
+    assert np.isclose(g["V"].sum(), (3.0 - 2.4 * 0.2) * 0.1, rtol=1e-12)
+    assert np.allclose(g["magSf"], 0.0025, rtol=1e-12)
+    assert np.allclose(g["weights"][:nI], 0.5, rtol=1e-12)
+    assert np.allclose(g["deltaCoeffs"][:nI], 40.0, rtol=1e-12)
+    assert np.allclose(g["deltaCoeffs"][nI:], 80.0, rtol=1e-12)
+    dm = _capi.DeviceMesh(m)
+    for key, which in (("Sf", _capi.G_SF), ("magSf", _capi.G_MAGSF), ("C", _capi.G_C), ("V", _capi.G_V),
+                       ("weights", _capi.G_WEIGHTS), ("deltaCoeffs", _capi.G_DELTACOEFFS), ("Cf", _capi.G_CF)):
+        a = dm.get_geom(which)
+        assert np.array_equal(a, g[key]), key  # bit-identical fvMesh geometry
+
+
+def test_jittered_geometry_matches_oracle(oracle):
+    m = meshgen.box_2d(17, 11, jitter=0.2, seed=1)
+    g = oracle.geometry(m)
+    assert np.isclose(g["V"].sum(), 0.1, rtol=1e-12)
+    dm = _capi.DeviceMesh(m)
+    for key, which in (("Sf", _capi.G_SF), ("C", _capi.G_C), ("V", _capi.G_V), ("weights", _capi.G_WEIGHTS),
+                       ("deltaCoeffs", _capi.G_DELTACOEFFS)):
+        a = dm.get_geom(which)
+        assert np.array_equal(a, g[key]), key
+
+
+@pytest.mark.parametrize("mesh_fn", [lambda: meshgen.forward_step(), lambda: meshgen.box_2d(9, 7, jitter=0.2)])
+def test_bcsr_colouring_levels(mesh_fn):
+    m = mesh_fn()
+    dm = _capi.DeviceMesh(m)
+    nI, n = m.n_internal, m.n_cells
+    own, nei = m.owner[:nI], m.neighbour
+    assert np.array_equal(dm.get_int(_capi.M_OWNER)[:nI], own)
+    assert np.array_equal(dm.get_int(_capi.M_NEIGHBOUR), nei)
+    rp, cols = dm.get_int(_capi.M_ROWPTR), dm.get_int(_capi.M_COLS)
+    assert rp[-1] == n + 2 * nI == dm.info["nnzb"]
+    row_of = np.repeat(np.arange(n), np.diff(rp))
+    su, sl, sd = dm.get_int(_capi.M_SLOT_UPPER), dm.get_int(_capi.M_SLOT_LOWER), dm.get_int(_capi.M_SLOT_DIAG)
+    assert np.array_equal(row_of[su], own) and np.array_equal(cols[su], nei)
+    assert np.array_equal(row_of[sl], nei) and np.array_equal(cols[sl], own)
+    assert np.array_equal(row_of[sd], np.arange(n)) and np.array_equal(cols[sd], np.arange(n))
+    for i in range(n):
+        assert np.all(np.diff(cols[rp[i]:rp[i + 1]]) > 0)
+    # face colouring: no two faces of one colour share a cell
+    col = dm.get_int(_capi.M_FACE_COLOUR)
+    for c in range(col.max() + 1):
+        cells = np.concatenate([own[col == c], nei[col == c]])
+        assert len(np.unique(cells)) == len(cells)
+    # cell colouring is a distance-1 colouring
+    cc = dm.get_int(_capi.M_CELL_COLOUR)
+    assert np.all(cc[own] != cc[nei])
+    # ILU levels: every lower neighbour sits on an earlier level
+    lev = dm.get_int(_capi.M_ILU_LEVEL)
+    assert np.all(lev[own] < lev[nei])
+    assert dm.info["n_levels_l"] == lev.max() + 1
+
+
+@pytest.mark.skipif(not os.path.isdir(RAE), reason="reference tutorials not mounted")
+def test_read_rae2822_polymesh(oracle):
+    m = meshgen.read_polymesh(RAE)
+    assert (m.n_cells, m.n_faces, m.n_internal) == (23552, 94608, 46704)
+    assert [p[0] for p in m.patches] == ["frontAndBackPlanes", "freestreamBC", "airfoil"]
+    assert [p[1] for p in m.patches] == [meshgen.PATCH_EMPTY, meshgen.PATCH_GENERIC, meshgen.PATCH_WALL]
+    own, nei = m.owner[:m.n_internal], m.neighbour
+    assert np.all(own < nei)
+    key = own.astype(np.int64) * m.n_cells + nei
+    assert np.all(np.diff(key) > 0)
+    g = oracle.geometry(m)
+    assert np.all(g["V"] > 0)
+    dm = _capi.DeviceMesh(m)
+    assert np.array_equal(dm.get_geom(_capi.G_V), g["V"])
+
+
+def test_polymesh_write_read_roundtrip(tmp_path):
+    m = meshgen.box_2d(5, 4, jitter=0.1)
+    meshgen.write_polymesh(m, str(tmp_path))
+    r = meshgen.read_polymesh(str(tmp_path))
+    assert np.array_equal(r.owner, m.owner) and np.array_equal(r.neighbour, m.neighbour)
+    assert np.array_equal(r.face_points, m.face_points) and np.allclose(r.points, m.points, rtol=0, atol=0)
+    assert r.patches == m.patches
+
+
+def test_header_symbols_exported():
+    """The built library exports every entry point include/rhodst.h declares."""
+    with open(os.path.join(ROOT, "include", "rhodst.h")) as fh:
+        text = fh.read()
+    declared = set(re.findall(r"\b(rdst_[a-z_0-9]+)\s*\(", text))
+    assert declared == set(_capi.EXPORTS)
+    L = C.CDLL(_capi.LIB_PATH)
+    for name in declared:
+        assert hasattr(L, name), name
+
+
+def _gpu_visible():
+    import torch
+    return torch.cuda.device_count() > 0
+
+
+def test_no_device_is_an_error():
+    if _gpu_visible():
+        pytest.skip("a GPU is visible")
+    rc = _capi.lib().rdst_init(0)
+    assert rc == _capi.RDST_ERR_NO_DEVICE
+    assert b"no HIP device" in _capi.lib().rdst_last_error()
+
+
+def test_unknown_flux_scheme_lists_names():
+    """Like newbaseFluxDST.C:72-82, an unknown scheme fails and names the six."""
+    spec = cases.forward_step_case(nx=15, ny=5)
+    spec.settings.flux_scheme = 9
+    dm = _capi.DeviceMesh(spec.mesh)
+    h = C.c_void_p()
+    rc = _capi.lib().rdst_case_create(dm.h, C.byref(spec.thermo), spec.bcs, C.byref(spec.settings), C.byref(h))
+    assert rc == _capi.RDST_ERR_UNKNOWN_NAME
+    msg = _capi.lib().rdst_last_error().decode()
+    for name in ("AUSM+", "AUSM+up", "HLL", "HLLC", "HLLCP", "Kurganov"):
+        assert name in msg
+
+
+def test_bad_mesh_rejected():
+    m = meshgen.box_2d(3, 3)
+    md = MeshDesc(m)
+    md.neighbour[0] = 0  # owner >= neighbour
+    h = C.c_void_p()
+    rc = _capi.lib().rdst_mesh_create(C.byref(md.desc), C.byref(h))
+    assert rc == _capi.RDST_ERR_ARG
