@@ -1,0 +1,144 @@
+"""Benchmark of the block-coupled implicit step on one MI355X.
+
+Workload: the transonic swept wing (semi-span 1, 30 deg sweep, taper 0.56,
+10 % analytic section) on the 2 027 520-cell C-H hex mesh at Mach 0.84,
+AoA 3.06 deg, AUSM+up, Roe-Jacobian implicit step with local time stepping
+(CFL ramp to 20), GMRES(30) + block-ILU(0), characteristic far field.
+
+A "step" is one Single::iterate (face flux + residual, Jacobian assembly,
+linear solve, primitive/BC update, CFL/LTS) with every array resident in
+HBM.  The line reports the face-flux kernel throughput (Gfaces/s) as the
+value, the per-step time of the FP64 and mixed-precision paths, the roofline
+of the face-flux kernel (algorithmic bytes / HIP-event kernel time), and the
+CPU oracle timed on a bounded sample of the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--small]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def k1_bytes(n_cells: int, n_int: int, n_bnd: int) -> float:
+    """Algorithmic bytes of one face-flux + residual call, first-order
+    inviscid: every array read once, R written once; per internal face the
+    two int32 cell indices and Sf, per boundary face the cell index, Sf and
+    the boundary state, per cell W, 1/V and R."""
+    return n_int * (8 + 24) + n_bnd * (4 + 24 + 40) + n_cells * (40 + 8 + 40)
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def run_path(spec, steps: int, warmup: int):
+    from rhodst_amd import DeviceCase
+    dc = DeviceCase(spec)
+    if warmup:
+        dc.iterate(warmup)
+    t0 = time.perf_counter()
+    res, perfs = dc.iterate(steps, with_perf=True)
+    t1 = time.perf_counter()
+    return dc, (t1 - t0) * 1e3 / steps, res
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--small", action="store_true", help="a 0.26 M-cell mesh instead of 2.03 M")
+    ap.add_argument("--flux-calls", type=int, default=50)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every path here runs on a single MI355X; "
+              "multi-GPU runs are not part of this build", file=sys.stderr)
+        return 2
+
+    from rhodst_amd import cases, meshgen
+    mesh_kw = dict(n_wake=24, n_surf=64, n_normal=64, n_span_wing=24, n_span_out=8) if args.small else {}
+    t = time.perf_counter()
+    mesh = meshgen.swept_wing(**mesh_kw)
+    log(f"mesh {mesh.n_cells} cells, {mesh.n_internal} internal faces in {time.perf_counter() - t:.1f}s")
+    spec64 = cases.swept_wing_case(mesh=mesh)
+    spec32 = cases.swept_wing_case(mesh=mesh, precision="mixed")
+
+    log("fp64 path")
+    dc64, ms64, res64 = run_path(spec64, args.steps, args.warmup)
+    log(f"fp64 {ms64:.2f} ms/step, |R| {res64[0]:.4e} -> {res64[-1]:.4e}")
+    info = dc64.mesh.info
+    nC, nI, nB = info["n_cells"], info["n_internal"], info["n_bfaces"]
+    flux_ms = dc64.time_flux(args.flux_calls)
+    del dc64
+    log("mixed path")
+    dc32, ms32, res32 = run_path(spec32, args.steps, args.warmup)
+    log(f"mixed {ms32:.2f} ms/step, |R| {res32[0]:.4e} -> {res32[-1]:.4e}")
+    del dc32
+
+    gfaces = (nI + nB) / (flux_ms * 1e-3) / 1e9
+    bytes_k1 = k1_bytes(nC, nI, nB)
+    achieved = bytes_k1 / (flux_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_flux_residual<AUSM+up>", "kernel_ms": round(flux_ms, 5),
+                "algorithmic_bytes": int(bytes_k1)}
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline (oracle, 1 thread)")
+        from oracle import binding
+        oc = binding.OracleCase(spec64)
+        n_calls, t_cpu = 0, 0.0
+        while t_cpu < args.cpu_seconds and n_calls < 50:
+            t0 = time.perf_counter()
+            oc.flux_residual()
+            t_cpu += time.perf_counter() - t0
+            n_calls += 1
+        cpu = {"value": round((nI + nB) * n_calls / t_cpu / 1e9, 5), "unit": "Gfaces/s", "cores": 1,
+               "kind": "port",
+               "sample": f"{n_calls} face-flux + residual calls of the CPU FP64 oracle on the same "
+                         f"{nC}-cell mesh and uniform free-stream state ({t_cpu:.1f} s)"}
+
+    out = {
+        "metric": "time to 1e-8 residual drop + Gfaces/s flux, 2M-cell 3D SA, mixed vs FP64",
+        "value": round(gfaces, 3),
+        "unit": "Gfaces/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms32, 3),
+        "ms_per_step_fp64": round(ms64, 3),
+        "ms_per_step_mixed": round(ms32, 3),
+        "mixed_speedup": round(ms64 / ms32, 3),
+        "residual_drop_fp64": float(res64[-1] / res64[0]),
+        "residual_drop_mixed": float(res32[-1] / res32[0]),
+        "time_to_1e-8_s": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"swept wing C-H mesh {nC} cells ({nI} internal + {nB} boundary faces), "
+                               "M0.84 AoA3.06, AUSM+up first order, inviscid slip wall, LTS CFL ramp to 20, "
+                               "GMRES(30)+block-ILU(0), 1 restart cycle per step",
+                   "cells": nC, "faces": nI + nB, "flux_scheme": "AUSM+up"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -86,3 +86,35 @@ def box_farfield_case(nx: int = 24, ny: int = 16, scheme: str = "HLLC", mach: fl
     kw.update(settings)
     return CaseSpec("boxFarField", mesh, make_thermo(Cp=Cp, R=R, mu=0.0, Pr=0.72),
                     make_bcs(mesh, spec), make_settings(**kw), p0, T0, np.ascontiguousarray(U0))
+
+
+def swept_wing_case(scheme: str = "AUSM+up", mach: float = 0.84, aoa_deg: float = 3.06, viscous: bool = False,
+                    mesh=None, mesh_kw: dict | None = None, **settings) -> CaseSpec:
+    """Transonic swept wing: semi-span 1, 30 deg leading-edge sweep, taper
+    0.56, 10 % analytic section, C-H hex mesh (meshgen.swept_wing, 2.03 M
+    cells by default), Mach 0.84 at 3.06 deg, characteristic far field,
+    symmetry plane at the root.  Air (R 287, gamma 1.4, T 288.15 K,
+    p 101325 Pa), uniform free-stream start.  Inviscid: slip wing."""
+    mesh = mesh if mesh is not None else meshgen.swept_wing(**(mesh_kw or {}))
+    gamma, R = 1.4, 287.0
+    Cp = gamma * R / (gamma - 1)
+    Tinf, pinf = 288.15, 101325.0
+    c = np.sqrt(gamma * R * Tinf)
+    a = np.deg2rad(aoa_deg)
+    Uinf = (mach * c * np.cos(a), mach * c * np.sin(a), 0.0)
+    ff = {"p": ("farField", pinf), "T": ("farField", Tinf), "U": ("farField", Uinf),
+          "inf": {"p": pinf, "T": Tinf, "U": Uinf}}
+    wall = {"p": "zeroGradient", "T": "zeroGradient", "U": ("fixedValue", (0.0, 0.0, 0.0)) if viscous else "slip"}
+    spec = {"wing": wall, "symmetry": {"p": "symmetryPlane", "T": "symmetryPlane", "U": "symmetryPlane"},
+            "farfield": ff}
+    n = mesh.n_cells
+    p0 = np.full(n, pinf)
+    T0 = np.full(n, Tinf)
+    U0 = np.tile(np.array(Uinf), (n, 1))
+    kw = dict(flux_scheme=scheme, neighbouring=1, initial_co=1.0, min_co=0.5, max_co=20.0,
+              solver="GMRES", precond="ILUC0", n_directions=30, min_iter=1, max_iter=1,
+              tolerance=1e-15, rel_tol=1e-2, p_bounds=(100.0, 1e7), T_bounds=(50.0, 2000.0),
+              rho_bounds=(1e-3, 100.0))
+    kw.update(settings)
+    return CaseSpec("sweptWing", mesh, make_thermo(Cp=Cp, R=R, mu=0.0, Pr=0.72),
+                    make_bcs(mesh, spec), make_settings(**kw), p0, T0, np.ascontiguousarray(U0))

@@ -90,6 +90,21 @@ def hex_polymesh(points: np.ndarray, hexes: np.ndarray, classify, patch_defs) ->
     io = np.lexsort((nei_cell, own_cell))
     own_slot, own_cell, nei_cell = own_slot[io], own_cell[io], nei_cell[io]
     internal_faces = flat[own_slot]
+    # two cells sharing more than one face (a folded C-mesh row): merge the
+    # faces of the pair into one polygon (they share an edge, both are
+    # oriented outward from the owner)
+    dup = np.nonzero((own_cell[1:] == own_cell[:-1]) & (nei_cell[1:] == nei_cell[:-1]))[0]
+    merged_faces = {}
+    if len(dup):
+        for d0 in dup:
+            merged_faces[int(d0)] = _merge_polygons(list(map(int, internal_faces[d0])),
+                                                    list(map(int, internal_faces[d0 + 1])))
+        keep = np.ones(len(own_cell), dtype=bool)
+        keep[dup + 1] = False
+        new_index = np.cumsum(keep) - 1
+        merged_faces = {int(new_index[k]): v for k, v in merged_faces.items()}
+        internal_faces = internal_faces[keep]
+        own_cell, nei_cell = own_cell[keep], nei_cell[keep]
     # boundary faces: slots that appear once
     is_paired = np.zeros(flat.shape[0], dtype=bool)
     is_paired[a] = True
@@ -104,20 +119,48 @@ def hex_polymesh(points: np.ndarray, hexes: np.ndarray, classify, patch_defs) ->
         raise ValueError("unclassified boundary faces")
     ob = np.lexsort((blf, bcell, bpatch))
     bfaces, bcell, bpatch = bfaces[ob], bcell[ob], bpatch[ob]
-    all_faces = np.concatenate([internal_faces, bfaces], axis=0)
     owner = np.concatenate([own_cell, bcell]).astype(np.int32)
     patches = []
-    start = internal_faces.shape[0]
+    n_int = len(own_cell)
+    start = n_int
     for pi, (name, ptype) in enumerate(patch_defs):
         size = int(np.count_nonzero(bpatch == pi))
         patches.append((name, ptype, start, size))
         start += size
+    all_faces = np.concatenate([internal_faces, bfaces], axis=0)
     nF = all_faces.shape[0]
+    sizes = np.full(nF, 4, dtype=np.int64)
+    for k, v in merged_faces.items():
+        sizes[k] = len(v)
+    offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    fpts = np.empty(offsets[-1], dtype=np.int64)
+    regular = np.ones(nF, dtype=bool)
+    regular[list(merged_faces.keys())] = False
+    ridx = np.nonzero(regular)[0]
+    fpts[offsets[ridx][:, None] + np.arange(4)] = all_faces[ridx]
+    for k, v in merged_faces.items():
+        fpts[offsets[k]:offsets[k + 1]] = v
+    offsets = offsets.astype(np.int32)
+    fpts = fpts.astype(np.int32)
     return PolyMesh(points=np.ascontiguousarray(points, dtype=np.float64),
-                    face_offsets=(4 * np.arange(nF + 1)).astype(np.int32),
-                    face_points=all_faces.reshape(-1).astype(np.int32),
+                    face_offsets=offsets, face_points=fpts,
                     owner=owner, neighbour=nei_cell.astype(np.int32),
                     n_cells=nC, patches=patches)
+
+
+def _merge_polygons(a, b):
+    """Union of two consistently oriented polygons sharing one edge
+    (traversed u->v in a and v->u in b): the boundary of a from v back to u,
+    then b from u back to v."""
+    na, nb = len(a), len(b)
+    for i in range(na):
+        u, v = a[i], a[(i + 1) % na]
+        for j in range(nb):
+            if b[j] == v and b[(j + 1) % nb] == u:
+                pa = [a[(i + 1 + k) % na] for k in range(na)]      # v ... u
+                pb = [b[(j + 2 + k) % nb] for k in range(nb - 2)]  # after u, before v
+                return pa + pb
+    raise ValueError("faces of a duplicated cell pair do not share an edge")
 
 
 def _block_cells(nx, ny, x0, x1, y0, y1, zlo, zhi, point_id):
@@ -343,3 +386,137 @@ def write_polymesh(mesh: PolyMesh, out_dir: str) -> None:
             fh.write(f"    {name}\n    {{\n        type            {inv[t]};\n"
                      f"        nFaces          {size};\n        startFace       {start};\n    }}\n")
         fh.write(")\n")
+
+
+# --------------------------------------------------------------------------- 3-D swept wing
+def naca4_thickness(x: np.ndarray, t: float) -> np.ndarray:
+    """Half-thickness of the analytic symmetric 4-digit section with a closed
+    trailing edge (last coefficient -0.1036), x in [0, 1]."""
+    return 5 * t * (0.2969 * np.sqrt(np.clip(x, 0, 1)) - 0.1260 * x - 0.3516 * x ** 2 + 0.2843 * x ** 3
+                    - 0.1036 * x ** 4)
+
+
+def _stretch(n: int, first: float, total: float) -> np.ndarray:
+    """n+1 nodes on [0, total] with the first interval 'first', geometric growth
+    (ratio found by bisection)."""
+    if first * n >= total:
+        return np.linspace(0.0, total, n + 1)
+    lo, hi = 1.0 + 1e-12, 2.0
+    while (hi ** n - 1) / (hi - 1) * first < total:
+        hi *= 1.5
+    for _ in range(200):
+        r = 0.5 * (lo + hi)
+        if (r ** n - 1) / (r - 1) * first > total:
+            hi = r
+        else:
+            lo = r
+    r = 0.5 * (lo + hi)
+    d = first * r ** np.arange(n)
+    s = np.concatenate([[0.0], np.cumsum(d)])
+    return s * (total / s[-1])
+
+
+def swept_wing(n_wake: int = 48, n_surf: int = 128, n_normal: int = 96, n_span_wing: int = 44,
+               n_span_out: int = 16, thickness: float = 0.10, sweep_deg: float = 30.0, taper: float = 0.56,
+               semi_span: float = 1.0, root_chord: float = 1.0, far: float = 12.0, first_cell: float = 2.0e-6,
+               n_layers: int = 32, layer_ratio: float = 1.15) -> PolyMesh:
+    """Structured C-H hex mesh around a semi-span swept, tapered wing.
+
+    Each spanwise station is a C-mesh about the local section (analytic
+    symmetric 4-digit thickness, 10 %), wrapped from the downstream far field
+    under the wake, round the section and back over the wake; the wake cut is
+    closed by sharing points.  Beyond the tip the section collapses to zero
+    thickness, so the cells above and below the continued chord plane share
+    faces (internal) and the first station past the tip closes the wing with a
+    wedge cap.  The first n_layers cells off the wall grow at layer_ratio from
+    first_cell (a y+ of about 1 at Re 1.2e7), then stretch to the far field.
+    Cells are numbered i (round the C) fastest, then j (off the wall), then k
+    (span).  Patches: wing (wall), symmetry (symmetryPlane, z = 0),
+    farfield (patch).  Default size 352 x 96 x 60 = 2 027 520 cells.
+    """
+    ni = 2 * n_wake + 2 * n_surf
+    nj, nk = n_normal, n_span_wing + n_span_out
+    # chordwise distribution on the section (cosine clustering at LE and TE)
+    beta = np.linspace(0.0, np.pi, n_surf + 1)
+    xs = 0.5 * (1 - np.cos(beta))                   # 0 (LE) -> 1 (TE)
+    # wake distribution from TE to the downstream boundary
+    xw = 1.0 + _stretch(n_wake, xs[-1] - xs[-2], far)
+    # i index: 0..n_wake wake lower (far -> TE), then lower surface TE -> LE,
+    # upper surface LE -> TE, wake upper TE -> far
+    x_lower = np.concatenate([xw[::-1], xs[::-1][1:]])          # wake(far..TE), surface(TE..LE)
+    x_upper = np.concatenate([xs[1:], xw[1:]])                  # surface(LE..TE), wake(TE..far)
+    xc = np.concatenate([x_lower, x_upper])                     # ni+1 values
+    side = np.concatenate([-np.ones(len(x_lower)), np.ones(len(x_upper))])
+    on_surf = (xc <= 1.0 + 1e-12)
+    assert len(xc) == ni + 1
+    # wall-normal distribution: n_layers geometric layers, then stretched to far
+    d_layers = first_cell * layer_ratio ** np.arange(n_layers)
+    s_in = np.concatenate([[0.0], np.cumsum(d_layers)])
+    s_out = _stretch(nj - n_layers, d_layers[-1] * layer_ratio, 1.0)
+    t_norm = np.concatenate([s_in, s_in[-1] + s_out[1:] * (1.0 - s_in[-1])])  # 0..1, nj+1 values
+    # spanwise stations
+    zw = semi_span * (1 - np.cos(np.linspace(0, np.pi / 2, n_span_wing + 1)))   # cluster at the tip
+    zo = semi_span + _stretch(n_span_out, zw[-1] - zw[-2], far)[1:]
+    zk = np.concatenate([zw, zo])
+    tan_sw = np.tan(np.deg2rad(sweep_deg))
+    P = np.zeros((nk + 1, nj + 1, ni + 1, 3))
+    for k, z in enumerate(zk):
+        zz = min(z, semi_span)
+        chord = root_chord * (1 - (1 - taper) * zz / semi_span)
+        x_le = zz * tan_sw
+        th = 1.0 if z <= semi_span + 1e-12 else 0.0
+        # inner line: section surface (x in chord units) or the wake/plate line
+        xi = x_le + chord * np.minimum(xc, 1.0) + np.where(xc > 1.0, (xc - 1.0) * root_chord, 0.0)
+        yi = np.where(on_surf, side * th * chord * naca4_thickness(np.minimum(xc, 1.0), thickness), 0.0)
+        # outer line: a half circle of radius 'far' centred on the local trailing
+        # edge for the section points (psi = pi/2 xc from the upstream point to
+        # the top/bottom), straight lines y = +-far above and below the wake
+        x_te = x_le + chord
+        psi = 0.5 * np.pi * np.minimum(xc, 1.0)
+        xo = np.where(on_surf, x_te - far * np.cos(psi), xi)
+        yo = np.where(on_surf, side * far * np.sin(psi), side * far)
+        for j, t in enumerate(t_norm):
+            P[k, j, :, 0] = xi + t * (xo - xi)
+            P[k, j, :, 1] = yi + t * (yo - yi)
+            P[k, j, :, 2] = z
+    points = P.reshape(-1, 3)
+    nI1, nJ1 = ni + 1, nj + 1
+
+    def pid(i, j, k):
+        return i + nI1 * (j + nJ1 * k)
+
+    # merge the wake cut (j = 0, wake lower i <-> wake upper ni - i) and the
+    # chord plane beyond the tip (j = 0, lower surface i <-> upper surface ni - i)
+    remap = np.arange(points.shape[0], dtype=np.int64)
+    for k in range(nk + 1):
+        beyond = zk[k] > semi_span + 1e-12
+        for i in range(0, ni // 2 + 1):
+            mirror = ni - i
+            if mirror == i:
+                continue
+            is_wake = xc[i] >= 1.0 - 1e-12   # the closed trailing edge and the wake
+            if is_wake or beyond:
+                remap[pid(mirror, 0, k)] = pid(i, 0, k)
+    # the leading-edge point beyond the tip is shared already (i = ni/2)
+    ii, jj, kk = np.meshgrid(np.arange(ni), np.arange(nj), np.arange(nk), indexing="ij")
+    ii, jj, kk = ii.transpose(2, 1, 0).ravel(), jj.transpose(2, 1, 0).ravel(), kk.transpose(2, 1, 0).ravel()
+    hexes = np.stack([pid(ii, jj, kk), pid(ii + 1, jj, kk), pid(ii + 1, jj + 1, kk), pid(ii, jj + 1, kk),
+                      pid(ii, jj, kk + 1), pid(ii + 1, jj, kk + 1), pid(ii + 1, jj + 1, kk + 1),
+                      pid(ii, jj + 1, kk + 1)], axis=1)
+    hexes = remap[hexes]
+    # compact the point list
+    used = np.unique(hexes)
+    newid = np.full(points.shape[0], -1, dtype=np.int64)
+    newid[used] = np.arange(len(used))
+    hexes = newid[hexes]
+    points = points[used]
+
+    def classify(cells, lf, cen):
+        out = np.full(len(cells), 2)          # far field by default
+        out[lf == 2] = 0                      # j = 0 faces left unpaired: the wing
+        out[(lf == 4) & (np.abs(cen[:, 2]) < 1e-12)] = 1  # z = 0 symmetry plane
+        return out
+
+    mesh = hex_polymesh(points, hexes, classify,
+                        [("wing", PATCH_WALL), ("symmetry", PATCH_SYMMETRY), ("farfield", PATCH_GENERIC)])
+    return mesh
