@@ -85,10 +85,12 @@ enum rdst_dependent { RDST_DEP_W = 0, RDST_DEP_DELTAW = 1 };
 /* arithmetic of the linear solve: plain FP64, or FP32 Jacobian/ILU/Krylov
  * inside an FP64 defect-correction loop */
 enum rdst_precision { RDST_PREC_FP64 = 0, RDST_PREC_MIXED = 1 };
-/* ordering of the block-ILU(0): natural cell order (the reference's
- * ILUC0 factor, run level-scheduled) or multicolour (a different,
- * more parallel preconditioner) */
-enum rdst_ilu_order { RDST_ILU_NATURAL = 0, RDST_ILU_COLOURED = 1 };
+/* ordering of the block-ILU(0): natural cell order over the whole mesh
+ * (the reference's serial ILUC0 factor, run level-scheduled), or
+ * block-Jacobi ILU(0) over compact cell partitions of at most 1024 cells
+ * (what the reference's processor-local ILUC0 computes when the case is
+ * decomposed, one partition per rank) */
+enum rdst_ilu_order { RDST_ILU_NATURAL = 0, RDST_ILU_PARTITIONED = 1 };
 
 typedef struct rdst_patch {
     char name[64];
@@ -194,7 +196,8 @@ enum rdst_mesh_array {
     RDST_M_FACE_COLOUR = 7, /* [n_internal] */
     RDST_M_ILU_LEVEL = 8,   /* [n_cells] level of the lower-triangular sweep */
     RDST_M_BFACE_PATCH = 9, /* [n_bfaces] patch index of each non-empty boundary face */
-    RDST_M_CELL_COLOUR = 10 /* [n_cells] */
+    RDST_M_CELL_COLOUR = 10, /* [n_cells] */
+    RDST_M_PARTITION = 11   /* [n_cells] partition of each cell (partitioned ILU) */
 };
 
 /* double arrays of a mesh */
@@ -212,7 +215,8 @@ enum rdst_mesh_geom {
 enum rdst_mesh_info_idx {
     RDST_I_NCELLS = 0, RDST_I_NINTERNAL = 1, RDST_I_NBFACES = 2, RDST_I_NNZB = 3,
     RDST_I_NCOLOURS = 4, RDST_I_NLEVELS_L = 5, RDST_I_NLEVELS_U = 6, RDST_I_NCELL_COLOURS = 7,
-    RDST_I_COUNT = 8
+    RDST_I_NPARTS = 8, RDST_I_MAX_PART_LEVELS = 9,
+    RDST_I_COUNT = 10
 };
 
 typedef struct rdst_mesh rdst_mesh;

@@ -52,6 +52,7 @@ def lib():
         L.orc_set_matrix.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_double)]
         L.orc_case_get.argtypes = [C.c_void_p, C.c_int, P(C.c_double)]
         L.orc_case_put.argtypes = [C.c_void_p, C.c_int, P(C.c_double)]
+        L.orc_set_partition.argtypes = [C.c_void_p, P(C.c_int32)]
         _lib = L
     return _lib
 
@@ -194,6 +195,10 @@ class OracleCase:
         perf = rdst_solver_perf()
         _check(lib().orc_linear_solve(self.h, dptr(b), dptr(x), C.byref(perf)))
         return x, perf
+
+    def set_partition(self, part):
+        a = np.ascontiguousarray(part, dtype=np.int32)
+        _check(lib().orc_set_partition(self.h, a.ctypes.data_as(C.POINTER(C.c_int32))))
 
     def set_matrix(self, diag, upper, lower):
         d = np.ascontiguousarray(diag, dtype=np.float64)

@@ -94,6 +94,7 @@ struct Mesh {
     std::vector<int> bCell;   // faceCells
     std::vector<double> w;    // weights, size nInternal + nB
     std::vector<double> dc;   // deltaCoeffs
+    std::vector<V3> corr;     // non-orthogonal correction vectors, internal faces
     std::vector<std::vector<int>> cellCells;
 };
 
@@ -207,6 +208,13 @@ static void buildMesh(Mesh& m, const rdst_mesh_desc* d) {
         // stabilised form, comment at timeDST.C:237
         m.dc[f] = 1.0 / std::max(dot(unitArea, delta), 0.05 * mag(delta));
     }
+    // correction vectors n - d deltaCoeffs (boundary: 0 on non-coupled patches)
+    m.corr.assign(m.nInternal, v3(0, 0, 0));
+    for (int f = 0; f < m.nInternal; ++f) {
+        V3 delta = m.C[m.neighbour[f]] - m.C[m.owner[f]];
+        V3 unitArea = m.Sf[f] / m.magSf[f];
+        m.corr[f] = unitArea - delta * m.dc[f];
+    }
     for (int b = 0; b < m.nB; ++b) {
         int f = m.bFace[b];
         V3 delta = m.Cf[f] - m.C[m.bCell[b]];
@@ -226,8 +234,9 @@ static inline double magSfOf(const Mesh& m, int k) { return k < m.nInternal ? m.
 
 // ------------------------------------------------------------------ case
 struct BCState {
-    // per boundary face mixed-BC data (valueFraction) kept between steps
+    // per boundary face mixed-BC data (valueFraction, U refValue) kept between steps
     std::vector<double> vf_p, vf_T, vf_U;
+    std::vector<V3> refU;
 };
 
 struct Case {
@@ -261,6 +270,8 @@ struct Case {
     std::vector<double> rDeltaT, courant, rhoOld, eOld, dS;  // dS [n][3]
     double residual = -1, residualRatio = 1, courantScalar = 1;
     bool hasState = false, hasMatrix = false;
+    // cell partition for the block-Jacobi ILU(0) (empty: one partition)
+    std::vector<int> part;
 };
 
 // ------------------------------------------------------------------ thermo
@@ -317,6 +328,7 @@ static void initialBoundary(Case& c) {
     int nB = m.nB;
     c.bp.assign(nB, 0); c.bT.assign(nB, 0); c.bU.assign(nB, v3(0, 0, 0));
     c.bcs.vf_p.assign(nB, 0.0); c.bcs.vf_T.assign(nB, 0.0); c.bcs.vf_U.assign(nB, 0.0);
+    c.bcs.refU.assign(nB, v3(0, 0, 0));
     for (int b = 0; b < nB; ++b) {
         const rdst_bc& bc = c.bc[m.bPatch[b]];
         int ci = m.bCell[b];
@@ -741,7 +753,143 @@ static void computeHLLCPf(const Case& c, std::vector<double>& f) {
     linearInterp(m, f3, b1, f);
 }
 
-// computeFluxDST + calcFields (baseFluxDST.C:223-255), inviscid part
+
+// ------------------------------------------------------------------ viscous terms of calcFields
+// rank-2 tensor, T[3*i + j]
+struct T9 { double a[9]; };
+
+// snGrad() of the U boundary condition on boundary face b (fvPatchField
+// snGrad: fixedValue dc (U_b - U_I); zeroGradient 0; basicSymmetry
+// (transform(I - 2nn, U_I) - U_I) dc/2; mixed vf (ref - U_I) dc + (1-vf) 0)
+static V3 snGradU_b(const Case& c, int b) {
+    const Mesh& m = c.m;
+    const rdst_bc& bc = c.bc[m.bPatch[b]];
+    int f = m.bFace[b], ci = m.bCell[b];
+    double dcb = m.dc[m.nInternal + b];
+    V3 UI = c.U[ci];
+    switch (bc.U_type) {
+        case RDST_BC_FIXED_VALUE: return dcb * (c.bU[b] - UI);
+        case RDST_BC_SYMMETRY: {
+            V3 n = m.Sf[f] / m.magSf[f];
+            double nU = dot(n, UI);
+            V3 t = v3(UI.x - 2.0 * n.x * nU, UI.y - 2.0 * n.y * nU, UI.z - 2.0 * n.z * nU);
+            return (t - UI) * (dcb / 2.0);
+        }
+        case RDST_BC_INLET_OUTLET: case RDST_BC_FARFIELD: {
+            double vf = c.bcs.vf_U[b];
+            return vf * (c.bcs.refU[b] - UI) * dcb + (1.0 - vf) * v3(0, 0, 0);
+        }
+        default: return v3(0, 0, 0);
+    }
+}
+
+// fvc::grad(U) Gauss linear (cells) and the corrected boundary gradient
+// (gaussGrad::correctBoundaryConditions: g_b = g_I + n (snGrad_b - n.g_I))
+static void gradU(const Case& c, std::vector<T9>& g, std::vector<T9>& gb) {
+    const Mesh& m = c.m;
+    int n = m.nCells;
+    g.assign(n, T9{});
+    for (int f = 0; f < m.nInternal; ++f) {
+        int o = m.owner[f], nb = m.neighbour[f];
+        double w = m.w[f];
+        V3 P = c.U[o], N = c.U[nb];
+        V3 Uf = v3(w * (P.x - N.x) + N.x, w * (P.y - N.y) + N.y, w * (P.z - N.z) + N.z);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double t = cmp(m.Sf[f], i) * cmp(Uf, j);
+                g[o].a[3 * i + j] += t;
+                g[nb].a[3 * i + j] -= t;
+            }
+    }
+    for (int b = 0; b < m.nB; ++b) {
+        int ci = m.bCell[b];
+        V3 sf = m.Sf[m.bFace[b]];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) g[ci].a[3 * i + j] += cmp(sf, i) * cmp(c.bU[b], j);
+    }
+    for (int ci = 0; ci < n; ++ci)
+        for (int k = 0; k < 9; ++k) g[ci].a[k] /= m.V[ci];
+    gb.assign(m.nB, T9{});
+    for (int b = 0; b < m.nB; ++b) {
+        int f = m.bFace[b], ci = m.bCell[b];
+        V3 nrm = m.Sf[f] / m.magSf[f];
+        const T9& gi = g[ci];
+        V3 nG = v3(nrm.x * gi.a[0] + nrm.y * gi.a[3] + nrm.z * gi.a[6],
+                   nrm.x * gi.a[1] + nrm.y * gi.a[4] + nrm.z * gi.a[7],
+                   nrm.x * gi.a[2] + nrm.y * gi.a[5] + nrm.z * gi.a[8]);
+        V3 d = snGradU_b(c, b) - nG;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) gb[b].a[3 * i + j] = gi.a[3 * i + j] + cmp(nrm, i) * cmp(d, j);
+    }
+}
+
+// tauMC = muEff dev2(T(grad U)), dev2(A) = A - (2/3) tr(A) I
+static T9 tauMC(double mu, const T9& g) {
+    T9 t;
+    double tr = g.a[0] + g.a[4] + g.a[8];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double a = g.a[3 * j + i] - (i == j ? (2.0 / 3.0) * tr : 0.0);
+            t.a[3 * i + j] = mu * a;
+        }
+    return t;
+}
+static inline V3 dotVT(V3 v, const T9& t) {  // (v & T)_j = v_i T_ij
+    return v3(v.x * t.a[0] + v.y * t.a[3] + v.z * t.a[6], v.x * t.a[1] + v.y * t.a[4] + v.z * t.a[7],
+              v.x * t.a[2] + v.y * t.a[5] + v.z * t.a[8]);
+}
+
+// baseFluxDST.C:229-250: R1 -= laplacian(muEff, U) + div(tauMC);
+// R4 -= div((interp(muEff)|S| snGrad(U) + Sf & interp(tauMC)) & Up).
+// Gauss linear gradient, "corrected" snGrad with non-orthogonal correction
+// corr & interp(grad U); no heat-conduction term (none in the reference).
+static void viscousTerms(Case& c) {
+    const Mesh& m = c.m;
+    int n = m.nCells;
+    std::vector<T9> g, gb;
+    gradU(c, g, gb);
+    std::vector<double> muEff = c.mu, bmuEff = c.bmu;   // laminar: muEff = mu (+ mut when turbulent)
+    std::vector<T9> tau(n), btau(m.nB);
+    for (int i = 0; i < n; ++i) tau[i] = tauMC(muEff[i], g[i]);
+    for (int b = 0; b < m.nB; ++b) btau[b] = tauMC(bmuEff[b], gb[b]);
+    std::vector<V3> lap(n, v3(0, 0, 0)), dtau(n, v3(0, 0, 0));
+    std::vector<double> sdu(n, 0.0);
+    for (int f = 0; f < m.nInternal; ++f) {
+        int o = m.owner[f], nb = m.neighbour[f];
+        double w = m.w[f];
+        double muf = w * (muEff[o] - muEff[nb]) + muEff[nb];
+        T9 gf, tf;
+        for (int k = 0; k < 9; ++k) {
+            gf.a[k] = w * (g[o].a[k] - g[nb].a[k]) + g[nb].a[k];
+            tf.a[k] = w * (tau[o].a[k] - tau[nb].a[k]) + tau[nb].a[k];
+        }
+        V3 sn = m.dc[f] * (c.U[nb] - c.U[o]) + dotVT(m.corr[f], gf);
+        V3 lapF = (muf * m.magSf[f]) * sn;
+        V3 tauF = dotVT(m.Sf[f], tf);
+        double sd = dot(lapF + tauF, c.Up[f]);
+        lap[o] = lap[o] + lapF; lap[nb] = lap[nb] - lapF;
+        dtau[o] = dtau[o] + tauF; dtau[nb] = dtau[nb] - tauF;
+        sdu[o] += sd; sdu[nb] -= sd;
+    }
+    for (int b = 0; b < m.nB; ++b) {
+        int ci = m.bCell[b], f = m.bFace[b], k = m.nInternal + b;
+        V3 lapF = (bmuEff[b] * m.magSf[f]) * snGradU_b(c, b);
+        V3 tauF = dotVT(m.Sf[f], btau[b]);
+        double sd = dot(lapF + tauF, c.Up[k]);
+        lap[ci] = lap[ci] + lapF;
+        dtau[ci] = dtau[ci] + tauF;
+        sdu[ci] += sd;
+    }
+    for (int i = 0; i < n; ++i) {
+        double V = m.V[i];
+        V3 L = lap[i] / V, D = dtau[i] / V;
+        V3 s = L + D;
+        c.R[5 * i + 1] -= s.x; c.R[5 * i + 2] -= s.y; c.R[5 * i + 3] -= s.z;
+        c.R[5 * i + 4] -= sdu[i] / V;
+    }
+}
+
+// computeFluxDST + calcFields (baseFluxDST.C:223-255)
 static void computeFlux(Case& c) {
     const Mesh& m = c.m;
     CellAux a;
@@ -787,6 +935,7 @@ static void computeFlux(Case& c) {
         c.R[5 * i + 1] = d1[i].x / m.V[i]; c.R[5 * i + 2] = d1[i].y / m.V[i]; c.R[5 * i + 3] = d1[i].z / m.V[i];
         c.R[5 * i + 4] = d4[i] / m.V[i];
     }
+    if (c.st.viscous) viscousTerms(c);
 }
 
 // ------------------------------------------------------------------ Jacobian
@@ -1030,8 +1179,12 @@ static void buildILU(const Case& c, ILU& il) {
     il.n = n;
     std::vector<std::vector<std::pair<int, const double*>>> rows(n);
     for (int i = 0; i < n; ++i) rows[i].push_back({i, &c.diag[25 * i]});
+    const bool blockJacobi = c.st.ilu_order == RDST_ILU_PARTITIONED;
+    if (blockJacobi && (int)c.part.size() != n) throw std::runtime_error("partitioned ILU needs a cell partition");
     for (int f = 0; f < m.nInternal; ++f) {
         int o = m.owner[f], nb = m.neighbour[f];
+        // block-Jacobi: the factor is the ILU(0) of the partition-diagonal blocks
+        if (blockJacobi && c.part[o] != c.part[nb]) continue;
         rows[o].push_back({nb, &c.upper[25 * f]});   // owner row, neighbour column
         rows[nb].push_back({o, &c.lower[25 * f]});   // neighbour row, owner column
     }
@@ -1332,6 +1485,7 @@ static void updateFields(Case& c) {
                 double vf = 1.0 - fe_pos(phib);
                 c.bcs.vf_U[b] = vf;
                 V3 ref = v3(bc.U_value[0], bc.U_value[1], bc.U_value[2]);
+                c.bcs.refU[b] = ref;
                 c.bU[b] = vf * ref + (1.0 - vf) * UI;
                 break;
             }
@@ -1350,6 +1504,7 @@ static void updateFields(Case& c) {
                 V3 ref = supIn * Uinf + subIn * subsInValue + subOut * subsOutValue;
                 double vf = 1 - supOut - subOut;
                 c.bcs.vf_U[b] = vf;
+                c.bcs.refU[b] = ref;
                 c.bU[b] = vf * ref + (1.0 - vf) * UI;
                 break;
             }
@@ -1703,6 +1858,14 @@ int orc_set_matrix(void* h, const double* diag, const double* upper, const doubl
         c.lower.assign(lower, lower + 25 * (size_t)F);
         c.source.assign(5 * (size_t)n, 0.0);
         c.hasMatrix = true;
+    })
+}
+
+// cell partition of the block-Jacobi ILU(0) (settings.ilu_order = partitioned)
+int orc_set_partition(void* h, const int32_t* part) {
+    ORC_TRY({
+        Case& c = *(Case*)h;
+        c.part.assign(part, part + c.m.nCells);
     })
 }
 

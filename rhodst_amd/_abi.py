@@ -140,7 +140,8 @@ def make_settings(**kw) -> rdst_settings:
     s.rho_min, s.rho_max = kw.get("rho_bounds", (1e-3, 1e4))
     s.p_min, s.p_max = kw.get("p_bounds", (1e-3, 1e4))
     s.T_min, s.T_max = kw.get("T_bounds", (1e-3, 1e4))
-    s.ilu_order = kw.get("ilu_order", 0)
+    order = kw.get("ilu_order", "natural")
+    s.ilu_order = {"natural": 0, "partitioned": 1}[order] if isinstance(order, str) else int(order)
     s.viscous = int(kw.get("viscous", 0))
     return s
 

@@ -84,9 +84,10 @@ def init(device: int = 0):
         _initialised = True
 
 
-INFO_KEYS = ["n_cells", "n_internal", "n_bfaces", "nnzb", "n_colours", "n_levels_l", "n_levels_u", "n_cell_colours"]
+INFO_KEYS = ["n_cells", "n_internal", "n_bfaces", "nnzb", "n_colours", "n_levels_l", "n_levels_u", "n_cell_colours",
+             "n_parts", "max_part_levels"]
 M_OWNER, M_NEIGHBOUR, M_ROWPTR, M_COLS, M_SLOT_UPPER, M_SLOT_LOWER, M_SLOT_DIAG, M_FACE_COLOUR, M_ILU_LEVEL, \
-    M_BFACE_PATCH, M_CELL_COLOUR = range(11)
+    M_BFACE_PATCH, M_CELL_COLOUR, M_PARTITION = range(12)
 G_SF, G_MAGSF, G_CF, G_C, G_V, G_WEIGHTS, G_DELTACOEFFS = range(7)
 
 
@@ -99,7 +100,7 @@ class DeviceMesh:
         h = C.c_void_p()
         check(lib().rdst_mesh_create(C.byref(self._md.desc), C.byref(h)))
         self.h = h
-        info = (C.c_int64 * 8)()
+        info = (C.c_int64 * 10)()
         check(lib().rdst_mesh_info(self.h, info))
         self.info = dict(zip(INFO_KEYS, [int(v) for v in info]))
 
@@ -113,7 +114,7 @@ class DeviceMesh:
         sizes = {M_OWNER: i["n_internal"] + i["n_bfaces"], M_NEIGHBOUR: i["n_internal"], M_ROWPTR: i["n_cells"] + 1,
                  M_COLS: i["nnzb"], M_SLOT_UPPER: i["n_internal"], M_SLOT_LOWER: i["n_internal"],
                  M_SLOT_DIAG: i["n_cells"], M_FACE_COLOUR: i["n_internal"], M_ILU_LEVEL: i["n_cells"],
-                 M_BFACE_PATCH: i["n_bfaces"], M_CELL_COLOUR: i["n_cells"]}
+                 M_BFACE_PATCH: i["n_bfaces"], M_CELL_COLOUR: i["n_cells"], M_PARTITION: i["n_cells"]}
         out = np.zeros(sizes[which], dtype=np.int32)
         check(lib().rdst_mesh_get_int(self.h, which, iptr(out)))
         return out

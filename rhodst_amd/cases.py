@@ -118,3 +118,35 @@ def swept_wing_case(scheme: str = "AUSM+up", mach: float = 0.84, aoa_deg: float 
     kw.update(settings)
     return CaseSpec("sweptWing", mesh, make_thermo(Cp=Cp, R=R, mu=0.0, Pr=0.72),
                     make_bcs(mesh, spec), make_settings(**kw), p0, T0, np.ascontiguousarray(U0))
+
+
+def box_viscous_case(nx: int = 24, ny: int = 16, scheme: str = "HLLC", mach: float = 0.3, jitter: float = 0.15,
+                     seed: int = 1, perturb: float = 0.05, wall_stretch: bool = True, **settings) -> CaseSpec:
+    """Laminar flow over a no-slip wall: a jittered 2-D box (0.01 x 0.01 m)
+    with a wall at the bottom (U fixedValue 0, p and T zeroGradient) and the
+    characteristic far field on the other three sides; Sutherland air, seeded
+    perturbed state.  Exercises the viscous calcFields terms, the wall
+    boundary gradient and the viscous spectral radius."""
+    mesh = meshgen.box_2d(nx, ny, lx=0.01, ly=0.01, jitter=jitter, seed=seed)
+    gamma, R = 1.4, 287.0
+    Cp = gamma * R / (gamma - 1)
+    Tinf, pinf = 300.0, 1.0e5
+    c = np.sqrt(gamma * R * Tinf)
+    Uinf = (mach * c, 0.0, 0.0)
+    ff = {"p": ("farField", pinf), "T": ("farField", Tinf), "U": ("farField", Uinf),
+          "inf": {"p": pinf, "T": Tinf, "U": Uinf}}
+    wall = {"p": "zeroGradient", "T": "zeroGradient", "U": ("fixedValue", (0.0, 0.0, 0.0))}
+    spec = {"left": ff, "right": ff, "top": ff, "bottom": wall}
+    n = mesh.n_cells
+    rng = np.random.default_rng(2)
+    p0 = pinf * (1 + perturb * rng.uniform(-1, 1, n))
+    T0 = Tinf * (1 + perturb * rng.uniform(-1, 1, n))
+    U0 = np.tile(np.array(Uinf), (n, 1)) * (1 + perturb * rng.uniform(-1, 1, (n, 1)))
+    U0[:, 1] = 0.1 * Uinf[0] * rng.uniform(-1, 1, n)
+    U0[:, 2] = 0.0
+    kw = dict(flux_scheme=scheme, neighbouring=1, initial_co=2.0, min_co=0.5, max_co=20.0, viscous=1,
+              solver="GMRES", precond="ILUC0", n_directions=10, min_iter=1, max_iter=2,
+              tolerance=1e-15, rel_tol=1e-3, p_bounds=(1.0, 1e7), T_bounds=(1.0, 1e4), rho_bounds=(1e-4, 1e3))
+    kw.update(settings)
+    return CaseSpec("boxViscous", mesh, make_thermo(Cp=Cp, R=R, Pr=0.72, sutherland=(1.458e-6, 110.4)),
+                    make_bcs(mesh, spec), make_settings(**kw), p0, T0, np.ascontiguousarray(U0))

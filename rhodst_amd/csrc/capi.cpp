@@ -23,6 +23,7 @@ MeshDev Case::meshDev() const {
     d.Sf = m.d_Sf.p; d.magSf = m.d_magSf.p; d.w = m.d_w.p; d.dc = m.d_dc.p; d.V = m.d_V.p;
     d.ellCode = m.d_ellCode.p; d.ellFace = m.d_ellFace.p; d.ellSlot = m.d_ellSlot.p; d.slotDiag = m.d_slotDiag.p;
     d.ellSf = m.d_ellSf.p; d.ellMagSf = m.d_ellMagSf.p; d.ellWt = m.d_ellWt.p; d.ellDc = m.d_ellDc.p;
+    d.ellCorr = m.d_ellCorr.p;
     return d;
 }
 
@@ -30,7 +31,8 @@ static void alloc_fields(Case& c) {
     const Mesh& m = *c.mesh;
     const size_t n = m.nC, nB = std::max(m.nB, 1), nF = m.nF;
     c.cellBuf.alloc(14 * n);
-    c.bndBuf.alloc(18 * nB);
+    c.bndBuf.alloc(21 * nB);
+    c.gradBuf.alloc(9 * n + 9 * nB);
     c.faceBuf.alloc(8 * nF);
     c.RBuf.alloc(5 * n);
     c.WBuf.alloc(5 * n);
@@ -38,15 +40,19 @@ static void alloc_fields(Case& c) {
     c.fHBuf.alloc(n);
     c.src.alloc(5 * n);
     c.cellBuf.zero(); c.bndBuf.zero(); c.faceBuf.zero(); c.RBuf.zero(); c.WBuf.zero(); c.timeBuf.zero(); c.fHBuf.zero();
+    c.gradBuf.zero();
     RDST_HIP(hipDeviceSynchronize());
     Fields& f = c.f;
     double* p = c.cellBuf.p;
     double** cells[14] = {&f.rho, &f.rhoUx, &f.rhoUy, &f.rhoUz, &f.rhoE, &f.p, &f.T, &f.h, &f.psi, &f.mu, &f.alpha, &f.Ux, &f.Uy, &f.Uz};
     for (int i = 0; i < 14; ++i) *cells[i] = p + i * n;
     p = c.bndBuf.p;
-    double** bnd[18] = {&f.brho, &f.bUx, &f.bUy, &f.bUz, &f.bp, &f.bT, &f.bh, &f.bpsi, &f.bmu, &f.balpha,
-                        &f.brhoUx, &f.brhoUy, &f.brhoUz, &f.brhoE, &f.vf_p, &f.vf_T, &f.vf_U, &f.TrefH};
-    for (int i = 0; i < 18; ++i) *bnd[i] = p + i * nB;
+    double** bnd[21] = {&f.brho, &f.bUx, &f.bUy, &f.bUz, &f.bp, &f.bT, &f.bh, &f.bpsi, &f.bmu, &f.balpha,
+                        &f.brhoUx, &f.brhoUy, &f.brhoUz, &f.brhoE, &f.vf_p, &f.vf_T, &f.vf_U, &f.TrefH,
+                        &f.refUx, &f.refUy, &f.refUz};
+    for (int i = 0; i < 21; ++i) *bnd[i] = p + i * nB;
+    f.gradU = c.gradBuf.p;
+    f.bgradU = c.gradBuf.p + 9 * n;
     p = c.faceBuf.p;
     double** face[8] = {&f.phi, &f.phiUx, &f.phiUy, &f.phiUz, &f.phiE, &f.Upx, &f.Upy, &f.Upz};
     for (int i = 0; i < 8; ++i) *face[i] = p + i * nF;
@@ -128,6 +134,8 @@ int rdst_mesh_info(const rdst_mesh* mesh, int64_t* info) {
         info[RDST_I_NLEVELS_L] = m->nLevL;
         info[RDST_I_NLEVELS_U] = m->nLevU;
         info[RDST_I_NCELL_COLOURS] = m->nCellColours;
+        info[RDST_I_NPARTS] = m->nParts;
+        info[RDST_I_MAX_PART_LEVELS] = m->maxPartLevels;
     })
 }
 
@@ -149,6 +157,7 @@ int rdst_mesh_get_int(const rdst_mesh* mesh, int which, int32_t* out) {
             case RDST_M_ILU_LEVEL: v = &m->levL; break;
             case RDST_M_BFACE_PATCH: v = &m->bPatch; break;
             case RDST_M_CELL_COLOUR: v = &m->cellColour; break;
+            case RDST_M_PARTITION: v = &m->part; break;
             default: throw ArgError("rdst_mesh_get_int: unknown array");
         }
         std::memcpy(out, v->data(), v->size() * sizeof(int32_t));
@@ -191,8 +200,8 @@ int rdst_case_create(rdst_mesh* mesh, const rdst_thermo* thermo, const rdst_bc* 
             throw ArgError("Unknown dependentVariable; valid: W deltaW", RDST_ERR_UNKNOWN_NAME);
         if (s.precision != RDST_PREC_FP64 && s.precision != RDST_PREC_MIXED) throw ArgError("Unknown precision");
         if (s.n_directions < 1 || s.max_iter < 0 || s.min_iter < 0) throw ArgError("bad solver controls");
-        if (s.viscous) throw ArgError("the viscous terms of calcFields are not provided by this build", RDST_ERR_UNKNOWN_NAME);
-        if (s.ilu_order != RDST_ILU_NATURAL) throw ArgError("only the natural-order ILU(0) is provided by this build");
+        if (s.ilu_order != RDST_ILU_NATURAL && s.ilu_order != RDST_ILU_PARTITIONED)
+            throw ArgError("Unknown ILU ordering; valid: natural partitioned", RDST_ERR_UNKNOWN_NAME);
         if (!(thermo->Cp > thermo->R) || !(thermo->R > 0)) throw ArgError("rdst_case_create: need Cp > R > 0");
         if (!m->uploaded) {
             mesh_upload(*m);

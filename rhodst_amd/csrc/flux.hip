@@ -270,6 +270,98 @@ __device__ inline Flux5 eval_flux(const Side& L, const Side& R, double sx, doubl
     else return flux_kt(L, R, sx, sy, sz, ms);
 }
 
+
+// ---------------------------------------------------------------- viscous terms
+// snGrad() of the U boundary condition (fixedValue dc (U_b - U_I);
+// zeroGradient 0; basicSymmetry (transform(I - 2nn, U_I) - U_I) dc/2;
+// mixed vf (ref - U_I) dc + (1 - vf) 0)
+__device__ inline void sngrad_u_bnd(const BcPatchDev& B, const Fields& F, int b, double nx, double ny, double nz,
+                                    double dcb, double uIx, double uIy, double uIz, double& ox, double& oy, double& oz) {
+    switch (B.U_type) {
+        case RDST_BC_FIXED_VALUE:
+            ox = dcb * (F.bUx[b] - uIx); oy = dcb * (F.bUy[b] - uIy); oz = dcb * (F.bUz[b] - uIz);
+            return;
+        case RDST_BC_SYMMETRY: {
+            double nU = nx * uIx + ny * uIy + nz * uIz;
+            double tx = uIx - 2.0 * nx * nU, ty = uIy - 2.0 * ny * nU, tz = uIz - 2.0 * nz * nU;
+            double h = dcb / 2.0;
+            ox = (tx - uIx) * h; oy = (ty - uIy) * h; oz = (tz - uIz) * h;
+            return;
+        }
+        case RDST_BC_INLET_OUTLET: case RDST_BC_FARFIELD: {
+            double vf = F.vf_U[b];
+            ox = vf * (F.refUx[b] - uIx) * dcb + (1.0 - vf) * 0.0;
+            oy = vf * (F.refUy[b] - uIy) * dcb + (1.0 - vf) * 0.0;
+            oz = vf * (F.refUz[b] - uIz) * dcb + (1.0 - vf) * 0.0;
+            return;
+        }
+        default: ox = oy = oz = 0.0;
+    }
+}
+
+// tauMC = muEff dev2(T(grad U)), dev2(A) = A - (2/3) tr(A) I
+__device__ inline void tau_mc(double mu, const double* g, double* t) {
+    const double tr = g[0] + g[4] + g[8];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) t[3 * i + j] = mu * (g[3 * j + i] - (i == j ? (2.0 / 3.0) * tr : 0.0));
+}
+
+// K2: fvc::grad(U), Gauss linear, one thread per cell
+__global__ __launch_bounds__(256) void k_grad_u(MeshDev m, Fields F) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nC) return;
+    const size_t E = (size_t)m.ellW * m.nC;
+    const double ux = F.Ux[c], uy = F.Uy[c], uz = F.Uz[c];
+    double g[9];
+    for (int q = 0; q < 9; ++q) g[q] = 0.0;
+    for (int k = 0; k < m.ellW; ++k) {
+        const size_t e = (size_t)k * m.nC + c;
+        const uint32_t code = m.ellCode[e];
+        const uint32_t kind = code >> 30;
+        if (kind == ELL_NONE) break;
+        const int idx = (int)(code & ELL_IDX_MASK);
+        const double S[3] = {m.ellSf[e], m.ellSf[E + e], m.ellSf[2 * E + e]};
+        double f[3];
+        double sgn = 1.0;
+        if (kind == ELL_BOUNDARY) {
+            f[0] = F.bUx[idx]; f[1] = F.bUy[idx]; f[2] = F.bUz[idx];
+        } else {
+            const double w = m.ellWt[e];
+            const double ox = F.Ux[idx], oy = F.Uy[idx], oz = F.Uz[idx];
+            const bool own = kind == ELL_OWNER;
+            const double Px = own ? ux : ox, Py = own ? uy : oy, Pz = own ? uz : oz;
+            const double Nx = own ? ox : ux, Ny = own ? oy : uy, Nz = own ? oz : uz;
+            f[0] = w * (Px - Nx) + Nx; f[1] = w * (Py - Ny) + Ny; f[2] = w * (Pz - Nz) + Nz;
+            if (!own) sgn = -1.0;
+        }
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                const double t = S[i] * f[j];
+                if (sgn > 0) g[3 * i + j] += t; else g[3 * i + j] -= t;
+            }
+    }
+    const double V = m.V[c];
+    for (int q = 0; q < 9; ++q) F.gradU[(size_t)q * m.nC + c] = g[q] / V;
+}
+
+// gaussGrad::correctBoundaryConditions: g_b = g_I + n (snGrad_b - n.g_I)
+__global__ void k_grad_u_bnd(MeshDev m, Fields F, const BcPatchDev* bc) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= m.nB) return;
+    const int ci = m.bcell[b], k = m.nI + b;
+    const double ms = m.magSf[k];
+    const double n[3] = {m.Sf[k] / ms, m.Sf[m.nF + k] / ms, m.Sf[2 * m.nF + k] / ms};
+    double gi[9];
+    for (int q = 0; q < 9; ++q) gi[q] = F.gradU[(size_t)q * m.nC + ci];
+    double sn[3];
+    sngrad_u_bnd(bc[m.bpatch[b]], F, b, n[0], n[1], n[2], m.dc[k], F.Ux[ci], F.Uy[ci], F.Uz[ci], sn[0], sn[1], sn[2]);
+    for (int j = 0; j < 3; ++j) {
+        const double nG = n[0] * gi[j] + n[1] * gi[3 + j] + n[2] * gi[6 + j];
+        const double d = sn[j] - nG;
+        for (int i = 0; i < 3; ++i) F.bgradU[(size_t)(3 * i + j) * m.nB + b] = gi[3 * i + j] + n[i] * d;
+    }
+}
+
 __device__ inline Side load_cell(const Fields& F, int i, double gamma) {
     return make_side(F.rho[i], F.rhoUx[i], F.rhoUy[i], F.rhoUz[i], F.rhoE[i], F.p[i], F.h[i], F.psi[i], gamma);
 }
@@ -278,13 +370,23 @@ __device__ inline Side load_bnd(const Fields& F, int b, double gamma) {
 }
 
 // K1: face flux + residual, one thread per cell.
-template <int S, bool STORE>
-__global__ __launch_bounds__(256) void k_flux_residual(MeshDev m, Fields F, double gamma, int lowMach, double Mainf) {
+template <int S, bool STORE, bool VISC>
+__global__ __launch_bounds__(256) void k_flux_residual(MeshDev m, Fields F, const BcPatchDev* bc, double gamma, int lowMach,
+                                                       double Mainf) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
     const Side me = load_cell(F, c, gamma);
     const double fHc = (S == RDST_FLUX_HLLCP) ? F.fH[c] : 0.0;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
+    // viscous accumulators: laplacian(muEff, U), div(tauMC), div(sigmaDotU)
+    double lx = 0.0, ly = 0.0, lz = 0.0, tx = 0.0, ty = 0.0, tz = 0.0, sd = 0.0;
+    double gC[9], tauC[9], muC = 0.0, uCx = 0.0, uCy = 0.0, uCz = 0.0;
+    if (VISC) {
+        muC = F.mu[c];
+        uCx = F.Ux[c]; uCy = F.Uy[c]; uCz = F.Uz[c];
+        for (int q = 0; q < 9; ++q) gC[q] = F.gradU[(size_t)q * m.nC + c];
+        tau_mc(muC, gC, tauC);
+    }
     const size_t E = (size_t)m.ellW * m.nC;
     for (int k = 0; k < m.ellW; ++k) {
         const size_t e = (size_t)k * m.nC + c;
@@ -311,6 +413,60 @@ __global__ __launch_bounds__(256) void k_flux_residual(MeshDev m, Fields F, doub
             }
         }
         const Flux5 o = eval_flux<S>(L, R, sx, sy, sz, ms, gamma, f, lowMach, Mainf);
+        if (VISC) {
+            // face viscous flux (interp(muEff)|S| snGrad(U) and Sf & interp(tauMC)), then its work on Up
+            double lfx, lfy, lfz, tfx, tfy, tfz;
+            if (kind == ELL_BOUNDARY) {
+                const double nx = sx / ms, ny = sy / ms, nz = sz / ms;
+                double snx, sny, snz;
+                sngrad_u_bnd(bc[m.bpatch[idx]], F, idx, nx, ny, nz, m.ellDc[e], uCx, uCy, uCz, snx, sny, snz);
+                const double mg = F.bmu[idx] * ms;
+                lfx = mg * snx; lfy = mg * sny; lfz = mg * snz;
+                double gb[9], tb[9];
+                for (int q = 0; q < 9; ++q) gb[q] = F.bgradU[(size_t)q * m.nB + idx];
+                tau_mc(F.bmu[idx], gb, tb);
+                tfx = sx * tb[0] + sy * tb[3] + sz * tb[6];
+                tfy = sx * tb[1] + sy * tb[4] + sz * tb[7];
+                tfz = sx * tb[2] + sy * tb[5] + sz * tb[8];
+            } else {
+                const bool own = kind == ELL_OWNER;
+                const double w = m.ellWt[e];
+                const double muO = F.mu[idx];
+                double gO[9], tauO[9];
+                for (int q = 0; q < 9; ++q) gO[q] = F.gradU[(size_t)q * m.nC + idx];
+                tau_mc(muO, gO, tauO);
+                const double* gP = own ? gC : gO;
+                const double* gN = own ? gO : gC;
+                const double* tP = own ? tauC : tauO;
+                const double* tN = own ? tauO : tauC;
+                const double muP = own ? muC : muO, muN = own ? muO : muC;
+                const double oUx = F.Ux[idx], oUy = F.Uy[idx], oUz = F.Uz[idx];
+                const double UPx = own ? uCx : oUx, UPy = own ? uCy : oUy, UPz = own ? uCz : oUz;
+                const double UNx = own ? oUx : uCx, UNy = own ? oUy : uCy, UNz = own ? oUz : uCz;
+                const double muf = w * (muP - muN) + muN;
+                double gf[9], tf[9];
+                for (int q = 0; q < 9; ++q) {
+                    gf[q] = w * (gP[q] - gN[q]) + gN[q];
+                    tf[q] = w * (tP[q] - tN[q]) + tN[q];
+                }
+                const double cx = m.ellCorr[e], cy = m.ellCorr[E + e], cz = m.ellCorr[2 * E + e];
+                const double dcf = m.ellDc[e];
+                const double snx = dcf * (UNx - UPx) + (cx * gf[0] + cy * gf[3] + cz * gf[6]);
+                const double sny = dcf * (UNy - UPy) + (cx * gf[1] + cy * gf[4] + cz * gf[7]);
+                const double snz = dcf * (UNz - UPz) + (cx * gf[2] + cy * gf[5] + cz * gf[8]);
+                const double mg = muf * ms;
+                lfx = mg * snx; lfy = mg * sny; lfz = mg * snz;
+                tfx = sx * tf[0] + sy * tf[3] + sz * tf[6];
+                tfy = sx * tf[1] + sy * tf[4] + sz * tf[7];
+                tfz = sx * tf[2] + sy * tf[5] + sz * tf[8];
+            }
+            const double sdf = (lfx + tfx) * o.upx + (lfy + tfy) * o.upy + (lfz + tfz) * o.upz;
+            if (kind == ELL_NEIGH) {
+                lx -= lfx; ly -= lfy; lz -= lfz; tx -= tfx; ty -= tfy; tz -= tfz; sd -= sdf;
+            } else {
+                lx += lfx; ly += lfy; lz += lfz; tx += tfx; ty += tfy; tz += tfz; sd += sdf;
+            }
+        }
         if (kind == ELL_NEIGH) {
             a0 -= o.phi; a1 -= o.fx; a2 -= o.fy; a3 -= o.fz; a4 -= o.fE;
         } else {
@@ -326,11 +482,19 @@ __global__ __launch_bounds__(256) void k_flux_residual(MeshDev m, Fields F, doub
     }
     const double V = m.V[c];
     const int n = m.nC;
+    double r1 = a1 / V, r2 = a2 / V, r3 = a3 / V, r4 = a4 / V;
+    if (VISC) {
+        // R1 -= laplacian + div(tauMC); R4 -= div(sigmaDotU)
+        r1 -= lx / V + tx / V;
+        r2 -= ly / V + ty / V;
+        r3 -= lz / V + tz / V;
+        r4 -= sd / V;
+    }
     F.R[c] = a0 / V;
-    F.R[n + c] = a1 / V;
-    F.R[2 * n + c] = a2 / V;
-    F.R[3 * n + c] = a3 / V;
-    F.R[4 * n + c] = a4 / V;
+    F.R[n + c] = r1;
+    F.R[2 * n + c] = r2;
+    F.R[3 * n + c] = r3;
+    F.R[4 * n + c] = r4;
 }
 
 // HLLCP pressure sensor (HLLCPDST.C:131-164): fCells = min over cellCells of
@@ -356,11 +520,23 @@ void launch_flux(Case& c, bool store) {
     MeshDev md = c.meshDev();
     const int bs = 256;
     const int nb = blocks_for(md.nC, bs);
+    const BcPatchDev* bc = c.d_bc.p;
+    if (c.st.viscous) {
+        hipLaunchKernelGGL(k_grad_u, dim3(nb), dim3(bs), 0, c.stream, md, c.f);
+        if (md.nB) hipLaunchKernelGGL(k_grad_u_bnd, dim3(blocks_for(md.nB, bs)), dim3(bs), 0, c.stream, md, c.f, bc);
+        if (store)
+            hipLaunchKernelGGL((k_flux_residual<S, true, true>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, bc, c.th.gamma,
+                               c.st.low_mach, c.st.Mainf);
+        else
+            hipLaunchKernelGGL((k_flux_residual<S, false, true>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, bc,
+                               c.th.gamma, c.st.low_mach, c.st.Mainf);
+        return;
+    }
     if (store)
-        hipLaunchKernelGGL((k_flux_residual<S, true>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, c.th.gamma,
+        hipLaunchKernelGGL((k_flux_residual<S, true, false>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, bc, c.th.gamma,
                            c.st.low_mach, c.st.Mainf);
     else
-        hipLaunchKernelGGL((k_flux_residual<S, false>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, c.th.gamma,
+        hipLaunchKernelGGL((k_flux_residual<S, false, false>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, bc, c.th.gamma,
                            c.st.low_mach, c.st.Mainf);
 }
 

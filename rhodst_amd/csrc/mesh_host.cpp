@@ -29,6 +29,91 @@ inline P3 crossp(P3 a, P3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * 
 inline double norm(P3 a) { return std::sqrt(dotp(a, a)); }
 }  // namespace
 
+namespace {
+
+// Recursive coordinate bisection: split the cell set at the median of its
+// longest extent (ties broken by cell index) until a set holds at most
+// maxCells cells.
+void rcb(const std::vector<double>& C, std::vector<int>& idx, int lo, int hi, int maxCells, std::vector<int32_t>& part,
+         int& nParts) {
+    if (hi - lo <= maxCells) {
+        for (int k = lo; k < hi; ++k) part[idx[k]] = nParts;
+        ++nParts;
+        return;
+    }
+    double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+    for (int k = lo; k < hi; ++k)
+        for (int d = 0; d < 3; ++d) {
+            mn[d] = std::min(mn[d], C[3 * (size_t)idx[k] + d]);
+            mx[d] = std::max(mx[d], C[3 * (size_t)idx[k] + d]);
+        }
+    int ax = 0;
+    for (int d = 1; d < 3; ++d) if (mx[d] - mn[d] > mx[ax] - mn[ax]) ax = d;
+    const int mid = lo + (hi - lo) / 2;
+    std::nth_element(idx.begin() + lo, idx.begin() + mid, idx.begin() + hi, [&](int a, int b) {
+        double ca = C[3 * (size_t)a + ax], cb = C[3 * (size_t)b + ax];
+        return ca < cb || (ca == cb && a < b);
+    });
+    rcb(C, idx, lo, mid, maxCells, part, nParts);
+    rcb(C, idx, mid, hi, maxCells, part, nParts);
+}
+
+}  // namespace
+
+// Partitions for the block-Jacobi ILU(0): couplings between partitions are
+// dropped, inside a partition the factor is the natural-order ILU(0) of the
+// partition's diagonal block, scheduled by local levels.
+void build_partitions(Mesh& m, int maxCells) {
+    const int n = m.nC;
+    m.part.assign(n, 0);
+    std::vector<int> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    m.nParts = 0;
+    rcb(m.C, idx, 0, n, maxCells, m.part, m.nParts);
+    // cells of each partition in increasing index
+    std::vector<std::vector<int>> cellsOf(m.nParts);
+    for (int c = 0; c < n; ++c) cellsOf[m.part[c]].push_back(c);
+    std::vector<int32_t> lev(n, 0), local(n, -1);
+    m.partRowStart.assign(m.nParts + 1, 0);
+    m.partLevStart.assign(m.nParts + 1, 0);
+    m.partRows.clear();
+    m.partLevPtr.clear();
+    m.maxPartRows = 0;
+    m.maxPartLevels = 0;
+    for (int p = 0; p < m.nParts; ++p) {
+        const std::vector<int>& cs = cellsOf[p];
+        int nl = 0;
+        for (int c : cs) {
+            int l = 0;
+            for (int s = m.rowPtr[c]; s < m.slotDiag[c]; ++s)
+                if (m.part[m.cols[s]] == p) l = std::max(l, lev[m.cols[s]] + 1);
+            lev[c] = l;
+            nl = std::max(nl, l + 1);
+        }
+        std::vector<int> order(cs.begin(), cs.end());
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return lev[a] < lev[b]; });
+        m.partRowStart[p + 1] = m.partRowStart[p] + (int)order.size();
+        for (size_t k = 0; k < order.size(); ++k) {
+            local[order[k]] = (int)k;
+            m.partRows.push_back(order[k]);
+        }
+        // level offsets (absolute positions in partRows)
+        int pos = m.partRowStart[p];
+        for (int l = 0; l < nl; ++l) {
+            m.partLevPtr.push_back(pos);
+            while (pos < m.partRowStart[p + 1] && lev[m.partRows[pos]] == l) ++pos;
+        }
+        m.partLevPtr.push_back(m.partRowStart[p + 1]);
+        m.partLevStart[p + 1] = (int)m.partLevPtr.size();
+        m.maxPartRows = std::max(m.maxPartRows, (int)order.size());
+        m.maxPartLevels = std::max(m.maxPartLevels, nl);
+    }
+    m.localCol.assign(m.nnzb, -1);
+    for (int c = 0; c < n; ++c)
+        for (int s = m.rowPtr[c]; s < m.rowPtr[c + 1]; ++s)
+            if (m.part[m.cols[s]] == m.part[c]) m.localCol[s] = local[m.cols[s]];
+}
+
 void mesh_build(Mesh& m, const rdst_mesh_desc* d) {
     if (!d || d->n_cells <= 0 || d->n_faces <= 0 || d->n_internal_faces < 0 || d->n_internal_faces > d->n_faces)
         throw ArgError("rdst_mesh_create: bad mesh sizes");
@@ -149,6 +234,14 @@ void mesh_build(Mesh& m, const rdst_mesh_desc* d) {
         int f = m.bPolyFace[b];
         P3 delta = sub(fCf[f], cC[d->owner[f]]);
         m.dc[nInt + b] = 1.0 / norm(delta);
+    }
+    // correction vectors n - d deltaCoeffs of the corrected snGrad
+    m.corr.resize(3 * (size_t)nInt);
+    for (int f = 0; f < nInt; ++f) {
+        P3 delta = sub(cC[d->neighbour[f]], cC[d->owner[f]]);
+        P3 ua = {fSf[f].x / fmag[f], fSf[f].y / fmag[f], fSf[f].z / fmag[f]};
+        P3 cv = sub(ua, scl(m.dc[f], delta));
+        m.corr[3 * f] = cv.x; m.corr[3 * f + 1] = cv.y; m.corr[3 * f + 2] = cv.z;
     }
 
     // --- ELL cell->face map, faces of a cell in surface-field order
@@ -279,6 +372,7 @@ void mesh_build(Mesh& m, const rdst_mesh_desc* d) {
     };
     bucket(m.levL, m.nLevL, m.levLPtr, m.levLRows);
     bucket(m.levU, m.nLevU, m.levUPtr, m.levURows);
+    build_partitions(m, kPartCells);
 }
 
 void mesh_upload(Mesh& m) {
@@ -305,11 +399,13 @@ void mesh_upload(Mesh& m) {
     m.d_ellFace.upload(m.ellFace);
     m.d_ellSlot.upload(m.ellSlot);
     const size_t E = (size_t)W * nC;
-    std::vector<double> esf(3 * E, 0.0), emag(E, 0.0), ew(E, 0.0), edc(E, 0.0);
+    std::vector<double> esf(3 * E, 0.0), emag(E, 0.0), ew(E, 0.0), edc(E, 0.0), ecorr(3 * E, 0.0);
     for (size_t e = 0; e < E; ++e) {
         int k = m.ellFace[e];
         if (k < 0) continue;
         for (int d = 0; d < 3; ++d) esf[(size_t)d * E + e] = m.Sf[3 * (size_t)k + d];
+        if (k < m.nI)
+            for (int d = 0; d < 3; ++d) ecorr[(size_t)d * E + e] = m.corr[3 * (size_t)k + d];
         emag[e] = m.magSf[k];
         ew[e] = m.w[k];
         edc[e] = m.dc[k];
@@ -318,6 +414,7 @@ void mesh_upload(Mesh& m) {
     m.d_ellMagSf.upload(emag);
     m.d_ellWt.upload(ew);
     m.d_ellDc.upload(edc);
+    m.d_ellCorr.upload(ecorr);
     m.d_rowPtr.upload(m.rowPtr);
     m.d_cols.upload(m.cols);
     m.d_slotUpper.upload(m.slotUpper);
@@ -325,6 +422,11 @@ void mesh_upload(Mesh& m) {
     m.d_slotDiag.upload(m.slotDiag);
     m.d_levLRows.upload(m.levLRows);
     m.d_levURows.upload(m.levURows);
+    m.d_partRowStart.upload(m.partRowStart);
+    m.d_partRows.upload(m.partRows);
+    m.d_partLevStart.upload(m.partLevStart);
+    m.d_partLevPtr.upload(m.partLevPtr);
+    m.d_localCol.upload(m.localCol);
 }
 
 }  // namespace rdst

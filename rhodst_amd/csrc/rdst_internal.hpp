@@ -79,6 +79,7 @@ struct Mesh {
     std::vector<rdst_patch> patches;
     // host geometry (surface fields: internal faces then compacted boundary faces)
     std::vector<double> Sf, Cf, magSf, w, dc;  // Sf/Cf [nF][3]
+    std::vector<double> corr;                  // [nI][3] non-orthogonal correction vectors n - d deltaCoeffs
     std::vector<double> C, V;                  // C [nC][3]
     std::vector<int32_t> owner, neigh;         // owner [nF] (boundary: face cell), neigh [nI]
     std::vector<int32_t> bPatch, bPolyFace;    // [nB]
@@ -99,6 +100,11 @@ struct Mesh {
     std::vector<int32_t> levL, levU;           // level of each row
     std::vector<int32_t> levLPtr, levLRows, levUPtr, levURows;
     int nLevL = 0, nLevU = 0;
+    // compact partitions (recursive coordinate bisection of the cell centres)
+    // for the block-Jacobi ILU(0): rows of each partition in (local level,
+    // cell) order, level offsets, and per block the partition-local column
+    int nParts = 0, maxPartRows = 0, maxPartLevels = 0;
+    std::vector<int32_t> part, partRowStart, partRows, partLevStart, partLevPtr, localCol;
     bool uploaded = false;  // device copies are made by the first rdst_case_create
 
     // ---- device copies
@@ -107,12 +113,18 @@ struct Mesh {
     DBuf<uint32_t> d_ellCode;
     DBuf<int32_t> d_ellFace, d_ellSlot;
     DBuf<double> d_ellSf, d_ellMagSf, d_ellWt, d_ellDc;  // [3][ellW][nC], [ellW][nC]
+    DBuf<double> d_ellCorr;                               // [3][ellW][nC]
     DBuf<int32_t> d_rowPtr, d_cols, d_slotUpper, d_slotLower, d_slotDiag;
     DBuf<int32_t> d_levLRows, d_levURows;
+    DBuf<int32_t> d_partRowStart, d_partRows, d_partLevStart, d_partLevPtr, d_localCol;
     DBuf<int32_t> d_bWall;  // [nB] 1 where the patch is a wall
 };
 
+// largest partition of the block-Jacobi ILU(0); its FP64 vector (40 B/cell)
+// is held in LDS by one workgroup
+constexpr int kPartCells = 1024;
 void mesh_build(Mesh& m, const rdst_mesh_desc* d);
+void build_partitions(Mesh& m, int maxCells);
 void mesh_upload(Mesh& m);
 
 // ---------------------------------------------------------------------------
@@ -136,6 +148,9 @@ struct Fields {
     // boundary faces
     double *brho, *bUx, *bUy, *bUz, *bp, *bT, *bh, *bpsi, *bmu, *balpha, *brhoUx, *brhoUy, *brhoUz, *brhoE;
     double *vf_p, *vf_T, *vf_U, *TrefH;
+    double *refUx, *refUy, *refUz;  // mixed U BCs: refValue
+    // viscous terms: grad(U) [9][nC] (T[3i+j] = dU_j/dx_i), corrected boundary gradient [9][nB]
+    double *gradU, *bgradU;
     // surface fields (internal then boundary)
     double *phi, *phiUx, *phiUy, *phiUz, *phiE, *Upx, *Upy, *Upz;
     // residual R_ [5][nC]
@@ -154,7 +169,7 @@ struct MeshDev {
     const uint32_t* ellCode;
     const int32_t *ellFace, *ellSlot;
     const int32_t* slotDiag;
-    const double *ellSf, *ellMagSf, *ellWt, *ellDc;
+    const double *ellSf, *ellMagSf, *ellWt, *ellDc, *ellCorr;
 };
 
 // ---------------------------------------------------------------------------
@@ -169,7 +184,7 @@ struct Case {
     float lastFluxMs = 0.f;
 
     // field storage
-    DBuf<double> cellBuf, bndBuf, faceBuf, RBuf, WBuf, timeBuf, fHBuf;
+    DBuf<double> cellBuf, bndBuf, faceBuf, RBuf, WBuf, timeBuf, fHBuf, gradBuf;
     Fields f{};
     // matrix
     DBuf<double> A64, src;          // [nnzb][25], [nC][5]

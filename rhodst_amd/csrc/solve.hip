@@ -363,6 +363,113 @@ __global__ __launch_bounds__(64) void k_ilu_bwd(int cnt, const int32_t* rows, co
         x[5 * (size_t)i + rr] = (TV)(D[5 * rr] * s[0] + D[5 * rr + 1] * s[1] + D[5 * rr + 2] * s[2] + D[5 * rr + 3] * s[3] +
                                      D[5 * rr + 4] * s[4]);
 }
+// ---------------------------------------------------------------- partitioned block ILU(0)
+// One workgroup per partition walks the partition's local levels; entries
+// whose column lies in another partition are dropped (block-Jacobi).
+template <class T>
+__global__ __launch_bounds__(256) void k_pilu_factor(const int32_t* partLevStart, const int32_t* partLevPtr,
+                                                     const int32_t* partRows, const int32_t* rowPtr, const int32_t* cols,
+                                                     const int32_t* diagPos, const int32_t* localCol, T* LU, T* Dinv,
+                                                     int* fail) {
+    const int pt = blockIdx.x;
+    const int l0 = partLevStart[pt], l1 = partLevStart[pt + 1];
+    for (int L = l0; L + 1 < l1; ++L) {
+        const int s = partLevPtr[L], e = partLevPtr[L + 1];
+        for (int t = s + (int)threadIdx.x; t < e; t += blockDim.x) {
+            const int i = partRows[t];
+            const int re = rowPtr[i + 1], di = diagPos[i];
+            for (int p = rowPtr[i]; p < di; ++p) {
+                if (localCol[p] < 0) continue;
+                const int k = cols[p];
+                T Lb[25];
+                mm5(LU + 25 * (size_t)p, Dinv + 25 * (size_t)k, Lb);
+                for (int q = 0; q < 25; ++q) LU[25 * (size_t)p + q] = Lb[q];
+                int qi = p + 1;
+                for (int q = diagPos[k] + 1; q < rowPtr[k + 1]; ++q) {
+                    if (localCol[q] < 0) continue;
+                    const int j = cols[q];
+                    while (qi < re && cols[qi] < j) ++qi;
+                    if (qi >= re) break;
+                    if (cols[qi] != j) continue;
+                    T P[25];
+                    mm5(Lb, LU + 25 * (size_t)q, P);
+                    for (int q2 = 0; q2 < 25; ++q2) LU[25 * (size_t)qi + q2] -= P[q2];
+                }
+            }
+            if (!inv5(LU + 25 * (size_t)di, Dinv + 25 * (size_t)i)) atomicExch(fail, 1);
+        }
+        __syncthreads();
+    }
+}
+
+// x = M^-1 r for the partitioned factor; the partition's vector stays in LDS
+// for the forward and the backward sweep (reverse local levels)
+template <class TM, class TV>
+__global__ __launch_bounds__(256) void k_pilu_apply(const int32_t* partRowStart, const int32_t* partLevStart,
+                                                    const int32_t* partLevPtr, const int32_t* partRows,
+                                                    const int32_t* rowPtr, const int32_t* cols, const int32_t* diagPos,
+                                                    const int32_t* localCol, const TM* LU, const TM* Dinv, const TV* r,
+                                                    TV* x) {
+    extern __shared__ unsigned char smem_raw[];
+    TV* y = reinterpret_cast<TV*>(smem_raw);
+    const int pt = blockIdx.x;
+    const int rs = partRowStart[pt], re = partRowStart[pt + 1];
+    const int nl5 = (re - rs) * 5;
+    for (int t = threadIdx.x; t < nl5; t += blockDim.x) y[t] = r[5 * (size_t)partRows[rs + t / 5] + t % 5];
+    __syncthreads();
+    const int l0 = partLevStart[pt], l1 = partLevStart[pt + 1];
+    for (int L = l0; L + 1 < l1; ++L) {
+        const int s = partLevPtr[L], e = partLevPtr[L + 1];
+        for (int t = s + (int)threadIdx.x; t < e; t += blockDim.x) {
+            const int i = partRows[t];
+            TV* yi = y + 5 * (t - rs);
+            TV a0 = yi[0], a1 = yi[1], a2 = yi[2], a3 = yi[3], a4 = yi[4];
+            for (int p = rowPtr[i]; p < diagPos[i]; ++p) {
+                const int lc = localCol[p];
+                if (lc < 0) continue;
+                const TM* B = LU + 25 * (size_t)p;
+                const TV* yv = y + 5 * lc;
+                const TV y0 = yv[0], y1 = yv[1], y2 = yv[2], y3 = yv[3], y4 = yv[4];
+                a0 -= (TV)(B[0] * y0 + B[1] * y1 + B[2] * y2 + B[3] * y3 + B[4] * y4);
+                a1 -= (TV)(B[5] * y0 + B[6] * y1 + B[7] * y2 + B[8] * y3 + B[9] * y4);
+                a2 -= (TV)(B[10] * y0 + B[11] * y1 + B[12] * y2 + B[13] * y3 + B[14] * y4);
+                a3 -= (TV)(B[15] * y0 + B[16] * y1 + B[17] * y2 + B[18] * y3 + B[19] * y4);
+                a4 -= (TV)(B[20] * y0 + B[21] * y1 + B[22] * y2 + B[23] * y3 + B[24] * y4);
+            }
+            yi[0] = a0; yi[1] = a1; yi[2] = a2; yi[3] = a3; yi[4] = a4;
+        }
+        __syncthreads();
+    }
+    for (int L = l1 - 2; L >= l0; --L) {
+        const int s = partLevPtr[L], e = partLevPtr[L + 1];
+        for (int t = s + (int)threadIdx.x; t < e; t += blockDim.x) {
+            const int i = partRows[t];
+            TV* yi = y + 5 * (t - rs);
+            TV a0 = yi[0], a1 = yi[1], a2 = yi[2], a3 = yi[3], a4 = yi[4];
+            for (int p = diagPos[i] + 1; p < rowPtr[i + 1]; ++p) {
+                const int lc = localCol[p];
+                if (lc < 0) continue;
+                const TM* B = LU + 25 * (size_t)p;
+                const TV* yv = y + 5 * lc;
+                const TV y0 = yv[0], y1 = yv[1], y2 = yv[2], y3 = yv[3], y4 = yv[4];
+                a0 -= (TV)(B[0] * y0 + B[1] * y1 + B[2] * y2 + B[3] * y3 + B[4] * y4);
+                a1 -= (TV)(B[5] * y0 + B[6] * y1 + B[7] * y2 + B[8] * y3 + B[9] * y4);
+                a2 -= (TV)(B[10] * y0 + B[11] * y1 + B[12] * y2 + B[13] * y3 + B[14] * y4);
+                a3 -= (TV)(B[15] * y0 + B[16] * y1 + B[17] * y2 + B[18] * y3 + B[19] * y4);
+                a4 -= (TV)(B[20] * y0 + B[21] * y1 + B[22] * y2 + B[23] * y3 + B[24] * y4);
+            }
+            const TM* D = Dinv + 25 * (size_t)i;
+            yi[0] = (TV)(D[0] * a0 + D[1] * a1 + D[2] * a2 + D[3] * a3 + D[4] * a4);
+            yi[1] = (TV)(D[5] * a0 + D[6] * a1 + D[7] * a2 + D[8] * a3 + D[9] * a4);
+            yi[2] = (TV)(D[10] * a0 + D[11] * a1 + D[12] * a2 + D[13] * a3 + D[14] * a4);
+            yi[3] = (TV)(D[15] * a0 + D[16] * a1 + D[17] * a2 + D[18] * a3 + D[19] * a4);
+            yi[4] = (TV)(D[20] * a0 + D[21] * a1 + D[22] * a2 + D[23] * a3 + D[24] * a4);
+        }
+        __syncthreads();
+    }
+    for (int t = threadIdx.x; t < nl5; t += blockDim.x) x[5 * (size_t)partRows[rs + t / 5] + t % 5] = y[t];
+}
+
 // diagonal (block-Jacobi) preconditioner
 template <class TM, class TV>
 __global__ void k_diag_apply(int n, const TM* Dinv, const TV* r, TV* x) {
@@ -487,7 +594,10 @@ void ilu_factor(Ctx& k, const T* A, T* LU, T* Dinv) {
     RDST_HIP(hipMemcpyAsync(LU, A, sizeof(T) * 25 * (size_t)m.nnzb, hipMemcpyDeviceToDevice, k.s));
     int* fail = (int*)(k.w.partB.p);  // scratch int
     RDST_HIP(hipMemsetAsync(fail, 0, sizeof(int), k.s));
-    if (k.c.st.precond == RDST_PRECOND_ILU0) {
+    if (k.c.st.precond == RDST_PRECOND_ILU0 && k.c.st.ilu_order == RDST_ILU_PARTITIONED) {
+        hipLaunchKernelGGL((k_pilu_factor<T>), dim3(m.nParts), dim3(256), 0, k.s, m.d_partLevStart.p, m.d_partLevPtr.p,
+                           m.d_partRows.p, m.d_rowPtr.p, m.d_cols.p, m.d_slotDiag.p, m.d_localCol.p, LU, Dinv, fail);
+    } else if (k.c.st.precond == RDST_PRECOND_ILU0) {
         for (int L = 0; L < m.nLevL; ++L) {
             int cnt = m.levLPtr[L + 1] - m.levLPtr[L];
             hipLaunchKernelGGL((k_ilu_factor<T>), dim3(blocks_for(cnt, 64)), dim3(64), 0, k.s, cnt,
@@ -515,6 +625,13 @@ void precondition(Ctx& k, const TM* LU, const TM* Dinv, const TV* r, TV* x, TV* 
     }
     if (pc == RDST_PRECOND_DIAG) {
         hipLaunchKernelGGL((k_diag_apply<TM, TV>), dim3(blocks_for(m.nC, 256)), dim3(256), 0, k.s, m.nC, Dinv, r, x);
+        return;
+    }
+    if (k.c.st.ilu_order == RDST_ILU_PARTITIONED) {
+        const size_t shmem = sizeof(TV) * 5 * (size_t)m.maxPartRows;
+        hipLaunchKernelGGL((k_pilu_apply<TM, TV>), dim3(m.nParts), dim3(256), shmem, k.s, m.d_partRowStart.p,
+                           m.d_partLevStart.p, m.d_partLevPtr.p, m.d_partRows.p, m.d_rowPtr.p, m.d_cols.p,
+                           m.d_slotDiag.p, m.d_localCol.p, LU, Dinv, r, x);
         return;
     }
     for (int L = 0; L < m.nLevL; ++L) {

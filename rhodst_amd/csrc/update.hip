@@ -99,6 +99,7 @@ __global__ void k_init_bnd(MeshDev m, Fields F, Thermo th, const BcPatchDev* bc)
     }
     F.bp[b] = pb; F.bT[b] = Tb; F.bUx[b] = ux; F.bUy[b] = uy; F.bUz[b] = uz;
     F.vf_p[b] = 0.0; F.vf_T[b] = 0.0; F.vf_U[b] = 0.0; F.TrefH[b] = 0.0;
+    F.refUx[b] = 0.0; F.refUy[b] = 0.0; F.refUz[b] = 0.0;
     double psi = 1.0 / (th.R * Tb), h = th.Cp * Tb;
     F.bpsi[b] = psi; F.bh[b] = h;
     F.bmu[b] = mu_of(th, Tb); F.balpha[b] = alpha_of(th, Tb);
@@ -165,6 +166,7 @@ __global__ void k_update_bnd_pre(MeshDev m, Fields F, Thermo th, const BcPatchDe
         case RDST_BC_INLET_OUTLET: {
             double vf = 1.0 - fepos(phib);
             F.vf_U[b] = vf;
+            F.refUx[b] = B.U_value[0]; F.refUy[b] = B.U_value[1]; F.refUz[b] = B.U_value[2];
             ux = vf * B.U_value[0] + (1.0 - vf) * uIx;
             uy = vf * B.U_value[1] + (1.0 - vf) * uIy;
             uz = vf * B.U_value[2] + (1.0 - vf) * uIz;
@@ -186,6 +188,7 @@ __global__ void k_update_bnd_pre(MeshDev m, Fields F, Thermo th, const BcPatchDe
             double rz = supIn * B.U_inf[2] + subIn * (B.U_inf[2] - nz * a) + subOut * (nz * o);
             double vf = 1 - supOut - subOut;
             F.vf_U[b] = vf;
+            F.refUx[b] = rx; F.refUy[b] = ry; F.refUz[b] = rz;
             ux = vf * rx + (1.0 - vf) * uIx;
             uy = vf * ry + (1.0 - vf) * uIy;
             uz = vf * rz + (1.0 - vf) * uIz;

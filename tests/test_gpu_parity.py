@@ -19,13 +19,15 @@ def rel(a, b):
 
 
 def make_spec(kind, scheme, **kw):
+    if kind == "viscous":
+        return cases.box_viscous_case(scheme=scheme, **kw)
     if kind == "step":
         return cases.forward_step_case(nx=60, ny=20, scheme=scheme, perturb=0.02, initial_co=0.5, min_co=0.1,
                                        max_co=5.0, **kw)
     return cases.box_farfield_case(nx=24, ny=16, scheme=scheme, jitter=0.2, perturb=0.05, **kw)
 
 
-@pytest.mark.parametrize("kind", ["step", "box"])
+@pytest.mark.parametrize("kind", ["step", "box", "viscous"])
 @pytest.mark.parametrize("scheme", SCHEMES)
 def test_flux_residual_parity(oracle, kind, scheme):
     print(f"[flux parity] {kind} {scheme}", flush=True)
@@ -39,7 +41,7 @@ def test_flux_residual_parity(oracle, kind, scheme):
         assert rel(dc.get(fid), oc.get(fid)) <= 1e-12, fid
 
 
-@pytest.mark.parametrize("kind", ["step", "box"])
+@pytest.mark.parametrize("kind", ["step", "box", "viscous"])
 @pytest.mark.parametrize("scheme", ["AUSM+", "HLLC", "Kurganov"])
 def test_assembly_parity(oracle, kind, scheme):
     print(f"[assembly parity] {kind} {scheme}", flush=True)
@@ -97,7 +99,8 @@ def test_update_fields_and_courant_parity(oracle):
     assert rel(dc.get(11), oc.get(11)) <= 1e-13
 
 
-@pytest.mark.parametrize("kind,scheme", [("step", "AUSM+"), ("box", "HLLC"), ("box", "AUSM+up"), ("step", "Kurganov")])
+@pytest.mark.parametrize("kind,scheme", [("step", "AUSM+"), ("box", "HLLC"), ("box", "AUSM+up"), ("step", "Kurganov"),
+                                         ("viscous", "HLLC"), ("viscous", "AUSM+up")])
 def test_iterate_parity(oracle, kind, scheme):
     print(f"[iterate parity] {kind} {scheme}", flush=True)
     spec = make_spec(kind, scheme)
@@ -123,3 +126,23 @@ def test_mixed_precision_tracks_fp64(oracle):
     assert np.all(np.isfinite(r32))
     assert r32[-1] / r32[0] < 0.5 * r64[0] / r64[0]
     assert abs(np.log10(r32[-1] / r32[0]) - np.log10(r64[-1] / r64[0])) < 0.5
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_partitioned_ilu_parity(oracle, precision):
+    """Block-Jacobi ILU(0) over the device mesh's RCB partitions: the FP64
+    path matches the oracle's ILU(0) of the partition-diagonal blocks; the
+    mixed path follows it to rounding drift of the FP32 solve."""
+    from rhodst_amd import _capi
+    print(f"[partitioned ILU] {precision}", flush=True)
+    spec = cases.box_farfield_case(nx=60, ny=50, ilu_order="partitioned", max_iter=2, precision=precision)
+    dc = DeviceCase(spec)
+    part = dc.mesh.get_int(_capi.M_PARTITION)
+    assert part.max() + 1 == dc.mesh.info["n_parts"] > 1
+    ospec = cases.box_farfield_case(nx=60, ny=50, ilu_order="partitioned", max_iter=2)
+    oc = oracle.OracleCase(ospec)
+    oc.set_partition(part)
+    r1 = dc.iterate(10)
+    r0 = oc.iterate(10)
+    tol = 1e-9 if precision == "fp64" else 1e-3
+    assert rel(r1, r0) <= tol, (r1, r0)
