@@ -22,6 +22,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -51,6 +53,43 @@ def run_path(spec, steps: int, warmup: int, mesh=None):
     return dc, (t1 - t0) * 1e3 / steps, res
 
 
+def converge(args, mesh, spec64, spec32) -> int:
+    """Time to |R|_2 / |R|_2(0) <= 1e-8 (timeDST.C:328-331 norm) for the mixed
+    and the FP64 path; prints one JSON line per path."""
+    from rhodst_amd import DeviceCase
+    out = {}
+    dmesh = None
+    for name, spec in (("mixed", spec32), ("fp64", spec64)):
+        dc = DeviceCase(spec, dmesh)
+        dmesh = dc.mesh
+        r0 = None
+        hist = []
+        t0 = time.perf_counter()
+        steps = 0
+        reached = False
+        while steps < args.max_steps:
+            res = dc.iterate(50)
+            if r0 is None:
+                r0 = res[0]
+            steps += 50
+            hist.extend(res.tolist())
+            drop = res[-1] / r0
+            log(f"{name}: step {steps} drop {drop:.3e} t {time.perf_counter() - t0:.1f}s")
+            if not np.isfinite(drop):
+                break
+            if np.min(np.array(hist) / r0) <= 1e-8:
+                reached = True
+                break
+        t1 = time.perf_counter()
+        h = np.array(hist) / r0
+        k = int(np.argmax(h <= 1e-8)) + 1 if reached else None
+        out[name] = {"reached": reached, "steps": k if reached else steps, "seconds": round(t1 - t0, 2),
+                     "final_drop": float(h[-1]), "min_drop": float(h.min())}
+        del dc
+    print(json.dumps({"converge": out, "cells": mesh.n_cells}), flush=True)
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,6 +99,9 @@ def main(argv=None) -> int:
     ap.add_argument("--flux-calls", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--converge", action="store_true",
+                    help="run the mixed and the FP64 paths (partitioned ILU, classical GS) to a 1e-8 drop of |R|_2")
+    ap.add_argument("--max-steps", type=int, default=4000)
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print("bench.py measures one GPU: every path here runs on a single MI355X; "
@@ -72,8 +114,11 @@ def main(argv=None) -> int:
     mesh = meshgen.swept_wing(**mesh_kw)
     log(f"mesh {mesh.n_cells} cells, {mesh.n_internal} internal faces in {time.perf_counter() - t:.1f}s")
     spec64 = cases.swept_wing_case(mesh=mesh)
-    spec64p = cases.swept_wing_case(mesh=mesh, ilu_order="partitioned")
-    spec32p = cases.swept_wing_case(mesh=mesh, ilu_order="partitioned", precision="mixed")
+    fast = dict(ilu_order="partitioned", gram_schmidt="classical")
+    spec64p = cases.swept_wing_case(mesh=mesh, **fast)
+    spec32p = cases.swept_wing_case(mesh=mesh, precision="mixed", **fast)
+    if args.converge:
+        return converge(args, mesh, spec64p, spec32p)
 
     log("fp64 path, natural-order ILU(0) (the plain path)")
     dc64, ms64, res64 = run_path(spec64, args.steps, args.warmup)
@@ -141,7 +186,7 @@ def main(argv=None) -> int:
         "config": {"workload": f"swept wing C-H mesh {nC} cells ({nI} internal + {nB} boundary faces), "
                                "M0.84 AoA3.06, AUSM+up first order, inviscid slip wall, LTS CFL ramp to 20, "
                                "GMRES(30)+block-ILU(0), 1 restart cycle per step; ms_per_step is the mixed-precision "
-                               "path with the partitioned block-ILU(0)",
+                               "path with the partitioned block-ILU(0) and classical Gram-Schmidt",
                    "cells": nC, "faces": nI + nB, "flux_scheme": "AUSM+up"},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -91,6 +91,10 @@ enum rdst_precision { RDST_PREC_FP64 = 0, RDST_PREC_MIXED = 1 };
  * (what the reference's processor-local ILUC0 computes when the case is
  * decomposed, one partition per rank) */
 enum rdst_ilu_order { RDST_ILU_NATURAL = 0, RDST_ILU_PARTITIONED = 1 };
+/* GMRES orthogonalisation: modified Gram-Schmidt as foam-extend's
+ * BlockGMRESSolver, or classical Gram-Schmidt with one fused multi-dot
+ * (same basis in exact arithmetic, about half the Krylov-vector traffic) */
+enum rdst_gram_schmidt { RDST_GS_MODIFIED = 0, RDST_GS_CLASSICAL = 1 };
 
 typedef struct rdst_patch {
     char name[64];
@@ -155,6 +159,8 @@ typedef struct rdst_settings {
     double rho_min, rho_max, p_min, p_max, T_min, T_max; /* fieldBounds */
     int32_t ilu_order;        /* rdst_ilu_order */
     int32_t viscous;          /* evaluate the viscous terms of calcFields */
+    int32_t gram_schmidt;     /* rdst_gram_schmidt: GMRES orthogonalisation */
+    int32_t pad2;
 } rdst_settings;
 
 /* BlockSolverPerformance<vector5>: per-component initial/final residuals */

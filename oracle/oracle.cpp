@@ -1362,10 +1362,18 @@ static Perf gmres(const Case& c, const Precond& M, std::vector<double>& x, const
                 for (size_t q = 0; q < N5; ++q) Vk[i][q] = wA[q] / beta;
                 Amul(c, Vk[i], rA);
                 M.apply(wA, rA);
-                for (int j = 0; j <= i; ++j) {
-                    beta = sumProd(wA, Vk[j]);
-                    H[(size_t)j * nDirs + i] = beta;
-                    for (size_t q = 0; q < N5; ++q) wA[q] -= beta * Vk[j][q];
+                if (st.gram_schmidt == RDST_GS_CLASSICAL) {
+                    // classical Gram-Schmidt: all projections of the same w, then one update
+                    for (int j = 0; j <= i; ++j) H[(size_t)j * nDirs + i] = sumProd(wA, Vk[j]);
+                    for (int j = 0; j <= i; ++j)
+                        for (size_t q = 0; q < N5; ++q) wA[q] -= H[(size_t)j * nDirs + i] * Vk[j][q];
+                } else {
+                    // modified Gram-Schmidt, as foam-extend's GMRES
+                    for (int j = 0; j <= i; ++j) {
+                        beta = sumProd(wA, Vk[j]);
+                        H[(size_t)j * nDirs + i] = beta;
+                        for (size_t q = 0; q < N5; ++q) wA[q] -= beta * Vk[j][q];
+                    }
                 }
                 beta = std::sqrt(sumProd(wA, wA));
                 for (int j = 0; j < i; ++j) {

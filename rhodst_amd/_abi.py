@@ -54,7 +54,8 @@ class rdst_settings(C.Structure):
                 ("tolerance", C.c_double), ("rel_tol", C.c_double),
                 ("rho_min", C.c_double), ("rho_max", C.c_double), ("p_min", C.c_double),
                 ("p_max", C.c_double), ("T_min", C.c_double), ("T_max", C.c_double),
-                ("ilu_order", C.c_int32), ("viscous", C.c_int32)]
+                ("ilu_order", C.c_int32), ("viscous", C.c_int32), ("gram_schmidt", C.c_int32),
+                ("pad2", C.c_int32)]
 
 
 class rdst_solver_perf(C.Structure):
@@ -143,6 +144,8 @@ def make_settings(**kw) -> rdst_settings:
     order = kw.get("ilu_order", "natural")
     s.ilu_order = {"natural": 0, "partitioned": 1}[order] if isinstance(order, str) else int(order)
     s.viscous = int(kw.get("viscous", 0))
+    gs = kw.get("gram_schmidt", "modified")
+    s.gram_schmidt = {"modified": 0, "classical": 1}[gs] if isinstance(gs, str) else int(gs)
     return s
 
 

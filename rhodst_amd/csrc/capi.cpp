@@ -200,6 +200,8 @@ int rdst_case_create(rdst_mesh* mesh, const rdst_thermo* thermo, const rdst_bc* 
             throw ArgError("Unknown dependentVariable; valid: W deltaW", RDST_ERR_UNKNOWN_NAME);
         if (s.precision != RDST_PREC_FP64 && s.precision != RDST_PREC_MIXED) throw ArgError("Unknown precision");
         if (s.n_directions < 1 || s.max_iter < 0 || s.min_iter < 0) throw ArgError("bad solver controls");
+        if (s.gram_schmidt != RDST_GS_MODIFIED && s.gram_schmidt != RDST_GS_CLASSICAL)
+            throw ArgError("Unknown Gram-Schmidt variant; valid: modified classical", RDST_ERR_UNKNOWN_NAME);
         if (s.ilu_order != RDST_ILU_NATURAL && s.ilu_order != RDST_ILU_PARTITIONED)
             throw ArgError("Unknown ILU ordering; valid: natural partitioned", RDST_ERR_UNKNOWN_NAME);
         if (!(thermo->Cp > thermo->R) || !(thermo->R > 0)) throw ArgError("rdst_case_create: need Cp > R > 0");

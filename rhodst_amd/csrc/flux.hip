@@ -15,6 +15,13 @@
 // take the patch value on both sides.
 #include "rdst_internal.hpp"
 
+// Floating-point contraction only inside one expression: a product rounded in
+// one statement must not be re-fused into a later subtraction, or x*y - x*y
+// (the same state on both sides of a boundary face) leaves the rounding error
+// of x*y instead of the exact zero the reference computes, and the sign tests
+// of the far-field regimes (mach <= -1, < 0, >= 0, >= 1) then flip on it.
+#pragma clang fp contract(on)
+
 namespace rdst {
 
 namespace {
@@ -24,6 +31,7 @@ constexpr double kVSMALL = 1.0e-300;
 
 struct Side {
     double rho, rux, ruy, ruz, rPsi, e, aStar, c;
+    double rr, ux, uy, uz;  // 1/rho and the velocity, formed once per side
 };
 struct Flux5 {
     double phi, fx, fy, fz, fE, upx, upy, upz;
@@ -44,9 +52,12 @@ __device__ inline Side make_side(double rho, double rux, double ruy, double ruz,
                                  double h, double psi, double gamma) {
     Side s;
     s.rho = rho; s.rux = rux; s.ruy = ruy; s.ruz = ruz;
+    s.rr = 1.0 / rho;
+    s.ux = rux * s.rr; s.uy = ruy * s.rr; s.uz = ruz * s.rr;
     s.rPsi = 1.0 / psi;
-    s.e = h - p / rho;
-    double H = fmax(rhoE / rho, kSMALL) + fmax(p / rho, kSMALL);
+    const double pr = p * s.rr;
+    s.e = h - pr;
+    double H = fmax(rhoE * s.rr, kSMALL) + fmax(pr, kSMALL);
     s.aStar = sqrt(2.0 * (gamma - 1.0) / (gamma + 1.0) * H);
     s.c = sqrt(gamma * s.rPsi);
     return s;
@@ -71,9 +82,10 @@ template <bool UP>
 __device__ inline Flux5 flux_ausm(const Side& L, const Side& R, double sx, double sy, double sz, double ms,
                                   int lowMach, double Mainf) {
     const double beta = 0.125;
-    double nx = sx / ms, ny = sy / ms, nz = sz / ms;
-    double uox = L.rux / L.rho, uoy = L.ruy / L.rho, uoz = L.ruz / L.rho;
-    double unx = R.rux / R.rho, uny = R.ruy / R.rho, unz = R.ruz / R.rho;
+    const double rms = 1.0 / ms;
+    double nx = sx * rms, ny = sy * rms, nz = sz * rms;
+    double uox = L.ux, uoy = L.uy, uoz = L.uz;
+    double unx = R.ux, uny = R.uy, unz = R.uz;
     double Uv_own = uox * nx + uoy * ny + uoz * nz;
     double Uv_nei = unx * nx + uny * ny + unz * nz;
     double rhoE_own = L.rho * (L.e + 0.5 * (uox * uox + uoy * uoy + uoz * uoz));
@@ -114,24 +126,25 @@ __device__ inline Flux5 flux_ausm(const Side& L, const Side& R, double sx, doubl
     o.fx = o.phi * o.upx + P12 * sx;
     o.fy = o.phi * o.upy + P12 * sy;
     o.fz = o.phi * o.upz + P12 * sz;
-    double H_own = (rhoE_own + p_own) / L.rho, H_nei = (rhoE_nei + p_nei) / R.rho;
+    double H_own = (rhoE_own + p_own) * L.rr, H_nei = (rhoE_nei + p_nei) * R.rr;
     o.fE = o.phi * (ps * H_own + ng * H_nei);
     return o;
 }
 
 // HLL (HLLDST.C:91-204)
 __device__ inline Flux5 flux_hll(const Side& L, const Side& R, double sx, double sy, double sz, double ms, double gamma) {
-    double uox = L.rux / L.rho, uoy = L.ruy / L.rho, uoz = L.ruz / L.rho;
-    double unx = R.rux / R.rho, uny = R.ruy / R.rho, unz = R.ruz / R.rho;
+    double uox = L.ux, uoy = L.uy, uoz = L.uz;
+    double unx = R.ux, uny = R.uy, unz = R.uz;
     double E_own = L.e + 0.5 * (uox * uox + uoy * uoy + uoz * uoz);
     double E_nei = R.e + 0.5 * (unx * unx + uny * uny + unz * unz);
     double p_own = L.rho * L.rPsi, p_nei = R.rho * R.rPsi;
-    double H_own = E_own + p_own / L.rho, H_nei = E_nei + p_nei / R.rho;
+    double H_own = E_own + p_own * L.rr, H_nei = E_nei + p_nei * R.rr;
     double coefR = sqrt(fmax(kVSMALL, R.rho) / fmax(kVSMALL, L.rho));
     double HT = (coefR * H_nei + H_own) / (coefR + 1.0);
     double utx = (coefR * unx + uox) / (coefR + 1.0), uty = (coefR * uny + uoy) / (coefR + 1.0), utz = (coefR * unz + uoz) / (coefR + 1.0);
     double aT = sqrt((gamma - 1.0) * (HT - 0.5 * (utx * utx + uty * uty + utz * utz)));
-    double nx = sx / ms, ny = sy / ms, nz = sz / ms;
+    const double rms = 1.0 / ms;
+    double nx = sx * rms, ny = sy * rms, nz = sz * rms;
     double Uv_own = uox * nx + uoy * ny + uoz * nz, Uv_nei = unx * nx + uny * ny + unz * nz;
     double UvT = utx * nx + uty * ny + utz * nz;
     double S_own = fmin(Uv_own - L.aStar, UvT - aT);
@@ -160,13 +173,14 @@ __device__ inline Flux5 flux_hll(const Side& L, const Side& R, double sx, double
 template <bool P>
 __device__ inline Flux5 flux_hllc(const Side& L, const Side& R, double sx, double sy, double sz, double ms, double gamma,
                                   double f) {
-    double uox = L.rux / L.rho, uoy = L.ruy / L.rho, uoz = L.ruz / L.rho;
-    double unx = R.rux / R.rho, uny = R.ruy / R.rho, unz = R.ruz / R.rho;
+    double uox = L.ux, uoy = L.uy, uoz = L.uz;
+    double unx = R.ux, uny = R.uy, unz = R.uz;
     double E_own = L.e + 0.5 * (uox * uox + uoy * uoy + uoz * uoz);
     double E_nei = R.e + 0.5 * (unx * unx + uny * uny + unz * unz);
     double p_own = L.rho * L.rPsi, p_nei = R.rho * R.rPsi;
-    double H_own = E_own + p_own / L.rho, H_nei = E_nei + p_nei / R.rho;
-    double nx = sx / ms, ny = sy / ms, nz = sz / ms;
+    double H_own = E_own + p_own * L.rr, H_nei = E_nei + p_nei * R.rr;
+    const double rms = 1.0 / ms;
+    double nx = sx * rms, ny = sy * rms, nz = sz * rms;
     double Uv_own = uox * nx + uoy * ny + uoz * nz, Uv_nei = unx * nx + uny * ny + unz * nz;
     double coefR = sqrt(fmax(kVSMALL, R.rho) / fmax(kVSMALL, L.rho));
     double HT = (coefR * H_nei + H_own) / (coefR + 1.0);
@@ -231,8 +245,8 @@ __device__ inline Flux5 flux_hllc(const Side& L, const Side& R, double sx, doubl
 
 // Kurganov-Tadmor central-upwind (KurganovDST.C:93-167)
 __device__ inline Flux5 flux_kt(const Side& L, const Side& R, double sx, double sy, double sz, double ms) {
-    double uox = L.rux / L.rho, uoy = L.ruy / L.rho, uoz = L.ruz / L.rho;
-    double unx = R.rux / R.rho, uny = R.ruy / R.rho, unz = R.ruz / R.rho;
+    double uox = L.ux, uoy = L.uy, uoz = L.uz;
+    double unx = R.ux, uny = R.uy, unz = R.uz;
     double rhoE_own = L.rho * (L.e + 0.5 * (uox * uox + uoy * uoy + uoz * uoz));
     double rhoE_nei = R.rho * (R.e + 0.5 * (unx * unx + uny * uny + unz * unz));
     double p_own = L.rho * L.rPsi, p_nei = R.rho * R.rPsi;
@@ -417,7 +431,8 @@ __global__ __launch_bounds__(256) void k_flux_residual(MeshDev m, Fields F, cons
             // face viscous flux (interp(muEff)|S| snGrad(U) and Sf & interp(tauMC)), then its work on Up
             double lfx, lfy, lfz, tfx, tfy, tfz;
             if (kind == ELL_BOUNDARY) {
-                const double nx = sx / ms, ny = sy / ms, nz = sz / ms;
+                const double rms = 1.0 / ms;
+                const double nx = sx * rms, ny = sy * rms, nz = sz * rms;
                 double snx, sny, snz;
                 sngrad_u_bnd(bc[m.bpatch[idx]], F, idx, nx, ny, nz, m.ellDc[e], uCx, uCy, uCz, snx, sny, snz);
                 const double mg = F.bmu[idx] * ms;

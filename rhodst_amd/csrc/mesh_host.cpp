@@ -373,6 +373,32 @@ void mesh_build(Mesh& m, const rdst_mesh_desc* d) {
     bucket(m.levL, m.nLevL, m.levLPtr, m.levLRows);
     bucket(m.levU, m.nLevU, m.levUPtr, m.levURows);
     build_partitions(m, kPartCells);
+    // sliced ELL (64 rows per slice)
+    m.nSlices = (nCells + 63) / 64;
+    m.sellPtr.assign(m.nSlices + 1, 0);
+    m.sellWidth.assign(m.nSlices, 0);
+    for (int sl = 0; sl < m.nSlices; ++sl) {
+        int w = 0;
+        for (int r = 64 * sl; r < std::min(nCells, 64 * sl + 64); ++r) w = std::max(w, m.rowPtr[r + 1] - m.rowPtr[r]);
+        m.sellWidth[sl] = w;
+        m.sellPtr[sl + 1] = m.sellPtr[sl] + 64 * (int64_t)w;
+    }
+    m.sellBlocks = m.sellPtr[m.nSlices];
+    m.sellCol.assign(m.sellBlocks, -1);
+    m.sellOfSlot.assign(m.nnzb, 0);
+    for (int r = 0; r < nCells; ++r) {
+        const int sl = r / 64, lane = r % 64;
+        for (int s = m.rowPtr[r]; s < m.rowPtr[r + 1]; ++s) {
+            const int p = s - m.rowPtr[r];
+            const int64_t pos = m.sellPtr[sl] + 64 * (int64_t)p + lane;
+            m.sellCol[pos] = m.cols[s];
+            m.sellOfSlot[s] = 25 * (m.sellPtr[sl] + 64 * (int64_t)p) + lane;
+        }
+        // padding blocks point at the row itself with zero values
+        for (int p = m.rowPtr[r + 1] - m.rowPtr[r]; p < m.sellWidth[sl]; ++p) m.sellCol[m.sellPtr[sl] + 64 * (int64_t)p + lane] = r;
+    }
+    for (int64_t q = 0; q < m.sellBlocks; ++q)
+        if (m.sellCol[q] < 0) m.sellCol[q] = 0;  // lanes past the last row of the last slice
 }
 
 void mesh_upload(Mesh& m) {
@@ -427,6 +453,10 @@ void mesh_upload(Mesh& m) {
     m.d_partLevStart.upload(m.partLevStart);
     m.d_partLevPtr.upload(m.partLevPtr);
     m.d_localCol.upload(m.localCol);
+    m.d_sellPtr.upload(m.sellPtr);
+    m.d_sellOfSlot.upload(m.sellOfSlot);
+    m.d_sellWidth.upload(m.sellWidth);
+    m.d_sellCol.upload(m.sellCol);
 }
 
 }  // namespace rdst

@@ -105,6 +105,15 @@ struct Mesh {
     // cell) order, level offsets, and per block the partition-local column
     int nParts = 0, maxPartRows = 0, maxPartLevels = 0;
     std::vector<int32_t> part, partRowStart, partRows, partLevStart, partLevPtr, localCol;
+    // sliced-ELL copy of the block-CSR pattern for the SpMV: slices of 64
+    // rows, slice s holds sellWidth[s] block columns; block p of row r sits at
+    // block position sellPtr[s] + 64 p + (r - 64 s), its entry e at value
+    // index 25 * (sellPtr[s] + 64 p) + 64 e + (r - 64 s)
+    int nSlices = 0;
+    long sellBlocks = 0;
+    std::vector<int64_t> sellPtr;
+    std::vector<int32_t> sellWidth, sellCol;
+    std::vector<int64_t> sellOfSlot;  // value index of entry 0 of each block-CSR slot
     bool uploaded = false;  // device copies are made by the first rdst_case_create
 
     // ---- device copies
@@ -117,6 +126,8 @@ struct Mesh {
     DBuf<int32_t> d_rowPtr, d_cols, d_slotUpper, d_slotLower, d_slotDiag;
     DBuf<int32_t> d_levLRows, d_levURows;
     DBuf<int32_t> d_partRowStart, d_partRows, d_partLevStart, d_partLevPtr, d_localCol;
+    DBuf<int64_t> d_sellPtr, d_sellOfSlot;
+    DBuf<int32_t> d_sellWidth, d_sellCol;
     DBuf<int32_t> d_bWall;  // [nB] 1 where the patch is a wall
 };
 
@@ -189,6 +200,8 @@ struct Case {
     // matrix
     DBuf<double> A64, src;          // [nnzb][25], [nC][5]
     DBuf<float> A32;                // FP32 copy for mixed precision
+    DBuf<double> S64;               // sliced-ELL copies for the SpMV
+    DBuf<float> S32;
     bool hasState = false, hasMatrix = false;
     // solver scratch owned by the solver module
     struct SolverWork* work = nullptr;

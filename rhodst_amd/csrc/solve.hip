@@ -38,6 +38,7 @@ struct SolverWork {
     DBuf<float> V32, w32, r32, t32;
     DBuf<double> scal;         // device scalars
     DBuf<double> partA, partB; // reduction partials
+    DBuf<double> multi;        // classical Gram-Schmidt partials [nDirs+1][G]
     std::vector<double> hscal;
     // BiCGStab
     DBuf<double> p, ph, v, t, th, rw;
@@ -260,6 +261,91 @@ __global__ __launch_bounds__(256) void k_bspmv(int n, const int32_t* rowPtr, con
                       (double)B[5 * r + 3] * x3 + (double)B[5 * r + 4] * x4;
     }
     for (int r = 0; r < 5; ++r) y[5 * (size_t)i + r] = (TY)acc[r];
+}
+
+// ---------------------------------------------------------------- sliced-ELL SpMV
+// block-CSR -> sliced ELL (64 rows per slice, entry-major inside a block column)
+template <class T>
+__global__ void k_to_sell(long nnzb, const int64_t* sellOfSlot, const T* A, T* S) {
+    for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < nnzb; j += (long)gridDim.x * blockDim.x) {
+        const int64_t base = sellOfSlot[j];
+        const T* a = A + 25 * (size_t)j;
+        for (int e = 0; e < 25; ++e) S[base + 64 * e] = a[e];
+    }
+}
+// one 64-lane wavefront per slice, lane = row: every value load is 64
+// consecutive entries (fully coalesced), x is gathered per block column
+template <class TA, class TX, class TY>
+__global__ __launch_bounds__(64) void k_sell_spmv(int n, const int64_t* sellPtr, const int32_t* sellWidth,
+                                                  const int32_t* sellCol, const TA* S, const TX* x, TY* y) {
+    const int sl = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int row = 64 * sl + lane;
+    const int64_t b0 = sellPtr[sl];
+    const int w = sellWidth[sl];
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
+    for (int p = 0; p < w; ++p) {
+        const int64_t blk = b0 + 64 * (int64_t)p;
+        const int col = sellCol[blk + lane];
+        const TX* xv = x + 5 * (size_t)col;
+        const double x0 = xv[0], x1 = xv[1], x2 = xv[2], x3 = xv[3], x4 = xv[4];
+        const TA* v = S + 25 * blk + lane;
+        a0 += (double)v[0] * x0 + (double)v[64] * x1 + (double)v[128] * x2 + (double)v[192] * x3 + (double)v[256] * x4;
+        a1 += (double)v[320] * x0 + (double)v[384] * x1 + (double)v[448] * x2 + (double)v[512] * x3 + (double)v[576] * x4;
+        a2 += (double)v[640] * x0 + (double)v[704] * x1 + (double)v[768] * x2 + (double)v[832] * x3 + (double)v[896] * x4;
+        a3 += (double)v[960] * x0 + (double)v[1024] * x1 + (double)v[1088] * x2 + (double)v[1152] * x3 + (double)v[1216] * x4;
+        a4 += (double)v[1280] * x0 + (double)v[1344] * x1 + (double)v[1408] * x2 + (double)v[1472] * x3 + (double)v[1536] * x4;
+    }
+    if (row < n) {
+        TY* yr = y + 5 * (size_t)row;
+        yr[0] = (TY)a0; yr[1] = (TY)a1; yr[2] = (TY)a2; yr[3] = (TY)a3; yr[4] = (TY)a4;
+    }
+}
+
+// ---------------------------------------------------------------- classical Gram-Schmidt
+// partial dots of w with NACC consecutive basis vectors V_j0 .. V_j0+NACC-1 (cnt of them valid)
+template <class T, int NACC>
+__global__ __launch_bounds__(RED_BS) void k_multidot_partial(long n, const T* w, const T* V, int j0, int cnt,
+                                                             double* part) {
+    __shared__ double sh[16];
+    double acc[NACC];
+    for (int q = 0; q < NACC; ++q) acc[q] = 0.0;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double wi = (double)w[i];
+#pragma unroll
+        for (int q = 0; q < NACC; ++q)
+            if (q < cnt) acc[q] += wi * (double)V[(size_t)(j0 + q) * n + i];
+    }
+#pragma unroll
+    for (int q = 0; q < NACC; ++q) {
+        if (q >= cnt) break;
+        const double t = block_sum(acc[q], sh);
+        if (threadIdx.x == 0) part[(size_t)(j0 + q) * gridDim.x + blockIdx.x] = t;
+    }
+}
+// H[j][col] = sum of the partials of dot j, j = 0..cnt-1 (one block)
+__global__ __launch_bounds__(RED_BS) void k_multidot_final(const double* part, int G, int cnt, double* H, int m, int col) {
+    __shared__ double sh[16];
+    for (int j = 0; j < cnt; ++j) {
+        const double t = reduce_partials(part + (size_t)j * G, G, sh);
+        if (threadIdx.x == 0) H[j * m + col] = t;
+    }
+}
+// w -= sum_j H[j][col] V_j, and the partial sums of |w|^2
+template <class T>
+__global__ __launch_bounds__(RED_BS) void k_cgs_update(long n, T* w, const T* V, const double* H, int m, int col, int cnt,
+                                                       double* part) {
+    __shared__ double sh[16];
+    double s = 0.0;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        double wi = (double)w[i];
+        for (int j = 0; j < cnt; ++j) wi -= H[j * m + col] * (double)V[(size_t)j * n + i];
+        const T wt = (T)wi;
+        w[i] = wt;
+        s += (double)wt * (double)wt;
+    }
+    const double t = block_sum(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
 // ---------------------------------------------------------------- block ILU(0)
@@ -564,6 +650,7 @@ static void ensure_work(Case& c) {
     w->hscal.assign(w->scal.n, 0.0);
     w->partA.alloc(5 * RED_MAXG);
     w->partB.alloc(5 * RED_MAXG);
+    w->multi.alloc((size_t)(w->nDirs + 1) * RED_MAXG);
 }
 
 void solver_free(Case& c) {
@@ -581,11 +668,31 @@ struct Ctx {
     int G;
 };
 
-// y = A x (FP64 matrix)
+// y = A x: the sliced-ELL copy of A when one was built for this step
 template <class TA, class TX, class TY>
 void spmv(Ctx& k, const TA* A, const TX* x, TY* y) {
+    const TA* S = nullptr;
+    if ((const void*)A == (const void*)k.c.A64.p) S = (const TA*)k.c.S64.p;
+    if ((const void*)A == (const void*)k.c.A32.p) S = (const TA*)k.c.S32.p;
+    if (S) {
+        hipLaunchKernelGGL((k_sell_spmv<TA, TX, TY>), dim3(k.m.nSlices), dim3(64), 0, k.s, k.m.nC, k.m.d_sellPtr.p,
+                           k.m.d_sellWidth.p, k.m.d_sellCol.p, S, x, y);
+        return;
+    }
     hipLaunchKernelGGL((k_bspmv<TA, TX, TY>), dim3(blocks_for(k.m.nC, 256)), dim3(256), 0, k.s, k.m.nC, k.m.d_rowPtr.p,
                        k.m.d_cols.p, A, x, y);
+}
+
+// refresh the sliced-ELL copy of the assembled matrix
+template <class T>
+void build_sell(Ctx& k, const T* A, DBuf<T>& S) {
+    const size_t nv = 25 * (size_t)k.m.sellBlocks;
+    if (S.n != nv) {
+        S.alloc(nv);
+        RDST_HIP(hipMemsetAsync(S.p, 0, sizeof(T) * nv, k.s));  // padding blocks stay zero
+    }
+    hipLaunchKernelGGL((k_to_sell<T>), dim3(std::min(4096, blocks_for(k.m.nnzb, 256))), dim3(256), 0, k.s,
+                       (long)k.m.nnzb, k.m.d_sellOfSlot.p, A, S.p);
 }
 
 template <class T>
@@ -713,15 +820,25 @@ void gmres_cycle(Ctx& k, const TM* A, const TM* LU, const TM* Dinv, TV* z, const
         hipLaunchKernelGGL((k_scale_div<TV>), dim3(G), dim3(RED_BS), 0, k.s, n5, Vi, wv, sc + S_BETA);
         spmv(k, A, Vi, rv);
         precondition(k, LU, Dinv, rv, wv, ytmp);
-        // partial dot(w, V0)
-        hipLaunchKernelGGL((k_dot_partial<TV, TV>), dim3(G), dim3(RED_BS), 0, k.s, n5, wv, V, k.w.partA.p);
         double* pin = k.w.partA.p;
         double* pout = k.w.partB.p;
-        for (int j = 0; j <= i; ++j) {
-            const TV* Vj = V + (size_t)j * n5;
-            const TV* Vn = (j < i) ? V + (size_t)(j + 1) * n5 : wv;
-            hipLaunchKernelGGL((k_mgs_step<TV>), dim3(G), dim3(RED_BS), 0, k.s, n5, wv, Vj, Vn, pin, G, pout, H + j * m + i);
-            std::swap(pin, pout);
+        if (k.c.st.gram_schmidt == RDST_GS_CLASSICAL) {
+            // all projections from one multi-dot, then one fused update (+ |w|^2 partials)
+            for (int j0 = 0; j0 <= i; j0 += 8)
+                hipLaunchKernelGGL((k_multidot_partial<TV, 8>), dim3(G), dim3(RED_BS), 0, k.s, n5, wv, V, j0,
+                                   std::min(8, i + 1 - j0), k.w.multi.p);
+            hipLaunchKernelGGL(k_multidot_final, dim3(1), dim3(RED_BS), 0, k.s, k.w.multi.p, G, i + 1, H, m, i);
+            hipLaunchKernelGGL((k_cgs_update<TV>), dim3(G), dim3(RED_BS), 0, k.s, n5, wv, V, H, m, i, i + 1, pin);
+        } else {
+            // partial dot(w, V0), then the fused MGS steps
+            hipLaunchKernelGGL((k_dot_partial<TV, TV>), dim3(G), dim3(RED_BS), 0, k.s, n5, wv, V, pin);
+            for (int j = 0; j <= i; ++j) {
+                const TV* Vj = V + (size_t)j * n5;
+                const TV* Vn = (j < i) ? V + (size_t)(j + 1) * n5 : wv;
+                hipLaunchKernelGGL((k_mgs_step<TV>), dim3(G), dim3(RED_BS), 0, k.s, n5, wv, Vj, Vn, pin, G, pout,
+                                   H + j * m + i);
+                std::swap(pin, pout);
+            }
         }
         // pin holds partials of |w|^2; rotations, and beta = |w| for the next V
         hipLaunchKernelGGL(k_gmres_column, dim3(1), dim3(RED_BS), 0, k.s, sc, pin, G, i, m);
@@ -841,9 +958,14 @@ void block_solve(Case& c, rdst_solver_perf* perfOut) {
     const bool mixed = c.st.precision == RDST_PREC_MIXED;
     if (mixed && c.st.solver != RDST_SOLVER_GMRES)
         throw ArgError("mixed precision is provided for the GMRES solver", RDST_ERR_UNKNOWN_NAME);
-    // preconditioner factors
-    if (mixed) ilu_factor<float>(k, c.A32.p, w.LU32.p, w.Dinv32.p);
-    else ilu_factor<double>(k, c.A64.p, w.LU.p, w.Dinv.p);
+    // preconditioner factors and the sliced-ELL copy for the SpMV
+    if (mixed) {
+        ilu_factor<float>(k, c.A32.p, w.LU32.p, w.Dinv32.p);
+        build_sell<float>(k, c.A32.p, c.S32);
+    } else {
+        ilu_factor<double>(k, c.A64.p, w.LU.p, w.Dinv.p);
+        build_sell<double>(k, c.A64.p, c.S64);
+    }
     rdst_solver_perf perf{};
     double* W = c.f.W;
     if (!mixed && c.st.dependent == RDST_DEP_W) {

@@ -18,6 +18,13 @@
 
 #include "rdst_internal.hpp"
 
+// Floating-point contraction only inside one expression: a product rounded in
+// one statement must not be re-fused into a later subtraction, or x*y - x*y
+// (the same state on both sides of a boundary face) leaves the rounding error
+// of x*y instead of the exact zero the reference computes, and the sign tests
+// of the far-field regimes (mach <= -1, < 0, >= 0, >= 1) then flip on it.
+#pragma clang fp contract(on)
+
 namespace rdst {
 
 namespace {

@@ -146,3 +146,39 @@ def test_partitioned_ilu_parity(oracle, precision):
     r0 = oc.iterate(10)
     tol = 1e-9 if precision == "fp64" else 1e-3
     assert rel(r1, r0) <= tol, (r1, r0)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_classical_gram_schmidt_parity(oracle, precision):
+    """Classical Gram-Schmidt GMRES (fused multi-dot) against the oracle's
+    classical Gram-Schmidt, with the partitioned ILU and the sliced-ELL SpMV."""
+    from rhodst_amd import _capi
+    print(f"[classical GS] {precision}", flush=True)
+    kw = dict(ilu_order="partitioned", max_iter=2, gram_schmidt="classical")
+    spec = cases.box_farfield_case(nx=60, ny=50, precision=precision, **kw)
+    dc = DeviceCase(spec)
+    oc = oracle.OracleCase(cases.box_farfield_case(nx=60, ny=50, **kw))
+    oc.set_partition(dc.mesh.get_int(_capi.M_PARTITION))
+    r1 = dc.iterate(10)
+    r0 = oc.iterate(10)
+    tol = 1e-9 if precision == "fp64" else 1e-3
+    assert rel(r1, r0) <= tol, (r1, r0)
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_tangential_farfield_mass_flux_is_zero(oracle, scheme):
+    """On boundary faces whose state is tangential (U_b . Sf = 0 exactly) the
+    reference's flux gives phi = 0 exactly, and the far-field regime masks
+    (P.C:174-191) then select subsonic outflow; a rounding-level phi of the
+    wrong sign would flip them to inflow."""
+    print(f"[tangential phi] {scheme}", flush=True)
+    spec = cases.box_viscous_case(scheme=scheme)
+    oc = oracle.OracleCase(spec)
+    dc = DeviceCase(spec)
+    oc.flux_residual()
+    dc.flux_residual()
+    m = spec.mesh
+    top = [p for p in m.patches if p[0] == "top"][0]
+    sl = slice(top[2], top[2] + top[3])
+    assert np.all(oc.get(12)[sl] == 0.0)
+    assert np.all(dc.get(12)[sl] == 0.0), dc.get(12)[sl]

@@ -213,3 +213,35 @@ def test_solvers_against_numpy(oracle, solver):
         x1, perf1 = oc1.linear_solve(b)
         assert perf1.n_iterations == 1
         assert np.linalg.norm(A @ x1 - b) < 1e-6 * np.linalg.norm(b)
+
+
+def test_classical_gram_schmidt_gmres(oracle):
+    """GMRES with classical Gram-Schmidt builds the same Krylov basis as the
+    modified form in exact arithmetic: same solution (numpy) and, on a small
+    well-conditioned cycle, the same iterate to rounding."""
+    mesh = meshgen.box_2d(3, 2, jitter=0.1)
+    bc = {k: ZG for k in ("left", "right", "bottom", "top")}
+    b = np.random.default_rng(5).normal(size=5 * mesh.n_cells)
+    xs = {}
+    for gs in ("modified", "classical"):
+        spec = small_case(mesh, bc, solver="GMRES", n_directions=30, max_iter=200, min_iter=1, tolerance=1e-11,
+                          rel_tol=0.0, gram_schmidt=gs)
+        oc = oracle.OracleCase(spec)
+        oc.flux_residual()
+        oc.jacobian_assemble()
+        A = dense_from_ldu(oc, mesh)
+        xs[gs], _ = oc.linear_solve(b)
+        # a short cycle (5 directions): both variants give the same iterate
+        spec5 = small_case(mesh, bc, solver="GMRES", n_directions=5, max_iter=1, min_iter=1, tolerance=0.0,
+                           rel_tol=0.0, gram_schmidt=gs)
+        o5 = oracle.OracleCase(spec5)
+        o5.flux_residual()
+        o5.jacobian_assemble()
+        xs[gs + "5"], _ = o5.linear_solve(b)
+    xr = np.linalg.solve(A, b)
+    assert np.allclose(xs["classical"], xr, rtol=1e-8, atol=1e-8 * np.abs(xr).max())
+    d = np.abs(xs["classical5"] - xs["modified5"]).max() / np.abs(xs["modified5"]).max()
+    assert d < 1e-9, d
+    # and the short cycle stopped well short of the solution, so the 1e-9
+    # agreement is not that of two converged solves
+    assert np.abs(xs["modified5"] - xr).max() > 1e-7 * np.abs(xr).max()
