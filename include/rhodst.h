@@ -163,6 +163,37 @@ typedef struct rdst_settings {
     int32_t pad2;
 } rdst_settings;
 
+/* compressible::RASModel selected by constant/RASProperties RASModel */
+enum rdst_turbulence_model {
+    RDST_TURB_LAMINAR = 0,
+    RDST_TURB_SANEG = 1,   /* SAneg (libraries/turbulence/SAneg/SAneg.C) */
+    RDST_TURB_SARC2 = 2    /* SARC2 (libraries/turbulence/SARC2/SARC2.C) */
+};
+
+/* RASProperties <model>Coeffs (SAneg.C:168-307, SARC2.C:172-285), plus
+ * fvSolution solvers.nuTilda (smoothSolver, symGaussSeidel) and
+ * relaxationFactors.nuTilda (tutorials/RAE2822/system/fvSolution:43-57) */
+typedef struct rdst_turbulence {
+    int32_t model;                /* rdst_turbulence_model */
+    int32_t negative_nu_tilda;    /* SAneg negativeNuTilda (default on) */
+    int32_t stilda_modification;  /* SAneg StildaModification (default on) */
+    int32_t rcm_correction;       /* SARC2 RCMCorrection (default on) */
+    double sigma_nut, kappa, Prt; /* 0.66666, 0.41, 1 */
+    double Cb1, Cb2, Cw2, Cw3, Cv1, Cv2;  /* 0.1355 0.622 0.3 2 7.1 5 */
+    double Cn1, Cn2, Cn3, Ct3, Ct4;       /* SAneg: 16 0.7 0.9 1.2 0.5 */
+    double Cr1, Cr2, Cr3;                 /* SARC2: 1 2 1 */
+    double relax;                 /* nuTilda relaxation factor (<= 0: none) */
+    double tolerance, rel_tol;    /* smoothSolver tolerance / relTol */
+    int32_t min_iter, max_iter;   /* symmetric Gauss-Seidel sweeps */
+} rdst_turbulence;
+
+/* 0/nuTilda boundary condition of one patch (the wall patches get mut = 0) */
+typedef struct rdst_scalar_bc {
+    int32_t type;   /* RDST_BC_FIXED_VALUE, _ZERO_GRADIENT, _INLET_OUTLET, _SYMMETRY, _EMPTY */
+    int32_t pad;
+    double value;   /* fixedValue value / inletOutlet inletValue */
+} rdst_scalar_bc;
+
 /* BlockSolverPerformance<vector5>: per-component initial/final residuals */
 typedef struct rdst_solver_perf {
     double initial_residual[5];
@@ -187,7 +218,17 @@ enum rdst_field {
     RDST_F_BCSR_VALUES = 24,      /* assembled FP64 blocks [nnzb][25] */
     RDST_F_SOURCE = 25,           /* assembled source [n_cells][5] */
     RDST_F_MU = 26,
-    RDST_F_RHO_OLD = 27, RDST_F_E_OLD = 28
+    RDST_F_RHO_OLD = 27, RDST_F_E_OLD = 28,
+    RDST_F_NU_TILDA = 29,         /* cells */
+    RDST_F_MUT = 30,              /* cells */
+    RDST_F_ALPHAT = 31,           /* cells */
+    RDST_F_WALL_DIST = 32,        /* cells: the wallDist d the caller supplied */
+    RDST_F_BNU_TILDA = 33,        /* boundary faces */
+    RDST_F_BMUT = 34,             /* boundary faces */
+    RDST_F_SA_DIAG = 35,          /* nuTilda matrix after relax(): diagonal incl. boundary coefficients [n_cells] */
+    RDST_F_SA_UPPER = 36,         /* [n_internal] (owner row, neighbour column) */
+    RDST_F_SA_LOWER = 37,         /* [n_internal] (neighbour row, owner column) */
+    RDST_F_SA_SOURCE = 38         /* [n_cells] incl. boundary coefficients and relaxation */
 };
 
 /* integer arrays of a mesh */
@@ -281,8 +322,28 @@ int rdst_set_courant(rdst_case* c, double* res_norm);
  * each step and perf[n_steps] (may be NULL) the solver performance */
 int rdst_iterate(rdst_case* c, int n_steps, double* res_norm, rdst_solver_perf* perf);
 
-/* forceCoeffs-style integral over one patch: force = sum p_b Sf (+ viscous
- * part when settings.viscous), out[0]=Cd, out[1]=Cl, out[2..4]=force */
+/* Spalart-Allmaras turbulence on the case: model coefficients, the 0/nuTilda
+ * boundary conditions [n_patches], the wall distance d [n_cells] (wallDist,
+ * supplied by the caller) and the initial nuTilda [n_cells]; computes mut
+ * and alphat = mut/Prt, which then enter muEff/alphaEff of calcFields
+ * (baseFluxDST.C:231) and of the Roe spectral radius (Roe.C:187-199).
+ * RDST_TURB_LAMINAR switches the model off (mut = 0). */
+int rdst_case_set_turbulence(rdst_case* c, const rdst_turbulence* turb, const rdst_scalar_bc* nu_tilda_bcs,
+                             const double* wall_dist, const double* nu_tilda);
+/* RASModel::correct() (SAneg.C:490-605, SARC2.C:418-488): assemble the
+ * nuTilda equation (steadyStateDST ddt with rDeltaTDST, upwind convection
+ * with phi, laplacian of DnuTildaEff, Cb2 gradient term, production and
+ * implicit destruction), relax, solve by symmetric Gauss-Seidel sweeps to
+ * tolerance/relTol, bound, update the boundary values, mut and alphat.
+ * perf->initial_residual[0]/final_residual[0] and n_iterations describe the
+ * scalar solve.  rdst_iterate calls it after rdst_update_fields when a model
+ * is set (Single.C:102). */
+int rdst_turbulence_correct(rdst_case* c, rdst_solver_perf* perf);
+
+/* forceCoeffs-style integral over one patch: force = sum (p_b - p_ref) Sf
+ * + sum Sf & devRhoReff_b with devRhoReff = -muEff dev(twoSymm(grad U))
+ * (the viscous part when settings.viscous); out[0]=Cd, out[1]=Cl,
+ * out[2..4]=force, out[5..7]=its viscous part */
 int rdst_forces(rdst_case* c, int patch, const double* drag_dir, const double* lift_dir,
                 double p_ref, double rho_inf, double mag_u_inf, double a_ref, double* out);
 

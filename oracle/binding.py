@@ -13,7 +13,7 @@ import subprocess
 import numpy as np
 
 from rhodst_amd._abi import (MeshDesc, dptr, rdst_solver_perf, rdst_mesh_desc, rdst_thermo,
-                             rdst_bc, rdst_settings)
+                             rdst_bc, rdst_settings, rdst_turbulence, rdst_scalar_bc)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
@@ -53,6 +53,11 @@ def lib():
         L.orc_case_get.argtypes = [C.c_void_p, C.c_int, P(C.c_double)]
         L.orc_case_put.argtypes = [C.c_void_p, C.c_int, P(C.c_double)]
         L.orc_set_partition.argtypes = [C.c_void_p, P(C.c_int32)]
+        L.orc_set_turbulence.argtypes = [C.c_void_p, P(rdst_turbulence), P(rdst_scalar_bc), P(C.c_double),
+                                         P(C.c_double)]
+        L.orc_turbulence_correct.argtypes = [C.c_void_p, P(rdst_solver_perf)]
+        L.orc_forces.argtypes = [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double), C.c_double, C.c_double,
+                                 C.c_double, C.c_double, P(C.c_double)]
         _lib = L
     return _lib
 
@@ -116,6 +121,25 @@ class OracleCase:
                                      C.byref(spec.settings), C.byref(h)))
         self.h = h
         self.set_state(spec.p0, spec.T0, spec.U0)
+        if getattr(spec, "turbulence", None) is not None:
+            self.set_turbulence(*spec.turbulence)
+
+    def set_turbulence(self, turb, bcs, wall_dist, nu_tilda):
+        d = np.ascontiguousarray(wall_dist, dtype=np.float64)
+        nt = np.ascontiguousarray(nu_tilda, dtype=np.float64)
+        _check(lib().orc_set_turbulence(self.h, C.byref(turb), bcs, dptr(d), dptr(nt)))
+
+    def turbulence_correct(self):
+        perf = rdst_solver_perf()
+        _check(lib().orc_turbulence_correct(self.h, C.byref(perf)))
+        return perf
+
+    def forces(self, patch, drag_dir, lift_dir, p_ref, rho_inf, mag_u_inf, a_ref=1.0):
+        d = np.ascontiguousarray(drag_dir, dtype=np.float64)
+        l_ = np.ascontiguousarray(lift_dir, dtype=np.float64)
+        out = np.zeros(8)
+        _check(lib().orc_forces(self.h, patch, dptr(d), dptr(l_), p_ref, rho_inf, mag_u_inf, a_ref, dptr(out)))
+        return out
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -131,8 +155,10 @@ class OracleCase:
     def _size(self, field):
         if field in (12, 13, 14, 15):
             return self.nI + self.nB
-        if 16 <= field <= 23:
+        if 16 <= field <= 23 or field in (33, 34):
             return self.nB
+        if field in (36, 37):
+            return self.nI
         return self.n
 
     def get(self, field):

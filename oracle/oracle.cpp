@@ -239,6 +239,22 @@ struct BCState {
     std::vector<V3> refU;
 };
 
+// Spalart-Allmaras state (SAneg / SARC2 members nuTilda_, mut_, alphat_, d_)
+struct Turb {
+    rdst_turbulence t{};
+    bool on = false;
+    std::vector<rdst_scalar_bc> bc;          // 0/nuTilda per patch
+    std::vector<double> d;                   // wall distance, cells
+    std::vector<double> nut, bnut;           // nuTilda: cells, boundary faces
+    std::vector<double> mut, bmut, alphat, balphat;
+    std::vector<double> rhoPrev;             // rho before this step's updateFields
+    bool rhoPrevValid = false;
+    int nCorrect = 0;
+    // nuTilda matrix after relax(), solve form: diagonal incl. internalCoeffs,
+    // source incl. boundaryCoeffs
+    std::vector<double> Ad, Au, Al, As;
+};
+
 struct Case {
     Mesh m;
     rdst_thermo th;
@@ -272,7 +288,22 @@ struct Case {
     bool hasState = false, hasMatrix = false;
     // cell partition for the block-Jacobi ILU(0) (empty: one partition)
     std::vector<int> part;
+    Turb tb;
 };
+
+// muEff = mut + mu, alphaEff = alphat + alpha (RASModel::muEff/alphaEff);
+// laminar: mut = alphat = 0
+static void muAlphaEff(const Case& c, std::vector<double>& mu, std::vector<double>& bmu, std::vector<double>* al,
+                       std::vector<double>* bal) {
+    mu = c.mu; bmu = c.bmu;
+    if (al) *al = c.alpha;
+    if (bal) *bal = c.balpha;
+    if (!c.tb.on) return;
+    for (size_t i = 0; i < mu.size(); ++i) mu[i] = c.tb.mut[i] + c.mu[i];
+    for (size_t b = 0; b < bmu.size(); ++b) bmu[b] = c.tb.bmut[b] + c.bmu[b];
+    if (al) for (size_t i = 0; i < al->size(); ++i) (*al)[i] = c.tb.alphat[i] + c.alpha[i];
+    if (bal) for (size_t b = 0; b < bal->size(); ++b) (*bal)[b] = c.tb.balphat[b] + c.balpha[b];
+}
 
 // ------------------------------------------------------------------ thermo
 static inline double muOf(const Case& c, double T) {
@@ -848,7 +879,8 @@ static void viscousTerms(Case& c) {
     int n = m.nCells;
     std::vector<T9> g, gb;
     gradU(c, g, gb);
-    std::vector<double> muEff = c.mu, bmuEff = c.bmu;   // laminar: muEff = mu (+ mut when turbulent)
+    std::vector<double> muEff, bmuEff;
+    muAlphaEff(c, muEff, bmuEff, nullptr, nullptr);
     std::vector<T9> tau(n), btau(m.nB);
     for (int i = 0; i < n; ++i) tau[i] = tauMC(muEff[i], g[i]);
     for (int b = 0; b < m.nB; ++b) btau[b] = tauMC(bmuEff[b], gb[b]);
@@ -1002,8 +1034,10 @@ static void computeJacobian(Case& c) {
     for (int b = 0; b < m.nB; ++b) bcc[b] = std::sqrt(c.th.Cp / c.Cv / c.bpsi[b]);
     linearInterp(m, cc, bcc, cf);
     std::vector<double> muEffSf, alphaEffSf, rhoSf, trho(n), btrho(m.nB);
-    linearInterp(m, c.mu, c.bmu, muEffSf);
-    linearInterp(m, c.alpha, c.balpha, alphaEffSf);
+    std::vector<double> muE, bmuE, alE, balE;
+    muAlphaEff(c, muE, bmuE, &alE, &balE);
+    linearInterp(m, muE, bmuE, muEffSf);
+    linearInterp(m, alE, balE, alphaEffSf);
     for (int i = 0; i < n; ++i) trho[i] = c.psi[i] * c.p[i];       // thermo.rho() = psi*p
     for (int b = 0; b < m.nB; ++b) btrho[b] = c.bpsi[b] * c.bp[b];
     linearInterp(m, trho, btrho, rhoSf);
@@ -1467,6 +1501,9 @@ static void updateFields(Case& c) {
     Mesh& m = c.m;
     int n = m.nCells, nB = m.nB;
     double Cp = c.th.Cp, Cv = c.Cv, gamma = Cp / Cv;
+    // rho.oldTime() of the turbulence ddt: the density before this update
+    c.tb.rhoPrev = c.rho;
+    c.tb.rhoPrevValid = true;
     // old values the BCs read before thermo.correct()
     std::vector<double> pOld = c.p, TOld = c.T, psiOld = c.psi;
     std::vector<double> bpsiOld = c.bpsi, brhoOld = c.brho;
@@ -1694,6 +1731,7 @@ static double setCourant(Case& c) {
 }
 
 // Single::iterate (Single.C:80-104)
+static void saCorrect(Case& c, Perf* perf);
 static double iterate(Case& c, Perf* perfOut) {
     computeFlux(c);
     computeJacobian(c);
@@ -1701,7 +1739,459 @@ static double iterate(Case& c, Perf* perfOut) {
     Perf perf = solve(c);
     if (perfOut) *perfOut = perf;
     updateFields(c);
+    if (c.tb.on) saCorrect(c, nullptr);
     return setCourant(c);
+}
+
+
+// ======================================================================= Spalart-Allmaras
+// SAneg (libraries/turbulence/SAneg/SAneg.C) and SARC2
+// (libraries/turbulence/SARC2/SARC2.C) correct(), field-at-a-time.
+// foam-extend helpers restated: pow3(s) = s*sqr(s), pow6(s) = pow3(sqr(s)),
+// pos(s) = s >= 0, neg(s) = s < 0.
+static inline double pow3(double s) { return s * sqr(s); }
+static inline double pow6(double s) { return pow3(sqr(s)); }
+
+struct SAField {
+    std::vector<double> chi, fv1, fv2, fv3, Omega, Stilda, fw, ft2, fr1, Gamma, bGamma;
+};
+
+// chi = rho nuTilda / mu (SAneg.C:62-65), fv1 (:68-72), fv2 (:75-83), fv3 (:86-100)
+static inline double saFv1(const rdst_turbulence& t, double chi) {
+    double chi3 = pow3(chi);
+    return chi3 / (chi3 + pow3(t.Cv1));
+}
+static inline double saFv2(const rdst_turbulence& t, double chi) { return 1.0 / pow3(1.0 + chi / t.Cv2); }
+static inline double saFv3(const rdst_turbulence& t, double chi, double fv1) {
+    double cb = (1.0 / t.Cv2) * chi;
+    return (1.0 + chi * fv1) * (1.0 / t.Cv2) * (3.0 * (1.0 + cb) + sqr(cb)) / pow3(1.0 + cb);
+}
+// fw (SAneg.C:103-120; SARC2.C:103-118)
+static inline double saFw(const rdst_turbulence& t, double Stilda, double nut, double d) {
+    double r = std::min(nut / (std::max(Stilda, SMALL) * sqr(t.kappa * d)), 10.0);
+    double g = r + t.Cw2 * (pow6(r) - r);
+    return g * std::pow((1.0 + pow6(t.Cw3)) / (pow6(g) + pow6(t.Cw3)), 1.0 / 6.0);
+}
+static inline double saCw1(const rdst_turbulence& t) { return t.Cb1 / sqr(t.kappa) + (1.0 + t.Cb2) / t.sigma_nut; }
+// DnuTildaEff: SAneg (SAneg.C:375-393) (rho nuTilda fn + mu)/sigmaNut with
+// fn = pos(chi) + neg(chi)(Cn1 + chi^3)/(Cn1 - chi^3); SARC2 (SARC2.H:160-170)
+// rho nuTilda/sigmaNut + mu
+static inline double saGamma(const rdst_turbulence& t, double rho, double nut, double mu) {
+    if (t.model == RDST_TURB_SANEG) {
+        double chi = rho * nut / mu;
+        double p3 = std::pow(chi, 3);
+        double fn = pos0(chi) + neg(chi) * (t.Cn1 + p3) / (t.Cn1 - p3);
+        return (rho * nut * fn + mu) / t.sigma_nut;
+    }
+    return rho * nut / t.sigma_nut + mu;
+}
+// magSqr of skew(G) and symm(G), G[3i+j] = dU_j/dx_i
+static inline double magSqrSkew(const T9& g) {
+    double s = 0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) s += sqr(0.5 * (g.a[3 * i + j] - g.a[3 * j + i]));
+    return s;
+}
+static inline double magSqrSymm(const T9& g) {
+    double s = 0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) s += sqr(0.5 * (g.a[3 * i + j] + g.a[3 * j + i]));
+    return s;
+}
+// SARC2::fr1 (SARC2.C:121-135)
+static inline double saFr1(const rdst_turbulence& t, const T9& g) {
+    if (!t.rcm_correction) return 1.0;
+    double sqrS = 2.0 * magSqrSymm(g), sqrW = 2.0 * magSqrSkew(g);
+    double rStar = std::sqrt(sqrS / std::max(sqrW, SMALL));
+    double rrStar = std::sqrt(sqrW / std::max(sqrS, SMALL));
+    double rTilda = rrStar * (rrStar - 1.0);
+    return (1.0 + t.Cr1) * 2.0 * rStar / (1.0 + rStar) * (1.0 - t.Cr3 * std::atan(t.Cr2 * rTilda)) - t.Cr1;
+}
+
+// nuTilda boundary values from 0/nuTilda (fixedValue, zeroGradient,
+// symmetryPlane = zeroGradient for a scalar, inletOutlet on phi)
+static void saNutBoundary(Case& c) {
+    const Mesh& m = c.m;
+    Turb& tb = c.tb;
+    tb.bnut.resize(m.nB);
+    for (int b = 0; b < m.nB; ++b) {
+        const rdst_scalar_bc& bc = tb.bc[m.bPatch[b]];
+        double vI = tb.nut[m.bCell[b]];
+        switch (bc.type) {
+            case RDST_BC_FIXED_VALUE: tb.bnut[b] = bc.value; break;
+            case RDST_BC_INLET_OUTLET: {
+                double vf = 1.0 - fe_pos(c.phi[m.nInternal + b]);
+                tb.bnut[b] = vf * bc.value + (1.0 - vf) * vI;
+                break;
+            }
+            default: tb.bnut[b] = vI;
+        }
+    }
+}
+// mut on the boundary: 0 on wall patches (the low-Re wall), elsewhere the
+// calculated rho_b nuTilda_b |fv1(chi_b)|, bounded at 0
+static void saMutBoundary(Case& c) {
+    const Mesh& m = c.m;
+    Turb& tb = c.tb;
+    tb.bmut.resize(m.nB);
+    tb.balphat.resize(m.nB);
+    for (int b = 0; b < m.nB; ++b) {
+        double v = 0.0;
+        if (m.patches[m.bPatch[b]].type != RDST_PATCH_WALL) {
+            double chi = c.brho[b] * tb.bnut[b] / c.bmu[b];
+            v = std::max(std::fabs(saFv1(tb.t, chi)) * tb.bnut[b] * c.brho[b], 0.0);
+        }
+        tb.bmut[b] = v;
+        tb.balphat[b] = v / tb.t.Prt;
+    }
+}
+
+// Gauss linear gradient of a scalar with boundary values
+static void gradScalar(const Mesh& m, const std::vector<double>& v, const std::vector<double>& bv,
+                       std::vector<V3>& g) {
+    int n = m.nCells;
+    g.assign(n, v3(0, 0, 0));
+    for (int f = 0; f < m.nInternal; ++f) {
+        int o = m.owner[f], nb = m.neighbour[f];
+        double vf = m.w[f] * (v[o] - v[nb]) + v[nb];
+        V3 t = m.Sf[f] * vf;
+        g[o] = g[o] + t;
+        g[nb] = g[nb] - t;
+    }
+    for (int b = 0; b < m.nB; ++b) g[m.bCell[b]] = g[m.bCell[b]] + m.Sf[m.bFace[b]] * bv[b];
+    for (int i = 0; i < n; ++i) g[i] = g[i] / m.V[i];
+}
+
+// foam bound(vsf, vsf0): where min(vsf) < vsf0, replace
+// vsf = max(max(vsf, average(max(vsf, vsf0)) pos(vsf0 - vsf)), vsf0), with
+// average() the |Sf|-weighted mean of the linear face values over each
+// cell's faces; the caller re-evaluates the boundary values, then
+// boundary = max(boundary, vsf0)
+static bool saBound(const Mesh& m, std::vector<double>& v, const std::vector<double>& bv, double v0) {
+    double mn = 1e300;
+    for (double x : v) mn = std::min(mn, x);
+    for (double x : bv) mn = std::min(mn, x);
+    if (!(mn < v0)) return false;
+    int n = m.nCells;
+    std::vector<double> mc(n), num(n, 0.0), den(n, 0.0);
+    for (int i = 0; i < n; ++i) mc[i] = std::max(v[i], v0);
+    for (int f = 0; f < m.nInternal; ++f) {
+        int o = m.owner[f], nb = m.neighbour[f];
+        double vf = m.w[f] * (mc[o] - mc[nb]) + mc[nb];
+        num[o] += m.magSf[f] * vf; num[nb] += m.magSf[f] * vf;
+        den[o] += m.magSf[f]; den[nb] += m.magSf[f];
+    }
+    for (int b = 0; b < m.nB; ++b) {
+        int ci = m.bCell[b];
+        double ms = m.magSf[m.bFace[b]];
+        num[ci] += ms * std::max(bv[b], v0);
+        den[ci] += ms;
+    }
+    for (int i = 0; i < n; ++i) {
+        double avg = num[i] / den[i];
+        v[i] = std::max(std::max(v[i], avg * pos0(v0 - v[i])), v0);
+    }
+    return true;
+}
+
+// turbulence.correct(): SAneg.C:490-605 / SARC2.C:418-488
+static void saCorrect(Case& c, Perf* perfOut) {
+    const Mesh& m = c.m;
+    Turb& tb = c.tb;
+    const rdst_turbulence& t = tb.t;
+    const int n = m.nCells, nI = m.nInternal, nB = m.nB;
+    const bool isNeg = t.model == RDST_TURB_SANEG;
+    const std::vector<double>& rhoOld = (tb.nCorrect > 0 && tb.rhoPrevValid) ? tb.rhoPrev : c.rho;
+    const double Cw1 = saCw1(t);
+    // fields of the current state
+    std::vector<T9> g, gb;
+    gradU(c, g, gb);
+    std::vector<V3> gn;
+    gradScalar(m, tb.nut, tb.bnut, gn);
+    std::vector<double> fv1(n), diag(n), src(n);
+    std::vector<double> G(n), bG(nB);
+    for (int i = 0; i < n; ++i) G[i] = saGamma(t, c.rho[i], tb.nut[i], c.mu[i]);
+    for (int b = 0; b < nB; ++b) bG[b] = saGamma(t, c.brho[b], tb.bnut[b], c.bmu[b]);
+    // fvm::ddt(rho, nuTilda), steadyStateDST.C:230-255
+    for (int i = 0; i < n; ++i) {
+        diag[i] = c.rDeltaT[i] * c.rho[i] * m.V[i];
+        src[i] = c.rDeltaT[i] * rhoOld[i] * tb.nut[i] * m.V[i];
+    }
+    std::vector<double> up(nI, 0.0), lo(nI, 0.0), ic(nB, 0.0), bcf(nB, 0.0);
+    // fvm::div(phi, nuTilda) Gauss upwind: weights pos(phi) (>= 0)
+    for (int f = 0; f < nI; ++f) {
+        double phi = c.phi[f];
+        double w = fe_pos(phi);
+        double l = -w * phi, u = l + phi;
+        lo[f] += l; up[f] += u;
+        diag[m.owner[f]] -= l;
+        diag[m.neighbour[f]] -= u;
+    }
+    for (int b = 0; b < nB; ++b) {
+        const rdst_scalar_bc& bc = tb.bc[m.bPatch[b]];
+        double phib = c.phi[nI + b];
+        double vic, vbc;
+        switch (bc.type) {
+            case RDST_BC_FIXED_VALUE: vic = 0.0; vbc = bc.value; break;
+            case RDST_BC_INLET_OUTLET: {
+                double vf = 1.0 - fe_pos(phib);
+                vic = 1.0 - vf; vbc = vf * bc.value;
+                break;
+            }
+            default: vic = 1.0; vbc = 0.0;
+        }
+        ic[b] += phib * vic;
+        bcf[b] += -phib * vbc;
+    }
+    // - fvm::laplacian(DnuTildaEff, nuTilda), Gauss linear corrected
+    for (int f = 0; f < nI; ++f) {
+        int o = m.owner[f], nb = m.neighbour[f];
+        double gf = m.w[f] * (G[o] - G[nb]) + G[nb];
+        double gms = gf * m.magSf[f];
+        double coef = gms * m.dc[f];
+        up[f] -= coef; lo[f] -= coef;
+        diag[o] += coef; diag[nb] += coef;
+        V3 gfn = m.w[f] * (gn[o] - gn[nb]) + gn[nb];
+        double corr = gms * dot(m.corr[f], gfn);
+        src[o] += corr;
+        src[nb] -= corr;
+    }
+    for (int b = 0; b < nB; ++b) {
+        const rdst_scalar_bc& bc = tb.bc[m.bPatch[b]];
+        double gms = bG[b] * m.magSf[m.bFace[b]];
+        double dcb = m.dc[nI + b];
+        switch (bc.type) {
+            case RDST_BC_FIXED_VALUE: ic[b] += gms * dcb; bcf[b] += gms * dcb * bc.value; break;
+            case RDST_BC_INLET_OUTLET: {
+                double vf = 1.0 - fe_pos(c.phi[nI + b]);
+                ic[b] += gms * vf * dcb;
+                bcf[b] += gms * vf * dcb * bc.value;
+                break;
+            }
+            default: break;
+        }
+    }
+    // cell sources
+    for (int i = 0; i < n; ++i) {
+        const double V = m.V[i], rho = c.rho[i], nut = tb.nut[i], d = tb.d[i];
+        const double chi = rho * nut / c.mu[i];
+        fv1[i] = saFv1(t, chi);
+        const double fv2 = saFv2(t, chi);
+        const double Omega = std::sqrt(2.0) * std::sqrt(magSqrSkew(g[i]));
+        // - Cb2/sigmaNut rho magSqr(grad nuTilda)
+        src[i] += t.Cb2 / t.sigma_nut * rho * magSqr(gn[i]) * V;
+        if (isNeg) {
+            double Stilda;
+            if (t.stilda_modification) {
+                double Sbar = fv2 * nut / sqr(t.kappa * d);
+                double a = t.Cn2 * Omega + Sbar;
+                Stilda = Omega + pos0(a) * Sbar
+                       + neg(a) * (Omega * (sqr(t.Cn2) * Omega + t.Cn3 * Sbar))
+                             / std::max((t.Cn3 - 2.0 * t.Cn2) * Omega - Sbar, SMALL);
+            } else {
+                Stilda = saFv3(t, chi, fv1[i]) * std::sqrt(2.0) * std::sqrt(magSqrSkew(g[i])) + fv2 * nut / sqr(t.kappa * d);
+            }
+            const double ft2 = t.negative_nu_tilda ? t.Ct3 * std::exp(-t.Ct4 * sqr(chi)) : 0.0;
+            const double fw = saFw(t, Stilda, nut, d);
+            if (pos0(nut) > 0) {
+                src[i] += t.Cb1 * (1.0 - ft2) * rho * Stilda * nut * V;
+                diag[i] += (Cw1 * fw - t.Cb1 / sqr(t.kappa) * ft2) * nut * rho / sqr(d) * V;
+            } else {
+                src[i] += t.Cb1 * (1.0 - t.Ct3) * rho * Omega * nut * V;
+                diag[i] -= Cw1 * nut * rho / sqr(d) * V;
+            }
+        } else {
+            const double Stilda = saFv3(t, chi, fv1[i]) * std::sqrt(2.0) * std::sqrt(magSqrSkew(g[i]))
+                                + fv2 * nut / sqr(t.kappa * d);
+            const double fw = saFw(t, Stilda, nut, d);
+            src[i] += t.Cb1 * rho * Stilda * nut * saFr1(t, g[i]) * V;
+            diag[i] += Cw1 * fw * nut * rho / sqr(d) * V;
+        }
+    }
+    // nuTildaEqn.relax(): fvMatrix::relax(alpha)
+    const double alpha = t.relax;
+    if (alpha > 0) {
+        std::vector<double> D0 = diag, sumOff(n, 0.0);
+        for (int f = 0; f < nI; ++f) {
+            sumOff[m.owner[f]] += std::fabs(up[f]);
+            sumOff[m.neighbour[f]] += std::fabs(lo[f]);
+        }
+        for (int b = 0; b < nB; ++b) {
+            int ci = m.bCell[b];
+            double ic0 = ic[b];
+            ic[b] = std::fabs(ic[b]);
+            sumOff[ci] -= ic[b];
+            ic[b] /= alpha;
+            src[ci] += (ic[b] - ic0) * tb.nut[ci];
+        }
+        for (int i = 0; i < n; ++i) {
+            diag[i] = std::max(diag[i], sumOff[i]);
+            diag[i] /= alpha;
+        }
+        for (int i = 0; i < n; ++i) src[i] += (diag[i] - D0[i]) * tb.nut[i];
+    }
+    // solve form: addBoundaryDiag / addBoundarySource
+    for (int b = 0; b < nB; ++b) {
+        diag[m.bCell[b]] += ic[b];
+        src[m.bCell[b]] += bcf[b];
+    }
+    tb.Ad = diag; tb.Au = up; tb.Al = lo; tb.As = src;
+    // smoothSolver + symGaussSeidel; block-Jacobi over the cell partition
+    // (one partition = the serial sweep): couplings across partitions use
+    // the values at the start of each sweep
+    std::vector<int> rs(n + 1, 0), cc;
+    std::vector<double> cv;
+    {
+        std::vector<int> cnt(n, 0);
+        for (int f = 0; f < nI; ++f) { cnt[m.owner[f]]++; cnt[m.neighbour[f]]++; }
+        for (int i = 0; i < n; ++i) rs[i + 1] = rs[i] + cnt[i];
+        cc.resize(rs[n]); cv.resize(rs[n]);
+        std::vector<int> pos(rs.begin(), rs.end() - 1);
+        for (int f = 0; f < nI; ++f) {
+            int o = m.owner[f], nb = m.neighbour[f];
+            cc[pos[o]] = nb; cv[pos[o]++] = up[f];
+            cc[pos[nb]] = o; cv[pos[nb]++] = lo[f];
+        }
+    }
+    std::vector<double>& x = tb.nut;
+    auto residual = [&](std::vector<double>& Ax) {
+        Ax.assign(n, 0.0);
+        for (int i = 0; i < n; ++i) {
+            double s = diag[i] * x[i];
+            for (int k = rs[i]; k < rs[i + 1]; ++k) s += cv[k] * x[cc[k]];
+            Ax[i] = s;
+        }
+    };
+    std::vector<double> Ax;
+    residual(Ax);
+    double xRef = 0;
+    for (int i = 0; i < n; ++i) xRef += x[i];
+    xRef /= n;
+    double nf = 0;
+    for (int i = 0; i < n; ++i) {
+        double sumA = diag[i];
+        for (int k = rs[i]; k < rs[i + 1]; ++k) sumA += cv[k];
+        double tmp = sumA * xRef;
+        nf += std::fabs(Ax[i] - tmp) + std::fabs(src[i] - tmp);
+    }
+    nf += 1.0e-20;
+    auto resNorm = [&]() {
+        residual(Ax);
+        double r = 0;
+        for (int i = 0; i < n; ++i) r += std::fabs(src[i] - Ax[i]);
+        return r / nf;
+    };
+    Perf pf{};
+    pf.init[0] = resNorm();
+    pf.fin[0] = pf.init[0];
+    pf.nIter = 0;
+    auto conv = [&]() { return pf.fin[0] < t.tolerance || (t.rel_tol > 0 && pf.fin[0] < t.rel_tol * pf.init[0]); };
+    const std::vector<int>& part = c.part;
+    auto samePart = [&](int a, int b) { return part.empty() || part[a] == part[b]; };
+    if (!conv() || t.min_iter > 0) {
+        std::vector<double> x0, bp(n);
+        do {
+            x0 = x;
+            for (int i = 0; i < n; ++i) {
+                double s = src[i];
+                for (int k = rs[i]; k < rs[i + 1]; ++k)
+                    if (!samePart(i, cc[k])) s -= cv[k] * x0[cc[k]];
+                bp[i] = s;
+            }
+            for (int i = 0; i < n; ++i) {
+                double s = bp[i];
+                for (int k = rs[i]; k < rs[i + 1]; ++k)
+                    if (samePart(i, cc[k])) s -= cv[k] * x[cc[k]];
+                x[i] = s / diag[i];
+            }
+            for (int i = n - 1; i >= 0; --i) {
+                double s = bp[i];
+                for (int k = rs[i]; k < rs[i + 1]; ++k)
+                    if (samePart(i, cc[k])) s -= cv[k] * x[cc[k]];
+                x[i] = s / diag[i];
+            }
+            pf.fin[0] = resNorm();
+            pf.nIter++;
+        } while ((pf.nIter < t.max_iter && !conv()) || pf.nIter < t.min_iter);
+    }
+    pf.conv = conv();
+    // bound, boundary values, mut, alphat
+    if (!isNeg || !t.negative_nu_tilda) {
+        saNutBoundary(c);
+        if (saBound(m, x, tb.bnut, 0.0)) {
+            saNutBoundary(c);
+            for (double& v : tb.bnut) v = std::max(v, 0.0);
+        }
+    } else {
+        saNutBoundary(c);
+    }
+    for (int i = 0; i < n; ++i) tb.mut[i] = (isNeg ? std::fabs(fv1[i]) : fv1[i]) * x[i] * c.rho[i];
+    saMutBoundary(c);
+    if (isNeg && t.negative_nu_tilda) {
+        if (saBound(m, tb.mut, tb.bmut, 0.0)) saMutBoundary(c);
+    }
+    for (int i = 0; i < n; ++i) tb.alphat[i] = tb.mut[i] / t.Prt;
+    tb.nCorrect++;
+    if (perfOut) *perfOut = pf;
+}
+
+static void saSet(Case& c, const rdst_turbulence* t, const rdst_scalar_bc* bcs, const double* d, const double* nut) {
+    Turb& tb = c.tb;
+    const Mesh& m = c.m;
+    int n = m.nCells;
+    tb = Turb();
+    tb.t = *t;
+    tb.on = t->model != RDST_TURB_LAMINAR;
+    if (!tb.on) return;
+    if (!c.hasState) throw std::runtime_error("set_turbulence: no state");
+    tb.bc.assign(bcs, bcs + m.patches.size());
+    tb.d.assign(d, d + n);
+    tb.nut.assign(nut, nut + n);
+    tb.mut.assign(n, 0.0);
+    tb.alphat.assign(n, 0.0);
+    saNutBoundary(c);
+    const bool isNeg = t->model == RDST_TURB_SANEG;
+    for (int i = 0; i < n; ++i) {
+        double chi = c.rho[i] * tb.nut[i] / c.mu[i];
+        double f = saFv1(*t, chi);
+        tb.mut[i] = std::max((isNeg ? std::fabs(f) : f) * tb.nut[i] * c.rho[i], 0.0);
+        tb.alphat[i] = tb.mut[i] / t->Prt;
+    }
+    saMutBoundary(c);
+}
+
+// forceCoeffs: pressure and viscous force on one patch; devRhoReff =
+// -muEff dev(twoSymm(grad U)) with the corrected boundary gradient
+static void forces(const Case& c, int patch, const double* dragDir, const double* liftDir, double pRef,
+                   double rhoInf, double magUInf, double aRef, double* out) {
+    const Mesh& m = c.m;
+    std::vector<T9> g, gb;
+    gradU(c, g, gb);
+    std::vector<double> muE, bmuE;
+    muAlphaEff(c, muE, bmuE, nullptr, nullptr);
+    V3 Fp = v3(0, 0, 0), Fv = v3(0, 0, 0);
+    for (int b = 0; b < m.nB; ++b) {
+        if (m.bPatch[b] != patch) continue;
+        V3 S = m.Sf[m.bFace[b]];
+        Fp = Fp + S * (c.bp[b] - pRef);
+        if (c.st.viscous) {
+            const T9& G = gb[b];
+            double tr = G.a[0] + G.a[4] + G.a[8];
+            T9 D;
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    double ts = G.a[3 * i + j] + G.a[3 * j + i];
+                    D.a[3 * i + j] = -bmuE[b] * (ts - (i == j ? (2.0 / 3.0) * tr : 0.0));
+                }
+            Fv = Fv + dotVT(S, D);
+        }
+    }
+    V3 F = Fp + Fv;
+    double q = 0.5 * rhoInf * magUInf * magUInf * aRef;
+    out[0] = (F.x * dragDir[0] + F.y * dragDir[1] + F.z * dragDir[2]) / q;
+    out[1] = (F.x * liftDir[0] + F.y * liftDir[1] + F.z * liftDir[2]) / q;
+    out[2] = F.x; out[3] = F.y; out[4] = F.z;
+    out[5] = Fv.x; out[6] = Fv.y; out[7] = Fv.z;
 }
 
 }  // namespace orc
@@ -1877,6 +2367,29 @@ int orc_set_partition(void* h, const int32_t* part) {
     })
 }
 
+int orc_set_turbulence(void* h, const rdst_turbulence* t, const rdst_scalar_bc* bcs, const double* d,
+                       const double* nut) {
+    ORC_TRY({ saSet(*(Case*)h, t, bcs, d, nut); })
+}
+int orc_turbulence_correct(void* h, rdst_solver_perf* perf) {
+    ORC_TRY({
+        Case& c = *(Case*)h;
+        if (!c.tb.on) throw std::runtime_error("turbulence_correct: no model");
+        Perf p{};
+        saCorrect(c, &p);
+        if (perf) {
+            *perf = rdst_solver_perf{};
+            perf->initial_residual[0] = p.init[0];
+            perf->final_residual[0] = p.fin[0];
+            perf->n_iterations = p.nIter;
+            perf->converged = p.conv ? 1 : 0;
+        }
+    })
+}
+int orc_forces(void* h, int patch, const double* drag, const double* lift, double pRef, double rhoInf,
+               double magUInf, double aRef, double* out) {
+    ORC_TRY({ forces(*(Case*)h, patch, drag, lift, pRef, rhoInf, magUInf, aRef, out); })
+}
 int orc_case_get(void* h, int field, double* out) {
     ORC_TRY({
         Case& c = *(Case*)h;
@@ -1900,6 +2413,11 @@ int orc_case_get(void* h, int field, double* out) {
             case RDST_F_BH: putS(c.bh); break; case RDST_F_BPSI: putS(c.bpsi); break;
             case RDST_F_SOURCE: putS(c.source); break; case RDST_F_MU: putS(c.mu); break;
             case RDST_F_RHO_OLD: putS(c.rhoOld); break; case RDST_F_E_OLD: putS(c.eOld); break;
+            case RDST_F_NU_TILDA: putS(c.tb.nut); break; case RDST_F_MUT: putS(c.tb.mut); break;
+            case RDST_F_ALPHAT: putS(c.tb.alphat); break; case RDST_F_WALL_DIST: putS(c.tb.d); break;
+            case RDST_F_BNU_TILDA: putS(c.tb.bnut); break; case RDST_F_BMUT: putS(c.tb.bmut); break;
+            case RDST_F_SA_DIAG: putS(c.tb.Ad); break; case RDST_F_SA_UPPER: putS(c.tb.Au); break;
+            case RDST_F_SA_LOWER: putS(c.tb.Al); break; case RDST_F_SA_SOURCE: putS(c.tb.As); break;
             default: throw std::runtime_error("bad field id");
         }
         (void)m;
@@ -1916,6 +2434,11 @@ int orc_case_put(void* h, int field, const double* in) {
             case RDST_F_RHOU: for (int i = 0; i < n; ++i) c.rhoU[i] = v3(in[3 * i], in[3 * i + 1], in[3 * i + 2]); break;
             case RDST_F_W: c.W.assign(in, in + 5 * (size_t)n); break;
             case RDST_F_RDELTAT: c.rDeltaT.assign(in, in + n); break;
+            case RDST_F_NU_TILDA:
+                if (!c.tb.on) throw std::runtime_error("no turbulence model");
+                c.tb.nut.assign(in, in + n);
+                saNutBoundary(c);
+                break;
             default: throw std::runtime_error("field not writable");
         }
     })

@@ -63,6 +63,61 @@ class rdst_solver_perf(C.Structure):
                 ("n_iterations", C.c_int32), ("converged", C.c_int32)]
 
 
+class rdst_turbulence(C.Structure):
+    _fields_ = [("model", C.c_int32), ("negative_nu_tilda", C.c_int32), ("stilda_modification", C.c_int32),
+                ("rcm_correction", C.c_int32),
+                ("sigma_nut", C.c_double), ("kappa", C.c_double), ("Prt", C.c_double),
+                ("Cb1", C.c_double), ("Cb2", C.c_double), ("Cw2", C.c_double), ("Cw3", C.c_double),
+                ("Cv1", C.c_double), ("Cv2", C.c_double),
+                ("Cn1", C.c_double), ("Cn2", C.c_double), ("Cn3", C.c_double), ("Ct3", C.c_double),
+                ("Ct4", C.c_double),
+                ("Cr1", C.c_double), ("Cr2", C.c_double), ("Cr3", C.c_double),
+                ("relax", C.c_double), ("tolerance", C.c_double), ("rel_tol", C.c_double),
+                ("min_iter", C.c_int32), ("max_iter", C.c_int32)]
+
+
+class rdst_scalar_bc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("pad", C.c_int32), ("value", C.c_double)]
+
+
+TURB_MODELS = {"laminar": 0, "SAneg": 1, "SARC2": 2}
+
+
+def make_turbulence(model="SAneg", **kw) -> rdst_turbulence:
+    """RASProperties <model>Coeffs with the reference's defaults
+    (SAneg.C:168-307, SARC2.C:172-285) plus the nuTilda solver settings of
+    tutorials/RAE2822/system/fvSolution (symGaussSeidel, tolerance 1e-6,
+    relTol 0.1, minIter 1, relaxation 0.5)."""
+    t = rdst_turbulence()
+    t.model = TURB_MODELS[model] if isinstance(model, str) else int(model)
+    t.negative_nu_tilda = int(kw.get("negative_nu_tilda", 1))
+    t.stilda_modification = int(kw.get("stilda_modification", 1))
+    t.rcm_correction = int(kw.get("rcm_correction", 1))
+    defaults = dict(sigma_nut=0.66666, kappa=0.41, Prt=1.0, Cb1=0.1355, Cb2=0.622, Cw2=0.3, Cw3=2.0, Cv1=7.1,
+                    Cv2=5.0, Cn1=16.0, Cn2=0.7, Cn3=0.9, Ct3=1.2, Ct4=0.5, Cr1=1.0, Cr2=2.0, Cr3=1.0,
+                    relax=0.5, tolerance=1e-6, rel_tol=0.1)
+    for k, v in defaults.items():
+        setattr(t, k, kw.get(k, v))
+    t.min_iter = int(kw.get("min_iter", 1))
+    t.max_iter = int(kw.get("max_iter", 1000))
+    return t
+
+
+def make_scalar_bcs(mesh, spec: dict):
+    """spec: patch name -> (type, value) or type for 0/nuTilda; empty patches
+    default to empty."""
+    arr = (rdst_scalar_bc * len(mesh.patches))()
+    for i, (name, t, _s, _n) in enumerate(mesh.patches):
+        if t == 3:
+            arr[i].type = BC["empty"]
+            continue
+        d = spec[name]
+        typ, val = d if isinstance(d, tuple) else (d, 0.0)
+        arr[i].type = BC[typ]
+        arr[i].value = val
+    return arr
+
+
 def dptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 

@@ -12,8 +12,8 @@ import os
 
 import numpy as np
 
-from ._abi import (MeshDesc, dptr, iptr, rdst_bc, rdst_mesh_desc, rdst_settings, rdst_solver_perf,
-                   rdst_thermo)
+from ._abi import (MeshDesc, dptr, iptr, rdst_bc, rdst_mesh_desc, rdst_scalar_bc, rdst_settings,
+                   rdst_solver_perf, rdst_thermo, rdst_turbulence)
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "librhodst_amd.so")
 
@@ -24,7 +24,8 @@ EXPORTS = ["rdst_last_error", "rdst_version", "rdst_init", "rdst_mesh_create", "
            "rdst_mesh_info", "rdst_mesh_get_int", "rdst_mesh_get_geom", "rdst_case_create",
            "rdst_case_destroy", "rdst_case_set_state", "rdst_case_get", "rdst_case_put",
            "rdst_flux_residual", "rdst_jacobian_assemble", "rdst_block_solve", "rdst_update_fields",
-           "rdst_set_courant", "rdst_iterate", "rdst_forces", "rdst_last_flux_ms", "rdst_time_flux"]
+           "rdst_set_courant", "rdst_iterate", "rdst_case_set_turbulence", "rdst_turbulence_correct",
+           "rdst_forces", "rdst_last_flux_ms", "rdst_time_flux"]
 
 _lib = None
 
@@ -63,6 +64,9 @@ def lib():
     L.rdst_iterate.argtypes = [C.c_void_p, C.c_int, P(C.c_double), P(rdst_solver_perf)]
     L.rdst_forces.argtypes = [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double), C.c_double, C.c_double,
                               C.c_double, C.c_double, P(C.c_double)]
+    L.rdst_case_set_turbulence.argtypes = [C.c_void_p, P(rdst_turbulence), P(rdst_scalar_bc), P(C.c_double),
+                                           P(C.c_double)]
+    L.rdst_turbulence_correct.argtypes = [C.c_void_p, P(rdst_solver_perf)]
     L.rdst_last_flux_ms.argtypes = [C.c_void_p, P(C.c_double)]
     L.rdst_time_flux.argtypes = [C.c_void_p, C.c_int, P(C.c_double)]
     _lib = L
@@ -146,6 +150,18 @@ class DeviceCase:
         i = self.mesh.info
         self.n, self.nI, self.nB, self.nnzb = i["n_cells"], i["n_internal"], i["n_bfaces"], i["nnzb"]
         self.set_state(spec.p0, spec.T0, spec.U0)
+        if getattr(spec, "turbulence", None) is not None:
+            self.set_turbulence(*spec.turbulence)
+
+    def set_turbulence(self, turb, bcs, wall_dist, nu_tilda):
+        d = np.ascontiguousarray(wall_dist, dtype=np.float64)
+        nt = np.ascontiguousarray(nu_tilda, dtype=np.float64)
+        check(lib().rdst_case_set_turbulence(self.h, C.byref(turb), bcs, dptr(d), dptr(nt)))
+
+    def turbulence_correct(self):
+        perf = rdst_solver_perf()
+        check(lib().rdst_turbulence_correct(self.h, C.byref(perf)))
+        return perf
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -161,8 +177,10 @@ class DeviceCase:
     def _size(self, field):
         if field in (12, 13, 14, 15):
             return self.nI + self.nB
-        if 16 <= field <= 23:
+        if 16 <= field <= 23 or field in (33, 34):
             return self.nB
+        if field in (36, 37):
+            return self.nI
         if field == 24:
             return self.nnzb
         return self.n
@@ -205,7 +223,7 @@ class DeviceCase:
     def forces(self, patch, drag_dir, lift_dir, p_ref, rho_inf, mag_u_inf, a_ref=1.0):
         d = np.ascontiguousarray(drag_dir, dtype=np.float64)
         l_ = np.ascontiguousarray(lift_dir, dtype=np.float64)
-        out = np.zeros(5)
+        out = np.zeros(8)
         check(lib().rdst_forces(self.h, patch, dptr(d), dptr(l_), p_ref, rho_inf, mag_u_inf, a_ref, dptr(out)))
         return out
 

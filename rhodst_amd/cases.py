@@ -20,6 +20,9 @@ class CaseSpec:
     p0: np.ndarray
     T0: np.ndarray
     U0: np.ndarray
+    # Spalart-Allmaras: (rdst_turbulence, rdst_scalar_bc[n_patches], wall
+    # distance [n_cells], initial nuTilda [n_cells]) or None (laminar)
+    turbulence: tuple | None = None
 
 
 def forward_step_case(nx: int = 120, ny: int = 40, scheme: str = "AUSM+", perturb: float = 0.0,
@@ -150,3 +153,28 @@ def box_viscous_case(nx: int = 24, ny: int = 16, scheme: str = "HLLC", mach: flo
     kw.update(settings)
     return CaseSpec("boxViscous", mesh, make_thermo(Cp=Cp, R=R, Pr=0.72, sutherland=(1.458e-6, 110.4)),
                     make_bcs(mesh, spec), make_settings(**kw), p0, T0, np.ascontiguousarray(U0))
+
+
+def box_sa_case(model: str = "SAneg", scheme: str = "HLLC", nx: int = 24, ny: int = 16, negative_fraction: float = 0.1,
+                seed: int = 2, turb_kw: dict | None = None, **settings) -> CaseSpec:
+    """The no-slip-wall box of box_viscous_case with the Spalart-Allmaras model:
+    nuTilda fixedValue 0 on the wall, inletOutlet (inletValue 3 nu) on the far
+    field, wall distance to the bottom wall, and a seeded initial nuTilda of
+    about 3 nu with a fraction of the cells negative (the SAneg branch)."""
+    from ._abi import make_scalar_bcs, make_turbulence
+    spec = box_viscous_case(scheme=scheme, nx=nx, ny=ny, **settings)
+    mesh = spec.mesh
+    rng = np.random.default_rng(seed)
+    mu = 1.458e-6 * np.sqrt(300.0) / (1.0 + 110.4 / 300.0)
+    nu = mu / (1.0e5 / (287.0 * 300.0))
+    nut0 = 3.0 * nu * (1.0 + 0.5 * rng.uniform(-1, 1, mesh.n_cells))
+    if negative_fraction > 0:
+        k = rng.choice(mesh.n_cells, int(negative_fraction * mesh.n_cells), replace=False)
+        nut0[k] = -0.3 * nut0[k]
+    ff = ("inletOutlet", 3.0 * nu)
+    bcs = make_scalar_bcs(mesh, {"left": ff, "right": ff, "top": ff, "bottom": ("fixedValue", 0.0)})
+    d = meshgen.wall_distance(mesh, ["bottom"])
+    turb = make_turbulence(model, **(turb_kw or {}))
+    spec.name = f"boxSA_{model}"
+    spec.turbulence = (turb, bcs, d, nut0)
+    return spec

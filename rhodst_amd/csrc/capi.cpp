@@ -41,6 +41,8 @@ static void alloc_fields(Case& c) {
     c.src.alloc(5 * n);
     c.cellBuf.zero(); c.bndBuf.zero(); c.faceBuf.zero(); c.RBuf.zero(); c.WBuf.zero(); c.timeBuf.zero(); c.fHBuf.zero();
     c.gradBuf.zero();
+    c.turbBuf.alloc(2 * n + 2 * nB);
+    c.turbBuf.zero();
     RDST_HIP(hipDeviceSynchronize());
     Fields& f = c.f;
     double* p = c.cellBuf.p;
@@ -61,6 +63,8 @@ static void alloc_fields(Case& c) {
     p = c.timeBuf.p;
     f.rDeltaT = p; f.courant = p + n; f.rhoOld = p + 2 * n; f.eOld = p + 3 * n; f.dS = p + 4 * n;
     f.fH = c.fHBuf.p;
+    f.mut = c.turbBuf.p; f.alphat = c.turbBuf.p + n;
+    f.bmut = c.turbBuf.p + 2 * n; f.balphat = c.turbBuf.p + 2 * n + nB;
 }
 
 static void download_soa(const double* const* comps, int k, size_t n, double* out) {
@@ -315,6 +319,29 @@ int rdst_case_get(rdst_case* cc, int field, double* out) {
             case RDST_F_MU: one(f.mu, n); break;
             case RDST_F_RHO_OLD: one(f.rhoOld, n); break;
             case RDST_F_E_OLD: one(f.eOld, n); break;
+            case RDST_F_MUT: one(f.mut, n); break;
+            case RDST_F_ALPHAT: one(f.alphat, n); break;
+            case RDST_F_BMUT: one(f.bmut, nB); break;
+            case RDST_F_NU_TILDA: case RDST_F_WALL_DIST: case RDST_F_BNU_TILDA: case RDST_F_SA_DIAG:
+            case RDST_F_SA_UPPER: case RDST_F_SA_LOWER: case RDST_F_SA_SOURCE: {
+                const SaState& s = c->sa;
+                if (!s.on) throw ArgError("rdst_case_get: no turbulence model on the case", RDST_ERR_STATE);
+                if (field == RDST_F_NU_TILDA) one(s.nut, n);
+                else if (field == RDST_F_WALL_DIST) one(s.d, n);
+                else if (field == RDST_F_BNU_TILDA) one(s.bnut, nB);
+                else if (field == RDST_F_SA_SOURCE) one(s.B, n);
+                else {
+                    std::vector<double> A(m.nnzb);
+                    RDST_HIP(hipMemcpy(A.data(), s.A.p, A.size() * sizeof(double), hipMemcpyDeviceToHost));
+                    if (field == RDST_F_SA_DIAG)
+                        for (size_t i = 0; i < n; ++i) out[i] = A[m.slotDiag[i]];
+                    else if (field == RDST_F_SA_UPPER)
+                        for (int k = 0; k < m.nI; ++k) out[k] = A[m.slotUpper[k]];
+                    else
+                        for (int k = 0; k < m.nI; ++k) out[k] = A[m.slotLower[k]];
+                }
+                break;
+            }
             default: throw ArgError("rdst_case_get: unknown field");
         }
     })
@@ -333,6 +360,7 @@ int rdst_case_put(rdst_case* cc, int field, const double* in) {
             case RDST_F_RHOE: one(f.rhoE, n); break;
             case RDST_F_W: one(f.W, 5 * n); break;
             case RDST_F_RDELTAT: one(f.rDeltaT, n); break;
+            case RDST_F_NU_TILDA: sa_put_nut(*c, in); break;
             case RDST_F_RHOU: {
                 std::vector<double> t(n);
                 double* d[3] = {f.rhoUx, f.rhoUy, f.rhoUz};
@@ -351,7 +379,7 @@ int rdst_flux_residual(rdst_case* cc) {
     RDST_API_BODY({
         Case* c = reinterpret_cast<Case*>(cc);
         if (!c) throw ArgError("null case");
-        flux_residual(*c, true);
+        flux_residual(*c, STORE_ALL);
     })
 }
 
@@ -400,11 +428,12 @@ int rdst_iterate(rdst_case* cc, int n_steps, double* res_norm, rdst_solver_perf*
         if (!c) throw ArgError("null case");
         if (!c->hasState) throw ArgError("rdst_iterate: no state", RDST_ERR_STATE);
         for (int s = 0; s < n_steps; ++s) {
-            flux_launch(*c, false);
+            flux_launch(*c, c->sa.on ? STORE_PHI : STORE_BPHI);
             jacobian_assemble(*c);
             rdst_solver_perf p{};
             block_solve(*c, &p);
             update_fields(*c);
+            if (c->sa.on) sa_correct(*c, nullptr);
             double r = set_courant(*c);
             if (res_norm) res_norm[s] = r;
             if (perf) perf[s] = p;
@@ -421,6 +450,36 @@ int rdst_forces(rdst_case* cc, int patch, const double* drag_dir, const double* 
         const Mesh& m = *c->mesh;
         if (patch < 0 || patch >= m.nPatches) throw ArgError("rdst_forces: patch index out of range");
         forces(*c, patch, drag_dir, lift_dir, p_ref, rho_inf, mag_u_inf, a_ref, out);
+    })
+}
+
+int rdst_case_set_turbulence(rdst_case* cc, const rdst_turbulence* turb, const rdst_scalar_bc* nu_tilda_bcs,
+                             const double* wall_dist, const double* nu_tilda) {
+    RDST_API_BODY({
+        Case* c = reinterpret_cast<Case*>(cc);
+        if (!c || !turb) throw ArgError("rdst_case_set_turbulence: null argument");
+        if (turb->model != RDST_TURB_LAMINAR && turb->model != RDST_TURB_SANEG && turb->model != RDST_TURB_SARC2)
+            throw ArgError("rdst_case_set_turbulence: unknown RASModel id (valid: laminar SAneg SARC2)",
+                           RDST_ERR_UNKNOWN_NAME);
+        if (turb->model != RDST_TURB_LAMINAR) {
+            if (!nu_tilda_bcs || !wall_dist || !nu_tilda) throw ArgError("rdst_case_set_turbulence: null array");
+            if (!c->hasState) throw ArgError("rdst_case_set_turbulence: no state (call rdst_case_set_state)", RDST_ERR_STATE);
+            if (!c->st.viscous) throw ArgError("rdst_case_set_turbulence: the viscous terms are off (settings.viscous)");
+            if (!(turb->sigma_nut > 0) || !(turb->kappa > 0) || !(turb->Prt > 0) || !(turb->Cv2 > 0))
+                throw ArgError("rdst_case_set_turbulence: sigmaNut, kappa, Prt and Cv2 must be > 0");
+            if (turb->relax > 1.0) throw ArgError("rdst_case_set_turbulence: relaxation factor > 1");
+            if (turb->max_iter < 1) throw ArgError("rdst_case_set_turbulence: max_iter < 1");
+        }
+        sa_set(*c, turb, nu_tilda_bcs, wall_dist, nu_tilda);
+    })
+}
+
+int rdst_turbulence_correct(rdst_case* cc, rdst_solver_perf* perf) {
+    RDST_API_BODY({
+        Case* c = reinterpret_cast<Case*>(cc);
+        if (!c) throw ArgError("null case");
+        sa_correct(*c, perf);
+        RDST_HIP(hipStreamSynchronize(c->stream));
     })
 }
 
@@ -442,26 +501,4 @@ int rdst_time_flux(rdst_case* cc, int n, double* ms_per_call) {
 
 }  // extern "C"
 
-namespace rdst {
 
-// forceCoeffs restated for the pressure part: F = sum_patch (p_b - pRef) Sf,
-// Cd = F.dragDir / (0.5 rhoInf |UInf|^2 Aref), Cl likewise (p in Pa).
-void forces(Case& c, int patch, const double* dragDir, const double* liftDir, double pRef, double rhoInf, double magUInf,
-            double aRef, double* out) {
-    const Mesh& m = *c.mesh;
-    RDST_HIP(hipStreamSynchronize(c.stream));
-    std::vector<double> bp(m.nB);
-    if (m.nB) RDST_HIP(hipMemcpy(bp.data(), c.f.bp, m.nB * sizeof(double), hipMemcpyDeviceToHost));
-    double F[3] = {0, 0, 0};
-    for (int b = 0; b < m.nB; ++b) {
-        if (m.bPatch[b] != patch) continue;
-        const int k = m.nI + b;
-        for (int d = 0; d < 3; ++d) F[d] += (bp[b] - pRef) * m.Sf[3 * (size_t)k + d];
-    }
-    const double q = 0.5 * rhoInf * magUInf * magUInf * aRef;
-    out[0] = (F[0] * dragDir[0] + F[1] * dragDir[1] + F[2] * dragDir[2]) / q;
-    out[1] = (F[0] * liftDir[0] + F[1] * liftDir[1] + F[2] * liftDir[2]) / q;
-    out[2] = F[0]; out[3] = F[1]; out[4] = F[2];
-}
-
-}  // namespace rdst

@@ -14,13 +14,14 @@
 // value, nei_=-1 -> neighbour value (baseFluxDST.C:144-171); boundary faces
 // take the patch value on both sides.
 #include "rdst_internal.hpp"
+#include "reduce.hpp"
 
-// Floating-point contraction only inside one expression: a product rounded in
-// one statement must not be re-fused into a later subtraction, or x*y - x*y
-// (the same state on both sides of a boundary face) leaves the rounding error
-// of x*y instead of the exact zero the reference computes, and the sign tests
-// of the far-field regimes (mach <= -1, < 0, >= 0, >= 1) then flip on it.
-#pragma clang fp contract(on)
+// No floating-point contraction: the reference evaluates every product
+// rounded, so a*x - a*x cancels exactly (a tangential state on both sides of
+// a boundary face gives phi = 0).  Fused into fma(a, x, -(a*x)) it leaves the
+// rounding error of a*x instead, and the sign tests that read phi (far-field
+// regimes mach <= -1, < 0, >= 0, >= 1; inletOutlet; upwind weights) flip on it.
+#pragma clang fp contract(off)
 
 namespace rdst {
 
@@ -384,7 +385,7 @@ __device__ inline Side load_bnd(const Fields& F, int b, double gamma) {
 }
 
 // K1: face flux + residual, one thread per cell.
-template <int S, bool STORE, bool VISC>
+template <int S, int STORE, bool VISC>
 __global__ __launch_bounds__(256) void k_flux_residual(MeshDev m, Fields F, const BcPatchDev* bc, double gamma, int lowMach,
                                                        double Mainf) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -396,7 +397,7 @@ __global__ __launch_bounds__(256) void k_flux_residual(MeshDev m, Fields F, cons
     double lx = 0.0, ly = 0.0, lz = 0.0, tx = 0.0, ty = 0.0, tz = 0.0, sd = 0.0;
     double gC[9], tauC[9], muC = 0.0, uCx = 0.0, uCy = 0.0, uCz = 0.0;
     if (VISC) {
-        muC = F.mu[c];
+        muC = F.mut[c] + F.mu[c];
         uCx = F.Ux[c]; uCy = F.Uy[c]; uCz = F.Uz[c];
         for (int q = 0; q < 9; ++q) gC[q] = F.gradU[(size_t)q * m.nC + c];
         tau_mc(muC, gC, tauC);
@@ -435,18 +436,19 @@ __global__ __launch_bounds__(256) void k_flux_residual(MeshDev m, Fields F, cons
                 const double nx = sx * rms, ny = sy * rms, nz = sz * rms;
                 double snx, sny, snz;
                 sngrad_u_bnd(bc[m.bpatch[idx]], F, idx, nx, ny, nz, m.ellDc[e], uCx, uCy, uCz, snx, sny, snz);
-                const double mg = F.bmu[idx] * ms;
+                const double bmuE = F.bmut[idx] + F.bmu[idx];
+                const double mg = bmuE * ms;
                 lfx = mg * snx; lfy = mg * sny; lfz = mg * snz;
                 double gb[9], tb[9];
                 for (int q = 0; q < 9; ++q) gb[q] = F.bgradU[(size_t)q * m.nB + idx];
-                tau_mc(F.bmu[idx], gb, tb);
+                tau_mc(bmuE, gb, tb);
                 tfx = sx * tb[0] + sy * tb[3] + sz * tb[6];
                 tfy = sx * tb[1] + sy * tb[4] + sz * tb[7];
                 tfz = sx * tb[2] + sy * tb[5] + sz * tb[8];
             } else {
                 const bool own = kind == ELL_OWNER;
                 const double w = m.ellWt[e];
-                const double muO = F.mu[idx];
+                const double muO = F.mut[idx] + F.mu[idx];
                 double gO[9], tauO[9];
                 for (int q = 0; q < 9; ++q) gO[q] = F.gradU[(size_t)q * m.nC + idx];
                 tau_mc(muO, gO, tauO);
@@ -486,10 +488,12 @@ __global__ __launch_bounds__(256) void k_flux_residual(MeshDev m, Fields F, cons
             a0 -= o.phi; a1 -= o.fx; a2 -= o.fy; a3 -= o.fz; a4 -= o.fE;
         } else {
             a0 += o.phi; a1 += o.fx; a2 += o.fy; a3 += o.fz; a4 += o.fE;
-            if (STORE) {
+            if (STORE == STORE_ALL) {
                 const int fi = m.ellFace[e];
                 F.phi[fi] = o.phi; F.phiUx[fi] = o.fx; F.phiUy[fi] = o.fy; F.phiUz[fi] = o.fz; F.phiE[fi] = o.fE;
                 F.Upx[fi] = o.upx; F.Upy[fi] = o.upy; F.Upz[fi] = o.upz;
+            } else if (STORE == STORE_PHI) {
+                F.phi[m.ellFace[e]] = o.phi;  // the turbulence convection reads phi on every face
             } else if (kind == ELL_BOUNDARY) {
                 F.phi[m.nI + idx] = o.phi;  // boundary mass flux feeds the BC updates
             }
@@ -530,34 +534,73 @@ __global__ void k_hllcp_sensor(MeshDev m, Fields F) {
     F.fH[c] = fc * fc * fc;
 }
 
-template <int S>
-void launch_flux(Case& c, bool store) {
+// forceCoeffs on one patch: sum (p_b - pRef) Sf and sum Sf & devRhoReff_b,
+// devRhoReff = -muEff dev(twoSymm(grad U)) from the corrected boundary
+// gradient; one block, fixed strided order (deterministic)
+__global__ __launch_bounds__(RED_BS) void k_forces(MeshDev m, Fields F, int b0, int b1, double pRef, int visc,
+                                                   double* out) {
+    __shared__ double sh[16];
+    double a[6] = {0, 0, 0, 0, 0, 0};
+    for (int b = b0 + (int)threadIdx.x; b < b1; b += blockDim.x) {
+        const int k = m.nI + b;
+        const double S[3] = {m.Sf[k], m.Sf[m.nF + k], m.Sf[2 * m.nF + k]};
+        const double dp = F.bp[b] - pRef;
+        for (int d = 0; d < 3; ++d) a[d] += S[d] * dp;
+        if (visc) {
+            double G[9];
+            for (int q = 0; q < 9; ++q) G[q] = F.bgradU[(size_t)q * m.nB + b];
+            const double tr = G[0] + G[4] + G[8];
+            const double muE = F.bmut[b] + F.bmu[b];
+            for (int j = 0; j < 3; ++j) {
+                double v = 0.0;
+                for (int i = 0; i < 3; ++i) {
+                    const double ts = G[3 * i + j] + G[3 * j + i];
+                    v += S[i] * (-muE * (ts - (i == j ? (2.0 / 3.0) * tr : 0.0)));
+                }
+                a[3 + j] += v;
+            }
+        }
+    }
+    for (int q = 0; q < 6; ++q) {
+        const double t = block_sum(a[q], sh);
+        if (threadIdx.x == 0) out[q] = t;
+    }
+}
+
+template <int S, int STORE>
+void launch_flux_mode(Case& c) {
     MeshDev md = c.meshDev();
     const int bs = 256;
     const int nb = blocks_for(md.nC, bs);
     const BcPatchDev* bc = c.d_bc.p;
-    if (c.st.viscous) {
-        hipLaunchKernelGGL(k_grad_u, dim3(nb), dim3(bs), 0, c.stream, md, c.f);
-        if (md.nB) hipLaunchKernelGGL(k_grad_u_bnd, dim3(blocks_for(md.nB, bs)), dim3(bs), 0, c.stream, md, c.f, bc);
-        if (store)
-            hipLaunchKernelGGL((k_flux_residual<S, true, true>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, bc, c.th.gamma,
-                               c.st.low_mach, c.st.Mainf);
-        else
-            hipLaunchKernelGGL((k_flux_residual<S, false, true>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, bc,
-                               c.th.gamma, c.st.low_mach, c.st.Mainf);
-        return;
-    }
-    if (store)
-        hipLaunchKernelGGL((k_flux_residual<S, true, false>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, bc, c.th.gamma,
+    if (c.st.viscous)
+        hipLaunchKernelGGL((k_flux_residual<S, STORE, true>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, bc, c.th.gamma,
                            c.st.low_mach, c.st.Mainf);
     else
-        hipLaunchKernelGGL((k_flux_residual<S, false, false>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, bc, c.th.gamma,
+        hipLaunchKernelGGL((k_flux_residual<S, STORE, false>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, bc, c.th.gamma,
                            c.st.low_mach, c.st.Mainf);
+}
+
+template <int S>
+void launch_flux(Case& c, int store) {
+    if (c.st.viscous) grad_u_launch(c);
+    switch (store) {
+        case STORE_ALL: launch_flux_mode<S, STORE_ALL>(c); break;
+        case STORE_PHI: launch_flux_mode<S, STORE_PHI>(c); break;
+        default: launch_flux_mode<S, STORE_BPHI>(c);
+    }
 }
 
 }  // namespace
 
-void flux_launch(Case& c, bool store) {
+void grad_u_launch(Case& c) {
+    MeshDev md = c.meshDev();
+    const int bs = 256;
+    hipLaunchKernelGGL(k_grad_u, dim3(blocks_for(md.nC, bs)), dim3(bs), 0, c.stream, md, c.f);
+    if (md.nB) hipLaunchKernelGGL(k_grad_u_bnd, dim3(blocks_for(md.nB, bs)), dim3(bs), 0, c.stream, md, c.f, c.d_bc.p);
+}
+
+void flux_launch(Case& c, int store) {
     MeshDev md = c.meshDev();
     if (c.st.flux_scheme == RDST_FLUX_HLLCP)
         hipLaunchKernelGGL(k_hllcp_sensor, dim3(blocks_for(md.nC, 256)), dim3(256), 0, c.stream, md, c.f);
@@ -573,7 +616,7 @@ void flux_launch(Case& c, bool store) {
     RDST_HIP(hipGetLastError());
 }
 
-void flux_residual(Case& c, bool storeFaceFluxes) {
+void flux_residual(Case& c, int storeFaceFluxes) {
     if (!c.hasState) throw ArgError("rdst_flux_residual: no state (call rdst_case_set_state)", RDST_ERR_STATE);
     RDST_HIP(hipEventRecord(c.ev0, c.stream));
     flux_launch(c, storeFaceFluxes);
@@ -584,14 +627,43 @@ void flux_residual(Case& c, bool storeFaceFluxes) {
 
 double flux_time(Case& c, int n) {
     if (!c.hasState) throw ArgError("rdst_time_flux: no state", RDST_ERR_STATE);
-    flux_launch(c, false);  // warm-up
+    flux_launch(c, STORE_BPHI);  // warm-up
     RDST_HIP(hipEventRecord(c.ev0, c.stream));
-    for (int i = 0; i < n; ++i) flux_launch(c, false);
+    for (int i = 0; i < n; ++i) flux_launch(c, STORE_BPHI);
     RDST_HIP(hipEventRecord(c.ev1, c.stream));
     RDST_HIP(hipStreamSynchronize(c.stream));
     float ms = 0.f;
     RDST_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
     return (double)ms / n;
+}
+
+void forces(Case& c, int patch, const double* dragDir, const double* liftDir, double pRef, double rhoInf, double magUInf,
+            double aRef, double* out) {
+    const Mesh& m = *c.mesh;
+    // boundary faces of the patch are contiguous in the compacted boundary order
+    int b0 = -1, b1 = -1;
+    for (int b = 0; b < m.nB; ++b)
+        if (m.bPatch[b] == patch) {
+            if (b0 < 0) b0 = b;
+            b1 = b + 1;
+        }
+    double F[6] = {0, 0, 0, 0, 0, 0};
+    if (b0 >= 0) {
+        if (c.st.viscous) grad_u_launch(c);
+        DBuf<double> d;
+        d.alloc(6);
+        hipLaunchKernelGGL(k_forces, dim3(1), dim3(RED_BS), 0, c.stream, c.meshDev(), c.f, b0, b1, pRef,
+                           c.st.viscous, d.p);
+        RDST_HIP(hipGetLastError());
+        RDST_HIP(hipMemcpyAsync(F, d.p, sizeof(F), hipMemcpyDeviceToHost, c.stream));
+        RDST_HIP(hipStreamSynchronize(c.stream));
+    }
+    const double Ft[3] = {F[0] + F[3], F[1] + F[4], F[2] + F[5]};
+    const double q = 0.5 * rhoInf * magUInf * magUInf * aRef;
+    out[0] = (Ft[0] * dragDir[0] + Ft[1] * dragDir[1] + Ft[2] * dragDir[2]) / q;
+    out[1] = (Ft[0] * liftDir[0] + Ft[1] * liftDir[1] + Ft[2] * liftDir[2]) / q;
+    out[2] = Ft[0]; out[3] = Ft[1]; out[4] = Ft[2];
+    out[5] = F[3]; out[6] = F[4]; out[7] = F[5];
 }
 
 }  // namespace rdst

@@ -32,7 +32,7 @@ __global__ __launch_bounds__(128) void k_assemble(MeshDev m, Fields F, Thermo th
     const double phiJ = 0.5 * (gam - 1) * (ux * ux + uy * uy + uz * uz);
     const double a1 = gam * rhoE / rho - phiJ;
     const double cc = sqrt(th.Cp / th.Cv / F.psi[c]);
-    const double muC = F.mu[c], alC = F.alpha[c], trC = F.psi[c] * F.p[c];
+    const double muC = F.mut[c] + F.mu[c], alC = F.alphat[c] + F.alpha[c], trC = F.psi[c] * F.p[c];
     double fd[25];
     for (int q = 0; q < 25; ++q) fd[q] = 0.0;
     const size_t E = (size_t)m.ellW * m.nC;
@@ -124,8 +124,9 @@ __global__ __launch_bounds__(128) void k_assemble(MeshDev m, Fields F, Thermo th
         const double Px = own ? ux : oux, Py = own ? uy : ouy, Pz = own ? uz : ouz, Nx = own ? oux : ux, Ny = own ? ouy : uy,
                      Nz = own ? ouz : uz;
         const double Pc = own ? cc : occ, Nc = own ? occ : cc;
-        const double Pmu = own ? muC : F.mu[idx], Nmu = own ? F.mu[idx] : muC;
-        const double Pal = own ? alC : F.alpha[idx], Nal = own ? F.alpha[idx] : alC;
+        const double omu = F.mut[idx] + F.mu[idx], oal = F.alphat[idx] + F.alpha[idx];
+        const double Pmu = own ? muC : omu, Nmu = own ? omu : muC;
+        const double Pal = own ? alC : oal, Nal = own ? oal : alC;
         const double otr = F.psi[idx] * F.p[idx];
         const double Ptr = own ? trC : otr, Ntr = own ? otr : trC;
         const double Ufx = w * (Px - Nx) + Nx, Ufy = w * (Py - Ny) + Ny, Ufz = w * (Pz - Nz) + Nz;

@@ -171,6 +171,25 @@ struct Fields {
     // time control
     double *rDeltaT, *courant, *rhoOld, *eOld, *dS;  // dS [3][nC]
     double* fH;      // HLLCP pressure-sensor cell values pow3(f) [nC]
+    // turbulence viscosity and diffusivity (zero when laminar): muEff = mut + mu,
+    // alphaEff = alphat + alpha
+    double *mut, *alphat, *bmut, *balphat;
+};
+
+// Spalart-Allmaras state of a case (SAneg/SARC2 nuTilda_, mut_, alphat_, d_)
+struct SaState {
+    bool on = false;
+    rdst_turbulence t{};
+    std::vector<rdst_scalar_bc> bc;
+    DBuf<rdst_scalar_bc> d_bc;
+    DBuf<double> buf;      // nut, d, rhoPrev, fv1o, B, x2 [n each], gn [3][n], bnut [nB]
+    DBuf<double> A;        // nuTilda matrix values per block-CSR slot [nnzb]
+    DBuf<double> part;     // reduction partials
+    DBuf<double> scal;     // device scalars
+    double *nut = nullptr, *d = nullptr, *rhoPrev = nullptr, *fv1o = nullptr, *B = nullptr, *x2 = nullptr,
+           *gn = nullptr, *bnut = nullptr;
+    bool rhoPrevValid = false;
+    int nCorrect = 0;
 };
 
 struct MeshDev {
@@ -195,7 +214,7 @@ struct Case {
     float lastFluxMs = 0.f;
 
     // field storage
-    DBuf<double> cellBuf, bndBuf, faceBuf, RBuf, WBuf, timeBuf, fHBuf, gradBuf;
+    DBuf<double> cellBuf, bndBuf, faceBuf, RBuf, WBuf, timeBuf, fHBuf, gradBuf, turbBuf;
     Fields f{};
     // matrix
     DBuf<double> A64, src;          // [nnzb][25], [nC][5]
@@ -208,13 +227,17 @@ struct Case {
     // time-control scalars
     double residual = -1, residualRatio = 1, courantScalar = 1;
     std::vector<double> hostScratch;
+    SaState sa;
 
     MeshDev meshDev() const;
 };
 
 // kernels / steps (defined in the .hip units)
-void flux_residual(Case& c, bool storeFaceFluxes);
-void flux_launch(Case& c, bool storeFaceFluxes);
+// face-field store modes of the face loop
+enum { STORE_BPHI = 0 /* boundary phi only */, STORE_PHI = 1 /* phi on every face */, STORE_ALL = 2 };
+void flux_residual(Case& c, int storeMode);
+void flux_launch(Case& c, int storeMode);
+void grad_u_launch(Case& c);
 void jacobian_assemble(Case& c);
 void block_solve(Case& c, rdst_solver_perf* perf);
 void update_fields(Case& c);
@@ -225,6 +248,11 @@ void solver_free(Case& c);
 double flux_time(Case& c, int n);
 void forces(Case& c, int patch, const double* dragDir, const double* liftDir, double pRef, double rhoInf,
             double magUInf, double aRef, double* out);
+
+void sa_set(Case& c, const rdst_turbulence* t, const rdst_scalar_bc* bcs, const double* wallDist,
+            const double* nuTilda);
+void sa_correct(Case& c, rdst_solver_perf* perf);
+void sa_put_nut(Case& c, const double* nuTilda);
 
 inline int blocks_for(long n, int bs) { return (int)((n + bs - 1) / bs); }
 

@@ -22,11 +22,9 @@
 #include <vector>
 
 #include "rdst_internal.hpp"
+#include "reduce.hpp"
 
 namespace rdst {
-
-constexpr int RED_BS = 256;     // threads per reduction block
-constexpr int RED_MAXG = 1024;  // max blocks of a reduction kernel
 
 struct SolverWork {
     int n = 0, n5 = 0, nDirs = 0, G = 0;
@@ -53,30 +51,6 @@ enum {
 namespace {
 
 // ---------------------------------------------------------------- reductions
-__device__ inline double block_sum(double v, double* sh) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) sh[wv] = v;
-    __syncthreads();
-    double t = 0.0;
-    if (threadIdx.x == 0)
-        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += sh[k];
-    __syncthreads();
-    return t;  // valid in thread 0
-}
-
-// every block sums the G partials of the previous kernel in the same order
-__device__ inline double reduce_partials(const double* part, int G, double* sh) {
-    double v = 0.0;
-    for (int i = threadIdx.x; i < G; i += blockDim.x) v += part[i];
-    double t = block_sum(v, sh);
-    if (threadIdx.x == 0) sh[8] = t;
-    __syncthreads();
-    double r = sh[8];
-    __syncthreads();
-    return r;
-}
-
 template <class TA, class TB>
 __global__ __launch_bounds__(RED_BS) void k_dot_partial(long n, const TA* a, const TB* b, double* part) {
     __shared__ double sh[16];

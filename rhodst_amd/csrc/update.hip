@@ -18,12 +18,12 @@
 
 #include "rdst_internal.hpp"
 
-// Floating-point contraction only inside one expression: a product rounded in
-// one statement must not be re-fused into a later subtraction, or x*y - x*y
-// (the same state on both sides of a boundary face) leaves the rounding error
-// of x*y instead of the exact zero the reference computes, and the sign tests
-// of the far-field regimes (mach <= -1, < 0, >= 0, >= 1) then flip on it.
-#pragma clang fp contract(on)
+// No floating-point contraction: the reference evaluates every product
+// rounded, so a*x - a*x cancels exactly (a tangential state on both sides of
+// a boundary face gives phi = 0).  Fused into fma(a, x, -(a*x)) it leaves the
+// rounding error of a*x instead, and the sign tests that read phi (far-field
+// regimes mach <= -1, < 0, >= 0, >= 1; inletOutlet; upwind weights) flip on it.
+#pragma clang fp contract(off)
 
 namespace rdst {
 
@@ -459,6 +459,11 @@ void update_fields(Case& c) {
     MeshDev md = c.meshDev();
     const int bs = 256;
     const rdst_settings& s = c.st;
+    if (c.sa.on) {
+        // rho.oldTime() of the turbulence ddt: the density before this update
+        RDST_HIP(hipMemcpyAsync(c.sa.rhoPrev, c.f.rho, sizeof(double) * m.nC, hipMemcpyDeviceToDevice, c.stream));
+        c.sa.rhoPrevValid = true;
+    }
     if (m.nB)
         hipLaunchKernelGGL(k_update_bnd_pre, dim3(blocks_for(m.nB, bs)), dim3(bs), 0, c.stream, md, c.f, c.th, c.d_bc.p,
                            s.rho_min, s.rho_max);

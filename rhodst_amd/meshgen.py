@@ -520,3 +520,105 @@ def swept_wing(n_wake: int = 48, n_surf: int = 128, n_normal: int = 96, n_span_w
     mesh = hex_polymesh(points, hexes, classify,
                         [("wing", PATCH_WALL), ("symmetry", PATCH_SYMMETRY), ("farfield", PATCH_GENERIC)])
     return mesh
+
+
+def _face_triangles(mesh: PolyMesh, faces: np.ndarray):
+    """Fan triangles (p0, p_k, p_k+1) of the given faces: arrays [nT,3] of
+    corner coordinates and the owning face of each triangle."""
+    off = mesh.face_offsets
+    A, B, Cc, owner = [], [], [], []
+    for f in faces:
+        pts = mesh.points[mesh.face_points[off[f]:off[f + 1]]]
+        for k in range(1, len(pts) - 1):
+            A.append(pts[0]); B.append(pts[k]); Cc.append(pts[k + 1]); owner.append(f)
+    return np.array(A), np.array(B), np.array(Cc), np.array(owner)
+
+
+def _point_triangle_distance(P, A, B, Cc):
+    """Exact distance from points P [n,3] to triangles (A,B,C) [n,3] each
+    (Ericson, Real-Time Collision Detection, 5.1.5, closest point on a
+    triangle), vectorised."""
+    ab, ac, ap = B - A, Cc - A, P - A
+    d1 = np.einsum("ij,ij->i", ab, ap); d2 = np.einsum("ij,ij->i", ac, ap)
+    bp = P - B
+    d3 = np.einsum("ij,ij->i", ab, bp); d4 = np.einsum("ij,ij->i", ac, bp)
+    cp = P - Cc
+    d5 = np.einsum("ij,ij->i", ab, cp); d6 = np.einsum("ij,ij->i", ac, cp)
+    va = d3 * d6 - d5 * d4
+    vb = d5 * d2 - d1 * d6
+    vc = d1 * d4 - d3 * d2
+    with np.errstate(divide="ignore", invalid="ignore"):
+        denom = 1.0 / (va + vb + vc)
+        v = vb * denom
+        w = vc * denom
+        Q = A + ab * v[:, None] + ac * w[:, None]                      # interior
+        # vertex and edge regions
+        t_ab = d1 / (d1 - d3)
+        t_ac = d2 / (d2 - d6)
+        t_bc = (d4 - d3) / ((d4 - d3) + (d5 - d6))
+    regA = (d1 <= 0) & (d2 <= 0)
+    regB = (d3 >= 0) & (d4 <= d3)
+    regC = (d6 >= 0) & (d5 <= d6)
+    regAB = (vc <= 0) & (d1 >= 0) & (d3 <= 0)
+    regAC = (vb <= 0) & (d2 >= 0) & (d6 <= 0)
+    regBC = (va <= 0) & ((d4 - d3) >= 0) & ((d5 - d6) >= 0)
+    Q = np.where(regBC[:, None], B + (Cc - B) * t_bc[:, None], Q)
+    Q = np.where(regAC[:, None], A + ac * t_ac[:, None], Q)
+    Q = np.where(regAB[:, None], A + ab * t_ab[:, None], Q)
+    Q = np.where(regC[:, None], Cc, Q)
+    Q = np.where(regB[:, None], B, Q)
+    Q = np.where(regA[:, None], A, Q)
+    return np.linalg.norm(P - Q, axis=1)
+
+
+def cell_centres_approx(mesh: PolyMesh) -> np.ndarray:
+    """Area-weighted mean of each cell's face centroids (vertex averages):
+    a cell-centre estimate for wall-distance work."""
+    off = mesh.face_offsets
+    nF = mesh.n_faces
+    cnt = np.diff(off)
+    fp = mesh.face_points
+    fid = np.repeat(np.arange(nF), cnt)
+    fc = np.zeros((nF, 3))
+    np.add.at(fc, fid, mesh.points[fp])
+    fc /= cnt[:, None]
+    C = np.zeros((mesh.n_cells, 3))
+    w = np.zeros(mesh.n_cells)
+    np.add.at(C, mesh.owner, fc)
+    np.add.at(w, mesh.owner, 1.0)
+    np.add.at(C, mesh.neighbour, fc[:mesh.n_internal])
+    np.add.at(w, mesh.neighbour, 1.0)
+    return C / w[:, None]
+
+
+def wall_distance(mesh: PolyMesh, wall_patches, centres: np.ndarray | None = None, k: int = 12) -> np.ndarray:
+    """Distance of every cell centre to the nearest face of the wall patches
+    (the wallDist d_ of the Spalart-Allmaras models).  Exact point-to-polygon
+    distance over the k wall faces whose centroids are nearest (KD tree)."""
+    from scipy.spatial import cKDTree
+    C = cell_centres_approx(mesh) if centres is None else np.asarray(centres, dtype=np.float64)
+    faces = []
+    for name in wall_patches:
+        _n, _t, start, size = mesh.patches[mesh.patch_index(name)]
+        faces.extend(range(start, start + size))
+    faces = np.array(faces, dtype=np.int64)
+    if len(faces) == 0:
+        raise ValueError("wall_distance: no wall faces")
+    A, B, Cc, fown = _face_triangles(mesh, faces)
+    off = mesh.face_offsets
+    cen = np.array([mesh.points[mesh.face_points[off[f]:off[f + 1]]].mean(axis=0) for f in faces])
+    tri_of_face = {}
+    for t_i, f in enumerate(fown):
+        tri_of_face.setdefault(f, []).append(t_i)
+    kk = min(k, len(faces))
+    _, nearest = cKDTree(cen).query(C, k=kk)
+    nearest = nearest.reshape(len(C), kk)
+    d = np.full(len(C), np.inf)
+    for j in range(kk):
+        fsel = faces[nearest[:, j]]
+        # every fan triangle of the candidate face
+        maxt = max(len(v) for v in tri_of_face.values())
+        for t in range(maxt):
+            tri = np.array([tri_of_face[f][t] if t < len(tri_of_face[f]) else tri_of_face[f][0] for f in fsel])
+            d = np.minimum(d, _point_triangle_distance(C, A[tri], B[tri], Cc[tri]))
+    return d

@@ -1,0 +1,352 @@
+"""Pins of the oracle's Spalart-Allmaras correct() (SAneg.C:490-605,
+SARC2.C:418-488): closed forms of the model functions, an independent numpy
+restatement of the whole nuTilda matrix (steadyStateDST ddt, upwind
+convection, corrected laplacian of DnuTildaEff, Cb2 term, production and
+implicit destruction, fvMatrix::relax) compared entry by entry, the
+symmetric Gauss-Seidel solve (serial and block-Jacobi over a partition)
+against a numpy sweep, and mut = |fv1(chi_old)| nuTilda rho."""
+import numpy as np
+import pytest
+
+from rhodst_amd import cases, meshgen
+
+SMALL = 1e-15
+
+
+# ---------------------------------------------------------------- numpy restatement
+def pow3(s):
+    return s * (s * s)
+
+
+def fv1(t, chi):
+    return pow3(chi) / (pow3(chi) + pow3(t.Cv1))
+
+
+def fv2(t, chi):
+    return 1.0 / pow3(1.0 + chi / t.Cv2)
+
+
+def fv3(t, chi, f1):
+    cb = chi / t.Cv2
+    return (1.0 + chi * f1) / t.Cv2 * (3.0 * (1.0 + cb) + cb ** 2) / pow3(1.0 + cb)
+
+
+def fw(t, Stilda, nut, d):
+    r = np.minimum(nut / (np.maximum(Stilda, SMALL) * (t.kappa * d) ** 2), 10.0)
+    g = r + t.Cw2 * (r ** 6 - r)
+    return g * ((1.0 + t.Cw3 ** 6) / (g ** 6 + t.Cw3 ** 6)) ** (1.0 / 6.0)
+
+
+def cw1(t):
+    return t.Cb1 / t.kappa ** 2 + (1.0 + t.Cb2) / t.sigma_nut
+
+
+def gamma_eff(t, rho, nut, mu):
+    if t.model == 1:
+        chi = rho * nut / mu
+        fn = np.where(chi >= 0, 1.0, (t.Cn1 + chi ** 3) / (t.Cn1 - chi ** 3))
+        return (rho * nut * fn + mu) / t.sigma_nut
+    return rho * nut / t.sigma_nut + mu
+
+
+def fr1(t, G):
+    if not t.rcm_correction:
+        return np.ones(len(G))
+    S = 0.5 * (G + G.transpose(0, 2, 1))
+    W = 0.5 * (G - G.transpose(0, 2, 1))
+    sqrS = 2 * (S ** 2).sum(axis=(1, 2))
+    sqrW = 2 * (W ** 2).sum(axis=(1, 2))
+    rs = np.sqrt(sqrS / np.maximum(sqrW, SMALL))
+    rr = np.sqrt(sqrW / np.maximum(sqrS, SMALL))
+    rt = rr * (rr - 1.0)
+    return (1 + t.Cr1) * 2.0 * rs / (1.0 + rs) * (1.0 - t.Cr3 * np.arctan(t.Cr2 * rt)) - t.Cr1
+
+
+class Geo:
+    """fvMesh geometry and addressing of a PolyMesh, from the oracle's
+    geometry (pinned separately) plus the polyMesh lists."""
+
+    def __init__(self, oracle, mesh):
+        g = oracle.geometry(mesh)
+        self.g = g
+        self.nI = mesh.n_internal
+        self.own = mesh.owner[:self.nI]
+        self.nei = mesh.neighbour
+        bc, bp = [], []
+        for pi, (_n, t, start, size) in enumerate(mesh.patches):
+            if t == meshgen.PATCH_EMPTY:
+                continue
+            bc.extend(mesh.owner[start:start + size])
+            bp.extend([pi] * size)
+        self.bcell = np.array(bc)
+        self.bpatch = np.array(bp)
+        self.nB = len(bc)
+        self.n = mesh.n_cells
+        Sf = g["Sf"]
+        dvec = g["C"][self.nei] - g["C"][self.own]
+        nrm = Sf[:self.nI] / g["magSf"][:self.nI, None]
+        self.corr = nrm - dvec * g["deltaCoeffs"][:self.nI, None]
+
+    def interp(self, cell, bnd):
+        w = self.g["weights"][:self.nI]
+        P, N = cell[self.own], cell[self.nei]
+        wi = w.reshape((-1,) + (1,) * (cell.ndim - 1))
+        return np.concatenate([wi * (P - N) + N, bnd])
+
+    def grad(self, cell, bnd):
+        """Gauss linear: sum Sf (x) phi_f / V; tensor index [i, j] = d phi_j / dx_i."""
+        vf = self.interp(cell, bnd)
+        Sf = self.g["Sf"]
+        if cell.ndim == 1:
+            t = Sf * vf[:, None]
+        else:
+            t = Sf[:, :, None] * vf[:, None, :]
+        out = np.zeros((self.n,) + t.shape[1:])
+        np.add.at(out, self.own, t[:self.nI])
+        np.subtract.at(out, self.nei, t[:self.nI])
+        np.add.at(out, self.bcell, t[self.nI:])
+        return out / self.g["V"].reshape((-1,) + (1,) * (t.ndim - 1))
+
+
+def numpy_sa_matrix(oracle, spec, oc, geo):
+    """The nuTilda fvScalarMatrix in solve form (diag incl. internalCoeffs,
+    source incl. boundaryCoeffs) at the oracle's current state, first
+    correct() call (rho.oldTime() = rho)."""
+    turb, sbc, d, _ = spec.turbulence
+    t = turb
+    th = spec.thermo
+    g = geo.g
+    V, nI, nB = g["V"], geo.nI, geo.nB
+    rho, U, nut = oc.get(0), oc.get(5), oc.get(29)
+    bU, brho, bT, bnut = oc.get(17), oc.get(16), oc.get(19), oc.get(33)
+    mu = oc.get(26)
+    bmu = th.As * np.sqrt(bT) / (1.0 + th.Ts / bT)
+    phi, rdt = oc.get(12), oc.get(10)
+    G = gamma_eff(t, rho, nut, mu)
+    bG = gamma_eff(t, brho, bnut, bmu)
+    gU = geo.grad(U, bU)
+    gn = geo.grad(nut, bnut)
+    diag = rdt * rho * V
+    src = rdt * rho * nut * V
+    up = np.zeros(nI)
+    lo = np.zeros(nI)
+    # upwind convection
+    w = (phi[:nI] >= 0).astype(float)
+    l_ = -w * phi[:nI]
+    u_ = l_ + phi[:nI]
+    lo += l_
+    up += u_
+    np.subtract.at(diag, geo.own, l_)
+    np.subtract.at(diag, geo.nei, u_)
+    types = np.array([sbc[p].type for p in geo.bpatch])
+    vals = np.array([sbc[p].value for p in geo.bpatch])
+    phib = phi[nI:]
+    vfio = 1.0 - (phib >= 0)
+    vic = np.where(types == 0, 0.0, np.where(types == 2, 1.0 - vfio, 1.0))
+    vbc = np.where(types == 0, vals, np.where(types == 2, vfio * vals, 0.0))
+    ic = phib * vic
+    bcf = -phib * vbc
+    # corrected laplacian
+    wI = g["weights"][:nI]
+    gf = wI * (G[geo.own] - G[geo.nei]) + G[geo.nei]
+    gms = gf * g["magSf"][:nI]
+    coef = gms * g["deltaCoeffs"][:nI]
+    up -= coef
+    lo -= coef
+    np.add.at(diag, geo.own, coef)
+    np.add.at(diag, geo.nei, coef)
+    gfn = wI[:, None] * (gn[geo.own] - gn[geo.nei]) + gn[geo.nei]
+    corr = gms * (geo.corr * gfn).sum(axis=1)
+    np.add.at(src, geo.own, corr)
+    np.subtract.at(src, geo.nei, corr)
+    bgms = bG * g["magSf"][nI:]
+    dcb = g["deltaCoeffs"][nI:]
+    lap_w = np.where(types == 0, 1.0, np.where(types == 2, vfio, 0.0))
+    lap_v = np.where(types == 0, vals, np.where(types == 2, vals, 0.0))
+    ic += bgms * lap_w * dcb
+    bcf += bgms * lap_w * dcb * lap_v
+    # sources
+    chi = rho * nut / mu
+    f1 = fv1(t, chi)
+    f2 = fv2(t, chi)
+    Wsk = 0.5 * (gU - gU.transpose(0, 2, 1))
+    Omega = np.sqrt(2.0) * np.sqrt((Wsk ** 2).sum(axis=(1, 2)))
+    src += t.Cb2 / t.sigma_nut * rho * (gn ** 2).sum(axis=1) * V
+    if t.model == 1:
+        if t.stilda_modification:
+            Sbar = f2 * nut / (t.kappa * d) ** 2
+            a = t.Cn2 * Omega + Sbar
+            St = Omega + (a >= 0) * Sbar + (a < 0) * (Omega * (t.Cn2 ** 2 * Omega + t.Cn3 * Sbar)) / np.maximum(
+                (t.Cn3 - 2 * t.Cn2) * Omega - Sbar, SMALL)
+        else:
+            St = fv3(t, chi, f1) * Omega + f2 * nut / (t.kappa * d) ** 2
+        ft2 = t.Ct3 * np.exp(-t.Ct4 * chi ** 2) if t.negative_nu_tilda else np.zeros_like(chi)
+        fW = fw(t, St, nut, d)
+        posn = nut >= 0
+        src += np.where(posn, t.Cb1 * (1 - ft2) * rho * St * nut * V, t.Cb1 * (1 - t.Ct3) * rho * Omega * nut * V)
+        diag += np.where(posn, (cw1(t) * fW - t.Cb1 / t.kappa ** 2 * ft2) * nut * rho / d ** 2 * V,
+                         -cw1(t) * nut * rho / d ** 2 * V)
+    else:
+        St = fv3(t, chi, f1) * Omega + f2 * nut / (t.kappa * d) ** 2
+        fW = fw(t, St, nut, d)
+        src += t.Cb1 * rho * St * nut * fr1(t, gU) * V
+        diag += cw1(t) * fW * nut * rho / d ** 2 * V
+    # relax
+    if t.relax > 0:
+        D0 = diag.copy()
+        sumOff = np.zeros(geo.n)
+        np.add.at(sumOff, geo.own, np.abs(up))
+        np.add.at(sumOff, geo.nei, np.abs(lo))
+        ic0 = ic.copy()
+        ic = np.abs(ic)
+        np.subtract.at(sumOff, geo.bcell, ic)
+        ic = ic / t.relax
+        np.add.at(src, geo.bcell, (ic - ic0) * nut[geo.bcell])
+        diag = np.maximum(diag, sumOff) / t.relax
+        src += (diag - D0) * nut
+    np.add.at(diag, geo.bcell, ic)
+    np.add.at(src, geo.bcell, bcf)
+    return diag, up, lo, src, f1
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+# ---------------------------------------------------------------- tests
+def test_sa_function_closed_forms():
+    from rhodst_amd._abi import make_turbulence
+    t = make_turbulence("SAneg")
+    assert fv1(t, t.Cv1) == pytest.approx(0.5, rel=1e-15)
+    assert fv2(t, t.Cv2) == pytest.approx(1.0 / 8.0, rel=1e-15)
+    assert fv1(t, 0.0) == 0.0 and fv2(t, 0.0) == 1.0
+    # r = 1 gives g = 1 and fw = 1; r is capped at 10
+    d = 0.01
+    S = 0.37
+    nut_r1 = S * (t.kappa * d) ** 2
+    assert fw(t, S, nut_r1, d) == pytest.approx(1.0, rel=1e-13)
+    assert fw(t, S, 1e3, d) == pytest.approx(fw(t, S, 10 * nut_r1 * 1e3, d), rel=1e-15)
+    assert cw1(t) == pytest.approx(0.1355 / 0.41 ** 2 + 1.622 / 0.66666, rel=1e-15)
+    # SARC2 fr1: pure shear (|S| = |W|) gives 1, pure rotation gives -Cr1,
+    # pure strain gives 1 + Cr1 - Cr1 * ... = 2 rStar/(1 + rStar) with rStar large -> (1 + Cr1)*2 - Cr1
+    t2 = make_turbulence("SARC2")
+    shear = np.zeros((1, 3, 3)); shear[0, 1, 0] = 3.0
+    rot = np.zeros((1, 3, 3)); rot[0, 1, 0] = 2.0; rot[0, 0, 1] = -2.0
+    strain = np.zeros((1, 3, 3)); strain[0, 0, 0] = 1.0; strain[0, 1, 1] = -1.0
+    assert fr1(t2, shear)[0] == pytest.approx(1.0, rel=1e-15)
+    assert fr1(t2, rot)[0] == pytest.approx(-t2.Cr1, rel=1e-15)
+    rs = np.sqrt(4.0 / SMALL)
+    assert fr1(t2, strain)[0] == pytest.approx((1 + t2.Cr1) * 2 * rs / (1 + rs) - t2.Cr1, rel=1e-12)
+
+
+@pytest.mark.parametrize("model,kw", [("SAneg", {}), ("SAneg", {"stilda_modification": 0}),
+                                      ("SAneg", {"negative_nu_tilda": 0}), ("SARC2", {}),
+                                      ("SARC2", {"rcm_correction": 0, "relax": 0.0})])
+def test_sa_matrix_against_numpy(oracle, model, kw):
+    spec = cases.box_sa_case(model, turb_kw=kw)
+    oc = oracle.OracleCase(spec)
+    oc.flux_residual()
+    geo = Geo(oracle, spec.mesh)
+    diag, up, lo, src, _ = numpy_sa_matrix(oracle, spec, oc, geo)
+    oc.turbulence_correct()
+    assert _rel(oc.get(35), diag) < 1e-12
+    assert _rel(oc.get(36), up) < 1e-12
+    assert _rel(oc.get(37), lo) < 1e-12
+    assert _rel(oc.get(38), src) < 1e-11
+
+
+def _sym_gs(diag, up, lo, src, x, geo, part, sweeps):
+    n = geo.n
+    rows = [[] for _ in range(n)]
+    for f in range(geo.nI):
+        rows[geo.own[f]].append((geo.nei[f], up[f]))
+        rows[geo.nei[f]].append((geo.own[f], lo[f]))
+    x = x.copy()
+    for _ in range(sweeps):
+        x0 = x.copy()
+        bp = src.copy()
+        for i in range(n):
+            for j, a in rows[i]:
+                if part[j] != part[i]:
+                    bp[i] -= a * x0[j]
+        for order in (range(n), range(n - 1, -1, -1)):
+            for i in order:
+                s = bp[i]
+                for j, a in rows[i]:
+                    if part[j] == part[i]:
+                        s -= a * x[j]
+                x[i] = s / diag[i]
+    return x
+
+
+@pytest.mark.parametrize("split", [False, True])
+def test_sa_symgs_solve(oracle, split):
+    """symGaussSeidel sweeps of the oracle vs a numpy sweep on the same
+    matrix: serial (one partition) and block-Jacobi over two partitions.
+    SAneg with negativeNuTilda applies no bound, so nuTilda after correct()
+    is the solver's output; mut = |fv1(chi_old)| nuTilda rho."""
+    spec = cases.box_sa_case("SAneg", turb_kw={"rel_tol": 1e-3, "tolerance": 1e-12})
+    oc = oracle.OracleCase(spec)
+    geo = Geo(oracle, spec.mesh)
+    part = np.zeros(geo.n, dtype=np.int32)
+    if split:
+        part = (geo.g["C"][:, 0] > 0.005).astype(np.int32)
+        oc.set_partition(part)
+    oc.flux_residual()
+    x0 = oc.get(29).copy()
+    rho = oc.get(0)
+    chi = rho * x0 / oc.get(26)
+    perf = oc.turbulence_correct()
+    assert perf.n_iterations >= 2
+    x1 = _sym_gs(oc.get(35), oc.get(36), oc.get(37), oc.get(38), x0, geo, part, perf.n_iterations)
+    assert _rel(oc.get(29), x1) < 1e-12
+    from rhodst_amd._abi import make_turbulence
+    t = make_turbulence("SAneg")
+    mut = np.abs(fv1(t, chi)) * x1 * rho
+    mut = np.where(mut < 0, np.nan, mut)
+    ok = ~np.isnan(mut)
+    if ok.all():
+        assert _rel(oc.get(30), mut) < 1e-12
+    else:
+        # bounding of mut took place (negative nuTilda cells); elsewhere unchanged values are kept
+        assert np.all(oc.get(30) >= 0)
+    # the residual reduction the solve stopped on
+    assert perf.final_residual[0] < 1e-3 * perf.initial_residual[0] or perf.final_residual[0] < 1e-12
+
+
+def test_sa_bound_nonnegative_and_wall(oracle):
+    """SARC2 bounds nuTilda at 0 (bound() by the local face average) and the
+    wall value is the fixedValue 0; mut is 0 on the wall patch."""
+    spec = cases.box_sa_case("SARC2", negative_fraction=0.3)
+    oc = oracle.OracleCase(spec)
+    r = oc.iterate(3)
+    assert np.all(np.isfinite(r))
+    assert oc.get(29).min() >= 0.0
+    geo = Geo(oracle, spec.mesh)
+    wall = geo.bpatch == spec.mesh.patch_index("bottom")
+    assert np.all(oc.get(33)[wall] == 0.0)
+    assert np.all(oc.get(34)[wall] == 0.0)
+
+
+def test_turbulent_viscosity_enters_viscous_residual(oracle):
+    """muEff = mut + mu (RASModel::muEff): with nuTilda = 0 everywhere (mut = 0
+    in the cells) the residual equals the laminar case's bit for bit away from
+    the far-field faces (whose mut_b follows the inletOutlet nuTilda_b); with
+    the seeded nuTilda the momentum residual changes."""
+    spec = cases.box_sa_case("SAneg", negative_fraction=0.0)
+    t, b, d, _ = spec.turbulence
+    spec0 = cases.box_sa_case("SAneg", negative_fraction=0.0)
+    spec0.turbulence = (t, b, d, np.zeros(spec.mesh.n_cells))
+    o0 = oracle.OracleCase(spec0)
+    ol = oracle.OracleCase(cases.box_viscous_case())
+    oc = oracle.OracleCase(spec)
+    for c in (o0, ol, oc):
+        c.flux_residual()
+    geo = Geo(oracle, spec.mesh)
+    assert np.all(o0.get(30) == 0.0)
+    touched = np.zeros(geo.n, bool)
+    touched[geo.bcell[o0.get(34) != 0.0]] = True
+    R0, Rl = o0.get(8), ol.get(8)
+    assert touched.any() and not touched.all()
+    assert np.array_equal(R0[~touched], Rl[~touched])
+    assert _rel(oc.get(8)[:, 1:4], Rl[:, 1:4]) > 1e-6
