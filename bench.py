@@ -53,13 +53,27 @@ def run_path(spec, steps: int, warmup: int, mesh=None):
     return dc, (t1 - t0) * 1e3 / steps, res
 
 
+def wing_coefficients(dc, mesh, mach=0.84, aoa_deg=3.06):
+    """Cd, Cl of the wing patch (forceCoeffs: pRef = pInf, q = rhoInf |UInf|^2 / 2,
+    Aref = the semi-span planform area 0.78)."""
+    gamma, R, Tinf, pinf = 1.4, 287.0, 288.15, 101325.0
+    a = np.deg2rad(aoa_deg)
+    U = mach * np.sqrt(gamma * R * Tinf)
+    rho = pinf / (R * Tinf)
+    out = dc.forces(mesh.patch_index("wing"), (np.cos(a), np.sin(a), 0.0), (-np.sin(a), np.cos(a), 0.0), pinf, rho,
+                    U, 0.5 * (1.0 + 0.56) * 1.0)
+    return float(out[0]), float(out[1])
+
+
 def converge(args, mesh, spec64, spec32) -> int:
     """Time to |R|_2 / |R|_2(0) <= 1e-8 (timeDST.C:328-331 norm) for the mixed
     and the FP64 path; prints one JSON line per path."""
     from rhodst_amd import DeviceCase
     out = {}
     dmesh = None
-    for name, spec in (("mixed", spec32), ("fp64", spec64)):
+    specs = {"mixed": spec32, "fp64": spec64}
+    for name in args.paths.split(","):
+        spec = specs[name]
         dc = DeviceCase(spec, dmesh)
         dmesh = dc.mesh
         r0 = None
@@ -83,8 +97,9 @@ def converge(args, mesh, spec64, spec32) -> int:
         t1 = time.perf_counter()
         h = np.array(hist) / r0
         k = int(np.argmax(h <= 1e-8)) + 1 if reached else None
+        cd, cl = wing_coefficients(dc, mesh)
         out[name] = {"reached": reached, "steps": k if reached else steps, "seconds": round(t1 - t0, 2),
-                     "final_drop": float(h[-1]), "min_drop": float(h.min())}
+                     "final_drop": float(h[-1]), "min_drop": float(h.min()), "Cd": cd, "Cl": cl}
         del dc
     print(json.dumps({"converge": out, "cells": mesh.n_cells}), flush=True)
     return 0
@@ -102,6 +117,10 @@ def main(argv=None) -> int:
     ap.add_argument("--converge", action="store_true",
                     help="run the mixed and the FP64 paths (partitioned ILU, classical GS) to a 1e-8 drop of |R|_2")
     ap.add_argument("--max-steps", type=int, default=4000)
+    ap.add_argument("--max-co", type=float, default=None, help="CFL ramp ceiling (default: the workload's 20)")
+    ap.add_argument("--gmres-cycles", type=int, default=None, help="GMRES restart cycles per step (default 1)")
+    ap.add_argument("--turbulence", default=None, help="SARC2 or SAneg: the viscous SA form of the workload")
+    ap.add_argument("--paths", default="mixed,fp64", help="converge mode: paths to run, in order")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print("bench.py measures one GPU: every path here runs on a single MI355X; "
@@ -113,8 +132,19 @@ def main(argv=None) -> int:
     t = time.perf_counter()
     mesh = meshgen.swept_wing(**mesh_kw)
     log(f"mesh {mesh.n_cells} cells, {mesh.n_internal} internal faces in {time.perf_counter() - t:.1f}s")
-    spec64 = cases.swept_wing_case(mesh=mesh)
-    fast = dict(ilu_order="partitioned", gram_schmidt="classical")
+    extra = {}
+    if args.max_co is not None:
+        extra["max_co"] = args.max_co
+    if args.gmres_cycles is not None:
+        extra["max_iter"] = args.gmres_cycles
+    wd = None
+    if args.turbulence:
+        t = time.perf_counter()
+        wd = meshgen.wall_distance(mesh, ["wing"])
+        extra.update(turbulence=args.turbulence, wall_dist=wd)
+        log(f"wall distance in {time.perf_counter() - t:.1f}s")
+    spec64 = cases.swept_wing_case(mesh=mesh, **extra)
+    fast = dict(ilu_order="partitioned", gram_schmidt="classical", **extra)
     spec64p = cases.swept_wing_case(mesh=mesh, **fast)
     spec32p = cases.swept_wing_case(mesh=mesh, precision="mixed", **fast)
     if args.converge:

@@ -92,12 +92,17 @@ def box_farfield_case(nx: int = 24, ny: int = 16, scheme: str = "HLLC", mach: fl
 
 
 def swept_wing_case(scheme: str = "AUSM+up", mach: float = 0.84, aoa_deg: float = 3.06, viscous: bool = False,
-                    mesh=None, mesh_kw: dict | None = None, **settings) -> CaseSpec:
+                    mesh=None, mesh_kw: dict | None = None, turbulence: str | None = None, reynolds: float = 1.2e7,
+                    wall_dist=None, **settings) -> CaseSpec:
     """Transonic swept wing: semi-span 1, 30 deg leading-edge sweep, taper
     0.56, 10 % analytic section, C-H hex mesh (meshgen.swept_wing, 2.03 M
     cells by default), Mach 0.84 at 3.06 deg, characteristic far field,
     symmetry plane at the root.  Air (R 287, gamma 1.4, T 288.15 K,
-    p 101325 Pa), uniform free-stream start.  Inviscid: slip wing."""
+    p 101325 Pa), uniform free-stream start.  Inviscid: slip wing.
+    turbulence="SARC2"/"SAneg": no-slip wing, viscous terms on, a constant
+    mu giving Re = reynolds on the root chord, nuTilda fixedValue 0 on the
+    wing, inletOutlet 3 nu on the far field, the exact wall distance
+    (meshgen.wall_distance, or wall_dist when given) and nuTilda = 3 nu."""
     mesh = mesh if mesh is not None else meshgen.swept_wing(**(mesh_kw or {}))
     gamma, R = 1.4, 287.0
     Cp = gamma * R / (gamma - 1)
@@ -107,6 +112,8 @@ def swept_wing_case(scheme: str = "AUSM+up", mach: float = 0.84, aoa_deg: float 
     Uinf = (mach * c * np.cos(a), mach * c * np.sin(a), 0.0)
     ff = {"p": ("farField", pinf), "T": ("farField", Tinf), "U": ("farField", Uinf),
           "inf": {"p": pinf, "T": Tinf, "U": Uinf}}
+    if turbulence:
+        viscous = True
     wall = {"p": "zeroGradient", "T": "zeroGradient", "U": ("fixedValue", (0.0, 0.0, 0.0)) if viscous else "slip"}
     spec = {"wing": wall, "symmetry": {"p": "symmetryPlane", "T": "symmetryPlane", "U": "symmetryPlane"},
             "farfield": ff}
@@ -119,8 +126,20 @@ def swept_wing_case(scheme: str = "AUSM+up", mach: float = 0.84, aoa_deg: float 
               tolerance=1e-15, rel_tol=1e-2, p_bounds=(100.0, 1e7), T_bounds=(50.0, 2000.0),
               rho_bounds=(1e-3, 100.0))
     kw.update(settings)
-    return CaseSpec("sweptWing", mesh, make_thermo(Cp=Cp, R=R, mu=0.0, Pr=0.72),
-                    make_bcs(mesh, spec), make_settings(**kw), p0, T0, np.ascontiguousarray(U0))
+    rho_inf = pinf / (R * Tinf)
+    mu = rho_inf * mach * c / reynolds if viscous else 0.0
+    kw.setdefault("viscous", 1 if viscous else 0)
+    spec_out = CaseSpec("sweptWing", mesh, make_thermo(Cp=Cp, R=R, mu=mu, Pr=0.72),
+                        make_bcs(mesh, spec), make_settings(**kw), p0, T0, np.ascontiguousarray(U0))
+    if turbulence:
+        from ._abi import make_scalar_bcs, make_turbulence
+        nu = mu / rho_inf
+        sbc = make_scalar_bcs(mesh, {"wing": ("fixedValue", 0.0), "symmetry": "symmetryPlane",
+                                     "farfield": ("inletOutlet", 3.0 * nu)})
+        d = wall_dist if wall_dist is not None else meshgen.wall_distance(mesh, ["wing"])
+        spec_out.turbulence = (make_turbulence(turbulence), sbc, d, np.full(n, 3.0 * nu))
+        spec_out.name = f"sweptWing_{turbulence}"
+    return spec_out
 
 
 def box_viscous_case(nx: int = 24, ny: int = 16, scheme: str = "HLLC", mach: float = 0.3, jitter: float = 0.15,

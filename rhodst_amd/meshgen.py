@@ -610,15 +610,16 @@ def wall_distance(mesh: PolyMesh, wall_patches, centres: np.ndarray | None = Non
     tri_of_face = {}
     for t_i, f in enumerate(fown):
         tri_of_face.setdefault(f, []).append(t_i)
+    maxt = max(len(v) for v in tri_of_face.values())
+    # [n_wall_faces, maxt] triangle table (short faces repeat their first triangle)
+    table = np.array([[tri_of_face[f][t] if t < len(tri_of_face[f]) else tri_of_face[f][0] for t in range(maxt)]
+                      for f in faces], dtype=np.int64)
     kk = min(k, len(faces))
     _, nearest = cKDTree(cen).query(C, k=kk)
     nearest = nearest.reshape(len(C), kk)
     d = np.full(len(C), np.inf)
     for j in range(kk):
-        fsel = faces[nearest[:, j]]
-        # every fan triangle of the candidate face
-        maxt = max(len(v) for v in tri_of_face.values())
         for t in range(maxt):
-            tri = np.array([tri_of_face[f][t] if t < len(tri_of_face[f]) else tri_of_face[f][0] for f in fsel])
+            tri = table[nearest[:, j], t]
             d = np.minimum(d, _point_triangle_distance(C, A[tri], B[tri], Cc[tri]))
     return d
