@@ -340,6 +340,37 @@ int rdst_case_set_turbulence(rdst_case* c, const rdst_turbulence* turb, const rd
  * is set (Single.C:102). */
 int rdst_turbulence_correct(rdst_case* c, rdst_solver_perf* perf);
 
+/* ----------------------------------------------------------- discrete adjoint
+ * New (no counterpart in the reference): exact linearisation of the
+ * first-order inviscid residual R(W) = (1/V) sum_f s_f F_f (baseFluxDST.C:
+ * 223-255 with the W -> primitive chain of updateFields.H:40-94 and the
+ * boundary conditions at the same state), its transpose, and the adjoint
+ * solve for force-coefficient sensitivities dJ/dalpha = dJ/dalpha|_W -
+ * lambda^T dR/dalpha|_W with J^T lambda = dJ/dW.  Flux schemes AUSM+,
+ * AUSM+up, HLL, HLLC, Kurganov (HLLCP's sensor couples second neighbours). */
+typedef struct rdst_adjoint_settings {
+    int32_t precision;      /* RDST_PREC_FP64, or RDST_PREC_MIXED: FP32 transposed ILU and Krylov cycles
+                               inside an FP64 defect correction on the exact FP64 J^T */
+    int32_t n_directions;   /* GMRES restart length */
+    int32_t max_cycles;     /* restart cycles */
+    int32_t pad;
+    double tolerance, rel_tol;  /* on the normalised residual |r| as for the primal solve */
+} rdst_adjoint_settings;
+
+/* J = dR/dW at the current state (and J^T), kept on the device */
+int rdst_adjoint_linearise(rdst_case* c);
+/* y = J x (transpose = 0) or y = J^T x (transpose = 1) with the last
+ * linearisation; x, y host [n_cells][5] */
+int rdst_adjoint_apply(rdst_case* c, int transpose, const double* x, double* y);
+/* R(W) [n_cells][5] by the residual chain that rdst_adjoint_linearise differentiates */
+int rdst_adjoint_residual(rdst_case* c, double* R);
+/* dJ/dW [n_cells][5] of J = sum_{faces of patch} (p_b - p_ref) (Sf . dir) / q */
+int rdst_adjoint_objective_gradient(rdst_case* c, int patch, const double* dir, double q, double* g);
+/* linearises at the current state, then solves J^T lambda = dJ/dW; lambda
+ * [n_cells][5] holds the initial guess (zeros for none) and receives the solution */
+int rdst_adjoint_solve(rdst_case* c, int patch, const double* dir, double q, const rdst_adjoint_settings* s,
+                       double* lambda, rdst_solver_perf* perf);
+
 /* forceCoeffs-style integral over one patch: force = sum (p_b - p_ref) Sf
  * + sum Sf & devRhoReff_b with devRhoReff = -muEff dev(twoSymm(grad U))
  * (the viscous part when settings.viscous); out[0]=Cd, out[1]=Cl,

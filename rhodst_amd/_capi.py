@@ -12,8 +12,8 @@ import os
 
 import numpy as np
 
-from ._abi import (MeshDesc, dptr, iptr, rdst_bc, rdst_mesh_desc, rdst_scalar_bc, rdst_settings,
-                   rdst_solver_perf, rdst_thermo, rdst_turbulence)
+from ._abi import (MeshDesc, dptr, iptr, make_adjoint_settings, rdst_adjoint_settings, rdst_bc, rdst_mesh_desc,
+                   rdst_scalar_bc, rdst_settings, rdst_solver_perf, rdst_thermo, rdst_turbulence)
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "librhodst_amd.so")
 
@@ -25,6 +25,8 @@ EXPORTS = ["rdst_last_error", "rdst_version", "rdst_init", "rdst_mesh_create", "
            "rdst_case_destroy", "rdst_case_set_state", "rdst_case_get", "rdst_case_put",
            "rdst_flux_residual", "rdst_jacobian_assemble", "rdst_block_solve", "rdst_update_fields",
            "rdst_set_courant", "rdst_iterate", "rdst_case_set_turbulence", "rdst_turbulence_correct",
+           "rdst_adjoint_linearise", "rdst_adjoint_apply", "rdst_adjoint_residual",
+           "rdst_adjoint_objective_gradient", "rdst_adjoint_solve",
            "rdst_forces", "rdst_last_flux_ms", "rdst_time_flux"]
 
 _lib = None
@@ -67,6 +69,12 @@ def lib():
     L.rdst_case_set_turbulence.argtypes = [C.c_void_p, P(rdst_turbulence), P(rdst_scalar_bc), P(C.c_double),
                                            P(C.c_double)]
     L.rdst_turbulence_correct.argtypes = [C.c_void_p, P(rdst_solver_perf)]
+    L.rdst_adjoint_linearise.argtypes = [C.c_void_p]
+    L.rdst_adjoint_apply.argtypes = [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double)]
+    L.rdst_adjoint_residual.argtypes = [C.c_void_p, P(C.c_double)]
+    L.rdst_adjoint_objective_gradient.argtypes = [C.c_void_p, C.c_int, P(C.c_double), C.c_double, P(C.c_double)]
+    L.rdst_adjoint_solve.argtypes = [C.c_void_p, C.c_int, P(C.c_double), C.c_double, P(rdst_adjoint_settings),
+                                     P(C.c_double), P(rdst_solver_perf)]
     L.rdst_last_flux_ms.argtypes = [C.c_void_p, P(C.c_double)]
     L.rdst_time_flux.argtypes = [C.c_void_p, C.c_int, P(C.c_double)]
     _lib = L
@@ -162,6 +170,35 @@ class DeviceCase:
         perf = rdst_solver_perf()
         check(lib().rdst_turbulence_correct(self.h, C.byref(perf)))
         return perf
+
+    # ---- discrete adjoint (first-order inviscid residual)
+    def adjoint_linearise(self):
+        check(lib().rdst_adjoint_linearise(self.h))
+
+    def adjoint_apply(self, x, transpose=False):
+        xa = np.ascontiguousarray(x, dtype=np.float64).reshape(self.n, 5)
+        y = np.zeros((self.n, 5))
+        check(lib().rdst_adjoint_apply(self.h, 1 if transpose else 0, dptr(xa), dptr(y)))
+        return y
+
+    def adjoint_residual(self):
+        R = np.zeros((self.n, 5))
+        check(lib().rdst_adjoint_residual(self.h, dptr(R)))
+        return R
+
+    def adjoint_objective_gradient(self, patch, direction, q):
+        d = np.ascontiguousarray(direction, dtype=np.float64)
+        g = np.zeros((self.n, 5))
+        check(lib().rdst_adjoint_objective_gradient(self.h, patch, dptr(d), float(q), dptr(g)))
+        return g
+
+    def adjoint_solve(self, patch, direction, q, lambda0=None, **settings):
+        d = np.ascontiguousarray(direction, dtype=np.float64)
+        lam = np.zeros((self.n, 5)) if lambda0 is None else np.ascontiguousarray(lambda0, dtype=np.float64).copy()
+        s = make_adjoint_settings(**settings)
+        perf = rdst_solver_perf()
+        check(lib().rdst_adjoint_solve(self.h, patch, dptr(d), float(q), C.byref(s), dptr(lam), C.byref(perf)))
+        return lam, perf
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

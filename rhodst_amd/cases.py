@@ -197,3 +197,29 @@ def box_sa_case(model: str = "SAneg", scheme: str = "HLLC", nx: int = 24, ny: in
     spec.name = f"boxSA_{model}"
     spec.turbulence = (turb, bcs, d, nut0)
     return spec
+
+
+def bump_case(amplitudes=(0.02, 0.01, -0.005, 0.004), nx: int = 48, ny: int = 24, mach: float = 0.5,
+              scheme: str = "HLLC", node_disp=None, **settings) -> CaseSpec:
+    """Inviscid subsonic flow through a channel over a Hicks-Henne bump on the
+    bottom wall (meshgen.bump_channel): characteristic far-field inlet and
+    outlet (subsonic inflow/outflow everywhere), slip bottom wall, symmetry
+    top.  Air at 300 K / 1e5 Pa.  The workload of the discrete-adjoint
+    checks (force coefficients of the bottom wall against wall shape)."""
+    mesh = meshgen.bump_channel(nx, ny, amplitudes=amplitudes, node_disp=node_disp)
+    gamma, R = 1.4, 287.0
+    Cp = gamma * R / (gamma - 1)
+    Tinf, pinf = 300.0, 1.0e5
+    U = (mach * np.sqrt(gamma * R * Tinf), 0.0, 0.0)
+    ff = {"p": ("farField", pinf), "T": ("farField", Tinf), "U": ("farField", U),
+          "inf": {"p": pinf, "T": Tinf, "U": U}}
+    spec = {"inlet": ff, "outlet": ff, "bottom": {"p": "zeroGradient", "T": "zeroGradient", "U": "slip"},
+            "top": {"p": "symmetryPlane", "T": "symmetryPlane", "U": "symmetryPlane"}}
+    n = mesh.n_cells
+    kw = dict(flux_scheme=scheme, neighbouring=1, initial_co=5.0, min_co=1.0, max_co=1.0e5,
+              solver="GMRES", precond="ILUC0", n_directions=30, min_iter=1, max_iter=4,
+              tolerance=1e-14, rel_tol=1e-6, p_bounds=(100.0, 1e7), T_bounds=(50.0, 2000.0),
+              rho_bounds=(1e-3, 100.0))
+    kw.update(settings)
+    return CaseSpec("bump", mesh, make_thermo(Cp=Cp, R=R, mu=0.0, Pr=0.72), make_bcs(mesh, spec),
+                    make_settings(**kw), np.full(n, pinf), np.full(n, Tinf), np.tile(np.array(U), (n, 1)))

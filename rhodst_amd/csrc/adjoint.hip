@@ -1,0 +1,419 @@
+// adjoint.hip -- discrete adjoint of the first-order inviscid residual for
+// force-coefficient sensitivities.
+//
+// The residual is R_c(W) = (1/V_c) sum_f s_f F_f(W_L, W_R), the steady form of
+// the face loop (baseFluxDST.C:223-255 without viscous terms), with the cell
+// primitives retrieved from W as updateFields.H:40-94 does and the boundary
+// states from the boundary conditions evaluated at the same state (the old
+// and new values of the far-field updateCoeffs coincide at a fixed point).
+// The linearisation is exact: every face flux is differentiated in forward
+// mode (dual numbers, fluxfun.hpp) through the flux scheme, the
+// reconstruction, the W -> primitive chain and the boundary conditions; only
+// the piecewise-constant masks (foam pos/neg/sign, the far-field regimes and
+// inletOutlet valueFractions read from the stored mass flux) carry no
+// derivative.  This is not the Roe-Jacobian of the implicit operator
+// (Roe.C:152-379), which approximates the same matrix.
+//
+// J = dR/dW is block-CSR on the mesh's slots (one 5x5 block per cell pair of
+// a face and the diagonal); its transpose is the same structure with the
+// upper/lower slots swapped and every block transposed.  The adjoint system
+// J^T lambda = dJ/dW is solved by GMRES + block ILU(0) of J^T (FP64), or by
+// FP32 GMRES cycles with FP32 transposed ILU inside an FP64 defect correction
+// on the exact FP64 J^T (mixed).  HLLCP is excluded: its pressure sensor
+// couples second neighbours, outside the block stencil.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "rdst_internal.hpp"
+#include "reduce.hpp"
+#include "fluxfun.hpp"
+
+namespace rdst {
+
+namespace {
+
+template <class T>
+__device__ inline T clampT(const T& x, double lo, double hi) {
+    if (val(x) < lo) return T(lo);
+    if (val(x) > hi) return T(hi);
+    return x;
+}
+
+template <class T>
+struct Prim {
+    T rho, rux, ruy, ruz, rhoE, ux, uy, uz, h, Tt, psi, p;
+};
+
+struct Bounds {
+    double rhoMin, rhoMax, hMin, hMax, pMin, pMax;
+};
+
+// updateFields.H:40-94 on one cell: rho bounded, U = rhoU/rho,
+// h = (Cp/Cv)(rhoE/rho - |U|^2/2) bounded, T = h/Cp, psi = 1/(R T),
+// p = rho/psi bounded (the bounds are inactive at a regular state)
+template <class T>
+__device__ inline Prim<T> prim_from_w(T w0, T w1, T w2, T w3, T w4, const Thermo& th, const Bounds& bd) {
+    Prim<T> q;
+    q.rho = clampT(w0, bd.rhoMin, bd.rhoMax);
+    q.rux = w1; q.ruy = w2; q.ruz = w3; q.rhoE = w4;
+    q.ux = w1 / q.rho; q.uy = w2 / q.rho; q.uz = w3 / q.rho;
+    T h = (th.Cp / th.Cv) * (w4 / q.rho - 0.5 * (q.ux * q.ux + q.uy * q.uy + q.uz * q.uz));
+    q.h = clampT(h, bd.hMin, bd.hMax);
+    q.Tt = q.h / th.Cp;
+    q.psi = 1.0 / (th.R * q.Tt);
+    q.p = clampT(q.rho / q.psi, bd.pMin, bd.pMax);
+    return q;
+}
+
+template <class T>
+struct BState {
+    T rho, rux, ruy, ruz, rhoE, p, h, psi;
+};
+
+__device__ inline double mpos0(double s) { return s >= 0.0 ? 1.0 : 0.0; }
+__device__ inline double mneg(double s) { return s < 0.0 ? 1.0 : 0.0; }
+__device__ inline double mneg0(double s) { return s <= 0.0 ? 1.0 : 0.0; }
+
+// the boundary state of face b from its cell (update.hip k_update_bnd_pre /
+// k_update_bnd_post with old = new); masks from the stored phi, rho_b, psi_b
+template <class T>
+__device__ inline BState<T> bc_state(const BcPatchDev& B, const Fields& F, int b, int k, double nx, double ny, double nz,
+                                     double ms, const Prim<T>& I, const Thermo& th, const Bounds& bd) {
+    const double gam = th.Cp / th.Cv;
+    const double phib = F.phi[k];
+    // far-field regime masks (P.C:174-191), frozen
+    double supIn = 0, supOut = 0, subIn = 0, subOut = 0;
+    if (B.U_type == RDST_BC_FARFIELD || B.T_type == RDST_BC_FARFIELD || B.p_type == RDST_BC_FARFIELD) {
+        const double cb = ::sqrt(gam / F.bpsi[b]);
+        const double mach = phib / F.brho[b] / ms / cb;
+        supIn = mneg0(mach + 1); supOut = mpos0(mach - 1);
+        subIn = mneg(mach) - supIn; subOut = mpos0(mach) - supOut;
+    }
+    const T cI = sqrt(gam / I.psi);
+    const T rho0c0 = I.rho * cI;
+    // velocity
+    T ux, uy, uz;
+    switch (B.U_type) {
+        case RDST_BC_FIXED_VALUE: ux = T(B.U_value[0]); uy = T(B.U_value[1]); uz = T(B.U_value[2]); break;
+        case RDST_BC_SYMMETRY: {
+            const T nU = nx * I.ux + ny * I.uy + nz * I.uz;
+            const T tx = I.ux - 2.0 * nx * nU, ty = I.uy - 2.0 * ny * nU, tz = I.uz - 2.0 * nz * nU;
+            ux = (tx + I.ux) / 2.0; uy = (ty + I.uy) / 2.0; uz = (tz + I.uz) / 2.0;
+            break;
+        }
+        case RDST_BC_INLET_OUTLET: {
+            const double vf = 1.0 - mpos0(phib);
+            ux = vf * B.U_value[0] + (1.0 - vf) * I.ux;
+            uy = vf * B.U_value[1] + (1.0 - vf) * I.uy;
+            uz = vf * B.U_value[2] + (1.0 - vf) * I.uz;
+            break;
+        }
+        case RDST_BC_FARFIELD: {
+            const T dUn = nx * (B.U_inf[0] - I.ux) + ny * (B.U_inf[1] - I.uy) + nz * (B.U_inf[2] - I.uz);
+            const T pb = 0.5 * (B.p_inf + I.p - rho0c0 * dUn);
+            const T a = (B.p_inf - pb) / rho0c0, o = (I.p - pb) / rho0c0;
+            const T rx = supIn * B.U_inf[0] + subIn * (B.U_inf[0] - nx * a) + subOut * (nx * o);
+            const T ry = supIn * B.U_inf[1] + subIn * (B.U_inf[1] - ny * a) + subOut * (ny * o);
+            const T rz = supIn * B.U_inf[2] + subIn * (B.U_inf[2] - nz * a) + subOut * (nz * o);
+            const double vf = 1 - supOut - subOut;
+            ux = vf * rx + (1.0 - vf) * I.ux;
+            uy = vf * ry + (1.0 - vf) * I.uy;
+            uz = vf * rz + (1.0 - vf) * I.uz;
+            break;
+        }
+        default: ux = I.ux; uy = I.uy; uz = I.uz;
+    }
+    // enthalpy from the temperature BC
+    T hb;
+    switch (B.T_type) {
+        case RDST_BC_FIXED_VALUE: hb = T(th.Cp * B.T_value); break;
+        case RDST_BC_INLET_OUTLET: {
+            const double vf = 1.0 - mpos0(phib);
+            hb = vf * (th.Cp * B.T_value) + (1.0 - vf) * I.h;
+            break;
+        }
+        case RDST_BC_FARFIELD: {
+            const T dUn = nx * (B.U_inf[0] - I.ux) + ny * (B.U_inf[1] - I.uy) + nz * (B.U_inf[2] - I.uz);
+            const T pb = 0.5 * (B.p_inf + I.p - rho0c0 * dUn);
+            const T RI = I.p / I.rho / I.Tt;
+            const T rhoa = B.p_inf / B.T_inf / RI;
+            const T cI2 = cI * cI;
+            const T subsIn = pb / (rhoa + (pb - B.p_inf) / cI2) / RI;
+            const T subsOutVF = pb / (I.p + RI * I.Tt * (pb - I.p) / cI2);
+            const T TrefH = supIn * B.T_inf + subIn * subsIn;
+            const T vf = 1 - supOut - subsOutVF * subOut;
+            hb = vf * (th.Cp * TrefH) + (1.0 - vf) * I.h;
+            break;
+        }
+        default: hb = I.h;
+    }
+    T Tb;
+    if (B.T_type == RDST_BC_FIXED_VALUE) { Tb = T(B.T_value); hb = T(th.Cp * B.T_value); }
+    else Tb = hb / th.Cp;
+    const T psib = 1.0 / (th.R * Tb);
+    // pressure
+    T pb;
+    switch (B.p_type) {
+        case RDST_BC_FIXED_VALUE: pb = T(B.p_value); break;
+        case RDST_BC_INLET_OUTLET: {
+            const double vf = 1.0 - mpos0(phib);
+            pb = vf * B.p_value + (1.0 - vf) * I.p;
+            break;
+        }
+        case RDST_BC_FARFIELD: {
+            const T dUn = nx * (B.U_inf[0] - I.ux) + ny * (B.U_inf[1] - I.uy) + nz * (B.U_inf[2] - I.uz);
+            const T pbf = B.p_inf - rho0c0 * dUn;
+            const T ref = supIn * B.p_inf + subIn * pbf + subOut * B.p_inf;
+            const double vf = 1 - supOut - 0.5 * subIn;
+            pb = vf * ref + (1.0 - vf) * I.p;
+            break;
+        }
+        default: pb = I.p;
+    }
+    pb = clampT(pb, bd.pMin, bd.pMax);
+    BState<T> s;
+    s.p = pb; s.h = hb; s.psi = psib;
+    s.rho = pb * psib;
+    s.rux = s.rho * ux; s.ruy = s.rho * uy; s.ruz = s.rho * uz;
+    s.rhoE = s.rho * (hb + 0.5 * (ux * ux + uy * uy + uz * uz)) - pb;
+    return s;
+}
+
+template <class T>
+__device__ inline SideT<T> side_of(const Prim<T>& q, double gamma) {
+    return make_side(q.rho, q.rux, q.ruy, q.ruz, q.rhoE, q.p, q.h, q.psi, gamma);
+}
+template <class T>
+__device__ inline SideT<T> side_of(const BState<T>& q, double gamma) {
+    return make_side(q.rho, q.rux, q.ruy, q.ruz, q.rhoE, q.p, q.h, q.psi, gamma);
+}
+
+__device__ inline Prim<D1> seeded(const double* W, int i, int k, const Thermo& th, const Bounds& bd) {
+    D1 w[5];
+    for (int q = 0; q < 5; ++q) w[q] = D1(W[5 * (size_t)i + q], q == k ? 1.0 : 0.0);
+    return prim_from_w(w[0], w[1], w[2], w[3], w[4], th, bd);
+}
+
+// one thread per cell: its row of J = dR/dW (diagonal block and the block of
+// every internal face, at this row's slots)
+template <int S>
+__global__ __launch_bounds__(256) void k_adj_jacobian(MeshDev m, Fields F, const BcPatchDev* bc, Thermo th, Bounds bd,
+                                                      int lowMach, double Mainf, double* J) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nC) return;
+    const size_t E = (size_t)m.ellW * m.nC;
+    const double rV = 1.0 / m.V[c];
+    const double gamma = th.gamma;
+    double Jcc[25];
+    for (int q = 0; q < 25; ++q) Jcc[q] = 0.0;
+    for (int ke = 0; ke < m.ellW; ++ke) {
+        const size_t e = (size_t)ke * m.nC + c;
+        const uint32_t code = m.ellCode[e];
+        const uint32_t kind = code >> 30;
+        if (kind == ELL_NONE) break;
+        const int idx = (int)(code & ELL_IDX_MASK);
+        const double sx = m.ellSf[e], sy = m.ellSf[E + e], sz = m.ellSf[2 * E + e], ms = m.ellMagSf[e];
+        if (kind == ELL_BOUNDARY) {
+            const int k = m.nI + idx;
+            const double nx = sx / ms, ny = sy / ms, nz = sz / ms;
+            for (int kk = 0; kk < 5; ++kk) {
+                const Prim<D1> I = seeded(F.W, c, kk, th, bd);
+                const BState<D1> bs = bc_state(bc[m.bpatch[idx]], F, idx, k, nx, ny, nz, ms, I, th, bd);
+                const SideT<D1> L = side_of(bs, gamma);
+                const Flux5T<D1> o = eval_flux<S>(L, L, sx, sy, sz, ms, gamma, 1.0, lowMach, Mainf);
+                Jcc[0 * 5 + kk] += o.phi.d * rV;
+                Jcc[1 * 5 + kk] += o.fx.d * rV;
+                Jcc[2 * 5 + kk] += o.fy.d * rV;
+                Jcc[3 * 5 + kk] += o.fz.d * rV;
+                Jcc[4 * 5 + kk] += o.fE.d * rV;
+            }
+            continue;
+        }
+        const bool own = kind == ELL_OWNER;
+        const double sgn = own ? rV : -rV;
+        const int fi = m.ellFace[e];
+        double* Jco = J + 25 * (size_t)(own ? m.slotUpper[fi] : m.slotLower[fi]);
+        // derivative w.r.t. this cell's W, then w.r.t. the other cell's W
+        for (int side = 0; side < 2; ++side) {
+            for (int kk = 0; kk < 5; ++kk) {
+                const Prim<D1> Pc = seeded(F.W, c, side == 0 ? kk : -1, th, bd);
+                const Prim<D1> Po = seeded(F.W, idx, side == 1 ? kk : -1, th, bd);
+                const SideT<D1> sc = side_of(Pc, gamma), so = side_of(Po, gamma);
+                const Flux5T<D1> o = own ? eval_flux<S>(sc, so, sx, sy, sz, ms, gamma, 1.0, lowMach, Mainf)
+                                         : eval_flux<S>(so, sc, sx, sy, sz, ms, gamma, 1.0, lowMach, Mainf);
+                const double d[5] = {o.phi.d, o.fx.d, o.fy.d, o.fz.d, o.fE.d};
+                if (side == 0)
+                    for (int r = 0; r < 5; ++r) Jcc[r * 5 + kk] += sgn * d[r];
+                else
+                    for (int r = 0; r < 5; ++r) Jco[r * 5 + kk] = sgn * d[r];
+            }
+        }
+    }
+    double* Jd = J + 25 * (size_t)m.slotDiag[c];
+    for (int q = 0; q < 25; ++q) Jd[q] = Jcc[q];
+}
+
+// the residual itself from W by the same chain (the value parts of the duals)
+template <int S>
+__global__ __launch_bounds__(256) void k_adj_residual(MeshDev m, Fields F, const BcPatchDev* bc, Thermo th, Bounds bd,
+                                                      int lowMach, double Mainf, double* R) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nC) return;
+    const size_t E = (size_t)m.ellW * m.nC;
+    const double gamma = th.gamma;
+    const Prim<double> Pc = prim_from_w(F.W[5 * (size_t)c], F.W[5 * (size_t)c + 1], F.W[5 * (size_t)c + 2],
+                                        F.W[5 * (size_t)c + 3], F.W[5 * (size_t)c + 4], th, bd);
+    const SideT<double> sc = side_of(Pc, gamma);
+    double a[5] = {0, 0, 0, 0, 0};
+    for (int ke = 0; ke < m.ellW; ++ke) {
+        const size_t e = (size_t)ke * m.nC + c;
+        const uint32_t code = m.ellCode[e];
+        const uint32_t kind = code >> 30;
+        if (kind == ELL_NONE) break;
+        const int idx = (int)(code & ELL_IDX_MASK);
+        const double sx = m.ellSf[e], sy = m.ellSf[E + e], sz = m.ellSf[2 * E + e], ms = m.ellMagSf[e];
+        Flux5T<double> o;
+        double sg = 1.0;
+        if (kind == ELL_BOUNDARY) {
+            const BState<double> bs = bc_state(bc[m.bpatch[idx]], F, idx, m.nI + idx, sx / ms, sy / ms, sz / ms, ms, Pc,
+                                               th, bd);
+            const SideT<double> L = side_of(bs, gamma);
+            o = eval_flux<S>(L, L, sx, sy, sz, ms, gamma, 1.0, lowMach, Mainf);
+        } else {
+            const double* Wo = F.W + 5 * (size_t)idx;
+            const Prim<double> Po = prim_from_w(Wo[0], Wo[1], Wo[2], Wo[3], Wo[4], th, bd);
+            const SideT<double> so = side_of(Po, gamma);
+            if (kind == ELL_OWNER) o = eval_flux<S>(sc, so, sx, sy, sz, ms, gamma, 1.0, lowMach, Mainf);
+            else { o = eval_flux<S>(so, sc, sx, sy, sz, ms, gamma, 1.0, lowMach, Mainf); sg = -1.0; }
+        }
+        a[0] += sg * o.phi; a[1] += sg * o.fx; a[2] += sg * o.fy; a[3] += sg * o.fz; a[4] += sg * o.fE;
+    }
+    const double V = m.V[c];
+    for (int r = 0; r < 5; ++r) R[5 * (size_t)c + r] = a[r] / V;
+}
+
+// dJ/dW of J = sum_{b in patch} (p_b - pRef)(Sf_b . dir)/q
+__global__ __launch_bounds__(256) void k_adj_objective(MeshDev m, Fields F, const BcPatchDev* bc, Thermo th, Bounds bd,
+                                                       int patch, double dx, double dy, double dz, double rq, double* g) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nC) return;
+    const size_t E = (size_t)m.ellW * m.nC;
+    double gc[5] = {0, 0, 0, 0, 0};
+    for (int ke = 0; ke < m.ellW; ++ke) {
+        const size_t e = (size_t)ke * m.nC + c;
+        const uint32_t code = m.ellCode[e];
+        const uint32_t kind = code >> 30;
+        if (kind == ELL_NONE) break;
+        if (kind != ELL_BOUNDARY) continue;
+        const int idx = (int)(code & ELL_IDX_MASK);
+        if (m.bpatch[idx] != patch) continue;
+        const double sx = m.ellSf[e], sy = m.ellSf[E + e], sz = m.ellSf[2 * E + e], ms = m.ellMagSf[e];
+        const double w = (sx * dx + sy * dy + sz * dz) * rq;
+        for (int kk = 0; kk < 5; ++kk) {
+            const Prim<D1> I = seeded(F.W, c, kk, th, bd);
+            const BState<D1> bs = bc_state(bc[patch], F, idx, m.nI + idx, sx / ms, sy / ms, sz / ms, ms, I, th, bd);
+            gc[kk] += bs.p.d * w;
+        }
+    }
+    for (int kk = 0; kk < 5; ++kk) g[5 * (size_t)c + kk] = gc[kk];
+}
+
+// J^T on the same slots: (r, c) <- (c, r)^T
+__global__ void k_transpose_faces(int nI, const int32_t* slotUpper, const int32_t* slotLower, const double* J,
+                                  double* JT) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nI) return;
+    const double* U = J + 25 * (size_t)slotUpper[f];
+    const double* L = J + 25 * (size_t)slotLower[f];
+    double* TU = JT + 25 * (size_t)slotUpper[f];
+    double* TL = JT + 25 * (size_t)slotLower[f];
+    for (int r = 0; r < 5; ++r)
+        for (int q = 0; q < 5; ++q) {
+            TU[r * 5 + q] = L[q * 5 + r];
+            TL[r * 5 + q] = U[q * 5 + r];
+        }
+}
+__global__ void k_transpose_diag(int n, const int32_t* slotDiag, const double* J, double* JT) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const double* D = J + 25 * (size_t)slotDiag[c];
+    double* T = JT + 25 * (size_t)slotDiag[c];
+    for (int r = 0; r < 5; ++r)
+        for (int q = 0; q < 5; ++q) T[r * 5 + q] = D[q * 5 + r];
+}
+
+Bounds bounds_of(const Case& c) {
+    Bounds b;
+    b.rhoMin = c.st.rho_min; b.rhoMax = c.st.rho_max;
+    b.hMin = c.th.Cp * c.st.T_min; b.hMax = c.th.Cp * c.st.T_max;
+    b.pMin = c.st.p_min; b.pMax = c.st.p_max;
+    return b;
+}
+
+template <int S>
+void launch_jac(Case& c, double* J) {
+    const int nb = blocks_for(c.mesh->nC, 256);
+    hipLaunchKernelGGL((k_adj_jacobian<S>), dim3(nb), dim3(256), 0, c.stream, c.meshDev(), c.f, c.d_bc.p, c.th,
+                       bounds_of(c), c.st.low_mach, c.st.Mainf, J);
+}
+template <int S>
+void launch_res(Case& c, double* R) {
+    const int nb = blocks_for(c.mesh->nC, 256);
+    hipLaunchKernelGGL((k_adj_residual<S>), dim3(nb), dim3(256), 0, c.stream, c.meshDev(), c.f, c.d_bc.p, c.th,
+                       bounds_of(c), c.st.low_mach, c.st.Mainf, R);
+}
+
+}  // namespace
+
+void adjoint_check_scheme(const Case& c) {
+    if (c.st.flux_scheme == RDST_FLUX_HLLCP)
+        throw ArgError("adjoint: HLLCP's pressure sensor couples second neighbours; use AUSM+, AUSM+up, HLL, HLLC or "
+                       "Kurganov", RDST_ERR_UNKNOWN_NAME);
+    if (c.st.viscous || c.sa.on)
+        throw ArgError("adjoint: the exact linearisation covers the inviscid residual (settings.viscous = 0, laminar)");
+}
+
+void adjoint_linearise(Case& c) {
+    adjoint_check_scheme(c);
+    const Mesh& m = *c.mesh;
+    const size_t nn = 25 * (size_t)m.nnzb;
+    if (c.adjJ.n != nn) { c.adjJ.alloc(nn); c.adjJT.alloc(nn); }
+    c.adjJ.zero();
+    switch (c.st.flux_scheme) {
+        case RDST_FLUX_AUSMPLUS: launch_jac<RDST_FLUX_AUSMPLUS>(c, c.adjJ.p); break;
+        case RDST_FLUX_AUSMPLUSUP: launch_jac<RDST_FLUX_AUSMPLUSUP>(c, c.adjJ.p); break;
+        case RDST_FLUX_HLL: launch_jac<RDST_FLUX_HLL>(c, c.adjJ.p); break;
+        case RDST_FLUX_HLLC: launch_jac<RDST_FLUX_HLLC>(c, c.adjJ.p); break;
+        default: launch_jac<RDST_FLUX_KURGANOV>(c, c.adjJ.p); break;
+    }
+    if (m.nI)
+        hipLaunchKernelGGL(k_transpose_faces, dim3(blocks_for(m.nI, 256)), dim3(256), 0, c.stream, m.nI, m.d_slotUpper.p,
+                           m.d_slotLower.p, c.adjJ.p, c.adjJT.p);
+    hipLaunchKernelGGL(k_transpose_diag, dim3(blocks_for(m.nC, 256)), dim3(256), 0, c.stream, m.nC, m.d_slotDiag.p,
+                       c.adjJ.p, c.adjJT.p);
+    RDST_HIP(hipGetLastError());
+    c.hasAdjJ = true;
+}
+
+void adjoint_residual(Case& c, double* R) {
+    adjoint_check_scheme(c);
+    switch (c.st.flux_scheme) {
+        case RDST_FLUX_AUSMPLUS: launch_res<RDST_FLUX_AUSMPLUS>(c, R); break;
+        case RDST_FLUX_AUSMPLUSUP: launch_res<RDST_FLUX_AUSMPLUSUP>(c, R); break;
+        case RDST_FLUX_HLL: launch_res<RDST_FLUX_HLL>(c, R); break;
+        case RDST_FLUX_HLLC: launch_res<RDST_FLUX_HLLC>(c, R); break;
+        default: launch_res<RDST_FLUX_KURGANOV>(c, R); break;
+    }
+    RDST_HIP(hipGetLastError());
+}
+
+void adjoint_objective_gradient(Case& c, int patch, const double* dir, double q, double* g) {
+    adjoint_check_scheme(c);
+    const Mesh& m = *c.mesh;
+    hipLaunchKernelGGL(k_adj_objective, dim3(blocks_for(m.nC, 256)), dim3(256), 0, c.stream, c.meshDev(), c.f, c.d_bc.p,
+                       c.th, bounds_of(c), patch, dir[0], dir[1], dir[2], 1.0 / q, g);
+    RDST_HIP(hipGetLastError());
+}
+
+}  // namespace rdst

@@ -22,6 +22,7 @@ MeshDev Case::meshDev() const {
     d.owner = m.d_owner.p; d.neigh = m.d_neigh.p; d.bcell = m.d_bcell.p; d.bpatch = m.d_bpatch.p; d.bwall = m.d_bWall.p;
     d.Sf = m.d_Sf.p; d.magSf = m.d_magSf.p; d.w = m.d_w.p; d.dc = m.d_dc.p; d.V = m.d_V.p;
     d.ellCode = m.d_ellCode.p; d.ellFace = m.d_ellFace.p; d.ellSlot = m.d_ellSlot.p; d.slotDiag = m.d_slotDiag.p;
+    d.slotUpper = m.d_slotUpper.p; d.slotLower = m.d_slotLower.p;
     d.ellSf = m.d_ellSf.p; d.ellMagSf = m.d_ellMagSf.p; d.ellWt = m.d_ellWt.p; d.ellDc = m.d_ellDc.p;
     d.ellCorr = m.d_ellCorr.p;
     return d;
@@ -480,6 +481,89 @@ int rdst_turbulence_correct(rdst_case* cc, rdst_solver_perf* perf) {
         if (!c) throw ArgError("null case");
         sa_correct(*c, perf);
         RDST_HIP(hipStreamSynchronize(c->stream));
+    })
+}
+
+static Case* adjoint_case(rdst_case* cc) {
+    Case* c = reinterpret_cast<Case*>(cc);
+    if (!c) throw ArgError("null case");
+    if (!c->hasState) throw ArgError("adjoint: no state", RDST_ERR_STATE);
+    adjoint_check_scheme(*c);
+    return c;
+}
+
+int rdst_adjoint_linearise(rdst_case* cc) {
+    RDST_API_BODY({
+        Case* c = adjoint_case(cc);
+        adjoint_linearise(*c);
+        RDST_HIP(hipStreamSynchronize(c->stream));
+    })
+}
+
+int rdst_adjoint_apply(rdst_case* cc, int transpose, const double* x, double* y) {
+    RDST_API_BODY({
+        Case* c = adjoint_case(cc);
+        if (!x || !y) throw ArgError("rdst_adjoint_apply: null array");
+        const size_t n5 = 5 * (size_t)c->mesh->nC;
+        DBuf<double> dx, dy;
+        dx.upload(x, n5);
+        dy.alloc(n5);
+        adjoint_apply(*c, transpose != 0, dx.p, dy.p);
+        RDST_HIP(hipStreamSynchronize(c->stream));
+        RDST_HIP(hipMemcpy(y, dy.p, n5 * sizeof(double), hipMemcpyDeviceToHost));
+    })
+}
+
+int rdst_adjoint_residual(rdst_case* cc, double* R) {
+    RDST_API_BODY({
+        Case* c = adjoint_case(cc);
+        if (!R) throw ArgError("rdst_adjoint_residual: null array");
+        const size_t n5 = 5 * (size_t)c->mesh->nC;
+        DBuf<double> d;
+        d.alloc(n5);
+        adjoint_residual(*c, d.p);
+        RDST_HIP(hipStreamSynchronize(c->stream));
+        RDST_HIP(hipMemcpy(R, d.p, n5 * sizeof(double), hipMemcpyDeviceToHost));
+    })
+}
+
+static void check_objective(const Case& c, int patch, const double* dir, double q) {
+    if (!dir) throw ArgError("adjoint: null direction");
+    if (patch < 0 || patch >= c.mesh->nPatches) throw ArgError("adjoint: patch index out of range");
+    if (!(q != 0.0) || !std::isfinite(q)) throw ArgError("adjoint: the reference dynamic pressure x area q must be finite and nonzero");
+}
+
+int rdst_adjoint_objective_gradient(rdst_case* cc, int patch, const double* dir, double q, double* g) {
+    RDST_API_BODY({
+        Case* c = adjoint_case(cc);
+        check_objective(*c, patch, dir, q);
+        if (!g) throw ArgError("rdst_adjoint_objective_gradient: null array");
+        const size_t n5 = 5 * (size_t)c->mesh->nC;
+        DBuf<double> d;
+        d.alloc(n5);
+        adjoint_objective_gradient(*c, patch, dir, q, d.p);
+        RDST_HIP(hipStreamSynchronize(c->stream));
+        RDST_HIP(hipMemcpy(g, d.p, n5 * sizeof(double), hipMemcpyDeviceToHost));
+    })
+}
+
+int rdst_adjoint_solve(rdst_case* cc, int patch, const double* dir, double q, const rdst_adjoint_settings* s,
+                       double* lambda, rdst_solver_perf* perf) {
+    RDST_API_BODY({
+        Case* c = adjoint_case(cc);
+        check_objective(*c, patch, dir, q);
+        if (!s || !lambda) throw ArgError("rdst_adjoint_solve: null argument");
+        if (s->precision != RDST_PREC_FP64 && s->precision != RDST_PREC_MIXED)
+            throw ArgError("rdst_adjoint_solve: unknown precision", RDST_ERR_UNKNOWN_NAME);
+        if (s->n_directions < 1 || s->max_cycles < 1) throw ArgError("rdst_adjoint_solve: n_directions, max_cycles >= 1");
+        adjoint_linearise(*c);  // at the current state
+        const size_t n5 = 5 * (size_t)c->mesh->nC;
+        DBuf<double> g, x;
+        g.alloc(n5);
+        x.upload(lambda, n5);
+        adjoint_objective_gradient(*c, patch, dir, q, g.p);
+        adjoint_solve(*c, g.p, x.p, *s, perf);
+        RDST_HIP(hipMemcpy(lambda, x.p, n5 * sizeof(double), hipMemcpyDeviceToHost));
     })
 }
 

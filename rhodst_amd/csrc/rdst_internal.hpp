@@ -199,6 +199,7 @@ struct MeshDev {
     const uint32_t* ellCode;
     const int32_t *ellFace, *ellSlot;
     const int32_t* slotDiag;
+    const int32_t *slotUpper, *slotLower;  // per internal face: (owner row, neighbour col), (neighbour row, owner col)
     const double *ellSf, *ellMagSf, *ellWt, *ellDc, *ellCorr;
 };
 
@@ -228,6 +229,10 @@ struct Case {
     double residual = -1, residualRatio = 1, courantScalar = 1;
     std::vector<double> hostScratch;
     SaState sa;
+    // discrete adjoint: exact J = dR/dW and its transpose on the block-CSR slots
+    DBuf<double> adjJ, adjJT;
+    DBuf<float> adjJT32;
+    bool hasAdjJ = false;
 
     MeshDev meshDev() const;
 };
@@ -253,6 +258,12 @@ void sa_set(Case& c, const rdst_turbulence* t, const rdst_scalar_bc* bcs, const 
             const double* nuTilda);
 void sa_correct(Case& c, rdst_solver_perf* perf);
 void sa_put_nut(Case& c, const double* nuTilda);
+void adjoint_check_scheme(const Case& c);
+void adjoint_linearise(Case& c);
+void adjoint_residual(Case& c, double* R);
+void adjoint_objective_gradient(Case& c, int patch, const double* dir, double q, double* g);
+void adjoint_apply(Case& c, bool transpose, const double* x, double* y);
+void adjoint_solve(Case& c, const double* g, double* lambda, const rdst_adjoint_settings& as, rdst_solver_perf* perf);
 
 inline int blocks_for(long n, int bs) { return (int)((n + bs - 1) / bs); }
 

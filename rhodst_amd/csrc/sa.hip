@@ -247,7 +247,7 @@ __global__ __launch_bounds__(256) void k_sa_assemble(MeshDev m, Fields F, SaDev 
         }
         off -= coef;
         diag += coef;
-        s.A[m.ellSlot[e]] = off;
+        s.A[own ? m.slotUpper[fi] : m.slotLower[fi]] = off;  // this row's slot of the face
         sumOff += fabs(off);
     }
     // cell sources

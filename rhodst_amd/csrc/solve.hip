@@ -463,13 +463,19 @@ __global__ __launch_bounds__(256) void k_pilu_factor(const int32_t* partLevStart
 }
 
 // x = M^-1 r for the partitioned factor; the partition's vector stays in LDS
-// for the forward and the backward sweep (reverse local levels)
+// for the forward and the backward sweep (reverse local levels).  Five lanes
+// per block row, lane = component r: each lane reads row r of every 5x5 block
+// (the five lanes of a row read the block's 25 contiguous entries together)
+// and a 64-lane wavefront carries 12 rows; the Dinv product gathers the five
+// components of a row with wavefront shuffles.  Every lane of the block runs
+// the same number of passes, so the shuffles and barriers stay uniform.
+constexpr int PILU_THREADS = 320;  // 5 wavefronts x 12 rows
 template <class TM, class TV>
-__global__ __launch_bounds__(256) void k_pilu_apply(const int32_t* partRowStart, const int32_t* partLevStart,
-                                                    const int32_t* partLevPtr, const int32_t* partRows,
-                                                    const int32_t* rowPtr, const int32_t* cols, const int32_t* diagPos,
-                                                    const int32_t* localCol, const TM* LU, const TM* Dinv, const TV* r,
-                                                    TV* x) {
+__global__ __launch_bounds__(PILU_THREADS) void k_pilu_apply(const int32_t* partRowStart, const int32_t* partLevStart,
+                                                             const int32_t* partLevPtr, const int32_t* partRows,
+                                                             const int32_t* rowPtr, const int32_t* cols,
+                                                             const int32_t* diagPos, const int32_t* localCol,
+                                                             const TM* LU, const TM* Dinv, const TV* r, TV* x) {
     extern __shared__ unsigned char smem_raw[];
     TV* y = reinterpret_cast<TV*>(smem_raw);
     const int pt = blockIdx.x;
@@ -477,53 +483,58 @@ __global__ __launch_bounds__(256) void k_pilu_apply(const int32_t* partRowStart,
     const int nl5 = (re - rs) * 5;
     for (int t = threadIdx.x; t < nl5; t += blockDim.x) y[t] = r[5 * (size_t)partRows[rs + t / 5] + t % 5];
     __syncthreads();
+    const int wl = threadIdx.x & 63;                 // lane in the wavefront
+    const int comp = wl % 5;                         // block row handled by this lane
+    const int rowInWave = wl / 5;                    // 0..11 (12 = idle lanes 60..63)
+    const bool laneOn = wl < 60;
+    const int slot = (threadIdx.x >> 6) * 12 + rowInWave;
+    const int rowsPerPass = (blockDim.x >> 6) * 12;
+    const int base5 = wl - comp;                     // first lane of this row in the wavefront
     const int l0 = partLevStart[pt], l1 = partLevStart[pt + 1];
     for (int L = l0; L + 1 < l1; ++L) {
         const int s = partLevPtr[L], e = partLevPtr[L + 1];
-        for (int t = s + (int)threadIdx.x; t < e; t += blockDim.x) {
-            const int i = partRows[t];
-            TV* yi = y + 5 * (t - rs);
-            TV a0 = yi[0], a1 = yi[1], a2 = yi[2], a3 = yi[3], a4 = yi[4];
-            for (int p = rowPtr[i]; p < diagPos[i]; ++p) {
-                const int lc = localCol[p];
-                if (lc < 0) continue;
-                const TM* B = LU + 25 * (size_t)p;
-                const TV* yv = y + 5 * lc;
-                const TV y0 = yv[0], y1 = yv[1], y2 = yv[2], y3 = yv[3], y4 = yv[4];
-                a0 -= (TV)(B[0] * y0 + B[1] * y1 + B[2] * y2 + B[3] * y3 + B[4] * y4);
-                a1 -= (TV)(B[5] * y0 + B[6] * y1 + B[7] * y2 + B[8] * y3 + B[9] * y4);
-                a2 -= (TV)(B[10] * y0 + B[11] * y1 + B[12] * y2 + B[13] * y3 + B[14] * y4);
-                a3 -= (TV)(B[15] * y0 + B[16] * y1 + B[17] * y2 + B[18] * y3 + B[19] * y4);
-                a4 -= (TV)(B[20] * y0 + B[21] * y1 + B[22] * y2 + B[23] * y3 + B[24] * y4);
+        for (int b = s; b < e; b += rowsPerPass) {
+            const int t = b + slot;
+            if (laneOn && t < e) {
+                const int i = partRows[t];
+                TV a = y[5 * (t - rs) + comp];
+                for (int p = rowPtr[i]; p < diagPos[i]; ++p) {
+                    const int lc = localCol[p];
+                    if (lc < 0) continue;
+                    const TM* B = LU + 25 * (size_t)p + 5 * comp;
+                    const TV* yv = y + 5 * lc;
+                    a -= (TV)(B[0] * yv[0] + B[1] * yv[1] + B[2] * yv[2] + B[3] * yv[3] + B[4] * yv[4]);
+                }
+                y[5 * (t - rs) + comp] = a;
             }
-            yi[0] = a0; yi[1] = a1; yi[2] = a2; yi[3] = a3; yi[4] = a4;
         }
         __syncthreads();
     }
     for (int L = l1 - 2; L >= l0; --L) {
         const int s = partLevPtr[L], e = partLevPtr[L + 1];
-        for (int t = s + (int)threadIdx.x; t < e; t += blockDim.x) {
-            const int i = partRows[t];
-            TV* yi = y + 5 * (t - rs);
-            TV a0 = yi[0], a1 = yi[1], a2 = yi[2], a3 = yi[3], a4 = yi[4];
-            for (int p = diagPos[i] + 1; p < rowPtr[i + 1]; ++p) {
-                const int lc = localCol[p];
-                if (lc < 0) continue;
-                const TM* B = LU + 25 * (size_t)p;
-                const TV* yv = y + 5 * lc;
-                const TV y0 = yv[0], y1 = yv[1], y2 = yv[2], y3 = yv[3], y4 = yv[4];
-                a0 -= (TV)(B[0] * y0 + B[1] * y1 + B[2] * y2 + B[3] * y3 + B[4] * y4);
-                a1 -= (TV)(B[5] * y0 + B[6] * y1 + B[7] * y2 + B[8] * y3 + B[9] * y4);
-                a2 -= (TV)(B[10] * y0 + B[11] * y1 + B[12] * y2 + B[13] * y3 + B[14] * y4);
-                a3 -= (TV)(B[15] * y0 + B[16] * y1 + B[17] * y2 + B[18] * y3 + B[19] * y4);
-                a4 -= (TV)(B[20] * y0 + B[21] * y1 + B[22] * y2 + B[23] * y3 + B[24] * y4);
+        for (int b = s; b < e; b += rowsPerPass) {
+            const int t = b + slot;
+            const bool on = laneOn && t < e;
+            TV a = 0;
+            int i = 0;
+            if (on) {
+                i = partRows[t];
+                a = y[5 * (t - rs) + comp];
+                for (int p = diagPos[i] + 1; p < rowPtr[i + 1]; ++p) {
+                    const int lc = localCol[p];
+                    if (lc < 0) continue;
+                    const TM* B = LU + 25 * (size_t)p + 5 * comp;
+                    const TV* yv = y + 5 * lc;
+                    a -= (TV)(B[0] * yv[0] + B[1] * yv[1] + B[2] * yv[2] + B[3] * yv[3] + B[4] * yv[4]);
+                }
             }
-            const TM* D = Dinv + 25 * (size_t)i;
-            yi[0] = (TV)(D[0] * a0 + D[1] * a1 + D[2] * a2 + D[3] * a3 + D[4] * a4);
-            yi[1] = (TV)(D[5] * a0 + D[6] * a1 + D[7] * a2 + D[8] * a3 + D[9] * a4);
-            yi[2] = (TV)(D[10] * a0 + D[11] * a1 + D[12] * a2 + D[13] * a3 + D[14] * a4);
-            yi[3] = (TV)(D[15] * a0 + D[16] * a1 + D[17] * a2 + D[18] * a3 + D[19] * a4);
-            yi[4] = (TV)(D[20] * a0 + D[21] * a1 + D[22] * a2 + D[23] * a3 + D[24] * a4);
+            // the five components of the row, from the row's lanes
+            const TV a0 = __shfl(a, base5 + 0, 64), a1 = __shfl(a, base5 + 1, 64), a2 = __shfl(a, base5 + 2, 64),
+                     a3 = __shfl(a, base5 + 3, 64), a4 = __shfl(a, base5 + 4, 64);
+            if (on) {
+                const TM* D = Dinv + 25 * (size_t)i + 5 * comp;
+                y[5 * (t - rs) + comp] = (TV)(D[0] * a0 + D[1] * a1 + D[2] * a2 + D[3] * a3 + D[4] * a4);
+            }
         }
         __syncthreads();
     }
@@ -710,7 +721,7 @@ void precondition(Ctx& k, const TM* LU, const TM* Dinv, const TV* r, TV* x, TV* 
     }
     if (k.c.st.ilu_order == RDST_ILU_PARTITIONED) {
         const size_t shmem = sizeof(TV) * 5 * (size_t)m.maxPartRows;
-        hipLaunchKernelGGL((k_pilu_apply<TM, TV>), dim3(m.nParts), dim3(256), shmem, k.s, m.d_partRowStart.p,
+        hipLaunchKernelGGL((k_pilu_apply<TM, TV>), dim3(m.nParts), dim3(PILU_THREADS), shmem, k.s, m.d_partRowStart.p,
                            m.d_partLevStart.p, m.d_partLevPtr.p, m.d_partRows.p, m.d_rowPtr.p, m.d_cols.p,
                            m.d_slotDiag.p, m.d_localCol.p, LU, Dinv, r, x);
         return;
@@ -822,10 +833,9 @@ void gmres_cycle(Ctx& k, const TM* A, const TM* LU, const TM* Dinv, TV* z, const
     hipLaunchKernelGGL((k_update_x<TV, TV>), dim3(G), dim3(RED_BS), 0, k.s, n5, z, V, yh, m);
 }
 
-rdst_solver_perf gmres_fp64(Ctx& k, double* x, const double* b) {
+rdst_solver_perf gmres_fp64(Ctx& k, const double* A, double* x, const double* b) {
     SolverWork& w = k.w;
     const long n5 = w.n5;
-    const double* A = k.c.A64.p;
     rdst_solver_perf perf{};
     norm_factor(k, A, x, b);
     spmv(k, A, x, w.wA.p);
@@ -845,15 +855,16 @@ rdst_solver_perf gmres_fp64(Ctx& k, double* x, const double* b) {
     return perf;
 }
 
-// mixed precision: FP64 outer defect correction r = b - A x (A held in FP32,
-// promoted), inner FP32 GMRES cycle on A e = r with FP32 ILU and basis
-rdst_solver_perf gmres_mixed(Ctx& k, double* x, const double* b) {
+// mixed precision: FP64 outer defect correction r = b - Aout x (Aout FP32
+// promoted for the implicit step, FP64 for the adjoint), inner FP32 GMRES
+// cycle on A e = r with the FP32 matrix, FP32 ILU and FP32 basis
+template <class TO>
+rdst_solver_perf gmres_mixed(Ctx& k, const TO* Aout, const float* A, double* x, const double* b) {
     SolverWork& w = k.w;
     const long n5 = w.n5;
-    const float* A = k.c.A32.p;
     rdst_solver_perf perf{};
-    norm_factor(k, A, x, b);
-    spmv(k, A, x, w.wA.p);
+    norm_factor(k, Aout, x, b);
+    spmv(k, Aout, x, w.wA.p);
     hipLaunchKernelGGL(k_sub, dim3(k.G), dim3(RED_BS), 0, k.s, n5, w.r.p, b, w.wA.p);
     cmptmag(k, w.r.p, w.scal.p + S_INIT, w.scal.p + S_NORM);
     read_scalars(k, S_INIT, 5, perf.initial_residual);
@@ -868,13 +879,18 @@ rdst_solver_perf gmres_mixed(Ctx& k, double* x, const double* b) {
         // x += e in FP64
         hipLaunchKernelGGL((k_copy<double, float>), dim3(k.G), dim3(RED_BS), 0, k.s, n5, w.wA.p, w.t32.p);
         hipLaunchKernelGGL(k_add, dim3(k.G), dim3(RED_BS), 0, k.s, n5, x, w.wA.p);
-        spmv(k, A, x, w.wA.p);
+        spmv(k, Aout, x, w.wA.p);
         hipLaunchKernelGGL(k_sub, dim3(k.G), dim3(RED_BS), 0, k.s, n5, w.r.p, b, w.wA.p);
         cmptmag(k, w.r.p, w.scal.p + S_FIN, w.scal.p + S_NORM);
         read_scalars(k, S_FIN, 5, perf.final_residual);
         perf.n_iterations++;
     } while (!converged(k.c.st, perf));
     return perf;
+}
+
+template <class TA, class TB>
+__global__ void k_cast(long n, const TA* a, TB* b) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = (TB)a[i];
 }
 
 // foam-extend BlockBiCGStabSolver (right-preconditioned), FP64
@@ -946,20 +962,68 @@ void block_solve(Case& c, rdst_solver_perf* perfOut) {
         // sourceW = A.W + s; solve A.W = sourceW from W (fvBlockMatrixDST.C:112-123)
         spmv(k, (const double*)c.A64.p, (const double*)W, w.b.p);
         hipLaunchKernelGGL(k_add, dim3(k.G), dim3(RED_BS), 0, k.s, n5, w.b.p, c.src.p);
-        perf = (c.st.solver == RDST_SOLVER_GMRES) ? gmres_fp64(k, W, w.b.p) : bicgstab_fp64(k, W, w.b.p);
+        perf = (c.st.solver == RDST_SOLVER_GMRES) ? gmres_fp64(k, c.A64.p, W, w.b.p) : bicgstab_fp64(k, W, w.b.p);
     } else {
         // A.dW = s from 0, W += dW (deltaW mode; the mixed form always solves for the correction)
         RDST_HIP(hipMemsetAsync(w.x0.p, 0, sizeof(double) * n5, k.s));
         DBuf<double> dW;
         dW.alloc(n5);
         RDST_HIP(hipMemsetAsync(dW.p, 0, sizeof(double) * n5, k.s));
-        if (mixed) perf = gmres_mixed(k, dW.p, c.src.p);
-        else perf = (c.st.solver == RDST_SOLVER_GMRES) ? gmres_fp64(k, dW.p, c.src.p) : bicgstab_fp64(k, dW.p, c.src.p);
+        if (mixed) perf = gmres_mixed<float>(k, c.A32.p, c.A32.p, dW.p, c.src.p);
+        else perf = (c.st.solver == RDST_SOLVER_GMRES) ? gmres_fp64(k, c.A64.p, dW.p, c.src.p) : bicgstab_fp64(k, dW.p, c.src.p);
         hipLaunchKernelGGL(k_add, dim3(k.G), dim3(RED_BS), 0, k.s, n5, W, dW.p);
         RDST_HIP(hipStreamSynchronize(k.s));
     }
     RDST_HIP(hipGetLastError());
     if (perfOut) *perfOut = perf;
+}
+
+// ---------------------------------------------------------------- adjoint
+void adjoint_apply(Case& c, bool transpose, const double* x, double* y) {
+    if (!c.hasAdjJ) throw ArgError("rdst_adjoint_apply: no linearisation (call rdst_adjoint_linearise)", RDST_ERR_STATE);
+    ensure_work(c);
+    Ctx k{c, *c.mesh, *c.work, c.stream, c.work->G};
+    spmv(k, (const double*)(transpose ? c.adjJT.p : c.adjJ.p), x, y);
+    RDST_HIP(hipGetLastError());
+}
+
+// J^T lambda = g with the primal solver machinery on the transposed
+// matrix: block ILU(0) of J^T (natural or partitioned order as the case's
+// settings) and GMRES(m); mixed: FP32 factors and cycles, FP64 outer residual
+void adjoint_solve(Case& c, const double* g, double* lambda, const rdst_adjoint_settings& as, rdst_solver_perf* perf) {
+    if (!c.hasAdjJ) throw ArgError("rdst_adjoint_solve: no linearisation", RDST_ERR_STATE);
+    const rdst_settings saved = c.st;
+    c.st.precision = as.precision;
+    c.st.n_directions = as.n_directions;
+    c.st.max_iter = as.max_cycles;
+    c.st.min_iter = 1;
+    c.st.tolerance = as.tolerance;
+    c.st.rel_tol = as.rel_tol;
+    c.st.solver = RDST_SOLVER_GMRES;
+    if (c.st.precond == RDST_PRECOND_NONE || c.st.precond == RDST_PRECOND_DIAG) c.st.precond = RDST_PRECOND_ILU0;
+    try {
+        ensure_work(c);
+        Mesh& m = *c.mesh;
+        Ctx k{c, m, *c.work, c.stream, c.work->G};
+        rdst_solver_perf p{};
+        if (as.precision == RDST_PREC_MIXED) {
+            const size_t nn = 25 * (size_t)m.nnzb;
+            if (c.adjJT32.n != nn) c.adjJT32.alloc(nn);
+            hipLaunchKernelGGL((k_cast<double, float>), dim3(std::min(4096, blocks_for((long)nn, 256))), dim3(256), 0,
+                               c.stream, (long)nn, c.adjJT.p, c.adjJT32.p);
+            ilu_factor<float>(k, c.adjJT32.p, k.w.LU32.p, k.w.Dinv32.p);
+            p = gmres_mixed<double>(k, c.adjJT.p, c.adjJT32.p, lambda, g);
+        } else {
+            ilu_factor<double>(k, c.adjJT.p, k.w.LU.p, k.w.Dinv.p);
+            p = gmres_fp64(k, c.adjJT.p, lambda, g);
+        }
+        RDST_HIP(hipStreamSynchronize(c.stream));
+        if (perf) *perf = p;
+    } catch (...) {
+        c.st = saved;
+        throw;
+    }
+    c.st = saved;
 }
 
 }  // namespace rdst

@@ -623,3 +623,64 @@ def wall_distance(mesh: PolyMesh, wall_patches, centres: np.ndarray | None = Non
             tri = table[nearest[:, j], t]
             d = np.minimum(d, _point_triangle_distance(C, A[tri], B[tri], Cc[tri]))
     return d
+
+
+def hicks_henne(xhat: np.ndarray, n_modes: int, width: float = 3.0) -> np.ndarray:
+    """Hicks-Henne bump functions sin(pi xhat^m_i)^width on xhat in [0, 1],
+    peaks at t_i = (i + 1)/(n_modes + 1) (m_i = ln 0.5 / ln t_i); [n_modes, len(xhat)]."""
+    x = np.clip(np.asarray(xhat, dtype=np.float64), 0.0, 1.0)
+    out = np.zeros((n_modes, x.size))
+    for i in range(n_modes):
+        t = (i + 1.0) / (n_modes + 1.0)
+        m = np.log(0.5) / np.log(t)
+        out[i] = np.sin(np.pi * x ** m) ** width
+    return out
+
+
+def bump_channel(nx: int = 48, ny: int = 24, lx: float = 2.0, ly: float = 1.0, bump=(0.5, 1.5),
+                 amplitudes=None, node_disp=None) -> PolyMesh:
+    """A 2-D channel (one cell thick) whose bottom wall carries a bump
+    y_b(x) = sum_i a_i HH_i((x - x0)/(x1 - x0)) over x in bump (Hicks-Henne
+    modes), or explicit wall-normal displacements node_disp[nx + 1] of the
+    bottom nodes; every grid line moves by y_b (1 - y/ly), so the top stays
+    flat.  Patches: inlet (x = 0), outlet (x = lx), bottom (wall), top
+    (symmetryPlane), frontAndBack (empty)."""
+    dx, dy = lx / nx, ly / ny
+    x = np.arange(nx + 1) * dx
+    yb = np.zeros(nx + 1)
+    if amplitudes is not None and len(amplitudes):
+        a = np.asarray(amplitudes, dtype=np.float64)
+        xh = (x - bump[0]) / (bump[1] - bump[0])
+        inside = (xh > 0.0) & (xh < 1.0)
+        yb = np.where(inside, a @ hicks_henne(xh, len(a)), 0.0)
+    if node_disp is not None:
+        yb = yb + np.asarray(node_disp, dtype=np.float64)
+    X, Y = np.meshgrid(x, np.arange(ny + 1) * dy, indexing="ij")
+    Y = Y + yb[:, None] * (1.0 - Y / ly)
+    pts = np.zeros((2, ny + 1, nx + 1, 3))
+    for k, z in enumerate((-0.05, 0.05)):
+        pts[k, :, :, 0] = X.T
+        pts[k, :, :, 1] = Y.T
+        pts[k, :, :, 2] = z
+    points = pts.reshape(-1, 3)
+
+    def p(i, j, k):
+        return i + (nx + 1) * (j + (ny + 1) * k)
+
+    ii, jj = np.meshgrid(np.arange(nx), np.arange(ny), indexing="xy")
+    ii, jj = ii.ravel(), jj.ravel()
+    hexes = np.stack([p(ii, jj, 0), p(ii + 1, jj, 0), p(ii + 1, jj + 1, 0), p(ii, jj + 1, 0),
+                      p(ii, jj, 1), p(ii + 1, jj, 1), p(ii + 1, jj + 1, 1), p(ii, jj + 1, 1)], axis=1)
+
+    def classify(cells, lf, cen):
+        out = np.full(len(cells), -1)
+        out[(lf == 4) | (lf == 5)] = 4
+        out[lf == 0] = 0
+        out[lf == 1] = 1
+        out[lf == 2] = 2
+        out[lf == 3] = 3
+        return out
+
+    return hex_polymesh(points, hexes, classify,
+                        [("inlet", PATCH_GENERIC), ("outlet", PATCH_GENERIC), ("bottom", PATCH_WALL),
+                         ("top", PATCH_SYMMETRY), ("frontAndBack", PATCH_EMPTY)])

@@ -80,7 +80,7 @@ def test_sa_iterate_parity(oracle, model):
 def test_forces_parity(oracle, kind):
     print(f"[forces] {kind}", flush=True)
     if kind == "step":
-        spec = cases.forward_step_case(nx=60, ny=20, perturb=0.02)
+        spec = cases.forward_step_case(nx=60, ny=20, perturb=0.02, initial_co=0.5, min_co=0.1, max_co=5.0)
         patch = spec.mesh.patch_index("obstacle")
     else:
         spec = cases.box_viscous_case() if kind == "viscous" else cases.box_sa_case("SAneg")
