@@ -95,6 +95,10 @@ enum rdst_ilu_order { RDST_ILU_NATURAL = 0, RDST_ILU_PARTITIONED = 1 };
  * BlockGMRESSolver, or classical Gram-Schmidt with one fused multi-dot
  * (same basis in exact arithmetic, about half the Krylov-vector traffic) */
 enum rdst_gram_schmidt { RDST_GS_MODIFIED = 0, RDST_GS_CLASSICAL = 1 };
+/* interpolationSchemes reconstruct(rho|U|T) (e.g. forwardStep/system/fvSchemes:41-44):
+ * upwind (first order), or the limitedScheme OSPRE for the scalars with OSPREV
+ * for rhoU (r-factor from the Gauss-linear cell gradients) */
+enum rdst_reconstruction { RDST_RECON_UPWIND = 0, RDST_RECON_OSPRE = 1 };
 
 typedef struct rdst_patch {
     char name[64];
@@ -160,7 +164,7 @@ typedef struct rdst_settings {
     int32_t ilu_order;        /* rdst_ilu_order */
     int32_t viscous;          /* evaluate the viscous terms of calcFields */
     int32_t gram_schmidt;     /* rdst_gram_schmidt: GMRES orthogonalisation */
-    int32_t pad2;
+    int32_t reconstruction;   /* rdst_reconstruction */
 } rdst_settings;
 
 /* compressible::RASModel selected by constant/RASProperties RASModel */

@@ -56,6 +56,8 @@ def lib():
         L.orc_set_turbulence.argtypes = [C.c_void_p, P(rdst_turbulence), P(rdst_scalar_bc), P(C.c_double),
                                          P(C.c_double)]
         L.orc_turbulence_correct.argtypes = [C.c_void_p, P(rdst_solver_perf)]
+        L.orc_reconstruct.argtypes = [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double), P(C.c_double),
+                                      P(C.c_double)]
         L.orc_forces.argtypes = [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double), C.c_double, C.c_double,
                                  C.c_double, C.c_double, P(C.c_double)]
         _lib = L
@@ -128,6 +130,16 @@ class OracleCase:
         d = np.ascontiguousarray(wall_dist, dtype=np.float64)
         nt = np.ascontiguousarray(nu_tilda, dtype=np.float64)
         _check(lib().orc_set_turbulence(self.h, C.byref(turb), bcs, dptr(d), dptr(nt)))
+
+    def reconstruct(self, cell, bnd):
+        """OSPRE (scalar [n]) / OSPREV (vector [n,3]) owner- and neighbour-side face values."""
+        cell = np.ascontiguousarray(cell, dtype=np.float64)
+        bnd = np.ascontiguousarray(bnd, dtype=np.float64)
+        vec = cell.ndim == 2
+        shape = (self.nI + self.nB, 3) if vec else (self.nI + self.nB,)
+        own, nei = np.zeros(shape), np.zeros(shape)
+        _check(lib().orc_reconstruct(self.h, 1 if vec else 0, dptr(cell), dptr(bnd), dptr(own), dptr(nei)))
+        return own, nei
 
     def turbulence_correct(self):
         perf = rdst_solver_perf()

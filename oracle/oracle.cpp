@@ -517,6 +517,91 @@ static void reconstruct(const Mesh& m, const std::vector<T>& cell, const std::ve
     for (int f = 0; f < m.nInternal; ++f) { own[f] = cell[m.owner[f]]; nei[f] = cell[m.neighbour[f]]; }
     for (int b = 0; b < m.nB; ++b) { own[m.nInternal + b] = bnd[b]; nei[m.nInternal + b] = bnd[b]; }
 }
+// limitedScheme OSPRE / OSPREV (foam LimitedScheme + NVDTVD / NVDVTVDV r-factor):
+// face value lambda (P - N) + N with lambda = lim w + (1 - lim) pos(faceFlux),
+// faceFlux = +1 on the owner side (own_) and -1 on the neighbour side (nei_);
+// r = 2 (d & grad_upwind)/(phiN - phiP) - 1 (scalars; for vectors both sides
+// projected on phiN - phiP), |d.grad| >= 1000 |phiN - phiP| capped, d = C_N - C_P;
+// OSPRE lim(r) = 1.5 r(r + 1)/(r(r + 1) + 1).  Boundary faces take the patch value.
+static inline double ospre(double r) {
+    double rrp1 = r * (r + 1.0);
+    return 1.5 * rrp1 / (rrp1 + 1.0);
+}
+static inline double rFactor(double gradcf, double gradf) {
+    if (std::fabs(gradcf) >= 1000.0 * std::fabs(gradf)) return 2.0 * 1000.0 * fsign(gradcf) * fsign(gradf) - 1.0;
+    return 2.0 * (gradcf / gradf) - 1.0;
+}
+static void gradScalarB(const Mesh& m, const std::vector<double>& v, const std::vector<double>& bv,
+                        std::vector<V3>& g);
+static void reconstructOSPRE(const Mesh& m, const std::vector<double>& cell, const std::vector<double>& bnd,
+                             std::vector<double>& own, std::vector<double>& nei) {
+    std::vector<V3> g;
+    gradScalarB(m, cell, bnd, g);
+    int nf = m.nInternal + m.nB;
+    own.resize(nf); nei.resize(nf);
+    for (int f = 0; f < m.nInternal; ++f) {
+        int o = m.owner[f], nb = m.neighbour[f];
+        double P = cell[o], N = cell[nb], w = m.w[f];
+        V3 d = m.C[nb] - m.C[o];
+        double gradf = N - P;
+        double lo = ospre(rFactor(dot(d, g[o]), gradf));
+        double ln = ospre(rFactor(dot(d, g[nb]), gradf));
+        double wo = lo * w + (1.0 - lo) * 1.0;
+        double wn = ln * w + (1.0 - ln) * 0.0;
+        own[f] = wo * (P - N) + N;
+        nei[f] = wn * (P - N) + N;
+    }
+    for (int b = 0; b < m.nB; ++b) { own[m.nInternal + b] = bnd[b]; nei[m.nInternal + b] = bnd[b]; }
+}
+static void reconstructOSPREV(const Mesh& m, const std::vector<V3>& cell, const std::vector<V3>& bnd,
+                              std::vector<V3>& own, std::vector<V3>& nei) {
+    // Gauss linear gradient of a vector field, G[3i+j] = d phi_j / dx_i
+    int n = m.nCells;
+    std::vector<double> gc(9 * (size_t)n, 0.0);
+    for (int f = 0; f < m.nInternal; ++f) {
+        int o = m.owner[f], nb = m.neighbour[f];
+        double w = m.w[f];
+        V3 P = cell[o], N = cell[nb];
+        V3 vf = v3(w * (P.x - N.x) + N.x, w * (P.y - N.y) + N.y, w * (P.z - N.z) + N.z);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double t = cmp(m.Sf[f], i) * cmp(vf, j);
+                gc[9 * (size_t)o + 3 * i + j] += t;
+                gc[9 * (size_t)nb + 3 * i + j] -= t;
+            }
+    }
+    for (int b = 0; b < m.nB; ++b) {
+        int ci = m.bCell[b];
+        V3 sf = m.Sf[m.bFace[b]];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) gc[9 * (size_t)ci + 3 * i + j] += cmp(sf, i) * cmp(bnd[b], j);
+    }
+    for (int c = 0; c < n; ++c)
+        for (int k = 0; k < 9; ++k) gc[9 * (size_t)c + k] /= m.V[c];
+    auto dG = [&](V3 d, int c) {  // d & grad
+        const double* G = &gc[9 * (size_t)c];
+        return v3(d.x * G[0] + d.y * G[3] + d.z * G[6], d.x * G[1] + d.y * G[4] + d.z * G[7],
+                  d.x * G[2] + d.y * G[5] + d.z * G[8]);
+    };
+    int nf = m.nInternal + m.nB;
+    own.resize(nf); nei.resize(nf);
+    for (int f = 0; f < m.nInternal; ++f) {
+        int o = m.owner[f], nb = m.neighbour[f];
+        V3 P = cell[o], N = cell[nb];
+        double w = m.w[f];
+        V3 d = m.C[nb] - m.C[o];
+        V3 gradfV = N - P;
+        double gradf = dot(gradfV, gradfV);
+        double lo = ospre(rFactor(dot(gradfV, dG(d, o)), gradf));
+        double ln = ospre(rFactor(dot(gradfV, dG(d, nb)), gradf));
+        double wo = lo * w + (1.0 - lo) * 1.0;
+        double wn = ln * w + (1.0 - ln) * 0.0;
+        own[f] = v3(wo * (P.x - N.x) + N.x, wo * (P.y - N.y) + N.y, wo * (P.z - N.z) + N.z);
+        nei[f] = v3(wn * (P.x - N.x) + N.x, wn * (P.y - N.y) + N.y, wn * (P.z - N.z) + N.z);
+    }
+    for (int b = 0; b < m.nB; ++b) { own[m.nInternal + b] = bnd[b]; nei[m.nInternal + b] = bnd[b]; }
+}
+
 // linear interpolation w*(P-N)+N; boundary value on boundary faces
 static void linearInterp(const Mesh& m, const std::vector<double>& cell, const std::vector<double>& bnd,
                          std::vector<double>& out) {
@@ -928,12 +1013,21 @@ static void computeFlux(Case& c) {
     makeAux(c, a);
     std::vector<double> rho_o, rho_n, rPsi_o, rPsi_n, e_o, e_n, a_o, a_n, c_o, c_n, gam_f, fH;
     std::vector<V3> rhoU_o, rhoU_n;
-    reconstruct(m, c.rho, c.brho, rho_o, rho_n);
-    reconstruct(m, c.rhoU, c.brhoU, rhoU_o, rhoU_n);
-    reconstruct(m, a.rPsi, a.brPsi, rPsi_o, rPsi_n);
-    reconstruct(m, a.e, a.be, e_o, e_n);
-    reconstruct(m, a.aStar, a.baStar, a_o, a_n);
-    reconstruct(m, a.c, a.bc, c_o, c_n);
+    if (c.st.reconstruction == RDST_RECON_OSPRE) {
+        reconstructOSPRE(m, c.rho, c.brho, rho_o, rho_n);           // reconstruct(rho) OSPRE
+        reconstructOSPREV(m, c.rhoU, c.brhoU, rhoU_o, rhoU_n);      // reconstruct(U) OSPREV
+        reconstructOSPRE(m, a.rPsi, a.brPsi, rPsi_o, rPsi_n);       // reconstruct(T) OSPRE
+        reconstructOSPRE(m, a.e, a.be, e_o, e_n);
+        reconstructOSPRE(m, a.aStar, a.baStar, a_o, a_n);
+        reconstructOSPRE(m, a.c, a.bc, c_o, c_n);
+    } else {
+        reconstruct(m, c.rho, c.brho, rho_o, rho_n);
+        reconstruct(m, c.rhoU, c.brhoU, rhoU_o, rhoU_n);
+        reconstruct(m, a.rPsi, a.brPsi, rPsi_o, rPsi_n);
+        reconstruct(m, a.e, a.be, e_o, e_n);
+        reconstruct(m, a.aStar, a.baStar, a_o, a_n);
+        reconstruct(m, a.c, a.bc, c_o, c_n);
+    }
     linearInterp(m, a.gam, a.bgam, gam_f);
     if (c.st.flux_scheme == RDST_FLUX_HLLCP) computeHLLCPf(c, fH);
     int nf = m.nInternal + m.nB;
@@ -1846,6 +1940,21 @@ static void saMutBoundary(Case& c) {
     }
 }
 
+static void gradScalarB(const Mesh& m, const std::vector<double>& v, const std::vector<double>& bv,
+                        std::vector<V3>& g) {
+    int n = m.nCells;
+    g.assign(n, v3(0, 0, 0));
+    for (int f = 0; f < m.nInternal; ++f) {
+        int o = m.owner[f], nb = m.neighbour[f];
+        double vf = m.w[f] * (v[o] - v[nb]) + v[nb];
+        V3 t = m.Sf[f] * vf;
+        g[o] = g[o] + t;
+        g[nb] = g[nb] - t;
+    }
+    for (int b = 0; b < m.nB; ++b) g[m.bCell[b]] = g[m.bCell[b]] + m.Sf[m.bFace[b]] * bv[b];
+    for (int i = 0; i < n; ++i) g[i] = g[i] / m.V[i];
+}
+
 // Gauss linear gradient of a scalar with boundary values
 static void gradScalar(const Mesh& m, const std::vector<double>& v, const std::vector<double>& bv,
                        std::vector<V3>& g) {
@@ -2367,6 +2476,29 @@ int orc_set_partition(void* h, const int32_t* part) {
     })
 }
 
+// the limited reconstruction alone: cell [n] (or [n][3]) and boundary values
+// [nB] ([nB][3]) -> owner-side and neighbour-side face values [nI + nB] (x3)
+int orc_reconstruct(void* h, int vector, const double* cell, const double* bnd, double* own, double* nei) {
+    ORC_TRY({
+        const Mesh& m = ((Case*)h)->m;
+        int n = m.nCells, nB = m.nB, nf = m.nInternal + m.nB;
+        if (!vector) {
+            std::vector<double> c(cell, cell + n), b(bnd, bnd + nB), o, e;
+            reconstructOSPRE(m, c, b, o, e);
+            std::memcpy(own, o.data(), nf * sizeof(double));
+            std::memcpy(nei, e.data(), nf * sizeof(double));
+        } else {
+            std::vector<V3> c(n), b(nB), o, e;
+            for (int i = 0; i < n; ++i) c[i] = v3(cell[3 * i], cell[3 * i + 1], cell[3 * i + 2]);
+            for (int i = 0; i < nB; ++i) b[i] = v3(bnd[3 * i], bnd[3 * i + 1], bnd[3 * i + 2]);
+            reconstructOSPREV(m, c, b, o, e);
+            for (int k = 0; k < nf; ++k) {
+                own[3 * k] = o[k].x; own[3 * k + 1] = o[k].y; own[3 * k + 2] = o[k].z;
+                nei[3 * k] = e[k].x; nei[3 * k + 1] = e[k].y; nei[3 * k + 2] = e[k].z;
+            }
+        }
+    })
+}
 int orc_set_turbulence(void* h, const rdst_turbulence* t, const rdst_scalar_bc* bcs, const double* d,
                        const double* nut) {
     ORC_TRY({ saSet(*(Case*)h, t, bcs, d, nut); })

@@ -55,7 +55,7 @@ class rdst_settings(C.Structure):
                 ("rho_min", C.c_double), ("rho_max", C.c_double), ("p_min", C.c_double),
                 ("p_max", C.c_double), ("T_min", C.c_double), ("T_max", C.c_double),
                 ("ilu_order", C.c_int32), ("viscous", C.c_int32), ("gram_schmidt", C.c_int32),
-                ("pad2", C.c_int32)]
+                ("reconstruction", C.c_int32)]
 
 
 class rdst_solver_perf(C.Structure):
@@ -214,6 +214,8 @@ def make_settings(**kw) -> rdst_settings:
     s.viscous = int(kw.get("viscous", 0))
     gs = kw.get("gram_schmidt", "modified")
     s.gram_schmidt = {"modified": 0, "classical": 1}[gs] if isinstance(gs, str) else int(gs)
+    rc = kw.get("reconstruction", "upwind")
+    s.reconstruction = {"upwind": 0, "OSPRE": 1}[rc] if isinstance(rc, str) else int(rc)
     return s
 
 

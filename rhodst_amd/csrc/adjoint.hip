@@ -370,6 +370,8 @@ void adjoint_check_scheme(const Case& c) {
     if (c.st.flux_scheme == RDST_FLUX_HLLCP)
         throw ArgError("adjoint: HLLCP's pressure sensor couples second neighbours; use AUSM+, AUSM+up, HLL, HLLC or "
                        "Kurganov", RDST_ERR_UNKNOWN_NAME);
+    if (c.st.reconstruction != RDST_RECON_UPWIND)
+        throw ArgError("adjoint: the exact linearisation covers first-order (upwind) reconstruction");
     if (c.st.viscous || c.sa.on)
         throw ArgError("adjoint: the exact linearisation covers the inviscid residual (settings.viscous = 0, laminar)");
 }

@@ -23,6 +23,7 @@ MeshDev Case::meshDev() const {
     d.Sf = m.d_Sf.p; d.magSf = m.d_magSf.p; d.w = m.d_w.p; d.dc = m.d_dc.p; d.V = m.d_V.p;
     d.ellCode = m.d_ellCode.p; d.ellFace = m.d_ellFace.p; d.ellSlot = m.d_ellSlot.p; d.slotDiag = m.d_slotDiag.p;
     d.slotUpper = m.d_slotUpper.p; d.slotLower = m.d_slotLower.p;
+    d.C = m.d_C.p;
     d.ellSf = m.d_ellSf.p; d.ellMagSf = m.d_ellMagSf.p; d.ellWt = m.d_ellWt.p; d.ellDc = m.d_ellDc.p;
     d.ellCorr = m.d_ellCorr.p;
     return d;
@@ -66,6 +67,12 @@ static void alloc_fields(Case& c) {
     f.fH = c.fHBuf.p;
     f.mut = c.turbBuf.p; f.alphat = c.turbBuf.p + n;
     f.bmut = c.turbBuf.p + 2 * n; f.balphat = c.turbBuf.p + 2 * n + nB;
+    f.rec = f.brec = f.rgrad = nullptr;
+    if (c.st.reconstruction == RDST_RECON_OSPRE) {
+        c.recBuf.alloc(28 * n + 4 * nB);
+        c.recBuf.zero();
+        f.rec = c.recBuf.p; f.rgrad = c.recBuf.p + 4 * n; f.brec = c.recBuf.p + 28 * n;
+    }
 }
 
 static void download_soa(const double* const* comps, int k, size_t n, double* out) {
@@ -207,6 +214,8 @@ int rdst_case_create(rdst_mesh* mesh, const rdst_thermo* thermo, const rdst_bc* 
         if (s.n_directions < 1 || s.max_iter < 0 || s.min_iter < 0) throw ArgError("bad solver controls");
         if (s.gram_schmidt != RDST_GS_MODIFIED && s.gram_schmidt != RDST_GS_CLASSICAL)
             throw ArgError("Unknown Gram-Schmidt variant; valid: modified classical", RDST_ERR_UNKNOWN_NAME);
+        if (s.reconstruction != RDST_RECON_UPWIND && s.reconstruction != RDST_RECON_OSPRE)
+            throw ArgError("rdst_case_create: unknown reconstruction (valid: upwind OSPRE)", RDST_ERR_UNKNOWN_NAME);
         if (s.ilu_order != RDST_ILU_NATURAL && s.ilu_order != RDST_ILU_PARTITIONED)
             throw ArgError("Unknown ILU ordering; valid: natural partitioned", RDST_ERR_UNKNOWN_NAME);
         if (!(thermo->Cp > thermo->R) || !(thermo->R > 0)) throw ArgError("rdst_case_create: need Cp > R > 0");

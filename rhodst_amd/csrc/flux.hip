@@ -126,8 +126,133 @@ __device__ inline Side load_bnd(const Fields& F, int b, double gamma) {
     return make_side(F.brho[b], F.brhoUx[b], F.brhoUy[b], F.brhoUz[b], F.brhoE[b], F.bp[b], F.bh[b], F.bpsi[b], gamma);
 }
 
+// ---------------------------------------------------------------- limited reconstruction
+// limitedScheme OSPRE (scalars) / OSPREV (rhoU): face value lambda (P - N) + N,
+// lambda = lim w + (1 - lim) pos(faceFlux) with faceFlux = +1 on the owner side
+// (own_) and -1 on the neighbour side (nei_); r = 2 (d . grad_upwind)/(phiN -
+// phiP) - 1 (NVDTVD; vectors projected on phiN - phiP, NVDVTVDV), capped where
+// |d . grad| >= 1000 |phiN - phiP|; OSPRE lim(r) = 1.5 r(r+1)/(r(r+1)+1).
+constexpr int NREC = 8;  // rho, rhoUx, rhoUy, rhoUz, rPsi, e, a*, c
+__device__ inline double ospre_lim(double r) {
+    const double rrp1 = r * (r + 1.0);
+    return 1.5 * rrp1 / (rrp1 + 1.0);
+}
+__device__ inline double r_factor(double gradcf, double gradf) {
+    if (fabs(gradcf) >= 1000.0 * fabs(gradf)) return 2.0 * 1000.0 * fsign(gradcf) * fsign(gradf) - 1.0;
+    return 2.0 * (gradcf / gradf) - 1.0;
+}
+
+// derived cell fields of the reconstruction: rPsi = 1/psi, e = h - p/rho,
+// a* from H = max(rhoE/rho, SMALL) + max(p/rho, SMALL), c = sqrt(gamma rPsi)
+__global__ void k_recon_prep(MeshDev m, Fields F, double gamma) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m.nC) {
+        const Side s = make_side(F.rho[i], F.rhoUx[i], F.rhoUy[i], F.rhoUz[i], F.rhoE[i], F.p[i], F.h[i], F.psi[i], gamma);
+        F.rec[i] = s.rPsi; F.rec[m.nC + i] = s.e; F.rec[2 * (size_t)m.nC + i] = s.aStar; F.rec[3 * (size_t)m.nC + i] = s.c;
+    }
+    if (i < m.nB) {
+        const Side s = make_side(F.brho[i], F.brhoUx[i], F.brhoUy[i], F.brhoUz[i], F.brhoE[i], F.bp[i], F.bh[i], F.bpsi[i],
+                                 gamma);
+        F.brec[i] = s.rPsi; F.brec[m.nB + i] = s.e; F.brec[2 * (size_t)m.nB + i] = s.aStar; F.brec[3 * (size_t)m.nB + i] = s.c;
+    }
+}
+__device__ inline void rec_cell(const Fields& F, int n, int i, double* v) {
+    v[0] = F.rho[i]; v[1] = F.rhoUx[i]; v[2] = F.rhoUy[i]; v[3] = F.rhoUz[i];
+    for (int q = 0; q < 4; ++q) v[4 + q] = F.rec[(size_t)q * n + i];
+}
+__device__ inline void rec_bnd(const Fields& F, int nB, int b, double* v) {
+    v[0] = F.brho[b]; v[1] = F.brhoUx[b]; v[2] = F.brhoUy[b]; v[3] = F.brhoUz[b];
+    for (int q = 0; q < 4; ++q) v[4 + q] = F.brec[(size_t)q * nB + b];
+}
+// Gauss linear gradients of the reconstructed fields, [24][nC]
+__global__ __launch_bounds__(256) void k_recon_grad(MeshDev m, Fields F) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nC) return;
+    const size_t E = (size_t)m.ellW * m.nC;
+    double vc[NREC], g[3 * NREC];
+    rec_cell(F, m.nC, c, vc);
+    for (int q = 0; q < 3 * NREC; ++q) g[q] = 0.0;
+    for (int k = 0; k < m.ellW; ++k) {
+        const size_t e = (size_t)k * m.nC + c;
+        const uint32_t code = m.ellCode[e];
+        const uint32_t kind = code >> 30;
+        if (kind == ELL_NONE) break;
+        const int idx = (int)(code & ELL_IDX_MASK);
+        const double S[3] = {m.ellSf[e], m.ellSf[E + e], m.ellSf[2 * E + e]};
+        double vf[NREC];
+        if (kind == ELL_BOUNDARY) {
+            rec_bnd(F, m.nB, idx, vf);
+        } else {
+            double vo[NREC];
+            rec_cell(F, m.nC, idx, vo);
+            const double w = m.ellWt[e];
+            for (int q = 0; q < NREC; ++q)
+                vf[q] = (kind == ELL_OWNER) ? w * (vc[q] - vo[q]) + vo[q] : w * (vo[q] - vc[q]) + vc[q];
+        }
+        for (int q = 0; q < NREC; ++q)
+            for (int i = 0; i < 3; ++i) {
+                const double t = S[i] * vf[q];
+                if (kind == ELL_NEIGH) g[3 * q + i] -= t; else g[3 * q + i] += t;
+            }
+    }
+    const double V = m.V[c];
+    for (int q = 0; q < 3 * NREC; ++q) F.rgrad[(size_t)q * m.nC + c] = g[q] / V;
+}
+// owner- and neighbour-side sides of internal face (P, N) by OSPRE/OSPREV
+__device__ inline void recon_sides(const MeshDev& m, const Fields& F, int P, int N, double w, Side& L, Side& R) {
+    const int n = m.nC;
+    double vP[NREC], vN[NREC];
+    rec_cell(F, n, P, vP);
+    rec_cell(F, n, N, vN);
+    const double d0 = m.C[N] - m.C[P], d1 = m.C[n + N] - m.C[n + P], d2 = m.C[2 * (size_t)n + N] - m.C[2 * (size_t)n + P];
+    double lo[NREC], ln[NREC];
+    // scalars rho (q = 0) and rPsi, e, a*, c (q = 4..7)
+    for (int q = 0; q < NREC; ++q) {
+        if (q >= 1 && q <= 3) continue;
+        const double gf = vN[q] - vP[q];
+        const double* gPq = F.rgrad + (size_t)(3 * q) * n;
+        const double gcfP = d0 * gPq[P] + d1 * gPq[n + P] + d2 * gPq[2 * (size_t)n + P];
+        const double gcfN = d0 * gPq[N] + d1 * gPq[n + N] + d2 * gPq[2 * (size_t)n + N];
+        lo[q] = ospre_lim(r_factor(gcfP, gf));
+        ln[q] = ospre_lim(r_factor(gcfN, gf));
+    }
+    // rhoU (OSPREV): one limiter from the projection on phiN - phiP
+    {
+        const double gv0 = vN[1] - vP[1], gv1 = vN[2] - vP[2], gv2 = vN[3] - vP[3];
+        const double gf = gv0 * gv0 + gv1 * gv1 + gv2 * gv2;
+        double cP = 0.0, cN = 0.0;
+        const double gv[3] = {gv0, gv1, gv2};
+        for (int j = 0; j < 3; ++j) {
+            const double* gj = F.rgrad + (size_t)(3 * (1 + j)) * n;  // d(rhoU_j)/dx_i at 3(1+j) + i
+            const double dgP = d0 * gj[P] + d1 * gj[n + P] + d2 * gj[2 * (size_t)n + P];
+            const double dgN = d0 * gj[N] + d1 * gj[n + N] + d2 * gj[2 * (size_t)n + N];
+            cP += gv[j] * dgP;
+            cN += gv[j] * dgN;
+        }
+        const double lP = ospre_lim(r_factor(cP, gf)), lN = ospre_lim(r_factor(cN, gf));
+        for (int q = 1; q <= 3; ++q) { lo[q] = lP; ln[q] = lN; }
+    }
+    double fo[NREC], fn[NREC];
+    for (int q = 0; q < NREC; ++q) {
+        const double wo = lo[q] * w + (1.0 - lo[q]) * 1.0;
+        const double wn = ln[q] * w + (1.0 - ln[q]) * 0.0;
+        fo[q] = wo * (vP[q] - vN[q]) + vN[q];
+        fn[q] = wn * (vP[q] - vN[q]) + vN[q];
+    }
+    const double* fs[2] = {fo, fn};
+    Side* out[2] = {&L, &R};
+    for (int k = 0; k < 2; ++k) {
+        const double* v = fs[k];
+        Side& sd = *out[k];
+        sd.rho = v[0]; sd.rux = v[1]; sd.ruy = v[2]; sd.ruz = v[3];
+        sd.rr = 1.0 / v[0];
+        sd.ux = v[1] * sd.rr; sd.uy = v[2] * sd.rr; sd.uz = v[3] * sd.rr;
+        sd.rPsi = v[4]; sd.e = v[5]; sd.aStar = v[6]; sd.c = v[7];
+    }
+}
+
 // K1: face flux + residual, one thread per cell.
-template <int S, int STORE, bool VISC>
+template <int S, int STORE, bool VISC, bool RECON>
 __global__ __launch_bounds__(256) void k_flux_residual(MeshDev m, Fields F, const BcPatchDev* bc, double gamma, int lowMach,
                                                        double Mainf) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -157,6 +282,9 @@ __global__ __launch_bounds__(256) void k_flux_residual(MeshDev m, Fields F, cons
         if (kind == ELL_BOUNDARY) {
             L = load_bnd(F, idx, gamma);
             R = L;
+        } else if (RECON) {
+            const bool ownK = kind == ELL_OWNER;
+            recon_sides(m, F, ownK ? c : idx, ownK ? idx : c, m.ellWt[e], L, R);
         } else {
             const Side other = load_cell(F, idx, gamma);
             if (kind == ELL_OWNER) { L = me; R = other; } else { L = other; R = me; }
@@ -309,27 +437,40 @@ __global__ __launch_bounds__(RED_BS) void k_forces(MeshDev m, Fields F, int b0, 
     }
 }
 
-template <int S, int STORE>
+template <int S, int STORE, bool RECON>
 void launch_flux_mode(Case& c) {
     MeshDev md = c.meshDev();
     const int bs = 256;
     const int nb = blocks_for(md.nC, bs);
     const BcPatchDev* bc = c.d_bc.p;
     if (c.st.viscous)
-        hipLaunchKernelGGL((k_flux_residual<S, STORE, true>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, bc, c.th.gamma,
-                           c.st.low_mach, c.st.Mainf);
+        hipLaunchKernelGGL((k_flux_residual<S, STORE, true, RECON>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, bc,
+                           c.th.gamma, c.st.low_mach, c.st.Mainf);
     else
-        hipLaunchKernelGGL((k_flux_residual<S, STORE, false>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, bc, c.th.gamma,
-                           c.st.low_mach, c.st.Mainf);
+        hipLaunchKernelGGL((k_flux_residual<S, STORE, false, RECON>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, bc,
+                           c.th.gamma, c.st.low_mach, c.st.Mainf);
+}
+
+template <int S, bool RECON>
+void launch_flux_store(Case& c, int store) {
+    switch (store) {
+        case STORE_ALL: launch_flux_mode<S, STORE_ALL, RECON>(c); break;
+        case STORE_PHI: launch_flux_mode<S, STORE_PHI, RECON>(c); break;
+        default: launch_flux_mode<S, STORE_BPHI, RECON>(c);
+    }
 }
 
 template <int S>
 void launch_flux(Case& c, int store) {
     if (c.st.viscous) grad_u_launch(c);
-    switch (store) {
-        case STORE_ALL: launch_flux_mode<S, STORE_ALL>(c); break;
-        case STORE_PHI: launch_flux_mode<S, STORE_PHI>(c); break;
-        default: launch_flux_mode<S, STORE_BPHI>(c);
+    if (c.st.reconstruction == RDST_RECON_OSPRE) {
+        MeshDev md = c.meshDev();
+        const int nmax = std::max(md.nC, md.nB);
+        hipLaunchKernelGGL(k_recon_prep, dim3(blocks_for(nmax, 256)), dim3(256), 0, c.stream, md, c.f, c.th.gamma);
+        hipLaunchKernelGGL(k_recon_grad, dim3(blocks_for(md.nC, 256)), dim3(256), 0, c.stream, md, c.f);
+        launch_flux_store<S, true>(c, store);
+    } else {
+        launch_flux_store<S, false>(c, store);
     }
 }
 

@@ -421,6 +421,12 @@ void mesh_upload(Mesh& m) {
     m.d_w.upload(m.w);
     m.d_dc.upload(m.dc);
     m.d_V.upload(m.V);
+    {
+        std::vector<double> cs(3 * (size_t)m.nC);
+        for (int c = 0; c < m.nC; ++c)
+            for (int d = 0; d < 3; ++d) cs[(size_t)d * m.nC + c] = m.C[3 * (size_t)c + d];
+        m.d_C.upload(cs);
+    }
     m.d_ellCode.upload(m.ellCode);
     m.d_ellFace.upload(m.ellFace);
     m.d_ellSlot.upload(m.ellSlot);

@@ -129,6 +129,7 @@ struct Mesh {
     DBuf<int64_t> d_sellPtr, d_sellOfSlot;
     DBuf<int32_t> d_sellWidth, d_sellCol;
     DBuf<int32_t> d_bWall;  // [nB] 1 where the patch is a wall
+    DBuf<double> d_C;       // cell centres SoA [3][nC]
 };
 
 // largest partition of the block-Jacobi ILU(0); its FP64 vector (40 B/cell)
@@ -174,6 +175,10 @@ struct Fields {
     // turbulence viscosity and diffusivity (zero when laminar): muEff = mut + mu,
     // alphaEff = alphat + alpha
     double *mut, *alphat, *bmut, *balphat;
+    // limited reconstruction: derived cell fields rPsi, e, a*, c [4][nC] and
+    // boundary values [4][nB]; Gauss gradients of rho, rhoU, rPsi, e, a*, c
+    // [24][nC] (component 3q + i = d(field q)/dx_i)
+    double *rec, *brec, *rgrad;
 };
 
 // Spalart-Allmaras state of a case (SAneg/SARC2 nuTilda_, mut_, alphat_, d_)
@@ -200,6 +205,7 @@ struct MeshDev {
     const int32_t *ellFace, *ellSlot;
     const int32_t* slotDiag;
     const int32_t *slotUpper, *slotLower;  // per internal face: (owner row, neighbour col), (neighbour row, owner col)
+    const double* C;                       // cell centres, SoA [3][nC]
     const double *ellSf, *ellMagSf, *ellWt, *ellDc, *ellCorr;
 };
 
@@ -215,7 +221,7 @@ struct Case {
     float lastFluxMs = 0.f;
 
     // field storage
-    DBuf<double> cellBuf, bndBuf, faceBuf, RBuf, WBuf, timeBuf, fHBuf, gradBuf, turbBuf;
+    DBuf<double> cellBuf, bndBuf, faceBuf, RBuf, WBuf, timeBuf, fHBuf, gradBuf, turbBuf, recBuf;
     Fields f{};
     // matrix
     DBuf<double> A64, src;          // [nnzb][25], [nC][5]

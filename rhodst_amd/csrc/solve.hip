@@ -426,39 +426,68 @@ __global__ __launch_bounds__(64) void k_ilu_bwd(int cnt, const int32_t* rows, co
 // ---------------------------------------------------------------- partitioned block ILU(0)
 // One workgroup per partition walks the partition's local levels; entries
 // whose column lies in another partition are dropped (block-Jacobi).
+// The partitioned factor with five lanes per block row (lane = block row r):
+// row r of L_ik = A_ik Dinv_k and of A_ij -= L_ik U_kj depends on row r of
+// the row's own blocks only, so the five lanes of a row work independently;
+// one lane inverts the diagonal block once its row is complete.  Same
+// arithmetic, in the same order, as a one-thread-per-row factor.
+constexpr int PILU_THREADS = 320;  // 5 wavefronts x 12 rows
 template <class T>
-__global__ __launch_bounds__(256) void k_pilu_factor(const int32_t* partLevStart, const int32_t* partLevPtr,
-                                                     const int32_t* partRows, const int32_t* rowPtr, const int32_t* cols,
-                                                     const int32_t* diagPos, const int32_t* localCol, T* LU, T* Dinv,
-                                                     int* fail) {
+__global__ __launch_bounds__(PILU_THREADS) void k_pilu_factor(const int32_t* partLevStart, const int32_t* partLevPtr,
+                                                              const int32_t* partRows, const int32_t* rowPtr,
+                                                              const int32_t* cols, const int32_t* diagPos,
+                                                              const int32_t* localCol, T* LU, T* Dinv, int* fail) {
     const int pt = blockIdx.x;
+    const int wl = threadIdx.x & 63;
+    const int comp = wl % 5;
+    const bool laneOn = wl < 60;
+    const int slot = (threadIdx.x >> 6) * 12 + wl / 5;
+    const int rowsPerPass = (blockDim.x >> 6) * 12;
     const int l0 = partLevStart[pt], l1 = partLevStart[pt + 1];
     for (int L = l0; L + 1 < l1; ++L) {
         const int s = partLevPtr[L], e = partLevPtr[L + 1];
-        for (int t = s + (int)threadIdx.x; t < e; t += blockDim.x) {
-            const int i = partRows[t];
-            const int re = rowPtr[i + 1], di = diagPos[i];
-            for (int p = rowPtr[i]; p < di; ++p) {
-                if (localCol[p] < 0) continue;
-                const int k = cols[p];
-                T Lb[25];
-                mm5(LU + 25 * (size_t)p, Dinv + 25 * (size_t)k, Lb);
-                for (int q = 0; q < 25; ++q) LU[25 * (size_t)p + q] = Lb[q];
-                int qi = p + 1;
-                for (int q = diagPos[k] + 1; q < rowPtr[k + 1]; ++q) {
-                    if (localCol[q] < 0) continue;
-                    const int j = cols[q];
-                    while (qi < re && cols[qi] < j) ++qi;
-                    if (qi >= re) break;
-                    if (cols[qi] != j) continue;
-                    T P[25];
-                    mm5(Lb, LU + 25 * (size_t)q, P);
-                    for (int q2 = 0; q2 < 25; ++q2) LU[25 * (size_t)qi + q2] -= P[q2];
+        for (int b = s; b < e; b += rowsPerPass) {
+            const int t = b + slot;
+            const bool on = laneOn && t < e;
+            int i = 0, di = 0;
+            if (on) {
+                i = partRows[t];
+                const int re = rowPtr[i + 1];
+                di = diagPos[i];
+                for (int p = rowPtr[i]; p < di; ++p) {
+                    if (localCol[p] < 0) continue;
+                    const int k = cols[p];
+                    T* Lp = LU + 25 * (size_t)p + 5 * comp;
+                    const T* Dk = Dinv + 25 * (size_t)k;
+                    T Lr[5];
+                    for (int jj = 0; jj < 5; ++jj) {
+                        T acc = 0;
+                        for (int q = 0; q < 5; ++q) acc += Lp[q] * Dk[5 * q + jj];
+                        Lr[jj] = acc;
+                    }
+                    for (int jj = 0; jj < 5; ++jj) Lp[jj] = Lr[jj];
+                    int qi = p + 1;
+                    for (int q = diagPos[k] + 1; q < rowPtr[k + 1]; ++q) {
+                        if (localCol[q] < 0) continue;
+                        const int j = cols[q];
+                        while (qi < re && cols[qi] < j) ++qi;
+                        if (qi >= re) break;
+                        if (cols[qi] != j) continue;
+                        const T* Uq = LU + 25 * (size_t)q;
+                        T* Ai = LU + 25 * (size_t)qi + 5 * comp;
+                        for (int jj = 0; jj < 5; ++jj) {
+                            T acc = 0;
+                            for (int q2 = 0; q2 < 5; ++q2) acc += Lr[q2] * Uq[5 * q2 + jj];
+                            Ai[jj] -= acc;
+                        }
+                    }
                 }
             }
-            if (!inv5(LU + 25 * (size_t)di, Dinv + 25 * (size_t)i)) atomicExch(fail, 1);
+            __syncthreads();
+            if (on && comp == 0)
+                if (!inv5(LU + 25 * (size_t)di, Dinv + 25 * (size_t)i)) atomicExch(fail, 1);
+            __syncthreads();
         }
-        __syncthreads();
     }
 }
 
@@ -469,7 +498,6 @@ __global__ __launch_bounds__(256) void k_pilu_factor(const int32_t* partLevStart
 // and a 64-lane wavefront carries 12 rows; the Dinv product gathers the five
 // components of a row with wavefront shuffles.  Every lane of the block runs
 // the same number of passes, so the shuffles and barriers stay uniform.
-constexpr int PILU_THREADS = 320;  // 5 wavefronts x 12 rows
 template <class TM, class TV>
 __global__ __launch_bounds__(PILU_THREADS) void k_pilu_apply(const int32_t* partRowStart, const int32_t* partLevStart,
                                                              const int32_t* partLevPtr, const int32_t* partRows,
@@ -687,7 +715,7 @@ void ilu_factor(Ctx& k, const T* A, T* LU, T* Dinv) {
     int* fail = (int*)(k.w.partB.p);  // scratch int
     RDST_HIP(hipMemsetAsync(fail, 0, sizeof(int), k.s));
     if (k.c.st.precond == RDST_PRECOND_ILU0 && k.c.st.ilu_order == RDST_ILU_PARTITIONED) {
-        hipLaunchKernelGGL((k_pilu_factor<T>), dim3(m.nParts), dim3(256), 0, k.s, m.d_partLevStart.p, m.d_partLevPtr.p,
+        hipLaunchKernelGGL((k_pilu_factor<T>), dim3(m.nParts), dim3(PILU_THREADS), 0, k.s, m.d_partLevStart.p, m.d_partLevPtr.p,
                            m.d_partRows.p, m.d_rowPtr.p, m.d_cols.p, m.d_slotDiag.p, m.d_localCol.p, LU, Dinv, fail);
     } else if (k.c.st.precond == RDST_PRECOND_ILU0) {
         for (int L = 0; L < m.nLevL; ++L) {

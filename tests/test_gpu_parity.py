@@ -182,3 +182,31 @@ def test_tangential_farfield_mass_flux_is_zero(oracle, scheme):
     sl = slice(top[2], top[2] + top[3])
     assert np.all(oc.get(12)[sl] == 0.0)
     assert np.all(dc.get(12)[sl] == 0.0), dc.get(12)[sl]
+
+
+@pytest.mark.parametrize("kind", ["step", "box"])
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_flux_residual_parity_ospre(oracle, kind, scheme):
+    """Second-order limited reconstruction (reconstruct(rho|T) OSPRE,
+    reconstruct(U) OSPREV): face fluxes and R_ against the oracle."""
+    print(f"[flux parity OSPRE] {kind} {scheme}", flush=True)
+    spec = make_spec(kind, scheme, reconstruction="OSPRE")
+    oc = oracle.OracleCase(spec)
+    dc = DeviceCase(spec)
+    oc.flux_residual()
+    dc.flux_residual()
+    assert rel(dc.get(8), oc.get(8)) <= 1e-12
+    for fid in (12, 13, 14, 15):
+        assert rel(dc.get(fid), oc.get(fid)) <= 1e-12, fid
+
+
+@pytest.mark.parametrize("kind,scheme", [("step", "AUSM+"), ("box", "HLLC"), ("viscous", "AUSM+up")])
+def test_iterate_parity_ospre(oracle, kind, scheme):
+    print(f"[iterate parity OSPRE] {kind} {scheme}", flush=True)
+    spec = make_spec(kind, scheme, reconstruction="OSPRE")
+    oc = oracle.OracleCase(spec)
+    dc = DeviceCase(spec)
+    r0 = oc.iterate(15)
+    r1 = dc.iterate(15)
+    assert np.all(np.isfinite(r1))
+    assert rel(r1, r0) <= 1e-9, (r1, r0)
