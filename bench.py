@@ -105,6 +105,22 @@ def converge(args, mesh, spec64, spec32) -> int:
     return 0
 
 
+def flux_only(args, spec) -> int:
+    """A 256 MiB device-to-device copy (the counter calibration: its
+    FETCH_SIZE and WRITE_SIZE are 256 MiB each) and args.flux_calls face-flux
+    calls on the workload."""
+    import torch
+    from rhodst_amd import DeviceCase
+    dc = DeviceCase(spec)
+    a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    ms = dc.time_flux(args.flux_calls)
+    print(json.dumps({"flux_ms": ms, "copy_bytes": a.numel() * 8, "calls": args.flux_calls}), flush=True)
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +137,9 @@ def main(argv=None) -> int:
     ap.add_argument("--gmres-cycles", type=int, default=None, help="GMRES restart cycles per step (default 1)")
     ap.add_argument("--turbulence", default=None, help="SARC2 or SAneg: the viscous SA form of the workload")
     ap.add_argument("--paths", default="mixed,fp64", help="converge mode: paths to run, in order")
+    ap.add_argument("--flux-only", action="store_true",
+                    help="only the face-flux kernel (n calls) and a 256 MiB device copy of known size: the "
+                         "short program the FETCH_SIZE / WRITE_SIZE counter runs profile")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print("bench.py measures one GPU: every path here runs on a single MI355X; "
@@ -149,6 +168,8 @@ def main(argv=None) -> int:
     spec32p = cases.swept_wing_case(mesh=mesh, precision="mixed", **fast)
     if args.converge:
         return converge(args, mesh, spec64p, spec32p)
+    if args.flux_only:
+        return flux_only(args, spec64)
 
     log("fp64 path, natural-order ILU(0) (the plain path)")
     dc64, ms64, res64 = run_path(spec64, args.steps, args.warmup)

@@ -66,14 +66,14 @@ class BumpProblem:
                 raise RuntimeError("bump case diverged")
             if res[-1] <= self.drop * r0:
                 return steps, res[-1] / r0
+            print(f"[bump] step {steps} drop {res[-1] / r0:.3e}", flush=True)
         raise RuntimeError(f"bump case did not reach a {self.drop:g} residual drop in {self.max_steps} steps "
                            f"({res[-1] / r0:.3e})")
 
     def coefficient(self, dc, pressure_only=True):
         patch = dc.spec.mesh.patch_index("bottom")
         out = dc.forces(patch, self.direction, (0.0, 1.0, 0.0), self.p_ref, 1.0, 1.0, 1.0)
-        # forces() scales by 0.5 rho |U|^2 A = 0.5 here; undo it and divide by q
-        F = out[2:5]
+        F = out[2:5]  # the force vector (pressure part: the case is inviscid)
         return float(F @ self.direction / self.q)
 
     def solve(self, amplitudes):

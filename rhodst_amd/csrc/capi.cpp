@@ -67,6 +67,8 @@ static void alloc_fields(Case& c) {
     f.fH = c.fHBuf.p;
     f.mut = c.turbBuf.p; f.alphat = c.turbBuf.p + n;
     f.bmut = c.turbBuf.p + 2 * n; f.balphat = c.turbBuf.p + 2 * n + nB;
+    c.sideBuf.alloc(12 * n);
+    f.side = c.sideBuf.p;
     f.rec = f.brec = f.rgrad = nullptr;
     if (c.st.reconstruction == RDST_RECON_OSPRE) {
         c.recBuf.alloc(28 * n + 4 * nB);

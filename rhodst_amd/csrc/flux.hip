@@ -119,9 +119,27 @@ __global__ void k_grad_u_bnd(MeshDev m, Fields F, const BcPatchDev* bc) {
     }
 }
 
-__device__ inline Side load_cell(const Fields& F, int i, double gamma) {
-    return make_side(F.rho[i], F.rhoUx[i], F.rhoUy[i], F.rhoUz[i], F.rhoE[i], F.p[i], F.h[i], F.psi[i], gamma);
+// the prepass: each cell's flux side, once per call (make_side of the cell)
+__global__ void k_cell_sides(int n, Fields F, double gamma) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Side s = make_side(F.rho[i], F.rhoUx[i], F.rhoUy[i], F.rhoUz[i], F.rhoE[i], F.p[i], F.h[i], F.psi[i], gamma);
+    double* o = F.side;
+    const size_t N = (size_t)n;
+    o[i] = s.rho; o[N + i] = s.rux; o[2 * N + i] = s.ruy; o[3 * N + i] = s.ruz;
+    o[4 * N + i] = s.rPsi; o[5 * N + i] = s.e; o[6 * N + i] = s.aStar; o[7 * N + i] = s.c;
+    o[8 * N + i] = s.rr; o[9 * N + i] = s.ux; o[10 * N + i] = s.uy; o[11 * N + i] = s.uz;
 }
+__device__ inline Side side_of_cell(const Fields& F, int n, int i) {
+    const double* o = F.side;
+    const size_t N = (size_t)n;
+    Side s;
+    s.rho = o[i]; s.rux = o[N + i]; s.ruy = o[2 * N + i]; s.ruz = o[3 * N + i];
+    s.rPsi = o[4 * N + i]; s.e = o[5 * N + i]; s.aStar = o[6 * N + i]; s.c = o[7 * N + i];
+    s.rr = o[8 * N + i]; s.ux = o[9 * N + i]; s.uy = o[10 * N + i]; s.uz = o[11 * N + i];
+    return s;
+}
+
 __device__ inline Side load_bnd(const Fields& F, int b, double gamma) {
     return make_side(F.brho[b], F.brhoUx[b], F.brhoUy[b], F.brhoUz[b], F.brhoE[b], F.bp[b], F.bh[b], F.bpsi[b], gamma);
 }
@@ -257,7 +275,7 @@ __global__ __launch_bounds__(256) void k_flux_residual(MeshDev m, Fields F, cons
                                                        double Mainf) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
-    const Side me = load_cell(F, c, gamma);
+    const Side me = side_of_cell(F, m.nC, c);
     const double fHc = (S == RDST_FLUX_HLLCP) ? F.fH[c] : 0.0;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
     // viscous accumulators: laplacian(muEff, U), div(tauMC), div(sigmaDotU)
@@ -286,7 +304,7 @@ __global__ __launch_bounds__(256) void k_flux_residual(MeshDev m, Fields F, cons
             const bool ownK = kind == ELL_OWNER;
             recon_sides(m, F, ownK ? c : idx, ownK ? idx : c, m.ellWt[e], L, R);
         } else {
-            const Side other = load_cell(F, idx, gamma);
+            const Side other = side_of_cell(F, m.nC, idx);
             if (kind == ELL_OWNER) { L = me; R = other; } else { L = other; R = me; }
         }
         if constexpr (S == RDST_FLUX_HLLCP) {
@@ -463,6 +481,8 @@ void launch_flux_store(Case& c, int store) {
 template <int S>
 void launch_flux(Case& c, int store) {
     if (c.st.viscous) grad_u_launch(c);
+    hipLaunchKernelGGL(k_cell_sides, dim3(blocks_for(c.mesh->nC, 256)), dim3(256), 0, c.stream, c.mesh->nC, c.f,
+                       c.th.gamma);
     if (c.st.reconstruction == RDST_RECON_OSPRE) {
         MeshDev md = c.meshDev();
         const int nmax = std::max(md.nC, md.nB);

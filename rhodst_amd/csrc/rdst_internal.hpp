@@ -179,6 +179,9 @@ struct Fields {
     // boundary values [4][nB]; Gauss gradients of rho, rhoU, rPsi, e, a*, c
     // [24][nC] (component 3q + i = d(field q)/dx_i)
     double *rec, *brec, *rgrad;
+    // per-cell flux-side state (rho, rhoU, rPsi, e, a*, c, 1/rho, U) [12][nC],
+    // formed once per call by the face loop's prepass
+    double* side;
 };
 
 // Spalart-Allmaras state of a case (SAneg/SARC2 nuTilda_, mut_, alphat_, d_)
@@ -221,7 +224,7 @@ struct Case {
     float lastFluxMs = 0.f;
 
     // field storage
-    DBuf<double> cellBuf, bndBuf, faceBuf, RBuf, WBuf, timeBuf, fHBuf, gradBuf, turbBuf, recBuf;
+    DBuf<double> cellBuf, bndBuf, faceBuf, RBuf, WBuf, timeBuf, fHBuf, gradBuf, turbBuf, recBuf, sideBuf;
     Fields f{};
     // matrix
     DBuf<double> A64, src;          // [nnzb][25], [nC][5]

@@ -684,3 +684,236 @@ def bump_channel(nx: int = 48, ny: int = 24, lx: float = 2.0, ly: float = 1.0, b
     return hex_polymesh(points, hexes, classify,
                         [("inlet", PATCH_GENERIC), ("outlet", PATCH_GENERIC), ("bottom", PATCH_WALL),
                          ("top", PATCH_SYMMETRY), ("frontAndBack", PATCH_EMPTY)])
+
+
+# ------------------------------------------------------------------ general cells
+PRISM_FACES = [[0, 2, 1], [3, 4, 5], [0, 1, 4, 3], [1, 2, 5, 4], [2, 0, 3, 5]]
+
+
+def cells_polymesh(points: np.ndarray, cells: list, classify, patch_defs) -> PolyMesh:
+    """Build a polyMesh from cells given as lists of outward-oriented faces
+    (point-label lists of 3 or 4 points): faces shared by two cells become
+    internal faces (owner = lower cell label, owner's orientation), ordered by
+    owner then neighbour; the rest are boundary faces, classified by
+    classify(cell_ids, local_face_ids, face_centres) and grouped by patch."""
+    fc, fl, fp = [], [], []
+    for ci, faces in enumerate(cells):
+        for li, f in enumerate(faces):
+            fc.append(ci); fl.append(li)
+            fp.append(list(f) + [-1] * (4 - len(f)))
+    fc = np.array(fc, dtype=np.int64)
+    fl = np.array(fl, dtype=np.int64)
+    fp = np.array(fp, dtype=np.int64)
+    key = np.sort(np.where(fp < 0, np.iinfo(np.int64).max, fp), axis=1)
+    order = np.lexsort((key[:, 3], key[:, 2], key[:, 1], key[:, 0]))
+    ks = key[order]
+    same = np.all(ks[1:] == ks[:-1], axis=1)
+    pi = np.nonzero(same)[0]
+    a, b = order[pi], order[pi + 1]
+    ca, cb = fc[a], fc[b]
+    own_slot = np.where(ca < cb, a, b)
+    own_cell = np.minimum(ca, cb)
+    nei_cell = np.maximum(ca, cb)
+    io = np.lexsort((nei_cell, own_cell))
+    own_slot, own_cell, nei_cell = own_slot[io], own_cell[io], nei_cell[io]
+    paired = np.zeros(len(fc), dtype=bool)
+    paired[a] = True
+    paired[b] = True
+    bs = np.nonzero(~paired)[0]
+    npts = (fp >= 0).sum(axis=1)
+    cen = np.array([points[fp[s, :npts[s]]].mean(axis=0) for s in bs]) if len(bs) else np.zeros((0, 3))
+    bpatch = np.asarray(classify(fc[bs], fl[bs], cen), dtype=np.int64)
+    if np.any(bpatch < 0):
+        raise ValueError("unclassified boundary faces")
+    ob = np.lexsort((fl[bs], fc[bs], bpatch))
+    bs, bpatch = bs[ob], bpatch[ob]
+    slots = np.concatenate([own_slot, bs])
+    sizes = npts[slots]
+    offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    fpts = np.concatenate([fp[s, :npts[s]] for s in slots]).astype(np.int32)
+    owner = np.concatenate([own_cell, fc[bs]]).astype(np.int32)
+    patches = []
+    start = len(own_cell)
+    for k, (name, ptype) in enumerate(patch_defs):
+        size = int(np.count_nonzero(bpatch == k))
+        patches.append((name, ptype, start, size))
+        start += size
+    return PolyMesh(points=np.ascontiguousarray(points, dtype=np.float64), face_offsets=offsets, face_points=fpts,
+                    owner=owner, neighbour=nei_cell.astype(np.int32), n_cells=len(cells), patches=patches)
+
+
+def naca4_blunt(x: np.ndarray, t: float) -> np.ndarray:
+    """Half-thickness of the symmetric 4-digit section with its open (blunt)
+    trailing edge (last coefficient -0.1015)."""
+    return 5 * t * (0.2969 * np.sqrt(np.clip(x, 0, 1)) - 0.1260 * x - 0.3516 * x ** 2 + 0.2843 * x ** 3
+                    - 0.1015 * x ** 4)
+
+
+def _zip_rings(ia: np.ndarray, ua: np.ndarray, ib: np.ndarray, ub: np.ndarray) -> list:
+    """Triangles of the strip between two closed rings of point labels ia, ib
+    carrying increasing parameters ua, ub in [0, 1) along the rings (both start
+    at parameter 0): the two point sequences are merged by parameter, so every
+    point of both rings is used and the strip shares the rings' edges."""
+    na, nb = len(ia), len(ib)
+    uaw = np.concatenate([ua, [1.0 + ua[0]]])
+    ubw = np.concatenate([ub, [1.0 + ub[0]]])
+    tris = []
+    i = j = 0
+    while i < na or j < nb:
+        if j >= nb or (i < na and uaw[i + 1] <= ubw[j + 1]):
+            tris.append((ia[i % na], ia[(i + 1) % na], ib[j % nb]))
+            i += 1
+        else:
+            tris.append((ia[i % na], ib[(j + 1) % nb], ib[j % nb]))
+            j += 1
+    return tris
+
+
+def _polygon_cell(poly2d: list, nb2: int, z_off: int) -> list:
+    """Outward faces of the prism/hex extruded from a counter-clockwise 2-D
+    polygon of point labels (bottom labels l, top labels l + z_off)."""
+    v = list(poly2d)
+    k = len(v)
+    faces = [[v[i] for i in range(k)][::-1], [v[i] + z_off for i in range(k)]]
+    for i in range(k):
+        a, b = v[i], v[(i + 1) % k]
+        faces.append([a, b, b + z_off, a + z_off])
+    return faces
+
+
+def hybrid_airfoil(n_surf: int = 400, n_layers: int = 40, first_cell: float = 2.0e-6, ratio: float = 1.15,
+                   far: float = 50.0, thickness: float = 0.12, outer_ratio: float = 1.15, layer_aspect: float = 1.8,
+                   jitter: float = 0.2, split_aspect: float = 2.5, seed: int = 1, depth: float = 0.1) -> PolyMesh:
+    """Hybrid mesh about a 12 % symmetric 4-digit section (blunt trailing edge):
+    an O-grid marched off the wall along smoothed normals, n_layers quad
+    layers at 'ratio' from 'first_cell', then layers growing at 'outer_ratio'
+    up to 'layer_aspect' times the median spacing along the ring and
+    following it from there (the ring points moved along the ring towards uniform arc length by at
+    most 0.3 of the layer height per layer, and beyond 5 chords pulled
+    radially towards a circle) until the ring lies outside a circle of radius 'far' chords, the
+    last ring projected onto that circle.  Outside the wall layers every
+    O-grid quad whose aspect ratio is within 'split_aspect' is split into two
+    triangles (diagonal alternating with the ring index; a split rectangle of
+    aspect a has face non-orthogonality acos(2a / (1 + a^2))), the others
+    stay quads; interior nodes outside the wall layers are jittered by up to
+    'jitter' of the smaller of the local tangential and normal spacings,
+    scaled down by the local aspect ratio (slivers are not jittered: their
+    non-orthogonality would follow the height variation along them)
+    (seeded).  Extruded one cell thick: hexes and prisms.  Patches: airfoil
+    (wall), farfield (patch), frontAndBack (empty).  The defaults give about
+    2e5 cells."""
+    rng = np.random.default_rng(seed)
+    beta = np.linspace(0.0, np.pi, n_surf // 2 + 1)
+    xs = 0.5 * (1 - np.cos(beta))
+    yt = naca4_blunt(xs, thickness)
+    lower = np.stack([xs[::-1], -yt[::-1]], axis=1)          # TE -> LE (lower), counter-clockwise start
+    upper = np.stack([xs[1:], yt[1:]], axis=1)                # LE -> TE (upper)
+    h_te = yt[-1]
+    n_base = max(2, int(np.ceil(2 * h_te / (xs[-1] - xs[-2]))))
+    base = np.stack([np.ones(n_base - 1), np.linspace(h_te, -h_te, n_base + 1)[1:-1]], axis=1)
+    loop = np.concatenate([lower, upper, base], axis=0)
+    # counter-clockwise check (signed area > 0)
+    area = 0.5 * np.sum(loop[:, 0] * np.roll(loop[:, 1], -1) - np.roll(loop[:, 0], -1) * loop[:, 1])
+    if area < 0:
+        loop = loop[::-1].copy()
+    ns = len(loop)
+    ctr = np.array([0.5, 0.0])
+    rings = [loop]
+    heights = []
+    cur = loop.copy()
+    h = first_cell
+    k = 0
+    while True:
+        prv, nxt = np.roll(cur, 1, axis=0), np.roll(cur, -1, axis=0)
+        tang = nxt - prv
+        nrm = np.stack([tang[:, 1], -tang[:, 0]], axis=1)
+        nrm /= np.linalg.norm(nrm, axis=1)[:, None]
+        for _ in range(4 if k < n_layers else 8):
+            nrm = 0.5 * nrm + 0.25 * (np.roll(nrm, 1, axis=0) + np.roll(nrm, -1, axis=0))
+            nrm /= np.linalg.norm(nrm, axis=1)[:, None]
+        new = cur + h * nrm
+        if k >= n_layers:
+            # move the points along the ring a little towards uniform arc length
+            seg = np.linalg.norm(np.roll(new, -1, axis=0) - new, axis=1)
+            sa = np.concatenate([[0.0], np.cumsum(seg)])
+            shift = 0.05 * (np.arange(ns) * sa[-1] / ns - sa[:-1])
+            tgt = sa[:-1] + np.clip(shift, -0.3 * h, 0.3 * h)    # shear of a layer <= 0.3
+            closed = np.concatenate([new, new[:1]], axis=0)
+            new = np.stack([np.interp(tgt, sa, closed[:, 0]), np.interp(tgt, sa, closed[:, 1])], axis=1)
+            # beyond 5 chords pull the ring towards a circle about the section
+            d = new - ctr
+            rad = np.linalg.norm(d, axis=1)
+            if rad.mean() > 5.0:
+                new = ctr + d * ((rad + 0.1 * (rad.mean() - rad)) / rad)[:, None]
+        heights.append(h)
+        k += 1
+        if np.min(np.linalg.norm(new - ctr, axis=1)) >= far:
+            d = new - ctr
+            new = ctr + far * d / np.linalg.norm(d, axis=1)[:, None]
+            rings.append(new)
+            break
+        rings.append(new)
+        cur = new
+        if k < n_layers:
+            h *= ratio
+        else:
+            # grow until the layer height reaches 'layer_aspect' times the
+            # median spacing along the ring, then follow that spacing
+            med = np.median(np.linalg.norm(np.roll(new, -1, axis=0) - new, axis=1))
+            h = min(h * outer_ratio, max(h, layer_aspect * med))
+    n_rings = len(rings)
+    P = np.stack(rings, axis=0)                                # [ring, i, 2]
+    # jitter the interior nodes outside the wall layers
+    for kk in range(n_layers + 1, n_rings - 1):
+        r = P[kk]
+        sp = np.linalg.norm(np.roll(r, -1, axis=0) - r, axis=1)
+        tang = np.minimum(sp, np.roll(sp, 1))
+        nrml = np.minimum(np.linalg.norm(P[kk + 1] - r, axis=1), np.linalg.norm(r - P[kk - 1], axis=1))
+        loc = np.minimum(tang, nrml)
+        iso = loc / np.maximum(tang, nrml)      # full jitter on isotropic cells, none on slivers
+        ang = rng.uniform(0.0, 2.0 * np.pi, ns)
+        mag = jitter * loc * iso * np.sqrt(rng.uniform(0.0, 1.0, ns))
+        P[kk] = r + mag[:, None] * np.stack([np.cos(ang), np.sin(ang)], axis=1)
+    pts2 = [P[kk] for kk in range(n_rings)]
+    lab = lambda kk, i: kk * ns + (i % ns)
+    cells2d = []
+    for kk in range(n_rings - 1):
+        for i in range(ns):
+            q = [lab(kk, i), lab(kk, i + 1), lab(kk + 1, i + 1), lab(kk + 1, i)]
+            if kk >= n_layers:
+                a = np.linalg.norm(P[kk, (i + 1) % ns] - P[kk, i]) / max(np.linalg.norm(P[kk + 1, i] - P[kk, i]), 1e-300)
+                if 1.0 / split_aspect <= a <= split_aspect:
+                    if (kk + i) % 2 == 0:
+                        cells2d += [[q[0], q[1], q[2]], [q[0], q[2], q[3]]]
+                    else:
+                        cells2d += [[q[0], q[1], q[3]], [q[1], q[2], q[3]]]
+                    continue
+            cells2d.append(q)
+    P2 = np.concatenate(pts2, axis=0)
+    # counter-clockwise orientation of every 2-D cell
+    for ci, c in enumerate(cells2d):
+        q = P2[c]
+        ar = 0.5 * np.sum(q[:, 0] * np.roll(q[:, 1], -1) - np.roll(q[:, 0], -1) * q[:, 1])
+        if ar < 0:
+            cells2d[ci] = c[::-1]
+        elif ar == 0:
+            raise ValueError("degenerate 2-D cell")
+    nP = len(P2)
+    points = np.zeros((2 * nP, 3))
+    points[:nP, :2] = P2
+    points[nP:, :2] = P2
+    points[:nP, 2] = -0.5 * depth
+    points[nP:, 2] = 0.5 * depth
+    cells = [_polygon_cell(c, len(c), nP) for c in cells2d]
+
+    def classify(cells_, lf, cen):
+        out = np.full(len(cells_), -1)
+        zf = np.abs(np.abs(cen[:, 2]) - 0.5 * depth) < 1e-12
+        out[zf] = 2
+        rr = np.linalg.norm(cen[:, :2] - ctr, axis=1)
+        out[(~zf) & (rr > 0.99 * far)] = 1
+        out[(~zf) & (rr < 2.0)] = 0
+        return out
+
+    return cells_polymesh(points, cells, classify,
+                          [("airfoil", PATCH_WALL), ("farfield", PATCH_GENERIC), ("frontAndBack", PATCH_EMPTY)])

@@ -53,6 +53,18 @@ def test_jacobian_against_central_differences(scheme):
     fd = (Rp - Rm) / (2 * h)
     scale = np.abs(fd).max(axis=0)
     err = np.abs(Jv - fd).max(axis=0) / scale
+    print("relative error per component", err, flush=True)
+    if not np.all(err < 1e-6):
+        cells = np.argsort(-np.abs(Jv - fd).max(axis=1) / scale.max())[:5]
+        print("worst cells", cells, np.abs(Jv - fd)[cells] / scale, flush=True)
+        for hh in (1e-5, 1e-7):
+            dc.put(9, W + hh * v)
+            Rp = dc.adjoint_residual()
+            dc.put(9, W - hh * v)
+            Rm = dc.adjoint_residual()
+            dc.put(9, W)
+            e2 = np.abs(Jv - (Rp - Rm) / (2 * hh)).max(axis=0) / scale
+            print(f"h {hh:g}: relative error per component", e2, flush=True)
     assert np.all(err < 1e-6), err
 
 
@@ -108,16 +120,19 @@ def test_shape_gradient_through_autograd():
     prob = BumpProblem(drop=1e-13, max_steps=6000)
     a = torch.tensor(a0, dtype=torch.float64, requires_grad=True)
     C = ForceCoefficient.apply(a, prob)
+    print("Cd", float(C), flush=True)
     C.backward()
     g_adj = a.grad.numpy().copy()
+    print("adjoint gradient", g_adj, flush=True)
     H = 1e-6
     fd = np.zeros(len(a0))
     for i in range(len(a0)):
         e = np.zeros(len(a0))
         e[i] = H
-        cp, _, _ = BumpProblem(drop=1e-13, max_steps=6000).solve(a0 + e)
-        cm, _, _ = BumpProblem(drop=1e-13, max_steps=6000).solve(a0 - e)
+        cp, sp, _ = BumpProblem(drop=1e-13, max_steps=6000).solve(a0 + e)
+        cm, sm, _ = BumpProblem(drop=1e-13, max_steps=6000).solve(a0 - e)
         fd[i] = (cp - cm) / (2 * H)
+        print(f"mode {i}: fd {fd[i]:.10e} ({sp}, {sm} steps)", flush=True)
     print("adjoint", g_adj, "fd", fd, flush=True)
     rel = np.abs(g_adj - fd) / np.abs(fd).max()
     assert np.all(rel < 1e-5), rel

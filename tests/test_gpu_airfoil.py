@@ -1,0 +1,59 @@
+"""GPU parity on the hybrid quad/triangle airfoil mesh (hexes and prisms):
+transonic viscous flow with the Spalart-Allmaras model and the HLLC flux,
+characteristic far field.  A coarse version of the 2e5-cell mesh (the same
+generator, fewer points) keeps the oracle to seconds."""
+import numpy as np
+import pytest
+
+from rhodst_amd import DeviceCase, cases
+
+pytestmark = pytest.mark.gpu
+
+SMALL = dict(n_surf=80, n_layers=16, first_cell=2e-5, far=8.0)
+
+
+def rel(a, b):
+    a = np.asarray(a); b = np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def coefficients(case, spec):
+    a = np.radians(2.3)
+    gamma, R, T, p = 1.4, 287.0, 288.15, 101325.0
+    U = 0.73 * np.sqrt(gamma * R * T)
+    return case.forces(spec.mesh.patch_index("airfoil"), (np.cos(a), np.sin(a), 0.0), (-np.sin(a), np.cos(a), 0.0),
+                       p, p / (R * T), U, 0.1)
+
+
+@pytest.mark.parametrize("model,kw", [("SAneg", {}), ("SARC2", {}),
+                                      ("SAneg", {"ilu_order": "partitioned", "gram_schmidt": "classical"})])
+def test_airfoil_iterate_parity(oracle, model, kw):
+    from rhodst_amd import _capi
+    print(f"[airfoil] {model} {kw}", flush=True)
+    spec = cases.airfoil_case(turbulence=model, mesh_kw=SMALL, **kw)
+    dc = DeviceCase(spec)
+    oc = oracle.OracleCase(spec)
+    if kw.get("ilu_order") == "partitioned":
+        oc.set_partition(dc.mesh.get_int(_capi.M_PARTITION))
+    r0 = oc.iterate(12)
+    r1 = dc.iterate(12)
+    assert np.all(np.isfinite(r1))
+    assert rel(r1, r0) <= 1e-9, (r1, r0)
+    for f in (9, 29, 30):
+        assert rel(dc.get(f), oc.get(f)) <= 1e-8, f
+    co, cd = coefficients(oc, spec), coefficients(dc, spec)
+    assert rel(cd[:2], co[:2]) <= 1e-8, (cd, co)
+
+
+def test_airfoil_mixed_tracks_fp64():
+    """FP32 Jacobian/ILU/Krylov inside the FP64 defect correction follows the
+    FP64 nonlinear history on the hybrid mesh."""
+    print("[airfoil] mixed vs fp64", flush=True)
+    kw = dict(ilu_order="partitioned", gram_schmidt="classical")
+    d64 = DeviceCase(cases.airfoil_case(mesh_kw=SMALL, **kw))
+    d32 = DeviceCase(cases.airfoil_case(mesh_kw=SMALL, precision="mixed", **kw))
+    r64 = d64.iterate(60)
+    r32 = d32.iterate(60)
+    assert np.all(np.isfinite(r32))
+    assert r32[-1] < 0.1 * r32[0]
+    assert abs(np.log10(r32[-1] / r32[0]) - np.log10(r64[-1] / r64[0])) < 0.3, (r32[-1] / r32[0], r64[-1] / r64[0])
