@@ -38,6 +38,14 @@ def k1_bytes(n_cells: int, n_int: int, n_bnd: int) -> float:
     return n_int * (8 + 24) + n_bnd * (4 + 24 + 40) + n_cells * (40 + 8 + 40)
 
 
+def k1_bytes_viscous(n_cells: int, n_int: int, n_bnd: int) -> float:
+    """The same for the viscous face loop (first order, laminar or SA): per
+    internal face also the weight, deltaCoeff and the non-orthogonal
+    correction vector; per boundary face also muEff_b and the corrected
+    boundary gradient of U; per cell also grad U and muEff."""
+    return n_int * (8 + 24 + 8 + 8 + 24) + n_bnd * (4 + 24 + 40 + 16 + 72) + n_cells * (40 + 8 + 72 + 16 + 40)
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -136,6 +144,10 @@ def main(argv=None) -> int:
     ap.add_argument("--max-co", type=float, default=None, help="CFL ramp ceiling (default: the workload's 20)")
     ap.add_argument("--gmres-cycles", type=int, default=None, help="GMRES restart cycles per step (default 1)")
     ap.add_argument("--turbulence", default=None, help="SARC2 or SAneg: the viscous SA form of the workload")
+    ap.add_argument("--jacobian", default="Roe", choices=["Roe", "exact"],
+                    help="implicit operator: the reference's Roe Jacobian, or the exact linearisation of the "
+                         "first-order inviscid residual (Newton)")
+    ap.add_argument("--rel-tol", type=float, default=None, help="GMRES relTol (default: the workload's 1e-2)")
     ap.add_argument("--paths", default="mixed,fp64", help="converge mode: paths to run, in order")
     ap.add_argument("--flux-only", action="store_true",
                     help="only the face-flux kernel (n calls) and a 256 MiB device copy of known size: the "
@@ -156,12 +168,19 @@ def main(argv=None) -> int:
         extra["max_co"] = args.max_co
     if args.gmres_cycles is not None:
         extra["max_iter"] = args.gmres_cycles
+    if args.rel_tol is not None:
+        extra["rel_tol"] = args.rel_tol
+    if args.jacobian != "Roe":
+        extra["jacobian"] = args.jacobian
     wd = None
+    mesh_dev = None
     if args.turbulence:
+        from rhodst_amd._capi import DeviceMesh
         t = time.perf_counter()
-        wd = meshgen.wall_distance(mesh, ["wing"])
+        mesh_dev = DeviceMesh(mesh)
+        wd = mesh_dev.wall_distance()
         extra.update(turbulence=args.turbulence, wall_dist=wd)
-        log(f"wall distance in {time.perf_counter() - t:.1f}s")
+        log(f"mesh upload + device wall distance in {time.perf_counter() - t:.1f}s")
     spec64 = cases.swept_wing_case(mesh=mesh, **extra)
     fast = dict(ilu_order="partitioned", gram_schmidt="classical", **extra)
     spec64p = cases.swept_wing_case(mesh=mesh, **fast)

@@ -165,7 +165,20 @@ typedef struct rdst_settings {
     int32_t viscous;          /* evaluate the viscous terms of calcFields */
     int32_t gram_schmidt;     /* rdst_gram_schmidt: GMRES orthogonalisation */
     int32_t reconstruction;   /* rdst_reconstruction */
+    int32_t jacobian;         /* rdst_jacobian_kind: the implicit operator */
 } rdst_settings;
+
+/* The implicit operator of the step.  ROE is the reference's Jacobian
+ * (Roe<tensor5>, the only entry of the JacobianDST table, JacobianDST.C:57-92).
+ * EXACT (no counterpart in the reference) assembles V/dtau I + V dR/dW with
+ * the exact linearisation of the first-order inviscid residual (the
+ * adjoint's, rdst_adjoint_linearise): a Newton step as dtau grows; it takes
+ * the same restrictions as the adjoint (upwind reconstruction, inviscid,
+ * no HLLCP). */
+enum rdst_jacobian_kind {
+    RDST_JACOBIAN_ROE = 0,
+    RDST_JACOBIAN_EXACT = 1
+};
 
 /* compressible::RASModel selected by constant/RASProperties RASModel */
 enum rdst_turbulence_model {
@@ -226,7 +239,7 @@ enum rdst_field {
     RDST_F_NU_TILDA = 29,         /* cells */
     RDST_F_MUT = 30,              /* cells */
     RDST_F_ALPHAT = 31,           /* cells */
-    RDST_F_WALL_DIST = 32,        /* cells: the wallDist d the caller supplied */
+    RDST_F_WALL_DIST = 32,        /* cells: the wallDist d (supplied, or rdst_mesh_wall_distance) */
     RDST_F_BNU_TILDA = 33,        /* boundary faces */
     RDST_F_BMUT = 34,             /* boundary faces */
     RDST_F_SA_DIAG = 35,          /* nuTilda matrix after relax(): diagonal incl. boundary coefficients [n_cells] */
@@ -291,6 +304,11 @@ int rdst_mesh_destroy(rdst_mesh* mesh);
 int rdst_mesh_info(const rdst_mesh* mesh, int64_t* info /* [RDST_I_COUNT] */);
 int rdst_mesh_get_int(const rdst_mesh* mesh, int which, int32_t* out);
 int rdst_mesh_get_geom(const rdst_mesh* mesh, int which, double* out);
+/* d[n_cells]: exact distance of every cell centre to the faces of the wall
+ * patches (RDST_PATCH_WALL), computed on the device.  Replaces foam-extend's
+ * wallDist (meshWave; held as d_ by SAneg.C:160 and SARC2.C:158), whose
+ * propagated distance is exact only next to the wall. */
+int rdst_mesh_wall_distance(const rdst_mesh* mesh, double* d);
 
 /* ----------------------------------------------------------- case
  * A case = mesh + thermo + per-patch BCs + rhoDSTDict/fvSolution settings
@@ -327,8 +345,9 @@ int rdst_set_courant(rdst_case* c, double* res_norm);
 int rdst_iterate(rdst_case* c, int n_steps, double* res_norm, rdst_solver_perf* perf);
 
 /* Spalart-Allmaras turbulence on the case: model coefficients, the 0/nuTilda
- * boundary conditions [n_patches], the wall distance d [n_cells] (wallDist,
- * supplied by the caller) and the initial nuTilda [n_cells]; computes mut
+ * boundary conditions [n_patches], the wall distance d [n_cells] (wallDist;
+ * NULL: rdst_mesh_wall_distance on the case's mesh) and the initial nuTilda
+ * [n_cells]; computes mut
  * and alphat = mut/Prt, which then enter muEff/alphaEff of calcFields
  * (baseFluxDST.C:231) and of the Roe spectral radius (Roe.C:187-199).
  * RDST_TURB_LAMINAR switches the model off (mut = 0). */
@@ -384,8 +403,10 @@ int rdst_forces(rdst_case* c, int patch, const double* drag_dir, const double* l
 
 /* device time of the last rdst_flux_residual face kernel(s), in ms (HIP events) */
 int rdst_last_flux_ms(rdst_case* c, double* ms);
-/* run the face-flux + residual kernels n times back to back on the case's
- * stream and report the mean device time per call (HIP events) */
+/* run the face-flux + residual kernel n times back to back on the case's
+ * stream and report its mean device time per launch (HIP events); the
+ * gradients it reads (viscous terms, limited reconstruction) are formed
+ * once before the timed launches */
 int rdst_time_flux(rdst_case* c, int n, double* ms_per_call);
 
 #ifdef __cplusplus

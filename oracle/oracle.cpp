@@ -2361,6 +2361,7 @@ int orc_euler_jacobian(const double* U, double gamma, double rhoE, double rho, c
 int orc_case_create(const rdst_mesh_desc* d, const rdst_thermo* th, const rdst_bc* bcs,
                     const rdst_settings* st, void** out) {
     ORC_TRY({
+        if (st->jacobian != RDST_JACOBIAN_ROE) throw std::runtime_error("the oracle restates the Roe Jacobian only");
         Case* c = new Case();
         buildMesh(c->m, d);
         c->th = *th;

@@ -21,7 +21,8 @@ RDST_OK, RDST_ERR_ARG, RDST_ERR_HIP, RDST_ERR_UNKNOWN_NAME, RDST_ERR_STATE, RDST
 
 # every entry point declared in include/rhodst.h
 EXPORTS = ["rdst_last_error", "rdst_version", "rdst_init", "rdst_mesh_create", "rdst_mesh_destroy",
-           "rdst_mesh_info", "rdst_mesh_get_int", "rdst_mesh_get_geom", "rdst_case_create",
+           "rdst_mesh_info", "rdst_mesh_get_int", "rdst_mesh_get_geom", "rdst_mesh_wall_distance",
+           "rdst_case_create",
            "rdst_case_destroy", "rdst_case_set_state", "rdst_case_get", "rdst_case_put",
            "rdst_flux_residual", "rdst_jacobian_assemble", "rdst_block_solve", "rdst_update_fields",
            "rdst_set_courant", "rdst_iterate", "rdst_case_set_turbulence", "rdst_turbulence_correct",
@@ -140,6 +141,13 @@ class DeviceMesh:
         check(lib().rdst_mesh_get_geom(self.h, which, dptr(out)))
         return out
 
+    def wall_distance(self):
+        """Exact distance of every cell centre to the wall patches (device)."""
+        init()
+        out = np.zeros(self.info["n_cells"])
+        check(lib().rdst_mesh_wall_distance(self.h, dptr(out)))
+        return out
+
 
 _FIELD_SHAPES = {1: 3, 5: 3, 8: 5, 9: 5, 13: 3, 15: 3, 17: 3, 20: 3, 24: 25, 25: 5}
 
@@ -162,9 +170,10 @@ class DeviceCase:
             self.set_turbulence(*spec.turbulence)
 
     def set_turbulence(self, turb, bcs, wall_dist, nu_tilda):
-        d = np.ascontiguousarray(wall_dist, dtype=np.float64)
+        """wall_dist None: the device computes the exact wall distance."""
+        d = None if wall_dist is None else np.ascontiguousarray(wall_dist, dtype=np.float64)
         nt = np.ascontiguousarray(nu_tilda, dtype=np.float64)
-        check(lib().rdst_case_set_turbulence(self.h, C.byref(turb), bcs, dptr(d), dptr(nt)))
+        check(lib().rdst_case_set_turbulence(self.h, C.byref(turb), bcs, None if d is None else dptr(d), dptr(nt)))
 
     def turbulence_correct(self):
         perf = rdst_solver_perf()

@@ -19,7 +19,7 @@ OBJ_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "librhodst_amd.so")
 ARCH = os.environ.get("RHODST_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["mesh_host.cpp", "capi.cpp", "flux.hip", "jac.hip", "solve.hip", "update.hip", "sa.hip", "adjoint.hip"]
+SOURCES = ["mesh_host.cpp", "capi.cpp", "flux.hip", "jac.hip", "solve.hip", "update.hip", "sa.hip", "adjoint.hip", "wall.hip"]
 
 CFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-variable",
           "-Wno-unused-function", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -376,12 +376,12 @@ void adjoint_check_scheme(const Case& c) {
         throw ArgError("adjoint: the exact linearisation covers the inviscid residual (settings.viscous = 0, laminar)");
 }
 
-void adjoint_linearise(Case& c) {
+void adjoint_jacobian(Case& c) {
     adjoint_check_scheme(c);
     const Mesh& m = *c.mesh;
     const size_t nn = 25 * (size_t)m.nnzb;
-    if (c.adjJ.n != nn) { c.adjJ.alloc(nn); c.adjJT.alloc(nn); }
-    c.adjJ.zero();
+    if (c.adjJ.n != nn) c.adjJ.alloc(nn);
+    c.adjJ.zero(c.stream);
     switch (c.st.flux_scheme) {
         case RDST_FLUX_AUSMPLUS: launch_jac<RDST_FLUX_AUSMPLUS>(c, c.adjJ.p); break;
         case RDST_FLUX_AUSMPLUSUP: launch_jac<RDST_FLUX_AUSMPLUSUP>(c, c.adjJ.p); break;
@@ -389,6 +389,14 @@ void adjoint_linearise(Case& c) {
         case RDST_FLUX_HLLC: launch_jac<RDST_FLUX_HLLC>(c, c.adjJ.p); break;
         default: launch_jac<RDST_FLUX_KURGANOV>(c, c.adjJ.p); break;
     }
+    RDST_HIP(hipGetLastError());
+}
+
+void adjoint_linearise(Case& c) {
+    adjoint_jacobian(c);
+    const Mesh& m = *c.mesh;
+    const size_t nn = 25 * (size_t)m.nnzb;
+    if (c.adjJT.n != nn) c.adjJT.alloc(nn);
     if (m.nI)
         hipLaunchKernelGGL(k_transpose_faces, dim3(blocks_for(m.nI, 256)), dim3(256), 0, c.stream, m.nI, m.d_slotUpper.p,
                            m.d_slotLower.p, c.adjJ.p, c.adjJT.p);

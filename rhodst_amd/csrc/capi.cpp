@@ -45,6 +45,8 @@ static void alloc_fields(Case& c) {
     c.gradBuf.zero();
     c.turbBuf.alloc(2 * n + 2 * nB);
     c.turbBuf.zero();
+    c.cell8Buf.alloc(8 * n);
+    c.cell8Buf.zero();
     RDST_HIP(hipDeviceSynchronize());
     Fields& f = c.f;
     double* p = c.cellBuf.p;
@@ -65,10 +67,10 @@ static void alloc_fields(Case& c) {
     p = c.timeBuf.p;
     f.rDeltaT = p; f.courant = p + n; f.rhoOld = p + 2 * n; f.eOld = p + 3 * n; f.dS = p + 4 * n;
     f.fH = c.fHBuf.p;
+    f.cell8 = c.cell8Buf.p;
+    c.packDirty = true;
     f.mut = c.turbBuf.p; f.alphat = c.turbBuf.p + n;
     f.bmut = c.turbBuf.p + 2 * n; f.balphat = c.turbBuf.p + 2 * n + nB;
-    c.sideBuf.alloc(12 * n);
-    f.side = c.sideBuf.p;
     f.rec = f.brec = f.rgrad = nullptr;
     if (c.st.reconstruction == RDST_RECON_OSPRE) {
         c.recBuf.alloc(28 * n + 4 * nB);
@@ -178,6 +180,14 @@ int rdst_mesh_get_int(const rdst_mesh* mesh, int which, int32_t* out) {
     })
 }
 
+int rdst_mesh_wall_distance(const rdst_mesh* mesh, double* d) {
+    RDST_API_BODY({
+        const Mesh* m = reinterpret_cast<const Mesh*>(mesh);
+        if (!m || !d) throw ArgError("rdst_mesh_wall_distance: null argument");
+        wall_distance(*m, 0, d);
+    })
+}
+
 int rdst_mesh_get_geom(const rdst_mesh* mesh, int which, double* out) {
     RDST_API_BODY({
         const Mesh* m = reinterpret_cast<const Mesh*>(mesh);
@@ -218,6 +228,12 @@ int rdst_case_create(rdst_mesh* mesh, const rdst_thermo* thermo, const rdst_bc* 
             throw ArgError("Unknown Gram-Schmidt variant; valid: modified classical", RDST_ERR_UNKNOWN_NAME);
         if (s.reconstruction != RDST_RECON_UPWIND && s.reconstruction != RDST_RECON_OSPRE)
             throw ArgError("rdst_case_create: unknown reconstruction (valid: upwind OSPRE)", RDST_ERR_UNKNOWN_NAME);
+        if (s.jacobian != RDST_JACOBIAN_ROE && s.jacobian != RDST_JACOBIAN_EXACT)
+            throw ArgError("rdst_case_create: unknown Jacobian (valid: Roe exact)", RDST_ERR_UNKNOWN_NAME);
+        if (s.jacobian == RDST_JACOBIAN_EXACT &&
+            (s.reconstruction != RDST_RECON_UPWIND || s.viscous || s.flux_scheme == RDST_FLUX_HLLCP))
+            throw ArgError("rdst_case_create: the exact Jacobian covers the first-order inviscid residual "
+                           "(upwind reconstruction, viscous 0, not HLLCP)");
         if (s.ilu_order != RDST_ILU_NATURAL && s.ilu_order != RDST_ILU_PARTITIONED)
             throw ArgError("Unknown ILU ordering; valid: natural partitioned", RDST_ERR_UNKNOWN_NAME);
         if (!(thermo->Cp > thermo->R) || !(thermo->R > 0)) throw ArgError("rdst_case_create: need Cp > R > 0");
@@ -366,6 +382,7 @@ int rdst_case_put(rdst_case* cc, int field, const double* in) {
         RDST_HIP(hipStreamSynchronize(c->stream));
         const size_t n = c->mesh->nC;
         Fields& f = c->f;
+        c->packDirty = true;
         auto one = [&](double* d, size_t cnt) { RDST_HIP(hipMemcpy(d, in, cnt * sizeof(double), hipMemcpyHostToDevice)); };
         switch (field) {
             case RDST_F_RHO: one(f.rho, n); break;
@@ -474,13 +491,19 @@ int rdst_case_set_turbulence(rdst_case* cc, const rdst_turbulence* turb, const r
             throw ArgError("rdst_case_set_turbulence: unknown RASModel id (valid: laminar SAneg SARC2)",
                            RDST_ERR_UNKNOWN_NAME);
         if (turb->model != RDST_TURB_LAMINAR) {
-            if (!nu_tilda_bcs || !wall_dist || !nu_tilda) throw ArgError("rdst_case_set_turbulence: null array");
+            if (!nu_tilda_bcs || !nu_tilda) throw ArgError("rdst_case_set_turbulence: null array");
             if (!c->hasState) throw ArgError("rdst_case_set_turbulence: no state (call rdst_case_set_state)", RDST_ERR_STATE);
             if (!c->st.viscous) throw ArgError("rdst_case_set_turbulence: the viscous terms are off (settings.viscous)");
             if (!(turb->sigma_nut > 0) || !(turb->kappa > 0) || !(turb->Prt > 0) || !(turb->Cv2 > 0))
                 throw ArgError("rdst_case_set_turbulence: sigmaNut, kappa, Prt and Cv2 must be > 0");
             if (turb->relax > 1.0) throw ArgError("rdst_case_set_turbulence: relaxation factor > 1");
             if (turb->max_iter < 1) throw ArgError("rdst_case_set_turbulence: max_iter < 1");
+        }
+        std::vector<double> wd;
+        if (turb->model != RDST_TURB_LAMINAR && !wall_dist) {
+            wd.resize(c->mesh->nC);
+            wall_distance(*c->mesh, c->stream, wd.data());
+            wall_dist = wd.data();
         }
         sa_set(*c, turb, nu_tilda_bcs, wall_dist, nu_tilda);
     })

