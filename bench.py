@@ -46,6 +46,24 @@ def k1_bytes_viscous(n_cells: int, n_int: int, n_bnd: int) -> float:
     return n_int * (8 + 24 + 8 + 8 + 24) + n_bnd * (4 + 24 + 40 + 16 + 72) + n_cells * (40 + 8 + 72 + 16 + 40)
 
 
+TRAFFIC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "pmc_k_flux_residual_swept_wing_2M.md")
+
+
+def measured_traffic():
+    """HBM bytes per face-flux launch on the default workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (the commands that
+    produce it are in that file); None when it is absent."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -208,11 +226,13 @@ def main(argv=None) -> int:
     del dc32
 
     gfaces = (nI + nB) / (flux_ms * 1e-3) / 1e9
-    bytes_k1 = k1_bytes(nC, nI, nB)
+    bytes_k1 = (k1_bytes_viscous if args.turbulence else k1_bytes)(nC, nI, nB)
     achieved = bytes_k1 / (flux_ms * 1e-3) / 1e9
+    traffic = measured_traffic() if not (args.small or args.turbulence) else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_flux_residual<AUSM+up>", "kernel_ms": round(flux_ms, 5),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "k_flux_residual<AUSM+up%s>" % (", viscous" if args.turbulence else ""),
+                "kernel_ms": round(flux_ms, 5),
                 "algorithmic_bytes": int(bytes_k1)}
 
     cpu = None
@@ -254,9 +274,12 @@ def main(argv=None) -> int:
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"swept wing C-H mesh {nC} cells ({nI} internal + {nB} boundary faces), "
-                               "M0.84 AoA3.06, AUSM+up first order, inviscid slip wall, LTS CFL ramp to 20, "
-                               "GMRES(30)+block-ILU(0), 1 restart cycle per step; ms_per_step is the mixed-precision "
-                               "path with the partitioned block-ILU(0) and classical Gram-Schmidt",
+                               "M0.84 AoA3.06, AUSM+up first order, "
+                               + (f"{args.turbulence} at Re 1.2e7, no-slip wing" if args.turbulence
+                                  else "inviscid slip wall")
+                               + ", LTS CFL ramp to 20, GMRES(30)+block-ILU(0), 1 restart cycle per step; "
+                                 "ms_per_step is the mixed-precision path with the partitioned block-ILU(0) and "
+                                 "classical Gram-Schmidt",
                    "cells": nC, "faces": nI + nB, "flux_scheme": "AUSM+up"},
         "roofline": roofline,
         "cpu_baseline": cpu,

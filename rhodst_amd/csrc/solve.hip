@@ -427,70 +427,119 @@ __global__ __launch_bounds__(64) void k_ilu_bwd(int cnt, const int32_t* rows, co
 // ---------------------------------------------------------------- partitioned block ILU(0)
 // One workgroup per partition walks the partition's local levels; entries
 // whose column lies in another partition are dropped (block-Jacobi).
-// The partitioned factor with five lanes per block row (lane = block row r):
-// row r of L_ik = A_ik Dinv_k and of A_ij -= L_ik U_kj depends on row r of
-// the row's own blocks only, so the five lanes of a row work independently;
-// one lane inverts the diagonal block once its row is complete.  Same
-// arithmetic, in the same order, as a one-thread-per-row factor.
-constexpr int PILU_THREADS = 320;  // 5 wavefronts x 12 rows
+// The factor: one thread per block row of a level (the rows of a level only
+// read rows of earlier levels); each thread forms L_ik = A_ik Dinv_k, the
+// updates A_ij -= L_ik U_kj and the inverse of its diagonal block.  (A
+// five-lanes-per-row form of this kernel measured 1.7x slower on MI355X: the
+// diagonal inversion then runs on one lane in five.)
+constexpr int PILU_THREADS = 320;  // the apply: 5 wavefronts x 12 rows
+constexpr int PILU_FACTOR_THREADS = 256;
 template <class T>
-__global__ __launch_bounds__(PILU_THREADS) void k_pilu_factor(const int32_t* partLevStart, const int32_t* partLevPtr,
-                                                              const int32_t* partRows, const int32_t* rowPtr,
-                                                              const int32_t* cols, const int32_t* diagPos,
-                                                              const int32_t* localCol, T* LU, T* Dinv, int* fail) {
+__global__ __launch_bounds__(PILU_FACTOR_THREADS) void k_pilu_factor(const int32_t* partLevStart,
+                                                                     const int32_t* partLevPtr, const int32_t* partRows,
+                                                                     const int32_t* rowPtr, const int32_t* cols,
+                                                                     const int32_t* diagPos, const int32_t* localCol,
+                                                                     T* LU, T* Dinv, int* fail) {
     const int pt = blockIdx.x;
-    const int wl = threadIdx.x & 63;
-    const int comp = wl % 5;
-    const bool laneOn = wl < 60;
-    const int slot = (threadIdx.x >> 6) * 12 + wl / 5;
-    const int rowsPerPass = (blockDim.x >> 6) * 12;
     const int l0 = partLevStart[pt], l1 = partLevStart[pt + 1];
     for (int L = l0; L + 1 < l1; ++L) {
         const int s = partLevPtr[L], e = partLevPtr[L + 1];
-        for (int b = s; b < e; b += rowsPerPass) {
-            const int t = b + slot;
-            const bool on = laneOn && t < e;
-            int i = 0, di = 0;
-            if (on) {
-                i = partRows[t];
-                const int re = rowPtr[i + 1];
-                di = diagPos[i];
-                for (int p = rowPtr[i]; p < di; ++p) {
-                    if (localCol[p] < 0) continue;
-                    const int k = cols[p];
-                    T* Lp = LU + 25 * (size_t)p + 5 * comp;
-                    const T* Dk = Dinv + 25 * (size_t)k;
-                    T Lr[5];
-                    for (int jj = 0; jj < 5; ++jj) {
-                        T acc = 0;
-                        for (int q = 0; q < 5; ++q) acc += Lp[q] * Dk[5 * q + jj];
-                        Lr[jj] = acc;
-                    }
-                    for (int jj = 0; jj < 5; ++jj) Lp[jj] = Lr[jj];
-                    int qi = p + 1;
-                    for (int q = diagPos[k] + 1; q < rowPtr[k + 1]; ++q) {
-                        if (localCol[q] < 0) continue;
-                        const int j = cols[q];
-                        while (qi < re && cols[qi] < j) ++qi;
-                        if (qi >= re) break;
-                        if (cols[qi] != j) continue;
-                        const T* Uq = LU + 25 * (size_t)q;
-                        T* Ai = LU + 25 * (size_t)qi + 5 * comp;
-                        for (int jj = 0; jj < 5; ++jj) {
-                            T acc = 0;
-                            for (int q2 = 0; q2 < 5; ++q2) acc += Lr[q2] * Uq[5 * q2 + jj];
-                            Ai[jj] -= acc;
-                        }
-                    }
+        for (int t = s + (int)threadIdx.x; t < e; t += blockDim.x) {
+            const int i = partRows[t];
+            const int re = rowPtr[i + 1], di = diagPos[i];
+            for (int p = rowPtr[i]; p < di; ++p) {
+                if (localCol[p] < 0) continue;
+                const int k = cols[p];
+                T Lb[25];
+                mm5(LU + 25 * (size_t)p, Dinv + 25 * (size_t)k, Lb);
+                for (int q = 0; q < 25; ++q) LU[25 * (size_t)p + q] = Lb[q];
+                int qi = p + 1;
+                for (int q = diagPos[k] + 1; q < rowPtr[k + 1]; ++q) {
+                    if (localCol[q] < 0) continue;
+                    const int j = cols[q];
+                    while (qi < re && cols[qi] < j) ++qi;
+                    if (qi >= re) break;
+                    if (cols[qi] != j) continue;
+                    T P[25];
+                    mm5(Lb, LU + 25 * (size_t)q, P);
+                    for (int q2 = 0; q2 < 25; ++q2) LU[25 * (size_t)qi + q2] -= P[q2];
                 }
             }
-            __syncthreads();
-            if (on && comp == 0)
-                if (!inv5(LU + 25 * (size_t)di, Dinv + 25 * (size_t)i)) atomicExch(fail, 1);
-            __syncthreads();
+            if (!inv5(LU + 25 * (size_t)di, Dinv + 25 * (size_t)i)) atomicExch(fail, 1);
         }
+        __syncthreads();
     }
 }
+
+// The same apply with one thread per block row: measured faster than the
+// five-lane form for the FP32 factors (the five-lane form wins in FP64,
+// where a row's 25 block entries span four cache lines).
+template <class TM, class TV>
+__global__ __launch_bounds__(256) void k_pilu_apply_rows(const int32_t* partRowStart, const int32_t* partLevStart,
+                                                    const int32_t* partLevPtr, const int32_t* partRows,
+                                                    const int32_t* rowPtr, const int32_t* cols, const int32_t* diagPos,
+                                                    const int32_t* localCol, const TM* LU, const TM* Dinv, const TV* r,
+                                                    TV* x) {
+    extern __shared__ unsigned char smem_raw[];
+    TV* y = reinterpret_cast<TV*>(smem_raw);
+    const int pt = blockIdx.x;
+    const int rs = partRowStart[pt], re = partRowStart[pt + 1];
+    const int nl5 = (re - rs) * 5;
+    for (int t = threadIdx.x; t < nl5; t += blockDim.x) y[t] = r[5 * (size_t)partRows[rs + t / 5] + t % 5];
+    __syncthreads();
+    const int l0 = partLevStart[pt], l1 = partLevStart[pt + 1];
+    for (int L = l0; L + 1 < l1; ++L) {
+        const int s = partLevPtr[L], e = partLevPtr[L + 1];
+        for (int t = s + (int)threadIdx.x; t < e; t += blockDim.x) {
+            const int i = partRows[t];
+            TV* yi = y + 5 * (t - rs);
+            TV a0 = yi[0], a1 = yi[1], a2 = yi[2], a3 = yi[3], a4 = yi[4];
+            for (int p = rowPtr[i]; p < diagPos[i]; ++p) {
+                const int lc = localCol[p];
+                if (lc < 0) continue;
+                const TM* B = LU + 25 * (size_t)p;
+                const TV* yv = y + 5 * lc;
+                const TV y0 = yv[0], y1 = yv[1], y2 = yv[2], y3 = yv[3], y4 = yv[4];
+                a0 -= (TV)(B[0] * y0 + B[1] * y1 + B[2] * y2 + B[3] * y3 + B[4] * y4);
+                a1 -= (TV)(B[5] * y0 + B[6] * y1 + B[7] * y2 + B[8] * y3 + B[9] * y4);
+                a2 -= (TV)(B[10] * y0 + B[11] * y1 + B[12] * y2 + B[13] * y3 + B[14] * y4);
+                a3 -= (TV)(B[15] * y0 + B[16] * y1 + B[17] * y2 + B[18] * y3 + B[19] * y4);
+                a4 -= (TV)(B[20] * y0 + B[21] * y1 + B[22] * y2 + B[23] * y3 + B[24] * y4);
+            }
+            yi[0] = a0; yi[1] = a1; yi[2] = a2; yi[3] = a3; yi[4] = a4;
+        }
+        __syncthreads();
+    }
+    for (int L = l1 - 2; L >= l0; --L) {
+        const int s = partLevPtr[L], e = partLevPtr[L + 1];
+        for (int t = s + (int)threadIdx.x; t < e; t += blockDim.x) {
+            const int i = partRows[t];
+            TV* yi = y + 5 * (t - rs);
+            TV a0 = yi[0], a1 = yi[1], a2 = yi[2], a3 = yi[3], a4 = yi[4];
+            for (int p = diagPos[i] + 1; p < rowPtr[i + 1]; ++p) {
+                const int lc = localCol[p];
+                if (lc < 0) continue;
+                const TM* B = LU + 25 * (size_t)p;
+                const TV* yv = y + 5 * lc;
+                const TV y0 = yv[0], y1 = yv[1], y2 = yv[2], y3 = yv[3], y4 = yv[4];
+                a0 -= (TV)(B[0] * y0 + B[1] * y1 + B[2] * y2 + B[3] * y3 + B[4] * y4);
+                a1 -= (TV)(B[5] * y0 + B[6] * y1 + B[7] * y2 + B[8] * y3 + B[9] * y4);
+                a2 -= (TV)(B[10] * y0 + B[11] * y1 + B[12] * y2 + B[13] * y3 + B[14] * y4);
+                a3 -= (TV)(B[15] * y0 + B[16] * y1 + B[17] * y2 + B[18] * y3 + B[19] * y4);
+                a4 -= (TV)(B[20] * y0 + B[21] * y1 + B[22] * y2 + B[23] * y3 + B[24] * y4);
+            }
+            const TM* D = Dinv + 25 * (size_t)i;
+            yi[0] = (TV)(D[0] * a0 + D[1] * a1 + D[2] * a2 + D[3] * a3 + D[4] * a4);
+            yi[1] = (TV)(D[5] * a0 + D[6] * a1 + D[7] * a2 + D[8] * a3 + D[9] * a4);
+            yi[2] = (TV)(D[10] * a0 + D[11] * a1 + D[12] * a2 + D[13] * a3 + D[14] * a4);
+            yi[3] = (TV)(D[15] * a0 + D[16] * a1 + D[17] * a2 + D[18] * a3 + D[19] * a4);
+            yi[4] = (TV)(D[20] * a0 + D[21] * a1 + D[22] * a2 + D[23] * a3 + D[24] * a4);
+        }
+        __syncthreads();
+    }
+    for (int t = threadIdx.x; t < nl5; t += blockDim.x) x[5 * (size_t)partRows[rs + t / 5] + t % 5] = y[t];
+}
+
 
 // x = M^-1 r for the partitioned factor; the partition's vector stays in LDS
 // for the forward and the backward sweep (reverse local levels).  Five lanes
@@ -717,7 +766,7 @@ void ilu_factor(Ctx& k, const T* A, T* LU, T* Dinv) {
     int* fail = (int*)(k.w.partB.p);  // scratch int
     RDST_HIP(hipMemsetAsync(fail, 0, sizeof(int), k.s));
     if (k.c.st.precond == RDST_PRECOND_ILU0 && k.c.st.ilu_order == RDST_ILU_PARTITIONED) {
-        hipLaunchKernelGGL((k_pilu_factor<T>), dim3(m.nParts), dim3(PILU_THREADS), 0, k.s, m.d_partLevStart.p, m.d_partLevPtr.p,
+        hipLaunchKernelGGL((k_pilu_factor<T>), dim3(m.nParts), dim3(PILU_FACTOR_THREADS), 0, k.s, m.d_partLevStart.p, m.d_partLevPtr.p,
                            m.d_partRows.p, m.d_rowPtr.p, m.d_cols.p, m.d_slotDiag.p, m.d_localCol.p, LU, Dinv, fail);
     } else if (k.c.st.precond == RDST_PRECOND_ILU0) {
         for (int L = 0; L < m.nLevL; ++L) {
@@ -751,9 +800,14 @@ void precondition(Ctx& k, const TM* LU, const TM* Dinv, const TV* r, TV* x, TV* 
     }
     if (k.c.st.ilu_order == RDST_ILU_PARTITIONED) {
         const size_t shmem = sizeof(TV) * 5 * (size_t)m.maxPartRows;
-        hipLaunchKernelGGL((k_pilu_apply<TM, TV>), dim3(m.nParts), dim3(PILU_THREADS), shmem, k.s, m.d_partRowStart.p,
-                           m.d_partLevStart.p, m.d_partLevPtr.p, m.d_partRows.p, m.d_rowPtr.p, m.d_cols.p,
-                           m.d_slotDiag.p, m.d_localCol.p, LU, Dinv, r, x);
+        if (sizeof(TM) == 4)
+            hipLaunchKernelGGL((k_pilu_apply_rows<TM, TV>), dim3(m.nParts), dim3(256), shmem, k.s, m.d_partRowStart.p,
+                               m.d_partLevStart.p, m.d_partLevPtr.p, m.d_partRows.p, m.d_rowPtr.p, m.d_cols.p,
+                               m.d_slotDiag.p, m.d_localCol.p, LU, Dinv, r, x);
+        else
+            hipLaunchKernelGGL((k_pilu_apply<TM, TV>), dim3(m.nParts), dim3(PILU_THREADS), shmem, k.s,
+                               m.d_partRowStart.p, m.d_partLevStart.p, m.d_partLevPtr.p, m.d_partRows.p, m.d_rowPtr.p,
+                               m.d_cols.p, m.d_slotDiag.p, m.d_localCol.p, LU, Dinv, r, x);
         return;
     }
     for (int L = 0; L < m.nLevL; ++L) {
