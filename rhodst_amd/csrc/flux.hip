@@ -284,11 +284,18 @@ __global__ __launch_bounds__(256) void k_flux_residual(MeshDev m, Fields F, cons
     const size_t E = (size_t)m.ellW * m.nC;
     for (int k = 0; k < m.ellW; ++k) {
         const size_t e = (size_t)k * m.nC + c;
-        const uint32_t code = m.ellCode[e];
+        // the ELL stream is read once: non-temporal loads keep it from
+        // evicting the cell records that neighbouring cells gather again
+        const uint32_t code = __builtin_nontemporal_load(m.ellCode + e);
         const uint32_t kind = code >> 30;
         if (kind == ELL_NONE) break;
         const int idx = (int)(code & ELL_IDX_MASK);
-        const double sx = m.ellSf[e], sy = m.ellSf[E + e], sz = m.ellSf[2 * E + e], ms = m.ellMagSf[e];
+        const double sx = __builtin_nontemporal_load(m.ellSf + e);
+        const double sy = __builtin_nontemporal_load(m.ellSf + E + e);
+        const double sz = __builtin_nontemporal_load(m.ellSf + 2 * E + e);
+        // |Sf| formed here instead of read (8 of the 36 bytes per ELL entry):
+        // the same rounded operations as the host's mag(Sf), IEEE sqrt
+        const double ms = sqrt(sx * sx + sy * sy + sz * sz);
         Side L, R;
         double f = 1.0;
         if (kind == ELL_BOUNDARY) {
@@ -391,11 +398,11 @@ __global__ __launch_bounds__(256) void k_flux_residual(MeshDev m, Fields F, cons
         r3 -= lz / V + tz / V;
         r4 -= sd / V;
     }
-    F.R[c] = a0 / V;
-    F.R[n + c] = r1;
-    F.R[2 * n + c] = r2;
-    F.R[3 * n + c] = r3;
-    F.R[4 * n + c] = r4;
+    __builtin_nontemporal_store(a0 / V, F.R + c);
+    __builtin_nontemporal_store(r1, F.R + n + c);
+    __builtin_nontemporal_store(r2, F.R + 2 * n + c);
+    __builtin_nontemporal_store(r3, F.R + 3 * n + c);
+    __builtin_nontemporal_store(r4, F.R + 4 * n + c);
 }
 
 // HLLCP pressure sensor (HLLCPDST.C:131-164): fCells = min over cellCells of

@@ -211,6 +211,17 @@ void jacobian_assemble(Case& c) {
     MeshDev md = c.meshDev();
     const bool mixed = c.st.precision == RDST_PREC_MIXED;
     if (c.st.jacobian == RDST_JACOBIAN_EXACT) {
+        // the reference's Roe operator, kept as the preconditioner source
+        const size_t nn = 25 * (size_t)m.nnzb;
+        if (mixed) {
+            if (c.P32.n != nn) c.P32.alloc(nn);
+            hipLaunchKernelGGL((k_assemble<true>), dim3(blocks_for(m.nC, 128)), dim3(128), 0, c.stream, md, c.f, c.th,
+                               c.d_bc.p, c.st.beta, (double*)nullptr, c.P32.p, c.src.p);
+        } else {
+            if (c.P64.n != nn) c.P64.alloc(nn);
+            hipLaunchKernelGGL((k_assemble<false>), dim3(blocks_for(m.nC, 128)), dim3(128), 0, c.stream, md, c.f, c.th,
+                               c.d_bc.p, c.st.beta, c.P64.p, (float*)nullptr, c.src.p);
+        }
         adjoint_jacobian(c);
         if (mixed) {
             if (c.A32.n != 25 * (size_t)m.nnzb) c.A32.alloc(25 * (size_t)m.nnzb);

@@ -244,6 +244,10 @@ struct Case {
     // matrix
     DBuf<double> A64, src;          // [nnzb][25], [nC][5]
     DBuf<float> A32;                // FP32 copy for mixed precision
+    // exact-Jacobian (Newton) mode: A holds the exact operator, P the Roe
+    // operator whose ILU(0) preconditions it (FP64 / FP32)
+    DBuf<double> P64;
+    DBuf<float> P32;
     DBuf<double> S64;               // sliced-ELL copies for the SpMV
     DBuf<float> S32;
     bool hasState = false, hasMatrix = false;

@@ -1032,12 +1032,16 @@ void block_solve(Case& c, rdst_solver_perf* perfOut) {
     const bool mixed = c.st.precision == RDST_PREC_MIXED;
     if (mixed && c.st.solver != RDST_SOLVER_GMRES)
         throw ArgError("mixed precision is provided for the GMRES solver", RDST_ERR_UNKNOWN_NAME);
-    // preconditioner factors and the sliced-ELL copy for the SpMV
+    // preconditioner factors and the sliced-ELL copy for the SpMV; with the
+    // exact Jacobian the ILU(0) is taken of the Roe operator P (diagonally
+    // dominant through its spectral-radius terms) and preconditions the
+    // exact operator A (Newton-Krylov)
+    const bool newton = c.st.jacobian == RDST_JACOBIAN_EXACT;
     if (mixed) {
-        ilu_factor<float>(k, c.A32.p, w.LU32.p, w.Dinv32.p);
+        ilu_factor<float>(k, newton ? c.P32.p : c.A32.p, w.LU32.p, w.Dinv32.p);
         build_sell<float>(k, c.A32.p, c.S32);
     } else {
-        ilu_factor<double>(k, c.A64.p, w.LU.p, w.Dinv.p);
+        ilu_factor<double>(k, newton ? c.P64.p : c.A64.p, w.LU.p, w.Dinv.p);
         build_sell<double>(k, c.A64.p, c.S64);
     }
     rdst_solver_perf perf{};

@@ -17,7 +17,16 @@ SCHEMES = ["AUSM+", "AUSM+up", "HLL", "HLLC", "Kurganov"]
 def _state_case(scheme):
     spec = cases.box_farfield_case(nx=16, ny=12, scheme=scheme)
     dc = DeviceCase(spec)
-    dc.iterate(10)
+    if scheme == "Kurganov":
+        # the seeded 5 % perturbation itself: iterated, the box flow relaxes
+        # to near uniform, where the central-upwind speeds of neighbouring
+        # cells tie (a kink of max/min) on most faces
+        dc.flux_residual()
+        dc.update_fields()
+        dc.flux_residual()
+        dc.update_fields()
+    else:
+        dc.iterate(10)
     return dc
 
 
@@ -32,6 +41,27 @@ def test_transpose_identity(scheme):
     a = np.sum(dc.adjoint_apply(v) * w)
     b = np.sum(v * dc.adjoint_apply(w, transpose=True))
     assert abs(a - b) <= 1e-12 * max(abs(a), abs(b))
+
+
+def _kurganov_near_ties(dc, W, tol):
+    from rhodst_amd._capi import G_SF
+    gamma, R = 1.4, 287.0
+    Cv = R / (gamma - 1)
+    own, nei = dc.mesh.get_int(0), dc.mesh.get_int(1)
+    Sf = dc.mesh.get_geom(G_SF)[:len(nei)]
+    rho = W[:, 0]
+    U = W[:, 1:4] / rho[:, None]
+    e = W[:, 4] / rho - 0.5 * np.sum(U * U, axis=1)
+    c = np.sqrt(gamma * R * e / Cv)
+    ms = np.linalg.norm(Sf, axis=1)
+    out = np.zeros(len(W), dtype=bool)
+    for sgn in (1.0, -1.0):
+        a = np.sum(U[own[:len(nei)]] * Sf, axis=1) + sgn * c[own[:len(nei)]] * ms
+        b = np.sum(U[nei] * Sf, axis=1) + sgn * c[nei] * ms
+        near = np.abs(a - b) <= tol * np.maximum(np.abs(a), np.abs(b))
+        out[own[:len(nei)][near]] = True
+        out[nei[near]] = True
+    return out
 
 
 @pytest.mark.parametrize("scheme", SCHEMES)
@@ -52,7 +82,17 @@ def test_jacobian_against_central_differences(scheme):
     dc.put(9, W)
     fd = (Rp - Rm) / (2 * h)
     scale = np.abs(fd).max(axis=0)
-    err = np.abs(Jv - fd).max(axis=0) / scale
+    keep = np.ones(dc.n, dtype=bool)
+    if scheme == "Kurganov":
+        # the central-upwind speeds a+ = max(phiv_o + c_o|S|, phiv_n + c_n|S|, 0)
+        # and a- = min(...) switch branch where the two sides tie; there the
+        # residual is not differentiable (J takes the owner branch, a central
+        # difference averages the two).  Rows of cells on a face within 1e-4
+        # of a tie are left out (a small fraction of this perturbed state).
+        keep = ~_kurganov_near_ties(dc, W, 1e-4)
+        print("rows left out near Kurganov ties:", int((~keep).sum()), "of", dc.n, flush=True)
+        assert keep.mean() > 0.9
+    err = np.abs(Jv - fd)[keep].max(axis=0) / scale
     print("relative error per component", err, flush=True)
     if not np.all(err < 1e-6):
         cells = np.argsort(-np.abs(Jv - fd).max(axis=1) / scale.max())[:5]

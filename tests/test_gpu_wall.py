@@ -30,6 +30,10 @@ def test_wall_distance_brute_force(kind):
     print(f"[wall distance] {kind}", flush=True)
     if kind == "box":
         mesh, walls = cases.box_sa_case().mesh, ["bottom"]
+        # the box's bottom is a generic patch carrying wall BCs; type it as a wall
+        i = mesh.patch_index("bottom")
+        name, _t, start, size = mesh.patches[i]
+        mesh.patches[i] = (name, meshgen.PATCH_WALL, start, size)
     elif kind == "airfoil":
         mesh, walls = meshgen.hybrid_airfoil(n_surf=80, n_layers=16, first_cell=2e-5, far=8.0), ["airfoil"]
     else:
