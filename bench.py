@@ -9,10 +9,13 @@ A "step" is one Single::iterate (face flux + residual, Jacobian assembly,
 linear solve, primitive/BC update, CFL/LTS) with every array resident in
 HBM.  The line reports the face-flux kernel throughput (Gfaces/s) as the
 value, the per-step time of the FP64 and mixed-precision paths, the roofline
-of the face-flux kernel (algorithmic bytes / HIP-event kernel time), and the
-CPU oracle timed on a bounded sample of the same workload.
+of the face-flux kernel (algorithmic bytes / HIP-event kernel time), the time
+to a 1e-8 drop of |R|_2 of the mixed and the FP64 path with the exact-Jacobian
+(Newton) operator, and the CPU oracle timed on a bounded sample of the same
+workload.  --turbulence SARC2 runs the viscous Spalart-Allmaras form of the
+step (throughput only).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--small]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--small] [--skip-converge]
 """
 from __future__ import annotations
 
@@ -91,42 +94,50 @@ def wing_coefficients(dc, mesh, mach=0.84, aoa_deg=3.06):
     return float(out[0]), float(out[1])
 
 
-def converge(args, mesh, spec64, spec32) -> int:
-    """Time to |R|_2 / |R|_2(0) <= 1e-8 (timeDST.C:328-331 norm) for the mixed
-    and the FP64 path; prints one JSON line per path."""
+def converge_path(name, spec, mesh, max_steps, dmesh=None, chunk=10):
+    """Iterate one path until |R|_2 / |R|_2(0) <= 1e-8 (timeDST.C:328-331 norm);
+    the time is taken at the end of the chunk of steps that reaches it."""
     from rhodst_amd import DeviceCase
+    dc = DeviceCase(spec, dmesh)
+    r0 = None
+    hist = []
+    t0 = time.perf_counter()
+    steps = 0
+    reached = False
+    next_log = 50
+    while steps < max_steps:
+        res = dc.iterate(chunk)
+        if r0 is None:
+            r0 = res[0]
+        steps += chunk
+        hist.extend(res.tolist())
+        drop = res[-1] / r0
+        if steps >= next_log or not np.isfinite(drop):
+            log(f"{name}: step {steps} drop {drop:.3e} t {time.perf_counter() - t0:.1f}s")
+            next_log += 50
+        if not np.isfinite(drop):
+            break
+        if np.min(np.array(hist) / r0) <= 1e-8:
+            reached = True
+            break
+    t1 = time.perf_counter()
+    h = np.array(hist) / r0
+    k = int(np.argmax(h <= 1e-8)) + 1 if reached else None
+    cd, cl = wing_coefficients(dc, mesh)
+    out = {"reached": reached, "steps": k if reached else steps, "seconds": round(t1 - t0, 2),
+           "final_drop": float(h[-1]), "min_drop": float(h.min()), "Cd": cd, "Cl": cl}
+    log(f"{name}: {'reached 1e-8 at step ' + str(k) if reached else 'not reached'} in {t1 - t0:.1f}s, Cd {cd:.12f}")
+    return out, dc.mesh
+
+
+def converge(args, mesh, spec64, spec32) -> int:
+    """Time to a 1e-8 drop of |R|_2 for the mixed and the FP64 path; prints
+    one JSON line."""
     out = {}
     dmesh = None
     specs = {"mixed": spec32, "fp64": spec64}
     for name in args.paths.split(","):
-        spec = specs[name]
-        dc = DeviceCase(spec, dmesh)
-        dmesh = dc.mesh
-        r0 = None
-        hist = []
-        t0 = time.perf_counter()
-        steps = 0
-        reached = False
-        while steps < args.max_steps:
-            res = dc.iterate(50)
-            if r0 is None:
-                r0 = res[0]
-            steps += 50
-            hist.extend(res.tolist())
-            drop = res[-1] / r0
-            log(f"{name}: step {steps} drop {drop:.3e} t {time.perf_counter() - t0:.1f}s")
-            if not np.isfinite(drop):
-                break
-            if np.min(np.array(hist) / r0) <= 1e-8:
-                reached = True
-                break
-        t1 = time.perf_counter()
-        h = np.array(hist) / r0
-        k = int(np.argmax(h <= 1e-8)) + 1 if reached else None
-        cd, cl = wing_coefficients(dc, mesh)
-        out[name] = {"reached": reached, "steps": k if reached else steps, "seconds": round(t1 - t0, 2),
-                     "final_drop": float(h[-1]), "min_drop": float(h.min()), "Cd": cd, "Cl": cl}
-        del dc
+        out[name], dmesh = converge_path(name, specs[name], mesh, args.max_steps, dmesh)
     print(json.dumps({"converge": out, "cells": mesh.n_cells}), flush=True)
     return 0
 
@@ -166,6 +177,10 @@ def main(argv=None) -> int:
                     help="implicit operator: the reference's Roe Jacobian, or the exact linearisation of the "
                          "first-order inviscid residual (Newton)")
     ap.add_argument("--rel-tol", type=float, default=None, help="GMRES relTol (default: the workload's 1e-2)")
+    ap.add_argument("--initial-co", type=float, default=None, help="initial CFL of the ramp (default 1)")
+    ap.add_argument("--skip-converge", action="store_true",
+                    help="leave out the time-to-1e-8 runs of the default mode")
+    ap.add_argument("--converge-steps", type=int, default=900, help="step limit of the time-to-1e-8 runs")
     ap.add_argument("--paths", default="mixed,fp64", help="converge mode: paths to run, in order")
     ap.add_argument("--flux-only", action="store_true",
                     help="only the face-flux kernel (n calls) and a 256 MiB device copy of known size: the "
@@ -188,6 +203,8 @@ def main(argv=None) -> int:
         extra["max_iter"] = args.gmres_cycles
     if args.rel_tol is not None:
         extra["rel_tol"] = args.rel_tol
+    if args.initial_co is not None:
+        extra["initial_co"] = args.initial_co
     if args.jacobian != "Roe":
         extra["jacobian"] = args.jacobian
     wd = None
@@ -235,6 +252,23 @@ def main(argv=None) -> int:
                 "kernel_ms": round(flux_ms, 5),
                 "algorithmic_bytes": int(bytes_k1)}
 
+    ttc = None
+    if not (args.skip_converge or args.small or args.turbulence):
+        # time to a 1e-8 drop: the exact-Jacobian (Newton) operator with the
+        # Roe ILU(0) preconditioner, CFL ramp from 50 to 1e4, GMRES(30) to
+        # relTol 1e-2 in at most 4 cycles, mixed and FP64
+        newton = dict(jacobian="exact", initial_co=50.0, max_co=1e4, rel_tol=1e-2, max_iter=4,
+                      ilu_order="partitioned", gram_schmidt="classical")
+        ttc = {}
+        for name, prec in (("mixed", "mixed"), ("fp64", "fp64")):
+            log(f"time to 1e-8, {name} path, exact Jacobian")
+            ttc[name], _ = converge_path(name, cases.swept_wing_case(mesh=mesh, precision=prec, **newton), mesh,
+                                         args.converge_steps, mesh_dev)
+        ttc["settings"] = "exact Jacobian + Roe-ILU(0) preconditioner, CFL 50 -> 1e4, GMRES(30) relTol 1e-2, 4 cycles"
+        if ttc["mixed"]["reached"] and ttc["fp64"]["reached"]:
+            ttc["speedup_mixed_vs_fp64"] = round(ttc["fp64"]["seconds"] / ttc["mixed"]["seconds"], 3)
+        ttc["abs_dCd_mixed_vs_fp64"] = abs(ttc["mixed"]["Cd"] - ttc["fp64"]["Cd"])
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle, 1 thread)")
@@ -267,7 +301,8 @@ def main(argv=None) -> int:
         "residual_drop_fp64": float(res64[-1] / res64[0]),
         "residual_drop_fp64_partitioned": float(res64p[-1] / res64p[0]),
         "residual_drop_mixed": float(res32[-1] / res32[0]),
-        "time_to_1e-8_s": None,
+        "time_to_1e-8_s": (ttc["mixed"]["seconds"] if ttc and ttc["mixed"]["reached"] else None),
+        "time_to_1e-8": ttc,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

@@ -399,6 +399,8 @@ def naca4_thickness(x: np.ndarray, t: float) -> np.ndarray:
 def _stretch(n: int, first: float, total: float) -> np.ndarray:
     """n+1 nodes on [0, total] with the first interval 'first', geometric growth
     (ratio found by bisection)."""
+    if n < 1:
+        raise ValueError(f"_stretch: {n} intervals")
     if first * n >= total:
         return np.linspace(0.0, total, n + 1)
     lo, hi = 1.0 + 1e-12, 2.0

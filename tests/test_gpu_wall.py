@@ -37,7 +37,7 @@ def test_wall_distance_brute_force(kind):
     elif kind == "airfoil":
         mesh, walls = meshgen.hybrid_airfoil(n_surf=80, n_layers=16, first_cell=2e-5, far=8.0), ["airfoil"]
     else:
-        mesh, walls = meshgen.swept_wing(n_wake=8, n_surf=16, n_normal=12, n_span_wing=6, n_span_out=2), ["wing"]
+        mesh, walls = meshgen.swept_wing(n_wake=8, n_surf=16, n_normal=16, n_span_wing=6, n_span_out=2, n_layers=8), ["wing"]
     dm = DeviceMesh(mesh)
     d = dm.wall_distance()
     ref = brute_force(mesh, walls, dm.get_geom(G_C))
