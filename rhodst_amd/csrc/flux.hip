@@ -456,95 +456,6 @@ __global__ __launch_bounds__(RED_BS) void k_forces(MeshDev m, Fields F, int b0, 
     }
 }
 
-// K1, first-order inviscid form with one thread per ELL entry: a workgroup
-// holds 64 consecutive cells x ellW face slots (wavefront k = slot k of the
-// 64 cells, so the ELL loads stay coalesced); every thread evaluates one face
-// flux and leaves its signed contribution in LDS, then the first wavefront
-// sums each cell's slots in slot order -- the same additions, in the same
-// order, as the one-thread-per-cell loop, so R_ is bit-identical -- while the
-// flux evaluations of a cell run in six waves instead of one serial chain.
-constexpr int SLOT_CELLS = 64;
-constexpr int SLOT_MAX_W = 8;
-template <int S, int STORE>
-__global__ __launch_bounds__(SLOT_CELLS * SLOT_MAX_W) void k_flux_slots(MeshDev m, Fields F, double gamma, int lowMach,
-                                                                         double Mainf) {
-    extern __shared__ double acc[];  // [ellW][5][64]
-    __shared__ int live[SLOT_MAX_W * SLOT_CELLS];  // slot k of cell lane holds a face
-    const int lane = threadIdx.x & (SLOT_CELLS - 1);
-    const int k = threadIdx.x / SLOT_CELLS;
-    const int cb = xcd_block(blockIdx.x, gridDim.x) * SLOT_CELLS;
-    const int c = cb + lane;
-    double* mine = acc + (size_t)k * 5 * SLOT_CELLS + lane;
-    double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0, v4 = 0.0;
-    bool any = false;
-    if (c < m.nC) {
-        const size_t E = (size_t)m.ellW * m.nC;
-        const size_t e = (size_t)k * m.nC + c;
-        const uint32_t code = __builtin_nontemporal_load(m.ellCode + e);
-        const uint32_t kind = code >> 30;
-        if (kind != ELL_NONE) {
-            any = true;
-            const int idx = (int)(code & ELL_IDX_MASK);
-            const double sx = __builtin_nontemporal_load(m.ellSf + e);
-            const double sy = __builtin_nontemporal_load(m.ellSf + E + e);
-            const double sz = __builtin_nontemporal_load(m.ellSf + 2 * E + e);
-            const double ms = sqrt(sx * sx + sy * sy + sz * sz);
-            Side L, R;
-            double f = 1.0;
-            if (kind == ELL_BOUNDARY) {
-                L = load_bnd(F, idx, gamma);
-                R = L;
-            } else {
-                const Side me = side_of_cell(F, c, gamma);
-                const Side other = side_of_cell(F, idx, gamma);
-                if (kind == ELL_OWNER) { L = me; R = other; } else { L = other; R = me; }
-                if constexpr (S == RDST_FLUX_HLLCP) {
-                    const double w = m.ellWt[e];
-                    const double fo = (kind == ELL_OWNER) ? F.fH[c] : F.fH[idx];
-                    const double fn = (kind == ELL_OWNER) ? F.fH[idx] : F.fH[c];
-                    f = w * (fo - fn) + fn;
-                }
-            }
-            const Flux5 o = eval_flux<S>(L, R, sx, sy, sz, ms, gamma, f, lowMach, Mainf);
-            if (kind == ELL_NEIGH) {
-                v0 = -o.phi; v1 = -o.fx; v2 = -o.fy; v3 = -o.fz; v4 = -o.fE;
-            } else {
-                v0 = o.phi; v1 = o.fx; v2 = o.fy; v3 = o.fz; v4 = o.fE;
-                if (STORE == STORE_ALL) {
-                    const int fi = m.ellFace[e];
-                    F.phi[fi] = o.phi; F.phiUx[fi] = o.fx; F.phiUy[fi] = o.fy; F.phiUz[fi] = o.fz; F.phiE[fi] = o.fE;
-                    F.Upx[fi] = o.upx; F.Upy[fi] = o.upy; F.Upz[fi] = o.upz;
-                } else if (STORE == STORE_PHI) {
-                    F.phi[m.ellFace[e]] = o.phi;
-                } else if (kind == ELL_BOUNDARY) {
-                    F.phi[m.nI + idx] = o.phi;
-                }
-            }
-        }
-    }
-    mine[0] = v0;
-    mine[SLOT_CELLS] = v1;
-    mine[2 * SLOT_CELLS] = v2;
-    mine[3 * SLOT_CELLS] = v3;
-    mine[4 * SLOT_CELLS] = v4;
-    live[k * SLOT_CELLS + lane] = any ? 1 : 0;
-    __syncthreads();
-    if (k != 0 || c >= m.nC) return;
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
-    for (int q = 0; q < m.ellW; ++q) {
-        if (!live[q * SLOT_CELLS + lane]) break;  // ELL_NONE: the cell's faces ended
-        const double* s = acc + (size_t)q * 5 * SLOT_CELLS + lane;
-        a0 += s[0]; a1 += s[SLOT_CELLS]; a2 += s[2 * SLOT_CELLS]; a3 += s[3 * SLOT_CELLS]; a4 += s[4 * SLOT_CELLS];
-    }
-    const double V = m.V[c];
-    const int n = m.nC;
-    __builtin_nontemporal_store(a0 / V, F.R + c);
-    __builtin_nontemporal_store(a1 / V, F.R + n + c);
-    __builtin_nontemporal_store(a2 / V, F.R + 2 * n + c);
-    __builtin_nontemporal_store(a3 / V, F.R + 3 * n + c);
-    __builtin_nontemporal_store(a4 / V, F.R + 4 * n + c);
-}
-
 template <int S, int STORE, bool RECON>
 void launch_flux_mode(Case& c) {
     MeshDev md = c.meshDev();
@@ -554,10 +465,6 @@ void launch_flux_mode(Case& c) {
     if (c.st.viscous)
         hipLaunchKernelGGL((k_flux_residual<S, STORE, true, RECON>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, bc,
                            c.th.gamma, c.st.low_mach, c.st.Mainf);
-    else if (!RECON && md.ellW <= SLOT_MAX_W)
-        hipLaunchKernelGGL((k_flux_slots<S, STORE>), dim3(blocks_for(md.nC, SLOT_CELLS)), dim3(SLOT_CELLS * md.ellW),
-                           sizeof(double) * 5 * SLOT_CELLS * md.ellW, c.stream, md, c.f, c.th.gamma, c.st.low_mach,
-                           c.st.Mainf);
     else
         hipLaunchKernelGGL((k_flux_residual<S, STORE, false, RECON>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, bc,
                            c.th.gamma, c.st.low_mach, c.st.Mainf);
