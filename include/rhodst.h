@@ -184,7 +184,8 @@ enum rdst_jacobian_kind {
 enum rdst_turbulence_model {
     RDST_TURB_LAMINAR = 0,
     RDST_TURB_SANEG = 1,   /* SAneg (libraries/turbulence/SAneg/SAneg.C) */
-    RDST_TURB_SARC2 = 2    /* SARC2 (libraries/turbulence/SARC2/SARC2.C) */
+    RDST_TURB_SARC2 = 2,   /* SARC2 (libraries/turbulence/SARC2/SARC2.C) */
+    RDST_TURB_SARC1 = 3    /* SARC1 (libraries/turbulence/SARC1/SARC1.C): Spalart-Shur fr1 with DS/Dt */
 };
 
 /* RASProperties <model>Coeffs (SAneg.C:168-307, SARC2.C:172-285), plus
@@ -194,11 +195,11 @@ typedef struct rdst_turbulence {
     int32_t model;                /* rdst_turbulence_model */
     int32_t negative_nu_tilda;    /* SAneg negativeNuTilda (default on) */
     int32_t stilda_modification;  /* SAneg StildaModification (default on) */
-    int32_t rcm_correction;       /* SARC2 RCMCorrection (default on) */
+    int32_t rcm_correction;       /* SARC2 RCMCorrection / SARC1 spalartShurCorrection (default on) */
     double sigma_nut, kappa, Prt; /* 0.66666, 0.41, 1 */
     double Cb1, Cb2, Cw2, Cw3, Cv1, Cv2;  /* 0.1355 0.622 0.3 2 7.1 5 */
     double Cn1, Cn2, Cn3, Ct3, Ct4;       /* SAneg: 16 0.7 0.9 1.2 0.5 */
-    double Cr1, Cr2, Cr3;                 /* SARC2: 1 2 1 */
+    double Cr1, Cr2, Cr3;                 /* SARC2: 1 2 1; SARC1: 1 12 1 */
     double relax;                 /* nuTilda relaxation factor (<= 0: none) */
     double tolerance, rel_tol;    /* smoothSolver tolerance / relTol */
     int32_t min_iter, max_iter;   /* symmetric Gauss-Seidel sweeps */

@@ -239,6 +239,8 @@ struct BCState {
     std::vector<V3> refU;
 };
 
+struct T9 { double a[9]; };
+
 // Spalart-Allmaras state (SAneg / SARC2 members nuTilda_, mut_, alphat_, d_)
 struct Turb {
     rdst_turbulence t{};
@@ -249,6 +251,7 @@ struct Turb {
     std::vector<double> mut, bmut, alphat, balphat;
     std::vector<double> rhoPrev;             // rho before this step's updateFields
     bool rhoPrevValid = false;
+    std::vector<T9> Sold;                    // SARC1: S = symm(grad U) of the previous correct()
     int nCorrect = 0;
     // nuTilda matrix after relax(), solve form: diagonal incl. internalCoeffs,
     // source incl. boundaryCoeffs
@@ -872,7 +875,6 @@ static void computeHLLCPf(const Case& c, std::vector<double>& f) {
 
 // ------------------------------------------------------------------ viscous terms of calcFields
 // rank-2 tensor, T[3*i + j]
-struct T9 { double a[9]; };
 
 // snGrad() of the U boundary condition on boundary face b (fvPatchField
 // snGrad: fixedValue dc (U_b - U_I); zeroGradient 0; basicSymmetry
@@ -1902,6 +1904,41 @@ static inline double saFr1(const rdst_turbulence& t, const T9& g) {
     return (1.0 + t.Cr1) * 2.0 * rStar / (1.0 + rStar) * (1.0 - t.Cr3 * std::atan(t.Cr2 * rTilda)) - t.Cr1;
 }
 
+// SARC1::fr1 (SARC1.C:125-176): Spalart-Shur rotation/curvature function.
+// S = symm(grad U), W = skew(grad U) per cell; DSDt[9] = ddt(rho, S_ij) +
+// div(phi, S_ij) per unit volume; sqrS = 2|S|^2, sqrW = 2|W|^2,
+// sqrD = (sqrS + sqrW)/2, rStar = sqrt(sqrS / max(sqrW, SMALL)),
+// rTilda = 2 / (rho max(sqrD, SMALL)^2) sum_ij (sum_k W_ik S_jk) DSDt_ij.
+static inline double saFr1SS(const rdst_turbulence& t, const T9& g, double rho, const double* DSDt) {
+    if (!t.rcm_correction) return 1.0;
+    double S[9], W[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            S[3 * i + j] = 0.5 * (g.a[3 * i + j] + g.a[3 * j + i]);
+            W[3 * i + j] = 0.5 * (g.a[3 * i + j] - g.a[3 * j + i]);
+        }
+    double sqrS = 0, sqrW = 0;
+    for (int q = 0; q < 9; ++q) { sqrS += S[q] * S[q]; sqrW += W[q] * W[q]; }
+    sqrS *= 2.0; sqrW *= 2.0;
+    const double sqrD = 0.5 * (sqrS + sqrW);
+    const double rStar = std::sqrt(sqrS / std::max(sqrW, SMALL));
+    double sum = 0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double ws = 0;
+            for (int k = 0; k < 3; ++k) ws += W[3 * i + k] * S[3 * j + k];
+            sum += ws * DSDt[3 * i + j];
+        }
+    const double rTilda = 2.0 / (rho * sqr(std::max(sqrD, SMALL))) * sum;
+    return (1.0 + t.Cr1) * 2.0 * rStar / (1.0 + rStar) * (1.0 - t.Cr3 * std::atan(t.Cr2 * rTilda)) - t.Cr1;
+}
+static inline T9 symmT(const T9& g) {
+    T9 s;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) s.a[3 * i + j] = 0.5 * (g.a[3 * i + j] + g.a[3 * j + i]);
+    return s;
+}
+
 // nuTilda boundary values from 0/nuTilda (fixedValue, zeroGradient,
 // symmetryPlane = zeroGradient for a scalar, inletOutlet on phi)
 static void saNutBoundary(Case& c) {
@@ -2080,6 +2117,36 @@ static void saCorrect(Case& c, Perf* perfOut) {
             default: break;
         }
     }
+    // SARC1: DS/Dt = ddt(rho, S) + div(phi, S) of the nine strain components
+    // (steadyStateDST::fvcDdt: rDeltaT (rho S - rho.oldTime() S.oldTime()),
+    // S.oldTime() the previous correct()'s S, the current S on the first;
+    // div Gauss upwind with the face mass flux; boundary faces carry symm of
+    // the corrected boundary gradient)
+    std::vector<T9> Scell, DSDt;
+    if (t.model == RDST_TURB_SARC1) {
+        Scell.resize(n);
+        DSDt.assign(n, T9{});
+        for (int i = 0; i < n; ++i) Scell[i] = symmT(g[i]);
+        const std::vector<T9>& So = (tb.nCorrect > 0 && (int)tb.Sold.size() == n) ? tb.Sold : Scell;
+        for (int f = 0; f < nI; ++f) {
+            const double phi = c.phi[f];
+            const double w = fe_pos(phi);
+            const int o = m.owner[f], nb = m.neighbour[f];
+            for (int q = 0; q < 9; ++q) {
+                const double sf = w * Scell[o].a[q] + (1.0 - w) * Scell[nb].a[q];
+                DSDt[o].a[q] += phi * sf;
+                DSDt[nb].a[q] -= phi * sf;
+            }
+        }
+        for (int b = 0; b < nB; ++b) {
+            const T9 Sb = symmT(gb[b]);
+            const int ci = m.bCell[b];
+            for (int q = 0; q < 9; ++q) DSDt[ci].a[q] += c.phi[nI + b] * Sb.a[q];
+        }
+        for (int i = 0; i < n; ++i)
+            for (int q = 0; q < 9; ++q)
+                DSDt[i].a[q] = c.rDeltaT[i] * (c.rho[i] * Scell[i].a[q] - rhoOld[i] * So[i].a[q]) + DSDt[i].a[q] / m.V[i];
+    }
     // cell sources
     for (int i = 0; i < n; ++i) {
         const double V = m.V[i], rho = c.rho[i], nut = tb.nut[i], d = tb.d[i];
@@ -2113,7 +2180,8 @@ static void saCorrect(Case& c, Perf* perfOut) {
             const double Stilda = saFv3(t, chi, fv1[i]) * std::sqrt(2.0) * std::sqrt(magSqrSkew(g[i]))
                                 + fv2 * nut / sqr(t.kappa * d);
             const double fw = saFw(t, Stilda, nut, d);
-            src[i] += t.Cb1 * rho * Stilda * nut * saFr1(t, g[i]) * V;
+            const double fr1 = (t.model == RDST_TURB_SARC1) ? saFr1SS(t, g[i], rho, DSDt[i].a) : saFr1(t, g[i]);
+            src[i] += t.Cb1 * rho * Stilda * nut * fr1 * V;
             diag[i] += Cw1 * fw * nut * rho / sqr(d) * V;
         }
     }
@@ -2240,6 +2308,7 @@ static void saCorrect(Case& c, Perf* perfOut) {
         if (saBound(m, tb.mut, tb.bmut, 0.0)) saMutBoundary(c);
     }
     for (int i = 0; i < n; ++i) tb.alphat[i] = tb.mut[i] / t.Prt;
+    if (t.model == RDST_TURB_SARC1) tb.Sold = Scell;
     tb.nCorrect++;
     if (perfOut) *perfOut = pf;
 }

@@ -80,12 +80,12 @@ class rdst_scalar_bc(C.Structure):
     _fields_ = [("type", C.c_int32), ("pad", C.c_int32), ("value", C.c_double)]
 
 
-TURB_MODELS = {"laminar": 0, "SAneg": 1, "SARC2": 2}
+TURB_MODELS = {"laminar": 0, "SAneg": 1, "SARC2": 2, "SARC1": 3}
 
 
 def make_turbulence(model="SAneg", **kw) -> rdst_turbulence:
     """RASProperties <model>Coeffs with the reference's defaults
-    (SAneg.C:168-307, SARC2.C:172-285) plus the nuTilda solver settings of
+    (SAneg.C:168-307, SARC2.C:172-285, SARC1.C:196-330: Cr2 = 12) plus the nuTilda solver settings of
     tutorials/RAE2822/system/fvSolution (symGaussSeidel, tolerance 1e-6,
     relTol 0.1, minIter 1, relaxation 0.5)."""
     t = rdst_turbulence()
@@ -96,6 +96,8 @@ def make_turbulence(model="SAneg", **kw) -> rdst_turbulence:
     defaults = dict(sigma_nut=0.66666, kappa=0.41, Prt=1.0, Cb1=0.1355, Cb2=0.622, Cw2=0.3, Cw3=2.0, Cv1=7.1,
                     Cv2=5.0, Cn1=16.0, Cn2=0.7, Cn3=0.9, Ct3=1.2, Ct4=0.5, Cr1=1.0, Cr2=2.0, Cr3=1.0,
                     relax=0.5, tolerance=1e-6, rel_tol=0.1)
+    if t.model == 3:
+        defaults["Cr2"] = 12.0
     for k, v in defaults.items():
         setattr(t, k, kw.get(k, v))
     t.min_iter = int(kw.get("min_iter", 1))

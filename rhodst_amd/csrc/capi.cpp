@@ -487,8 +487,9 @@ int rdst_case_set_turbulence(rdst_case* cc, const rdst_turbulence* turb, const r
     RDST_API_BODY({
         Case* c = reinterpret_cast<Case*>(cc);
         if (!c || !turb) throw ArgError("rdst_case_set_turbulence: null argument");
-        if (turb->model != RDST_TURB_LAMINAR && turb->model != RDST_TURB_SANEG && turb->model != RDST_TURB_SARC2)
-            throw ArgError("rdst_case_set_turbulence: unknown RASModel id (valid: laminar SAneg SARC2)",
+        if (turb->model != RDST_TURB_LAMINAR && turb->model != RDST_TURB_SANEG && turb->model != RDST_TURB_SARC2 &&
+            turb->model != RDST_TURB_SARC1)
+            throw ArgError("rdst_case_set_turbulence: unknown RASModel id (valid: laminar SAneg SARC2 SARC1)",
                            RDST_ERR_UNKNOWN_NAME);
         if (turb->model != RDST_TURB_LAMINAR) {
             if (!nu_tilda_bcs || !nu_tilda) throw ArgError("rdst_case_set_turbulence: null array");

@@ -210,6 +210,7 @@ struct SaState {
     DBuf<double> scal;     // device scalars
     double *nut = nullptr, *d = nullptr, *rhoPrev = nullptr, *fv1o = nullptr, *B = nullptr, *x2 = nullptr,
            *gn = nullptr, *bnut = nullptr;
+    double* Sold = nullptr;  // SARC1: symm(grad U) of the previous correct() [9][nC]
     bool rhoPrevValid = false;
     int nCorrect = 0;
 };

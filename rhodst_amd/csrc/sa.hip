@@ -39,7 +39,7 @@ constexpr double kSMALL = 1.0e-15;
 
 struct SaDev {
     const rdst_scalar_bc* bc;  // per patch
-    double *nut, *bnut, *d, *rhoPrev, *fv1o, *gn, *B, *x2, *A;
+    double *nut, *bnut, *d, *rhoPrev, *fv1o, *gn, *B, *x2, *A, *Sold;
 };
 
 __device__ inline double sq(double s) { return s * s; }
@@ -90,6 +90,45 @@ __device__ inline double sa_fr1(const rdst_turbulence& t, const double* g) {
     const double rrStar = sqrt(sqrW / fmax(sqrS, kSMALL));
     const double rTilda = rrStar * (rrStar - 1.0);
     return (1.0 + t.Cr1) * 2.0 * rStar / (1.0 + rStar) * (1.0 - t.Cr3 * atan(t.Cr2 * rTilda)) - t.Cr1;
+}
+
+// SARC1::fr1 (SARC1.C:125-176): Spalart-Shur, with DSDt[9] = ddt(rho, S_ij) +
+// div(phi, S_ij) per unit volume; sqrS = 2|S|^2, sqrW = 2|W|^2, sqrD =
+// (sqrS + sqrW)/2, rTilda = 2 / (rho max(sqrD, SMALL)^2) sum_ij (sum_k W_ik
+// S_jk) DSDt_ij
+__device__ inline double sa_fr1_ss(const rdst_turbulence& t, const double* g, double rho, const double* DSDt) {
+    if (!t.rcm_correction) return 1.0;
+    double S[9], W[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            S[3 * i + j] = 0.5 * (g[3 * i + j] + g[3 * j + i]);
+            W[3 * i + j] = 0.5 * (g[3 * i + j] - g[3 * j + i]);
+        }
+    double sqrS = 0.0, sqrW = 0.0;
+    for (int q = 0; q < 9; ++q) { sqrS += S[q] * S[q]; sqrW += W[q] * W[q]; }
+    sqrS *= 2.0;
+    sqrW *= 2.0;
+    const double sqrD = 0.5 * (sqrS + sqrW);
+    const double rStar = sqrt(sqrS / fmax(sqrW, kSMALL));
+    double sum = 0.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double ws = 0.0;
+            for (int k = 0; k < 3; ++k) ws += W[3 * i + k] * S[3 * j + k];
+            sum += ws * DSDt[3 * i + j];
+        }
+    const double rTilda = 2.0 / (rho * sq(fmax(sqrD, kSMALL))) * sum;
+    return (1.0 + t.Cr1) * 2.0 * rStar / (1.0 + rStar) * (1.0 - t.Cr3 * atan(t.Cr2 * rTilda)) - t.Cr1;
+}
+__device__ inline double symm_comp(const double* gU, size_t stride, size_t i, int q) {
+    const int a = q / 3, b = q % 3;
+    return 0.5 * (gU[(size_t)(3 * a + b) * stride + i] + gU[(size_t)(3 * b + a) * stride + i]);
+}
+// S.oldTime() for the next correct()
+__global__ void k_sa_store_S(int n, Fields F, SaDev s) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    for (int q = 0; q < 9; ++q) s.Sold[(size_t)q * n + c] = symm_comp(F.gradU, n, c, q);
 }
 
 // ---------------------------------------------------------------- boundary values
@@ -163,7 +202,9 @@ __global__ __launch_bounds__(256) void k_sa_grad(MeshDev m, SaDev s) {
 // one thread per cell: the row of the relaxed nuTilda matrix in solve form
 // (diagonal incl. internalCoeffs at slotDiag, off-diagonals at the face
 // slots, source incl. boundaryCoeffs), and fv1(chi) of the current state
-__global__ __launch_bounds__(256) void k_sa_assemble(MeshDev m, Fields F, SaDev s, rdst_turbulence t, int useRhoPrev) {
+template <bool SS>
+__global__ __launch_bounds__(256) void k_sa_assemble(MeshDev m, Fields F, SaDev s, rdst_turbulence t, int useRhoPrev,
+                                                     int useSold) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
     const size_t E = (size_t)m.ellW * m.nC;
@@ -177,6 +218,10 @@ __global__ __launch_bounds__(256) void k_sa_assemble(MeshDev m, Fields F, SaDev 
     double src = F.rDeltaT[c] * rhoOld * nut * V;
     double sumOff = 0.0, icRel = 0.0, icRaw = 0.0, bcfSum = 0.0, srcBndRelax = 0.0;
     const double alpha = t.relax;
+    // SARC1: div(phi, S_ij), Gauss upwind, and S of this cell
+    double dS[9], Sc[9];
+    if (SS)
+        for (int q = 0; q < 9; ++q) { dS[q] = 0.0; Sc[q] = symm_comp(F.gradU, n, c, q); }
     for (int k = 0; k < m.ellW; ++k) {
         const size_t e = (size_t)k * m.nC + c;
         const uint32_t code = m.ellCode[e];
@@ -215,6 +260,8 @@ __global__ __launch_bounds__(256) void k_sa_assemble(MeshDev m, Fields F, SaDev 
             }
             icRaw += ic;
             bcfSum += bcf;
+            if (SS)
+                for (int q = 0; q < 9; ++q) dS[q] += phib * symm_comp(F.bgradU, m.nB, idx, q);
             continue;
         }
         const bool own = kind == ELL_OWNER;
@@ -223,6 +270,15 @@ __global__ __launch_bounds__(256) void k_sa_assemble(MeshDev m, Fields F, SaDev 
         // upwind convection: lower = -w phi, upper = lower + phi, w = pos(phi)
         const double lo = -fpos(phi) * phi;
         const double up = lo + phi;
+        if (SS) {
+            const double wo = fpos(phi);  // upwind weight of the owner side
+            for (int q = 0; q < 9; ++q) {
+                const double So = symm_comp(F.gradU, n, idx, q);
+                const double sf = own ? wo * Sc[q] + (1.0 - wo) * So : wo * So + (1.0 - wo) * Sc[q];
+                if (own) dS[q] += phi * sf;
+                else dS[q] -= phi * sf;
+            }
+        }
         // laplacian coefficient Gamma_f |Sf| deltaCoeffs and the explicit correction
         const double w = m.ellWt[e];
         const double Go = sa_gamma(t, F.rho[idx], s.nut[idx], F.mu[idx]);
@@ -281,7 +337,18 @@ __global__ __launch_bounds__(256) void k_sa_assemble(MeshDev m, Fields F, SaDev 
     } else {
         const double Stilda = sa_fv3(t, chi, fv1) * sqrt(2.0) * sqrt(mag_sqr_skew(g)) + fv2 * nut / sq(t.kappa * d);
         const double fw = sa_fw(t, Stilda, nut, d);
-        src += t.Cb1 * rho * Stilda * nut * sa_fr1(t, g) * V;
+        double fr1;
+        if (SS) {
+            // steadyStateDST::fvcDdt(rho, S): rDeltaT (rho S - rho.oldTime() S.oldTime())
+            for (int q = 0; q < 9; ++q) {
+                const double So = useSold ? s.Sold[(size_t)q * n + c] : Sc[q];
+                dS[q] = F.rDeltaT[c] * (rho * Sc[q] - rhoOld * So) + dS[q] / V;
+            }
+            fr1 = sa_fr1_ss(t, g, rho, dS);
+        } else {
+            fr1 = sa_fr1(t, g);
+        }
+        src += t.Cb1 * rho * Stilda * nut * fr1 * V;
         diag += Cw1 * fw * nut * rho / sq(d) * V;
     }
     // fvMatrix::relax(alpha) and the boundary coefficients
@@ -469,7 +536,7 @@ static SaDev sa_dev(Case& c) {
     SaDev d;
     d.bc = s.d_bc.p;
     d.nut = s.nut; d.bnut = s.bnut; d.d = s.d; d.rhoPrev = s.rhoPrev; d.fv1o = s.fv1o; d.gn = s.gn;
-    d.B = s.B; d.x2 = s.x2; d.A = s.A.p;
+    d.B = s.B; d.x2 = s.x2; d.A = s.A.p; d.Sold = s.Sold;
     return d;
 }
 
@@ -528,11 +595,11 @@ void sa_set(Case& c, const rdst_turbulence* t, const rdst_scalar_bc* bcs, const 
         if (!(wallDist[i] > 0.0)) throw ArgError("rdst_case_set_turbulence: wall distance must be > 0 in every cell");
     s.bc.assign(bcs, bcs + m.nPatches);
     s.d_bc.upload(s.bc);
-    s.buf.alloc(9 * n + nB);
+    s.buf.alloc(18 * n + nB);
     s.buf.zero();
     double* p = s.buf.p;
     s.nut = p; s.d = p + n; s.rhoPrev = p + 2 * n; s.fv1o = p + 3 * n; s.B = p + 4 * n; s.x2 = p + 5 * n;
-    s.gn = p + 6 * n; s.bnut = p + 9 * n;
+    s.gn = p + 6 * n; s.bnut = p + 9 * n; s.Sold = p + 9 * n + nB;
     s.A.alloc(m.nnzb);
     s.A.zero();
     s.part.alloc(2 * RED_MAXG);
@@ -567,8 +634,15 @@ void sa_correct(Case& c, rdst_solver_perf* perf) {
     // grad(U) of the updated state, grad(nuTilda), the matrix
     grad_u_launch(c);
     hipLaunchKernelGGL(k_sa_grad, dim3(nb), dim3(bs), 0, c.stream, md, sa_dev(c));
-    hipLaunchKernelGGL(k_sa_assemble, dim3(nb), dim3(bs), 0, c.stream, md, c.f, sa_dev(c), t,
-                       (s.nCorrect > 0 && s.rhoPrevValid) ? 1 : 0);
+    const int useRhoPrev = (s.nCorrect > 0 && s.rhoPrevValid) ? 1 : 0;
+    if (t.model == RDST_TURB_SARC1) {
+        hipLaunchKernelGGL((k_sa_assemble<true>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, sa_dev(c), t, useRhoPrev,
+                           s.nCorrect > 0 ? 1 : 0);
+        hipLaunchKernelGGL(k_sa_store_S, dim3(nb), dim3(bs), 0, c.stream, n, c.f, sa_dev(c));
+    } else {
+        hipLaunchKernelGGL((k_sa_assemble<false>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, sa_dev(c), t, useRhoPrev,
+                           0);
+    }
     RDST_HIP(hipGetLastError());
     // smoothSolver: normFactor, initial residual
     const int G = reduce_grid(n);

@@ -25,7 +25,7 @@ def coefficients(case, spec):
                        p, p / (R * T), U, 0.1)
 
 
-@pytest.mark.parametrize("model,kw", [("SAneg", {}), ("SARC2", {}),
+@pytest.mark.parametrize("model,kw", [("SAneg", {}), ("SARC2", {}), ("SARC1", {}),
                                       ("SAneg", {"ilu_order": "partitioned", "gram_schmidt": "classical"})])
 def test_airfoil_iterate_parity(oracle, model, kw):
     from rhodst_amd import _capi

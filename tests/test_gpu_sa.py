@@ -16,7 +16,7 @@ def rel(a, b):
 
 
 VARIANTS = [("SAneg", {}), ("SAneg", {"stilda_modification": 0}), ("SAneg", {"negative_nu_tilda": 0}),
-            ("SARC2", {}), ("SARC2", {"rcm_correction": 0, "relax": 0.0})]
+            ("SARC2", {}), ("SARC2", {"rcm_correction": 0, "relax": 0.0}), ("SARC1", {})]
 
 
 @pytest.mark.parametrize("model,kw", VARIANTS)
@@ -62,7 +62,7 @@ def test_sa_partitioned_sweep_parity(oracle, model):
     assert rel(dc.get(30), oc.get(30)) <= 1e-10
 
 
-@pytest.mark.parametrize("model", ["SAneg", "SARC2"])
+@pytest.mark.parametrize("model", ["SAneg", "SARC2", "SARC1"])
 def test_sa_iterate_parity(oracle, model):
     print(f"[SA iterate] {model}", flush=True)
     spec = cases.box_sa_case(model)

@@ -350,3 +350,34 @@ def test_turbulent_viscosity_enters_viscous_residual(oracle):
     assert touched.any() and not touched.all()
     assert np.array_equal(R0[~touched], Rl[~touched])
     assert _rel(oc.get(8)[:, 1:4], Rl[:, 1:4]) > 1e-6
+
+
+def test_sarc1_without_correction_is_sarc2_without(oracle):
+    """SARC1 and SARC2 differ only in fr1 (SARC1.C:125-176, SARC2.C:121-135):
+    with the correction switched off both use fr1 = 1, and the nuTilda matrices
+    of two consecutive correct() calls agree exactly."""
+    out = []
+    for model in ("SARC1", "SARC2"):
+        spec = cases.box_sa_case(model, turb_kw={"rcm_correction": 0})
+        oc = oracle.OracleCase(spec)
+        oc.iterate(2)
+        oc.flux_residual()
+        oc.turbulence_correct()
+        out.append([oc.get(f) for f in (35, 36, 37, 38, 29)])
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
+def test_sarc1_rotation_correction_acts(oracle):
+    """With spalartShurCorrection on, SARC1's production differs from the
+    uncorrected form (fr1 != 1 where the flow has strain and rotation), and
+    from the second correct() on the ddt part of DS/Dt enters (S.oldTime() is
+    the previous call's S)."""
+    res = {}
+    for on in (1, 0):
+        spec = cases.box_sa_case("SARC1", turb_kw={"rcm_correction": on})
+        oc = oracle.OracleCase(spec)
+        oc.iterate(3)
+        res[on] = oc.get(29)
+        assert np.all(np.isfinite(res[on]))
+    assert _rel(res[1], res[0]) > 1e-8
