@@ -196,14 +196,17 @@ __device__ inline Prim<D1> seeded(const double* W, int i, int k, const Thermo& t
 }
 
 // one thread per cell: its row of J = dR/dW (diagonal block and the block of
-// every internal face, at this row's slots)
-template <int S>
+// every internal face, at this row's slots).  NEWTON: the row of the Newton
+// operator instead, V_c dR_c/dW + rDeltaTDST V_c I (the face sums not divided
+// by V), stored as TJ (FP32 for the mixed path), and the Newton source -R_c V_c
+// into src (the ddt part of the steady-state source vanishes)
+template <int S, class TJ, bool NEWTON>
 __global__ __launch_bounds__(256) void k_adj_jacobian(MeshDev m, Fields F, const BcPatchDev* bc, Thermo th, Bounds bd,
-                                                      int lowMach, double Mainf, double* J) {
+                                                      int lowMach, double Mainf, TJ* J, double* src) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
     const size_t E = (size_t)m.ellW * m.nC;
-    const double rV = 1.0 / m.V[c];
+    const double rV = NEWTON ? 1.0 : 1.0 / m.V[c];
     const double gamma = th.gamma;
     double Jcc[25];
     for (int q = 0; q < 25; ++q) Jcc[q] = 0.0;
@@ -233,7 +236,7 @@ __global__ __launch_bounds__(256) void k_adj_jacobian(MeshDev m, Fields F, const
         const bool own = kind == ELL_OWNER;
         const double sgn = own ? rV : -rV;
         const int fi = m.ellFace[e];
-        double* Jco = J + 25 * (size_t)(own ? m.slotUpper[fi] : m.slotLower[fi]);
+        TJ* Jco = J + 25 * (size_t)(own ? m.slotUpper[fi] : m.slotLower[fi]);
         // derivative w.r.t. this cell's W, then w.r.t. the other cell's W
         for (int side = 0; side < 2; ++side) {
             for (int kk = 0; kk < 5; ++kk) {
@@ -246,12 +249,17 @@ __global__ __launch_bounds__(256) void k_adj_jacobian(MeshDev m, Fields F, const
                 if (side == 0)
                     for (int r = 0; r < 5; ++r) Jcc[r * 5 + kk] += sgn * d[r];
                 else
-                    for (int r = 0; r < 5; ++r) Jco[r * 5 + kk] = sgn * d[r];
+                    for (int r = 0; r < 5; ++r) Jco[r * 5 + kk] = (TJ)(sgn * d[r]);
             }
         }
     }
-    double* Jd = J + 25 * (size_t)m.slotDiag[c];
-    for (int q = 0; q < 25; ++q) Jd[q] = Jcc[q];
+    TJ* Jd = J + 25 * (size_t)m.slotDiag[c];
+    const double dd = NEWTON ? F.rDeltaT[c] * m.V[c] : 0.0;
+    for (int q = 0; q < 25; ++q) Jd[q] = (TJ)(Jcc[q] + ((q % 6 == 0) ? dd : 0.0));
+    if (NEWTON) {
+        const double V = m.V[c];
+        for (int q = 0; q < 5; ++q) src[5 * (size_t)c + q] = 0.0 - F.R[(size_t)q * m.nC + c] * V;
+    }
 }
 
 // the residual itself from W by the same chain (the value parts of the duals)
@@ -351,11 +359,22 @@ Bounds bounds_of(const Case& c) {
     return b;
 }
 
-template <int S>
-void launch_jac(Case& c, double* J) {
+template <int S, class TJ, bool NEWTON>
+void launch_jac(Case& c, TJ* J) {
     const int nb = blocks_for(c.mesh->nC, 256);
-    hipLaunchKernelGGL((k_adj_jacobian<S>), dim3(nb), dim3(256), 0, c.stream, c.meshDev(), c.f, c.d_bc.p, c.th,
-                       bounds_of(c), c.st.low_mach, c.st.Mainf, J);
+    hipLaunchKernelGGL((k_adj_jacobian<S, TJ, NEWTON>), dim3(nb), dim3(256), 0, c.stream, c.meshDev(), c.f, c.d_bc.p,
+                       c.th, bounds_of(c), c.st.low_mach, c.st.Mainf, J, NEWTON ? c.src.p : (double*)nullptr);
+}
+template <class TJ, bool NEWTON>
+void launch_jac_scheme(Case& c, TJ* J) {
+    switch (c.st.flux_scheme) {
+        case RDST_FLUX_AUSMPLUS: launch_jac<RDST_FLUX_AUSMPLUS, TJ, NEWTON>(c, J); break;
+        case RDST_FLUX_AUSMPLUSUP: launch_jac<RDST_FLUX_AUSMPLUSUP, TJ, NEWTON>(c, J); break;
+        case RDST_FLUX_HLL: launch_jac<RDST_FLUX_HLL, TJ, NEWTON>(c, J); break;
+        case RDST_FLUX_HLLC: launch_jac<RDST_FLUX_HLLC, TJ, NEWTON>(c, J); break;
+        default: launch_jac<RDST_FLUX_KURGANOV, TJ, NEWTON>(c, J); break;
+    }
+    RDST_HIP(hipGetLastError());
 }
 template <int S>
 void launch_res(Case& c, double* R) {
@@ -382,14 +401,22 @@ void adjoint_jacobian(Case& c) {
     const size_t nn = 25 * (size_t)m.nnzb;
     if (c.adjJ.n != nn) c.adjJ.alloc(nn);
     c.adjJ.zero(c.stream);
-    switch (c.st.flux_scheme) {
-        case RDST_FLUX_AUSMPLUS: launch_jac<RDST_FLUX_AUSMPLUS>(c, c.adjJ.p); break;
-        case RDST_FLUX_AUSMPLUSUP: launch_jac<RDST_FLUX_AUSMPLUSUP>(c, c.adjJ.p); break;
-        case RDST_FLUX_HLL: launch_jac<RDST_FLUX_HLL>(c, c.adjJ.p); break;
-        case RDST_FLUX_HLLC: launch_jac<RDST_FLUX_HLLC>(c, c.adjJ.p); break;
-        default: launch_jac<RDST_FLUX_KURGANOV>(c, c.adjJ.p); break;
+    launch_jac_scheme<double, false>(c, c.adjJ.p);
+}
+
+void newton_operator(Case& c) {
+    adjoint_check_scheme(c);
+    const Mesh& m = *c.mesh;
+    const size_t nn = 25 * (size_t)m.nnzb;
+    if (c.st.precision == RDST_PREC_MIXED) {
+        if (c.A32.n != nn) c.A32.alloc(nn);
+        c.A32.zero(c.stream);
+        launch_jac_scheme<float, true>(c, c.A32.p);
+    } else {
+        if (c.A64.n != nn) c.A64.alloc(nn);
+        c.A64.zero(c.stream);
+        launch_jac_scheme<double, true>(c, c.A64.p);
     }
-    RDST_HIP(hipGetLastError());
 }
 
 void adjoint_linearise(Case& c) {

@@ -181,28 +181,6 @@ __global__ __launch_bounds__(128) void k_assemble(MeshDev m, Fields F, Thermo th
     }
 }
 
-// Newton operator from the exact linearisation J = dR/dW (R per unit
-// volume, FP64 block-CSR): row c of A is V_c J_c plus rDeltaTDST V_c I on the
-// diagonal; the source is the Roe path's (the ddt part vanishes, -R V).
-template <bool F32>
-__global__ __launch_bounds__(256) void k_assemble_exact(MeshDev m, Fields F, const int32_t* rowPtr, const double* J,
-                                                        double* A64, float* A32, double* src) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= m.nC) return;
-    const double V = m.V[c];
-    const double diagS = F.rDeltaT[c] * V;
-    const int ds = m.slotDiag[c];
-    for (int s = rowPtr[c]; s < rowPtr[c + 1]; ++s) {
-        const double* b = J + 25 * (size_t)s;
-        for (int q = 0; q < 25; ++q) {
-            const double v = V * b[q] + ((s == ds && q % 6 == 0) ? diagS : 0.0);
-            if (F32) A32[25 * (size_t)s + q] = (float)v;
-            else A64[25 * (size_t)s + q] = v;
-        }
-    }
-    for (int q = 0; q < 5; ++q) src[5 * (size_t)c + q] = 0.0 - F.R[(size_t)q * m.nC + c] * V;
-}
-
 }  // namespace
 
 void jacobian_assemble(Case& c) {
@@ -222,16 +200,7 @@ void jacobian_assemble(Case& c) {
             hipLaunchKernelGGL((k_assemble<false>), dim3(blocks_for(m.nC, 128)), dim3(128), 0, c.stream, md, c.f, c.th,
                                c.d_bc.p, c.st.beta, c.P64.p, (float*)nullptr, c.src.p);
         }
-        adjoint_jacobian(c);
-        if (mixed) {
-            if (c.A32.n != 25 * (size_t)m.nnzb) c.A32.alloc(25 * (size_t)m.nnzb);
-            hipLaunchKernelGGL((k_assemble_exact<true>), dim3(blocks_for(m.nC, 256)), dim3(256), 0, c.stream, md, c.f,
-                               m.d_rowPtr.p, c.adjJ.p, (double*)nullptr, c.A32.p, c.src.p);
-        } else {
-            if (c.A64.n != 25 * (size_t)m.nnzb) c.A64.alloc(25 * (size_t)m.nnzb);
-            hipLaunchKernelGGL((k_assemble_exact<false>), dim3(blocks_for(m.nC, 256)), dim3(256), 0, c.stream, md, c.f,
-                               m.d_rowPtr.p, c.adjJ.p, c.A64.p, (float*)nullptr, c.src.p);
-        }
+        newton_operator(c);
         RDST_HIP(hipGetLastError());
         c.hasMatrix = true;
         return;

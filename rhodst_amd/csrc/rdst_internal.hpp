@@ -291,6 +291,7 @@ void sa_correct(Case& c, rdst_solver_perf* perf);
 void sa_put_nut(Case& c, const double* nuTilda);
 void adjoint_check_scheme(const Case& c);
 void adjoint_jacobian(Case& c);   // J = dR/dW only
+void newton_operator(Case& c);    // V dR/dW + rDeltaTDST V I into the case's A64 / A32
 void adjoint_linearise(Case& c);  // J and its transpose
 void adjoint_residual(Case& c, double* R);
 void adjoint_objective_gradient(Case& c, int patch, const double* dir, double q, double* g);

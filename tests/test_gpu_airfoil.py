@@ -41,10 +41,15 @@ def test_airfoil_iterate_parity(oracle, model, kw):
     assert np.all(np.isfinite(r1))
     assert rel(r1, r0) <= 1e-9, (r1, r0)
     # twelve steps amplify the rounding differences of the reductions (device
-    # tree sums vs the oracle's sequential sums) to about 1e-8 in nuTilda
+    # tree sums vs the oracle's sequential sums) to about 1e-8 in nuTilda.
+    # SARC1 amplifies them to ~2e-5 within three steps in the oracle alone
+    # (a 1e-14 perturbation of W): in the irrotational free stream its
+    # rTilda = 2 (W S : DS/Dt) / (rho max(D^2, SMALL)^2) divides rounding
+    # noise by SMALL^2 and atan(Cr2 rTilda) saturates either way
     assert rel(dc.get(9), oc.get(9)) <= 1e-8
+    tol = 1e-4 if model == "SARC1" else 1e-7
     for f in (29, 30):
-        assert rel(dc.get(f), oc.get(f)) <= 1e-7, f
+        assert rel(dc.get(f), oc.get(f)) <= tol, f
     co, cd = coefficients(oc, spec), coefficients(dc, spec)
     assert rel(cd[:2], co[:2]) <= 1e-8, (cd, co)
 
