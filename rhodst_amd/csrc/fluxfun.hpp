@@ -92,6 +92,24 @@ __device__ inline SideT<T> make_side(T rho, T rux, T ruy, T ruz, T rhoE, T p, T 
     return s;
 }
 
+// the packed face-loop record of cell i (Fields::cell8), from its state
+__device__ inline void store_cell8(const Fields& F, int i, double rho, double rux, double ruy, double ruz, double rhoE,
+                                   double p, double h, double psi, double gamma) {
+    double2* o = reinterpret_cast<double2*>(F.cell8 + 8 * (size_t)i);
+    if (F.cell8Kind == CELL8_SIDE) {
+        const SideT<double> sd = make_side<double>(rho, rux, ruy, ruz, rhoE, p, h, psi, gamma);
+        o[0] = make_double2(sd.rho, sd.ux);
+        o[1] = make_double2(sd.uy, sd.uz);
+        o[2] = make_double2(sd.rr, sd.rPsi);
+        o[3] = make_double2(sd.e, sd.aStar);
+    } else {
+        o[0] = make_double2(rho, rux);
+        o[1] = make_double2(ruy, ruz);
+        o[2] = make_double2(rhoE, p);
+        o[3] = make_double2(h, psi);
+    }
+}
+
 // AUSM+ split Mach/pressure polynomials with the foam sign() masks
 // (sign(0) = +1, so at |M| = 1 both branches contribute).
 template <class T>

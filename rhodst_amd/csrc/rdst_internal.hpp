@@ -73,6 +73,62 @@ static constexpr uint32_t ELL_IDX_MASK = 0x3fffffffu;
 
 // ---------------------------------------------------------------------------
 // Mesh: polyMesh primitives, fvMesh geometry (host), device copies.
+// One sweep of the partitioned ILU(0) apply laid out for coalesced loads.
+// The rows of a partition-local level are independent; a level is cut into
+// sub-levels of at most kPiluSubRows rows, and inside a sub-level the rows are
+// ordered by their block count, largest first, so that block k of every row
+// that has one forms a contiguous "jagged diagonal": diagonal k of sub-level
+// s holds n_k blocks at stream positions blk_k .. blk_k + n_k - 1, row j's at
+// blk_k + j.  Block values are stored per diagonal as five planes (column c of
+// the block) of 5 n_k values, entry (j, r, c) at
+// 25 blk_k + 5 n_k c + 5 j + r, so the five lanes of a row and the rows of a
+// wavefront read contiguous memory.  Sub-level s is described by
+// sub[kPiluSubW s ..]: rowOff (into ord), nRows, kmax, 0, blk_0..blk_7,
+// n_0..n_7.  The forward stream holds each row's in-partition L blocks (rows
+// without one are left out); the backward stream holds, per row, Dinv first
+// (its column is the row itself) and then Dinv U for each in-partition U
+// block, so x = Dinv y - sum (Dinv U) x needs no separate diagonal solve.
+constexpr int kPiluKD = 8;                  // most blocks of a row in one sweep
+constexpr int kPiluSubW = 4 + 2 * kPiluKD;  // ints per sub-level descriptor
+constexpr int kPiluSubRows = 48;            // 4 wavefronts x 12 rows of 5 lanes
+struct PiluStream {
+    int kmax = 0;                  // largest row block count (0: not built)
+    long nBlk = 0;
+    std::vector<int32_t> subStart;  // [nParts + 1]
+    std::vector<int32_t> sub;       // [nSub + 1][kPiluSubW] (one zero descriptor of padding)
+    std::vector<int32_t> ord;       // partition-local row of each sub-level position
+    std::vector<int32_t> lc;        // [nBlk] partition-local column
+    std::vector<int32_t> src;       // [nBlk] block-CSR slot, or -(cell + 1) for the row's Dinv
+    std::vector<int32_t> row;       // [nBlk] cell of the block's row
+    std::vector<int32_t> jn;        // [nBlk] j | n_k << 8
+    DBuf<int32_t> d_subStart, d_sub, d_ord, d_lc, d_src, d_row, d_jn;
+};
+
+// Tiles of the first-order inviscid face loop (k_flux_tiles): cells cut into
+// tiles of at most kTileCells by recursive bisection of the cell centres,
+// each cut across the axis with the most cells (extent over the median cell
+// size along it), so a tile is compact in cell counts and most faces have
+// both cells in one tile.  Such a face is evaluated once, by its owner's
+// thread, and handed to the neighbour's thread through LDS; every other face
+// side is evaluated by its own cell as in the ELL loop.  Positions p =
+// kTileCells * tile + lane; per position and slot k (layout [W][NP]):
+// code (ELL code word), share (LDS slot of a shared face, -1 otherwise),
+// face (surface-field index), Sf [3][W][NP].  In each cell's list the
+// shared neighbour-side entries come last.
+constexpr int kTileCells = 256;
+struct FluxTiles {
+    int nTiles = 0, W = 0, maxShared = 0;
+    std::vector<int32_t> cell;   // [NP] cell at each position, -1 for padding
+    std::vector<uint32_t> code;  // [W][NP]
+    std::vector<int16_t> share;  // [W][NP]
+    std::vector<int32_t> face;   // [W][NP]
+    std::vector<double> Sf;      // [3][W][NP]
+    DBuf<int32_t> d_cell, d_face;
+    DBuf<uint32_t> d_code;
+    DBuf<int16_t> d_share;
+    DBuf<double> d_Sf;
+};
+
 struct Mesh {
     int nC = 0, nI = 0, nB = 0, nF = 0;  // nF = nI + nB surface-field length
     int nPatches = 0;
@@ -105,6 +161,10 @@ struct Mesh {
     // cell) order, level offsets, and per block the partition-local column
     int nParts = 0, maxPartRows = 0, maxPartLevels = 0;
     std::vector<int32_t> part, partRowStart, partRows, partLevStart, partLevPtr, localCol;
+    // coalesced streams of the partitioned ILU(0) apply (forward: the
+    // in-partition L blocks; backward: Dinv and Dinv U): see PiluStream
+    PiluStream pf, pb;
+    FluxTiles tiles;
     // sliced-ELL copy of the block-CSR pattern for the SpMV: slices of 64
     // rows, slice s holds sellWidth[s] block columns; block p of row r sits at
     // block position sellPtr[s] + 64 p + (r - 64 s), its entry e at value
@@ -139,6 +199,7 @@ struct Mesh {
 constexpr int kPartCells = 1024;
 void mesh_build(Mesh& m, const rdst_mesh_desc* d);
 void build_partitions(Mesh& m, int maxCells);
+void build_flux_tiles(Mesh& m);
 void mesh_upload(Mesh& m);
 
 // ---------------------------------------------------------------------------
@@ -181,22 +242,22 @@ struct Fields {
     // boundary values [4][nB]; Gauss gradients of rho, rhoU, rPsi, e, a*, c
     // [24][nC] (component 3q + i = d(field q)/dx_i)
     double *rec, *brec, *rgrad;
-    // the face loop's copy of the cell state, AoS [nC][8] = rho, rhoUx, rhoUy,
-    // rhoUz, rhoE, p, h, psi: 64 B per cell, so a neighbour gather touches
-    // one cache line instead of eight SoA lines; written by the kernels that
-    // produce the state (k_init_cells, k_update_cells)
+    // the face loop's copy of the cell state, AoS [nC][8], 64 B per cell, so
+    // a neighbour gather touches one cache line instead of eight SoA lines;
+    // written by the kernels that produce the state (k_init_cells,
+    // k_update_cells).  CELL8_STATE: rho, rhoUx, rhoUy, rhoUz, rhoE, p, h,
+    // psi.  CELL8_SIDE (the schemes whose flux reads only rho, U, 1/rho,
+    // 1/psi, e and a* of a side): those eight quantities themselves, formed
+    // once per cell instead of once per face side.
     double* cell8;
+    int cell8Kind;
 };
-#ifdef __HIPCC__
-__device__ inline void store_cell8(const Fields& F, int i, double rho, double rux, double ruy, double ruz, double rhoE,
-                                   double p, double h, double psi) {
-    double2* o = reinterpret_cast<double2*>(F.cell8 + 8 * (size_t)i);
-    o[0] = make_double2(rho, rux);
-    o[1] = make_double2(ruy, ruz);
-    o[2] = make_double2(rhoE, p);
-    o[3] = make_double2(h, psi);
+enum { CELL8_STATE = 0, CELL8_SIDE = 1 };
+inline int cell8_kind(int fluxScheme) {
+    return (fluxScheme == RDST_FLUX_AUSMPLUS || fluxScheme == RDST_FLUX_AUSMPLUSUP || fluxScheme == RDST_FLUX_HLL)
+               ? CELL8_SIDE
+               : CELL8_STATE;
 }
-#endif
 
 // Spalart-Allmaras state of a case (SAneg/SARC2 nuTilda_, mut_, alphat_, d_)
 struct SaState {

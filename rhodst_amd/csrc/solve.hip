@@ -38,6 +38,8 @@ struct SolverWork {
     DBuf<double> scal;         // device scalars
     DBuf<double> partA, partB; // reduction partials
     DBuf<double> multi;        // classical Gram-Schmidt partials [nDirs+1][G]
+    DBuf<double> sf, sb;       // FP64 forward / backward ILU(0) apply streams
+    DBuf<float> sf32, sb32;    // FP32 ones
     std::vector<double> hscal;
     // BiCGStab
     DBuf<double> p, ph, v, t, th, rw;
@@ -278,18 +280,40 @@ __global__ __launch_bounds__(64) void k_sell_spmv(int n, const int64_t* sellPtr,
 }
 
 // ---------------------------------------------------------------- classical Gram-Schmidt
-// partial dots of w with NACC consecutive basis vectors V_j0 .. V_j0+NACC-1 (cnt of them valid)
+// partial dots of w with NACC consecutive basis vectors V_j0 .. V_j0+NACC-1
+// (cnt of them valid).  Each thread walks its grid-stride elements four at a
+// time so that 4 (NACC+1) loads are in flight; the accumulation order per
+// thread is the plain grid-stride order, so the partials do not depend on it.
 template <class T, int NACC>
 __global__ __launch_bounds__(RED_BS) void k_multidot_partial(long n, const T* w, const T* V, int j0, int cnt,
                                                              double* part) {
     __shared__ double sh[16];
     double acc[NACC];
     for (int q = 0; q < NACC; ++q) acc[q] = 0.0;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long st = (long)gridDim.x * blockDim.x;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const T* Vb = V + (size_t)j0 * n;
+    for (; i + 3 * st < n; i += 4 * st) {
+        double wi[4];
+        T v[4][NACC];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) wi[u] = (double)w[i + u * st];
+#pragma unroll
+        for (int q = 0; q < NACC; ++q)
+            if (q < cnt)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) v[u][q] = Vb[(size_t)q * n + i + u * st];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int q = 0; q < NACC; ++q)
+                if (q < cnt) acc[q] += wi[u] * (double)v[u][q];
+    }
+    for (; i < n; i += st) {
         const double wi = (double)w[i];
 #pragma unroll
         for (int q = 0; q < NACC; ++q)
-            if (q < cnt) acc[q] += wi * (double)V[(size_t)(j0 + q) * n + i];
+            if (q < cnt) acc[q] += wi * (double)Vb[(size_t)q * n + i];
     }
 #pragma unroll
     for (int q = 0; q < NACC; ++q) {
@@ -298,23 +322,49 @@ __global__ __launch_bounds__(RED_BS) void k_multidot_partial(long n, const T* w,
         if (threadIdx.x == 0) part[(size_t)(j0 + q) * gridDim.x + blockIdx.x] = t;
     }
 }
-// H[j][col] = sum of the partials of dot j, j = 0..cnt-1 (one block)
+// H[j][col] = sum of the G partials of dot j; block j reduces dot j
 __global__ __launch_bounds__(RED_BS) void k_multidot_final(const double* part, int G, int cnt, double* H, int m, int col) {
     __shared__ double sh[16];
-    for (int j = 0; j < cnt; ++j) {
-        const double t = reduce_partials(part + (size_t)j * G, G, sh);
-        if (threadIdx.x == 0) H[j * m + col] = t;
-    }
+    const int j = blockIdx.x;
+    if (j >= cnt) return;
+    const double t = reduce_partials(part + (size_t)j * G, G, sh);
+    if (threadIdx.x == 0) H[j * m + col] = t;
 }
-// w -= sum_j H[j][col] V_j, and the partial sums of |w|^2
+// w -= sum_j H[j][col] V_j, and the partial sums of |w|^2; four grid-stride
+// elements per pass (same per-element arithmetic and partial order as one)
 template <class T>
 __global__ __launch_bounds__(RED_BS) void k_cgs_update(long n, T* w, const T* V, const double* H, int m, int col, int cnt,
                                                        double* part) {
+    constexpr int HC = RED_BS;
     __shared__ double sh[16];
+    __shared__ double hc[HC];
+    for (int t = threadIdx.x; t < cnt && t < HC; t += blockDim.x) hc[t] = H[t * m + col];
+    __syncthreads();
     double s = 0.0;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long st = (long)gridDim.x * blockDim.x;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * st < n; i += 4 * st) {
+        double wi[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) wi[u] = (double)w[i + u * st];
+        for (int j = 0; j < cnt; ++j) {
+            const double h = j < HC ? hc[j] : H[j * m + col];
+            T v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = V[(size_t)j * n + i + u * st];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) wi[u] -= h * (double)v[u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const T wt = (T)wi[u];
+            w[i + u * st] = wt;
+            s += (double)wt * (double)wt;
+        }
+    }
+    for (; i < n; i += st) {
         double wi = (double)w[i];
-        for (int j = 0; j < cnt; ++j) wi -= H[j * m + col] * (double)V[(size_t)j * n + i];
+        for (int j = 0; j < cnt; ++j) wi -= (j < HC ? hc[j] : H[j * m + col]) * (double)V[(size_t)j * n + i];
         const T wt = (T)wi;
         w[i] = wt;
         s += (double)wt * (double)wt;
@@ -469,6 +519,102 @@ __global__ __launch_bounds__(PILU_FACTOR_THREADS) void k_pilu_factor(const int32
         }
         __syncthreads();
     }
+}
+
+// Scatter the factor into the apply streams (layout of PiluStream): one
+// thread per (stream block, block row r).  Backward blocks other than the
+// row's Dinv are stored as Dinv U, formed in FP64.
+template <class T, bool BWD>
+__global__ __launch_bounds__(256) void k_pilu_pack(long nBlk, const int32_t* src, const int32_t* rowc,
+                                                   const int32_t* jn, const T* LU, const T* Dinv, T* val) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 5 * nBlk) return;
+    const long b = e / 5;
+    const int r = (int)(e - 5 * b);
+    const int code = jn[b];
+    const int j = code & 255, nk = code >> 8;
+    T* dst = val + 25 * (size_t)(b - j) + 5 * j + r;
+    const int sl = src[b];
+    if (!BWD || sl < 0) {
+        const T* B = BWD ? Dinv + 25 * (size_t)(-sl - 1) + 5 * r : LU + 25 * (size_t)sl + 5 * r;
+        for (int c = 0; c < 5; ++c) dst[(size_t)c * 5 * nk] = B[c];
+    } else {
+        const T* D = Dinv + 25 * (size_t)rowc[b] + 5 * r;
+        const T* U = LU + 25 * (size_t)sl;
+        for (int c = 0; c < 5; ++c) {
+            double a = 0.0;
+            for (int q = 0; q < 5; ++q) a += (double)D[q] * (double)U[5 * q + c];
+            dst[(size_t)c * 5 * nk] = (T)a;
+        }
+    }
+}
+
+// One sweep over the sub-levels [s0, s1) of a partition (y in LDS).  Lane
+// layout: 12 rows of five lanes per wavefront (lanes 60..63 idle), so the
+// lanes of a row share a wavefront and the backward row's read of its own y
+// (for Dinv y) precedes its write in every lane.  All loads of a sub-level
+// depend only on its descriptor, so they are in flight together.
+template <class TM, class TV, int KDU, bool BWD>
+__device__ inline void pilu_sweep(int s0, int s1, const int32_t* __restrict__ sub, const int32_t* __restrict__ ord,
+                                  const int32_t* __restrict__ lcs, const TM* __restrict__ val, TV* y) {
+    const int l = threadIdx.x & 63, g = l / 5, r = l - 5 * g;
+    const int j = 12 * (int)(threadIdx.x >> 6) + g;
+    const bool laneOn = g < 12;
+    for (int s = s0; s < s1; ++s) {
+        const int32_t* d = sub + (size_t)kPiluSubW * s;
+        const int rowOff = d[0], nRows = d[1], kmax = d[2];
+        if (laneOn && j < nRows) {
+            const int t = ord[rowOff + j];
+            TM v[KDU][5];
+            int lc[KDU];
+#pragma unroll
+            for (int k = 0; k < KDU; ++k) {
+                if (k < kmax && j < d[4 + kPiluKD + k]) {
+                    const int bk = d[4 + k], nk = d[4 + kPiluKD + k];
+                    lc[k] = lcs[bk + j];
+                    const TM* vb = val + 25 * (size_t)bk + 5 * j + r;
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) v[k][c] = vb[(size_t)c * 5 * nk];
+                }
+            }
+            TV acc = BWD ? TV(0) : y[5 * t + r];
+#pragma unroll
+            for (int k = 0; k < KDU; ++k) {
+                if (k < kmax && j < d[4 + kPiluKD + k]) {
+                    const TV* yv = y + 5 * lc[k];
+                    const TV dot = (TV)(v[k][0] * yv[0] + v[k][1] * yv[1] + v[k][2] * yv[2] + v[k][3] * yv[3] +
+                                        v[k][4] * yv[4]);
+                    if (BWD && k == 0) acc += dot;
+                    else acc -= dot;
+                }
+            }
+            y[5 * t + r] = acc;
+        }
+        __syncthreads();
+    }
+}
+
+// x = M^-1 r for the partitioned factor from its streams; one workgroup per
+// partition, the partition's vector in LDS for both sweeps
+template <class TM, class TV, int KDU>
+__global__ __launch_bounds__(256) void k_pilu_stream(const int32_t* __restrict__ partRowStart,
+                                                     const int32_t* __restrict__ partRows,
+                                                     const int32_t* __restrict__ fStart, const int32_t* __restrict__ fSub,
+                                                     const int32_t* __restrict__ fOrd, const int32_t* __restrict__ fLc,
+                                                     const TM* __restrict__ fVal, const int32_t* __restrict__ bStart,
+                                                     const int32_t* __restrict__ bSub, const int32_t* __restrict__ bOrd,
+                                                     const int32_t* __restrict__ bLc, const TM* __restrict__ bVal,
+                                                     const TV* __restrict__ r, TV* __restrict__ x) {
+    extern __shared__ unsigned char smem_raw[];
+    TV* y = reinterpret_cast<TV*>(smem_raw);
+    const int pt = blockIdx.x;
+    const int rs = partRowStart[pt], re = partRowStart[pt + 1];
+    const int nl5 = (re - rs) * 5;
+    for (int t = threadIdx.x; t < nl5; t += blockDim.x) y[t] = r[5 * (size_t)partRows[rs + t / 5] + t % 5];
+    __syncthreads();
+    pilu_sweep<TM, TV, KDU, false>(fStart[pt], fStart[pt + 1], fSub, fOrd, fLc, fVal, y);
+    pilu_sweep<TM, TV, KDU, true>(bStart[pt], bStart[pt + 1], bSub, bOrd, bLc, bVal, y);
+    for (int t = threadIdx.x; t < nl5; t += blockDim.x) x[5 * (size_t)partRows[rs + t / 5] + t % 5] = y[t];
 }
 
 // The same apply with one thread per block row: measured faster than the
@@ -759,6 +905,30 @@ void build_sell(Ctx& k, const T* A, DBuf<T>& S) {
                        (long)k.m.nnzb, k.m.d_sellOfSlot.p, A, S.p);
 }
 
+// the apply-stream buffers of the factor type
+static DBuf<double>& stream_buf(SolverWork& w, bool bwd, const double*) { return bwd ? w.sb : w.sf; }
+static DBuf<float>& stream_buf(SolverWork& w, bool bwd, const float*) { return bwd ? w.sb32 : w.sf32; }
+static bool streams_built(const Mesh& m) { return m.pf.kmax > 0 && m.pb.kmax > 0; }
+
+template <class T>
+void pilu_pack(Ctx& k, const T* LU, const T* Dinv) {
+    Mesh& m = k.m;
+    for (int bwd = 0; bwd < 2; ++bwd) {
+        PiluStream& ps = bwd ? m.pb : m.pf;
+        DBuf<T>& buf = stream_buf(k.w, bwd != 0, LU);
+        const size_t nv = 25 * (size_t)std::max(1L, ps.nBlk);
+        if (buf.n != nv) buf.alloc(nv);
+        if (ps.nBlk == 0) continue;
+        const int nb = (int)((5 * ps.nBlk + 255) / 256);
+        if (bwd)
+            hipLaunchKernelGGL((k_pilu_pack<T, true>), dim3(nb), dim3(256), 0, k.s, ps.nBlk, ps.d_src.p, ps.d_row.p,
+                               ps.d_jn.p, LU, Dinv, buf.p);
+        else
+            hipLaunchKernelGGL((k_pilu_pack<T, false>), dim3(nb), dim3(256), 0, k.s, ps.nBlk, ps.d_src.p, ps.d_row.p,
+                               ps.d_jn.p, LU, Dinv, buf.p);
+    }
+}
+
 template <class T>
 void ilu_factor(Ctx& k, const T* A, T* LU, T* Dinv) {
     Mesh& m = k.m;
@@ -768,6 +938,7 @@ void ilu_factor(Ctx& k, const T* A, T* LU, T* Dinv) {
     if (k.c.st.precond == RDST_PRECOND_ILU0 && k.c.st.ilu_order == RDST_ILU_PARTITIONED) {
         hipLaunchKernelGGL((k_pilu_factor<T>), dim3(m.nParts), dim3(PILU_FACTOR_THREADS), 0, k.s, m.d_partLevStart.p, m.d_partLevPtr.p,
                            m.d_partRows.p, m.d_rowPtr.p, m.d_cols.p, m.d_slotDiag.p, m.d_localCol.p, LU, Dinv, fail);
+        if (streams_built(m)) pilu_pack<T>(k, LU, Dinv);
     } else if (k.c.st.precond == RDST_PRECOND_ILU0) {
         for (int L = 0; L < m.nLevL; ++L) {
             int cnt = m.levLPtr[L + 1] - m.levLPtr[L];
@@ -800,6 +971,20 @@ void precondition(Ctx& k, const TM* LU, const TM* Dinv, const TV* r, TV* x, TV* 
     }
     if (k.c.st.ilu_order == RDST_ILU_PARTITIONED) {
         const size_t shmem = sizeof(TV) * 5 * (size_t)m.maxPartRows;
+        const DBuf<TM>& sfb = stream_buf(k.w, false, LU);
+        const DBuf<TM>& sbb = stream_buf(k.w, true, LU);
+        if (streams_built(m) && sfb.n > 0 && sbb.n > 0) {
+            const PiluStream &f = m.pf, &b = m.pb;
+            if (std::max(f.kmax, b.kmax) <= 4)
+                hipLaunchKernelGGL((k_pilu_stream<TM, TV, 4>), dim3(m.nParts), dim3(256), shmem, k.s,
+                                   m.d_partRowStart.p, m.d_partRows.p, f.d_subStart.p, f.d_sub.p, f.d_ord.p, f.d_lc.p,
+                                   sfb.p, b.d_subStart.p, b.d_sub.p, b.d_ord.p, b.d_lc.p, sbb.p, r, x);
+            else
+                hipLaunchKernelGGL((k_pilu_stream<TM, TV, kPiluKD>), dim3(m.nParts), dim3(256), shmem, k.s,
+                                   m.d_partRowStart.p, m.d_partRows.p, f.d_subStart.p, f.d_sub.p, f.d_ord.p, f.d_lc.p,
+                                   sfb.p, b.d_subStart.p, b.d_sub.p, b.d_ord.p, b.d_lc.p, sbb.p, r, x);
+            return;
+        }
         if (sizeof(TM) == 4)
             hipLaunchKernelGGL((k_pilu_apply_rows<TM, TV>), dim3(m.nParts), dim3(256), shmem, k.s, m.d_partRowStart.p,
                                m.d_partLevStart.p, m.d_partLevPtr.p, m.d_partRows.p, m.d_rowPtr.p, m.d_cols.p,
@@ -896,7 +1081,7 @@ void gmres_cycle(Ctx& k, const TM* A, const TM* LU, const TM* Dinv, TV* z, const
             for (int j0 = 0; j0 <= i; j0 += 8)
                 hipLaunchKernelGGL((k_multidot_partial<TV, 8>), dim3(G), dim3(RED_BS), 0, k.s, n5, wv, V, j0,
                                    std::min(8, i + 1 - j0), k.w.multi.p);
-            hipLaunchKernelGGL(k_multidot_final, dim3(1), dim3(RED_BS), 0, k.s, k.w.multi.p, G, i + 1, H, m, i);
+            hipLaunchKernelGGL(k_multidot_final, dim3(i + 1), dim3(RED_BS), 0, k.s, k.w.multi.p, G, i + 1, H, m, i);
             hipLaunchKernelGGL((k_cgs_update<TV>), dim3(G), dim3(RED_BS), 0, k.s, n5, wv, V, H, m, i, i + 1, pin);
         } else {
             // partial dot(w, V0), then the fused MGS steps

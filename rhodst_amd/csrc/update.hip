@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "rdst_internal.hpp"
+#include "fluxfun.hpp"
 
 // No floating-point contraction: the reference evaluates every product
 // rounded, so a*x - a*x cancels exactly (a tangential state on both sides of
@@ -73,7 +74,7 @@ __global__ void k_init_cells(int n, Fields F, Thermo th, const double* p, const 
     F.rhoE[i] = rhoE;
     F.W[5 * i] = rho; F.W[5 * i + 1] = rho * ux; F.W[5 * i + 2] = rho * uy; F.W[5 * i + 3] = rho * uz;
     F.W[5 * i + 4] = rhoE;
-    store_cell8(F, i, rho, rho * ux, rho * uy, rho * uz, rhoE, pi, h, psi);
+    store_cell8(F, i, rho, rho * ux, rho * uy, rho * uz, rhoE, pi, h, psi, th.gamma);
     F.rhoOld[i] = rho;                       // thermo rho
     F.eOld[i] = h * th.Cv / th.Cp;
 }
@@ -247,7 +248,7 @@ __global__ void k_update_cells(int n, Fields F, Thermo th, double rhoMin, double
     F.mu[i] = mu_of(th, T); F.alpha[i] = alpha_of(th, T);
     const double p = clampd(rho / psi, pMin, pMax);
     F.p[i] = p;
-    store_cell8(F, i, rho, rux, ruy, ruz, rhoE, p, h, psi);
+    store_cell8(F, i, rho, rux, ruy, ruz, rhoE, p, h, psi, th.gamma);
 }
 
 // kernel 3: boundary h, T, psi, p and the conserved boundary values
