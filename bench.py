@@ -49,16 +49,16 @@ def k1_bytes_viscous(n_cells: int, n_int: int, n_bnd: int) -> float:
     return n_int * (8 + 24 + 8 + 8 + 24) + n_bnd * (4 + 24 + 40 + 16 + 72) + n_cells * (40 + 8 + 72 + 16 + 40)
 
 
-TRAFFIC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                               "pmc_k_flux_residual_swept_wing_2M.md")
+PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
 
 
-def measured_traffic():
-    """HBM bytes per face-flux launch on the default workload, from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (the commands that
-    produce it are in that file); None when it is absent."""
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of the face-flux kernel on the default workload,
+    from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of that
+    kernel (the commands that produce it are in the file); None when it is
+    absent."""
     try:
-        with open(TRAFFIC_SUMMARY) as f:
+        with open(os.path.join(PROFILES, f"pmc_{kernel}_swept_wing_2M.md")) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
                     return int(line.split(":")[1])
@@ -233,6 +233,7 @@ def main(argv=None) -> int:
     flux_ms = dc64.time_flux(args.flux_calls)
     mesh_dev = dc64.mesh
     del dc64
+    log(f"face-flux kernel {flux_ms:.4f} ms")
     log("fp64 path, partitioned block-ILU(0)")
     dc64p, ms64p, res64p = run_path(spec64p, args.steps, args.warmup, mesh_dev)
     log(f"fp64 partitioned {ms64p:.2f} ms/step, |R| {res64p[0]:.4e} -> {res64p[-1]:.4e}")
@@ -245,10 +246,11 @@ def main(argv=None) -> int:
     gfaces = (nI + nB) / (flux_ms * 1e-3) / 1e9
     bytes_k1 = (k1_bytes_viscous if args.turbulence else k1_bytes)(nC, nI, nB)
     achieved = bytes_k1 / (flux_ms * 1e-3) / 1e9
-    traffic = measured_traffic() if not (args.small or args.turbulence) else None
+    kname = "k_flux_residual"
+    traffic = measured_traffic(kname) if not (args.small or args.turbulence) else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_flux_residual<AUSM+up%s>" % (", viscous" if args.turbulence else ""),
+                "kernel": "%s<AUSM+up%s>" % (kname, ", viscous" if args.turbulence else ""),
                 "kernel_ms": round(flux_ms, 5),
                 "algorithmic_bytes": int(bytes_k1)}
 

@@ -166,19 +166,7 @@ typedef struct rdst_settings {
     int32_t gram_schmidt;     /* rdst_gram_schmidt: GMRES orthogonalisation */
     int32_t reconstruction;   /* rdst_reconstruction */
     int32_t jacobian;         /* rdst_jacobian_kind: the implicit operator */
-    int32_t face_loop;        /* rdst_face_loop: organisation of the face-flux kernel (same R up to the
-                                 order of its sums) */
 } rdst_settings;
-
-/* How the face loop of the first-order inviscid residual is organised on the
- * GPU (no counterpart in the reference, whose fvc::div loops over faces).
- * TILES: cells in tiles of 256, a face inside a tile evaluated once and
- * shared through LDS; CELLS: every cell evaluates all of its face sides.
- * Viscous, reconstructed and HLLCP residuals always use the cell loop. */
-enum rdst_face_loop {
-    RDST_FACE_LOOP_TILES = 0,
-    RDST_FACE_LOOP_CELLS = 1
-};
 
 /* The implicit operator of the step.  ROE is the reference's Jacobian
  * (Roe<tensor5>, the only entry of the JacobianDST table, JacobianDST.C:57-92).

@@ -55,8 +55,7 @@ class rdst_settings(C.Structure):
                 ("rho_min", C.c_double), ("rho_max", C.c_double), ("p_min", C.c_double),
                 ("p_max", C.c_double), ("T_min", C.c_double), ("T_max", C.c_double),
                 ("ilu_order", C.c_int32), ("viscous", C.c_int32), ("gram_schmidt", C.c_int32),
-                ("reconstruction", C.c_int32), ("jacobian", C.c_int32),
-                ("face_loop", C.c_int32)]
+                ("reconstruction", C.c_int32), ("jacobian", C.c_int32)]
 
 
 class rdst_solver_perf(C.Structure):
@@ -221,8 +220,6 @@ def make_settings(**kw) -> rdst_settings:
     s.reconstruction = {"upwind": 0, "OSPRE": 1}[rc] if isinstance(rc, str) else int(rc)
     jk = kw.get("jacobian", "Roe")
     s.jacobian = {"Roe": 0, "exact": 1}[jk] if isinstance(jk, str) else int(jk)
-    fl = kw.get("face_loop", "tiles")
-    s.face_loop = {"tiles": 0, "cells": 1}[fl] if isinstance(fl, str) else int(fl)
     return s
 
 

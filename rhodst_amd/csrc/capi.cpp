@@ -235,8 +235,6 @@ int rdst_case_create(rdst_mesh* mesh, const rdst_thermo* thermo, const rdst_bc* 
             (s.reconstruction != RDST_RECON_UPWIND || s.viscous || s.flux_scheme == RDST_FLUX_HLLCP))
             throw ArgError("rdst_case_create: the exact Jacobian covers the first-order inviscid residual "
                            "(upwind reconstruction, viscous 0, not HLLCP)");
-        if (s.face_loop != RDST_FACE_LOOP_TILES && s.face_loop != RDST_FACE_LOOP_CELLS)
-            throw ArgError("rdst_case_create: unknown face loop (valid: tiles cells)", RDST_ERR_UNKNOWN_NAME);
         if (s.ilu_order != RDST_ILU_NATURAL && s.ilu_order != RDST_ILU_PARTITIONED)
             throw ArgError("Unknown ILU ordering; valid: natural partitioned", RDST_ERR_UNKNOWN_NAME);
         if (!(thermo->Cp > thermo->R) || !(thermo->R > 0)) throw ArgError("rdst_case_create: need Cp > R > 0");

@@ -416,95 +416,6 @@ __global__ __launch_bounds__(256) void k_flux_residual(MeshDev m, Fields F, cons
     __builtin_nontemporal_store(r4, F.R + 4 * n + c);
 }
 
-// K1 over flux tiles (first-order inviscid, not HLLCP): block = tile.  A
-// face with both cells in the tile is evaluated once, by the owner's thread,
-// which publishes the flux in LDS; after the barrier the neighbour's thread
-// subtracts it.  The flux of such a face is the owner-thread evaluation of
-// the same (owner, neighbour) sides the neighbour's thread would form, so
-// R differs from the cell-by-cell loop only in the order of its sums.
-struct TilesDev {
-    int nTiles, W, maxShared;
-    const int32_t* cell;
-    const uint32_t* code;
-    const int16_t* share;
-    const int32_t* face;
-    const double* Sf;
-};
-
-template <int S, int STORE>
-__global__ __launch_bounds__(256) void k_flux_tiles(MeshDev m, Fields F, TilesDev T, double gamma, int lowMach,
-                                                    double Mainf) {
-    extern __shared__ double shf[];  // [5][maxShared]
-    const int tile = xcd_block(blockIdx.x, gridDim.x);
-    const size_t NP = (size_t)T.nTiles * kTileCells;
-    const size_t E = (size_t)T.W * NP;
-    const size_t p = (size_t)tile * kTileCells + threadIdx.x;
-    const int MS = T.maxShared;
-    const int c = T.cell[p];
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
-    int kRead = T.W;
-    if (c >= 0) {
-        const Side me = side_of_cell<S>(F, c, gamma);
-        for (int k = 0; k < T.W; ++k) {
-            const size_t e = (size_t)k * NP + p;
-            const uint32_t code = __builtin_nontemporal_load(T.code + e);
-            const uint32_t kind = code >> 30;
-            const int sh = T.share[e];
-            if (kind == ELL_NONE || (kind == ELL_NEIGH && sh >= 0)) {
-                kRead = k;
-                break;
-            }
-            const int idx = (int)(code & ELL_IDX_MASK);
-            const double sx = __builtin_nontemporal_load(T.Sf + e);
-            const double sy = __builtin_nontemporal_load(T.Sf + E + e);
-            const double sz = __builtin_nontemporal_load(T.Sf + 2 * E + e);
-            const double ms = sqrt(sx * sx + sy * sy + sz * sz);
-            Side L, R;
-            if (kind == ELL_BOUNDARY) {
-                L = load_bnd(F, idx, gamma);
-                R = L;
-            } else {
-                const Side other = side_of_cell<S>(F, idx, gamma);
-                if (kind == ELL_OWNER) { L = me; R = other; } else { L = other; R = me; }
-            }
-            const Flux5 o = eval_flux<S>(L, R, sx, sy, sz, ms, gamma, 1.0, lowMach, Mainf);
-            if (kind == ELL_NEIGH) {
-                a0 -= o.phi; a1 -= o.fx; a2 -= o.fy; a3 -= o.fz; a4 -= o.fE;
-            } else {
-                a0 += o.phi; a1 += o.fx; a2 += o.fy; a3 += o.fz; a4 += o.fE;
-                if (sh >= 0) {
-                    shf[sh] = o.phi; shf[MS + sh] = o.fx; shf[2 * MS + sh] = o.fy; shf[3 * MS + sh] = o.fz;
-                    shf[4 * MS + sh] = o.fE;
-                }
-                if (STORE == STORE_ALL) {
-                    const int fi = T.face[e];
-                    F.phi[fi] = o.phi; F.phiUx[fi] = o.fx; F.phiUy[fi] = o.fy; F.phiUz[fi] = o.fz; F.phiE[fi] = o.fE;
-                    F.Upx[fi] = o.upx; F.Upy[fi] = o.upy; F.Upz[fi] = o.upz;
-                } else if (STORE == STORE_PHI) {
-                    F.phi[T.face[e]] = o.phi;
-                } else if (kind == ELL_BOUNDARY) {
-                    F.phi[m.nI + idx] = o.phi;
-                }
-            }
-        }
-    }
-    __syncthreads();
-    if (c < 0) return;
-    for (int k = kRead; k < T.W; ++k) {
-        const size_t e = (size_t)k * NP + p;
-        if ((T.code[e] >> 30) == ELL_NONE) break;
-        const int sh = T.share[e];
-        a0 -= shf[sh]; a1 -= shf[MS + sh]; a2 -= shf[2 * MS + sh]; a3 -= shf[3 * MS + sh]; a4 -= shf[4 * MS + sh];
-    }
-    const double V = m.V[c];
-    const int n = m.nC;
-    __builtin_nontemporal_store(a0 / V, F.R + c);
-    __builtin_nontemporal_store(a1 / V, F.R + n + c);
-    __builtin_nontemporal_store(a2 / V, F.R + 2 * n + c);
-    __builtin_nontemporal_store(a3 / V, F.R + 3 * n + c);
-    __builtin_nontemporal_store(a4 / V, F.R + 4 * n + c);
-}
-
 // HLLCP pressure sensor (HLLCPDST.C:131-164): fCells = min over cellCells of
 // min(p_i/p_j, p_j/p_i), starting from 1; store pow3.
 __global__ void k_hllcp_sensor(MeshDev m, Fields F) {
@@ -562,17 +473,6 @@ void launch_flux_mode(Case& c) {
     const int bs = 256;
     const int nb = blocks_for(md.nC, bs);
     const BcPatchDev* bc = c.d_bc.p;
-    const FluxTiles& ft = c.mesh->tiles;
-    if constexpr (!RECON && S != RDST_FLUX_HLLCP) {
-        if (!c.st.viscous && ft.nTiles > 0 && c.st.face_loop == RDST_FACE_LOOP_TILES) {
-            const TilesDev td{ft.nTiles, ft.W, std::max(ft.maxShared, 1), ft.d_cell.p, ft.d_code.p, ft.d_share.p,
-                              ft.d_face.p, ft.d_Sf.p};
-            const size_t shmem = sizeof(double) * 5 * (size_t)td.maxShared;
-            hipLaunchKernelGGL((k_flux_tiles<S, STORE>), dim3(ft.nTiles), dim3(kTileCells), shmem, c.stream, md, c.f,
-                               td, c.th.gamma, c.st.low_mach, c.st.Mainf);
-            return;
-        }
-    }
     if (c.st.viscous)
         hipLaunchKernelGGL((k_flux_residual<S, STORE, true, RECON>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, bc,
                            c.th.gamma, c.st.low_mach, c.st.Mainf);

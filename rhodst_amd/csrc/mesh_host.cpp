@@ -60,116 +60,6 @@ void rcb(const std::vector<double>& C, std::vector<int>& idx, int lo, int hi, in
 
 }  // namespace
 
-namespace {
-// bisection of idx[lo, hi) into tiles of at most kTileCells cells, cut across
-// the axis with the most cells: (max - min of the centres) / (median cell
-// size along the axis); tiles are numbered depth first, so consecutive tiles
-// are neighbours in space
-void tile_rcb(const std::vector<double>& C, const std::vector<double>& h, std::vector<int>& idx, int lo, int hi,
-              std::vector<int32_t>& tileOf, int& nTiles, std::vector<double>& scratch) {
-    if (hi - lo <= kTileCells) {
-        for (int k = lo; k < hi; ++k) tileOf[idx[k]] = nTiles;
-        ++nTiles;
-        return;
-    }
-    double best = -1.0;
-    int ax = 0;
-    for (int d = 0; d < 3; ++d) {
-        double mn = 1e300, mx = -1e300;
-        scratch.clear();
-        for (int k = lo; k < hi; ++k) {
-            const double x = C[3 * (size_t)idx[k] + d];
-            mn = std::min(mn, x);
-            mx = std::max(mx, x);
-            scratch.push_back(h[3 * (size_t)idx[k] + d]);
-        }
-        std::nth_element(scratch.begin(), scratch.begin() + scratch.size() / 2, scratch.end());
-        const double med = std::max(scratch[scratch.size() / 2], 1e-300);
-        const double cells = (mx - mn) / med;
-        if (cells > best) { best = cells; ax = d; }
-    }
-    const int mid = lo + (hi - lo) / 2;
-    std::nth_element(idx.begin() + lo, idx.begin() + mid, idx.begin() + hi, [&](int a, int b) {
-        const double ca = C[3 * (size_t)a + ax], cb = C[3 * (size_t)b + ax];
-        return ca < cb || (ca == cb && a < b);
-    });
-    tile_rcb(C, h, idx, lo, mid, tileOf, nTiles, scratch);
-    tile_rcb(C, h, idx, mid, hi, tileOf, nTiles, scratch);
-}
-}  // namespace
-
-void build_flux_tiles(Mesh& m) {
-    FluxTiles& T = m.tiles;
-    const int n = m.nC, W = m.ellW;
-    // cell size along each axis: V over half the summed |Sf_d| of its faces
-    std::vector<double> A(3 * (size_t)n, 0.0), h(3 * (size_t)n);
-    for (int f = 0; f < m.nF; ++f)
-        for (int d = 0; d < 3; ++d) {
-            const double a = 0.5 * std::fabs(m.Sf[3 * (size_t)f + d]);
-            A[3 * (size_t)m.owner[f] + d] += a;
-            if (f < m.nI) A[3 * (size_t)m.neigh[f] + d] += a;
-        }
-    for (size_t q = 0; q < h.size(); ++q) h[q] = m.V[q / 3] / std::max(A[q], 1e-300);
-    std::vector<int32_t> tileOf(n, 0);
-    std::vector<int> idx(n);
-    std::iota(idx.begin(), idx.end(), 0);
-    std::vector<double> scratch;
-    int nTiles = 0;
-    tile_rcb(m.C, h, idx, 0, n, tileOf, nTiles, scratch);
-    T.nTiles = nTiles;
-    T.W = W;
-    const size_t NP = (size_t)nTiles * kTileCells;
-    T.cell.assign(NP, -1);
-    std::vector<int32_t> pos(n);
-    {
-        std::vector<int> fill(nTiles, 0);
-        for (int c = 0; c < n; ++c) {  // ascending cell order inside a tile
-            const size_t p = (size_t)tileOf[c] * kTileCells + fill[tileOf[c]]++;
-            T.cell[p] = c;
-            pos[c] = (int32_t)p;
-        }
-    }
-    // LDS slot of each face with both cells in one tile, per tile in face order
-    std::vector<int32_t> slotOf(m.nI, -1), nShared(nTiles, 0);
-    for (int f = 0; f < m.nI; ++f) {
-        const int t = tileOf[m.owner[f]];
-        if (t == tileOf[m.neigh[f]]) slotOf[f] = nShared[t]++;
-    }
-    T.maxShared = 0;
-    for (int t = 0; t < nTiles; ++t) T.maxShared = std::max(T.maxShared, nShared[t]);
-    if (40 * (size_t)T.maxShared > 65536) {  // five FP64 fluxes per shared face must fit 64 KiB of LDS
-        T.nTiles = 0;                         // the face loop stays cell by cell
-        T.cell.clear();
-        return;
-    }
-    const size_t E = (size_t)W * NP;
-    T.code.assign(E, (uint32_t)ELL_NONE << 30);
-    T.share.assign(E, -1);
-    T.face.assign(E, -1);
-    T.Sf.assign(3 * E, 0.0);
-    for (int c = 0; c < n; ++c) {
-        const size_t p = (size_t)pos[c];
-        int k = 0;
-        for (int pass = 0; pass < 2; ++pass)  // pass 1: the shared neighbour-side entries, last
-            for (int s = 0; s < W; ++s) {
-                const size_t e = (size_t)s * n + c;
-                const uint32_t code = m.ellCode[e];
-                const uint32_t kind = code >> 30;
-                if (kind == ELL_NONE) break;
-                const int fi = m.ellFace[e];
-                const int sh = (kind != ELL_BOUNDARY) ? slotOf[fi] : -1;
-                const bool readSide = kind == ELL_NEIGH && sh >= 0;
-                if (readSide != (pass == 1)) continue;
-                const size_t o = (size_t)k * NP + p;
-                T.code[o] = code;
-                T.share[o] = (int16_t)sh;
-                T.face[o] = fi;
-                for (int d = 0; d < 3; ++d) T.Sf[(size_t)d * E + o] = m.Sf[3 * (size_t)fi + d];
-                ++k;
-            }
-    }
-}
-
 // The forward (L) or backward (Dinv, Dinv U) stream of the partitioned
 // ILU(0) apply, in the layout PiluStream describes.  Left empty (kmax = 0)
 // when a row has more than kPiluKD blocks in the sweep.
@@ -582,7 +472,6 @@ void mesh_build(Mesh& m, const rdst_mesh_desc* d) {
     bucket(m.levL, m.nLevL, m.levLPtr, m.levLRows);
     bucket(m.levU, m.nLevU, m.levUPtr, m.levURows);
     build_partitions(m, kPartCells);
-    build_flux_tiles(m);
     // sliced ELL (64 rows per slice)
     m.nSlices = (nCells + 63) / 64;
     m.sellPtr.assign(m.nSlices + 1, 0);
@@ -669,11 +558,6 @@ void mesh_upload(Mesh& m) {
     m.d_partLevStart.upload(m.partLevStart);
     m.d_partLevPtr.upload(m.partLevPtr);
     m.d_localCol.upload(m.localCol);
-    m.tiles.d_cell.upload(m.tiles.cell);
-    m.tiles.d_code.upload(m.tiles.code);
-    m.tiles.d_share.upload(m.tiles.share);
-    m.tiles.d_face.upload(m.tiles.face);
-    m.tiles.d_Sf.upload(m.tiles.Sf);
     for (PiluStream* ps : {&m.pf, &m.pb}) {
         if (ps->kmax == 0) continue;
         ps->d_subStart.upload(ps->subStart);

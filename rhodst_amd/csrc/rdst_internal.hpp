@@ -90,7 +90,7 @@ static constexpr uint32_t ELL_IDX_MASK = 0x3fffffffu;
 // block, so x = Dinv y - sum (Dinv U) x needs no separate diagonal solve.
 constexpr int kPiluKD = 8;                  // most blocks of a row in one sweep
 constexpr int kPiluSubW = 4 + 2 * kPiluKD;  // ints per sub-level descriptor
-constexpr int kPiluSubRows = 48;            // 4 wavefronts x 12 rows of 5 lanes
+constexpr int kPiluSubRows = 24;            // 2 wavefronts x 12 rows of 5 lanes
 struct PiluStream {
     int kmax = 0;                  // largest row block count (0: not built)
     long nBlk = 0;
@@ -102,31 +102,6 @@ struct PiluStream {
     std::vector<int32_t> row;       // [nBlk] cell of the block's row
     std::vector<int32_t> jn;        // [nBlk] j | n_k << 8
     DBuf<int32_t> d_subStart, d_sub, d_ord, d_lc, d_src, d_row, d_jn;
-};
-
-// Tiles of the first-order inviscid face loop (k_flux_tiles): cells cut into
-// tiles of at most kTileCells by recursive bisection of the cell centres,
-// each cut across the axis with the most cells (extent over the median cell
-// size along it), so a tile is compact in cell counts and most faces have
-// both cells in one tile.  Such a face is evaluated once, by its owner's
-// thread, and handed to the neighbour's thread through LDS; every other face
-// side is evaluated by its own cell as in the ELL loop.  Positions p =
-// kTileCells * tile + lane; per position and slot k (layout [W][NP]):
-// code (ELL code word), share (LDS slot of a shared face, -1 otherwise),
-// face (surface-field index), Sf [3][W][NP].  In each cell's list the
-// shared neighbour-side entries come last.
-constexpr int kTileCells = 256;
-struct FluxTiles {
-    int nTiles = 0, W = 0, maxShared = 0;
-    std::vector<int32_t> cell;   // [NP] cell at each position, -1 for padding
-    std::vector<uint32_t> code;  // [W][NP]
-    std::vector<int16_t> share;  // [W][NP]
-    std::vector<int32_t> face;   // [W][NP]
-    std::vector<double> Sf;      // [3][W][NP]
-    DBuf<int32_t> d_cell, d_face;
-    DBuf<uint32_t> d_code;
-    DBuf<int16_t> d_share;
-    DBuf<double> d_Sf;
 };
 
 struct Mesh {
@@ -164,7 +139,6 @@ struct Mesh {
     // coalesced streams of the partitioned ILU(0) apply (forward: the
     // in-partition L blocks; backward: Dinv and Dinv U): see PiluStream
     PiluStream pf, pb;
-    FluxTiles tiles;
     // sliced-ELL copy of the block-CSR pattern for the SpMV: slices of 64
     // rows, slice s holds sellWidth[s] block columns; block p of row r sits at
     // block position sellPtr[s] + 64 p + (r - 64 s), its entry e at value
@@ -199,7 +173,6 @@ struct Mesh {
 constexpr int kPartCells = 1024;
 void mesh_build(Mesh& m, const rdst_mesh_desc* d);
 void build_partitions(Mesh& m, int maxCells);
-void build_flux_tiles(Mesh& m);
 void mesh_upload(Mesh& m);
 
 // ---------------------------------------------------------------------------
