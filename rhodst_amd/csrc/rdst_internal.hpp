@@ -90,7 +90,7 @@ static constexpr uint32_t ELL_IDX_MASK = 0x3fffffffu;
 // block, so x = Dinv y - sum (Dinv U) x needs no separate diagonal solve.
 constexpr int kPiluKD = 8;                  // most blocks of a row in one sweep
 constexpr int kPiluSubW = 4 + 2 * kPiluKD;  // ints per sub-level descriptor
-constexpr int kPiluSubRows = 24;            // 2 wavefronts x 12 rows of 5 lanes
+constexpr int kPiluSubRows = 48;            // 4 wavefronts x 12 rows of 5 lanes
 struct PiluStream {
     int kmax = 0;                  // largest row block count (0: not built)
     long nBlk = 0;

@@ -552,93 +552,59 @@ __global__ __launch_bounds__(256) void k_pilu_pack(long nBlk, const int32_t* src
 // One sweep over the sub-levels [s0, s1) of a partition (y in LDS).  Lane
 // layout: 12 rows of five lanes per wavefront (lanes 60..63 idle), so the
 // lanes of a row share a wavefront and the backward row's read of its own y
-// (for Dinv y) precedes its write in every lane.  The loads of a sub-level
-// depend only on its descriptor, never on y, so they are software-pipelined:
-// while sub-level s is computed, the loads of s+1 .. s+PD-1 are in flight in
-// PD register stages.
-template <class TM, int KDU>
-struct PiluStage {
-    TM v[KDU][5];
-    int lc[KDU];
-    int t, nRows, kmax;
-    int nk[KDU];
-};
-
-template <class TM, int KDU>
-__device__ inline void pilu_load(PiluStage<TM, KDU>& st, int s, int j, int r, const int32_t* __restrict__ sub,
-                                 const int32_t* __restrict__ ord, const int32_t* __restrict__ lcs,
-                                 const TM* __restrict__ val) {
-    const int32_t* d = sub + (size_t)kPiluSubW * s;
-    st.nRows = d[1];
-    st.kmax = d[2];
-#pragma unroll
-    for (int k = 0; k < KDU; ++k) st.nk[k] = k < st.kmax ? d[4 + kPiluKD + k] : 0;
-    if (j < st.nRows) {
-        st.t = ord[d[0] + j];
-#pragma unroll
-        for (int k = 0; k < KDU; ++k) {
-            if (j < st.nk[k]) {
-                const int bk = d[4 + k];
-                st.lc[k] = lcs[bk + j];
-                const TM* vb = val + 25 * (size_t)bk + 5 * j + r;
-#pragma unroll
-                for (int c = 0; c < 5; ++c) st.v[k][c] = vb[(size_t)c * 5 * st.nk[k]];
-            }
-        }
-    }
-}
-
+// (for Dinv y) precedes its write in every lane.  All loads of a sub-level
+// depend only on its descriptor, so they are in flight together.
 template <class TM, class TV, int KDU, bool BWD>
-__device__ inline void pilu_compute(const PiluStage<TM, KDU>& st, int j, int r, TV* y) {
-    if (j < st.nRows) {
-        TV acc = BWD ? TV(0) : y[5 * st.t + r];
-#pragma unroll
-        for (int k = 0; k < KDU; ++k) {
-            if (j < st.nk[k]) {
-                const TV* yv = y + 5 * st.lc[k];
-                const TV dot = (TV)(st.v[k][0] * yv[0] + st.v[k][1] * yv[1] + st.v[k][2] * yv[2] +
-                                    st.v[k][3] * yv[3] + st.v[k][4] * yv[4]);
-                if (BWD && k == 0) acc += dot;
-                else acc -= dot;
-            }
-        }
-        y[5 * st.t + r] = acc;
-    }
-}
-
-template <class TM, class TV, int KDU, int PD, bool BWD>
 __device__ inline void pilu_sweep(int s0, int s1, const int32_t* __restrict__ sub, const int32_t* __restrict__ ord,
                                   const int32_t* __restrict__ lcs, const TM* __restrict__ val, TV* y) {
     const int l = threadIdx.x & 63, g = l / 5, r = l - 5 * g;
-    // lanes 60..63 take a row index past every sub-level, so they load and
-    // compute nothing
-    const int j = g < 12 ? 12 * (int)(threadIdx.x >> 6) + g : kPiluSubRows;
-    PiluStage<TM, KDU> st[PD];
+    const int j = 12 * (int)(threadIdx.x >> 6) + g;
+    const bool laneOn = g < 12;
+    for (int s = s0; s < s1; ++s) {
+        const int32_t* d = sub + (size_t)kPiluSubW * s;
+        const int rowOff = d[0], nRows = d[1], kmax = d[2];
+        if (laneOn && j < nRows) {
+            const int t = ord[rowOff + j];
+            TM v[KDU][5];
+            int lc[KDU];
 #pragma unroll
-    for (int q = 0; q < PD - 1; ++q)
-        if (s0 + q < s1) pilu_load(st[q], s0 + q, j, r, sub, ord, lcs, val);
-    for (int s = s0; s < s1; s += PD) {
+            for (int k = 0; k < KDU; ++k) {
+                if (k < kmax && j < d[4 + kPiluKD + k]) {
+                    const int bk = d[4 + k], nk = d[4 + kPiluKD + k];
+                    lc[k] = lcs[bk + j];
+                    const TM* vb = val + 25 * (size_t)bk + 5 * j + r;
 #pragma unroll
-        for (int q = 0; q < PD; ++q) {
-            const int sc = s + q;
-            if (sc >= s1) break;
-            if (sc + PD - 1 < s1) pilu_load(st[(q + PD - 1) % PD], sc + PD - 1, j, r, sub, ord, lcs, val);
-            pilu_compute<TM, TV, KDU, BWD>(st[q], j, r, y);
-            __syncthreads();
+                    for (int c = 0; c < 5; ++c) v[k][c] = vb[(size_t)c * 5 * nk];
+                }
+            }
+            TV acc = BWD ? TV(0) : y[5 * t + r];
+#pragma unroll
+            for (int k = 0; k < KDU; ++k) {
+                if (k < kmax && j < d[4 + kPiluKD + k]) {
+                    const TV* yv = y + 5 * lc[k];
+                    const TV dot = (TV)(v[k][0] * yv[0] + v[k][1] * yv[1] + v[k][2] * yv[2] + v[k][3] * yv[3] +
+                                        v[k][4] * yv[4]);
+                    if (BWD && k == 0) acc += dot;
+                    else acc -= dot;
+                }
+            }
+            y[5 * t + r] = acc;
         }
+        __syncthreads();
     }
 }
 
-// x = M^-1 r for the partitioned factor from its streams; one workgroup of
-// kPiluThreads per partition, the partition's vector in LDS for both sweeps
-constexpr int kPiluThreads = 64 * (kPiluSubRows / 12);
-template <class TM, class TV, int KDU, int PD>
-__global__ __launch_bounds__(kPiluThreads) void k_pilu_stream(
-    const int32_t* __restrict__ partRowStart, const int32_t* __restrict__ partRows, const int32_t* __restrict__ fStart,
-    const int32_t* __restrict__ fSub, const int32_t* __restrict__ fOrd, const int32_t* __restrict__ fLc,
-    const TM* __restrict__ fVal, const int32_t* __restrict__ bStart, const int32_t* __restrict__ bSub,
-    const int32_t* __restrict__ bOrd, const int32_t* __restrict__ bLc, const TM* __restrict__ bVal,
-    const TV* __restrict__ r, TV* __restrict__ x) {
+// x = M^-1 r for the partitioned factor from its streams; one workgroup per
+// partition, the partition's vector in LDS for both sweeps
+template <class TM, class TV, int KDU>
+__global__ __launch_bounds__(256) void k_pilu_stream(const int32_t* __restrict__ partRowStart,
+                                                     const int32_t* __restrict__ partRows,
+                                                     const int32_t* __restrict__ fStart, const int32_t* __restrict__ fSub,
+                                                     const int32_t* __restrict__ fOrd, const int32_t* __restrict__ fLc,
+                                                     const TM* __restrict__ fVal, const int32_t* __restrict__ bStart,
+                                                     const int32_t* __restrict__ bSub, const int32_t* __restrict__ bOrd,
+                                                     const int32_t* __restrict__ bLc, const TM* __restrict__ bVal,
+                                                     const TV* __restrict__ r, TV* __restrict__ x) {
     extern __shared__ unsigned char smem_raw[];
     TV* y = reinterpret_cast<TV*>(smem_raw);
     const int pt = blockIdx.x;
@@ -646,8 +612,8 @@ __global__ __launch_bounds__(kPiluThreads) void k_pilu_stream(
     const int nl5 = (re - rs) * 5;
     for (int t = threadIdx.x; t < nl5; t += blockDim.x) y[t] = r[5 * (size_t)partRows[rs + t / 5] + t % 5];
     __syncthreads();
-    pilu_sweep<TM, TV, KDU, PD, false>(fStart[pt], fStart[pt + 1], fSub, fOrd, fLc, fVal, y);
-    pilu_sweep<TM, TV, KDU, PD, true>(bStart[pt], bStart[pt + 1], bSub, bOrd, bLc, bVal, y);
+    pilu_sweep<TM, TV, KDU, false>(fStart[pt], fStart[pt + 1], fSub, fOrd, fLc, fVal, y);
+    pilu_sweep<TM, TV, KDU, true>(bStart[pt], bStart[pt + 1], bSub, bOrd, bLc, bVal, y);
     for (int t = threadIdx.x; t < nl5; t += blockDim.x) x[5 * (size_t)partRows[rs + t / 5] + t % 5] = y[t];
 }
 
@@ -1010,13 +976,13 @@ void precondition(Ctx& k, const TM* LU, const TM* Dinv, const TV* r, TV* x, TV* 
         if (streams_built(m) && sfb.n > 0 && sbb.n > 0) {
             const PiluStream &f = m.pf, &b = m.pb;
             if (std::max(f.kmax, b.kmax) <= 4)
-                hipLaunchKernelGGL((k_pilu_stream<TM, TV, 4, 3>), dim3(m.nParts), dim3(kPiluThreads), shmem, k.s,
+                hipLaunchKernelGGL((k_pilu_stream<TM, TV, 4>), dim3(m.nParts), dim3(256), shmem, k.s,
                                    m.d_partRowStart.p, m.d_partRows.p, f.d_subStart.p, f.d_sub.p, f.d_ord.p, f.d_lc.p,
                                    sfb.p, b.d_subStart.p, b.d_sub.p, b.d_ord.p, b.d_lc.p, sbb.p, r, x);
             else
-                hipLaunchKernelGGL((k_pilu_stream<TM, TV, kPiluKD, 2>), dim3(m.nParts), dim3(kPiluThreads), shmem,
-                                   k.s, m.d_partRowStart.p, m.d_partRows.p, f.d_subStart.p, f.d_sub.p, f.d_ord.p,
-                                   f.d_lc.p, sfb.p, b.d_subStart.p, b.d_sub.p, b.d_ord.p, b.d_lc.p, sbb.p, r, x);
+                hipLaunchKernelGGL((k_pilu_stream<TM, TV, kPiluKD>), dim3(m.nParts), dim3(256), shmem, k.s,
+                                   m.d_partRowStart.p, m.d_partRows.p, f.d_subStart.p, f.d_sub.p, f.d_ord.p, f.d_lc.p,
+                                   sfb.p, b.d_subStart.p, b.d_sub.p, b.d_ord.p, b.d_lc.p, sbb.p, r, x);
             return;
         }
         if (sizeof(TM) == 4)
