@@ -166,7 +166,26 @@ typedef struct rdst_settings {
     int32_t gram_schmidt;     /* rdst_gram_schmidt: GMRES orthogonalisation */
     int32_t reconstruction;   /* rdst_reconstruction */
     int32_t jacobian;         /* rdst_jacobian_kind: the implicit operator */
+    int32_t time_integration; /* rdst_time_integration: timeIntegration.timeIntegration (IOdictDST.C:206-209) */
+    int32_t adjust_time_step; /* timeControls.adjustTimeStep, default 1 (IOdictDST.C:137-140) */
+    int32_t max_inner_iter;   /* timeIntegration.maxInnerIter, default 20 (IOdictDST.C:222-225) */
+    int32_t iteration_start;  /* residualControls.iterationStart, default 10 when unsteady (IOdictDST.C:202-210) */
+    double mag_converged_slope; /* residualControls.magConvergedSlope, default 0.1 (IOdictDST.C:212-215) */
+    double min_residual;      /* residualControls.minResudual, default 1e-5 (IOdictDST.C:217-220) */
+    double delta_t;           /* controlDict deltaT: the deltaT0 of the first physical step */
 } rdst_settings;
+
+/* Time integration of the case (timeIntegration.timeIntegration).  SINGLE:
+ * the steady pseudo-time step Single::iterate (steadyStateDST ddt, local time
+ * stepping).  DUAL: dual time stepping Dual::iterate (Dual.C:80-124): each
+ * physical step of dual2ndDST (second-order backward differences in physical
+ * time, dual2ndDST.C:377-411) is converged by inner pseudo-time iterations
+ * until steadyCheck (steadyCheck.C:69-98) declares them converged; the
+ * physical time step follows setTransientDeltaT (timeDST.C:275-314). */
+enum rdst_time_integration {
+    RDST_TIME_SINGLE = 0,
+    RDST_TIME_DUAL = 1
+};
 
 /* The implicit operator of the step.  ROE is the reference's Jacobian
  * (Roe<tensor5>, the only entry of the JacobianDST table, JacobianDST.C:57-92).
@@ -344,6 +363,21 @@ int rdst_set_courant(rdst_case* c, double* res_norm);
 /* Single::iterate, n_steps times; res_norm[n_steps] receives ||R||_2 of
  * each step and perf[n_steps] (may be NULL) the solver performance */
 int rdst_iterate(rdst_case* c, int n_steps, double* res_norm, rdst_solver_perf* perf);
+
+/* One physical time step of a DUAL case: runTime++ (rhoDST.C:73: time +=
+ * deltaT, deltaT0 = the previous deltaT, the old time levels of W shifted),
+ * then Dual::iterate (Dual.C:80-124): inner iterations of flux, dual2ndDST +
+ * flux Jacobian assembly, block solve, updateFields and the pseudo-time
+ * setCourantDST(&solverPerf), until steadyCheck converges; then
+ * setCourantDST() with |R|_2, which sets the next physical deltaT when
+ * adjust_time_step is on.  n_inner receives the number of inner iterations,
+ * inner_residuals (NULL or max_inner_iter + 1 entries) the mean of the five
+ * initial residuals of each inner solve, res_norm (NULL or one value) the
+ * |R|_2 of the last setCourantDST.  RDST_ERR_STATE for a SINGLE case. */
+int rdst_dual_step(rdst_case* c, int32_t* n_inner, double* inner_residuals, double* res_norm);
+
+/* time, deltaT, deltaT0 and the physical courant number of the case */
+int rdst_time_info(const rdst_case* c, double* out4);
 
 /* Spalart-Allmaras turbulence on the case: model coefficients, the 0/nuTilda
  * boundary conditions [n_patches], the wall distance d [n_cells] (wallDist;

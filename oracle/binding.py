@@ -45,6 +45,9 @@ def lib():
         L.orc_block_solve.argtypes = [C.c_void_p, P(rdst_solver_perf)]
         L.orc_set_courant.argtypes = [C.c_void_p, P(C.c_double)]
         L.orc_iterate.argtypes = [C.c_void_p, C.c_int, P(C.c_double), P(rdst_solver_perf)]
+        L.orc_dual_step.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_double), P(C.c_double)]
+        L.orc_time_info.argtypes = [C.c_void_p, P(C.c_double)]
+        L.orc_steady_check.argtypes = [P(C.c_double), C.c_int, P(rdst_settings), P(C.c_int32), P(C.c_double)]
         L.orc_get_matrix.argtypes = [C.c_void_p, C.c_int, P(C.c_double)]
         L.orc_amul.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
         L.orc_ilu_apply.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
@@ -208,6 +211,20 @@ class OracleCase:
         _check(lib().orc_iterate(self.h, n_steps, dptr(res), perfs))
         return (res, perfs) if with_perf else res
 
+    def dual_step(self):
+        """One physical time step of a Dual case: (inner iterations, their
+        mean initial residuals, |R|_2 of the closing setCourantDST)."""
+        k = C.c_int32()
+        inner = np.zeros(self.spec.settings.max_inner_iter + 1)
+        r = C.c_double()
+        _check(lib().orc_dual_step(self.h, C.byref(k), dptr(inner), C.byref(r)))
+        return k.value, inner[:k.value].copy(), r.value
+
+    def time_info(self):
+        out = np.zeros(4)
+        _check(lib().orc_time_info(self.h, dptr(out)))
+        return {"time": out[0], "deltaT": out[1], "deltaT0": out[2], "courant": out[3]}
+
     def matrix(self, which):
         sizes = {0: (self.n, 25), 1: (self.nI, 25), 2: (self.nI, 25), 3: (self.n, 5), 4: (self.nI, 1),
                  5: (self.nI, 25), 6: (self.nI, 25), 7: (self.nB, 25)}
@@ -243,3 +260,13 @@ class OracleCase:
         u = np.ascontiguousarray(upper, dtype=np.float64)
         l_ = np.ascontiguousarray(lower, dtype=np.float64)
         _check(lib().orc_set_matrix(self.h, dptr(d), dptr(u), dptr(l_)))
+
+
+def steady_check(residuals, settings):
+    """steadyCheck on a residual history (each entry the inner residual of one
+    iteration): (1-based iteration of first convergence or 0, scaled slope)."""
+    r = np.ascontiguousarray(residuals, dtype=np.float64)
+    at = C.c_int32()
+    slope = C.c_double()
+    _check(lib().orc_steady_check(dptr(r), len(r), C.byref(settings), C.byref(at), C.byref(slope)))
+    return at.value, slope.value

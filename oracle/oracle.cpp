@@ -288,6 +288,10 @@ struct Case {
     // time control
     std::vector<double> rDeltaT, courant, rhoOld, eOld, dS;  // dS [n][3]
     double residual = -1, residualRatio = 1, courantScalar = 1;
+    // dual time stepping: old time levels of W [n][5], physical time control
+    std::vector<double> WOld, WOldOld;
+    double time = 0, deltaT = 0, deltaT0 = 0, deltaTSave = 0;
+    int timeIndex = 0;
     bool hasState = false, hasMatrix = false;
     // cell partition for the block-Jacobi ILU(0) (empty: one partition)
     std::vector<int> part;
@@ -356,6 +360,7 @@ static inline V3 vectorVIC(int type, double vf, V3 n) {
 // ------------------------------------------------------------------ set state
 // createFields.H:21-72 and compressibleCreatePhi; timeDST::initialize (timeDST.C:67-208)
 static void setSteadyStateDeltaT(Case& c);
+static void setTransientDeltaT(Case& c);
 
 static void initialBoundary(Case& c) {
     const Mesh& m = c.m;
@@ -465,6 +470,14 @@ static void setState(Case& c, const double* p, const double* T, const double* U)
     c.courantScalar = c.st.initial_co;
     c.courant.assign(n, c.st.initial_co);
     c.rDeltaT.assign(n, 0.0);
+    c.time = 0; c.timeIndex = 0;
+    c.WOld.clear(); c.WOldOld.clear();
+    if (c.st.time_integration == RDST_TIME_DUAL) {
+        // timeDST::initialize (timeDST.C:193-207): the physical deltaT from the
+        // initial courant, then the pseudo time step
+        setTransientDeltaT(c);
+        c.deltaT0 = c.deltaTSave = c.st.delta_t > 0 ? c.st.delta_t : c.deltaT;
+    }
     setSteadyStateDeltaT(c);
     c.hasState = true; c.hasMatrix = false;
 }
@@ -1200,16 +1213,41 @@ static void computeJacobian(Case& c) {
 // ------------------------------------------------------------------ assembly
 // fvmDST::ddt (fvmDST.C:77-143) with steadyStateDST::fvmDdt (steadyStateDST.C:173-198)
 // + fluxDST::integrate (fluxDST.C:73-127), summed by operator+ (fvBlockMatrixDST.C:201-213)
+// dual2ndDST coefficients (dual2ndDST.C:395-401): a = 1/deltaT + 1/(deltaT +
+// deltaT0), c = deltaT/(deltaT0 (deltaT + deltaT0)), b = -(a + c)
+static void dualCoeffs(const Case& c, double& a, double& b, double& cc) {
+    double rDeltaT = 1.0 / c.deltaT;
+    double deltaT = c.deltaT, deltaT0 = c.deltaT0;
+    a = rDeltaT + 1.0 / (deltaT + deltaT0);
+    cc = deltaT / (deltaT0 * (deltaT + deltaT0));
+    b = -(a + cc);
+}
+
 static void assemble(Case& c) {
     const Mesh& m = c.m;
     int n = m.nCells, F = m.nInternal;
     double beta = c.st.beta;
     std::vector<double> dd(25 * n, 0.0), sd(5 * n, 0.0);
+    const bool dual = c.st.time_integration == RDST_TIME_DUAL;
+    double da = 0, db = 0, dcf = 0;
+    if (dual) dualCoeffs(c, da, db, dcf);
     for (int i = 0; i < n; ++i) {
+        double x[5] = {c.rho[i], c.rhoU[i].x, c.rhoU[i].y, c.rhoU[i].z, c.rhoE[i]};
+        if (dual) {
+            // dual2ndDST::fvmDdt (dual2ndDST.C:377-411): diag = (rDeltaTau + a) V,
+            // source = V (rDeltaTau psi - b psi.oldTime() - c psi.oldTime().oldTime());
+            // the block keeps source - diag*psi (fvmDST.C:101-133)
+            double diagS = (c.rDeltaT[i] + da) * m.V[i];
+            for (int r = 0; r < 5; ++r) dd[25 * i + 6 * r] = diagS;
+            for (int k = 0; k < 5; ++k) {
+                double src = m.V[i] * (c.rDeltaT[i] * x[k] - db * c.WOld[5 * i + k] - dcf * c.WOldOld[5 * i + k]);
+                sd[5 * i + k] = src - diagS * x[k];
+            }
+            continue;
+        }
         double diagS = c.rDeltaT[i] * m.V[i];
         for (int r = 0; r < 5; ++r) dd[25 * i + 6 * r] = diagS;  // I * rhoEqn.diag()
         // source - diag*psi with source = rDeltaT*psi_old*V and psi_old = psi
-        double x[5] = {c.rho[i], c.rhoU[i].x, c.rhoU[i].y, c.rhoU[i].z, c.rhoE[i]};
         for (int k = 0; k < 5; ++k) sd[5 * i + k] = c.rDeltaT[i] * x[k] * m.V[i] - diagS * x[k];
     }
     // flux matrix
@@ -1791,18 +1829,66 @@ static void setSteadyStateDeltaT(Case& c) {
     for (int i = 0; i < n; ++i) c.rDeltaT[i] /= c.courant[i];
 }
 
-// setCourantDST (timeDST.C:316-443), steady branch
-static double setCourant(Case& c) {
+// setTransientDeltaT (timeDST.C:275-314): the physical deltaT, the minimum over
+// the internal faces of |Sf| / ((|U_f . Sf| + |Sf| c_f) deltaCoeffs) courant
+// (neighbouring) or over the cells of courant V / ((|U| + c) . dS)
+static void setTransientDeltaT(Case& c) {
     const Mesh& m = c.m;
     int n = m.nCells;
-    double ss = 0;
-    for (int i = 0; i < n; ++i) {
-        double mg = mag5(&c.R[5 * i]);
-        ss += mg * mg;
+    double dt = 1e300;
+    if (!c.st.neighbouring) {
+        for (int i = 0; i < n; ++i) {
+            double cc = std::sqrt(c.th.Cp / c.Cv / c.psi[i]);
+            double lam = (std::fabs(c.U[i].x) + cc) * c.dS[3 * i] + (std::fabs(c.U[i].y) + cc) * c.dS[3 * i + 1]
+                       + (std::fabs(c.U[i].z) + cc) * c.dS[3 * i + 2];
+            dt = std::min(dt, c.courantScalar * m.V[i] / lam);
+        }
+    } else {
+        double gam = c.th.Cp / c.Cv;
+        std::vector<V3> Uf;
+        linearInterpV(m, c.U, c.bU, Uf);
+        std::vector<double> cc(n), bcc(m.nB), cf;
+        for (int i = 0; i < n; ++i) cc[i] = std::sqrt(gam / c.psi[i]);
+        for (int b = 0; b < m.nB; ++b) bcc[b] = std::sqrt(gam / c.bpsi[b]);
+        linearInterp(m, cc, bcc, cf);
+        for (int f = 0; f < m.nInternal; ++f) {
+            double amax = std::fabs(dot(Uf[f], m.Sf[f])) + m.magSf[f] * cf[f];
+            double dtSf = m.magSf[f] / (amax * m.dc[f]);
+            dt = std::min(dt, dtSf * c.courantScalar);
+        }
     }
-    double residualR = std::sqrt(ss);
+    c.deltaT = dt;
+}
+
+// setCourantDST (timeDST.C:316-443).  residualR: |solverPerf.initialResidual()|
+// when lin is given (the inner iterations of Dual), else |R|_2.  pseudo: the
+// pseudo-time branch (steady state, or dual time while the inner iterations
+// are not converged): ramp of the per-cell courant and setSteadyStateDeltaT;
+// otherwise the physical branch: with adjustTimeStep the scalar courant
+// follows the residual ratio and setTransientDeltaT sets the next deltaT.
+static double setCourantDST(Case& c, const double* lin, bool pseudo) {
+    const Mesh& m = c.m;
+    int n = m.nCells;
+    double residualR;
+    if (lin) {
+        residualR = mag5(lin);
+    } else {
+        double ss = 0;
+        for (int i = 0; i < n; ++i) {
+            double mg = mag5(&c.R[5 * i]);
+            ss += mg * mg;
+        }
+        residualR = std::sqrt(ss);
+    }
     if (c.residual != -1) c.residualRatio = std::min(std::max(c.residual / residualR, 0.1), 2.0);
     c.residual = residualR;
+    if (!pseudo) {
+        if (c.st.adjust_time_step) {
+            c.courantScalar = std::min(std::max(c.courantScalar * c.residualRatio, c.st.min_co), c.st.max_co);
+            setTransientDeltaT(c);
+        }
+        return residualR;
+    }
     std::vector<double> eNew(n), factor(n, 1.0);
     for (int i = 0; i < n; ++i) eNew[i] = c.rhoE[i] / c.rho[i] - 0.5 * magSqr(c.rhoU[i] / c.rho[i]);
     for (int f = 0; f < m.nInternal; ++f) {
@@ -1826,6 +1912,8 @@ static double setCourant(Case& c) {
     return residualR;
 }
 
+static double setCourant(Case& c) { return setCourantDST(c, nullptr, true); }
+
 // Single::iterate (Single.C:80-104)
 static void saCorrect(Case& c, Perf* perf);
 static double iterate(Case& c, Perf* perfOut) {
@@ -1837,6 +1925,93 @@ static double iterate(Case& c, Perf* perfOut) {
     updateFields(c);
     if (c.tb.on) saCorrect(c, nullptr);
     return setCourant(c);
+}
+
+// residualControls + steadyCheck (residualControls.C:37-64, steadyCheck.C:37-98):
+// the inner residual of Dual is the mean of the five initial residuals of
+// the inner solve; converged when more than maxInnerIter inner iterations
+// ran, the residual fell below minResudual, or (from iterationStart on) the
+// scaled slope of log(residual) over the last 20 % of the history fell below
+// magConvergedSlope.  converged() refreshes the slope; the const form (used
+// by setCourantDST) reads the last one.
+struct SteadyCheck {
+    std::vector<double> residuals;
+    int iterationCount = 0, iterationStart = 10, maxInnerIter = 20;
+    double residual = 1, magScaledSlope = 1, magConvergedSlope = 0.1, minResidual = 1e-5;
+    void addResidual(const double* res) {
+        double meanResidual = 0;
+        for (int i = 0; i < 5; ++i) meanResidual += res[i];
+        meanResidual /= 5;
+        iterationCount++;
+        residual = meanResidual;
+        residuals.push_back(meanResidual);
+    }
+    bool convergedConst() const {
+        return iterationCount > maxInnerIter || (residual < minResidual || magScaledSlope < magConvergedSlope);
+    }
+    bool converged() {
+        if (iterationCount >= iterationStart) {
+            double sumXi = 0, sumYi = 0, sumXiSqr = 0, sumYiXi = 0;
+            int size = (int)residuals.size();
+            int N = 0.2 * size;
+            double maxY = std::log(*std::max_element(residuals.begin(), residuals.end()));
+            double minY = std::log(*std::min_element(residuals.begin(), residuals.end()));
+            double deltaY = maxY - minY;
+            for (int i = 0; i < N; i++) {
+                int xi = size - 1 - i;
+                double yi = std::log(residuals[xi]);
+                sumXi += xi;
+                sumXiSqr += std::pow(xi, 2);
+                sumYi += yi;
+                sumYiXi += yi * xi;
+            }
+            magScaledSlope = (size - 1) / deltaY *
+                             std::fabs((sumYiXi - sumXi * sumYi / N) / (sumXiSqr - std::pow(sumXi, 2) / N));
+        }
+        return convergedConst();
+    }
+};
+
+static SteadyCheck makeSteadyCheck(const rdst_settings& st) {
+    SteadyCheck sc;
+    sc.iterationStart = st.iteration_start;
+    sc.maxInnerIter = st.max_inner_iter;
+    sc.magConvergedSlope = st.mag_converged_slope;
+    sc.minResidual = st.min_residual;
+    return sc;
+}
+
+// runTime++ (rhoDST.C:73; foam Time: deltaT0 = the deltaT of the previous
+// step, the controlDict deltaT before the first) and Dual::iterate
+// (Dual.C:80-124) for a laminar case
+static double dualStep(Case& c, int* nInner, double* innerRes) {
+    const int n = c.m.nCells;
+    c.deltaT0 = c.deltaTSave;
+    c.deltaTSave = c.deltaT;
+    c.time += c.deltaT;
+    c.timeIndex++;
+    std::vector<double> Wc(5 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        Wc[5 * i] = c.rho[i]; Wc[5 * i + 1] = c.rhoU[i].x; Wc[5 * i + 2] = c.rhoU[i].y; Wc[5 * i + 3] = c.rhoU[i].z;
+        Wc[5 * i + 4] = c.rhoE[i];
+    }
+    c.WOldOld = (c.timeIndex == 1) ? Wc : c.WOld;  // oldTime().oldTime() of the first step is oldTime()
+    c.WOld = Wc;
+    SteadyCheck sc = makeSteadyCheck(c.st);
+    int k = 0;
+    do {
+        computeFlux(c);
+        computeJacobian(c);
+        assemble(c);
+        Perf perf = solve(c);
+        updateFields(c);
+        sc.addResidual(perf.init);
+        if (innerRes) innerRes[k] = sc.residual;
+        ++k;
+        setCourantDST(c, perf.init, !sc.convergedConst());
+    } while (!sc.converged());
+    if (nInner) *nInner = k;
+    return setCourantDST(c, nullptr, !sc.convergedConst());
 }
 
 
@@ -2460,6 +2635,38 @@ int orc_block_solve(void* h, rdst_solver_perf* perf) {
 }
 int orc_update_fields(void* h) { ORC_TRY({ updateFields(*(Case*)h); }) }
 int orc_set_courant(void* h, double* res) { ORC_TRY({ double r = setCourant(*(Case*)h); if (res) *res = r; }) }
+int orc_dual_step(void* h, int32_t* nInner, double* innerRes, double* res) {
+    ORC_TRY({
+        Case& c = *(Case*)h;
+        if (c.st.time_integration != RDST_TIME_DUAL) throw std::runtime_error("orc_dual_step: not a Dual case");
+        int k = 0;
+        double r = dualStep(c, &k, innerRes);
+        if (nInner) *nInner = k;
+        if (res) *res = r;
+    })
+}
+int orc_time_info(void* h, double* out) {
+    ORC_TRY({
+        const Case& c = *(Case*)h;
+        out[0] = c.time; out[1] = c.deltaT; out[2] = c.deltaT0; out[3] = c.courantScalar;
+    })
+}
+// steadyCheck on a given history: residuals are added one at a time and
+// converged() is asked after each, as Dual::iterate does; returns the
+// 1-based iteration at which it first converged (0: never) and the scaled
+// slope at that point
+int orc_steady_check(const double* residuals, int n, const rdst_settings* st, int32_t* at, double* slope) {
+    ORC_TRY({
+        SteadyCheck sc = makeSteadyCheck(*st);
+        *at = 0;
+        for (int i = 0; i < n; ++i) {
+            double r5[5] = {residuals[i], residuals[i], residuals[i], residuals[i], residuals[i]};
+            sc.addResidual(r5);
+            if (sc.converged()) { *at = i + 1; break; }
+        }
+        *slope = sc.magScaledSlope;
+    })
+}
 int orc_iterate(void* h, int nSteps, double* res, rdst_solver_perf* perfs) {
     ORC_TRY({
         Case& c = *(Case*)h;

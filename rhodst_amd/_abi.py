@@ -55,7 +55,10 @@ class rdst_settings(C.Structure):
                 ("rho_min", C.c_double), ("rho_max", C.c_double), ("p_min", C.c_double),
                 ("p_max", C.c_double), ("T_min", C.c_double), ("T_max", C.c_double),
                 ("ilu_order", C.c_int32), ("viscous", C.c_int32), ("gram_schmidt", C.c_int32),
-                ("reconstruction", C.c_int32), ("jacobian", C.c_int32)]
+                ("reconstruction", C.c_int32), ("jacobian", C.c_int32),
+                ("time_integration", C.c_int32), ("adjust_time_step", C.c_int32), ("max_inner_iter", C.c_int32),
+                ("iteration_start", C.c_int32), ("mag_converged_slope", C.c_double), ("min_residual", C.c_double),
+                ("delta_t", C.c_double)]
 
 
 class rdst_solver_perf(C.Structure):
@@ -220,6 +223,14 @@ def make_settings(**kw) -> rdst_settings:
     s.reconstruction = {"upwind": 0, "OSPRE": 1}[rc] if isinstance(rc, str) else int(rc)
     jk = kw.get("jacobian", "Roe")
     s.jacobian = {"Roe": 0, "exact": 1}[jk] if isinstance(jk, str) else int(jk)
+    ti = kw.get("time_integration", "Single")
+    s.time_integration = {"Single": 0, "Dual": 1}[ti] if isinstance(ti, str) else int(ti)
+    s.adjust_time_step = int(kw.get("adjust_time_step", 1))
+    s.max_inner_iter = int(kw.get("max_inner_iter", 20))
+    s.iteration_start = int(kw.get("iteration_start", 10 if s.time_integration == 1 else 500))
+    s.mag_converged_slope = float(kw.get("mag_converged_slope", 0.1))
+    s.min_residual = float(kw.get("min_residual", 1e-5))
+    s.delta_t = float(kw.get("delta_t", 0.0))
     return s
 
 

@@ -25,7 +25,7 @@ EXPORTS = ["rdst_last_error", "rdst_version", "rdst_init", "rdst_mesh_create", "
            "rdst_case_create",
            "rdst_case_destroy", "rdst_case_set_state", "rdst_case_get", "rdst_case_put",
            "rdst_flux_residual", "rdst_jacobian_assemble", "rdst_block_solve", "rdst_update_fields",
-           "rdst_set_courant", "rdst_iterate", "rdst_case_set_turbulence", "rdst_turbulence_correct",
+           "rdst_set_courant", "rdst_iterate", "rdst_dual_step", "rdst_time_info", "rdst_case_set_turbulence", "rdst_turbulence_correct",
            "rdst_adjoint_linearise", "rdst_adjoint_apply", "rdst_adjoint_residual",
            "rdst_adjoint_objective_gradient", "rdst_adjoint_solve",
            "rdst_forces", "rdst_last_flux_ms", "rdst_time_flux"]
@@ -65,6 +65,8 @@ def lib():
     L.rdst_block_solve.argtypes = [C.c_void_p, P(rdst_solver_perf)]
     L.rdst_set_courant.argtypes = [C.c_void_p, P(C.c_double)]
     L.rdst_iterate.argtypes = [C.c_void_p, C.c_int, P(C.c_double), P(rdst_solver_perf)]
+    L.rdst_dual_step.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_double), P(C.c_double)]
+    L.rdst_time_info.argtypes = [C.c_void_p, P(C.c_double)]
     L.rdst_forces.argtypes = [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double), C.c_double, C.c_double,
                               C.c_double, C.c_double, P(C.c_double)]
     L.rdst_case_set_turbulence.argtypes = [C.c_void_p, P(rdst_turbulence), P(rdst_scalar_bc), P(C.c_double),
@@ -265,6 +267,21 @@ class DeviceCase:
         perfs = (rdst_solver_perf * n_steps)()
         check(lib().rdst_iterate(self.h, n_steps, dptr(res), perfs))
         return (res, perfs) if with_perf else res
+
+    def dual_step(self):
+        """One physical time step of a Dual case (rdst_dual_step): (inner
+        iterations, their mean initial residuals, |R|_2 of the closing
+        setCourantDST)."""
+        k = C.c_int32()
+        inner = np.zeros(self.spec.settings.max_inner_iter + 1)
+        r = C.c_double()
+        check(lib().rdst_dual_step(self.h, C.byref(k), dptr(inner), C.byref(r)))
+        return k.value, inner[:k.value].copy(), r.value
+
+    def time_info(self):
+        out = np.zeros(4)
+        check(lib().rdst_time_info(self.h, dptr(out)))
+        return {"time": out[0], "deltaT": out[1], "deltaT0": out[2], "courant": out[3]}
 
     def forces(self, patch, drag_dir, lift_dir, p_ref, rho_inf, mag_u_inf, a_ref=1.0):
         d = np.ascontiguousarray(drag_dir, dtype=np.float64)

@@ -3,6 +3,7 @@
 // Every entry point converts exceptions to an RDST_ERR_* code and keeps the
 // message for rdst_last_error(); the reference aborts through FatalError
 // instead (e.g. newbaseFluxDST.C:72-82 lists the valid fluxScheme names).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -41,6 +42,10 @@ static void alloc_fields(Case& c) {
     c.timeBuf.alloc(7 * n + 4096);
     c.fHBuf.alloc(n);
     c.src.alloc(5 * n);
+    if (c.st.time_integration == RDST_TIME_DUAL) {
+        c.WOld.alloc(5 * n);
+        c.WOldOld.alloc(5 * n);
+    }
     c.cellBuf.zero(); c.bndBuf.zero(); c.faceBuf.zero(); c.RBuf.zero(); c.WBuf.zero(); c.timeBuf.zero(); c.fHBuf.zero();
     c.gradBuf.zero();
     c.turbBuf.alloc(2 * n + 2 * nB);
@@ -235,6 +240,14 @@ int rdst_case_create(rdst_mesh* mesh, const rdst_thermo* thermo, const rdst_bc* 
             (s.reconstruction != RDST_RECON_UPWIND || s.viscous || s.flux_scheme == RDST_FLUX_HLLCP))
             throw ArgError("rdst_case_create: the exact Jacobian covers the first-order inviscid residual "
                            "(upwind reconstruction, viscous 0, not HLLCP)");
+        if (s.time_integration != RDST_TIME_SINGLE && s.time_integration != RDST_TIME_DUAL)
+            throw ArgError("Unknown timeIntegration; valid: Single Dual", RDST_ERR_UNKNOWN_NAME);
+        if (s.time_integration == RDST_TIME_DUAL) {
+            if (s.jacobian != RDST_JACOBIAN_ROE)
+                throw ArgError("rdst_case_create: dual time stepping is provided with the Roe Jacobian");
+            if (s.max_inner_iter < 1 || !(s.delta_t >= 0) || !(s.min_residual >= 0))
+                throw ArgError("rdst_case_create: bad dual-time controls (maxInnerIter >= 1, deltaT >= 0)");
+        }
         if (s.ilu_order != RDST_ILU_NATURAL && s.ilu_order != RDST_ILU_PARTITIONED)
             throw ArgError("Unknown ILU ordering; valid: natural partitioned", RDST_ERR_UNKNOWN_NAME);
         if (!(thermo->Cp > thermo->R) || !(thermo->R > 0)) throw ArgError("rdst_case_create: need Cp > R > 0");
@@ -457,6 +470,8 @@ int rdst_iterate(rdst_case* cc, int n_steps, double* res_norm, rdst_solver_perf*
         Case* c = reinterpret_cast<Case*>(cc);
         if (!c) throw ArgError("null case");
         if (!c->hasState) throw ArgError("rdst_iterate: no state", RDST_ERR_STATE);
+        if (c->st.time_integration == RDST_TIME_DUAL)
+            throw ArgError("rdst_iterate: a Dual case advances with rdst_dual_step", RDST_ERR_STATE);
         for (int s = 0; s < n_steps; ++s) {
             flux_launch(*c, c->sa.on ? STORE_PHI : STORE_BPHI);
             jacobian_assemble(*c);
@@ -469,6 +484,105 @@ int rdst_iterate(rdst_case* cc, int n_steps, double* res_norm, rdst_solver_perf*
             if (perf) perf[s] = p;
         }
         RDST_HIP(hipStreamSynchronize(c->stream));
+    })
+}
+
+namespace {
+// residualControls + steadyCheck (residualControls.C:37-64, steadyCheck.C:37-98):
+// the inner residual is the mean of the five initial residuals of the inner
+// solve; converged after more than maxInnerIter inner iterations, below
+// minResudual, or (from iterationStart on) when the scaled slope of
+// log(residual) over the last 20 % of the history drops below
+// magConvergedSlope.  converged() refreshes the slope, converged_const()
+// (what setCourantDST asks) reads the last one.
+struct SteadyCheck {
+    std::vector<double> residuals;
+    int iterationCount = 0, iterationStart, maxInnerIter;
+    double residual = 1, magScaledSlope = 1, magConvergedSlope, minResidual;
+    explicit SteadyCheck(const rdst_settings& st)
+        : iterationStart(st.iteration_start), maxInnerIter(st.max_inner_iter),
+          magConvergedSlope(st.mag_converged_slope), minResidual(st.min_residual) {}
+    void add_residual(const double* res) {
+        double mean = 0;
+        for (int i = 0; i < 5; ++i) mean += res[i];
+        mean /= 5;
+        ++iterationCount;
+        residual = mean;
+        residuals.push_back(mean);
+    }
+    bool converged_const() const {
+        return iterationCount > maxInnerIter || residual < minResidual || magScaledSlope < magConvergedSlope;
+    }
+    bool converged() {
+        if (iterationCount >= iterationStart) {
+            const int size = (int)residuals.size();
+            const int N = 0.2 * size;
+            const double maxY = std::log(*std::max_element(residuals.begin(), residuals.end()));
+            const double minY = std::log(*std::min_element(residuals.begin(), residuals.end()));
+            const double deltaY = maxY - minY;
+            double sumXi = 0, sumYi = 0, sumXiSqr = 0, sumYiXi = 0;
+            for (int i = 0; i < N; ++i) {
+                const int xi = size - 1 - i;
+                const double yi = std::log(residuals[xi]);
+                sumXi += xi;
+                sumXiSqr += std::pow(xi, 2);
+                sumYi += yi;
+                sumYiXi += yi * xi;
+            }
+            magScaledSlope = (size - 1) / deltaY *
+                             std::fabs((sumYiXi - sumXi * sumYi / N) / (sumXiSqr - std::pow(sumXi, 2) / N));
+        }
+        return converged_const();
+    }
+};
+}  // namespace
+
+int rdst_dual_step(rdst_case* cc, int32_t* n_inner, double* inner_residuals, double* res_norm) {
+    RDST_API_BODY({
+        Case* c = reinterpret_cast<Case*>(cc);
+        if (!c) throw ArgError("null case");
+        if (!c->hasState) throw ArgError("rdst_dual_step: no state", RDST_ERR_STATE);
+        if (c->st.time_integration != RDST_TIME_DUAL)
+            throw ArgError("rdst_dual_step: the case is not Dual (settings.time_integration)", RDST_ERR_STATE);
+        const size_t n5 = 5 * (size_t)c->mesh->nC;
+        // runTime++: time, deltaT0 = the previous deltaT, old time levels
+        c->deltaT0 = c->deltaTSave;
+        c->deltaTSave = c->deltaT;
+        c->time += c->deltaT;
+        ++c->timeIndex;
+        // oldTime().oldTime() of the first step is oldTime() itself
+        if (c->timeIndex == 1) store_conserved(*c, c->WOldOld.p);
+        else RDST_HIP(hipMemcpyAsync(c->WOldOld.p, c->WOld.p, sizeof(double) * n5, hipMemcpyDeviceToDevice, c->stream));
+        store_conserved(*c, c->WOld.p);
+        // Dual::iterate
+        SteadyCheck sc(c->st);
+        int k = 0;
+        do {
+            flux_launch(*c, STORE_BPHI);
+            jacobian_assemble(*c);
+            rdst_solver_perf p{};
+            block_solve(*c, &p);
+            update_fields(*c);
+            sc.add_residual(p.initial_residual);
+            if (inner_residuals) inner_residuals[k] = sc.residual;
+            ++k;
+            set_courant_dst(*c, p.initial_residual, !sc.converged_const());
+        } while (!sc.converged());
+        const double r = set_courant_dst(*c, nullptr, !sc.converged_const());
+        if (n_inner) *n_inner = k;
+        if (res_norm) *res_norm = r;
+        RDST_HIP(hipStreamSynchronize(c->stream));
+    })
+}
+
+int rdst_time_info(const rdst_case* cc, double* out) {
+    RDST_API_BODY({
+        const Case* c = reinterpret_cast<const Case*>(cc);
+        if (!c || !out) throw ArgError("rdst_time_info: null argument");
+        out[0] = c->time;
+        out[1] = c->deltaT;
+        out[2] = c->deltaT0;
+        out[3] = c->courantScalar;
     })
 }
 
@@ -493,6 +607,9 @@ int rdst_case_set_turbulence(rdst_case* cc, const rdst_turbulence* turb, const r
             throw ArgError("rdst_case_set_turbulence: unknown RASModel id (valid: laminar SAneg SARC2 SARC1)",
                            RDST_ERR_UNKNOWN_NAME);
         if (turb->model != RDST_TURB_LAMINAR) {
+            if (c->st.time_integration == RDST_TIME_DUAL)
+                throw ArgError("rdst_case_set_turbulence: the Spalart-Allmaras models run with Single time "
+                               "integration (the dual2ndDST nuTilda ddt is not built)");
             if (!nu_tilda_bcs || !nu_tilda) throw ArgError("rdst_case_set_turbulence: null array");
             if (!c->hasState) throw ArgError("rdst_case_set_turbulence: no state (call rdst_case_set_state)", RDST_ERR_STATE);
             if (!c->st.viscous) throw ArgError("rdst_case_set_turbulence: the viscous terms are off (settings.viscous)");

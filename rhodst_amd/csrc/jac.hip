@@ -21,9 +21,18 @@ namespace rdst {
 
 namespace {
 
+// the physical-time part of the dual2ndDST ddt (dual2ndDST.C:395-411):
+// a = 1/deltaT + 1/(deltaT + deltaT0), b = -(a + c), c = deltaT/(deltaT0
+// (deltaT + deltaT0)) and the old time levels of W; on = 0: steadyStateDST
+struct DualDdt {
+    int on;
+    double a, b, c;
+    const double *Wo, *Woo;
+};
+
 template <bool F32>
 __global__ __launch_bounds__(128) void k_assemble(MeshDev m, Fields F, Thermo th, const BcPatchDev* bc, double beta,
-                                                  double* A64, float* A32, double* src) {
+                                                  double* A64, float* A32, double* src, DualDdt dd2) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
     const double gam = th.Cp / th.Cv;
@@ -162,9 +171,10 @@ __global__ __launch_bounds__(128) void k_assemble(MeshDev m, Fields F, Thermo th
             fd[q] -= v;
         }
     }
-    // ddt diagonal (steadyStateDST: rDeltaTDST V on all five rows) + flux diagonal
+    // ddt diagonal (steadyStateDST: rDeltaTDST V on all five rows; dual2ndDST:
+    // (rDeltaTDST + a) V) + flux diagonal
     const double V = m.V[c];
-    const double diagS = F.rDeltaT[c] * V;
+    const double diagS = dd2.on ? (F.rDeltaT[c] + dd2.a) * V : F.rDeltaT[c] * V;
     const int ds = m.slotDiag[c];
     for (int q = 0; q < 25; ++q) {
         const double dd = (q % 6 == 0) ? diagS : 0.0;
@@ -172,10 +182,14 @@ __global__ __launch_bounds__(128) void k_assemble(MeshDev m, Fields F, Thermo th
         if (F32) A32[25 * (size_t)ds + q] = (float)v;
         else A64[25 * (size_t)ds + q] = v;
     }
-    // source: (rDeltaT psi_old V - diag psi) + (-R V), psi_old = psi
+    // source: (rDeltaT psi_old V - diag psi) + (-R V), psi_old = psi; dual2ndDST:
+    // V (rDeltaT psi - b psi.oldTime() - c psi.oldTime().oldTime()) - diag psi
     const double x[5] = {rho, F.rhoUx[c], F.rhoUy[c], F.rhoUz[c], rhoE};
     for (int q = 0; q < 5; ++q) {
-        const double sd = F.rDeltaT[c] * x[q] * V - diagS * x[q];
+        const double sd = dd2.on ? V * (F.rDeltaT[c] * x[q] - dd2.b * dd2.Wo[5 * (size_t)c + q] -
+                                        dd2.c * dd2.Woo[5 * (size_t)c + q]) -
+                                       diagS * x[q]
+                                 : F.rDeltaT[c] * x[q] * V - diagS * x[q];
         const double fs = 0.0 - F.R[(size_t)q * m.nC + c] * V;
         src[5 * (size_t)c + q] = sd + fs;
     }
@@ -188,17 +202,27 @@ void jacobian_assemble(Case& c) {
     Mesh& m = *c.mesh;
     MeshDev md = c.meshDev();
     const bool mixed = c.st.precision == RDST_PREC_MIXED;
+    DualDdt dd2{0, 0.0, 0.0, 0.0, nullptr, nullptr};
+    if (c.st.time_integration == RDST_TIME_DUAL && c.timeIndex > 0) {
+        const double rDeltaT = 1.0 / c.deltaT;
+        dd2.on = 1;
+        dd2.a = rDeltaT + 1.0 / (c.deltaT + c.deltaT0);
+        dd2.c = c.deltaT / (c.deltaT0 * (c.deltaT + c.deltaT0));
+        dd2.b = -(dd2.a + dd2.c);
+        dd2.Wo = c.WOld.p;
+        dd2.Woo = c.WOldOld.p;
+    }
     if (c.st.jacobian == RDST_JACOBIAN_EXACT) {
         // the reference's Roe operator, kept as the preconditioner source
         const size_t nn = 25 * (size_t)m.nnzb;
         if (mixed) {
             if (c.P32.n != nn) c.P32.alloc(nn);
             hipLaunchKernelGGL((k_assemble<true>), dim3(blocks_for(m.nC, 128)), dim3(128), 0, c.stream, md, c.f, c.th,
-                               c.d_bc.p, c.st.beta, (double*)nullptr, c.P32.p, c.src.p);
+                               c.d_bc.p, c.st.beta, (double*)nullptr, c.P32.p, c.src.p, dd2);
         } else {
             if (c.P64.n != nn) c.P64.alloc(nn);
             hipLaunchKernelGGL((k_assemble<false>), dim3(blocks_for(m.nC, 128)), dim3(128), 0, c.stream, md, c.f, c.th,
-                               c.d_bc.p, c.st.beta, c.P64.p, (float*)nullptr, c.src.p);
+                               c.d_bc.p, c.st.beta, c.P64.p, (float*)nullptr, c.src.p, dd2);
         }
         newton_operator(c);
         RDST_HIP(hipGetLastError());
@@ -208,11 +232,11 @@ void jacobian_assemble(Case& c) {
     if (mixed) {
         if (c.A32.n != 25 * (size_t)m.nnzb) c.A32.alloc(25 * (size_t)m.nnzb);
         hipLaunchKernelGGL((k_assemble<true>), dim3(blocks_for(m.nC, 128)), dim3(128), 0, c.stream, md, c.f, c.th, c.d_bc.p,
-                           c.st.beta, (double*)nullptr, c.A32.p, c.src.p);
+                           c.st.beta, (double*)nullptr, c.A32.p, c.src.p, dd2);
     } else {
         if (c.A64.n != 25 * (size_t)m.nnzb) c.A64.alloc(25 * (size_t)m.nnzb);
         hipLaunchKernelGGL((k_assemble<false>), dim3(blocks_for(m.nC, 128)), dim3(128), 0, c.stream, md, c.f, c.th, c.d_bc.p,
-                           c.st.beta, c.A64.p, (float*)nullptr, c.src.p);
+                           c.st.beta, c.A64.p, (float*)nullptr, c.src.p, dd2);
     }
     RDST_HIP(hipGetLastError());
     c.hasMatrix = true;

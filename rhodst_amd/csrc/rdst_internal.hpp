@@ -290,6 +290,10 @@ struct Case {
     struct SolverWork* work = nullptr;
     // time-control scalars
     double residual = -1, residualRatio = 1, courantScalar = 1;
+    // dual time stepping: old time levels of W [nC][5] and the physical time
+    DBuf<double> WOld, WOldOld;
+    double time = 0, deltaT = 0, deltaT0 = 0, deltaTSave = 0;
+    int timeIndex = 0;
     std::vector<double> hostScratch;
     SaState sa;
     // discrete adjoint: exact J = dR/dW and its transpose on the block-CSR slots
@@ -312,6 +316,11 @@ void jacobian_assemble(Case& c);
 void block_solve(Case& c, rdst_solver_perf* perf);
 void update_fields(Case& c);
 double set_courant(Case& c);
+// setCourantDST with the inner-iteration residual |lin| (lin != nullptr) or
+// |R|_2; pseudo: the pseudo-time branch, else the physical one
+double set_courant_dst(Case& c, const double* lin, bool pseudo);
+void set_transient_deltaT(Case& c);
+void store_conserved(Case& c, double* out);  // rho, rhoU, rhoE as [nC][5]
 void set_steady_state_deltaT(Case& c);
 void set_state(Case& c, const double* p, const double* T, const double* U);
 void solver_free(Case& c);

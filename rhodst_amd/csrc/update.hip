@@ -18,6 +18,7 @@
 
 #include "rdst_internal.hpp"
 #include "fluxfun.hpp"
+#include "reduce.hpp"
 
 // No floating-point contraction: the reference evaluates every product
 // rounded, so a*x - a*x cancels exactly (a tangential state on both sides of
@@ -364,6 +365,54 @@ __global__ void k_lts_blazek(MeshDev m, Fields F, Thermo th, double courant) {
     F.rDeltaT[c] = lam / (courant * m.V[c]);
 }
 
+// setTransientDeltaT (timeDST.C:275-314): per-block minima of the physical
+// time step, over the internal faces |Sf| / ((|U_f . Sf| + |Sf| c_f)
+// deltaCoeffs) courant with linear interpolation (neighbouring), or over the
+// cells courant V / ((|U| + c) . dS)
+__global__ __launch_bounds__(256) void k_dt_faces(MeshDev m, Fields F, Thermo th, double courant, double* part) {
+    __shared__ double sh[16];
+    const double gam = th.Cp / th.Cv;
+    double dt = 1e300;
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < m.nI; f += gridDim.x * blockDim.x) {
+        const int o = m.owner[f], nb = m.neigh[f];
+        const double w = m.w[f];
+        const double fx = w * (F.Ux[o] - F.Ux[nb]) + F.Ux[nb];
+        const double fy = w * (F.Uy[o] - F.Uy[nb]) + F.Uy[nb];
+        const double fz = w * (F.Uz[o] - F.Uz[nb]) + F.Uz[nb];
+        const double co = sqrt(gam / F.psi[o]), cn = sqrt(gam / F.psi[nb]);
+        const double fc = w * (co - cn) + cn;
+        const double sx = m.Sf[f], sy = m.Sf[m.nF + f], sz = m.Sf[2 * (size_t)m.nF + f];
+        const double amax = fabs(fx * sx + fy * sy + fz * sz) + m.magSf[f] * fc;
+        const double dtSf = m.magSf[f] / (amax * m.dc[f]);
+        dt = fmin(dt, dtSf * courant);
+    }
+    const double t = block_min(dt, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+__global__ __launch_bounds__(256) void k_dt_cells(MeshDev m, Fields F, Thermo th, double courant, double* part) {
+    __shared__ double sh[16];
+    double dt = 1e300;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < m.nC; c += gridDim.x * blockDim.x) {
+        const double cc = sqrt(th.Cp / th.Cv / F.psi[c]);
+        const double lam = (fabs(F.Ux[c]) + cc) * F.dS[c] + (fabs(F.Uy[c]) + cc) * F.dS[m.nC + c] +
+                           (fabs(F.Uz[c]) + cc) * F.dS[2 * m.nC + c];
+        dt = fmin(dt, courant * m.V[c] / lam);
+    }
+    const double t = block_min(dt, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// the conserved fields as [nC][5] (rho, rhoU, rhoE): an old time level of W
+__global__ void k_conserved_aos(int n, Fields F, double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[5 * (size_t)i] = F.rho[i];
+    out[5 * (size_t)i + 1] = F.rhoUx[i];
+    out[5 * (size_t)i + 2] = F.rhoUy[i];
+    out[5 * (size_t)i + 3] = F.rhoUz[i];
+    out[5 * (size_t)i + 4] = F.rhoE[i];
+}
+
 // ---------------------------------------------------------------- CFL ramp
 // per-block partial sums of |R_c|^2 (mag then square, timeDST.C:328-331)
 __global__ __launch_bounds__(256) void k_rnorm_partial(int n, const double* R, double* partial) {
@@ -452,6 +501,14 @@ void set_state(Case& c, const double* p, const double* T, const double* U) {
     c.courantScalar = c.st.initial_co;
     std::vector<double> co(n, c.st.initial_co);
     RDST_HIP(hipMemcpyAsync(c.f.courant, co.data(), sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+    c.time = 0;
+    c.timeIndex = 0;
+    if (c.st.time_integration == RDST_TIME_DUAL) {
+        // timeDST::initialize (timeDST.C:193-207): the physical deltaT from the
+        // initial courant; deltaT0 of the first step is the controlDict deltaT
+        set_transient_deltaT(c);
+        c.deltaT0 = c.deltaTSave = c.st.delta_t > 0 ? c.st.delta_t : c.deltaT;
+    }
     set_steady_state_deltaT(c);
     RDST_HIP(hipGetLastError());
     RDST_HIP(hipStreamSynchronize(c.stream));
@@ -481,7 +538,36 @@ void update_fields(Case& c) {
     RDST_HIP(hipGetLastError());
 }
 
-double set_courant(Case& c) {
+void store_conserved(Case& c, double* out) {
+    const int n = c.mesh->nC;
+    hipLaunchKernelGGL(k_conserved_aos, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, c.f, out);
+    RDST_HIP(hipGetLastError());
+}
+
+void set_transient_deltaT(Case& c) {
+    Mesh& m = *c.mesh;
+    MeshDev md = c.meshDev();
+    const int bs = 256;
+    const int nItems = c.st.neighbouring ? m.nI : m.nC;
+    const int nb = std::max(1, std::min(1024, blocks_for(nItems, bs)));
+    if (c.hostScratch.size() < (size_t)nb) c.hostScratch.resize(nb);
+    double* dpart = c.f.dS + 3 * (size_t)m.nC;  // partial scratch reserved after dS
+    if (c.st.neighbouring)
+        hipLaunchKernelGGL(k_dt_faces, dim3(nb), dim3(bs), 0, c.stream, md, c.f, c.th, c.courantScalar, dpart);
+    else
+        hipLaunchKernelGGL(k_dt_cells, dim3(nb), dim3(bs), 0, c.stream, md, c.f, c.th, c.courantScalar, dpart);
+    RDST_HIP(hipGetLastError());
+    RDST_HIP(hipMemcpyAsync(c.hostScratch.data(), dpart, sizeof(double) * nb, hipMemcpyDeviceToHost, c.stream));
+    RDST_HIP(hipStreamSynchronize(c.stream));
+    double dt = 1e300;
+    for (int i = 0; i < nb; ++i) dt = std::min(dt, c.hostScratch[i]);
+    c.deltaT = dt;
+}
+
+double set_courant(Case& c) { return set_courant_dst(c, nullptr, true); }
+
+// setCourantDST (timeDST.C:316-443)
+double set_courant_dst(Case& c, const double* lin, bool pseudo) {
     Mesh& m = *c.mesh;
     MeshDev md = c.meshDev();
     const int bs = 256;
@@ -490,15 +576,32 @@ double set_courant(Case& c) {
     DBuf<double>& part = c.timeBuf;  // scratch after the time fields
     (void)part;
     double* dpart = c.f.dS + 3 * (size_t)m.nC;  // partial-sum scratch reserved after dS
-    hipLaunchKernelGGL(k_rnorm_partial, dim3(nb), dim3(bs), 0, c.stream, m.nC, c.f.R, dpart);
-    RDST_HIP(hipGetLastError());
-    RDST_HIP(hipMemcpyAsync(c.hostScratch.data(), dpart, sizeof(double) * nb, hipMemcpyDeviceToHost, c.stream));
-    RDST_HIP(hipStreamSynchronize(c.stream));
-    double ss = 0.0;
-    for (int i = 0; i < nb; ++i) ss += c.hostScratch[i];
-    double residualR = std::sqrt(ss);
+    double residualR;
+    if (lin) {
+        // |solverPerf.initialResidual()| (timeDST.C:321-325)
+        double s2 = 0.0;
+        for (int q = 0; q < 5; ++q) s2 += lin[q] * lin[q];
+        residualR = std::sqrt(s2);
+    } else {
+        hipLaunchKernelGGL(k_rnorm_partial, dim3(nb), dim3(bs), 0, c.stream, m.nC, c.f.R, dpart);
+        RDST_HIP(hipGetLastError());
+        RDST_HIP(hipMemcpyAsync(c.hostScratch.data(), dpart, sizeof(double) * nb, hipMemcpyDeviceToHost, c.stream));
+        RDST_HIP(hipStreamSynchronize(c.stream));
+        double ss = 0.0;
+        for (int i = 0; i < nb; ++i) ss += c.hostScratch[i];
+        residualR = std::sqrt(ss);
+    }
     if (c.residual != -1) c.residualRatio = std::min(std::max(c.residual / residualR, 0.1), 2.0);
     c.residual = residualR;
+    if (!pseudo) {
+        // the physical branch: with adjustTimeStep the scalar courant follows
+        // the residual ratio and sets the next physical deltaT
+        if (c.st.adjust_time_step) {
+            c.courantScalar = std::min(std::max(c.courantScalar * c.residualRatio, c.st.min_co), c.st.max_co);
+            set_transient_deltaT(c);
+        }
+        return residualR;
+    }
     hipLaunchKernelGGL(k_courant, dim3(blocks_for(m.nC, bs)), dim3(bs), 0, c.stream, md, c.f, c.residualRatio,
                        c.st.min_co, c.st.max_co);
     hipLaunchKernelGGL(k_store_old, dim3(blocks_for(m.nC, bs)), dim3(bs), 0, c.stream, m.nC, c.f);
