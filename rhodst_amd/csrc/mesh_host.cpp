@@ -144,6 +144,42 @@ static void build_pilu_stream(const Mesh& m, bool bwd, PiluStream& ps) {
     ps.jn.swap(jn);
 }
 
+// Partition-local level of every cell (longest chain of in-partition lower
+// neighbours, cells in increasing index); returns the deepest level count.
+static int partition_levels(const Mesh& m, std::vector<int32_t>& lev) {
+    int deepest = 0;
+    for (int c = 0; c < m.nC; ++c) {
+        int l = 0;
+        for (int s = m.rowPtr[c]; s < m.slotDiag[c]; ++s)
+            if (m.part[m.cols[s]] == m.part[c]) l = std::max(l, lev[m.cols[s]] + 1);
+        lev[c] = l;
+        deepest = std::max(deepest, l + 1);
+    }
+    return deepest;
+}
+
+// The apply of a partition is a chain of its levels; a partition deeper than
+// maxLevels is cut at its middle level (the cells of the upper half form a
+// new partition), until none is.
+static void split_deep_partitions(Mesh& m, int maxLevels) {
+    std::vector<int32_t> lev(m.nC, 0);
+    for (int pass = 0; pass < 16; ++pass) {
+        partition_levels(m, lev);
+        std::vector<int> depth(m.nParts, 0);
+        for (int c = 0; c < m.nC; ++c) depth[m.part[c]] = std::max(depth[m.part[c]], lev[c] + 1);
+        std::vector<int> newId(m.nParts, -1);
+        int added = 0;
+        for (int p = 0; p < m.nParts; ++p)
+            if (depth[p] > maxLevels) newId[p] = m.nParts + added++;
+        if (!added) return;
+        for (int c = 0; c < m.nC; ++c) {
+            const int p = m.part[c];
+            if (newId[p] >= 0 && lev[c] >= depth[p] / 2) m.part[c] = newId[p];
+        }
+        m.nParts += added;
+    }
+}
+
 // Partitions for the block-Jacobi ILU(0): couplings between partitions are
 // dropped, inside a partition the factor is the natural-order ILU(0) of the
 // partition's diagonal block, scheduled by local levels.
@@ -154,6 +190,7 @@ void build_partitions(Mesh& m, int maxCells) {
     std::iota(idx.begin(), idx.end(), 0);
     m.nParts = 0;
     rcb(m.C, idx, 0, n, maxCells, m.part, m.nParts);
+    split_deep_partitions(m, kMaxPartLevels);
     // cells of each partition in increasing index
     std::vector<std::vector<int>> cellsOf(m.nParts);
     for (int c = 0; c < n; ++c) cellsOf[m.part[c]].push_back(c);

@@ -171,6 +171,8 @@ struct Mesh {
 // largest partition of the block-Jacobi ILU(0); its FP64 vector (40 B/cell)
 // is held in LDS by one workgroup
 constexpr int kPartCells = 1024;
+// deepest partition-local level chain kept; deeper partitions are cut in two
+constexpr int kMaxPartLevels = 48;
 void mesh_build(Mesh& m, const rdst_mesh_desc* d);
 void build_partitions(Mesh& m, int maxCells);
 void mesh_upload(Mesh& m);

@@ -51,11 +51,11 @@ def test_dual_step_parity(oracle, name):
 
 
 def test_dual_entry_point_errors():
-    single = DeviceCase(cases.forward_step_case(nx=20, ny=8, scheme="AUSM+"))
+    single = DeviceCase(cases.forward_step_case(nx=30, ny=10, scheme="AUSM+"))
     with pytest.raises(RdstError):
         single.dual_step()
-    dual = DeviceCase(cases.forward_step_case(nx=20, ny=8, scheme="AUSM+", time_integration="Dual"))
+    dual = DeviceCase(cases.forward_step_case(nx=30, ny=10, scheme="AUSM+", time_integration="Dual"))
     with pytest.raises(RdstError):
         dual.iterate(1)
     with pytest.raises(RdstError):
-        DeviceCase(cases.forward_step_case(nx=20, ny=8, scheme="AUSM+", time_integration="Dual", jacobian="exact"))
+        DeviceCase(cases.forward_step_case(nx=30, ny=10, scheme="AUSM+", time_integration="Dual", jacobian="exact"))
