@@ -199,10 +199,18 @@ __device__ inline Prim<D1> seeded(const double* W, int i, int k, const Thermo& t
 // every internal face, at this row's slots).  NEWTON: the row of the Newton
 // operator instead, V_c dR_c/dW + rDeltaTDST V_c I (the face sums not divided
 // by V), stored as TJ (FP32 for the mixed path), and the Newton source -R_c V_c
-// into src (the ddt part of the steady-state source vanishes)
+// into src (the ddt part of the steady-state source vanishes).  The Newton
+// operator feeds only the SpMV, so it is written in the sliced-ELL layout
+// (entry q of slot s at sellOfSlot[s] + 64 q) rather than block-CSR.
 template <int S, class TJ, bool NEWTON>
 __global__ __launch_bounds__(256) void k_adj_jacobian(MeshDev m, Fields F, const BcPatchDev* bc, Thermo th, Bounds bd,
-                                                      int lowMach, double Mainf, TJ* J, double* src) {
+                                                      int lowMach, double Mainf, TJ* J, double* src,
+                                                      const int64_t* sellOfSlot) {
+    // value index of entry q of block-CSR slot s
+    auto at = [&](int s, int q) -> size_t {
+        if constexpr (NEWTON) return (size_t)sellOfSlot[s] + 64 * (size_t)q;
+        else return 25 * (size_t)s + q;
+    };
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
     const size_t E = (size_t)m.ellW * m.nC;
@@ -236,7 +244,7 @@ __global__ __launch_bounds__(256) void k_adj_jacobian(MeshDev m, Fields F, const
         const bool own = kind == ELL_OWNER;
         const double sgn = own ? rV : -rV;
         const int fi = m.ellFace[e];
-        TJ* Jco = J + 25 * (size_t)(own ? m.slotUpper[fi] : m.slotLower[fi]);
+        const int sco = own ? m.slotUpper[fi] : m.slotLower[fi];
         // derivative w.r.t. this cell's W, then w.r.t. the other cell's W
         for (int side = 0; side < 2; ++side) {
             for (int kk = 0; kk < 5; ++kk) {
@@ -249,13 +257,13 @@ __global__ __launch_bounds__(256) void k_adj_jacobian(MeshDev m, Fields F, const
                 if (side == 0)
                     for (int r = 0; r < 5; ++r) Jcc[r * 5 + kk] += sgn * d[r];
                 else
-                    for (int r = 0; r < 5; ++r) Jco[r * 5 + kk] = (TJ)(sgn * d[r]);
+                    for (int r = 0; r < 5; ++r) J[at(sco, r * 5 + kk)] = (TJ)(sgn * d[r]);
             }
         }
     }
-    TJ* Jd = J + 25 * (size_t)m.slotDiag[c];
+    const int sd = m.slotDiag[c];
     const double dd = NEWTON ? F.rDeltaT[c] * m.V[c] : 0.0;
-    for (int q = 0; q < 25; ++q) Jd[q] = (TJ)(Jcc[q] + ((q % 6 == 0) ? dd : 0.0));
+    for (int q = 0; q < 25; ++q) J[at(sd, q)] = (TJ)(Jcc[q] + ((q % 6 == 0) ? dd : 0.0));
     if (NEWTON) {
         const double V = m.V[c];
         for (int q = 0; q < 5; ++q) src[5 * (size_t)c + q] = 0.0 - F.R[(size_t)q * m.nC + c] * V;
@@ -363,7 +371,8 @@ template <int S, class TJ, bool NEWTON>
 void launch_jac(Case& c, TJ* J) {
     const int nb = blocks_for(c.mesh->nC, 256);
     hipLaunchKernelGGL((k_adj_jacobian<S, TJ, NEWTON>), dim3(nb), dim3(256), 0, c.stream, c.meshDev(), c.f, c.d_bc.p,
-                       c.th, bounds_of(c), c.st.low_mach, c.st.Mainf, J, NEWTON ? c.src.p : (double*)nullptr);
+                       c.th, bounds_of(c), c.st.low_mach, c.st.Mainf, J, NEWTON ? c.src.p : (double*)nullptr,
+                       c.mesh->d_sellOfSlot.p);
 }
 template <class TJ, bool NEWTON>
 void launch_jac_scheme(Case& c, TJ* J) {
@@ -408,15 +417,47 @@ void newton_operator(Case& c) {
     adjoint_check_scheme(c);
     const Mesh& m = *c.mesh;
     const size_t nn = 25 * (size_t)m.nnzb;
+    // every slot is written (a cell's diagonal and its side of each face);
+    // the padding blocks of the slices stay zero from the allocation.  The
+    // block-CSR buffers keep their identity for the SpMV lookup and receive
+    // the operator only when the matrix is read back (newton_operator_csr).
+    const size_t nv = 25 * (size_t)m.sellBlocks;
     if (c.st.precision == RDST_PREC_MIXED) {
         if (c.A32.n != nn) c.A32.alloc(nn);
-        c.A32.zero(c.stream);
-        launch_jac_scheme<float, true>(c, c.A32.p);
+        if (c.S32.n != nv) {
+            c.S32.alloc(nv);
+            c.S32.zero(c.stream);
+        }
+        launch_jac_scheme<float, true>(c, c.S32.p);
     } else {
         if (c.A64.n != nn) c.A64.alloc(nn);
-        c.A64.zero(c.stream);
-        launch_jac_scheme<double, true>(c, c.A64.p);
+        if (c.S64.n != nv) {
+            c.S64.alloc(nv);
+            c.S64.zero(c.stream);
+        }
+        launch_jac_scheme<double, true>(c, c.S64.p);
     }
+    c.sellFresh = true;
+}
+
+namespace {
+template <class T>
+__global__ void k_from_sell(long nnzb, const int64_t* sellOfSlot, const T* S, T* A) {
+    for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < nnzb; j += (long)gridDim.x * blockDim.x) {
+        const int64_t base = sellOfSlot[j];
+        for (int e = 0; e < 25; ++e) A[25 * (size_t)j + e] = S[base + 64 * e];
+    }
+}
+}  // namespace
+
+// the FP64 Newton operator in block-CSR (A64), for a read-back
+void newton_operator_csr(Case& c) {
+    if (!c.sellFresh || c.st.precision == RDST_PREC_MIXED) return;
+    const Mesh& m = *c.mesh;
+    hipLaunchKernelGGL((k_from_sell<double>), dim3(std::min(4096, blocks_for(m.nnzb, 256))), dim3(256), 0, c.stream,
+                       (long)m.nnzb, m.d_sellOfSlot.p, c.S64.p, c.A64.p);
+    RDST_HIP(hipGetLastError());
+    RDST_HIP(hipStreamSynchronize(c.stream));
 }
 
 void adjoint_linearise(Case& c) {

@@ -355,6 +355,7 @@ int rdst_case_get(rdst_case* cc, int field, double* out) {
             case RDST_F_BPSI: one(f.bpsi, nB); break;
             case RDST_F_BCSR_VALUES:
                 if (!c->hasMatrix || !c->A64.p) throw ArgError("no FP64 matrix assembled", RDST_ERR_STATE);
+                newton_operator_csr(*c);
                 one(c->A64.p, 25 * (size_t)m.nnzb);
                 break;
             case RDST_F_SOURCE: one(c->src.p, 5 * n); break;

@@ -229,6 +229,7 @@ void jacobian_assemble(Case& c) {
         c.hasMatrix = true;
         return;
     }
+    c.sellFresh = false;  // the sliced-ELL copy is refreshed from block-CSR in the solve
     if (mixed) {
         if (c.A32.n != 25 * (size_t)m.nnzb) c.A32.alloc(25 * (size_t)m.nnzb);
         hipLaunchKernelGGL((k_assemble<true>), dim3(blocks_for(m.nC, 128)), dim3(128), 0, c.stream, md, c.f, c.th, c.d_bc.p,

@@ -287,6 +287,7 @@ struct Case {
     DBuf<float> P32;
     DBuf<double> S64;               // sliced-ELL copies for the SpMV
     DBuf<float> S32;
+    bool sellFresh = false;  // S64/S32 were written by the Newton operator itself (no block-CSR copy)
     bool hasState = false, hasMatrix = false;
     // solver scratch owned by the solver module
     struct SolverWork* work = nullptr;
@@ -336,7 +337,8 @@ void sa_correct(Case& c, rdst_solver_perf* perf);
 void sa_put_nut(Case& c, const double* nuTilda);
 void adjoint_check_scheme(const Case& c);
 void adjoint_jacobian(Case& c);   // J = dR/dW only
-void newton_operator(Case& c);    // V dR/dW + rDeltaTDST V I into the case's A64 / A32
+void newton_operator(Case& c);
+void newton_operator_csr(Case& c);  // block-CSR copy of the FP64 Newton operator (read-back)    // V dR/dW + rDeltaTDST V I into the case's A64 / A32
 void adjoint_linearise(Case& c);  // J and its transpose
 void adjoint_residual(Case& c, double* R);
 void adjoint_objective_gradient(Case& c, int patch, const double* dir, double q, double* g);

@@ -1224,10 +1224,10 @@ void block_solve(Case& c, rdst_solver_perf* perfOut) {
     const bool newton = c.st.jacobian == RDST_JACOBIAN_EXACT;
     if (mixed) {
         ilu_factor<float>(k, newton ? c.P32.p : c.A32.p, w.LU32.p, w.Dinv32.p);
-        build_sell<float>(k, c.A32.p, c.S32);
+        if (!c.sellFresh) build_sell<float>(k, c.A32.p, c.S32);
     } else {
         ilu_factor<double>(k, newton ? c.P64.p : c.A64.p, w.LU.p, w.Dinv.p);
-        build_sell<double>(k, c.A64.p, c.S64);
+        if (!c.sellFresh) build_sell<double>(k, c.A64.p, c.S64);
     }
     rdst_solver_perf perf{};
     double* W = c.f.W;

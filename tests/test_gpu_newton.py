@@ -48,3 +48,29 @@ def test_exact_jacobian_rejects_what_it_does_not_linearise():
         DeviceCase(cases.bump_case(scheme="HLLCP", **NEWTON))
     with pytest.raises(RdstError):
         DeviceCase(cases.bump_case(reconstruction="OSPRE", **NEWTON))
+
+
+def test_newton_operator_read_back_is_v_j_plus_diagonal(oracle):
+    """The Newton operator is written straight into the sliced-ELL layout of
+    the SpMV; read back in block-CSR it must be V dR/dW + rDeltaTDST V I:
+    A v = V (J v) + rDeltaT V v for a random v, with J v from the adjoint's
+    exact linearisation (rdst_adjoint_apply)."""
+    print("[newton] operator read-back, bump HLLC", flush=True)
+    dc = DeviceCase(cases.bump_case(**NEWTON))
+    dc.iterate(3)
+    dc.flux_residual()
+    dc.jacobian_assemble()
+    rp, cols, vals = dc.bcsr()
+    n = len(rp) - 1
+    rng = np.random.default_rng(5)
+    v = rng.standard_normal((n, 5))
+    Av = np.zeros((n, 5))
+    for i in range(n):
+        for s in range(rp[i], rp[i + 1]):
+            Av[i] += vals[s] @ v[cols[s]]
+    dc.adjoint_linearise()
+    Jv = dc.adjoint_apply(v)
+    V = np.asarray(oracle.geometry(dc.spec.mesh)["V"])
+    rdt = dc.get(10)
+    ref = V[:, None] * Jv + (rdt * V)[:, None] * v
+    assert np.abs(Av - ref).max() <= 1e-10 * np.abs(ref).max()
