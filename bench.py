@@ -172,6 +172,7 @@ def main(argv=None) -> int:
     ap.add_argument("--max-steps", type=int, default=4000)
     ap.add_argument("--max-co", type=float, default=None, help="CFL ramp ceiling (default: the workload's 20)")
     ap.add_argument("--gmres-cycles", type=int, default=None, help="GMRES restart cycles per step (default 1)")
+    ap.add_argument("--krylov-dirs", type=int, default=None, help="GMRES restart length (default 30)")
     ap.add_argument("--turbulence", default=None, help="SARC2 or SAneg: the viscous SA form of the workload")
     ap.add_argument("--jacobian", default="Roe", choices=["Roe", "exact"],
                     help="implicit operator: the reference's Roe Jacobian, or the exact linearisation of the "
@@ -201,6 +202,8 @@ def main(argv=None) -> int:
         extra["max_co"] = args.max_co
     if args.gmres_cycles is not None:
         extra["max_iter"] = args.gmres_cycles
+    if args.krylov_dirs is not None:
+        extra["n_directions"] = args.krylov_dirs
     if args.rel_tol is not None:
         extra["rel_tol"] = args.rel_tol
     if args.initial_co is not None:
