@@ -172,7 +172,7 @@ struct Mesh {
 // is held in LDS by one workgroup
 constexpr int kPartCells = 1024;
 // deepest partition-local level chain kept; deeper partitions are cut in two
-constexpr int kMaxPartLevels = 48;
+constexpr int kMaxPartLevels = 32;
 void mesh_build(Mesh& m, const rdst_mesh_desc* d);
 void build_partitions(Mesh& m, int maxCells);
 void mesh_upload(Mesh& m);
