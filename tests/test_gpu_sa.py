@@ -5,7 +5,7 @@ several), bound/mut/boundary values, and whole Single::iterate steps."""
 import numpy as np
 import pytest
 
-from rhodst_amd import DeviceCase, cases
+from rhodst_amd import DeviceCase, _capi, cases
 
 pytestmark = pytest.mark.gpu
 
@@ -25,6 +25,7 @@ def test_sa_set_and_correct_parity(oracle, model, kw):
     spec = cases.box_sa_case(model, turb_kw=kw)
     oc = oracle.OracleCase(spec)
     dc = DeviceCase(spec)
+    oc.set_partition(dc.mesh.get_int(_capi.M_PARTITION))  # the SA sweep's block-Jacobi partitions
     for f in (30, 31, 33, 34):
         assert rel(dc.get(f), oc.get(f)) <= 1e-14, f
     oc.flux_residual()
@@ -68,6 +69,7 @@ def test_sa_iterate_parity(oracle, model):
     spec = cases.box_sa_case(model)
     oc = oracle.OracleCase(spec)
     dc = DeviceCase(spec)
+    oc.set_partition(dc.mesh.get_int(_capi.M_PARTITION))  # the SA sweep's block-Jacobi partitions
     r0 = oc.iterate(15)
     r1 = dc.iterate(15)
     assert np.all(np.isfinite(r1))
@@ -87,6 +89,7 @@ def test_forces_parity(oracle, kind):
         patch = spec.mesh.patch_index("bottom")
     oc = oracle.OracleCase(spec)
     dc = DeviceCase(spec)
+    oc.set_partition(dc.mesh.get_int(_capi.M_PARTITION))  # the SA sweep's block-Jacobi partitions
     oc.iterate(3)
     dc.iterate(3)
     fo = oc.forces(patch, (1, 0, 0), (0, 1, 0), 1e5 if kind != "step" else 1.0, 1.2, 100.0, 1e-4)
