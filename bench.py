@@ -11,8 +11,9 @@ HBM.  The line reports the face-flux kernel throughput (Gfaces/s) as the
 value, the per-step time of the FP64 and mixed-precision paths, the roofline
 of the face-flux kernel (algorithmic bytes / HIP-event kernel time), the time
 to a 1e-8 drop of |R|_2 of the mixed and the FP64 path with the exact-Jacobian
-(Newton) operator, and the CPU oracle timed on a bounded sample of the same
-workload.  --turbulence SARC2 runs the viscous Spalart-Allmaras form of the
+(Newton) operator (Roe-ILU(0) preconditioner, CFL ramp from 2 to 1e4, GMRES(30)
+to relTol 1e-2 in at most 4 cycles per step), and the CPU oracle timed on a
+bounded sample of the same workload.  --turbulence SARC2 runs the viscous Spalart-Allmaras form of the
 step (throughput only).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--small] [--skip-converge]
