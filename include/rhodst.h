@@ -181,7 +181,8 @@ typedef struct rdst_settings {
  * physical step of dual2ndDST (second-order backward differences in physical
  * time, dual2ndDST.C:377-411) is converged by inner pseudo-time iterations
  * until steadyCheck (steadyCheck.C:69-98) declares them converged; the
- * physical time step follows setTransientDeltaT (timeDST.C:275-314). */
+ * physical time step follows setTransientDeltaT (timeDST.C:275-314).  DUAL
+ * runs with precision FP64, the Roe Jacobian and no turbulence model. */
 enum rdst_time_integration {
     RDST_TIME_SINGLE = 0,
     RDST_TIME_DUAL = 1

@@ -245,6 +245,12 @@ int rdst_case_create(rdst_mesh* mesh, const rdst_thermo* thermo, const rdst_bc* 
         if (s.time_integration == RDST_TIME_DUAL) {
             if (s.jacobian != RDST_JACOBIAN_ROE)
                 throw ArgError("rdst_case_create: dual time stepping is provided with the Roe Jacobian");
+            // the mixed form solves for the correction from zero, so its
+            // normalised initial residual is ~1 on every inner iteration and
+            // steadyCheck (which reads it) could never see convergence
+            if (s.precision != RDST_PREC_FP64)
+                throw ArgError("rdst_case_create: dual time stepping runs in FP64 (the inner convergence test reads the "
+                               "W-form initial residual)");
             if (s.max_inner_iter < 1 || !(s.delta_t >= 0) || !(s.min_residual >= 0))
                 throw ArgError("rdst_case_create: bad dual-time controls (maxInnerIter >= 1, deltaT >= 0)");
         }

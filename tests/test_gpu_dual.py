@@ -59,3 +59,13 @@ def test_dual_entry_point_errors():
         dual.iterate(1)
     with pytest.raises(RdstError):
         DeviceCase(cases.forward_step_case(nx=30, ny=10, scheme="AUSM+", time_integration="Dual", jacobian="exact"))
+
+
+def test_dual_refuses_mixed_precision():
+    """The mixed form solves for the correction from zero: its normalised
+    initial residual is ~1 on every inner iteration, which steadyCheck cannot
+    use (the inner loop would run to maxInnerIter), so a Dual case is FP64."""
+    s32 = _step_spec()
+    s32.settings.precision = 1  # RDST_PREC_MIXED
+    with pytest.raises(RdstError):
+        DeviceCase(s32)
