@@ -437,6 +437,20 @@ int rdst_adjoint_solve(rdst_case* c, int patch, const double* dir, double q, con
 int rdst_forces(rdst_case* c, int patch, const double* drag_dir, const double* lift_dir,
                 double p_ref, double rho_inf, double mag_u_inf, double a_ref, double* out);
 
+/* rdst_forces without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory during
+ * the call (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments, kernels and result as
+ * rdst_forces; the block partials and sums live in a static device array of
+ * 6 x 1024 + 8 doubles (49 216 bytes) that each GPU holds until the process
+ * ends, found with hipGetSymbolAddress by rdst_init (or by the first call on
+ * a device).  The partials are grid-stride sums over at most 1024 blocks, so
+ * every patch size fits and none is rejected.  All calls on one GPU share
+ * that array: calls that overlap on one GPU (two host threads, or two cases
+ * on different streams) race and must be serialised by the caller. */
+int rdst_forces_noalloc(rdst_case* c, int patch, const double* drag_dir, const double* lift_dir,
+                        double p_ref, double rho_inf, double mag_u_inf, double a_ref, double* out);
+
 /* device time of the last rdst_flux_residual face kernel(s), in ms (HIP events) */
 int rdst_last_flux_ms(rdst_case* c, double* ms);
 /* run the face-flux + residual kernel n times back to back on the case's

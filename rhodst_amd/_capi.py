@@ -28,7 +28,7 @@ EXPORTS = ["rdst_last_error", "rdst_version", "rdst_init", "rdst_mesh_create", "
            "rdst_set_courant", "rdst_iterate", "rdst_dual_step", "rdst_time_info", "rdst_case_set_turbulence", "rdst_turbulence_correct",
            "rdst_adjoint_linearise", "rdst_adjoint_apply", "rdst_adjoint_residual",
            "rdst_adjoint_objective_gradient", "rdst_adjoint_solve",
-           "rdst_forces", "rdst_last_flux_ms", "rdst_time_flux"]
+           "rdst_forces", "rdst_forces_noalloc", "rdst_last_flux_ms", "rdst_time_flux"]
 
 _lib = None
 
@@ -69,6 +69,7 @@ def lib():
     L.rdst_time_info.argtypes = [C.c_void_p, P(C.c_double)]
     L.rdst_forces.argtypes = [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double), C.c_double, C.c_double,
                               C.c_double, C.c_double, P(C.c_double)]
+    L.rdst_forces_noalloc.argtypes = L.rdst_forces.argtypes
     L.rdst_case_set_turbulence.argtypes = [C.c_void_p, P(rdst_turbulence), P(rdst_scalar_bc), P(C.c_double),
                                            P(C.c_double)]
     L.rdst_turbulence_correct.argtypes = [C.c_void_p, P(rdst_solver_perf)]

@@ -123,6 +123,7 @@ int rdst_init(int device) {
         if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw ArgError("no HIP device visible", RDST_ERR_NO_DEVICE);
         if (device < 0 || device >= n) throw ArgError("rdst_init: device index out of range");
         RDST_HIP(hipSetDevice(device));
+        forces_scratch_setup(device);
     })
 }
 
@@ -600,7 +601,20 @@ int rdst_forces(rdst_case* cc, int patch, const double* drag_dir, const double* 
         if (!c || !drag_dir || !lift_dir || !out) throw ArgError("rdst_forces: null argument");
         const Mesh& m = *c->mesh;
         if (patch < 0 || patch >= m.nPatches) throw ArgError("rdst_forces: patch index out of range");
-        forces(*c, patch, drag_dir, lift_dir, p_ref, rho_inf, mag_u_inf, a_ref, out);
+        forces(*c, patch, drag_dir, lift_dir, p_ref, rho_inf, mag_u_inf, a_ref, out, nullptr);
+    })
+}
+
+int rdst_forces_noalloc(rdst_case* cc, int patch, const double* drag_dir, const double* lift_dir, double p_ref,
+                        double rho_inf, double mag_u_inf, double a_ref, double* out) {
+    RDST_API_BODY({
+        Case* c = reinterpret_cast<Case*>(cc);
+        if (!c || !drag_dir || !lift_dir || !out) throw ArgError("rdst_forces_noalloc: null argument");
+        const Mesh& m = *c->mesh;
+        if (patch < 0 || patch >= m.nPatches) throw ArgError("rdst_forces_noalloc: patch index out of range");
+        int dev = -1;
+        RDST_HIP(hipGetDevice(&dev));
+        forces(*c, patch, drag_dir, lift_dir, p_ref, rho_inf, mag_u_inf, a_ref, out, forces_scratch(dev));
     })
 }
 

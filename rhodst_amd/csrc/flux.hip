@@ -13,6 +13,8 @@
 // First-order (upwind) reconstruction is the closed form own_=+1 -> owner
 // value, nei_=-1 -> neighbour value (baseFluxDST.C:144-171); boundary faces
 // take the patch value on both sides.
+#include <mutex>
+
 #include "rdst_internal.hpp"
 #include "reduce.hpp"
 #include "fluxfun.hpp"
@@ -437,24 +439,28 @@ __global__ void k_hllcp_sensor(MeshDev m, Fields F) {
 // forceCoeffs on one patch: sum (p_b - pRef) Sf and sum Sf & devRhoReff_b,
 // devRhoReff = -muEff dev(twoSymm(grad U)) from the corrected boundary
 // gradient; one block, fixed strided order (deterministic)
-__global__ __launch_bounds__(RED_BS) void k_forces(MeshDev m, Fields F, int b0, int b1, double pRef, int visc,
-                                                   double* out) {
+// Force integral over the boundary faces [b0, b1) of one patch, first pass:
+// block g sums its grid-stride share of the faces into part[q G + g]
+// (q = pressure force x, y, z, viscous force x, y, z).
+__global__ __launch_bounds__(RED_BS) void k_forces_part(MeshDev m, Fields F, int b0, int b1, double pRef, int visc,
+                                                        double* part) {
     __shared__ double sh[16];
     double a[6] = {0, 0, 0, 0, 0, 0};
-    for (int b = b0 + (int)threadIdx.x; b < b1; b += blockDim.x) {
+    const int G = (int)gridDim.x;
+    for (int b = b0 + (int)(blockIdx.x * blockDim.x + threadIdx.x); b < b1; b += G * (int)blockDim.x) {
         const int k = m.nI + b;
         const double S[3] = {m.Sf[k], m.Sf[m.nF + k], m.Sf[2 * m.nF + k]};
         const double dp = F.bp[b] - pRef;
         for (int d = 0; d < 3; ++d) a[d] += S[d] * dp;
         if (visc) {
-            double G[9];
-            for (int q = 0; q < 9; ++q) G[q] = F.bgradU[(size_t)q * m.nB + b];
-            const double tr = G[0] + G[4] + G[8];
+            double Gu[9];
+            for (int q = 0; q < 9; ++q) Gu[q] = F.bgradU[(size_t)q * m.nB + b];
+            const double tr = Gu[0] + Gu[4] + Gu[8];
             const double muE = F.bmut[b] + F.bmu[b];
             for (int j = 0; j < 3; ++j) {
                 double v = 0.0;
                 for (int i = 0; i < 3; ++i) {
-                    const double ts = G[3 * i + j] + G[3 * j + i];
+                    const double ts = Gu[3 * i + j] + Gu[3 * j + i];
                     v += S[i] * (-muE * (ts - (i == j ? (2.0 / 3.0) * tr : 0.0)));
                 }
                 a[3 + j] += v;
@@ -463,6 +469,15 @@ __global__ __launch_bounds__(RED_BS) void k_forces(MeshDev m, Fields F, int b0, 
     }
     for (int q = 0; q < 6; ++q) {
         const double t = block_sum(a[q], sh);
+        if (threadIdx.x == 0) part[(size_t)q * G + blockIdx.x] = t;
+    }
+}
+
+// second pass: one block sums the G partials of each component in block order
+__global__ __launch_bounds__(RED_BS) void k_forces_final(const double* part, int G, double* out) {
+    __shared__ double sh[16];
+    for (int q = 0; q < 6; ++q) {
+        const double t = reduce_partials(part + (size_t)q * G, G, sh);
         if (threadIdx.x == 0) out[q] = t;
     }
 }
@@ -574,8 +589,42 @@ double flux_time(Case& c, int n) {
     return (double)ms / n;
 }
 
+// Device scratch of the allocation-free force integral (rdst_forces_noalloc):
+// the [6][RED_MAXG] block partials and the six sums.  Each GPU holds its own
+// copy for the life of the process; host code reaches it only through the
+// address hipGetSymbolAddress returns on that device (forces_scratch_setup).
+constexpr size_t kForcesScratch = 6 * (size_t)RED_MAXG + 8;
+static __device__ __align__(256) double scratch[kForcesScratch];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMutex;
+double* g_scratchPtr[kMaxDevices] = {};
+}  // namespace
+
+void forces_scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) throw ArgError("forces scratch: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (g_scratchPtr[device]) return;
+    void* p = nullptr;
+    RDST_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratchPtr[device] = static_cast<double*>(p);
+}
+
+double* forces_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) throw ArgError("forces scratch: device index out of range");
+    {
+        std::lock_guard<std::mutex> lock(g_scratchMutex);
+        if (g_scratchPtr[device]) return g_scratchPtr[device];
+    }
+    forces_scratch_setup(device);
+    return g_scratchPtr[device];
+}
+
+size_t forces_work_size() { return kForcesScratch; }
+
 void forces(Case& c, int patch, const double* dragDir, const double* liftDir, double pRef, double rhoInf, double magUInf,
-            double aRef, double* out) {
+            double aRef, double* out, double* work) {
     const Mesh& m = *c.mesh;
     // boundary faces of the patch are contiguous in the compacted boundary order
     int b0 = -1, b1 = -1;
@@ -588,11 +637,19 @@ void forces(Case& c, int patch, const double* dragDir, const double* liftDir, do
     if (b0 >= 0) {
         if (c.st.viscous) grad_u_launch(c);
         DBuf<double> d;
-        d.alloc(6);
-        hipLaunchKernelGGL(k_forces, dim3(1), dim3(RED_BS), 0, c.stream, c.meshDev(), c.f, b0, b1, pRef,
-                           c.st.viscous, d.p);
+        if (!work) {
+            d.alloc(kForcesScratch);
+            work = d.p;
+        }
+        const int G = reduce_grid(b1 - b0);
+        if (G > RED_MAXG) throw ArgError("forces: more reduction blocks than the scratch holds");
+        double* res = work + 6 * (size_t)RED_MAXG;
+        hipLaunchKernelGGL(k_forces_part, dim3(G), dim3(RED_BS), 0, c.stream, c.meshDev(), c.f, b0, b1, pRef,
+                           c.st.viscous, work);
         RDST_HIP(hipGetLastError());
-        RDST_HIP(hipMemcpyAsync(F, d.p, sizeof(F), hipMemcpyDeviceToHost, c.stream));
+        hipLaunchKernelGGL(k_forces_final, dim3(1), dim3(RED_BS), 0, c.stream, work, G, res);
+        RDST_HIP(hipGetLastError());
+        RDST_HIP(hipMemcpyAsync(F, res, sizeof(F), hipMemcpyDeviceToHost, c.stream));
         RDST_HIP(hipStreamSynchronize(c.stream));
     }
     const double Ft[3] = {F[0] + F[3], F[1] + F[4], F[2] + F[5]};

@@ -328,8 +328,15 @@ void set_steady_state_deltaT(Case& c);
 void set_state(Case& c, const double* p, const double* T, const double* U);
 void solver_free(Case& c);
 double flux_time(Case& c, int n);
+// force integral of one patch; work: device scratch of forces_work_size()
+// doubles, or nullptr to allocate it for the call
 void forces(Case& c, int patch, const double* dragDir, const double* liftDir, double pRef, double rhoInf,
-            double magUInf, double aRef, double* out);
+            double magUInf, double aRef, double* out, double* work);
+size_t forces_work_size();
+// the per-device address of the force integral's static device scratch,
+// looked up once per device with that device current
+void forces_scratch_setup(int device);
+double* forces_scratch(int device);
 
 void sa_set(Case& c, const rdst_turbulence* t, const rdst_scalar_bc* bcs, const double* wallDist,
             const double* nuTilda);
