@@ -2843,6 +2843,12 @@ int orc_case_put(void* h, int field, const double* in) {
             case RDST_F_RHOU: for (int i = 0; i < n; ++i) c.rhoU[i] = v3(in[3 * i], in[3 * i + 1], in[3 * i + 2]); break;
             case RDST_F_W: c.W.assign(in, in + 5 * (size_t)n); break;
             case RDST_F_RDELTAT: c.rDeltaT.assign(in, in + n); break;
+            // scripted inputs of setCourantDST (timeDST.C:316-414): the
+            // residual R_, the time-control state and the previous rho, e
+            case RDST_F_R: c.R.assign(in, in + 5 * (size_t)n); break;
+            case RDST_F_COURANT: c.courant.assign(in, in + n); break;
+            case RDST_F_RHO_OLD: c.rhoOld.assign(in, in + n); break;
+            case RDST_F_E_OLD: c.eOld.assign(in, in + n); break;
             case RDST_F_NU_TILDA:
                 if (!c.tb.on) throw std::runtime_error("no turbulence model");
                 c.tb.nut.assign(in, in + n);
