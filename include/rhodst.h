@@ -282,7 +282,9 @@ enum rdst_mesh_array {
     RDST_M_ILU_LEVEL = 8,   /* [n_cells] level of the lower-triangular sweep */
     RDST_M_BFACE_PATCH = 9, /* [n_bfaces] patch index of each non-empty boundary face */
     RDST_M_CELL_COLOUR = 10, /* [n_cells] */
-    RDST_M_PARTITION = 11   /* [n_cells] partition of each cell (partitioned ILU) */
+    RDST_M_PARTITION = 11,  /* [n_cells] partition of each cell (partitioned ILU) */
+    RDST_M_SA_PARTITION = 12 /* [n_cells] partition of the nuTilda Gauss-Seidel sweep (the RCB boxes of at
+                                most 1024 cells before the ILU's level cut; one box = the serial sweep) */
 };
 
 /* double arrays of a mesh */
@@ -321,6 +323,26 @@ int rdst_init(int device);
  * on the mesh uploads it to the GPU.  Replaces the fvMesh the reference's
  * operators read through mesh_. */
 int rdst_mesh_create(const rdst_mesh_desc* desc, rdst_mesh** out);
+
+/* The finite-volume geometry of the host's fvMesh, in polyMesh face order
+ * (internal faces, then the faces of every patch; entries of empty-patch
+ * faces are ignored): what the reference's operators read through mesh_
+ * (Sf, magSf, weights, deltaCoeffs: Roe.C:195-199; V: timeDST.C:237). */
+typedef struct rdst_mesh_geometry {
+    const double* Sf;           /* [n_faces][3] face area vectors */
+    const double* mag_sf;       /* [n_faces] |Sf|, or NULL to form it from Sf */
+    const double* Cf;           /* [n_faces][3] face centres */
+    const double* C;            /* [n_cells][3] cell centres */
+    const double* V;            /* [n_cells] cell volumes */
+    const double* weights;      /* [n_faces] interpolation weights (boundary entries ignored: 1) */
+    const double* delta_coeffs; /* [n_faces] deltaCoeffs (boundary: 1/|Cf - C|) */
+} rdst_mesh_geometry;
+
+/* rdst_mesh_create with the caller's geometry kept bit for bit instead of
+ * the one computed from the points (the non-orthogonal correction vectors
+ * n - d deltaCoeffs are formed from it).  The points still define the wall
+ * triangles of rdst_mesh_wall_distance. */
+int rdst_mesh_create_geometry(const rdst_mesh_desc* desc, const rdst_mesh_geometry* geom, rdst_mesh** out);
 int rdst_mesh_destroy(rdst_mesh* mesh);
 int rdst_mesh_info(const rdst_mesh* mesh, int64_t* info /* [RDST_I_COUNT] */);
 int rdst_mesh_get_int(const rdst_mesh* mesh, int which, int32_t* out);
@@ -356,6 +378,27 @@ int rdst_jacobian_assemble(rdst_case* c);
 /* fvBlockMatrixDST::solve: W mode solves A.W = A.W + s; deltaW mode
  * solves A.dW = s then W += dW */
 int rdst_block_solve(rdst_case* c, rdst_solver_perf* perf);
+
+/* The linear solve on a system the caller assembled, in foam-extend's LDU
+ * order: what fvBlockMatrixDST<vector5>::solve hands to
+ * BlockLduSolver<vector5>::New(name, A, dict)->solve(x, b)
+ * (fvBlockMatrixDST.C:112-123, :136-144; the GMRES/BiCGStab + ILUC0 solvers
+ * of Vector5typedefs.C:1596-1711).  diag [n_cells][25], upper and lower
+ * [n_internal_faces][25] (tensor5 row-major, entry (r,c) at 5r+c) in
+ * polyMesh internal-face order: the owner row holds upper x x_neighbour,
+ * the neighbour row lower x x_owner (foam's LDU convention).  b and x are
+ * [n_cells][5] (vector5).  The case's settings choose the solver,
+ * preconditioner, nDirections, tolerances and precision (mixed: FP32 ILU and
+ * Krylov cycles in an FP64 defect correction on this FP64 matrix).  The
+ * coefficients replace the case's assembled matrix.  Host arrays cross
+ * PCIe on every call (the matrix is ~2.8 GB at 2 M cells).
+ *   RDST_LDU_PLAIN:  A x = b from the initial guess x (BlockLduSolver::solve)
+ *   RDST_LDU_W:      A x' = A x + b from x, x holding W and b the source
+ *                    (fvBlockMatrixDST::solve, dependent W)
+ *   RDST_LDU_DELTAW: A d = b from 0, then x += d (dependent deltaW) */
+enum rdst_ldu_mode { RDST_LDU_PLAIN = 0, RDST_LDU_W = 1, RDST_LDU_DELTAW = 2 };
+int rdst_ldu_solve(rdst_case* c, const double* diag, const double* upper, const double* lower, const double* b,
+                   double* x, int mode, rdst_solver_perf* perf);
 /* updateFields.H: primitives from W, bounding, BC updates */
 int rdst_update_fields(rdst_case* c);
 /* timeDST::setCourantDST: ||R||_2, CFL ramp, setSteadyStateDeltaT;
@@ -374,7 +417,15 @@ int rdst_iterate(rdst_case* c, int n_steps, double* res_norm, rdst_solver_perf* 
  * adjust_time_step is on.  n_inner receives the number of inner iterations,
  * inner_residuals (NULL or max_inner_iter + 1 entries) the mean of the five
  * initial residuals of each inner solve, res_norm (NULL or one value) the
- * |R|_2 of the last setCourantDST.  RDST_ERR_STATE for a SINGLE case. */
+ * |R|_2 of the last setCourantDST.  RDST_ERR_STATE for a SINGLE case.
+ * When the call fails part-way, the time, deltaT0 and time index go back to
+ * their values before the call and the case has no valid state until
+ * rdst_case_set_state (later calls return RDST_ERR_STATE).  An FP64 case
+ * with dependent deltaW starts each inner solve from zero, as the reference
+ * does, so the normalised initial residuals steadyCheck reads stay near 1
+ * (a flat history, not the decrease the W form shows); dependent W is the
+ * usual choice with Dual.  Mixed precision is refused for the same reason
+ * (its correction form also starts from zero). */
 int rdst_dual_step(rdst_case* c, int32_t* n_inner, double* inner_residuals, double* res_norm);
 
 /* time, deltaT, deltaT0 and the physical courant number of the case */

@@ -96,6 +96,13 @@ struct Mesh {
     std::vector<double> dc;   // deltaCoeffs
     std::vector<V3> corr;     // non-orthogonal correction vectors, internal faces
     std::vector<std::vector<int>> cellCells;
+    // faces of each cell in the order the serial face loops reach it
+    // (internal faces ascending, then the boundary faces): entry k + 1 where
+    // the cell is the owner of surface-field face k (every boundary face),
+    // -(k + 1) where it is the neighbour.  A cell-parallel gather over this
+    // list adds the same terms in the same order as the serial scatter, so
+    // the OpenMP build of the face loop gives bit-identical sums.
+    std::vector<int> cfStart, cfIdx;
 };
 
 // primitiveMesh::makeFaceCentresAndAreas (foam-extend primitiveMeshFaceCentresAndAreas.C)
@@ -220,6 +227,19 @@ static void buildMesh(Mesh& m, const rdst_mesh_desc* d) {
         V3 delta = m.Cf[f] - m.C[m.bCell[b]];
         m.dc[m.nInternal + b] = 1.0 / mag(delta);
     }
+    m.cfStart.assign(m.nCells + 1, 0);
+    for (int f = 0; f < m.nInternal; ++f) { m.cfStart[m.owner[f] + 1]++; m.cfStart[m.neighbour[f] + 1]++; }
+    for (int b = 0; b < m.nB; ++b) m.cfStart[m.bCell[b] + 1]++;
+    for (int i = 0; i < m.nCells; ++i) m.cfStart[i + 1] += m.cfStart[i];
+    m.cfIdx.assign(m.cfStart[m.nCells], 0);
+    {
+        std::vector<int> fill(m.cfStart.begin(), m.cfStart.end() - 1);
+        for (int f = 0; f < m.nInternal; ++f) {
+            m.cfIdx[fill[m.owner[f]]++] = f + 1;
+            m.cfIdx[fill[m.neighbour[f]]++] = -(f + 1);
+        }
+        for (int b = 0; b < m.nB; ++b) m.cfIdx[fill[m.bCell[b]]++] = m.nInternal + b + 1;
+    }
     // mesh.cellCells(): neighbours across internal faces
     m.cellCells.assign(m.nCells, std::vector<int>());
     for (int f = 0; f < m.nInternal; ++f) {
@@ -295,6 +315,8 @@ struct Case {
     bool hasState = false, hasMatrix = false;
     // cell partition for the block-Jacobi ILU(0) (empty: one partition)
     std::vector<int> part;
+    // cell partition of the nuTilda Gauss-Seidel sweep (empty: the serial sweep)
+    std::vector<int> saPart;
     Turb tb;
 };
 
@@ -505,6 +527,7 @@ static void makeAux(const Case& c, CellAux& a) {
     int n = m.nCells, nB = m.nB;
     double Cp = c.th.Cp, Cv = c.Cv;
     a.rPsi.resize(n); a.e.resize(n); a.aStar.resize(n); a.c.resize(n); a.gam.resize(n);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) {
         a.rPsi[i] = 1.0 / c.psi[i];
         a.e[i] = c.h[i] - c.p[i] / c.rho[i];                       // AUSMPLDST.C:145
@@ -530,6 +553,7 @@ static void reconstruct(const Mesh& m, const std::vector<T>& cell, const std::ve
                         std::vector<T>& own, std::vector<T>& nei) {
     int nf = m.nInternal + m.nB;
     own.resize(nf); nei.resize(nf);
+#pragma omp parallel for schedule(static)
     for (int f = 0; f < m.nInternal; ++f) { own[f] = cell[m.owner[f]]; nei[f] = cell[m.neighbour[f]]; }
     for (int b = 0; b < m.nB; ++b) { own[m.nInternal + b] = bnd[b]; nei[m.nInternal + b] = bnd[b]; }
 }
@@ -623,6 +647,7 @@ static void linearInterp(const Mesh& m, const std::vector<double>& cell, const s
                          std::vector<double>& out) {
     int nf = m.nInternal + m.nB;
     out.resize(nf);
+#pragma omp parallel for schedule(static)
     for (int f = 0; f < m.nInternal; ++f) {
         double P = cell[m.owner[f]], N = cell[m.neighbour[f]];
         out[f] = m.w[f] * (P - N) + N;
@@ -920,26 +945,31 @@ static void gradU(const Case& c, std::vector<T9>& g, std::vector<T9>& gb) {
     const Mesh& m = c.m;
     int n = m.nCells;
     g.assign(n, T9{});
+    // face values of U (linear interpolation; the boundary value on boundary faces)
+    std::vector<V3> Uface(m.nInternal + m.nB);
+#pragma omp parallel for schedule(static)
     for (int f = 0; f < m.nInternal; ++f) {
-        int o = m.owner[f], nb = m.neighbour[f];
         double w = m.w[f];
-        V3 P = c.U[o], N = c.U[nb];
-        V3 Uf = v3(w * (P.x - N.x) + N.x, w * (P.y - N.y) + N.y, w * (P.z - N.z) + N.z);
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) {
-                double t = cmp(m.Sf[f], i) * cmp(Uf, j);
-                g[o].a[3 * i + j] += t;
-                g[nb].a[3 * i + j] -= t;
-            }
+        V3 P = c.U[m.owner[f]], N = c.U[m.neighbour[f]];
+        Uface[f] = v3(w * (P.x - N.x) + N.x, w * (P.y - N.y) + N.y, w * (P.z - N.z) + N.z);
     }
-    for (int b = 0; b < m.nB; ++b) {
-        int ci = m.bCell[b];
-        V3 sf = m.Sf[m.bFace[b]];
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) g[ci].a[3 * i + j] += cmp(sf, i) * cmp(c.bU[b], j);
-    }
-    for (int ci = 0; ci < n; ++ci)
+    for (int b = 0; b < m.nB; ++b) Uface[m.nInternal + b] = c.bU[b];
+    // Gauss sum in face order (owner +, neighbour -, boundary +), then / V
+#pragma omp parallel for schedule(static)
+    for (int ci = 0; ci < n; ++ci) {
+        for (int q = m.cfStart[ci]; q < m.cfStart[ci + 1]; ++q) {
+            const int e = m.cfIdx[q];
+            const int k = e > 0 ? e - 1 : -e - 1;
+            const V3 sf = SfOf(m, k), Uf = Uface[k];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    double t = cmp(sf, i) * cmp(Uf, j);
+                    if (e > 0) g[ci].a[3 * i + j] += t;
+                    else g[ci].a[3 * i + j] -= t;
+                }
+        }
         for (int k = 0; k < 9; ++k) g[ci].a[k] /= m.V[ci];
+    }
     gb.assign(m.nB, T9{});
     for (int b = 0; b < m.nB; ++b) {
         int f = m.bFace[b], ci = m.bCell[b];
@@ -982,10 +1012,16 @@ static void viscousTerms(Case& c) {
     std::vector<double> muEff, bmuEff;
     muAlphaEff(c, muEff, bmuEff, nullptr, nullptr);
     std::vector<T9> tau(n), btau(m.nB);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) tau[i] = tauMC(muEff[i], g[i]);
     for (int b = 0; b < m.nB; ++b) btau[b] = tauMC(bmuEff[b], gb[b]);
     std::vector<V3> lap(n, v3(0, 0, 0)), dtau(n, v3(0, 0, 0));
     std::vector<double> sdu(n, 0.0);
+    // face terms, then summed per cell in face order
+    const int nf = m.nInternal + m.nB;
+    std::vector<V3> lapFace(nf), tauFace(nf);
+    std::vector<double> sdFace(nf);
+#pragma omp parallel for schedule(static)
     for (int f = 0; f < m.nInternal; ++f) {
         int o = m.owner[f], nb = m.neighbour[f];
         double w = m.w[f];
@@ -998,19 +1034,30 @@ static void viscousTerms(Case& c) {
         V3 sn = m.dc[f] * (c.U[nb] - c.U[o]) + dotVT(m.corr[f], gf);
         V3 lapF = (muf * m.magSf[f]) * sn;
         V3 tauF = dotVT(m.Sf[f], tf);
-        double sd = dot(lapF + tauF, c.Up[f]);
-        lap[o] = lap[o] + lapF; lap[nb] = lap[nb] - lapF;
-        dtau[o] = dtau[o] + tauF; dtau[nb] = dtau[nb] - tauF;
-        sdu[o] += sd; sdu[nb] -= sd;
+        lapFace[f] = lapF;
+        tauFace[f] = tauF;
+        sdFace[f] = dot(lapF + tauF, c.Up[f]);
     }
     for (int b = 0; b < m.nB; ++b) {
-        int ci = m.bCell[b], f = m.bFace[b], k = m.nInternal + b;
+        int f = m.bFace[b], k = m.nInternal + b;
         V3 lapF = (bmuEff[b] * m.magSf[f]) * snGradU_b(c, b);
         V3 tauF = dotVT(m.Sf[f], btau[b]);
-        double sd = dot(lapF + tauF, c.Up[k]);
-        lap[ci] = lap[ci] + lapF;
-        dtau[ci] = dtau[ci] + tauF;
-        sdu[ci] += sd;
+        lapFace[k] = lapF;
+        tauFace[k] = tauF;
+        sdFace[k] = dot(lapF + tauF, c.Up[k]);
+    }
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        for (int q = m.cfStart[i]; q < m.cfStart[i + 1]; ++q) {
+            const int e = m.cfIdx[q];
+            if (e > 0) {
+                const int k = e - 1;
+                lap[i] = lap[i] + lapFace[k]; dtau[i] = dtau[i] + tauFace[k]; sdu[i] += sdFace[k];
+            } else {
+                const int k = -e - 1;
+                lap[i] = lap[i] - lapFace[k]; dtau[i] = dtau[i] - tauFace[k]; sdu[i] -= sdFace[k];
+            }
+        }
     }
     for (int i = 0; i < n; ++i) {
         double V = m.V[i];
@@ -1022,12 +1069,26 @@ static void viscousTerms(Case& c) {
 }
 
 // computeFluxDST + calcFields (baseFluxDST.C:223-255)
+// the face loop's arrays, kept from call to call (every entry is rewritten
+// by each call; first-touch page faults of fresh arrays otherwise dominate a
+// threaded call)
+struct FluxWork {
+    CellAux a;
+    std::vector<double> rho_o, rho_n, rPsi_o, rPsi_n, e_o, e_n, a_o, a_n, c_o, c_n, gam_f;
+    std::vector<V3> rhoU_o, rhoU_n;
+    std::vector<double> d0, d4;
+    std::vector<V3> d1;
+};
+
 static void computeFlux(Case& c) {
     const Mesh& m = c.m;
-    CellAux a;
+    static FluxWork w;
+    CellAux& a = w.a;
     makeAux(c, a);
-    std::vector<double> rho_o, rho_n, rPsi_o, rPsi_n, e_o, e_n, a_o, a_n, c_o, c_n, gam_f, fH;
-    std::vector<V3> rhoU_o, rhoU_n;
+    std::vector<double>&rho_o = w.rho_o, &rho_n = w.rho_n, &rPsi_o = w.rPsi_o, &rPsi_n = w.rPsi_n, &e_o = w.e_o,
+                       &e_n = w.e_n, &a_o = w.a_o, &a_n = w.a_n, &c_o = w.c_o, &c_n = w.c_n, &gam_f = w.gam_f;
+    std::vector<double> fH;
+    std::vector<V3>&rhoU_o = w.rhoU_o, &rhoU_n = w.rhoU_n;
     if (c.st.reconstruction == RDST_RECON_OSPRE) {
         reconstructOSPRE(m, c.rho, c.brho, rho_o, rho_n);           // reconstruct(rho) OSPRE
         reconstructOSPREV(m, c.rhoU, c.brhoU, rhoU_o, rhoU_n);      // reconstruct(U) OSPREV
@@ -1046,6 +1107,7 @@ static void computeFlux(Case& c) {
     linearInterp(m, a.gam, a.bgam, gam_f);
     if (c.st.flux_scheme == RDST_FLUX_HLLCP) computeHLLCPf(c, fH);
     int nf = m.nInternal + m.nB;
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < nf; ++k) {
         FaceIn in;
         in.rho_o = rho_o[k]; in.rho_n = rho_n[k]; in.rPsi_o = rPsi_o[k]; in.rPsi_n = rPsi_n[k];
@@ -1059,18 +1121,26 @@ static void computeFlux(Case& c) {
     }
     // calcFields: fvc::div = surfaceIntegrate (sum owner +, neighbour -, boundary +) / V
     int n = m.nCells;
-    std::vector<double> d0(n, 0.0), d4(n, 0.0);
-    std::vector<V3> d1(n, v3(0, 0, 0));
-    for (int f = 0; f < m.nInternal; ++f) {
-        int o = m.owner[f], nb = m.neighbour[f];
-        d0[o] += c.phi[f]; d0[nb] -= c.phi[f];
-        d1[o] = d1[o] + c.phiUp[f]; d1[nb] = d1[nb] - c.phiUp[f];
-        d4[o] += c.phiEp[f]; d4[nb] -= c.phiEp[f];
+    std::vector<double>&d0 = w.d0, &d4 = w.d4;
+    std::vector<V3>& d1 = w.d1;
+    d0.resize(n); d4.resize(n); d1.resize(n);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        double s0 = 0.0, s4 = 0.0;
+        V3 s1 = v3(0, 0, 0);
+        for (int q = m.cfStart[i]; q < m.cfStart[i + 1]; ++q) {
+            const int e = m.cfIdx[q];
+            if (e > 0) {
+                const int k = e - 1;
+                s0 += c.phi[k]; s1 = s1 + c.phiUp[k]; s4 += c.phiEp[k];
+            } else {
+                const int k = -e - 1;
+                s0 -= c.phi[k]; s1 = s1 - c.phiUp[k]; s4 -= c.phiEp[k];
+            }
+        }
+        d0[i] = s0; d1[i] = s1; d4[i] = s4;
     }
-    for (int b = 0; b < m.nB; ++b) {
-        int ci = m.bCell[b], k = m.nInternal + b;
-        d0[ci] += c.phi[k]; d1[ci] = d1[ci] + c.phiUp[k]; d4[ci] += c.phiEp[k];
-    }
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) {
         c.R[5 * i] = d0[i] / m.V[i];
         c.R[5 * i + 1] = d1[i].x / m.V[i]; c.R[5 * i + 2] = d1[i].y / m.V[i]; c.R[5 * i + 3] = d1[i].z / m.V[i];
@@ -2438,7 +2508,7 @@ static void saCorrect(Case& c, Perf* perfOut) {
     pf.fin[0] = pf.init[0];
     pf.nIter = 0;
     auto conv = [&]() { return pf.fin[0] < t.tolerance || (t.rel_tol > 0 && pf.fin[0] < t.rel_tol * pf.init[0]); };
-    const std::vector<int>& part = c.part;
+    const std::vector<int>& part = c.saPart;
     auto samePart = [&](int a, int b) { return part.empty() || part[a] == part[b]; };
     if (!conv() || t.min_iter > 0) {
         std::vector<double> x0, bp(n);
@@ -2750,6 +2820,14 @@ int orc_set_partition(void* h, const int32_t* part) {
     ORC_TRY({
         Case& c = *(Case*)h;
         c.part.assign(part, part + c.m.nCells);
+    })
+}
+
+// cell partition of the block-Jacobi symmetric Gauss-Seidel sweep of nuTilda
+int orc_set_sa_partition(void* h, const int32_t* part) {
+    ORC_TRY({
+        Case& c = *(Case*)h;
+        c.saPart.assign(part, part + c.m.nCells);
     })
 }
 

@@ -144,6 +144,25 @@ def iptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_int32))
 
 
+class rdst_mesh_geometry(C.Structure):
+    _fields_ = [("Sf", C.POINTER(C.c_double)), ("mag_sf", C.POINTER(C.c_double)), ("Cf", C.POINTER(C.c_double)),
+                ("C", C.POINTER(C.c_double)), ("V", C.POINTER(C.c_double)), ("weights", C.POINTER(C.c_double)),
+                ("delta_coeffs", C.POINTER(C.c_double))]
+
+
+class MeshGeometry:
+    """Owns the arrays behind an rdst_mesh_geometry: the host fvMesh's Sf,
+    magSf (or None), Cf, C, V, weights and deltaCoeffs in polyMesh face
+    order (internal faces, then every patch's faces)."""
+
+    def __init__(self, Sf, Cf, C_, V, weights, delta_coeffs, mag_sf=None):
+        self.arrays = [np.ascontiguousarray(a, dtype=np.float64) for a in (Sf, Cf, C_, V, weights, delta_coeffs)]
+        self.mag = None if mag_sf is None else np.ascontiguousarray(mag_sf, dtype=np.float64)
+        Sf_, Cf_, C2, V_, w_, dc_ = self.arrays
+        self.geom = rdst_mesh_geometry(Sf=dptr(Sf_), mag_sf=dptr(self.mag) if self.mag is not None else None,
+                                       Cf=dptr(Cf_), C=dptr(C2), V=dptr(V_), weights=dptr(w_), delta_coeffs=dptr(dc_))
+
+
 class MeshDesc:
     """Owns the contiguous arrays behind an rdst_mesh_desc."""
 

@@ -12,7 +12,7 @@ import os
 
 import numpy as np
 
-from ._abi import (MeshDesc, dptr, iptr, make_adjoint_settings, rdst_adjoint_settings, rdst_bc, rdst_mesh_desc,
+from ._abi import (MeshDesc, MeshGeometry, dptr, iptr, rdst_mesh_geometry, make_adjoint_settings, rdst_adjoint_settings, rdst_bc, rdst_mesh_desc,
                    rdst_scalar_bc, rdst_settings, rdst_solver_perf, rdst_thermo, rdst_turbulence)
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "librhodst_amd.so")
@@ -20,11 +20,12 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "lib
 RDST_OK, RDST_ERR_ARG, RDST_ERR_HIP, RDST_ERR_UNKNOWN_NAME, RDST_ERR_STATE, RDST_ERR_NO_DEVICE = range(6)
 
 # every entry point declared in include/rhodst.h
-EXPORTS = ["rdst_last_error", "rdst_version", "rdst_init", "rdst_mesh_create", "rdst_mesh_destroy",
+EXPORTS = ["rdst_last_error", "rdst_version", "rdst_init", "rdst_mesh_create", "rdst_mesh_create_geometry",
+           "rdst_mesh_destroy",
            "rdst_mesh_info", "rdst_mesh_get_int", "rdst_mesh_get_geom", "rdst_mesh_wall_distance",
            "rdst_case_create",
            "rdst_case_destroy", "rdst_case_set_state", "rdst_case_get", "rdst_case_put",
-           "rdst_flux_residual", "rdst_jacobian_assemble", "rdst_block_solve", "rdst_update_fields",
+           "rdst_flux_residual", "rdst_jacobian_assemble", "rdst_block_solve", "rdst_ldu_solve", "rdst_update_fields",
            "rdst_set_courant", "rdst_iterate", "rdst_dual_step", "rdst_time_info", "rdst_case_set_turbulence", "rdst_turbulence_correct",
            "rdst_adjoint_linearise", "rdst_adjoint_apply", "rdst_adjoint_residual",
            "rdst_adjoint_objective_gradient", "rdst_adjoint_solve",
@@ -51,6 +52,7 @@ def lib():
     L.rdst_last_error.restype = C.c_char_p
     L.rdst_init.argtypes = [C.c_int]
     L.rdst_mesh_create.argtypes = [P(rdst_mesh_desc), P(C.c_void_p)]
+    L.rdst_mesh_create_geometry.argtypes = [P(rdst_mesh_desc), P(rdst_mesh_geometry), P(C.c_void_p)]
     L.rdst_mesh_destroy.argtypes = [C.c_void_p]
     L.rdst_mesh_info.argtypes = [C.c_void_p, P(C.c_int64)]
     L.rdst_mesh_get_int.argtypes = [C.c_void_p, C.c_int, P(C.c_int32)]
@@ -63,6 +65,8 @@ def lib():
     for name in ("rdst_flux_residual", "rdst_jacobian_assemble", "rdst_update_fields"):
         getattr(L, name).argtypes = [C.c_void_p]
     L.rdst_block_solve.argtypes = [C.c_void_p, P(rdst_solver_perf)]
+    L.rdst_ldu_solve.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_double),
+                                 P(C.c_double), C.c_int, P(rdst_solver_perf)]
     L.rdst_set_courant.argtypes = [C.c_void_p, P(C.c_double)]
     L.rdst_iterate.argtypes = [C.c_void_p, C.c_int, P(C.c_double), P(rdst_solver_perf)]
     L.rdst_dual_step.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_double), P(C.c_double)]
@@ -103,18 +107,24 @@ def init(device: int = 0):
 INFO_KEYS = ["n_cells", "n_internal", "n_bfaces", "nnzb", "n_colours", "n_levels_l", "n_levels_u", "n_cell_colours",
              "n_parts", "max_part_levels"]
 M_OWNER, M_NEIGHBOUR, M_ROWPTR, M_COLS, M_SLOT_UPPER, M_SLOT_LOWER, M_SLOT_DIAG, M_FACE_COLOUR, M_ILU_LEVEL, \
-    M_BFACE_PATCH, M_CELL_COLOUR, M_PARTITION = range(12)
+    M_BFACE_PATCH, M_CELL_COLOUR, M_PARTITION, M_SA_PARTITION = range(13)
 G_SF, G_MAGSF, G_CF, G_C, G_V, G_WEIGHTS, G_DELTACOEFFS = range(7)
 
 
 class DeviceMesh:
     """A mesh uploaded to the GPU (rdst_mesh_create)."""
 
-    def __init__(self, polymesh):
+    def __init__(self, polymesh, geometry: MeshGeometry | None = None):
+        """geometry: the host fvMesh's own geometry (rdst_mesh_create_geometry),
+        kept bit for bit; None: computed from the points (rdst_mesh_create)."""
         self.polymesh = polymesh
         self._md = MeshDesc(polymesh)
         h = C.c_void_p()
-        check(lib().rdst_mesh_create(C.byref(self._md.desc), C.byref(h)))
+        if geometry is None:
+            check(lib().rdst_mesh_create(C.byref(self._md.desc), C.byref(h)))
+        else:
+            self._geometry = geometry
+            check(lib().rdst_mesh_create_geometry(C.byref(self._md.desc), C.byref(geometry.geom), C.byref(h)))
         self.h = h
         info = (C.c_int64 * 10)()
         check(lib().rdst_mesh_info(self.h, info))
@@ -130,7 +140,7 @@ class DeviceMesh:
         sizes = {M_OWNER: i["n_internal"] + i["n_bfaces"], M_NEIGHBOUR: i["n_internal"], M_ROWPTR: i["n_cells"] + 1,
                  M_COLS: i["nnzb"], M_SLOT_UPPER: i["n_internal"], M_SLOT_LOWER: i["n_internal"],
                  M_SLOT_DIAG: i["n_cells"], M_FACE_COLOUR: i["n_internal"], M_ILU_LEVEL: i["n_cells"],
-                 M_BFACE_PATCH: i["n_bfaces"], M_CELL_COLOUR: i["n_cells"], M_PARTITION: i["n_cells"]}
+                 M_BFACE_PATCH: i["n_bfaces"], M_CELL_COLOUR: i["n_cells"], M_PARTITION: i["n_cells"], M_SA_PARTITION: i["n_cells"]}
         out = np.zeros(sizes[which], dtype=np.int32)
         check(lib().rdst_mesh_get_int(self.h, which, iptr(out)))
         return out
@@ -243,6 +253,20 @@ class DeviceCase:
     def put(self, field, arr):
         a = np.ascontiguousarray(arr, dtype=np.float64)
         check(lib().rdst_case_put(self.h, field, dptr(a)))
+
+    def ldu_solve(self, diag, upper, lower, b, x, mode=0):
+        """rdst_ldu_solve on a caller-assembled LDU system: diag [n,5,5],
+        upper/lower [nI,5,5] (polyMesh internal-face order), b [n,5], x [n,5]
+        initial guess; mode 0 PLAIN (A x = b), 1 W (A x' = A x + b), 2 deltaW
+        (A d = b, x += d).  Returns (x, perf)."""
+        d = np.ascontiguousarray(diag, dtype=np.float64)
+        u = np.ascontiguousarray(upper, dtype=np.float64)
+        l_ = np.ascontiguousarray(lower, dtype=np.float64)
+        bb = np.ascontiguousarray(b, dtype=np.float64)
+        xx = np.ascontiguousarray(x, dtype=np.float64).copy()
+        perf = rdst_solver_perf()
+        check(lib().rdst_ldu_solve(self.h, dptr(d), dptr(u), dptr(l_), dptr(bb), dptr(xx), mode, C.byref(perf)))
+        return xx, perf
 
     def flux_residual(self):
         check(lib().rdst_flux_residual(self.h))

@@ -60,7 +60,29 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_abi_driver(verbose)
     return LIB
+
+
+ROOT = os.path.dirname(HERE)
+DRIVER_SRC = os.path.join(ROOT, "tests", "abi", "abi_driver.cpp")
+DRIVER = os.path.join(ROOT, "tests", "abi", "build", "abi_driver")
+
+
+def build_abi_driver(verbose: bool = False) -> str | None:
+    """The C++ caller of the C-ABI used by the GPU tests: plain g++ against
+    include/rhodst.h, linked to the library just built (rpath to _lib)."""
+    if not os.path.exists(DRIVER_SRC):
+        return None
+    os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"), DRIVER_SRC,
+           "-L", OUT_DIR, "-lrhodst_amd", "-Wl,-rpath,$ORIGIN/../../../rhodst_amd/_lib", "-o", DRIVER]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed on the ABI driver:\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print("built", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -141,6 +141,20 @@ int rdst_mesh_create(const rdst_mesh_desc* desc, rdst_mesh** out) {
     })
 }
 
+int rdst_mesh_create_geometry(const rdst_mesh_desc* desc, const rdst_mesh_geometry* geom, rdst_mesh** out) {
+    RDST_API_BODY({
+        if (!out || !geom) throw ArgError("rdst_mesh_create_geometry: null argument");
+        Mesh* m = new Mesh();
+        try {
+            mesh_build(*m, desc, geom);
+        } catch (...) {
+            delete m;
+            throw;
+        }
+        *out = reinterpret_cast<rdst_mesh*>(m);
+    })
+}
+
 int rdst_mesh_destroy(rdst_mesh* mesh) {
     RDST_API_BODY({ delete reinterpret_cast<Mesh*>(mesh); })
 }
@@ -181,6 +195,7 @@ int rdst_mesh_get_int(const rdst_mesh* mesh, int which, int32_t* out) {
             case RDST_M_BFACE_PATCH: v = &m->bPatch; break;
             case RDST_M_CELL_COLOUR: v = &m->cellColour; break;
             case RDST_M_PARTITION: v = &m->part; break;
+            case RDST_M_SA_PARTITION: v = &m->saPart; break;
             default: throw ArgError("rdst_mesh_get_int: unknown array");
         }
         std::memcpy(out, v->data(), v->size() * sizeof(int32_t));
@@ -452,6 +467,18 @@ int rdst_block_solve(rdst_case* cc, rdst_solver_perf* perf) {
     })
 }
 
+int rdst_ldu_solve(rdst_case* cc, const double* diag, const double* upper, const double* lower, const double* b,
+                   double* x, int mode, rdst_solver_perf* perf) {
+    RDST_API_BODY({
+        Case* c = reinterpret_cast<Case*>(cc);
+        if (!c || !diag || !b || !x || (c->mesh->nI && (!upper || !lower)))
+            throw ArgError("rdst_ldu_solve: null argument");
+        if (mode != RDST_LDU_PLAIN && mode != RDST_LDU_W && mode != RDST_LDU_DELTAW)
+            throw ArgError("rdst_ldu_solve: unknown mode; valid: PLAIN W deltaW", RDST_ERR_UNKNOWN_NAME);
+        ldu_solve(*c, diag, upper, lower, b, x, mode, perf);
+    })
+}
+
 int rdst_update_fields(rdst_case* cc) {
     RDST_API_BODY({
         Case* c = reinterpret_cast<Case*>(cc);
@@ -553,6 +580,23 @@ int rdst_dual_step(rdst_case* cc, int32_t* n_inner, double* inner_residuals, dou
         if (c->st.time_integration != RDST_TIME_DUAL)
             throw ArgError("rdst_dual_step: the case is not Dual (settings.time_integration)", RDST_ERR_STATE);
         const size_t n5 = 5 * (size_t)c->mesh->nC;
+        // An error inside the step leaves the inner iterations' W behind: the
+        // time control goes back to the start of the step and the case needs
+        // rdst_case_set_state again, so a retry cannot advance time twice or
+        // run on a shifted BDF2 history.
+        const double time0 = c->time, deltaT00 = c->deltaT0, deltaTSave0 = c->deltaTSave;
+        const int timeIndex0 = c->timeIndex;
+        struct Rollback {
+            Case* c;
+            double time, deltaT0, deltaTSave;
+            int timeIndex;
+            bool armed = true;
+            ~Rollback() {
+                if (!armed) return;
+                c->time = time; c->deltaT0 = deltaT0; c->deltaTSave = deltaTSave; c->timeIndex = timeIndex;
+                c->hasState = false;
+            }
+        } rollback{c, time0, deltaT00, deltaTSave0, timeIndex0};
         // runTime++: time, deltaT0 = the previous deltaT, old time levels
         c->deltaT0 = c->deltaTSave;
         c->deltaTSave = c->deltaT;
@@ -580,6 +624,7 @@ int rdst_dual_step(rdst_case* cc, int32_t* n_inner, double* inner_residuals, dou
         if (n_inner) *n_inner = k;
         if (res_norm) *res_norm = r;
         RDST_HIP(hipStreamSynchronize(c->stream));
+        rollback.armed = false;
     })
 }
 

@@ -180,9 +180,60 @@ static void split_deep_partitions(Mesh& m, int maxLevels) {
     }
 }
 
+// Rows of each partition in (partition-local level, cell) order, the level
+// offsets, and per block-CSR slot the partition-local column (-1 outside).
+static void order_partitions(const Mesh& m, const std::vector<int32_t>& part, int nParts, PartSet& ps) {
+    const int n = m.nC;
+    std::vector<std::vector<int>> cellsOf(nParts);
+    for (int c = 0; c < n; ++c) cellsOf[part[c]].push_back(c);
+    std::vector<int32_t> lev(n, 0), local(n, -1);
+    ps.nParts = nParts;
+    ps.rowStart.assign(nParts + 1, 0);
+    ps.levStart.assign(nParts + 1, 0);
+    ps.rows.clear();
+    ps.levPtr.clear();
+    ps.maxRows = 0;
+    ps.maxLevels = 0;
+    for (int p = 0; p < nParts; ++p) {
+        const std::vector<int>& cs = cellsOf[p];
+        int nl = 0;
+        for (int c : cs) {
+            int l = 0;
+            for (int s = m.rowPtr[c]; s < m.slotDiag[c]; ++s)
+                if (part[m.cols[s]] == p) l = std::max(l, lev[m.cols[s]] + 1);
+            lev[c] = l;
+            nl = std::max(nl, l + 1);
+        }
+        std::vector<int> order(cs.begin(), cs.end());
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return lev[a] < lev[b]; });
+        ps.rowStart[p + 1] = ps.rowStart[p] + (int)order.size();
+        for (size_t k = 0; k < order.size(); ++k) {
+            local[order[k]] = (int)k;
+            ps.rows.push_back(order[k]);
+        }
+        // level offsets (absolute positions in rows)
+        int pos = ps.rowStart[p];
+        for (int l = 0; l < nl; ++l) {
+            ps.levPtr.push_back(pos);
+            while (pos < ps.rowStart[p + 1] && lev[ps.rows[pos]] == l) ++pos;
+        }
+        ps.levPtr.push_back(ps.rowStart[p + 1]);
+        ps.levStart[p + 1] = (int)ps.levPtr.size();
+        ps.maxRows = std::max(ps.maxRows, (int)order.size());
+        ps.maxLevels = std::max(ps.maxLevels, nl);
+    }
+    ps.localCol.assign(m.nnzb, -1);
+    for (int c = 0; c < n; ++c)
+        for (int s = m.rowPtr[c]; s < m.rowPtr[c + 1]; ++s)
+            if (part[m.cols[s]] == part[c]) ps.localCol[s] = local[m.cols[s]];
+}
+
 // Partitions for the block-Jacobi ILU(0): couplings between partitions are
 // dropped, inside a partition the factor is the natural-order ILU(0) of the
-// partition's diagonal block, scheduled by local levels.
+// partition's diagonal block, scheduled by local levels.  The RCB boxes as
+// they come (at most maxCells cells, no level cut) also drive the
+// Spalart-Allmaras Gauss-Seidel sweep: a mesh of at most maxCells cells is
+// one box, i.e. the serial natural-order sweep.
 void build_partitions(Mesh& m, int maxCells) {
     const int n = m.nC;
     m.part.assign(n, 0);
@@ -190,54 +241,23 @@ void build_partitions(Mesh& m, int maxCells) {
     std::iota(idx.begin(), idx.end(), 0);
     m.nParts = 0;
     rcb(m.C, idx, 0, n, maxCells, m.part, m.nParts);
+    m.saPart = m.part;
+    order_partitions(m, m.saPart, m.nParts, m.sa);
     split_deep_partitions(m, kMaxPartLevels);
-    // cells of each partition in increasing index
-    std::vector<std::vector<int>> cellsOf(m.nParts);
-    for (int c = 0; c < n; ++c) cellsOf[m.part[c]].push_back(c);
-    std::vector<int32_t> lev(n, 0), local(n, -1);
-    m.partRowStart.assign(m.nParts + 1, 0);
-    m.partLevStart.assign(m.nParts + 1, 0);
-    m.partRows.clear();
-    m.partLevPtr.clear();
-    m.maxPartRows = 0;
-    m.maxPartLevels = 0;
-    for (int p = 0; p < m.nParts; ++p) {
-        const std::vector<int>& cs = cellsOf[p];
-        int nl = 0;
-        for (int c : cs) {
-            int l = 0;
-            for (int s = m.rowPtr[c]; s < m.slotDiag[c]; ++s)
-                if (m.part[m.cols[s]] == p) l = std::max(l, lev[m.cols[s]] + 1);
-            lev[c] = l;
-            nl = std::max(nl, l + 1);
-        }
-        std::vector<int> order(cs.begin(), cs.end());
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return lev[a] < lev[b]; });
-        m.partRowStart[p + 1] = m.partRowStart[p] + (int)order.size();
-        for (size_t k = 0; k < order.size(); ++k) {
-            local[order[k]] = (int)k;
-            m.partRows.push_back(order[k]);
-        }
-        // level offsets (absolute positions in partRows)
-        int pos = m.partRowStart[p];
-        for (int l = 0; l < nl; ++l) {
-            m.partLevPtr.push_back(pos);
-            while (pos < m.partRowStart[p + 1] && lev[m.partRows[pos]] == l) ++pos;
-        }
-        m.partLevPtr.push_back(m.partRowStart[p + 1]);
-        m.partLevStart[p + 1] = (int)m.partLevPtr.size();
-        m.maxPartRows = std::max(m.maxPartRows, (int)order.size());
-        m.maxPartLevels = std::max(m.maxPartLevels, nl);
-    }
-    m.localCol.assign(m.nnzb, -1);
-    for (int c = 0; c < n; ++c)
-        for (int s = m.rowPtr[c]; s < m.rowPtr[c + 1]; ++s)
-            if (m.part[m.cols[s]] == m.part[c]) m.localCol[s] = local[m.cols[s]];
+    PartSet ps;
+    order_partitions(m, m.part, m.nParts, ps);
+    m.partRowStart.swap(ps.rowStart);
+    m.partRows.swap(ps.rows);
+    m.partLevStart.swap(ps.levStart);
+    m.partLevPtr.swap(ps.levPtr);
+    m.localCol.swap(ps.localCol);
+    m.maxPartRows = ps.maxRows;
+    m.maxPartLevels = ps.maxLevels;
     build_pilu_stream(m, false, m.pf);
     build_pilu_stream(m, true, m.pb);
 }
 
-void mesh_build(Mesh& m, const rdst_mesh_desc* d) {
+void mesh_build(Mesh& m, const rdst_mesh_desc* d, const rdst_mesh_geometry* g) {
     if (!d || d->n_cells <= 0 || d->n_faces <= 0 || d->n_internal_faces < 0 || d->n_internal_faces > d->n_faces)
         throw ArgError("rdst_mesh_create: bad mesh sizes");
     if (!d->points || !d->face_offsets || !d->face_points || !d->owner || (d->n_internal_faces && !d->neighbour))
@@ -295,7 +315,10 @@ void mesh_build(Mesh& m, const rdst_mesh_desc* d) {
         }
         fmag[f] = norm(fSf[f]);
     }
-    // --- wall triangles (the surface the wall distance is measured to)
+    // --- wall triangles (the surface the wall distance is measured to): each
+    // wall face fanned from the mean of its points, one triangle per edge, so
+    // a warped quad is followed on both of its diagonals (a fan from one
+    // corner can pass a first-cell centre at many times its height)
     for (int pi = 0; pi < d->n_patches; ++pi) {
         const rdst_patch& pa = d->patches[pi];
         if (pa.type != RDST_PATCH_WALL) continue;
@@ -303,9 +326,15 @@ void mesh_build(Mesh& m, const rdst_mesh_desc* d) {
         for (int f = pa.start; f < pa.start + pa.size; ++f) {
             const int s = d->face_offsets[f], n = d->face_offsets[f + 1] - s;
             const int32_t* fp = d->face_points + s;
-            for (int k = 1; k + 1 < n; ++k)
-                for (int q : {fp[0], fp[k], fp[k + 1]})
+            double c[3] = {d->points[3 * fp[0]], d->points[3 * fp[0] + 1], d->points[3 * fp[0] + 2]};
+            for (int k = 1; k < n; ++k)
+                for (int x = 0; x < 3; ++x) c[x] = c[x] + d->points[3 * fp[k] + x];
+            for (int x = 0; x < 3; ++x) c[x] = c[x] / n;
+            for (int k = 0; k < n; ++k) {
+                for (int x = 0; x < 3; ++x) m.wallTri.push_back(c[x]);
+                for (int q : {fp[k], fp[(k + 1) % n]})
                     for (int x = 0; x < 3; ++x) m.wallTri.push_back(d->points[3 * q + x]);
+            }
         }
     }
     // --- cell centres and volumes
@@ -371,11 +400,36 @@ void mesh_build(Mesh& m, const rdst_mesh_desc* d) {
         P3 delta = sub(fCf[f], cC[d->owner[f]]);
         m.dc[nInt + b] = 1.0 / norm(delta);
     }
+    // the host fvMesh's own geometry, kept bit for bit (faces of empty
+    // patches, which have no fvPatch faces, are dropped)
+    if (g) {
+        if (!g->Sf || !g->Cf || !g->C || !g->V || !g->weights || !g->delta_coeffs)
+            throw ArgError("rdst_mesh_create_geometry: null geometry array");
+        for (int k = 0; k < nF; ++k) {
+            const int f = k < nInt ? k : m.bPolyFace[k - nInt];
+            for (int x = 0; x < 3; ++x) {
+                m.Sf[3 * k + x] = g->Sf[3 * (size_t)f + x];
+                m.Cf[3 * k + x] = g->Cf[3 * (size_t)f + x];
+            }
+            m.magSf[k] = g->mag_sf ? g->mag_sf[f] : std::sqrt(m.Sf[3 * k] * m.Sf[3 * k] + m.Sf[3 * k + 1] * m.Sf[3 * k + 1] +
+                                                             m.Sf[3 * k + 2] * m.Sf[3 * k + 2]);
+            m.w[k] = k < nInt ? g->weights[f] : 1.0;
+            m.dc[k] = g->delta_coeffs[f];
+            if (!(m.magSf[k] > 0)) throw ArgError("rdst_mesh_create_geometry: face of zero area");
+        }
+        for (int c = 0; c < nCells; ++c) {
+            for (int x = 0; x < 3; ++x) m.C[3 * (size_t)c + x] = g->C[3 * (size_t)c + x];
+            m.V[c] = g->V[c];
+            if (!(m.V[c] > 0)) throw ArgError("rdst_mesh_create_geometry: non-positive cell volume");
+        }
+    }
     // correction vectors n - d deltaCoeffs of the corrected snGrad
     m.corr.resize(3 * (size_t)nInt);
     for (int f = 0; f < nInt; ++f) {
-        P3 delta = sub(cC[d->neighbour[f]], cC[d->owner[f]]);
-        P3 ua = {fSf[f].x / fmag[f], fSf[f].y / fmag[f], fSf[f].z / fmag[f]};
+        const int o = d->owner[f], nb = d->neighbour[f];
+        P3 delta = {m.C[3 * (size_t)nb] - m.C[3 * (size_t)o], m.C[3 * (size_t)nb + 1] - m.C[3 * (size_t)o + 1],
+                    m.C[3 * (size_t)nb + 2] - m.C[3 * (size_t)o + 2]};
+        P3 ua = {m.Sf[3 * f] / m.magSf[f], m.Sf[3 * f + 1] / m.magSf[f], m.Sf[3 * f + 2] / m.magSf[f]};
         P3 cv = sub(ua, scl(m.dc[f], delta));
         m.corr[3 * f] = cv.x; m.corr[3 * f + 1] = cv.y; m.corr[3 * f + 2] = cv.z;
     }
@@ -595,6 +649,11 @@ void mesh_upload(Mesh& m) {
     m.d_partLevStart.upload(m.partLevStart);
     m.d_partLevPtr.upload(m.partLevPtr);
     m.d_localCol.upload(m.localCol);
+    m.sa.d_rowStart.upload(m.sa.rowStart);
+    m.sa.d_rows.upload(m.sa.rows);
+    m.sa.d_levStart.upload(m.sa.levStart);
+    m.sa.d_levPtr.upload(m.sa.levPtr);
+    m.sa.d_localCol.upload(m.sa.localCol);
     for (PiluStream* ps : {&m.pf, &m.pb}) {
         if (ps->kmax == 0) continue;
         ps->d_subStart.upload(ps->subStart);

@@ -104,6 +104,17 @@ struct PiluStream {
     DBuf<int32_t> d_subStart, d_sub, d_ord, d_lc, d_src, d_row, d_jn;
 };
 
+// A set of compact cell partitions in sweep order: rows of partition p at
+// rows[rowStart[p] .. rowStart[p+1]) in (partition-local level, cell) order,
+// level L's rows at rows[levPtr[L] .. levPtr[L+1]) for L in
+// [levStart[p], levStart[p+1] - 1), localCol = partition-local column of
+// each block-CSR slot (-1 outside the partition).
+struct PartSet {
+    int nParts = 0, maxRows = 0, maxLevels = 0;
+    std::vector<int32_t> rowStart, rows, levStart, levPtr, localCol;
+    DBuf<int32_t> d_rowStart, d_rows, d_levStart, d_levPtr, d_localCol;
+};
+
 struct Mesh {
     int nC = 0, nI = 0, nB = 0, nF = 0;  // nF = nI + nB surface-field length
     int nPatches = 0;
@@ -136,6 +147,10 @@ struct Mesh {
     // cell) order, level offsets, and per block the partition-local column
     int nParts = 0, maxPartRows = 0, maxPartLevels = 0;
     std::vector<int32_t> part, partRowStart, partRows, partLevStart, partLevPtr, localCol;
+    // the RCB boxes before the level cut: the partitions of the
+    // Spalart-Allmaras Gauss-Seidel sweep
+    std::vector<int32_t> saPart;
+    PartSet sa;
     // coalesced streams of the partitioned ILU(0) apply (forward: the
     // in-partition L blocks; backward: Dinv and Dinv U): see PiluStream
     PiluStream pf, pb;
@@ -173,7 +188,7 @@ struct Mesh {
 constexpr int kPartCells = 1024;
 // deepest partition-local level chain kept; deeper partitions are cut in two
 constexpr int kMaxPartLevels = 32;
-void mesh_build(Mesh& m, const rdst_mesh_desc* d);
+void mesh_build(Mesh& m, const rdst_mesh_desc* d, const rdst_mesh_geometry* g = nullptr);
 void build_partitions(Mesh& m, int maxCells);
 void mesh_upload(Mesh& m);
 
@@ -291,6 +306,8 @@ struct Case {
     bool hasState = false, hasMatrix = false;
     // solver scratch owned by the solver module
     struct SolverWork* work = nullptr;
+    // device copies of x and b of rdst_ldu_solve (allocated on first use)
+    DBuf<double> lduX, lduB;
     // time-control scalars
     double residual = -1, residualRatio = 1, courantScalar = 1;
     // dual time stepping: old time levels of W [nC][5] and the physical time
@@ -317,6 +334,8 @@ void flux_launch(Case& c, int storeMode);
 void grad_u_launch(Case& c);
 void jacobian_assemble(Case& c);
 void block_solve(Case& c, rdst_solver_perf* perf);
+void ldu_solve(Case& c, const double* diag, const double* upper, const double* lower, const double* b, double* x,
+               int mode, rdst_solver_perf* perf);
 void update_fields(Case& c);
 double set_courant(Case& c);
 // setCourantDST with the inner-iteration residual |lin| (lin != nullptr) or

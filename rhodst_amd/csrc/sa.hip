@@ -664,13 +664,13 @@ void sa_correct(Case& c, rdst_solver_perf* perf) {
     int nIter = 0;
     auto conv = [&]() { return fin < t.tolerance || (t.rel_tol > 0 && fin < t.rel_tol * init); };
     if (!conv() || t.min_iter > 0) {
-        const size_t shmem = 2 * sizeof(double) * (size_t)m.maxPartRows;
+        const size_t shmem = 2 * sizeof(double) * (size_t)m.sa.maxRows;
         double* xa = s.nut;
         double* xb = s.x2;
         do {
-            hipLaunchKernelGGL(k_sa_sgs, dim3(m.nParts), dim3(256), shmem, c.stream, m.d_partRowStart.p,
-                               m.d_partLevStart.p, m.d_partLevPtr.p, m.d_partRows.p, m.d_rowPtr.p, m.d_cols.p,
-                               m.d_slotDiag.p, m.d_localCol.p, s.A.p, s.B, xa, xb);
+            hipLaunchKernelGGL(k_sa_sgs, dim3(m.sa.nParts), dim3(256), shmem, c.stream, m.sa.d_rowStart.p,
+                               m.sa.d_levStart.p, m.sa.d_levPtr.p, m.sa.d_rows.p, m.d_rowPtr.p, m.d_cols.p,
+                               m.d_slotDiag.p, m.sa.d_localCol.p, s.A.p, s.B, xa, xb);
             std::swap(xa, xb);
             hipLaunchKernelGGL(k_sa_resid_partial, dim3(G), dim3(RED_BS), 0, c.stream, n, m.d_rowPtr.p, m.d_cols.p,
                                s.A.p, s.B, xa, part);

@@ -1249,6 +1249,74 @@ void block_solve(Case& c, rdst_solver_perf* perfOut) {
     if (perfOut) *perfOut = perf;
 }
 
+// BlockLduSolver<vector5>::solve(x, b) on a caller-assembled LDU system
+// (fvBlockMatrixDST.C:98-155): diag [nC][25], upper/lower [nI][25] in
+// polyMesh internal-face order (owner row x upper x neighbour value,
+// neighbour row x lower x owner value), b and x [nC][5] on the host.  The
+// coefficients are scattered into the block-CSR slots on the host and
+// uploaded; the case's solver settings select GMRES/BiCGStab, ILU(0) and the
+// precision.  mode PLAIN: A x = b from x; W: A x' = A x + b from x;
+// DELTAW: A d = b from 0, x += d.  The mixed path solves for the correction
+// in FP32 inside the FP64 defect correction on the caller's FP64 matrix.
+void ldu_solve(Case& c, const double* diag, const double* upper, const double* lower, const double* b, double* x,
+               int mode, rdst_solver_perf* perfOut) {
+    Mesh& m = *c.mesh;
+    const size_t nn = 25 * (size_t)m.nnzb, n5 = 5 * (size_t)m.nC;
+    const bool mixed = c.st.precision == RDST_PREC_MIXED;
+    if (mixed && c.st.solver != RDST_SOLVER_GMRES)
+        throw ArgError("mixed precision is provided for the GMRES solver", RDST_ERR_UNKNOWN_NAME);
+    std::vector<double> vals(nn);
+    for (int i = 0; i < m.nC; ++i)
+        std::copy(diag + 25 * (size_t)i, diag + 25 * (size_t)i + 25, vals.begin() + 25 * (size_t)m.slotDiag[i]);
+    for (int f = 0; f < m.nI; ++f) {
+        std::copy(upper + 25 * (size_t)f, upper + 25 * (size_t)f + 25, vals.begin() + 25 * (size_t)m.slotUpper[f]);
+        std::copy(lower + 25 * (size_t)f, lower + 25 * (size_t)f + 25, vals.begin() + 25 * (size_t)m.slotLower[f]);
+    }
+    c.A64.upload(vals);
+    ensure_work(c);
+    SolverWork& w = *c.work;
+    Ctx k{c, m, w, c.stream, w.G};
+    if (c.lduX.n != n5) c.lduX.alloc(n5);
+    if (c.lduB.n != n5) c.lduB.alloc(n5);
+    RDST_HIP(hipMemcpyAsync(c.lduX.p, x, sizeof(double) * n5, hipMemcpyHostToDevice, c.stream));
+    RDST_HIP(hipMemcpyAsync(c.lduB.p, b, sizeof(double) * n5, hipMemcpyHostToDevice, c.stream));
+    c.sellFresh = false;
+    c.hasMatrix = true;
+    if (mixed) {
+        if (c.A32.n != nn) c.A32.alloc(nn);
+        hipLaunchKernelGGL((k_cast<double, float>), dim3(std::min(4096, blocks_for((long)nn, 256))), dim3(256), 0, c.stream,
+                           (long)nn, c.A64.p, c.A32.p);
+        ilu_factor<float>(k, c.A32.p, w.LU32.p, w.Dinv32.p);
+        build_sell<float>(k, c.A32.p, c.S32);
+        c.S64.release();  // the FP64 outer products read the block-CSR A64, never a stale sliced copy
+    } else {
+        ilu_factor<double>(k, c.A64.p, w.LU.p, w.Dinv.p);
+        build_sell<double>(k, c.A64.p, c.S64);
+    }
+    rdst_solver_perf perf{};
+    double* X = c.lduX.p;
+    if (mode == RDST_LDU_PLAIN && !mixed) {
+        perf = (c.st.solver == RDST_SOLVER_GMRES) ? gmres_fp64(k, c.A64.p, X, c.lduB.p) : bicgstab_fp64(k, X, c.lduB.p);
+    } else if (mode == RDST_LDU_PLAIN) {
+        perf = gmres_mixed<double>(k, c.A64.p, c.A32.p, X, c.lduB.p);
+    } else if (mode == RDST_LDU_W && !mixed) {
+        spmv(k, (const double*)c.A64.p, (const double*)X, w.b.p);
+        hipLaunchKernelGGL(k_add, dim3(k.G), dim3(RED_BS), 0, k.s, (long)n5, w.b.p, c.lduB.p);
+        perf = (c.st.solver == RDST_SOLVER_GMRES) ? gmres_fp64(k, c.A64.p, X, w.b.p) : bicgstab_fp64(k, X, w.b.p);
+    } else {
+        // deltaW, and the W form in mixed precision (A x' = A x + b is A (x' - x) = b)
+        double* dW = w.dW.p;
+        RDST_HIP(hipMemsetAsync(dW, 0, sizeof(double) * n5, k.s));
+        if (mixed) perf = gmres_mixed<double>(k, c.A64.p, c.A32.p, dW, c.lduB.p);
+        else perf = (c.st.solver == RDST_SOLVER_GMRES) ? gmres_fp64(k, c.A64.p, dW, c.lduB.p) : bicgstab_fp64(k, dW, c.lduB.p);
+        hipLaunchKernelGGL(k_add, dim3(k.G), dim3(RED_BS), 0, k.s, (long)n5, X, dW);
+    }
+    RDST_HIP(hipGetLastError());
+    RDST_HIP(hipMemcpyAsync(x, X, sizeof(double) * n5, hipMemcpyDeviceToHost, c.stream));
+    RDST_HIP(hipStreamSynchronize(c.stream));
+    if (perfOut) *perfOut = perf;
+}
+
 // ---------------------------------------------------------------- adjoint
 void adjoint_apply(Case& c, bool transpose, const double* x, double* y) {
     if (!c.hasAdjJ) throw ArgError("rdst_adjoint_apply: no linearisation (call rdst_adjoint_linearise)", RDST_ERR_STATE);

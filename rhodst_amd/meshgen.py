@@ -418,20 +418,47 @@ def _stretch(n: int, first: float, total: float) -> np.ndarray:
     return s * (total / s[-1])
 
 
+def naca4_slope(x: np.ndarray, t: float) -> np.ndarray:
+    """d/dx of naca4_thickness (x > 0)."""
+    return 5 * t * (0.2969 * 0.5 / np.sqrt(x) - 0.1260 - 2 * 0.3516 * x + 3 * 0.2843 * x ** 2 - 4 * 0.1036 * x ** 3)
+
+
+def _smoothstep(x):
+    x = np.clip(x, 0.0, 1.0)
+    return x * x * (3.0 - 2.0 * x)
+
+
 def swept_wing(n_wake: int = 48, n_surf: int = 128, n_normal: int = 96, n_span_wing: int = 44,
                n_span_out: int = 16, thickness: float = 0.10, sweep_deg: float = 30.0, taper: float = 0.56,
                semi_span: float = 1.0, root_chord: float = 1.0, far: float = 12.0, first_cell: float = 2.0e-6,
-               n_layers: int = 32, layer_ratio: float = 1.15) -> PolyMesh:
+               n_layers: int = 32, layer_ratio: float = 1.15, normal_blend: float = 0.08, tip_blend: int = 4,
+               tip_first: float | None = None, tip_round: float = 0.08) -> PolyMesh:
     """Structured C-H hex mesh around a semi-span swept, tapered wing.
 
     Each spanwise station is a C-mesh about the local section (analytic
-    symmetric 4-digit thickness, 10 %), wrapped from the downstream far field
-    under the wake, round the section and back over the wake; the wake cut is
-    closed by sharing points.  Beyond the tip the section collapses to zero
-    thickness, so the cells above and below the continued chord plane share
-    faces (internal) and the first station past the tip closes the wing with a
-    wedge cap.  The first n_layers cells off the wall grow at layer_ratio from
-    first_cell (a y+ of about 1 at Re 1.2e7), then stretch to the far field.
+    symmetric 4-digit thickness, 10 %, closed trailing edge), wrapped from the
+    downstream far field under the wake, round the section and back over the
+    wake; the wake cut is closed by sharing points.  Beyond the tip the
+    section is a zero-thickness sheet in the chord plane, so the cells above
+    and below the sheet share faces (internal).  The tip is rounded: over
+    the outer tip_round of the span the section thickness falls as 1 - s^2
+    (s from 0 at span - tip_round to 1 at the tip station), so the wing
+    surface closes at the tip station itself and no cell is a wedge between
+    the surface and the sheet.
+
+    Wall-normal lines leave the section along its outward normal and bend
+    smoothly (over the first normal_blend of the way out) into the straight
+    line to their far-field point, so the boundary-layer cells are
+    orthogonal to the wall; the far-field points are spaced evenly in the
+    section's cosine parameter (a half circle of radius far about the
+    trailing edge), and over the wake their spacing grows from the last
+    section spacing.  The first cell is first_cell high (root chords, a y+
+    of about 1 at Re 1.2e7), n_layers grow at layer_ratio, then the spacing
+    stretches to the far field.  Beyond the tip the sheet has no wall: its
+    normal lines take the tip section's normals (a fan round a virtual
+    leading edge) and the first spacing grows over tip_blend stations to
+    tip_first (default four times the tip chord's leading-edge spacing), so
+    no cell there is thinner than the chordwise spacing it sits on.
     Cells are numbered i (round the C) fastest, then j (off the wall), then k
     (span).  Patches: wing (wall), symmetry (symmetryPlane, z = 0),
     farfield (patch).  Default size 352 x 96 x 60 = 2 027 520 cells.
@@ -443,43 +470,87 @@ def swept_wing(n_wake: int = 48, n_surf: int = 128, n_normal: int = 96, n_span_w
     xs = 0.5 * (1 - np.cos(beta))                   # 0 (LE) -> 1 (TE)
     # wake distribution from TE to the downstream boundary
     xw = 1.0 + _stretch(n_wake, xs[-1] - xs[-2], far)
+    # far-field spacing over the wake: from the last section arc spacing
+    sw = _stretch(n_wake, far * 0.5 * np.pi / n_surf, far)
     # i index: 0..n_wake wake lower (far -> TE), then lower surface TE -> LE,
     # upper surface LE -> TE, wake upper TE -> far
     x_lower = np.concatenate([xw[::-1], xs[::-1][1:]])          # wake(far..TE), surface(TE..LE)
     x_upper = np.concatenate([xs[1:], xw[1:]])                  # surface(LE..TE), wake(TE..far)
     xc = np.concatenate([x_lower, x_upper])                     # ni+1 values
+    bc = np.concatenate([np.full(n_wake + 1, np.pi), beta[::-1][1:], beta[1:], np.full(n_wake, np.pi)])
+    so = np.concatenate([sw[::-1], np.zeros(n_surf), np.zeros(n_surf), sw[1:]])
     side = np.concatenate([-np.ones(len(x_lower)), np.ones(len(x_upper))])
     on_surf = (xc <= 1.0 + 1e-12)
     assert len(xc) == ni + 1
-    # wall-normal distribution: n_layers geometric layers, then stretched to far
-    d_layers = first_cell * layer_ratio ** np.arange(n_layers)
-    s_in = np.concatenate([[0.0], np.cumsum(d_layers)])
-    s_out = _stretch(nj - n_layers, d_layers[-1] * layer_ratio, 1.0)
-    t_norm = np.concatenate([s_in, s_in[-1] + s_out[1:] * (1.0 - s_in[-1])])  # 0..1, nj+1 values
-    # spanwise stations
-    zw = semi_span * (1 - np.cos(np.linspace(0, np.pi / 2, n_span_wing + 1)))   # cluster at the tip
+    # unit outward normals of the full-thickness section (chord-independent);
+    # the leading edge points upstream, the wake lines straight up/down
+    xcs = np.clip(xc, 0.0, 1.0)
+    slope = np.where(xcs > 0, naca4_slope(np.where(xcs > 0, xcs, 1.0), thickness), 0.0)
+    nx_ = np.where(on_surf, -slope, 0.0)
+    ny_ = side.copy()
+    le = on_surf & (xcs == 0.0)
+    nx_[le], ny_[le] = -1.0, 0.0
+    nn = np.hypot(nx_, ny_)
+    nx_, ny_ = nx_ / nn, ny_ / nn
+    # the closed trailing edge is a concave corner of the flow region (the
+    # surface normals there tilt aft by ~7 deg, the wake's are vertical): turn
+    # the normals to vertical over the rear 40 % of the chord, so that the
+    # normal lines there diverge instead of crossing
+    g = np.where(on_surf, _smoothstep((xcs - 0.6) / 0.4), 1.0)
+    nx_, ny_ = (1.0 - g) * nx_, (1.0 - g) * ny_ + g * side
+    nn = np.hypot(nx_, ny_)
+    nx_, ny_ = nx_ / nn, ny_ / nn
+    # wall-normal distribution (distance / far): n_layers geometric layers
+    # from first_cell, then stretched to the far field
+    def normal_dist(first):
+        d_layers = first * layer_ratio ** np.arange(n_layers)
+        s_in = np.concatenate([[0.0], np.cumsum(d_layers)])
+        s_out = _stretch(nj - n_layers, d_layers[-1] * layer_ratio, far - s_in[-1])
+        return np.concatenate([s_in, s_in[-1] + s_out[1:]]) / far
+    t_wall = normal_dist(first_cell)
+    # spanwise stations: half uniform, half sine (finer towards the tip)
+    th_sp = np.linspace(0, np.pi / 2, n_span_wing + 1)
+    zw = semi_span * 0.5 * (np.sin(th_sp) + th_sp / (np.pi / 2))
+    zw[-1] = semi_span
     zo = semi_span + _stretch(n_span_out, zw[-1] - zw[-2], far)[1:]
     zk = np.concatenate([zw, zo])
+    if tip_first is None:
+        tip_first = 4.0 * taper * root_chord * xs[1]
+    # beyond the tip: spacing tip_first off the sheet until the wall
+    # distribution's own spacing is larger (the outer levels are unchanged)
+    t_tip = np.maximum(t_wall, np.arange(nj + 1) * tip_first / far)
     tan_sw = np.tan(np.deg2rad(sweep_deg))
     P = np.zeros((nk + 1, nj + 1, ni + 1, 3))
     for k, z in enumerate(zk):
         zz = min(z, semi_span)
         chord = root_chord * (1 - (1 - taper) * zz / semi_span)
         x_le = zz * tan_sw
-        th = 1.0 if z <= semi_span + 1e-12 else 0.0
-        # inner line: section surface (x in chord units) or the wake/plate line
+        beyond = z > semi_span + 1e-12
+        if z >= semi_span - 1e-12:
+            th = 0.0
+        elif z > semi_span - tip_round:
+            sr = (z - (semi_span - tip_round)) / tip_round
+            th = 1.0 - sr * sr
+        else:
+            th = 1.0
+        b = _smoothstep((k - n_span_wing) / tip_blend) if beyond else 0.0
+        t_norm = (1.0 - b) * t_wall + b * t_tip
+        # inner line: section surface (x in chord units) or the wake/sheet line
         xi = x_le + chord * np.minimum(xc, 1.0) + np.where(xc > 1.0, (xc - 1.0) * root_chord, 0.0)
         yi = np.where(on_surf, side * th * chord * naca4_thickness(np.minimum(xc, 1.0), thickness), 0.0)
-        # outer line: a half circle of radius 'far' centred on the local trailing
-        # edge for the section points (psi = pi/2 xc from the upstream point to
-        # the top/bottom), straight lines y = +-far above and below the wake
+        # outer line: a half circle of radius far about the local trailing
+        # edge for the section points (psi = beta/2 from the upstream point
+        # to the top/bottom), straight lines y = +-far above and below the wake
         x_te = x_le + chord
-        psi = 0.5 * np.pi * np.minimum(xc, 1.0)
-        xo = np.where(on_surf, x_te - far * np.cos(psi), xi)
+        psi = 0.5 * bc
+        xo = np.where(on_surf, x_te - far * np.cos(psi), x_te + so)
         yo = np.where(on_surf, side * far * np.sin(psi), side * far)
+        dx, dy = xo - xi, yo - yi
+        D = np.hypot(dx, dy)
         for j, t in enumerate(t_norm):
-            P[k, j, :, 0] = xi + t * (xo - xi)
-            P[k, j, :, 1] = yi + t * (yo - yi)
+            w = _smoothstep(t / normal_blend)
+            P[k, j, :, 0] = xi + t * ((1.0 - w) * D * nx_ + w * dx)
+            P[k, j, :, 1] = yi + t * ((1.0 - w) * D * ny_ + w * dy)
             P[k, j, :, 2] = z
     points = P.reshape(-1, 3)
     nI1, nJ1 = ni + 1, nj + 1
@@ -491,7 +562,7 @@ def swept_wing(n_wake: int = 48, n_surf: int = 128, n_normal: int = 96, n_span_w
     # chord plane beyond the tip (j = 0, lower surface i <-> upper surface ni - i)
     remap = np.arange(points.shape[0], dtype=np.int64)
     for k in range(nk + 1):
-        beyond = zk[k] > semi_span + 1e-12
+        beyond = zk[k] >= semi_span - 1e-12   # the tip station is a sheet too
         for i in range(0, ni // 2 + 1):
             mirror = ni - i
             if mirror == i:
@@ -525,14 +596,21 @@ def swept_wing(n_wake: int = 48, n_surf: int = 128, n_normal: int = 96, n_span_w
 
 
 def _face_triangles(mesh: PolyMesh, faces: np.ndarray):
-    """Fan triangles (p0, p_k, p_k+1) of the given faces: arrays [nT,3] of
-    corner coordinates and the owning face of each triangle."""
+    """Triangles (c, p_k, p_k+1) of the given faces, c the mean of the face's
+    points (summed in point order, as the device does), one per edge: arrays
+    [nT,3] of corner coordinates and the owning face of each triangle.  A
+    warped quad is followed on both diagonals."""
     off = mesh.face_offsets
     A, B, Cc, owner = [], [], [], []
     for f in faces:
         pts = mesh.points[mesh.face_points[off[f]:off[f + 1]]]
-        for k in range(1, len(pts) - 1):
-            A.append(pts[0]); B.append(pts[k]); Cc.append(pts[k + 1]); owner.append(f)
+        n = len(pts)
+        c = pts[0].copy()
+        for k in range(1, n):
+            c = c + pts[k]
+        c = c / n
+        for k in range(n):
+            A.append(c); B.append(pts[k]); Cc.append(pts[(k + 1) % n]); owner.append(f)
     return np.array(A), np.array(B), np.array(Cc), np.array(owner)
 
 

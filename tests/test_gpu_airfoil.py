@@ -34,7 +34,8 @@ def test_airfoil_iterate_parity(oracle, model, kw):
     dc = DeviceCase(spec)
     oc = oracle.OracleCase(spec)
     # the device sweeps the nuTilda equation block-Jacobi over the mesh's RCB
-    # partitions (and the partitioned ILU uses the same partition)
+    # boxes; the partitioned ILU uses the boxes after the level cut
+    oc.set_sa_partition(dc.mesh.get_int(_capi.M_SA_PARTITION))
     oc.set_partition(dc.mesh.get_int(_capi.M_PARTITION))
     r0 = oc.iterate(12)
     r1 = dc.iterate(12)

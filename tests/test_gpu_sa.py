@@ -25,7 +25,9 @@ def test_sa_set_and_correct_parity(oracle, model, kw):
     spec = cases.box_sa_case(model, turb_kw=kw)
     oc = oracle.OracleCase(spec)
     dc = DeviceCase(spec)
-    oc.set_partition(dc.mesh.get_int(_capi.M_PARTITION))  # the SA sweep's block-Jacobi partitions
+    # 384 cells: one RCB box, so the device sweeps nuTilda serially, as the
+    # oracle does by default (the reference's smoothSolver symGaussSeidel)
+    assert dc.mesh.get_int(_capi.M_SA_PARTITION).max() == 0
     for f in (30, 31, 33, 34):
         assert rel(dc.get(f), oc.get(f)) <= 1e-14, f
     oc.flux_residual()
@@ -50,10 +52,10 @@ def test_sa_partitioned_sweep_parity(oracle, model):
     print(f"[SA partitions] {model}", flush=True)
     spec = cases.box_sa_case(model, nx=60, ny=50, turb_kw={"rel_tol": 1e-3, "tolerance": 1e-12})
     dc = DeviceCase(spec)
-    part = dc.mesh.get_int(_capi.M_PARTITION)
+    part = dc.mesh.get_int(_capi.M_SA_PARTITION)
     assert part.max() >= 1
     oc = oracle.OracleCase(spec)
-    oc.set_partition(part)
+    oc.set_sa_partition(part)
     oc.flux_residual()
     dc.flux_residual()
     po = oc.turbulence_correct()
@@ -69,7 +71,7 @@ def test_sa_iterate_parity(oracle, model):
     spec = cases.box_sa_case(model)
     oc = oracle.OracleCase(spec)
     dc = DeviceCase(spec)
-    oc.set_partition(dc.mesh.get_int(_capi.M_PARTITION))  # the SA sweep's block-Jacobi partitions
+    assert dc.mesh.get_int(_capi.M_SA_PARTITION).max() == 0   # serial sweep on both sides
     r0 = oc.iterate(15)
     r1 = dc.iterate(15)
     assert np.all(np.isfinite(r1))
@@ -89,7 +91,6 @@ def test_forces_parity(oracle, kind):
         patch = spec.mesh.patch_index("bottom")
     oc = oracle.OracleCase(spec)
     dc = DeviceCase(spec)
-    oc.set_partition(dc.mesh.get_int(_capi.M_PARTITION))  # the SA sweep's block-Jacobi partitions
     oc.iterate(3)
     dc.iterate(3)
     fo = oc.forces(patch, (1, 0, 0), (0, 1, 0), 1e5 if kind != "step" else 1.0, 1.2, 100.0, 1e-4)

@@ -291,7 +291,7 @@ def test_sa_symgs_solve(oracle, split):
     part = np.zeros(geo.n, dtype=np.int32)
     if split:
         part = (geo.g["C"][:, 0] > 0.005).astype(np.int32)
-        oc.set_partition(part)
+        oc.set_sa_partition(part)
     oc.flux_residual()
     x0 = oc.get(29).copy()
     rho = oc.get(0)

@@ -37,7 +37,7 @@ def test_forces_noalloc_matches_forces(oracle):
     dc = DeviceCase(big)
     oc = oracle.OracleCase(big)
     from rhodst_amd import _capi
-    oc.set_partition(dc.mesh.get_int(_capi.M_PARTITION))
+    oc.set_sa_partition(dc.mesh.get_int(_capi.M_SA_PARTITION))
     dc.iterate(3)
     oc.iterate(3)
     wall = big.mesh.patch_index("airfoil")
