@@ -1,29 +1,48 @@
 """Benchmark of the block-coupled implicit step on one MI355X.
 
-Workload: the transonic swept wing (semi-span 1, 30 deg sweep, taper 0.56,
-10 % analytic section) on the 2 027 520-cell C-H hex mesh at Mach 0.84,
-AoA 3.06 deg, AUSM+up, Roe-Jacobian implicit step with local time stepping
-(CFL ramp to 20), GMRES(30) + block-ILU(0), characteristic far field.
+Workload (default): the transonic swept wing (semi-span 1, 30 deg sweep,
+taper 0.56, 10 % analytic section) on the 2 027 520-cell C-H hex mesh with
+32 wall layers from a first cell of 2e-6 chord (y+ ~ 1), Mach 0.84,
+AoA 3.06 deg, Re 1.2e7, no-slip wing with the Spalart-Allmaras model with
+rotation-curvature correction (SARC2), viscous terms on, AUSM+up first
+order, the Roe-Jacobian implicit step with local time stepping (CFL ramp to
+20), GMRES(30) + block-ILU(0), characteristic far field.
 
 A "step" is one Single::iterate (face flux + residual, Jacobian assembly,
-linear solve, primitive/BC update, CFL/LTS) with every array resident in
-HBM.  The line reports the face-flux kernel throughput (Gfaces/s) as the
-value, the per-step time of the FP64 and mixed-precision paths, the roofline
-of the face-flux kernel (algorithmic bytes / HIP-event kernel time), the time
-to a 1e-8 drop of |R|_2 of the mixed and the FP64 path with the exact-Jacobian
-(Newton) operator (Roe-ILU(0) preconditioner, CFL ramp from 2 to 1e4, GMRES(30)
-to relTol 1e-2 in at most 4 cycles per step), and the CPU oracle timed on a
-bounded sample of the same workload.  --turbulence SARC2 runs the viscous Spalart-Allmaras form of the
-step (throughput only).
+linear solve, primitive/BC update, nuTilda equation, CFL/LTS) with every
+array resident in HBM.  The line reports:
+  value           Gfaces/s of the face-flux + residual kernel of this
+                  workload (viscous, muEff with the eddy viscosity), HIP
+                  events over back-to-back launches;
+  ms_per_step_*   the plain FP64 path (natural-order ILU, modified
+                  Gram-Schmidt), FP64 with the partitioned ILU, and mixed
+                  precision (FP32 Jacobian/ILU/Krylov in FP64 defect
+                  correction, FP64 residual);
+  roofline        of the face-flux kernel: algorithmic bytes / kernel time
+                  against 8 TB/s, the measured HBM traffic and the FP64
+                  vector-ALU rate against 78.6 TFLOP/s from the committed
+                  rocprofv3 counter summaries;
+  time_to_1e-8    wall-clock to |R|_2/|R|_2(0) <= 1e-8 (timeDST.C:328-331):
+                  the reference's Roe operator at CFL 20 over a bounded
+                  number of steps (it stalls near 5e-5 on this mesh), and the
+                  exact-Jacobian Newton-Krylov operator (an extension: the
+                  exact first-order inviscid linearisation plus the compact
+                  viscous terms, Roe ILU(0) preconditioner, CFL ramp to 1e4),
+                  mixed and FP64, with Cd of both;
+  cpu_baseline    the CPU FP64 oracle's face flux + residual on the same
+                  mesh and state, all host threads (OpenMP) and one.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--small] [--skip-converge]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--small]
+       [--turbulence SARC2|SAneg|none] [--skip-converge] [--no-cpu]
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -32,10 +51,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_VECTOR_PEAK_TFLOPS = 78.6
+PROFILES = os.path.join(ROOT, "profiles")
 
 
 def k1_bytes(n_cells: int, n_int: int, n_bnd: int) -> float:
-    """Algorithmic bytes of one face-flux + residual call, first-order
+    """Algorithmic bytes of one face-flux + residual launch, first-order
     inviscid: every array read once, R written once; per internal face the
     two int32 cell indices and Sf, per boundary face the cell index, Sf and
     the boundary state, per cell W, 1/V and R."""
@@ -50,19 +71,14 @@ def k1_bytes_viscous(n_cells: int, n_int: int, n_bnd: int) -> float:
     return n_int * (8 + 24 + 8 + 8 + 24) + n_bnd * (4 + 24 + 40 + 16 + 72) + n_cells * (40 + 8 + 72 + 16 + 40)
 
 
-PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
-
-
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of the face-flux kernel on the default workload,
-    from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of that
-    kernel (the commands that produce it are in the file); None when it is
-    absent."""
+def profile_value(name: str, key: str):
+    """A number a committed rocprofv3 summary under profiles/ states as
+    'key: value' (the commands that produce it are in the file)."""
     try:
-        with open(os.path.join(PROFILES, f"pmc_{kernel}_swept_wing_2M.md")) as f:
+        with open(os.path.join(PROFILES, name)) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return int(line.split(":")[1])
+                if line.startswith(key + ":"):
+                    return float(line.split(":", 1)[1])
     except OSError:
         pass
     return None
@@ -78,7 +94,7 @@ def run_path(spec, steps: int, warmup: int, mesh=None):
     if warmup:
         dc.iterate(warmup)
     t0 = time.perf_counter()
-    res, perfs = dc.iterate(steps, with_perf=True)
+    res = dc.iterate(steps)
     t1 = time.perf_counter()
     return dc, (t1 - t0) * 1e3 / steps, res
 
@@ -95,9 +111,9 @@ def wing_coefficients(dc, mesh, mach=0.84, aoa_deg=3.06):
     return float(out[0]), float(out[1])
 
 
-def converge_path(name, spec, mesh, max_steps, dmesh=None, chunk=10):
-    """Iterate one path until |R|_2 / |R|_2(0) <= 1e-8 (timeDST.C:328-331 norm);
-    the time is taken at the end of the chunk of steps that reaches it."""
+def converge_path(name, spec, mesh, max_steps, dmesh=None, chunk=20):
+    """Iterate one path until |R|_2 / |R|_2(0) <= 1e-8 (timeDST.C:328-331 norm)
+    or max_steps; the time is taken at the end of the chunk that reaches it."""
     from rhodst_amd import DeviceCase
     dc = DeviceCase(spec, dmesh)
     r0 = None
@@ -105,17 +121,17 @@ def converge_path(name, spec, mesh, max_steps, dmesh=None, chunk=10):
     t0 = time.perf_counter()
     steps = 0
     reached = False
-    next_log = 50
+    next_log = 100
     while steps < max_steps:
-        res = dc.iterate(chunk)
+        res = dc.iterate(min(chunk, max_steps - steps))
         if r0 is None:
             r0 = res[0]
-        steps += chunk
+        steps += len(res)
         hist.extend(res.tolist())
         drop = res[-1] / r0
         if steps >= next_log or not np.isfinite(drop):
             log(f"{name}: step {steps} drop {drop:.3e} t {time.perf_counter() - t0:.1f}s")
-            next_log += 50
+            next_log += 100
         if not np.isfinite(drop):
             break
         if np.min(np.array(hist) / r0) <= 1e-8:
@@ -128,35 +144,36 @@ def converge_path(name, spec, mesh, max_steps, dmesh=None, chunk=10):
     out = {"reached": reached, "steps": k if reached else steps, "seconds": round(t1 - t0, 2),
            "final_drop": float(h[-1]), "min_drop": float(h.min()), "Cd": cd, "Cl": cl}
     log(f"{name}: {'reached 1e-8 at step ' + str(k) if reached else 'not reached'} in {t1 - t0:.1f}s, Cd {cd:.12f}")
-    return out, dc.mesh
+    return out
 
 
-def converge(args, mesh, spec64, spec32) -> int:
-    """Time to a 1e-8 drop of |R|_2 for the mixed and the FP64 path; prints
-    one JSON line."""
+def cpu_baseline(args, mesh_kw, case_kw, wall_dist, faces):
+    """The oracle's face flux + residual on the host: all threads (OpenMP
+    build, bit-identical to the serial one) and one thread, each in a child
+    process started before any OpenMP runtime."""
     out = {}
-    dmesh = None
-    specs = {"mixed": spec32, "fp64": spec64}
-    for name in args.paths.split(","):
-        out[name], dmesh = converge_path(name, specs[name], mesh, args.max_steps, dmesh)
-    print(json.dumps({"converge": out, "cells": mesh.n_cells}), flush=True)
-    return 0
-
-
-def flux_only(args, spec) -> int:
-    """A 256 MiB device-to-device copy (the counter calibration: its
-    FETCH_SIZE and WRITE_SIZE are 256 MiB each) and args.flux_calls face-flux
-    calls on the workload."""
-    import torch
-    from rhodst_amd import DeviceCase
-    dc = DeviceCase(spec)
-    a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
-    b = torch.empty_like(a)
-    b.copy_(a)
-    torch.cuda.synchronize()
-    ms = dc.time_flux(args.flux_calls)
-    print(json.dumps({"flux_ms": ms, "copy_bytes": a.numel() * 8, "calls": args.flux_calls}), flush=True)
-    return 0
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "spec.npz")
+        extra = {} if wall_dist is None else {"wall_dist": wall_dist}
+        np.savez(path, mesh_kw=np.array(json.dumps(mesh_kw)), case_kw=np.array(json.dumps(case_kw)), **extra)
+        all_threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        for threads in (all_threads, 1):
+            env = dict(os.environ, OMP_NUM_THREADS=str(threads), ORACLE_OPENMP="1")
+            log(f"cpu baseline: oracle on {threads} thread(s)")
+            r = subprocess.run([sys.executable, "-m", "oracle.cpu_baseline", "--spec", path, "--seconds",
+                                str(args.cpu_seconds)], cwd=ROOT, env=env, capture_output=True, text=True,
+                               timeout=600)
+            if r.returncode != 0:
+                log(f"cpu baseline failed: {r.stderr[-500:]}")
+                return None
+            out[threads] = json.loads(r.stdout.strip().splitlines()[-1])
+    a, one = out[all_threads], out[1]
+    return {"value": round(a["gfaces_per_s"], 5), "unit": "Gfaces/s", "cores": all_threads, "kind": "port",
+            "value_1_core": round(one["gfaces_per_s"], 5), "cpu_model": a["cpu_model"], "nproc": a["nproc"],
+            "omp_num_threads": all_threads,
+            "sample": f"{a['calls']} (all threads) and {one['calls']} (1 thread) face-flux + residual calls of the "
+                      f"CPU FP64 oracle (OpenMP build, results bit-identical to the serial checker) on the same "
+                      f"{a['cells']}-cell mesh and free-stream state ({a['seconds']:.1f} s / {one['seconds']:.1f} s)"}
 
 
 def main(argv=None) -> int:
@@ -164,26 +181,16 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--small", action="store_true", help="a 0.26 M-cell mesh instead of 2.03 M")
+    ap.add_argument("--small", action="store_true", help="a 0.36 M-cell mesh instead of 2.03 M")
+    ap.add_argument("--turbulence", default="SARC2", help="SARC2, SARC1, SAneg, or none (inviscid slip wing)")
     ap.add_argument("--flux-calls", type=int, default=50)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--converge", action="store_true",
-                    help="run the mixed and the FP64 paths (partitioned ILU, classical GS) to a 1e-8 drop of |R|_2")
-    ap.add_argument("--max-steps", type=int, default=4000)
-    ap.add_argument("--max-co", type=float, default=None, help="CFL ramp ceiling (default: the workload's 20)")
-    ap.add_argument("--gmres-cycles", type=int, default=None, help="GMRES restart cycles per step (default 1)")
-    ap.add_argument("--krylov-dirs", type=int, default=None, help="GMRES restart length (default 30)")
-    ap.add_argument("--turbulence", default=None, help="SARC2 or SAneg: the viscous SA form of the workload")
-    ap.add_argument("--jacobian", default="Roe", choices=["Roe", "exact"],
-                    help="implicit operator: the reference's Roe Jacobian, or the exact linearisation of the "
-                         "first-order inviscid residual (Newton)")
-    ap.add_argument("--rel-tol", type=float, default=None, help="GMRES relTol (default: the workload's 1e-2)")
-    ap.add_argument("--initial-co", type=float, default=None, help="initial CFL of the ramp (default 1)")
-    ap.add_argument("--skip-converge", action="store_true",
-                    help="leave out the time-to-1e-8 runs of the default mode")
-    ap.add_argument("--converge-steps", type=int, default=1200, help="step limit of the time-to-1e-8 runs")
-    ap.add_argument("--paths", default="mixed,fp64", help="converge mode: paths to run, in order")
+    ap.add_argument("--skip-converge", action="store_true", help="leave out the time-to-1e-8 runs")
+    ap.add_argument("--roe-steps", type=int, default=300, help="steps of the Roe operator at CFL 20 per path")
+    ap.add_argument("--converge-steps", type=int, default=1500, help="step limit of the Newton time-to-1e-8 runs")
+    ap.add_argument("--paths", default="mixed,fp64", help="time-to-1e-8 paths, in order")
+    ap.add_argument("--converge-only", action="store_true", help="only the Newton time-to-1e-8 runs (one JSON line)")
     ap.add_argument("--flux-only", action="store_true",
                     help="only the face-flux kernel (n calls) and a 256 MiB device copy of known size: the "
                          "short program the FETCH_SIZE / WRITE_SIZE counter runs profile")
@@ -194,103 +201,105 @@ def main(argv=None) -> int:
         return 2
 
     from rhodst_amd import cases, meshgen
+    from rhodst_amd._capi import DeviceMesh
     mesh_kw = dict(n_wake=24, n_surf=64, n_normal=64, n_span_wing=24, n_span_out=8) if args.small else {}
+    turb = None if args.turbulence.lower() == "none" else args.turbulence
     t = time.perf_counter()
     mesh = meshgen.swept_wing(**mesh_kw)
     log(f"mesh {mesh.n_cells} cells, {mesh.n_internal} internal faces in {time.perf_counter() - t:.1f}s")
-    extra = {}
-    if args.max_co is not None:
-        extra["max_co"] = args.max_co
-    if args.gmres_cycles is not None:
-        extra["max_iter"] = args.gmres_cycles
-    if args.krylov_dirs is not None:
-        extra["n_directions"] = args.krylov_dirs
-    if args.rel_tol is not None:
-        extra["rel_tol"] = args.rel_tol
-    if args.initial_co is not None:
-        extra["initial_co"] = args.initial_co
-    if args.jacobian != "Roe":
-        extra["jacobian"] = args.jacobian
-    wd = None
-    mesh_dev = None
-    if args.turbulence:
-        from rhodst_amd._capi import DeviceMesh
-        t = time.perf_counter()
-        mesh_dev = DeviceMesh(mesh)
-        wd = mesh_dev.wall_distance()
-        extra.update(turbulence=args.turbulence, wall_dist=wd)
-        log(f"mesh upload + device wall distance in {time.perf_counter() - t:.1f}s")
-    spec64 = cases.swept_wing_case(mesh=mesh, **extra)
-    fast = dict(ilu_order="partitioned", gram_schmidt="classical", **extra)
-    spec64p = cases.swept_wing_case(mesh=mesh, **fast)
-    spec32p = cases.swept_wing_case(mesh=mesh, precision="mixed", **fast)
-    if args.converge:
-        return converge(args, mesh, spec64p, spec32p)
+    t = time.perf_counter()
+    dm = DeviceMesh(mesh)
+    wd = dm.wall_distance() if turb else None
+    log(f"mesh upload{' + device wall distance' if turb else ''} in {time.perf_counter() - t:.1f}s")
+    base = dict(turbulence=turb, wall_dist=wd) if turb else {}
+    fast = dict(ilu_order="partitioned", gram_schmidt="classical")
+    spec64 = cases.swept_wing_case(mesh=mesh, **base)
+    spec64p = cases.swept_wing_case(mesh=mesh, **fast, **base)
+    spec32p = cases.swept_wing_case(mesh=mesh, precision="mixed", **fast, **base)
+    newton = dict(jacobian="exact", initial_co=2.0, max_co=1e4, rel_tol=1e-2, max_iter=4, **fast)
+
     if args.flux_only:
-        return flux_only(args, spec64)
+        import torch
+        from rhodst_amd import DeviceCase
+        dc = DeviceCase(spec64, dm)
+        a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        ms = dc.time_flux(args.flux_calls)
+        print(json.dumps({"flux_ms": ms, "copy_bytes": a.numel() * 8, "calls": args.flux_calls}), flush=True)
+        return 0
+    if args.converge_only:
+        out = {}
+        for name in args.paths.split(","):
+            prec = "mixed" if name == "mixed" else "fp64"
+            out[name] = converge_path(f"newton {name}", cases.swept_wing_case(mesh=mesh, precision=prec, **newton,
+                                                                              **base), mesh, args.converge_steps, dm)
+        print(json.dumps({"converge": out, "cells": mesh.n_cells}), flush=True)
+        return 0
 
     log("fp64 path, natural-order ILU(0) (the plain path)")
-    dc64, ms64, res64 = run_path(spec64, args.steps, args.warmup)
+    dc64, ms64, res64 = run_path(spec64, args.steps, args.warmup, dm)
     log(f"fp64 {ms64:.2f} ms/step, |R| {res64[0]:.4e} -> {res64[-1]:.4e}")
-    info = dc64.mesh.info
+    info = dm.info
     nC, nI, nB = info["n_cells"], info["n_internal"], info["n_bfaces"]
     flux_ms = dc64.time_flux(args.flux_calls)
-    mesh_dev = dc64.mesh
     del dc64
     log(f"face-flux kernel {flux_ms:.4f} ms")
     log("fp64 path, partitioned block-ILU(0)")
-    dc64p, ms64p, res64p = run_path(spec64p, args.steps, args.warmup, mesh_dev)
+    dc64p, ms64p, res64p = run_path(spec64p, args.steps, args.warmup, dm)
     log(f"fp64 partitioned {ms64p:.2f} ms/step, |R| {res64p[0]:.4e} -> {res64p[-1]:.4e}")
     del dc64p
     log("mixed path, partitioned block-ILU(0)")
-    dc32, ms32, res32 = run_path(spec32p, args.steps, args.warmup, mesh_dev)
+    dc32, ms32, res32 = run_path(spec32p, args.steps, args.warmup, dm)
     log(f"mixed {ms32:.2f} ms/step, |R| {res32[0]:.4e} -> {res32[-1]:.4e}")
     del dc32
 
     gfaces = (nI + nB) / (flux_ms * 1e-3) / 1e9
-    bytes_k1 = (k1_bytes_viscous if args.turbulence else k1_bytes)(nC, nI, nB)
+    viscous = turb is not None
+    bytes_k1 = (k1_bytes_viscous if viscous else k1_bytes)(nC, nI, nB)
     achieved = bytes_k1 / (flux_ms * 1e-3) / 1e9
     kname = "k_flux_residual"
-    traffic = measured_traffic(kname) if not (args.small or args.turbulence) else None
+    tag = "swept_wing_2M_" + ("sa" if viscous else "inviscid")
+    prof = f"pmc_{kname}_{tag}.md"
+    traffic = None if args.small else profile_value(prof, "traffic_bytes_per_launch")
+    flops = None if args.small else profile_value(prof, "fp64_flop_per_launch")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "%s<AUSM+up%s>" % (kname, ", viscous" if args.turbulence else ""),
-                "kernel_ms": round(flux_ms, 5),
-                "algorithmic_bytes": int(bytes_k1)}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": int(traffic) if traffic else None,
+                "kernel": "%s<AUSM+up%s>" % (kname, ", viscous" if viscous else ""),
+                "kernel_ms": round(flux_ms, 5), "algorithmic_bytes": int(bytes_k1)}
+    if flops:
+        tf = flops / (flux_ms * 1e-3) / 1e12
+        roofline["fp64_valu"] = {"achieved": round(tf, 2), "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                 "frac": round(tf / FP64_VECTOR_PEAK_TFLOPS, 4), "flop_per_launch": int(flops)}
 
     ttc = None
-    if not (args.skip_converge or args.small or args.turbulence):
-        # time to a 1e-8 drop: the exact-Jacobian (Newton) operator with the
-        # Roe ILU(0) preconditioner, timeDST's CFL ramp started at 2 (initialCo) up to
-        # 1e4, GMRES(30) to relTol 1e-2 in at most 4 cycles, mixed and FP64
-        newton = dict(jacobian="exact", initial_co=2.0, max_co=1e4, rel_tol=1e-2, max_iter=4,
-                      ilu_order="partitioned", gram_schmidt="classical")
-        ttc = {}
-        for name, prec in (("mixed", "mixed"), ("fp64", "fp64")):
-            log(f"time to 1e-8, {name} path, exact Jacobian")
-            ttc[name], _ = converge_path(name, cases.swept_wing_case(mesh=mesh, precision=prec, **newton), mesh,
-                                         args.converge_steps, mesh_dev)
-        ttc["settings"] = "exact Jacobian + Roe-ILU(0) preconditioner, CFL ramp 2 -> 1e4, GMRES(30) relTol 1e-2, 4 cycles"
-        if ttc["mixed"]["reached"] and ttc["fp64"]["reached"]:
-            ttc["speedup_mixed_vs_fp64"] = round(ttc["fp64"]["seconds"] / ttc["mixed"]["seconds"], 3)
-        ttc["abs_dCd_mixed_vs_fp64"] = abs(ttc["mixed"]["Cd"] - ttc["fp64"]["Cd"])
+    if not (args.skip_converge or args.small):
+        ttc = {"roe_cfl20": {}, "newton": {}}
+        for name, spec in (("mixed", spec32p), ("fp64", spec64p)):
+            log(f"Roe operator at CFL 20, {name}, {args.roe_steps} steps")
+            ttc["roe_cfl20"][name] = converge_path(f"roe {name}", spec, mesh, args.roe_steps, dm)
+        for name in args.paths.split(","):
+            prec = "mixed" if name == "mixed" else "fp64"
+            log(f"time to 1e-8, {name} path, Newton operator")
+            ttc["newton"][name] = converge_path(f"newton {name}",
+                                                cases.swept_wing_case(mesh=mesh, precision=prec, **newton, **base),
+                                                mesh, args.converge_steps, dm)
+        nw = ttc["newton"]
+        if all(nw.get(p, {}).get("reached") for p in ("mixed", "fp64")):
+            nw["speedup_mixed_vs_fp64"] = round(nw["fp64"]["seconds"] / nw["mixed"]["seconds"], 3)
+        if "mixed" in nw and "fp64" in nw:
+            nw["abs_dCd_mixed_vs_fp64"] = abs(nw["mixed"]["Cd"] - nw["fp64"]["Cd"])
+        nw["settings"] = ("exact first-order inviscid Jacobian + compact viscous terms (muEff frozen), Roe-ILU(0) "
+                          "preconditioner, CFL ramp 2 -> 1e4, GMRES(30) relTol 1e-2, 4 cycles")
+        ttc["roe_cfl20"]["settings"] = "the reference's Roe operator, LTS CFL ramp to 20, GMRES(30) 1 cycle relTol 1e-2"
 
     cpu = None
     if not args.no_cpu:
-        log("cpu baseline (oracle, 1 thread)")
-        from oracle import binding
-        oc = binding.OracleCase(spec64)
-        n_calls, t_cpu = 0, 0.0
-        while t_cpu < args.cpu_seconds and n_calls < 50:
-            t0 = time.perf_counter()
-            oc.flux_residual()
-            t_cpu += time.perf_counter() - t0
-            n_calls += 1
-        cpu = {"value": round((nI + nB) * n_calls / t_cpu / 1e9, 5), "unit": "Gfaces/s", "cores": 1,
-               "kind": "port",
-               "sample": f"{n_calls} face-flux + residual calls of the CPU FP64 oracle on the same "
-                         f"{nC}-cell mesh and uniform free-stream state ({t_cpu:.1f} s)"}
+        case_kw = {"turbulence": turb} if turb else {}
+        cpu = cpu_baseline(args, mesh_kw, case_kw, wd, nI + nB)
 
+    newton_mixed = ttc["newton"].get("mixed") if ttc else None
     out = {
         "metric": "time to 1e-8 residual drop + Gfaces/s flux, 2M-cell 3D SA, mixed vs FP64",
         "value": round(gfaces, 3),
@@ -299,6 +308,7 @@ def main(argv=None) -> int:
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms32, 3),
+        "ms_per_step_path": "mixed precision (FP32 Jacobian, ILU and Krylov in FP64 defect correction; FP64 residual)",
         "ms_per_step_fp64": round(ms64, 3),
         "ms_per_step_fp64_partitioned": round(ms64p, 3),
         "ms_per_step_mixed": round(ms32, 3),
@@ -307,21 +317,21 @@ def main(argv=None) -> int:
         "residual_drop_fp64": float(res64[-1] / res64[0]),
         "residual_drop_fp64_partitioned": float(res64p[-1] / res64p[0]),
         "residual_drop_mixed": float(res32[-1] / res32[0]),
-        "time_to_1e-8_s": (ttc["mixed"]["seconds"] if ttc and ttc["mixed"]["reached"] else None),
+        "time_to_1e-8_s": newton_mixed["seconds"] if newton_mixed and newton_mixed["reached"] else None,
+        "time_to_1e-8_path": "mixed precision, exact-Jacobian Newton-Krylov operator (extension)",
         "time_to_1e-8": ttc,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
+        "dtype_note": "value: the FP64 face-flux kernel; ms_per_step: the mixed path",
         "data": "synthetic",
         "config": {"workload": f"swept wing C-H mesh {nC} cells ({nI} internal + {nB} boundary faces), "
-                               "M0.84 AoA3.06, AUSM+up first order, "
-                               + (f"{args.turbulence} at Re 1.2e7, no-slip wing" if args.turbulence
+                               "M0.84 AoA3.06 Re1.2e7, AUSM+up first order, "
+                               + (f"{turb} Spalart-Allmaras, no-slip wing, viscous terms" if turb
                                   else "inviscid slip wall")
-                               + ", LTS CFL ramp to 20, GMRES(30)+block-ILU(0), 1 restart cycle per step; "
-                                 "ms_per_step is the mixed-precision path with the partitioned block-ILU(0) and "
-                                 "classical Gram-Schmidt",
-                   "cells": nC, "faces": nI + nB, "flux_scheme": "AUSM+up"},
+                               + ", Roe-Jacobian implicit step, LTS CFL ramp to 20, GMRES(30)+block-ILU(0)",
+                   "cells": nC, "faces": nI + nB, "flux_scheme": "AUSM+up", "turbulence": turb or "none"},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -205,7 +205,10 @@ enum rdst_turbulence_model {
     RDST_TURB_LAMINAR = 0,
     RDST_TURB_SANEG = 1,   /* SAneg (libraries/turbulence/SAneg/SAneg.C) */
     RDST_TURB_SARC2 = 2,   /* SARC2 (libraries/turbulence/SARC2/SARC2.C) */
-    RDST_TURB_SARC1 = 3    /* SARC1 (libraries/turbulence/SARC1/SARC1.C): Spalart-Shur fr1 with DS/Dt */
+    RDST_TURB_SARC1 = 3,   /* SARC1 (libraries/turbulence/SARC1/SARC1.C): Spalart-Shur fr1 with DS/Dt */
+    RDST_TURB_SA = 4       /* foam-extend's stock SpalartAllmaras (every turbulent tutorial,
+                              e.g. RAE2822/constant/RASProperties:19): the fv3 form SARC2.C:418-488 is
+                              built on, without the rotation-curvature multiplier (fr1 = 1) */
 };
 
 /* RASProperties <model>Coeffs (SAneg.C:168-307, SARC2.C:172-285), plus

@@ -2564,6 +2564,12 @@ static void saSet(Case& c, const rdst_turbulence* t, const rdst_scalar_bc* bcs, 
     int n = m.nCells;
     tb = Turb();
     tb.t = *t;
+    // foam-extend's stock SpalartAllmaras::correct is SARC2::correct with
+    // fr1 = 1 (SARC2.C:121-140: RCMCorrection off returns 1)
+    if (tb.t.model == RDST_TURB_SA) {
+        tb.t.model = RDST_TURB_SARC2;
+        tb.t.rcm_correction = 0;
+    }
     tb.on = t->model != RDST_TURB_LAMINAR;
     if (!tb.on) return;
     if (!c.hasState) throw std::runtime_error("set_turbulence: no state");

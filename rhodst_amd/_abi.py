@@ -83,7 +83,7 @@ class rdst_scalar_bc(C.Structure):
     _fields_ = [("type", C.c_int32), ("pad", C.c_int32), ("value", C.c_double)]
 
 
-TURB_MODELS = {"laminar": 0, "SAneg": 1, "SARC2": 2, "SARC1": 3}
+TURB_MODELS = {"laminar": 0, "SAneg": 1, "SARC2": 2, "SARC1": 3, "SpalartAllmaras": 4}
 
 
 def make_turbulence(model="SAneg", **kw) -> rdst_turbulence:

@@ -2,6 +2,7 @@
 the workloads the block-coupled implicit step is run on."""
 from __future__ import annotations
 
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -182,6 +183,47 @@ def airfoil_case(scheme: str = "HLLC", mach: float = 0.73, aoa_deg: float = 2.3,
         spec_out.turbulence = (make_turbulence(turbulence), sbc, d, np.full(n, 3.0 * nu))
         spec_out.name = f"airfoil_{turbulence}"
     return spec_out
+
+
+RAE2822_MESH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests", "golden",
+                            "rae2822_polymesh.npz")
+
+
+def rae2822_case(scheme: str = "Kurganov", turbulence: str | None = "SpalartAllmaras", mesh=None, wall_dist=None,
+                 **settings) -> CaseSpec:
+    """The reference's RAE2822 tutorial (tutorials/RAE2822): its polyMesh
+    (23 552 cells; patches frontAndBackPlanes empty, freestreamBC, airfoil
+    wall), free stream 0/include/freestreamConditions (U (233.44 9.417 0),
+    p 108987.773, T 255.556) with the farField*DST conditions, no-slip
+    adiabatic airfoil, hPsiThermo with Sutherland transport (W 28.966,
+    Cp 1005, As 1.4584e-6, Ts 110.33; R = RR/W with RR = 8314.47), rhoDSTDict
+    (Kurganov, neighbouring, CFL 1 -> 500), fvSchemes (OSPRE/OSPREV
+    reconstruction), fvSolution (BiCGStab + ILUC0, maxIter 100, tolerance
+    and relTol 1e-10; fieldBounds rho 0.01 10, p 100 1e6, T 10 800) and the
+    SpalartAllmaras model with 0/nuTilda (4.4e-5 inside and at the far field,
+    0 on the wall).  The mesh is read from the committed array copy of the
+    tutorial's polyMesh."""
+    mesh = mesh if mesh is not None else meshgen.polymesh_from_npz(RAE2822_MESH)
+    Uinf, pinf, Tinf = (233.44, 9.417, 0.0), 108987.773, 255.556
+    R = 8314.47 / 28.966
+    ff = {"p": ("farField", pinf), "T": ("farField", Tinf), "U": ("farField", Uinf),
+          "inf": {"p": pinf, "T": Tinf, "U": Uinf}}
+    wall = {"p": "zeroGradient", "T": "zeroGradient", "U": ("fixedValue", (0.0, 0.0, 0.0))}
+    kw = dict(flux_scheme=scheme, neighbouring=1, initial_co=1.0, min_co=1.0, max_co=500.0, viscous=1,
+              solver="BiCGStab", precond="ILUC0", min_iter=1, max_iter=100, tolerance=1e-10, rel_tol=1e-10,
+              reconstruction="OSPRE", p_bounds=(100.0, 1e6), T_bounds=(10.0, 800.0), rho_bounds=(0.01, 10.0))
+    kw.update(settings)
+    n = mesh.n_cells
+    spec = CaseSpec("RAE2822", mesh, make_thermo(Cp=1005.0, R=R, Pr=0.72, sutherland=(1.4584e-6, 110.33)),
+                    make_bcs(mesh, {"freestreamBC": ff, "airfoil": wall}), make_settings(**kw),
+                    np.full(n, pinf), np.full(n, Tinf), np.ascontiguousarray(np.tile(np.array(Uinf), (n, 1))))
+    if turbulence:
+        from ._abi import make_scalar_bcs, make_turbulence
+        sbc = make_scalar_bcs(mesh, {"airfoil": ("fixedValue", 0.0), "freestreamBC": ("inletOutlet", 4.4e-5)})
+        d = wall_dist if wall_dist is not None else meshgen.wall_distance(mesh, ["airfoil"])
+        spec.turbulence = (make_turbulence(turbulence), sbc, d, np.full(n, 4.4e-5))
+        spec.name = f"RAE2822_{turbulence}"
+    return spec
 
 
 def box_viscous_case(nx: int = 24, ny: int = 16, scheme: str = "HLLC", mach: float = 0.3, jitter: float = 0.15,

@@ -202,7 +202,44 @@ __device__ inline Prim<D1> seeded(const double* W, int i, int k, const Thermo& t
 // into src (the ddt part of the steady-state source vanishes).  The Newton
 // operator feeds only the SpMV, so it is written in the sliced-ELL layout
 // (entry q of slot s at sellOfSlot[s] + 64 q) rather than block-CSR.
-template <int S, class TJ, bool NEWTON>
+// VISC (Newton operator only): the viscous face terms of calcFields
+// (baseFluxDST.C:229-250) are added with their compact linearisation: the
+// interp(muEff)|S| deltaCoeffs (U_N - U_P) part of the corrected snGrad (and
+// its boundary form) and the work term's dependence on it and on Up are
+// differentiated; the gradient-borne parts (non-orthogonal correction,
+// Sf & interp(tauMC)) and muEff (the turbulence frozen) keep their current
+// values.  Their full linearisation would reach second neighbours, outside
+// the block stencil.
+template <class T>
+__device__ inline void sngrad_u_bnd_t(const BcPatchDev& B, const Fields& F, int b, double nx, double ny, double nz,
+                                      double dcb, const T& uIx, const T& uIy, const T& uIz, T& ox, T& oy, T& oz) {
+    switch (B.U_type) {
+        case RDST_BC_FIXED_VALUE:
+            ox = dcb * (F.bUx[b] - uIx); oy = dcb * (F.bUy[b] - uIy); oz = dcb * (F.bUz[b] - uIz);
+            return;
+        case RDST_BC_SYMMETRY: {
+            const T nU = nx * uIx + ny * uIy + nz * uIz;
+            const T tx = uIx - 2.0 * nx * nU, ty = uIy - 2.0 * ny * nU, tz = uIz - 2.0 * nz * nU;
+            const double h = dcb / 2.0;
+            ox = (tx - uIx) * h; oy = (ty - uIy) * h; oz = (tz - uIz) * h;
+            return;
+        }
+        case RDST_BC_INLET_OUTLET: case RDST_BC_FARFIELD: {
+            const double vf = F.vf_U[b];
+            ox = vf * (F.refUx[b] - uIx) * dcb; oy = vf * (F.refUy[b] - uIy) * dcb; oz = vf * (F.refUz[b] - uIz) * dcb;
+            return;
+        }
+        default: ox = T(0.0); oy = T(0.0); oz = T(0.0);
+    }
+}
+
+__device__ inline void tau_mc_a(double mu, const double* g, double* t) {
+    const double tr = g[0] + g[4] + g[8];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) t[3 * i + j] = mu * (g[3 * j + i] - (i == j ? (2.0 / 3.0) * tr : 0.0));
+}
+
+template <int S, class TJ, bool NEWTON, bool VISC>
 __global__ __launch_bounds__(256) void k_adj_jacobian(MeshDev m, Fields F, const BcPatchDev* bc, Thermo th, Bounds bd,
                                                       int lowMach, double Mainf, TJ* J, double* src,
                                                       const int64_t* sellOfSlot) {
@@ -228,16 +265,35 @@ __global__ __launch_bounds__(256) void k_adj_jacobian(MeshDev m, Fields F, const
         if (kind == ELL_BOUNDARY) {
             const int k = m.nI + idx;
             const double nx = sx / ms, ny = sy / ms, nz = sz / ms;
+            // frozen viscous boundary data: muEff_b |S| and Sf & tauMC_b
+            double mgb = 0.0, tbx = 0.0, tby = 0.0, tbz = 0.0;
+            if constexpr (VISC) {
+                const double bmuE = F.bmut[idx] + F.bmu[idx];
+                mgb = bmuE * ms;
+                double gb[9], tb[9];
+                for (int q = 0; q < 9; ++q) gb[q] = F.bgradU[(size_t)q * m.nB + idx];
+                tau_mc_a(bmuE, gb, tb);
+                tbx = sx * tb[0] + sy * tb[3] + sz * tb[6];
+                tby = sx * tb[1] + sy * tb[4] + sz * tb[7];
+                tbz = sx * tb[2] + sy * tb[5] + sz * tb[8];
+            }
             for (int kk = 0; kk < 5; ++kk) {
                 const Prim<D1> I = seeded(F.W, c, kk, th, bd);
                 const BState<D1> bs = bc_state(bc[m.bpatch[idx]], F, idx, k, nx, ny, nz, ms, I, th, bd);
                 const SideT<D1> L = side_of(bs, gamma);
                 const Flux5T<D1> o = eval_flux<S>(L, L, sx, sy, sz, ms, gamma, 1.0, lowMach, Mainf);
+                D1 vx(0.0), vy(0.0), vz(0.0), ve(0.0);
+                if constexpr (VISC) {
+                    D1 snx, sny, snz;
+                    sngrad_u_bnd_t(bc[m.bpatch[idx]], F, idx, nx, ny, nz, m.ellDc[e], I.ux, I.uy, I.uz, snx, sny, snz);
+                    vx = mgb * snx; vy = mgb * sny; vz = mgb * snz;
+                    ve = (vx + tbx) * o.upx + (vy + tby) * o.upy + (vz + tbz) * o.upz;
+                }
                 Jcc[0 * 5 + kk] += o.phi.d * rV;
-                Jcc[1 * 5 + kk] += o.fx.d * rV;
-                Jcc[2 * 5 + kk] += o.fy.d * rV;
-                Jcc[3 * 5 + kk] += o.fz.d * rV;
-                Jcc[4 * 5 + kk] += o.fE.d * rV;
+                Jcc[1 * 5 + kk] += (o.fx.d - vx.d) * rV;
+                Jcc[2 * 5 + kk] += (o.fy.d - vy.d) * rV;
+                Jcc[3 * 5 + kk] += (o.fz.d - vz.d) * rV;
+                Jcc[4 * 5 + kk] += (o.fE.d - ve.d) * rV;
             }
             continue;
         }
@@ -245,6 +301,35 @@ __global__ __launch_bounds__(256) void k_adj_jacobian(MeshDev m, Fields F, const
         const double sgn = own ? rV : -rV;
         const int fi = m.ellFace[e];
         const int sco = own ? m.slotUpper[fi] : m.slotLower[fi];
+        // frozen viscous face data: interp(muEff)|S| deltaCoeffs, the
+        // correction part of snGrad(U) and Sf & interp(tauMC)
+        double mgd = 0.0, cgx = 0.0, cgy = 0.0, cgz = 0.0, tfx = 0.0, tfy = 0.0, tfz = 0.0, mg = 0.0;
+        if constexpr (VISC) {
+            const double w = m.ellWt[e];
+            const double muC = F.mut[c] + F.mu[c], muO = F.mut[idx] + F.mu[idx];
+            const double muP = own ? muC : muO, muN = own ? muO : muC;
+            const double muf = w * (muP - muN) + muN;
+            double gP[9], gN[9], tP[9], tN[9], gf[9], tf[9];
+            for (int q = 0; q < 9; ++q) {
+                gP[q] = F.gradU[(size_t)q * m.nC + (own ? c : idx)];
+                gN[q] = F.gradU[(size_t)q * m.nC + (own ? idx : c)];
+            }
+            tau_mc_a(muP, gP, tP);
+            tau_mc_a(muN, gN, tN);
+            for (int q = 0; q < 9; ++q) {
+                gf[q] = w * (gP[q] - gN[q]) + gN[q];
+                tf[q] = w * (tP[q] - tN[q]) + tN[q];
+            }
+            const double cx = m.ellCorr[e], cy = m.ellCorr[E + e], cz = m.ellCorr[2 * E + e];
+            cgx = cx * gf[0] + cy * gf[3] + cz * gf[6];
+            cgy = cx * gf[1] + cy * gf[4] + cz * gf[7];
+            cgz = cx * gf[2] + cy * gf[5] + cz * gf[8];
+            mg = muf * ms;
+            mgd = m.ellDc[e];
+            tfx = sx * tf[0] + sy * tf[3] + sz * tf[6];
+            tfy = sx * tf[1] + sy * tf[4] + sz * tf[7];
+            tfz = sx * tf[2] + sy * tf[5] + sz * tf[8];
+        }
         // derivative w.r.t. this cell's W, then w.r.t. the other cell's W
         for (int side = 0; side < 2; ++side) {
             for (int kk = 0; kk < 5; ++kk) {
@@ -253,7 +338,16 @@ __global__ __launch_bounds__(256) void k_adj_jacobian(MeshDev m, Fields F, const
                 const SideT<D1> sc = side_of(Pc, gamma), so = side_of(Po, gamma);
                 const Flux5T<D1> o = own ? eval_flux<S>(sc, so, sx, sy, sz, ms, gamma, 1.0, lowMach, Mainf)
                                          : eval_flux<S>(so, sc, sx, sy, sz, ms, gamma, 1.0, lowMach, Mainf);
-                const double d[5] = {o.phi.d, o.fx.d, o.fy.d, o.fz.d, o.fE.d};
+                double d[5] = {o.phi.d, o.fx.d, o.fy.d, o.fz.d, o.fE.d};
+                if constexpr (VISC) {
+                    const Prim<D1>& QP = own ? Pc : Po;
+                    const Prim<D1>& QN = own ? Po : Pc;
+                    const D1 lx = mg * (mgd * (QN.ux - QP.ux) + cgx);
+                    const D1 ly = mg * (mgd * (QN.uy - QP.uy) + cgy);
+                    const D1 lz = mg * (mgd * (QN.uz - QP.uz) + cgz);
+                    const D1 sdf = (lx + tfx) * o.upx + (ly + tfy) * o.upy + (lz + tfz) * o.upz;
+                    d[1] -= lx.d; d[2] -= ly.d; d[3] -= lz.d; d[4] -= sdf.d;
+                }
                 if (side == 0)
                     for (int r = 0; r < 5; ++r) Jcc[r * 5 + kk] += sgn * d[r];
                 else
@@ -370,9 +464,13 @@ Bounds bounds_of(const Case& c) {
 template <int S, class TJ, bool NEWTON>
 void launch_jac(Case& c, TJ* J) {
     const int nb = blocks_for(c.mesh->nC, 256);
-    hipLaunchKernelGGL((k_adj_jacobian<S, TJ, NEWTON>), dim3(nb), dim3(256), 0, c.stream, c.meshDev(), c.f, c.d_bc.p,
-                       c.th, bounds_of(c), c.st.low_mach, c.st.Mainf, J, NEWTON ? c.src.p : (double*)nullptr,
-                       c.mesh->d_sellOfSlot.p);
+    if (NEWTON && c.st.viscous)
+        hipLaunchKernelGGL((k_adj_jacobian<S, TJ, NEWTON, true>), dim3(nb), dim3(256), 0, c.stream, c.meshDev(), c.f,
+                           c.d_bc.p, c.th, bounds_of(c), c.st.low_mach, c.st.Mainf, J, c.src.p, c.mesh->d_sellOfSlot.p);
+    else
+        hipLaunchKernelGGL((k_adj_jacobian<S, TJ, NEWTON, false>), dim3(nb), dim3(256), 0, c.stream, c.meshDev(), c.f,
+                           c.d_bc.p, c.th, bounds_of(c), c.st.low_mach, c.st.Mainf, J,
+                           NEWTON ? c.src.p : (double*)nullptr, c.mesh->d_sellOfSlot.p);
 }
 template <class TJ, bool NEWTON>
 void launch_jac_scheme(Case& c, TJ* J) {
@@ -414,7 +512,8 @@ void adjoint_jacobian(Case& c) {
 }
 
 void newton_operator(Case& c) {
-    adjoint_check_scheme(c);
+    if (c.st.flux_scheme == RDST_FLUX_HLLCP || c.st.reconstruction != RDST_RECON_UPWIND)
+        throw ArgError("exact Jacobian: first-order (upwind) reconstruction and a flux other than HLLCP");
     const Mesh& m = *c.mesh;
     const size_t nn = 25 * (size_t)m.nnzb;
     // every slot is written (a cell's diagonal and its side of each face);

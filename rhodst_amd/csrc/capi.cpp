@@ -252,10 +252,9 @@ int rdst_case_create(rdst_mesh* mesh, const rdst_thermo* thermo, const rdst_bc* 
             throw ArgError("rdst_case_create: unknown reconstruction (valid: upwind OSPRE)", RDST_ERR_UNKNOWN_NAME);
         if (s.jacobian != RDST_JACOBIAN_ROE && s.jacobian != RDST_JACOBIAN_EXACT)
             throw ArgError("rdst_case_create: unknown Jacobian (valid: Roe exact)", RDST_ERR_UNKNOWN_NAME);
-        if (s.jacobian == RDST_JACOBIAN_EXACT &&
-            (s.reconstruction != RDST_RECON_UPWIND || s.viscous || s.flux_scheme == RDST_FLUX_HLLCP))
-            throw ArgError("rdst_case_create: the exact Jacobian covers the first-order inviscid residual "
-                           "(upwind reconstruction, viscous 0, not HLLCP)");
+        if (s.jacobian == RDST_JACOBIAN_EXACT && (s.reconstruction != RDST_RECON_UPWIND || s.flux_scheme == RDST_FLUX_HLLCP))
+            throw ArgError("rdst_case_create: the exact Jacobian covers the first-order residual "
+                           "(upwind reconstruction, not HLLCP)");
         if (s.time_integration != RDST_TIME_SINGLE && s.time_integration != RDST_TIME_DUAL)
             throw ArgError("Unknown timeIntegration; valid: Single Dual", RDST_ERR_UNKNOWN_NAME);
         if (s.time_integration == RDST_TIME_DUAL) {
@@ -669,9 +668,9 @@ int rdst_case_set_turbulence(rdst_case* cc, const rdst_turbulence* turb, const r
         Case* c = reinterpret_cast<Case*>(cc);
         if (!c || !turb) throw ArgError("rdst_case_set_turbulence: null argument");
         if (turb->model != RDST_TURB_LAMINAR && turb->model != RDST_TURB_SANEG && turb->model != RDST_TURB_SARC2 &&
-            turb->model != RDST_TURB_SARC1)
-            throw ArgError("rdst_case_set_turbulence: unknown RASModel id (valid: laminar SAneg SARC2 SARC1)",
-                           RDST_ERR_UNKNOWN_NAME);
+            turb->model != RDST_TURB_SARC1 && turb->model != RDST_TURB_SA)
+            throw ArgError("rdst_case_set_turbulence: unknown RASModel id (valid: laminar SAneg SARC2 SARC1 "
+                           "SpalartAllmaras)", RDST_ERR_UNKNOWN_NAME);
         if (turb->model != RDST_TURB_LAMINAR) {
             if (c->st.time_integration == RDST_TIME_DUAL)
                 throw ArgError("rdst_case_set_turbulence: the Spalart-Allmaras models run with Single time "

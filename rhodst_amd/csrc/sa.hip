@@ -576,6 +576,12 @@ void sa_set(Case& c, const rdst_turbulence* t, const rdst_scalar_bc* bcs, const 
     const size_t n = m.nC, nB = std::max(m.nB, 1);
     s.on = false;
     s.t = *t;
+    // SpalartAllmaras: SARC2's equation with fr1 = 1 (SARC2.C:121-140 with
+    // RCMCorrection off is foam-extend's stock model)
+    if (s.t.model == RDST_TURB_SA) {
+        s.t.model = RDST_TURB_SARC2;
+        s.t.rcm_correction = 0;
+    }
     s.nCorrect = 0;
     s.rhoPrevValid = false;
     // laminar: mut = alphat = 0

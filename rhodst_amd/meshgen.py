@@ -353,6 +353,16 @@ def read_polymesh(case_dir: str) -> PolyMesh:
                     n_cells=n_cells, patches=patches)
 
 
+def polymesh_from_npz(path: str) -> PolyMesh:
+    """A polyMesh stored as arrays (points, face_offsets, face_points, owner,
+    neighbour, n_cells, patch_names, patch_meta [type, start, size])."""
+    with np.load(path, allow_pickle=False) as z:
+        patches = [(str(n), int(t), int(s), int(c)) for n, (t, s, c) in zip(z["patch_names"], z["patch_meta"])]
+        return PolyMesh(points=np.ascontiguousarray(z["points"]), face_offsets=np.ascontiguousarray(z["face_offsets"]),
+                        face_points=np.ascontiguousarray(z["face_points"]), owner=np.ascontiguousarray(z["owner"]),
+                        neighbour=np.ascontiguousarray(z["neighbour"]), n_cells=int(z["n_cells"]), patches=patches)
+
+
 def write_polymesh(mesh: PolyMesh, out_dir: str) -> None:
     """Write the mesh as an ASCII foam polyMesh."""
     os.makedirs(out_dir, exist_ok=True)
