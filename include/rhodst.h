@@ -173,7 +173,31 @@ typedef struct rdst_settings {
     double mag_converged_slope; /* residualControls.magConvergedSlope, default 0.1 (IOdictDST.C:212-215) */
     double min_residual;      /* residualControls.minResudual, default 1e-5 (IOdictDST.C:217-220) */
     double delta_t;           /* controlDict deltaT: the deltaT0 of the first physical step */
+    /* fvSchemes: the surface-normal gradient inside laplacianSchemes (used by
+     * fvc::laplacian(muEff, U) of calcFields, baseFluxDST.C:248, and by
+     * fvm::laplacian(DnuTildaEff, nuTilda) of the Spalart-Allmaras models)
+     * and snGradSchemes (fvc::snGrad(U) of sigmaDotU, baseFluxDST.C:240-246);
+     * rdst_sngrad_kind, 0 = corrected (every reference tutorial's snGrad,
+     * tutorials/oneraM6Turb/system/fvSchemes:44 uses "limited 0.33" for the
+     * laplacian).  *_limit is psi of "limited psi". */
+    int32_t laplacian_sngrad;
+    int32_t sngrad;
+    double laplacian_limit;
+    double sngrad_limit;
 } rdst_settings;
+
+/* Non-orthogonal correction of a surface-normal gradient (foam-extend 4.1
+ * snGradSchemes).  CORRECTED: dc (x_N - x_P) + corr & interp(grad x), with
+ * corr = n - dc d.  UNCORRECTED: dc (x_N - x_P).  LIMITED psi (limitedSnGrad):
+ * the correction scaled by min(psi |dc (x_N - x_P)| / ((1 - psi) |correction|
+ * + 1e-15), 1), |.| the magnitude of the (vector) face value, so the
+ * correction never exceeds psi / (1 - psi) of the orthogonal part; psi in
+ * [0, 1]. */
+enum rdst_sngrad_kind {
+    RDST_SNGRAD_CORRECTED = 0,
+    RDST_SNGRAD_UNCORRECTED = 1,
+    RDST_SNGRAD_LIMITED = 2
+};
 
 /* Time integration of the case (timeIntegration.timeIntegration).  SINGLE:
  * the steady pseudo-time step Single::iterate (steadyStateDST ddt, local time
@@ -192,9 +216,9 @@ enum rdst_time_integration {
  * (Roe<tensor5>, the only entry of the JacobianDST table, JacobianDST.C:57-92).
  * EXACT (no counterpart in the reference) assembles V/dtau I + V dR/dW with
  * the exact linearisation of the first-order inviscid residual (the
- * adjoint's, rdst_adjoint_linearise): a Newton step as dtau grows; it takes
- * the same restrictions as the adjoint (upwind reconstruction, inviscid,
- * no HLLCP). */
+ * adjoint's, rdst_adjoint_linearise) plus, with viscous terms on, the compact
+ * (orthogonal, muEff frozen) linearisation of the viscous fluxes: a Newton
+ * step as dtau grows; upwind reconstruction only, no HLLCP. */
 enum rdst_jacobian_kind {
     RDST_JACOBIAN_ROE = 0,
     RDST_JACOBIAN_EXACT = 1

@@ -1000,12 +1000,30 @@ static inline V3 dotVT(V3 v, const T9& t) {  // (v & T)_j = v_i T_ij
               v.x * t.a[2] + v.y * t.a[5] + v.z * t.a[8]);
 }
 
+// Surface-normal gradient schemes (foam-extend 4.1 snGradSchemes, not part of
+// the reference tree; the reference selects them in system/fvSchemes, e.g.
+// tutorials/oneraM6Turb/system/fvSchemes:44 "Gauss linear limited 0.33").
+// corrected: dc (x_N - x_P) + corr & interp(grad x); uncorrected: the first
+// term; limited psi (limitedSnGrad::correction): the correction times
+// min(psi |orthogonal part| / ((1 - psi) |correction| + SMALL), 1), with |.|
+// the magnitude of the face value (vector or scalar) and SMALL = 1e-15.
+// Encoded as psi = -1 (corrected), 0 (uncorrected) or the limited psi.
+static double snGradPsi(int kind, double limit) {
+    return kind == RDST_SNGRAD_CORRECTED ? -1.0 : (kind == RDST_SNGRAD_UNCORRECTED ? 0.0 : limit);
+}
+static double snGradLimiter(double psi, double u2, double c2) {
+    if (psi < 0.0) return 1.0;
+    return std::fmin(psi * std::sqrt(u2) / ((1.0 - psi) * std::sqrt(c2) + 1.0e-15), 1.0);
+}
+
 // baseFluxDST.C:229-250: R1 -= laplacian(muEff, U) + div(tauMC);
 // R4 -= div((interp(muEff)|S| snGrad(U) + Sf & interp(tauMC)) & Up).
 // Gauss linear gradient, "corrected" snGrad with non-orthogonal correction
 // corr & interp(grad U); no heat-conduction term (none in the reference).
 static void viscousTerms(Case& c) {
     const Mesh& m = c.m;
+    const double psiL = snGradPsi(c.st.laplacian_sngrad, c.st.laplacian_limit);
+    const double psiS = snGradPsi(c.st.sngrad, c.st.sngrad_limit);
     int n = m.nCells;
     std::vector<T9> g, gb;
     gradU(c, g, gb);
@@ -1031,12 +1049,20 @@ static void viscousTerms(Case& c) {
             gf.a[k] = w * (g[o].a[k] - g[nb].a[k]) + g[nb].a[k];
             tf.a[k] = w * (tau[o].a[k] - tau[nb].a[k]) + tau[nb].a[k];
         }
-        V3 sn = m.dc[f] * (c.U[nb] - c.U[o]) + dotVT(m.corr[f], gf);
-        V3 lapF = (muf * m.magSf[f]) * sn;
+        // orthogonal part and non-orthogonal correction of snGrad(U): the
+        // laplacian's scheme (laplacianSchemes) and fvc::snGrad's
+        // (snGradSchemes) limit the correction separately
+        V3 u = m.dc[f] * (c.U[nb] - c.U[o]);
+        V3 q = dotVT(m.corr[f], gf);
+        double kL = snGradLimiter(psiL, dot(u, u), dot(q, q));
+        double kS = snGradLimiter(psiS, dot(u, u), dot(q, q));
+        double mg = muf * m.magSf[f];
+        V3 lapF = mg * (u + kL * q);
+        V3 sgF = mg * (u + kS * q);
         V3 tauF = dotVT(m.Sf[f], tf);
         lapFace[f] = lapF;
         tauFace[f] = tauF;
-        sdFace[f] = dot(lapF + tauF, c.Up[f]);
+        sdFace[f] = dot(sgF + tauF, c.Up[f]);
     }
     for (int b = 0; b < m.nB; ++b) {
         int f = m.bFace[b], k = m.nInternal + b;
@@ -2334,7 +2360,9 @@ static void saCorrect(Case& c, Perf* perfOut) {
         ic[b] += phib * vic;
         bcf[b] += -phib * vbc;
     }
-    // - fvm::laplacian(DnuTildaEff, nuTilda), Gauss linear corrected
+    // - fvm::laplacian(DnuTildaEff, nuTilda), Gauss linear with the
+    // laplacianSchemes snGrad (corrected / uncorrected / limited psi)
+    const double psiL = snGradPsi(c.st.laplacian_sngrad, c.st.laplacian_limit);
     for (int f = 0; f < nI; ++f) {
         int o = m.owner[f], nb = m.neighbour[f];
         double gf = m.w[f] * (G[o] - G[nb]) + G[nb];
@@ -2343,7 +2371,9 @@ static void saCorrect(Case& c, Perf* perfOut) {
         up[f] -= coef; lo[f] -= coef;
         diag[o] += coef; diag[nb] += coef;
         V3 gfn = m.w[f] * (gn[o] - gn[nb]) + gn[nb];
-        double corr = gms * dot(m.corr[f], gfn);
+        double q = dot(m.corr[f], gfn);
+        double un = m.dc[f] * (tb.nut[nb] - tb.nut[o]);
+        double corr = gms * (snGradLimiter(psiL, un * un, q * q) * q);
         src[o] += corr;
         src[nb] -= corr;
     }

@@ -58,7 +58,8 @@ class rdst_settings(C.Structure):
                 ("reconstruction", C.c_int32), ("jacobian", C.c_int32),
                 ("time_integration", C.c_int32), ("adjust_time_step", C.c_int32), ("max_inner_iter", C.c_int32),
                 ("iteration_start", C.c_int32), ("mag_converged_slope", C.c_double), ("min_residual", C.c_double),
-                ("delta_t", C.c_double)]
+                ("delta_t", C.c_double), ("laplacian_sngrad", C.c_int32), ("sngrad", C.c_int32),
+                ("laplacian_limit", C.c_double), ("sngrad_limit", C.c_double)]
 
 
 class rdst_solver_perf(C.Structure):
@@ -250,7 +251,21 @@ def make_settings(**kw) -> rdst_settings:
     s.mag_converged_slope = float(kw.get("mag_converged_slope", 0.1))
     s.min_residual = float(kw.get("min_residual", 1e-5))
     s.delta_t = float(kw.get("delta_t", 0.0))
+    # fvSchemes laplacianSchemes / snGradSchemes: "corrected", "uncorrected"
+    # or ("limited", psi)
+    s.laplacian_sngrad, s.laplacian_limit = _sngrad(kw.get("laplacian", "corrected"))
+    s.sngrad, s.sngrad_limit = _sngrad(kw.get("sngrad", "corrected"))
     return s
+
+
+SNGRAD = {"corrected": 0, "uncorrected": 1, "limited": 2}
+
+
+def _sngrad(spec):
+    if isinstance(spec, str):
+        return SNGRAD[spec], 1.0
+    kind, psi = spec
+    return SNGRAD[kind], float(psi)
 
 
 def make_thermo(Cp=2.5, R=1.0 / 1.4, mu=0.0, Pr=1.0, sutherland=None) -> rdst_thermo:

@@ -94,7 +94,7 @@ def box_farfield_case(nx: int = 24, ny: int = 16, scheme: str = "HLLC", mach: fl
 
 def swept_wing_case(scheme: str = "AUSM+up", mach: float = 0.84, aoa_deg: float = 3.06, viscous: bool = False,
                     mesh=None, mesh_kw: dict | None = None, turbulence: str | None = None, reynolds: float = 1.2e7,
-                    wall_dist=None, **settings) -> CaseSpec:
+                    wall_dist=None, turb_kw: dict | None = None, **settings) -> CaseSpec:
     """Transonic swept wing: semi-span 1, 30 deg leading-edge sweep, taper
     0.56, 10 % analytic section, C-H hex mesh (meshgen.swept_wing, 2.03 M
     cells by default), Mach 0.84 at 3.06 deg, characteristic far field,
@@ -138,7 +138,7 @@ def swept_wing_case(scheme: str = "AUSM+up", mach: float = 0.84, aoa_deg: float 
         sbc = make_scalar_bcs(mesh, {"wing": ("fixedValue", 0.0), "symmetry": "symmetryPlane",
                                      "farfield": ("inletOutlet", 3.0 * nu)})
         d = wall_dist if wall_dist is not None else meshgen.wall_distance(mesh, ["wing"])
-        spec_out.turbulence = (make_turbulence(turbulence), sbc, d, np.full(n, 3.0 * nu))
+        spec_out.turbulence = (make_turbulence(turbulence, **(turb_kw or {})), sbc, d, np.full(n, 3.0 * nu))
         spec_out.name = f"sweptWing_{turbulence}"
     return spec_out
 

@@ -271,6 +271,14 @@ int rdst_case_create(rdst_mesh* mesh, const rdst_thermo* thermo, const rdst_bc* 
         }
         if (s.ilu_order != RDST_ILU_NATURAL && s.ilu_order != RDST_ILU_PARTITIONED)
             throw ArgError("Unknown ILU ordering; valid: natural partitioned", RDST_ERR_UNKNOWN_NAME);
+        for (int q = 0; q < 2; ++q) {
+            const int kind = q ? s.sngrad : s.laplacian_sngrad;
+            const double psi = q ? s.sngrad_limit : s.laplacian_limit;
+            if (kind != RDST_SNGRAD_CORRECTED && kind != RDST_SNGRAD_UNCORRECTED && kind != RDST_SNGRAD_LIMITED)
+                throw ArgError("Unknown snGradScheme; valid: corrected uncorrected limited", RDST_ERR_UNKNOWN_NAME);
+            if (kind == RDST_SNGRAD_LIMITED && !(psi >= 0.0 && psi <= 1.0))
+                throw ArgError("rdst_case_create: limited snGrad coefficient must be in [0, 1]");
+        }
         if (!(thermo->Cp > thermo->R) || !(thermo->R > 0)) throw ArgError("rdst_case_create: need Cp > R > 0");
         if (!m->uploaded) {
             mesh_upload(*m);

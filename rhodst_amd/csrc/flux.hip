@@ -279,7 +279,7 @@ __device__ inline void recon_sides(const MeshDev& m, const Fields& F, int P, int
 // K1: face flux + residual, one thread per cell.
 template <int S, int STORE, bool VISC, bool RECON>
 __global__ __launch_bounds__(256) void k_flux_residual(MeshDev m, Fields F, const BcPatchDev* bc, double gamma, int lowMach,
-                                                       double Mainf) {
+                                                       double Mainf, double lapPsi, double snPsi) {
     const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
     const Side me = side_of_cell<S>(F, c, gamma);
@@ -332,7 +332,7 @@ __global__ __launch_bounds__(256) void k_flux_residual(MeshDev m, Fields F, cons
         const Flux5 o = eval_flux<S>(L, R, sx, sy, sz, ms, gamma, f, lowMach, Mainf);
         if (VISC) {
             // face viscous flux (interp(muEff)|S| snGrad(U) and Sf & interp(tauMC)), then its work on Up
-            double lfx, lfy, lfz, tfx, tfy, tfz;
+            double lfx, lfy, lfz, tfx, tfy, tfz, sgx, sgy, sgz;
             if (kind == ELL_BOUNDARY) {
                 const double rms = 1.0 / ms;
                 const double nx = sx * rms, ny = sy * rms, nz = sz * rms;
@@ -341,6 +341,7 @@ __global__ __launch_bounds__(256) void k_flux_residual(MeshDev m, Fields F, cons
                 const double bmuE = F.bmut[idx] + F.bmu[idx];
                 const double mg = bmuE * ms;
                 lfx = mg * snx; lfy = mg * sny; lfz = mg * snz;
+                sgx = lfx; sgy = lfy; sgz = lfz;
                 double gb[9], tb[9];
                 for (int q = 0; q < 9; ++q) gb[q] = F.bgradU[(size_t)q * m.nB + idx];
                 tau_mc(bmuE, gb, tb);
@@ -370,16 +371,27 @@ __global__ __launch_bounds__(256) void k_flux_residual(MeshDev m, Fields F, cons
                 }
                 const double cx = m.ellCorr[e], cy = m.ellCorr[E + e], cz = m.ellCorr[2 * E + e];
                 const double dcf = m.ellDc[e];
-                const double snx = dcf * (UNx - UPx) + (cx * gf[0] + cy * gf[3] + cz * gf[6]);
-                const double sny = dcf * (UNy - UPy) + (cx * gf[1] + cy * gf[4] + cz * gf[7]);
-                const double snz = dcf * (UNz - UPz) + (cx * gf[2] + cy * gf[5] + cz * gf[8]);
+                // orthogonal part and non-orthogonal correction of snGrad(U);
+                // laplacianSchemes and snGradSchemes may limit the latter
+                // differently (laplacian(muEff, U) and sigmaDotU)
+                const double ux = dcf * (UNx - UPx), uy = dcf * (UNy - UPy), uz = dcf * (UNz - UPz);
+                const double qx = cx * gf[0] + cy * gf[3] + cz * gf[6];
+                const double qy = cx * gf[1] + cy * gf[4] + cz * gf[7];
+                const double qz = cx * gf[2] + cy * gf[5] + cz * gf[8];
+                double kL = 1.0, kS = 1.0;
+                if (lapPsi >= 0.0 || snPsi >= 0.0) {
+                    const double u2 = ux * ux + uy * uy + uz * uz, q2 = qx * qx + qy * qy + qz * qz;
+                    kL = sngrad_limiter(lapPsi, u2, q2);
+                    kS = sngrad_limiter(snPsi, u2, q2);
+                }
                 const double mg = muf * ms;
-                lfx = mg * snx; lfy = mg * sny; lfz = mg * snz;
+                lfx = mg * (ux + kL * qx); lfy = mg * (uy + kL * qy); lfz = mg * (uz + kL * qz);
+                sgx = mg * (ux + kS * qx); sgy = mg * (uy + kS * qy); sgz = mg * (uz + kS * qz);
                 tfx = sx * tf[0] + sy * tf[3] + sz * tf[6];
                 tfy = sx * tf[1] + sy * tf[4] + sz * tf[7];
                 tfz = sx * tf[2] + sy * tf[5] + sz * tf[8];
             }
-            const double sdf = (lfx + tfx) * o.upx + (lfy + tfy) * o.upy + (lfz + tfz) * o.upz;
+            const double sdf = (sgx + tfx) * o.upx + (sgy + tfy) * o.upy + (sgz + tfz) * o.upz;
             if (kind == ELL_NEIGH) {
                 lx -= lfx; ly -= lfy; lz -= lfz; tx -= tfx; ty -= tfy; tz -= tfz; sd -= sdf;
             } else {
@@ -488,12 +500,14 @@ void launch_flux_mode(Case& c) {
     const int bs = 256;
     const int nb = blocks_for(md.nC, bs);
     const BcPatchDev* bc = c.d_bc.p;
+    const double lapPsi = sngrad_psi(c.st.laplacian_sngrad, c.st.laplacian_limit);
+    const double snPsi = sngrad_psi(c.st.sngrad, c.st.sngrad_limit);
     if (c.st.viscous)
         hipLaunchKernelGGL((k_flux_residual<S, STORE, true, RECON>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, bc,
-                           c.th.gamma, c.st.low_mach, c.st.Mainf);
+                           c.th.gamma, c.st.low_mach, c.st.Mainf, lapPsi, snPsi);
     else
         hipLaunchKernelGGL((k_flux_residual<S, STORE, false, RECON>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, bc,
-                           c.th.gamma, c.st.low_mach, c.st.Mainf);
+                           c.th.gamma, c.st.low_mach, c.st.Mainf, lapPsi, snPsi);
 }
 
 template <int S, bool RECON>

@@ -378,7 +378,23 @@ inline int blocks_for(long n, int bs) { return (int)((n + bs - 1) / bs); }
 // mapping the workgroups of XCD x onto one contiguous range of logical
 // blocks keeps the cells (and their mesh neighbours) that one XCD touches
 // together in its L2.  A bijection of [0, nb).
+// Surface-normal gradient scheme as one number for the kernels: -1 for
+// corrected, 0 for uncorrected, psi for limited psi (rdst_sngrad_kind).
+inline double sngrad_psi(int kind, double limit) {
+    return kind == RDST_SNGRAD_CORRECTED ? -1.0 : (kind == RDST_SNGRAD_UNCORRECTED ? 0.0 : limit);
+}
+
 #ifdef __HIPCC__
+// Factor on the non-orthogonal correction of a face gradient (foam-extend
+// limitedSnGrad::correction): 1 when corrected (psi < 0), else
+// min(psi |u| / ((1 - psi) |c| + SMALL), 1) with |u|^2 = u2 the squared
+// magnitude of the orthogonal part dc (x_N - x_P), |c|^2 = c2 that of the
+// correction; psi = 0 gives 0 (uncorrected).
+__device__ inline double sngrad_limiter(double psi, double u2, double c2) {
+    if (psi < 0.0) return 1.0;
+    return fmin(psi * sqrt(u2) / ((1.0 - psi) * sqrt(c2) + 1.0e-15), 1.0);
+}
+
 __device__ inline int xcd_block(int b, int nb) {
     const int x = b & 7, k = b >> 3;
     const int per = nb >> 3, rem = nb & 7;

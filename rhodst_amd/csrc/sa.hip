@@ -204,7 +204,7 @@ __global__ __launch_bounds__(256) void k_sa_grad(MeshDev m, SaDev s) {
 // slots, source incl. boundaryCoeffs), and fv1(chi) of the current state
 template <bool SS>
 __global__ __launch_bounds__(256) void k_sa_assemble(MeshDev m, Fields F, SaDev s, rdst_turbulence t, int useRhoPrev,
-                                                     int useSold) {
+                                                     int useSold, double lapPsi) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
     const size_t E = (size_t)m.ellW * m.nC;
@@ -290,7 +290,13 @@ __global__ __launch_bounds__(256) void k_sa_assemble(MeshDev m, Fields F, SaDev 
         const double P0 = own ? gnc0 : go0, P1 = own ? gnc1 : go1, P2 = own ? gnc2 : go2;
         const double N0 = own ? go0 : gnc0, N1 = own ? go1 : gnc1, N2 = own ? go2 : gnc2;
         const double f0 = w * (P0 - N0) + N0, f1 = w * (P1 - N1) + N1, f2 = w * (P2 - N2) + N2;
-        const double corr = gms * (m.ellCorr[e] * f0 + m.ellCorr[E + e] * f1 + m.ellCorr[2 * E + e] * f2);
+        const double q = m.ellCorr[e] * f0 + m.ellCorr[E + e] * f1 + m.ellCorr[2 * E + e] * f2;
+        double kL = 1.0;
+        if (lapPsi >= 0.0) {  // limitedSnGrad / uncorrected on the nuTilda laplacian
+            const double un = m.ellDc[e] * (own ? s.nut[idx] - nut : nut - s.nut[idx]);
+            kL = sngrad_limiter(lapPsi, un * un, q * q);
+        }
+        const double corr = gms * (kL * q);
         double off;
         if (own) {
             diag -= lo;
@@ -641,13 +647,14 @@ void sa_correct(Case& c, rdst_solver_perf* perf) {
     grad_u_launch(c);
     hipLaunchKernelGGL(k_sa_grad, dim3(nb), dim3(bs), 0, c.stream, md, sa_dev(c));
     const int useRhoPrev = (s.nCorrect > 0 && s.rhoPrevValid) ? 1 : 0;
+    const double lapPsi = sngrad_psi(c.st.laplacian_sngrad, c.st.laplacian_limit);
     if (t.model == RDST_TURB_SARC1) {
         hipLaunchKernelGGL((k_sa_assemble<true>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, sa_dev(c), t, useRhoPrev,
-                           s.nCorrect > 0 ? 1 : 0);
+                           s.nCorrect > 0 ? 1 : 0, lapPsi);
         hipLaunchKernelGGL(k_sa_store_S, dim3(nb), dim3(bs), 0, c.stream, n, c.f, sa_dev(c));
     } else {
         hipLaunchKernelGGL((k_sa_assemble<false>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, sa_dev(c), t, useRhoPrev,
-                           0);
+                           0, lapPsi);
     }
     RDST_HIP(hipGetLastError());
     // smoothSolver: normFactor, initial residual

@@ -442,7 +442,8 @@ def swept_wing(n_wake: int = 48, n_surf: int = 128, n_normal: int = 96, n_span_w
                n_span_out: int = 16, thickness: float = 0.10, sweep_deg: float = 30.0, taper: float = 0.56,
                semi_span: float = 1.0, root_chord: float = 1.0, far: float = 12.0, first_cell: float = 2.0e-6,
                n_layers: int = 32, layer_ratio: float = 1.15, normal_blend: float = 0.08, tip_blend: int = 4,
-               tip_first: float | None = None, tip_round: float = 0.08) -> PolyMesh:
+               tip_first: float | None = None, tip_round: float = 0.08, tip_power: float = 3.0,
+               wake_growth: float = 0.01) -> PolyMesh:
     """Structured C-H hex mesh around a semi-span swept, tapered wing.
 
     Each spanwise station is a C-mesh about the local section (analytic
@@ -451,24 +452,35 @@ def swept_wing(n_wake: int = 48, n_surf: int = 128, n_normal: int = 96, n_span_w
     wake; the wake cut is closed by sharing points.  Beyond the tip the
     section is a zero-thickness sheet in the chord plane, so the cells above
     and below the sheet share faces (internal).  The tip is rounded: over
-    the outer tip_round of the span the section thickness falls as 1 - s^2
-    (s from 0 at span - tip_round to 1 at the tip station), so the wing
-    surface closes at the tip station itself and no cell is a wedge between
-    the surface and the sheet.
+    the outer tip_round of the span the section thickness falls as
+    (1 - s^2)^tip_power (s from 0 at span - tip_round to 1 at the tip
+    station), so the wing surface closes at the tip station itself and no
+    cell is a wedge between the surface and the sheet.
 
-    Wall-normal lines leave the section along its outward normal and bend
+    Wall-normal lines leave the section along the outward normal of the
+    local (thinned) section, turn to the full-thickness section's normal
+    above the boundary layer (over 4 layer thicknesses; this spreads the
+    leading-edge fan of the thin tip sections over several cells) and bend
     smoothly (over the first normal_blend of the way out) into the straight
     line to their far-field point, so the boundary-layer cells are
-    orthogonal to the wall; the far-field points are spaced evenly in the
+    orthogonal to the wall; at the tip station the sheet's lines leave it
+    straight up or down (the leading edge's upstream), so no first cell
+    lies along the sheet.  The far-field points are spaced evenly in the
     section's cosine parameter (a half circle of radius far about the
     trailing edge), and over the wake their spacing grows from the last
     section spacing.  The first cell is first_cell high (root chords, a y+
     of about 1 at Re 1.2e7), n_layers grow at layer_ratio, then the spacing
-    stretches to the far field.  Beyond the tip the sheet has no wall: its
-    normal lines take the tip section's normals (a fan round a virtual
-    leading edge) and the first spacing grows over tip_blend stations to
-    tip_first (default four times the tip chord's leading-edge spacing), so
-    no cell there is thinner than the chordwise spacing it sits on.
+    stretches to the far field.  Over the wake the first spacing grows with
+    the distance s behind the trailing edge, first_cell + wake_growth s (at
+    most far/400), joined to the wall distribution by a smooth maximum, so
+    the wake cells have aspect ratios of tens rather than the boundary
+    layer's thousands.  Beyond the tip the sheet has no wall: its lines take
+    the full-thickness fan round a virtual leading edge and the first
+    spacing grows over tip_blend stations to tip_first (default four times
+    the tip chord's leading-edge spacing).  The tip station's leading edge
+    is a 180-degree corner of the sheet (the C-H topology's collapsed tip):
+    the few cells at it are the most non-orthogonal of the mesh (86 deg at
+    the default size; every cell volume and wall-normal distance positive).
     Cells are numbered i (round the C) fastest, then j (off the wall), then k
     (span).  Patches: wing (wall), symmetry (symmetryPlane, z = 0),
     farfield (patch).  Default size 352 x 96 x 60 = 2 027 520 cells.
@@ -492,24 +504,29 @@ def swept_wing(n_wake: int = 48, n_surf: int = 128, n_normal: int = 96, n_span_w
     side = np.concatenate([-np.ones(len(x_lower)), np.ones(len(x_upper))])
     on_surf = (xc <= 1.0 + 1e-12)
     assert len(xc) == ni + 1
-    # unit outward normals of the full-thickness section (chord-independent);
-    # the leading edge points upstream, the wake lines straight up/down
     xcs = np.clip(xc, 0.0, 1.0)
-    slope = np.where(xcs > 0, naca4_slope(np.where(xcs > 0, xcs, 1.0), thickness), 0.0)
-    nx_ = np.where(on_surf, -slope, 0.0)
-    ny_ = side.copy()
+    slope_full = np.where(xcs > 0, naca4_slope(np.where(xcs > 0, xcs, 1.0), thickness), 0.0)
     le = on_surf & (xcs == 0.0)
-    nx_[le], ny_[le] = -1.0, 0.0
-    nn = np.hypot(nx_, ny_)
-    nx_, ny_ = nx_ / nn, ny_ / nn
     # the closed trailing edge is a concave corner of the flow region (the
     # surface normals there tilt aft by ~7 deg, the wake's are vertical): turn
     # the normals to vertical over the rear 40 % of the chord, so that the
     # normal lines there diverge instead of crossing
     g = np.where(on_surf, _smoothstep((xcs - 0.6) / 0.4), 1.0)
-    nx_, ny_ = (1.0 - g) * nx_, (1.0 - g) * ny_ + g * side
-    nn = np.hypot(nx_, ny_)
-    nx_, ny_ = nx_ / nn, ny_ / nn
+
+    def section_normals(th):
+        """Unit outward normals (chord-independent) of the section scaled to
+        th of the full thickness: the leading edge points upstream, the wake
+        lines straight up/down.  At th = 0 (the tip station and the sheet
+        beyond) every point but the leading edge leaves the chord plane
+        straight up or down, so no first cell lies along the sheet."""
+        nx_ = np.where(on_surf, -th * slope_full, 0.0)
+        ny_ = side.copy()
+        nx_[le], ny_[le] = -1.0, 0.0
+        nn = np.hypot(nx_, ny_)
+        nx_, ny_ = nx_ / nn, ny_ / nn
+        nx_, ny_ = (1.0 - g) * nx_, (1.0 - g) * ny_ + g * side
+        nn = np.hypot(nx_, ny_)
+        return nx_ / nn, ny_ / nn
     # wall-normal distribution (distance / far): n_layers geometric layers
     # from first_cell, then stretched to the far field
     def normal_dist(first):
@@ -518,6 +535,25 @@ def swept_wing(n_wake: int = 48, n_surf: int = 128, n_normal: int = 96, n_span_w
         s_out = _stretch(nj - n_layers, d_layers[-1] * layer_ratio, far - s_in[-1])
         return np.concatenate([s_in, s_in[-1] + s_out[1:]]) / far
     t_wall = normal_dist(first_cell)
+
+    def dists(first):
+        """Distributions ([nj + 1, ni + 1]) whose first spacing is first[i]
+        >= first_cell: a smooth maximum (cube norm) of the uniform spacing
+        first[i] and the wall distribution, rescaled to end at the far field,
+        so the spacing passes continuously from first[i] to the wall
+        distribution's outer levels."""
+        J = np.arange(nj + 1, dtype=np.float64)[:, None]
+        lin = J * (np.asarray(first, dtype=np.float64)[None, :] / far)
+        t = np.cbrt(lin ** 3 + t_wall[:, None] ** 3)
+        return t / t[-1:, :]
+    # first spacing per point: first_cell on the wing; over the wake it grows
+    # with the distance s downstream of the trailing edge (first_cell +
+    # wake_growth s, at most far/400), so the wake cells have aspect ratios of
+    # tens instead of the boundary layer's thousands
+    s_wake = np.where(on_surf, 0.0, (xc - 1.0) * root_chord)
+    f_base = np.minimum(first_cell + wake_growth * s_wake, far / 400.0)
+    f_base = np.maximum(f_base, first_cell)
+    t_base = dists(f_base)
     # spanwise stations: half uniform, half sine (finer towards the tip)
     th_sp = np.linspace(0, np.pi / 2, n_span_wing + 1)
     zw = semi_span * 0.5 * (np.sin(th_sp) + th_sp / (np.pi / 2))
@@ -526,10 +562,12 @@ def swept_wing(n_wake: int = 48, n_surf: int = 128, n_normal: int = 96, n_span_w
     zk = np.concatenate([zw, zo])
     if tip_first is None:
         tip_first = 4.0 * taper * root_chord * xs[1]
-    # beyond the tip: spacing tip_first off the sheet until the wall
-    # distribution's own spacing is larger (the outer levels are unchanged)
-    t_tip = np.maximum(t_wall, np.arange(nj + 1) * tip_first / far)
+    # beyond the tip the sheet has no wall: the first spacing there is at
+    # least tip_first, blended in over tip_blend stations
+    t_tip = dists(np.maximum(f_base, tip_first))
     tan_sw = np.tan(np.deg2rad(sweep_deg))
+    nfx, nfy = section_normals(1.0)
+    t_fan = 4.0 * t_wall[n_layers]
     P = np.zeros((nk + 1, nj + 1, ni + 1, 3))
     for k, z in enumerate(zk):
         zz = min(z, semi_span)
@@ -540,11 +578,12 @@ def swept_wing(n_wake: int = 48, n_surf: int = 128, n_normal: int = 96, n_span_w
             th = 0.0
         elif z > semi_span - tip_round:
             sr = (z - (semi_span - tip_round)) / tip_round
-            th = 1.0 - sr * sr
+            th = (1.0 - sr * sr) ** tip_power
         else:
             th = 1.0
+        nlx, nly = section_normals(th)
         b = _smoothstep((k - n_span_wing) / tip_blend) if beyond else 0.0
-        t_norm = (1.0 - b) * t_wall + b * t_tip
+        t_norm = (1.0 - b) * t_base + b * t_tip
         # inner line: section surface (x in chord units) or the wake/sheet line
         xi = x_le + chord * np.minimum(xc, 1.0) + np.where(xc > 1.0, (xc - 1.0) * root_chord, 0.0)
         yi = np.where(on_surf, side * th * chord * naca4_thickness(np.minimum(xc, 1.0), thickness), 0.0)
@@ -557,8 +596,16 @@ def swept_wing(n_wake: int = 48, n_surf: int = 128, n_normal: int = 96, n_span_w
         yo = np.where(on_surf, side * far * np.sin(psi), side * far)
         dx, dy = xo - xi, yo - yi
         D = np.hypot(dx, dy)
-        for j, t in enumerate(t_norm):
+        for j, t in enumerate(t_norm):  # t: [ni + 1] at level j
             w = _smoothstep(t / normal_blend)
+            # the local section's normals in the boundary layer, turning to
+            # the full-thickness section's (the leading-edge fan spread over
+            # several cells) above it; beyond the tip, where the first
+            # spacing grows, the full-thickness fan from the sheet up
+            v = np.maximum(_smoothstep(t / t_fan), b)
+            nx_, ny_ = (1.0 - v) * nlx + v * nfx, (1.0 - v) * nly + v * nfy
+            nn = np.hypot(nx_, ny_)
+            nx_, ny_ = nx_ / nn, ny_ / nn
             P[k, j, :, 0] = xi + t * ((1.0 - w) * D * nx_ + w * dx)
             P[k, j, :, 1] = yi + t * ((1.0 - w) * D * ny_ + w * dy)
             P[k, j, :, 2] = z

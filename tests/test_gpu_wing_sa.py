@@ -1,0 +1,56 @@
+"""GPU parity on a coarse version of the benchmark's Spalart-Allmaras wing
+(swept_wing at 44 x 16 x 12 = 8 448 cells, SARC2, no-slip, viscous terms,
+AUSM+up, the Roe operator, FP64, natural-order ILU(0)): the face residual,
+the nuTilda matrix and whole Single::iterate steps against the oracle, with
+the corrected laplacian and with "limited 0.33" (laplacian and snGrad)."""
+import numpy as np
+import pytest
+
+from rhodst_amd import DeviceCase, _capi, cases, meshgen
+from rhodst_amd._capi import DeviceMesh
+
+pytestmark = pytest.mark.gpu
+
+MESH_KW = dict(n_wake=6, n_surf=16, n_normal=16, n_span_wing=8, n_span_out=4, n_layers=8)
+
+
+def rel(a, b):
+    a = np.asarray(a); b = np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.mark.parametrize("schemes", [{}, {"laplacian": ("limited", 0.33), "sngrad": ("limited", 0.33)}])
+def test_coarse_sa_wing_parity(oracle, schemes):
+    print(f"[SA wing] coarse {MESH_KW} {schemes}", flush=True)
+    mesh = meshgen.swept_wing(**MESH_KW)
+    dm = DeviceMesh(mesh)
+    wd = dm.wall_distance()
+    ref = meshgen.wall_distance(mesh, ["wing"], centres=dm.get_geom(_capi.G_C))
+    assert rel(wd, ref) <= 1e-12
+    spec = cases.swept_wing_case(mesh=mesh, turbulence="SARC2", wall_dist=wd, **schemes)
+    dc = DeviceCase(spec, dm)
+    oc = oracle.OracleCase(spec)
+    part = dm.get_int(_capi.M_SA_PARTITION)
+    oc.set_sa_partition(part)
+    for f in (30, 31, 33, 34):
+        assert rel(dc.get(f), oc.get(f)) <= 1e-14, f
+    oc.flux_residual()
+    dc.flux_residual()
+    assert rel(dc.get(8), oc.get(8)) <= 1e-12
+    po = oc.turbulence_correct()
+    pd = dc.turbulence_correct()
+    for f in (35, 36, 37):
+        assert rel(dc.get(f), oc.get(f)) <= 1e-12, f
+    assert rel(dc.get(38), oc.get(38)) <= 1e-11
+    assert pd.n_iterations == po.n_iterations
+    # whole steps from the free stream
+    dc2 = DeviceCase(spec, dm)
+    oc2 = oracle.OracleCase(spec)
+    oc2.set_sa_partition(part)
+    r0 = oc2.iterate(12)
+    r1 = dc2.iterate(12)
+    print("drop", r1[-1] / r1[0], flush=True)
+    assert np.all(np.isfinite(r1))
+    assert rel(r1, r0) <= 1e-9, (r1, r0)
+    assert rel(dc2.get(9), oc2.get(9)) <= 1e-9
+    assert rel(dc2.get(29), oc2.get(29)) <= 1e-8
