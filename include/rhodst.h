@@ -184,6 +184,12 @@ typedef struct rdst_settings {
     int32_t sngrad;
     double laplacian_limit;
     double sngrad_limit;
+    /* Heat conduction in the energy residual (no counterpart in the
+     * reference: calcFields, baseFluxDST.C:223-255, has no heat-flux term).
+     * 1 adds -laplacian(alphaEff, h) with its orthogonal part,
+     * alphaEff_f |Sf| dc (h_N - h_P) (boundary faces: h_b - h_P), alphaEff =
+     * alpha + alphat; 0 (default) is the reference's energy equation. */
+    int32_t heat_conduction;
 } rdst_settings;
 
 /* Non-orthogonal correction of a surface-normal gradient (foam-extend 4.1

@@ -1085,12 +1085,37 @@ static void viscousTerms(Case& c) {
             }
         }
     }
+    // heat conduction (an extension, off in the reference's equations):
+    // R4 -= laplacian(alphaEff, h), orthogonal part, summed in face order
+    std::vector<double> hq(n, 0.0);
+    if (c.st.heat_conduction) {
+        std::vector<double> al, bal;
+        std::vector<double> mu_, bmu_;
+        muAlphaEff(c, mu_, bmu_, &al, &bal);
+        std::vector<double> qFace(nf);
+        for (int f = 0; f < m.nInternal; ++f) {
+            int o = m.owner[f], nb = m.neighbour[f];
+            double alf = m.w[f] * (al[o] - al[nb]) + al[nb];
+            qFace[f] = alf * m.magSf[f] * (m.dc[f] * (c.h[nb] - c.h[o]));
+        }
+        for (int b = 0; b < m.nB; ++b) {
+            int f = m.bFace[b], k = m.nInternal + b;
+            qFace[k] = bal[b] * m.magSf[f] * (m.dc[k] * (c.bh[b] - c.h[m.bCell[b]]));
+        }
+        for (int i = 0; i < n; ++i)
+            for (int q = m.cfStart[i]; q < m.cfStart[i + 1]; ++q) {
+                const int e = m.cfIdx[q];
+                if (e > 0) hq[i] += qFace[e - 1];
+                else hq[i] -= qFace[-e - 1];
+            }
+    }
     for (int i = 0; i < n; ++i) {
         double V = m.V[i];
         V3 L = lap[i] / V, D = dtau[i] / V;
         V3 s = L + D;
         c.R[5 * i + 1] -= s.x; c.R[5 * i + 2] -= s.y; c.R[5 * i + 3] -= s.z;
         c.R[5 * i + 4] -= sdu[i] / V;
+        if (c.st.heat_conduction) c.R[5 * i + 4] -= hq[i] / V;
     }
 }
 

@@ -279,6 +279,8 @@ int rdst_case_create(rdst_mesh* mesh, const rdst_thermo* thermo, const rdst_bc* 
             if (kind == RDST_SNGRAD_LIMITED && !(psi >= 0.0 && psi <= 1.0))
                 throw ArgError("rdst_case_create: limited snGrad coefficient must be in [0, 1]");
         }
+        if (s.heat_conduction != 0 && s.heat_conduction != 1)
+            throw ArgError("rdst_case_create: heat_conduction is 0 or 1");
         if (!(thermo->Cp > thermo->R) || !(thermo->R > 0)) throw ArgError("rdst_case_create: need Cp > R > 0");
         if (!m->uploaded) {
             mesh_upload(*m);

@@ -279,14 +279,14 @@ __device__ inline void recon_sides(const MeshDev& m, const Fields& F, int P, int
 // K1: face flux + residual, one thread per cell.
 template <int S, int STORE, bool VISC, bool RECON>
 __global__ __launch_bounds__(256) void k_flux_residual(MeshDev m, Fields F, const BcPatchDev* bc, double gamma, int lowMach,
-                                                       double Mainf, double lapPsi, double snPsi) {
+                                                       double Mainf, double lapPsi, double snPsi, int heat) {
     const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
     const Side me = side_of_cell<S>(F, c, gamma);
     const double fHc = (S == RDST_FLUX_HLLCP) ? F.fH[c] : 0.0;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
     // viscous accumulators: laplacian(muEff, U), div(tauMC), div(sigmaDotU)
-    double lx = 0.0, ly = 0.0, lz = 0.0, tx = 0.0, ty = 0.0, tz = 0.0, sd = 0.0;
+    double lx = 0.0, ly = 0.0, lz = 0.0, tx = 0.0, ty = 0.0, tz = 0.0, sd = 0.0, hq = 0.0;
     double gC[9], tauC[9], muC = 0.0, uCx = 0.0, uCy = 0.0, uCz = 0.0;
     if (VISC) {
         muC = F.mut[c] + F.mu[c];
@@ -332,7 +332,7 @@ __global__ __launch_bounds__(256) void k_flux_residual(MeshDev m, Fields F, cons
         const Flux5 o = eval_flux<S>(L, R, sx, sy, sz, ms, gamma, f, lowMach, Mainf);
         if (VISC) {
             // face viscous flux (interp(muEff)|S| snGrad(U) and Sf & interp(tauMC)), then its work on Up
-            double lfx, lfy, lfz, tfx, tfy, tfz, sgx, sgy, sgz;
+            double lfx, lfy, lfz, tfx, tfy, tfz, sgx, sgy, sgz, qh = 0.0;
             if (kind == ELL_BOUNDARY) {
                 const double rms = 1.0 / ms;
                 const double nx = sx * rms, ny = sy * rms, nz = sz * rms;
@@ -348,6 +348,8 @@ __global__ __launch_bounds__(256) void k_flux_residual(MeshDev m, Fields F, cons
                 tfx = sx * tb[0] + sy * tb[3] + sz * tb[6];
                 tfy = sx * tb[1] + sy * tb[4] + sz * tb[7];
                 tfz = sx * tb[2] + sy * tb[5] + sz * tb[8];
+                if (heat)  // alphaEff_b |Sf| dc (h_b - h_P)
+                    qh = (F.balphat[idx] + F.balpha[idx]) * ms * (m.ellDc[e] * (F.bh[idx] - F.h[c]));
             } else {
                 const bool own = kind == ELL_OWNER;
                 const double w = m.ellWt[e];
@@ -390,12 +392,18 @@ __global__ __launch_bounds__(256) void k_flux_residual(MeshDev m, Fields F, cons
                 tfx = sx * tf[0] + sy * tf[3] + sz * tf[6];
                 tfy = sx * tf[1] + sy * tf[4] + sz * tf[7];
                 tfz = sx * tf[2] + sy * tf[5] + sz * tf[8];
+                if (heat) {  // interp(alphaEff) |Sf| dc (h_N - h_P)
+                    const double alC = F.alphat[c] + F.alpha[c], alO = F.alphat[idx] + F.alpha[idx];
+                    const double alP = own ? alC : alO, alN = own ? alO : alC;
+                    const double hP = own ? F.h[c] : F.h[idx], hN = own ? F.h[idx] : F.h[c];
+                    qh = (w * (alP - alN) + alN) * ms * (dcf * (hN - hP));
+                }
             }
             const double sdf = (sgx + tfx) * o.upx + (sgy + tfy) * o.upy + (sgz + tfz) * o.upz;
             if (kind == ELL_NEIGH) {
-                lx -= lfx; ly -= lfy; lz -= lfz; tx -= tfx; ty -= tfy; tz -= tfz; sd -= sdf;
+                lx -= lfx; ly -= lfy; lz -= lfz; tx -= tfx; ty -= tfy; tz -= tfz; sd -= sdf; hq -= qh;
             } else {
-                lx += lfx; ly += lfy; lz += lfz; tx += tfx; ty += tfy; tz += tfz; sd += sdf;
+                lx += lfx; ly += lfy; lz += lfz; tx += tfx; ty += tfy; tz += tfz; sd += sdf; hq += qh;
             }
         }
         if (kind == ELL_NEIGH) {
@@ -422,6 +430,7 @@ __global__ __launch_bounds__(256) void k_flux_residual(MeshDev m, Fields F, cons
         r2 -= ly / V + ty / V;
         r3 -= lz / V + tz / V;
         r4 -= sd / V;
+        if (heat) r4 -= hq / V;
     }
     __builtin_nontemporal_store(a0 / V, F.R + c);
     __builtin_nontemporal_store(r1, F.R + n + c);
@@ -504,10 +513,10 @@ void launch_flux_mode(Case& c) {
     const double snPsi = sngrad_psi(c.st.sngrad, c.st.sngrad_limit);
     if (c.st.viscous)
         hipLaunchKernelGGL((k_flux_residual<S, STORE, true, RECON>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, bc,
-                           c.th.gamma, c.st.low_mach, c.st.Mainf, lapPsi, snPsi);
+                           c.th.gamma, c.st.low_mach, c.st.Mainf, lapPsi, snPsi, c.st.heat_conduction);
     else
         hipLaunchKernelGGL((k_flux_residual<S, STORE, false, RECON>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, bc,
-                           c.th.gamma, c.st.low_mach, c.st.Mainf, lapPsi, snPsi);
+                           c.th.gamma, c.st.low_mach, c.st.Mainf, lapPsi, snPsi, 0);
 }
 
 template <int S, bool RECON>

@@ -163,7 +163,7 @@ struct Mesh {
     std::vector<int64_t> sellPtr;
     std::vector<int32_t> sellWidth, sellCol;
     std::vector<int64_t> sellOfSlot;  // value index of entry 0 of each block-CSR slot
-    // wall-patch faces fanned from their first point: [nT][3 corners][3]
+    // wall-patch faces fanned from the mean of their points: [nT][3 corners][3]
     std::vector<double> wallTri;
     bool uploaded = false;  // device copies are made by the first rdst_case_create
 

@@ -2,7 +2,7 @@
 // wallDist d_ of the Spalart-Allmaras models (SAneg.C:160, SARC2.C:158 hold a
 // wallDist; foam-extend computes it by meshWave propagation).  Here it is the
 // exact Euclidean distance to the nearest wall triangle (each wall face fanned
-// from its first point), evaluated on the device: cells in blocks of 256
+// from the mean of its points), evaluated on the device: cells in blocks of 256
 // threads, the wall triangles streamed through LDS in tiles, each triangle
 // first rejected by its bounding sphere against the running minimum, then the
 // closest point found by the region tests of Ericson, Real-Time Collision

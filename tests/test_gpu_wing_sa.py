@@ -2,7 +2,8 @@
 (swept_wing at 44 x 16 x 12 = 8 448 cells, SARC2, no-slip, viscous terms,
 AUSM+up, the Roe operator, FP64, natural-order ILU(0)): the face residual,
 the nuTilda matrix and whole Single::iterate steps against the oracle, with
-the corrected laplacian and with "limited 0.33" (laplacian and snGrad)."""
+the corrected laplacian, with "limited 0.33" (laplacian and snGrad), and
+with the heat-conduction extension."""
 import numpy as np
 import pytest
 
@@ -19,7 +20,8 @@ def rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
 
 
-@pytest.mark.parametrize("schemes", [{}, {"laplacian": ("limited", 0.33), "sngrad": ("limited", 0.33)}])
+@pytest.mark.parametrize("schemes", [{}, {"laplacian": ("limited", 0.33), "sngrad": ("limited", 0.33)},
+                                     {"heat_conduction": 1}])
 def test_coarse_sa_wing_parity(oracle, schemes):
     print(f"[SA wing] coarse {MESH_KW} {schemes}", flush=True)
     mesh = meshgen.swept_wing(**MESH_KW)

@@ -91,3 +91,22 @@ def test_limited_ends_on_the_wing(oracle):
     assert np.array_equal(run(laplacian=("limited", 0.0), sngrad=("limited", 0.0)), Ru)
     R1 = run(laplacian=("limited", 1.0), sngrad=("limited", 1.0))
     assert np.abs(R1 - Rc).max() <= 1e-13 * np.abs(Rc).max()
+
+
+def test_heat_conduction_term_on_one_face(oracle):
+    """The heat-conduction extension on the two cells: with T zeroGradient on
+    every patch the only conductive flux is the shared face's,
+    alpha |Sf| dc (h_1 - h_0) with alpha = mu / Pr (constant transport),
+    h = Cp T, |Sf| = 1, dc = 1, V = 1: R4 of the cube drops by it, R4 of the
+    other cell rises by it, nothing else changes."""
+    U0, U1 = (10.0, 2.0, 1.0), (12.0, -3.0, 0.5)
+    spec0 = two_cells(U0, U1)
+    spec1 = two_cells(U0, U1, heat_conduction=1)
+    for s in (spec0, spec1):
+        s.T0 = np.array([300.0, 340.0])
+    R0 = residual(oracle, spec0)
+    R1 = residual(oracle, spec1)
+    q = (1e-3 / 0.72) * 1.0 * 1.0 * (1004.5 * 340.0 - 1004.5 * 300.0)
+    assert np.isclose(R1[0, 4] - R0[0, 4], -q, rtol=1e-12)
+    assert np.isclose(R1[1, 4] - R0[1, 4], q, rtol=1e-12)
+    assert np.array_equal(R1[:, :4], R0[:, :4])
