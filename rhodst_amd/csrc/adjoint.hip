@@ -242,7 +242,7 @@ __device__ inline void tau_mc_a(double mu, const double* g, double* t) {
 template <int S, class TJ, bool NEWTON, bool VISC>
 __global__ __launch_bounds__(256) void k_adj_jacobian(MeshDev m, Fields F, const BcPatchDev* bc, Thermo th, Bounds bd,
                                                       int lowMach, double Mainf, TJ* J, double* src,
-                                                      const int64_t* sellOfSlot) {
+                                                      const int64_t* sellOfSlot, int heat) {
     // value index of entry q of block-CSR slot s
     auto at = [&](int s, int q) -> size_t {
         if constexpr (NEWTON) return (size_t)sellOfSlot[s] + 64 * (size_t)q;
@@ -288,6 +288,8 @@ __global__ __launch_bounds__(256) void k_adj_jacobian(MeshDev m, Fields F, const
                     sngrad_u_bnd_t(bc[m.bpatch[idx]], F, idx, nx, ny, nz, m.ellDc[e], I.ux, I.uy, I.uz, snx, sny, snz);
                     vx = mgb * snx; vy = mgb * sny; vz = mgb * snz;
                     ve = (vx + tbx) * o.upx + (vy + tby) * o.upy + (vz + tbz) * o.upz;
+                    if (heat)  // conduction extension: alphaEff_b |S| dc (h_b - h_P), alphaEff frozen
+                        ve = ve + ((F.balphat[idx] + F.balpha[idx]) * ms * m.ellDc[e]) * (bs.h - I.h);
                 }
                 Jcc[0 * 5 + kk] += o.phi.d * rV;
                 Jcc[1 * 5 + kk] += (o.fx.d - vx.d) * rV;
@@ -303,7 +305,7 @@ __global__ __launch_bounds__(256) void k_adj_jacobian(MeshDev m, Fields F, const
         const int sco = own ? m.slotUpper[fi] : m.slotLower[fi];
         // frozen viscous face data: interp(muEff)|S| deltaCoeffs, the
         // correction part of snGrad(U) and Sf & interp(tauMC)
-        double mgd = 0.0, cgx = 0.0, cgy = 0.0, cgz = 0.0, tfx = 0.0, tfy = 0.0, tfz = 0.0, mg = 0.0;
+        double mgd = 0.0, cgx = 0.0, cgy = 0.0, cgz = 0.0, tfx = 0.0, tfy = 0.0, tfz = 0.0, mg = 0.0, ahd = 0.0;
         if constexpr (VISC) {
             const double w = m.ellWt[e];
             const double muC = F.mut[c] + F.mu[c], muO = F.mut[idx] + F.mu[idx];
@@ -329,6 +331,11 @@ __global__ __launch_bounds__(256) void k_adj_jacobian(MeshDev m, Fields F, const
             tfx = sx * tf[0] + sy * tf[3] + sz * tf[6];
             tfy = sx * tf[1] + sy * tf[4] + sz * tf[7];
             tfz = sx * tf[2] + sy * tf[5] + sz * tf[8];
+            if (heat) {  // conduction extension: interp(alphaEff) |S| dc, frozen
+                const double alC = F.alphat[c] + F.alpha[c], alO = F.alphat[idx] + F.alpha[idx];
+                const double alP = own ? alC : alO, alN = own ? alO : alC;
+                ahd = (w * (alP - alN) + alN) * ms * mgd;
+            }
         }
         // derivative w.r.t. this cell's W, then w.r.t. the other cell's W
         for (int side = 0; side < 2; ++side) {
@@ -347,6 +354,7 @@ __global__ __launch_bounds__(256) void k_adj_jacobian(MeshDev m, Fields F, const
                     const D1 lz = mg * (mgd * (QN.uz - QP.uz) + cgz);
                     const D1 sdf = (lx + tfx) * o.upx + (ly + tfy) * o.upy + (lz + tfz) * o.upz;
                     d[1] -= lx.d; d[2] -= ly.d; d[3] -= lz.d; d[4] -= sdf.d;
+                    if (heat) d[4] -= (ahd * (QN.h - QP.h)).d;
                 }
                 if (side == 0)
                     for (int r = 0; r < 5; ++r) Jcc[r * 5 + kk] += sgn * d[r];
@@ -466,11 +474,12 @@ void launch_jac(Case& c, TJ* J) {
     const int nb = blocks_for(c.mesh->nC, 256);
     if (NEWTON && c.st.viscous)
         hipLaunchKernelGGL((k_adj_jacobian<S, TJ, NEWTON, true>), dim3(nb), dim3(256), 0, c.stream, c.meshDev(), c.f,
-                           c.d_bc.p, c.th, bounds_of(c), c.st.low_mach, c.st.Mainf, J, c.src.p, c.mesh->d_sellOfSlot.p);
+                           c.d_bc.p, c.th, bounds_of(c), c.st.low_mach, c.st.Mainf, J, c.src.p, c.mesh->d_sellOfSlot.p,
+                           c.st.heat_conduction);
     else
         hipLaunchKernelGGL((k_adj_jacobian<S, TJ, NEWTON, false>), dim3(nb), dim3(256), 0, c.stream, c.meshDev(), c.f,
                            c.d_bc.p, c.th, bounds_of(c), c.st.low_mach, c.st.Mainf, J,
-                           NEWTON ? c.src.p : (double*)nullptr, c.mesh->d_sellOfSlot.p);
+                           NEWTON ? c.src.p : (double*)nullptr, c.mesh->d_sellOfSlot.p, 0);
 }
 template <class TJ, bool NEWTON>
 void launch_jac_scheme(Case& c, TJ* J) {

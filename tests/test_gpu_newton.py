@@ -43,24 +43,24 @@ def test_newton_converges_to_the_oracle_root(oracle, precision):
     assert rnorm(oc.get(8)) <= 1e-9 * res[0], rnorm(oc.get(8)) / res[0]
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_viscous_newton_converges_to_the_oracle_root(oracle, precision):
+@pytest.mark.parametrize("precision,heat", [("fp64", 0), ("mixed", 0), ("fp64", 1)])
+def test_viscous_newton_converges_to_the_oracle_root(oracle, precision, heat):
     """Laminar no-slip box (Sutherland air): the Newton operator with the
-    compact linearisation of the viscous terms converges where the Roe
-    operator after the same steps is far behind, to a root of the oracle's
-    viscous residual."""
-    print(f"[newton] viscous box HLLC {precision}", flush=True)
-    kw = dict(NEWTON, rel_tol=1e-4, max_iter=6)
+    compact linearisation of the viscous terms (and of the heat-conduction
+    extension when on) converges where the Roe operator after the same steps
+    is far behind, to a root of the oracle's viscous residual."""
+    print(f"[newton] viscous box HLLC {precision} heat {heat}", flush=True)
+    kw = dict(NEWTON, rel_tol=1e-4, max_iter=6, heat_conduction=heat)
     dc = DeviceCase(cases.box_viscous_case(nx=24, ny=16, precision=precision, perturb=0.02, **kw))
     res = dc.iterate(150)
     print("drop per 10 steps", (res[9::10] / res[0]).tolist(), flush=True)
     assert np.all(np.isfinite(res))
     k = int(np.argmax(res / res[0] <= 1e-10))
     assert res[k] / res[0] <= 1e-10, res[-1] / res[0]
-    droe = DeviceCase(cases.box_viscous_case(nx=24, ny=16, precision=precision, perturb=0.02))
+    droe = DeviceCase(cases.box_viscous_case(nx=24, ny=16, precision=precision, perturb=0.02, heat_conduction=heat))
     rroe = droe.iterate(k + 1)
     assert rroe[-1] / rroe[0] > 1e3 * res[k] / res[0]
-    oc = oracle.OracleCase(cases.box_viscous_case(nx=24, ny=16, perturb=0.02))
+    oc = oracle.OracleCase(cases.box_viscous_case(nx=24, ny=16, perturb=0.02, heat_conduction=heat))
     oc.put(9, dc.get(9))
     oc.update_fields()
     oc.flux_residual()
