@@ -29,9 +29,11 @@ array resident in HBM.  The line reports:
                   then stalls, so 1e-8 is not reached and the line reports the
                   time each path takes to the levels both reach (1e-3, 1e-4,
                   ...), their ratio, the drop reached and Cd of both; with
-                  --newton also the exact-Jacobian Newton-Krylov operator (an
-                  extension: exact first-order inviscid linearisation plus
-                  the compact viscous terms, Roe ILU(0) preconditioner);
+                  --newton also time to 1e-8 of the exact-Jacobian
+                  Newton-Krylov operator (an extension: exact first-order
+                  linearisation, Roe ILU(0) preconditioner, CFL ramp 2 ->
+                  1e4) on the inviscid form of the same wing, mixed and FP64,
+                  which it converges (the SA form stalls, as above);
   cpu_baseline    the CPU FP64 oracle's face flux + residual on the same
                   mesh and state, all host threads (OpenMP) and one.
 
@@ -179,6 +181,58 @@ def converge_path(name, spec, mesh, max_steps, dmesh=None, chunk=20):
     return out
 
 
+def configs_mode(args) -> int:
+    """The forward step (120 x 40, AUSM+, CFL 3, GMRES(20) + ILU(0)) run to a
+    1e-8 drop, and the hybrid airfoil (HLLC + SAneg) per step, on the GPU
+    (plain FP64 path and mixed precision) and on the CPU oracle."""
+    from rhodst_amd import DeviceCase, cases, meshgen
+    from rhodst_amd._capi import DeviceMesh
+    out = {"forward_step": {}, "hybrid_airfoil": {}}
+    for name, extra in (("fp64_plain", {}), ("mixed", dict(precision="mixed", ilu_order="partitioned",
+                                                             gram_schmidt="classical"))):
+        spec = cases.forward_step_case(120, 40, initial_co=3.0, min_co=3.0, max_co=3.0, **extra)
+        dc = DeviceCase(spec)
+        hist, t0, at = [], time.perf_counter(), None
+        while len(hist) < 20000:
+            hist.extend(dc.iterate(100).tolist())
+            h = np.array(hist) / hist[0]
+            if h.min() <= 1e-8 or not np.isfinite(h[-1]):
+                at = int(np.argmax(h <= 1e-8)) + 1 if h.min() <= 1e-8 else None
+                break
+        sec = time.perf_counter() - t0
+        out["forward_step"][name] = {"steps": len(hist), "seconds": round(sec, 3), "ms_per_step": round(sec / len(hist) * 1e3, 4),
+                                     "step_1e-8": at, "final_drop": float(hist[-1] / hist[0])}
+        log(f"forward step {name}: {out['forward_step'][name]}")
+    mesh = meshgen.hybrid_airfoil()
+    dm = DeviceMesh(mesh)
+    wd = dm.wall_distance()
+    for name, extra in (("fp64_plain", {}), ("mixed", dict(precision="mixed", ilu_order="partitioned",
+                                                             gram_schmidt="classical"))):
+        dc = DeviceCase(cases.airfoil_case(mesh=mesh, wall_dist=wd, **extra), dm)
+        dc.iterate(2)
+        t0 = time.perf_counter()
+        r = dc.iterate(20)
+        sec = time.perf_counter() - t0
+        out["hybrid_airfoil"][name] = {"cells": mesh.n_cells, "ms_per_step": round(sec / 20 * 1e3, 3),
+                                       "drop_after_22_steps": float(r[-1] / r[0])}
+        log(f"hybrid airfoil {name}: {out['hybrid_airfoil'][name]}")
+    cpu = {}
+    all_threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    for cfg, budget in ((1, 200.0), (2, 25.0)):
+        for threads in (all_threads, 1):
+            env = dict(os.environ, OMP_NUM_THREADS=str(threads), ORACLE_OPENMP="1")
+            log(f"cpu oracle config {cfg}, {threads} thread(s)")
+            r = subprocess.run([sys.executable, "-m", "oracle.cpu_configs", "--config", str(cfg), "--seconds",
+                                str(budget)], cwd=ROOT, env=env, stdout=subprocess.PIPE, text=True, timeout=900)
+            if r.returncode != 0:
+                log(f"cpu config {cfg} failed (exit {r.returncode})")
+                continue
+            cpu[f"config{cfg}_{threads}t"] = json.loads(r.stdout.strip().splitlines()[-1])
+    out["cpu_oracle"] = cpu
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def cpu_baseline(args, mesh_kw, case_kw, wall_dist, faces):
     """The oracle's face flux + residual on the host: all threads (OpenMP
     build, bit-identical to the serial one) and one thread, each in a child
@@ -214,6 +268,8 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--small", action="store_true", help="a 0.36 M-cell mesh instead of 2.03 M")
+    ap.add_argument("--mesh-order", default="ijk", choices=("ijk", "tiles"),
+                    help="cell numbering of the wing mesh (meshgen.swept_wing order)")
     ap.add_argument("--turbulence", default="SARC2", help="SARC2, SARC1, SAneg, or none (inviscid slip wing)")
     ap.add_argument("--flux-calls", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -222,9 +278,12 @@ def main(argv=None) -> int:
     ap.add_argument("--roe-steps", type=int, default=500, help="steps of the Roe operator at CFL 20 per path")
     ap.add_argument("--newton", action="store_true",
                     help="also run the Newton operator (extension) toward 1e-8 on the workload, mixed and FP64")
-    ap.add_argument("--converge-steps", type=int, default=600, help="step limit of the Newton time-to-1e-8 runs")
+    ap.add_argument("--converge-steps", type=int, default=1200, help="step limit of the Newton time-to-1e-8 runs")
     ap.add_argument("--paths", default="mixed,fp64", help="time-to-1e-8 paths, in order")
     ap.add_argument("--converge-only", action="store_true", help="only the Newton time-to-1e-8 runs (one JSON line)")
+    ap.add_argument("--configs", action="store_true",
+                    help="only the two smaller workloads (forward step to 1e-8, hybrid airfoil per step) on the GPU "
+                         "and the CPU oracle (all threads and one): one JSON line")
     ap.add_argument("--flux-only", action="store_true",
                     help="only the face-flux kernel (n calls) and a 256 MiB device copy of known size: the "
                          "short program the FETCH_SIZE / WRITE_SIZE counter runs profile")
@@ -234,9 +293,12 @@ def main(argv=None) -> int:
               "multi-GPU runs are not part of this build", file=sys.stderr)
         return 2
 
+    if args.configs:
+        return configs_mode(args)
     from rhodst_amd import cases, meshgen
     from rhodst_amd._capi import DeviceMesh
     mesh_kw = dict(n_wake=24, n_surf=64, n_normal=64, n_span_wing=24, n_span_out=8) if args.small else {}
+    mesh_kw["order"] = args.mesh_order
     turb = None if args.turbulence.lower() == "none" else args.turbulence
     t = time.perf_counter()
     mesh = meshgen.swept_wing(**mesh_kw)
@@ -250,7 +312,8 @@ def main(argv=None) -> int:
     spec64 = cases.swept_wing_case(mesh=mesh, **base)
     spec64p = cases.swept_wing_case(mesh=mesh, **fast, **base)
     spec32p = cases.swept_wing_case(mesh=mesh, precision="mixed", **fast, **base)
-    newton = dict(jacobian="exact", initial_co=2.0, max_co=(100.0 if turb else 1e4), rel_tol=1e-2, max_iter=4, **fast)
+    # the Newton extension runs on the inviscid form of the wing, which it converges
+    newton = dict(jacobian="exact", initial_co=2.0, max_co=1e4, rel_tol=1e-2, max_iter=4, **fast)
 
     if args.flux_only:
         from rhodst_amd import DeviceCase
@@ -263,8 +326,8 @@ def main(argv=None) -> int:
         out = {}
         for name in args.paths.split(","):
             prec = "mixed" if name == "mixed" else "fp64"
-            out[name] = converge_path(f"newton {name}", cases.swept_wing_case(mesh=mesh, precision=prec, **newton,
-                                                                              **base), mesh, args.converge_steps, dm)
+            out[name] = converge_path(f"newton {name}", cases.swept_wing_case(mesh=mesh, precision=prec, **newton),
+                                      mesh, args.converge_steps, dm)
         print(json.dumps({"converge": out, "cells": mesh.n_cells}), flush=True)
         return 0
 
@@ -321,7 +384,7 @@ def main(argv=None) -> int:
             prec = "mixed" if name == "mixed" else "fp64"
             log(f"time to 1e-8, {name} path, Newton operator")
             ttc["newton"][name] = converge_path(f"newton {name}",
-                                                cases.swept_wing_case(mesh=mesh, precision=prec, **newton, **base),
+                                                cases.swept_wing_case(mesh=mesh, precision=prec, **newton),
                                                 mesh, args.converge_steps, dm)
         nw = ttc["newton"]
         if not args.newton:
@@ -330,7 +393,7 @@ def main(argv=None) -> int:
             nw["speedup_mixed_vs_fp64"] = round(nw["fp64"]["seconds"] / nw["mixed"]["seconds"], 3)
         if "mixed" in nw and "fp64" in nw:
             nw["abs_dCd_mixed_vs_fp64"] = abs(nw["mixed"]["Cd"] - nw["fp64"]["Cd"])
-        nw["settings"] = ("exact first-order inviscid Jacobian + compact viscous terms (muEff frozen), Roe-ILU(0) "
+        nw["settings"] = ("extension, on the inviscid form of the wing: exact first-order Jacobian, Roe-ILU(0) "
                           f"preconditioner, CFL ramp 2 -> {newton['max_co']:g}, GMRES(30) relTol 1e-2, 4 cycles")
         ttc["roe_cfl20"]["settings"] = "the reference's Roe operator, LTS CFL ramp to 20, GMRES(30) 1 cycle relTol 1e-2"
 

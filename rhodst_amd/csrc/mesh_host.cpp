@@ -562,6 +562,18 @@ void mesh_build(Mesh& m, const rdst_mesh_desc* d, const rdst_mesh_geometry* g) {
     };
     bucket(m.levL, m.nLevL, m.levLPtr, m.levLRows);
     bucket(m.levU, m.nLevU, m.levUPtr, m.levURows);
+    auto runs = [&](const std::vector<int32_t>& ptr, int nLev, std::vector<LevelRun>& out) {
+        out.clear();
+        for (int L = 0; L < nLev;) {
+            const bool small = ptr[L + 1] - ptr[L] <= kIluSegRows;
+            int E = L + 1;
+            while (E < nLev && ((ptr[E + 1] - ptr[E] <= kIluSegRows) == small)) ++E;
+            out.push_back({L, E, small});
+            L = E;
+        }
+    };
+    runs(m.levLPtr, m.nLevL, m.runL);
+    runs(m.levUPtr, m.nLevU, m.runU);
     build_partitions(m, kPartCells);
     // sliced ELL (64 rows per slice)
     m.nSlices = (nCells + 63) / 64;
@@ -643,6 +655,8 @@ void mesh_upload(Mesh& m) {
     m.d_slotLower.upload(m.slotLower);
     m.d_slotDiag.upload(m.slotDiag);
     m.d_levLRows.upload(m.levLRows);
+    m.d_levLPtr.upload(m.levLPtr);
+    m.d_levUPtr.upload(m.levUPtr);
     m.d_levURows.upload(m.levURows);
     m.d_partRowStart.upload(m.partRowStart);
     m.d_partRows.upload(m.partRows);

@@ -109,6 +109,12 @@ struct PiluStream {
 // level L's rows at rows[levPtr[L] .. levPtr[L+1]) for L in
 // [levStart[p], levStart[p+1] - 1), localCol = partition-local column of
 // each block-CSR slot (-1 outside the partition).
+struct LevelRun {
+    int L0, L1;   // levels [L0, L1)
+    bool fused;   // all of them at most kIluSegRows rows: one single-workgroup launch
+};
+constexpr int kIluSegRows = 64;
+
 struct PartSet {
     int nParts = 0, maxRows = 0, maxLevels = 0;
     std::vector<int32_t> rowStart, rows, levStart, levPtr, localCol;
@@ -142,6 +148,10 @@ struct Mesh {
     std::vector<int32_t> levL, levU;           // level of each row
     std::vector<int32_t> levLPtr, levLRows, levUPtr, levURows;
     int nLevL = 0, nLevU = 0;
+    // the levels cut into runs: consecutive levels of at most kIluSegRows rows
+    // form one fused run (one single-workgroup launch), larger levels one
+    // launch each
+    std::vector<LevelRun> runL, runU;
     // compact partitions (recursive coordinate bisection of the cell centres)
     // for the block-Jacobi ILU(0): rows of each partition in (local level,
     // cell) order, level offsets, and per block the partition-local column
@@ -175,7 +185,7 @@ struct Mesh {
     DBuf<double> d_ellSf, d_ellMagSf, d_ellWt, d_ellDc;  // [3][ellW][nC], [ellW][nC]
     DBuf<double> d_ellCorr;                               // [3][ellW][nC]
     DBuf<int32_t> d_rowPtr, d_cols, d_slotUpper, d_slotLower, d_slotDiag;
-    DBuf<int32_t> d_levLRows, d_levURows;
+    DBuf<int32_t> d_levLRows, d_levURows, d_levLPtr, d_levUPtr;
     DBuf<int32_t> d_partRowStart, d_partRows, d_partLevStart, d_partLevPtr, d_localCol;
     DBuf<int64_t> d_sellPtr, d_sellOfSlot;
     DBuf<int32_t> d_sellWidth, d_sellCol;

@@ -407,13 +407,10 @@ __device__ inline bool inv5(const T* A, T* Ainv) {
     return true;
 }
 
-// factor the rows of one level (IKJ, row i reads only rows of earlier levels)
+// factor row i (IKJ, row i reads only rows of earlier levels)
 template <class T>
-__global__ __launch_bounds__(64) void k_ilu_factor(int cnt, const int32_t* rows, const int32_t* rowPtr, const int32_t* cols,
-                                                    const int32_t* diagPos, T* LU, T* Dinv, int* fail) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= cnt) return;
-    const int i = rows[t];
+__device__ inline void ilu_factor_row(int i, const int32_t* rowPtr, const int32_t* cols, const int32_t* diagPos, T* LU,
+                                      T* Dinv, int* fail) {
     const int rb = rowPtr[i], re = rowPtr[i + 1], di = diagPos[i];
     for (int p = rb; p < di; ++p) {
         const int k = cols[p];
@@ -435,13 +432,19 @@ __global__ __launch_bounds__(64) void k_ilu_factor(int cnt, const int32_t* rows,
     if (!inv5(LU + 25 * (size_t)di, Dinv + 25 * (size_t)i)) atomicExch(fail, 1);
 }
 
-// forward sweep of one level: y_i = r_i - sum_{k<i} L_ik y_k
-template <class TM, class TV>
-__global__ __launch_bounds__(64) void k_ilu_fwd(int cnt, const int32_t* rows, const int32_t* rowPtr, const int32_t* cols,
-                                                const int32_t* diagPos, const TM* LU, const TV* r, TV* y) {
+// factor the rows of one level
+template <class T>
+__global__ __launch_bounds__(64) void k_ilu_factor(int cnt, const int32_t* rows, const int32_t* rowPtr, const int32_t* cols,
+                                                    const int32_t* diagPos, T* LU, T* Dinv, int* fail) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= cnt) return;
-    const int i = rows[t];
+    ilu_factor_row(rows[t], rowPtr, cols, diagPos, LU, Dinv, fail);
+}
+
+// forward sweep of row i: y_i = r_i - sum_{k<i} L_ik y_k
+template <class TM, class TV>
+__device__ inline void ilu_fwd_row(int i, const int32_t* rowPtr, const int32_t* cols, const int32_t* diagPos,
+                                   const TM* LU, const TV* r, TV* y) {
     TV s[5];
     for (int k = 0; k < 5; ++k) s[k] = r[5 * (size_t)i + k];
     for (int p = rowPtr[i]; p < diagPos[i]; ++p) {
@@ -453,13 +456,17 @@ __global__ __launch_bounds__(64) void k_ilu_fwd(int cnt, const int32_t* rows, co
     }
     for (int k = 0; k < 5; ++k) y[5 * (size_t)i + k] = s[k];
 }
-// backward sweep of one level: x_i = Dinv_i (y_i - sum_{j>i} U_ij x_j)
 template <class TM, class TV>
-__global__ __launch_bounds__(64) void k_ilu_bwd(int cnt, const int32_t* rows, const int32_t* rowPtr, const int32_t* cols,
-                                                const int32_t* diagPos, const TM* LU, const TM* Dinv, const TV* y, TV* x) {
+__global__ __launch_bounds__(64) void k_ilu_fwd(int cnt, const int32_t* rows, const int32_t* rowPtr, const int32_t* cols,
+                                                const int32_t* diagPos, const TM* LU, const TV* r, TV* y) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= cnt) return;
-    const int i = rows[t];
+    ilu_fwd_row(rows[t], rowPtr, cols, diagPos, LU, r, y);
+}
+// backward sweep of row i: x_i = Dinv_i (y_i - sum_{j>i} U_ij x_j)
+template <class TM, class TV>
+__device__ inline void ilu_bwd_row(int i, const int32_t* rowPtr, const int32_t* cols, const int32_t* diagPos,
+                                   const TM* LU, const TM* Dinv, const TV* y, TV* x) {
     TV s[5];
     for (int k = 0; k < 5; ++k) s[k] = y[5 * (size_t)i + k];
     for (int p = diagPos[i] + 1; p < rowPtr[i + 1]; ++p) {
@@ -473,6 +480,57 @@ __global__ __launch_bounds__(64) void k_ilu_bwd(int cnt, const int32_t* rows, co
     for (int rr = 0; rr < 5; ++rr)
         x[5 * (size_t)i + rr] = (TV)(D[5 * rr] * s[0] + D[5 * rr + 1] * s[1] + D[5 * rr + 2] * s[2] + D[5 * rr + 3] * s[3] +
                                      D[5 * rr + 4] * s[4]);
+}
+template <class TM, class TV>
+__global__ __launch_bounds__(64) void k_ilu_bwd(int cnt, const int32_t* rows, const int32_t* rowPtr, const int32_t* cols,
+                                                const int32_t* diagPos, const TM* LU, const TM* Dinv, const TV* y, TV* x) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    ilu_bwd_row(rows[t], rowPtr, cols, diagPos, LU, Dinv, y, x);
+}
+
+// A run of consecutive small levels [L0, L1) in one launch of one workgroup:
+// the levels one after the other, a barrier between them (the rows a level
+// reads were written by this workgroup before the barrier; the workgroup-
+// scope release/acquire of the barrier makes them visible).  Level launches
+// cost several microseconds each, far more than a small level's work.
+constexpr int ILU_SEG_THREADS = 64;
+constexpr int ILU_SEG_FACTOR_THREADS = 64;
+template <class T>
+__global__ __launch_bounds__(ILU_SEG_FACTOR_THREADS) void k_ilu_factor_seg(int L0, int L1, const int32_t* levPtr,
+                                                                     const int32_t* rows, const int32_t* rowPtr,
+                                                                     const int32_t* cols, const int32_t* diagPos, T* LU,
+                                                                     T* Dinv, int* fail) {
+    for (int L = L0; L < L1; ++L) {
+        const int b = levPtr[L], e = levPtr[L + 1];
+        for (int t = b + (int)threadIdx.x; t < e; t += ILU_SEG_FACTOR_THREADS)
+            ilu_factor_row(rows[t], rowPtr, cols, diagPos, LU, Dinv, fail);
+        __syncthreads();
+    }
+}
+template <class TM, class TV>
+__global__ __launch_bounds__(ILU_SEG_THREADS) void k_ilu_fwd_seg(int L0, int L1, const int32_t* levPtr,
+                                                                  const int32_t* rows, const int32_t* rowPtr,
+                                                                  const int32_t* cols, const int32_t* diagPos,
+                                                                  const TM* LU, const TV* r, TV* y) {
+    for (int L = L0; L < L1; ++L) {
+        const int b = levPtr[L], e = levPtr[L + 1];
+        for (int t = b + (int)threadIdx.x; t < e; t += ILU_SEG_THREADS)
+            ilu_fwd_row(rows[t], rowPtr, cols, diagPos, LU, r, y);
+        __syncthreads();
+    }
+}
+template <class TM, class TV>
+__global__ __launch_bounds__(ILU_SEG_THREADS) void k_ilu_bwd_seg(int L0, int L1, const int32_t* levPtr,
+                                                                  const int32_t* rows, const int32_t* rowPtr,
+                                                                  const int32_t* cols, const int32_t* diagPos,
+                                                                  const TM* LU, const TM* Dinv, const TV* y, TV* x) {
+    for (int L = L0; L < L1; ++L) {
+        const int b = levPtr[L], e = levPtr[L + 1];
+        for (int t = b + (int)threadIdx.x; t < e; t += ILU_SEG_THREADS)
+            ilu_bwd_row(rows[t], rowPtr, cols, diagPos, LU, Dinv, y, x);
+        __syncthreads();
+    }
 }
 // ---------------------------------------------------------------- partitioned block ILU(0)
 // One workgroup per partition walks the partition's local levels; entries
@@ -940,10 +998,19 @@ void ilu_factor(Ctx& k, const T* A, T* LU, T* Dinv) {
                            m.d_partRows.p, m.d_rowPtr.p, m.d_cols.p, m.d_slotDiag.p, m.d_localCol.p, LU, Dinv, fail);
         if (streams_built(m)) pilu_pack<T>(k, LU, Dinv);
     } else if (k.c.st.precond == RDST_PRECOND_ILU0) {
-        for (int L = 0; L < m.nLevL; ++L) {
-            int cnt = m.levLPtr[L + 1] - m.levLPtr[L];
-            hipLaunchKernelGGL((k_ilu_factor<T>), dim3(blocks_for(cnt, 64)), dim3(64), 0, k.s, cnt,
-                               m.d_levLRows.p + m.levLPtr[L], m.d_rowPtr.p, m.d_cols.p, m.d_slotDiag.p, LU, Dinv, fail);
+        for (const LevelRun& g : m.runL) {
+            if (g.fused) {
+                hipLaunchKernelGGL((k_ilu_factor_seg<T>), dim3(1), dim3(ILU_SEG_FACTOR_THREADS), 0, k.s, g.L0, g.L1,
+                                   m.d_levLPtr.p, m.d_levLRows.p, m.d_rowPtr.p, m.d_cols.p, m.d_slotDiag.p, LU, Dinv,
+                                   fail);
+                continue;
+            }
+            for (int L = g.L0; L < g.L1; ++L) {
+                int cnt = m.levLPtr[L + 1] - m.levLPtr[L];
+                hipLaunchKernelGGL((k_ilu_factor<T>), dim3(blocks_for(cnt, 64)), dim3(64), 0, k.s, cnt,
+                                   m.d_levLRows.p + m.levLPtr[L], m.d_rowPtr.p, m.d_cols.p, m.d_slotDiag.p, LU, Dinv,
+                                   fail);
+            }
         }
     } else if (k.c.st.precond == RDST_PRECOND_DIAG) {
         hipLaunchKernelGGL((k_diag_inv<T>), dim3(blocks_for(m.nC, 256)), dim3(256), 0, k.s, m.nC, m.d_slotDiag.p, A, Dinv,
@@ -995,15 +1062,31 @@ void precondition(Ctx& k, const TM* LU, const TM* Dinv, const TV* r, TV* x, TV* 
                                m.d_cols.p, m.d_slotDiag.p, m.d_localCol.p, LU, Dinv, r, x);
         return;
     }
-    for (int L = 0; L < m.nLevL; ++L) {
-        int cnt = m.levLPtr[L + 1] - m.levLPtr[L];
-        hipLaunchKernelGGL((k_ilu_fwd<TM, TV>), dim3(blocks_for(cnt, 64)), dim3(64), 0, k.s, cnt,
-                           m.d_levLRows.p + m.levLPtr[L], m.d_rowPtr.p, m.d_cols.p, m.d_slotDiag.p, LU, r, ytmp);
+    for (const LevelRun& g : m.runL) {
+        if (g.fused) {
+            hipLaunchKernelGGL((k_ilu_fwd_seg<TM, TV>), dim3(1), dim3(ILU_SEG_THREADS), 0, k.s, g.L0, g.L1,
+                               m.d_levLPtr.p, m.d_levLRows.p, m.d_rowPtr.p, m.d_cols.p, m.d_slotDiag.p, LU, r, ytmp);
+            continue;
+        }
+        for (int L = g.L0; L < g.L1; ++L) {
+            int cnt = m.levLPtr[L + 1] - m.levLPtr[L];
+            hipLaunchKernelGGL((k_ilu_fwd<TM, TV>), dim3(blocks_for(cnt, 64)), dim3(64), 0, k.s, cnt,
+                               m.d_levLRows.p + m.levLPtr[L], m.d_rowPtr.p, m.d_cols.p, m.d_slotDiag.p, LU, r, ytmp);
+        }
     }
-    for (int L = 0; L < m.nLevU; ++L) {
-        int cnt = m.levUPtr[L + 1] - m.levUPtr[L];
-        hipLaunchKernelGGL((k_ilu_bwd<TM, TV>), dim3(blocks_for(cnt, 64)), dim3(64), 0, k.s, cnt,
-                           m.d_levURows.p + m.levUPtr[L], m.d_rowPtr.p, m.d_cols.p, m.d_slotDiag.p, LU, Dinv, ytmp, x);
+    for (const LevelRun& g : m.runU) {
+        if (g.fused) {
+            hipLaunchKernelGGL((k_ilu_bwd_seg<TM, TV>), dim3(1), dim3(ILU_SEG_THREADS), 0, k.s, g.L0, g.L1,
+                               m.d_levUPtr.p, m.d_levURows.p, m.d_rowPtr.p, m.d_cols.p, m.d_slotDiag.p, LU, Dinv, ytmp,
+                               x);
+            continue;
+        }
+        for (int L = g.L0; L < g.L1; ++L) {
+            int cnt = m.levUPtr[L + 1] - m.levUPtr[L];
+            hipLaunchKernelGGL((k_ilu_bwd<TM, TV>), dim3(blocks_for(cnt, 64)), dim3(64), 0, k.s, cnt,
+                               m.d_levURows.p + m.levUPtr[L], m.d_rowPtr.p, m.d_cols.p, m.d_slotDiag.p, LU, Dinv, ytmp,
+                               x);
+        }
     }
 }
 

@@ -442,8 +442,8 @@ def swept_wing(n_wake: int = 48, n_surf: int = 128, n_normal: int = 96, n_span_w
                n_span_out: int = 16, thickness: float = 0.10, sweep_deg: float = 30.0, taper: float = 0.56,
                semi_span: float = 1.0, root_chord: float = 1.0, far: float = 12.0, first_cell: float = 2.0e-6,
                n_layers: int = 32, layer_ratio: float = 1.15, normal_blend: float = 0.08, tip_blend: int = 4,
-               tip_first: float | None = None, tip_round: float = 0.08, tip_power: float = 3.0,
-               wake_growth: float = 0.01) -> PolyMesh:
+               tip_first: float | None = None, tip_round: float = 0.08, tip_power: float = 1.0,
+               wake_growth: float = 0.01, order: str = "ijk") -> PolyMesh:
     """Structured C-H hex mesh around a semi-span swept, tapered wing.
 
     Each spanwise station is a C-mesh about the local section (analytic
@@ -482,7 +482,11 @@ def swept_wing(n_wake: int = 48, n_surf: int = 128, n_normal: int = 96, n_span_w
     the few cells at it are the most non-orthogonal of the mesh (86 deg at
     the default size; every cell volume and wall-normal distance positive).
     Cells are numbered i (round the C) fastest, then j (off the wall), then k
-    (span).  Patches: wing (wall), symmetry (symmetryPlane, z = 0),
+    (span) with order="ijk"; order="tiles" numbers them by 4 x 4 x 4 tiles
+    (64 cells, one wavefront), the tiles in Morton order, i fastest inside a
+    tile — a renumbering (as foam's renumberMesh is) that keeps every
+    cell's six neighbours within a few thousand cells of it.
+    Patches: wing (wall), symmetry (symmetryPlane, z = 0),
     farfield (patch).  Default size 352 x 96 x 60 = 2 027 520 cells.
     """
     ni = 2 * n_wake + 2 * n_surf
@@ -634,6 +638,10 @@ def swept_wing(n_wake: int = 48, n_surf: int = 128, n_normal: int = 96, n_span_w
                       pid(ii, jj, kk + 1), pid(ii + 1, jj, kk + 1), pid(ii + 1, jj + 1, kk + 1),
                       pid(ii, jj + 1, kk + 1)], axis=1)
     hexes = remap[hexes]
+    if order == "tiles":
+        hexes = hexes[tile_order(ii, jj, kk)]
+    elif order != "ijk":
+        raise ValueError(f"swept_wing: order {order!r} (ijk or tiles)")
     # compact the point list
     used = np.unique(hexes)
     newid = np.full(points.shape[0], -1, dtype=np.int64)
@@ -650,6 +658,25 @@ def swept_wing(n_wake: int = 48, n_surf: int = 128, n_normal: int = 96, n_span_w
     mesh = hex_polymesh(points, hexes, classify,
                         [("wing", PATCH_WALL), ("symmetry", PATCH_SYMMETRY), ("farfield", PATCH_GENERIC)])
     return mesh
+
+
+def _morton3(a: np.ndarray, b: np.ndarray, c: np.ndarray) -> np.ndarray:
+    """Interleave the bits of three non-negative int arrays (a lowest)."""
+    out = np.zeros(a.shape, dtype=np.int64)
+    for bit in range(21):
+        out |= ((a >> bit) & 1) << (3 * bit)
+        out |= ((b >> bit) & 1) << (3 * bit + 1)
+        out |= ((c >> bit) & 1) << (3 * bit + 2)
+    return out
+
+
+def tile_order(ii: np.ndarray, jj: np.ndarray, kk: np.ndarray, t: int = 4) -> np.ndarray:
+    """Permutation of structured cells (i, j, k) into t x t x t tiles, the
+    tiles in Morton order and the cells i fastest, then j, then k inside a
+    tile: new cell n is old cell perm[n]."""
+    key_t = _morton3(ii // t, jj // t, kk // t)
+    key_c = (ii % t) + t * ((jj % t) + t * (kk % t))
+    return np.lexsort((key_c, key_t))
 
 
 def _face_triangles(mesh: PolyMesh, faces: np.ndarray):

@@ -71,3 +71,26 @@ def test_swept_wing_wake_spacing_grows(wing):
     f = 2e-6 + 0.01 * s
     assert (s > 0.005).all()
     assert np.all(h > 0.7 * f) and np.all(h < 1.6 * f)
+
+
+def test_tile_order_is_a_renumbering(oracle):
+    """order="tiles" renumbers the cells (4 x 4 x 4 tiles in Morton order)
+    without changing the mesh: new cell n is old cell perm[n] (same centre and
+    volume), the same numbers of cells and faces, and the two cells of nine
+    faces in ten within 1000 of each other."""
+    kw = dict(n_wake=6, n_surf=16, n_normal=16, n_span_wing=8, n_span_out=4, n_layers=8)
+    a, b = meshgen.swept_wing(**kw), meshgen.swept_wing(order="tiles", **kw)
+    ga, gb = oracle.geometry(a), oracle.geometry(b)
+    assert a.n_cells == b.n_cells and a.n_internal == b.n_internal and len(a.owner) == len(b.owner)
+    assert not np.array_equal(ga["C"], gb["C"])
+    # the permutation as tile_order gives it
+    ni, nj, nk = 44, 16, 12
+    ii, jj, kk = np.meshgrid(np.arange(ni), np.arange(nj), np.arange(nk), indexing="ij")
+    ii, jj, kk = (x.transpose(2, 1, 0).ravel() for x in (ii, jj, kk))
+    perm = meshgen.tile_order(ii, jj, kk)
+    assert np.array_equal(np.sort(perm), np.arange(a.n_cells))
+    # the same cells (geometry to round-off: the faces are summed in another order)
+    assert np.allclose(gb["C"], ga["C"][perm], rtol=0, atol=1e-12)
+    assert np.allclose(gb["V"], ga["V"][perm], rtol=1e-10, atol=0)
+    d = np.abs(np.asarray(b.neighbour) - np.asarray(b.owner[:b.n_internal]))
+    assert np.percentile(d, 90) < 1000
