@@ -190,6 +190,11 @@ typedef struct rdst_settings {
      * alphaEff_f |Sf| dc (h_N - h_P) (boundary faces: h_b - h_P), alphaEff =
      * alpha + alphat; 0 (default) is the reference's energy equation. */
     int32_t heat_conduction;
+    /* Update limit (no counterpart in the reference; default 0 = off): when
+     * > 0, each cell's update of W in updateFields is scaled so that rho and
+     * rhoE change by at most this fraction of their values before the update
+     * (a per-cell damping for the exact-Jacobian operator at large CFL). */
+    double update_limit;
 } rdst_settings;
 
 /* Non-orthogonal correction of a surface-normal gradient (foam-extend 4.1

@@ -1762,6 +1762,20 @@ static void updateFields(Case& c) {
     // old values the BCs read before thermo.correct()
     std::vector<double> pOld = c.p, TOld = c.T, psiOld = c.psi;
     std::vector<double> bpsiOld = c.bpsi, brhoOld = c.brho;
+    // update limit (an extension, off in the reference): scale each cell's
+    // change of W so that rho and rhoE move by at most update_limit
+    if (c.st.update_limit > 0.0) {
+        const double lim = c.st.update_limit;
+        for (int i = 0; i < n; ++i) {
+            const double o[5] = {c.rho[i], c.rhoU[i].x, c.rhoU[i].y, c.rhoU[i].z, c.rhoE[i]};
+            const double dr = std::fabs(c.W[5 * i] - o[0]), de = std::fabs(c.W[5 * i + 4] - o[4]);
+            double a = 1.0;
+            if (dr > lim * o[0]) a = std::fmin(a, lim * o[0] / dr);
+            if (de > lim * o[4]) a = std::fmin(a, lim * o[4] / de);
+            if (a < 1.0)
+                for (int q = 0; q < 5; ++q) c.W[5 * i + q] = o[q] + a * (c.W[5 * i + q] - o[q]);
+        }
+    }
     // DST::retrieve (updateFields.H:40-42), boundMinMax(rho) as a clamp
     for (int i = 0; i < n; ++i) {
         c.rho[i] = clampv(c.W[5 * i], c.st.rho_min, c.st.rho_max);

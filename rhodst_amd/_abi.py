@@ -59,7 +59,8 @@ class rdst_settings(C.Structure):
                 ("time_integration", C.c_int32), ("adjust_time_step", C.c_int32), ("max_inner_iter", C.c_int32),
                 ("iteration_start", C.c_int32), ("mag_converged_slope", C.c_double), ("min_residual", C.c_double),
                 ("delta_t", C.c_double), ("laplacian_sngrad", C.c_int32), ("sngrad", C.c_int32),
-                ("laplacian_limit", C.c_double), ("sngrad_limit", C.c_double), ("heat_conduction", C.c_int32)]
+                ("laplacian_limit", C.c_double), ("sngrad_limit", C.c_double), ("heat_conduction", C.c_int32),
+                ("update_limit", C.c_double)]
 
 
 class rdst_solver_perf(C.Structure):
@@ -256,6 +257,7 @@ def make_settings(**kw) -> rdst_settings:
     s.laplacian_sngrad, s.laplacian_limit = _sngrad(kw.get("laplacian", "corrected"))
     s.sngrad, s.sngrad_limit = _sngrad(kw.get("sngrad", "corrected"))
     s.heat_conduction = int(kw.get("heat_conduction", 0))
+    s.update_limit = float(kw.get("update_limit", 0.0))
     return s
 
 

@@ -281,6 +281,7 @@ int rdst_case_create(rdst_mesh* mesh, const rdst_thermo* thermo, const rdst_bc* 
         }
         if (s.heat_conduction != 0 && s.heat_conduction != 1)
             throw ArgError("rdst_case_create: heat_conduction is 0 or 1");
+        if (!(s.update_limit >= 0.0)) throw ArgError("rdst_case_create: update_limit must be >= 0");
         if (!(thermo->Cp > thermo->R) || !(thermo->R > 0)) throw ArgError("rdst_case_create: need Cp > R > 0");
         if (!m->uploaded) {
             mesh_upload(*m);

@@ -231,6 +231,21 @@ __global__ void k_update_bnd_pre(MeshDev m, Fields F, Thermo th, const BcPatchDe
 }
 
 // kernel 2: cells -- retrieve from W, bound rho, U, h (bounded), thermo, p (bounded)
+// update limit (an extension, before the rest of the update reads W):
+// scale each cell's change of W so that rho and rhoE move by at most lim of
+// their values before the update
+__global__ void k_limit_update(int n, Fields F, double lim) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double o[5] = {F.rho[i], F.rhoUx[i], F.rhoUy[i], F.rhoUz[i], F.rhoE[i]};
+    const double dr = fabs(F.W[5 * i] - o[0]), de = fabs(F.W[5 * i + 4] - o[4]);
+    double a = 1.0;
+    if (dr > lim * o[0]) a = fmin(a, lim * o[0] / dr);
+    if (de > lim * o[4]) a = fmin(a, lim * o[4] / de);
+    if (a < 1.0)
+        for (int q = 0; q < 5; ++q) F.W[5 * i + q] = o[q] + a * (F.W[5 * i + q] - o[q]);
+}
+
 __global__ void k_update_cells(int n, Fields F, Thermo th, double rhoMin, double rhoMax, double hMin, double hMax,
                                double pMin, double pMax) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -526,6 +541,8 @@ void update_fields(Case& c) {
         RDST_HIP(hipMemcpyAsync(c.sa.rhoPrev, c.f.rho, sizeof(double) * m.nC, hipMemcpyDeviceToDevice, c.stream));
         c.sa.rhoPrevValid = true;
     }
+    if (s.update_limit > 0.0)
+        hipLaunchKernelGGL(k_limit_update, dim3(blocks_for(m.nC, bs)), dim3(bs), 0, c.stream, m.nC, c.f, s.update_limit);
     if (m.nB)
         hipLaunchKernelGGL(k_update_bnd_pre, dim3(blocks_for(m.nB, bs)), dim3(bs), 0, c.stream, md, c.f, c.th, c.d_bc.p,
                            s.rho_min, s.rho_max);

@@ -113,3 +113,17 @@ def test_newton_operator_read_back_is_v_j_plus_diagonal(oracle):
     rdt = dc.get(10)
     ref = V[:, None] * Jv + (rdt * V)[:, None] * v
     assert np.abs(Av - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+def test_update_limit_parity(oracle):
+    """The update-limit extension on the device against the oracle: ten
+    steps of the perturbed forward step at CFL 5 with every cell's change of
+    rho and rhoE limited to 2 %."""
+    print("[newton] update limit parity, forward step", flush=True)
+    kw = dict(perturb=0.02, initial_co=5.0, min_co=5.0, max_co=5.0, update_limit=0.02)
+    dc = DeviceCase(cases.forward_step_case(30, 10, **kw))
+    oc = oracle.OracleCase(cases.forward_step_case(30, 10, **kw))
+    r1, r0 = dc.iterate(10), oc.iterate(10)
+    assert np.abs(r1 - r0).max() <= 1e-9 * r0.max()
+    W1, W0 = dc.get(9), oc.get(9)
+    assert np.abs(W1 - W0).max() <= 1e-9 * np.abs(W0).max()

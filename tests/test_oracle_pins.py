@@ -293,3 +293,31 @@ def test_update_fields_round_trip_and_bounds(oracle):
     assert p_b[1] == pytest.approx(1.0 * R_N * 3.0, rel=1e-14)
     assert p_b[2] == 4.0                                    # rho R T = 4 R_N 2 = 5.71 -> pMax
     assert oc.get(F_W) == pytest.approx(Wb, rel=1e-15)      # W is not re-synced
+
+
+def test_update_limit_scales_large_changes_only(oracle):
+    """The update-limit extension: after one step from the forward step's
+    uniform start, every cell's change of rho and rhoE is within the limit of
+    its old value, the cells whose unlimited change stayed inside it are
+    unchanged, and the others moved along the unlimited direction with the
+    binding component at the limit."""
+    from rhodst_amd import cases
+    lim = 0.02
+    a = oracle.OracleCase(cases.forward_step_case(30, 10, perturb=0.02, initial_co=5.0, min_co=5.0, max_co=5.0))
+    b = oracle.OracleCase(cases.forward_step_case(30, 10, perturb=0.02, initial_co=5.0, min_co=5.0, max_co=5.0,
+                                                 update_limit=lim))
+    W0 = a.get(9).copy()
+    a.iterate(1)
+    b.iterate(1)
+    Wa, Wb = a.get(9), b.get(9)
+    da, db = Wa - W0, Wb - W0
+    ra = np.maximum(np.abs(da[:, 0]) / W0[:, 0], np.abs(da[:, 4]) / W0[:, 4])
+    rb = np.maximum(np.abs(db[:, 0]) / W0[:, 0], np.abs(db[:, 4]) / W0[:, 4])
+    assert (ra > lim).any() and (ra <= lim).any()
+    assert (rb <= lim * (1 + 1e-12)).all()
+    inside = ra <= lim
+    assert np.array_equal(Wb[inside], Wa[inside])
+    out = ~inside
+    assert np.allclose(rb[out], lim, rtol=1e-12)
+    scale = (rb[out] / ra[out])[:, None]
+    assert np.allclose(db[out], scale * da[out], rtol=1e-9, atol=1e-12 * np.abs(W0).max())
