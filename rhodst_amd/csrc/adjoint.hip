@@ -546,6 +546,7 @@ void newton_operator(Case& c) {
         launch_jac_scheme<double, true>(c, c.S64.p);
     }
     c.sellFresh = true;
+    c.csrStale = true;
 }
 
 namespace {
@@ -560,12 +561,13 @@ __global__ void k_from_sell(long nnzb, const int64_t* sellOfSlot, const T* S, T*
 
 // the FP64 Newton operator in block-CSR (A64), for a read-back
 void newton_operator_csr(Case& c) {
-    if (!c.sellFresh || c.st.precision == RDST_PREC_MIXED) return;
+    if (!c.csrStale || c.st.precision == RDST_PREC_MIXED) return;
     const Mesh& m = *c.mesh;
     hipLaunchKernelGGL((k_from_sell<double>), dim3(std::min(4096, blocks_for(m.nnzb, 256))), dim3(256), 0, c.stream,
                        (long)m.nnzb, m.d_sellOfSlot.p, c.S64.p, c.A64.p);
     RDST_HIP(hipGetLastError());
     RDST_HIP(hipStreamSynchronize(c.stream));
+    c.csrStale = false;
 }
 
 void adjoint_linearise(Case& c) {

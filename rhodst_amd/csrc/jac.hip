@@ -14,7 +14,10 @@
 // (row neighbour, column owner), as neighbour the upper coefficient
 // beta ((1-w) A n |S| - Lambda I) (row owner, column neighbour).  The
 // diagonal is the negated sum of the same blocks in face order
-// (negSumDiag), plus the boundary chain rule, plus rDeltaTDST V I.
+// (negSumDiag), plus the boundary chain rule, plus rDeltaTDST V I.  With
+// sellOf given, every block is also stored into the sliced-ELL copy the
+// Krylov SpMV reads (entry q of slot s at sellOf[s] + 64 q), so the solve
+// needs no separate block-CSR -> sliced-ELL pass.
 #include "rdst_internal.hpp"
 
 namespace rdst {
@@ -32,7 +35,8 @@ struct DualDdt {
 
 template <bool F32>
 __global__ __launch_bounds__(128) void k_assemble(MeshDev m, Fields F, Thermo th, const BcPatchDev* bc, double beta,
-                                                  double* A64, float* A32, double* src, DualDdt dd2) {
+                                                  double* A64, float* A32, double* src, DualDdt dd2,
+                                                  const int64_t* sellOf, double* S64, float* S32) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
     const double gam = th.Cp / th.Cv;
@@ -162,12 +166,17 @@ __global__ __launch_bounds__(128) void k_assemble(MeshDev m, Fields F, Thermo th
         const int slot = m.ellSlot[e];
         double* blk64 = A64 + 25 * (size_t)slot;
         float* blk32 = F32 ? A32 + 25 * (size_t)slot : nullptr;
+        const size_t sb = sellOf ? (size_t)sellOf[slot] : 0;
         for (int q = 0; q < 25; ++q) {
             const double lq = (q % 6 == 0) ? lam : 0.0;
             const double jw = J[q] * wt;
             const double v = own ? -beta * (jw * ms + lq) : beta * (jw * ms - lq);
             if (F32) blk32[q] = (float)v;
             else blk64[q] = v;
+            if (sellOf) {
+                if (F32) S32[sb + 64 * q] = (float)v;
+                else S64[sb + 64 * q] = v;
+            }
             fd[q] -= v;
         }
     }
@@ -176,11 +185,16 @@ __global__ __launch_bounds__(128) void k_assemble(MeshDev m, Fields F, Thermo th
     const double V = m.V[c];
     const double diagS = dd2.on ? (F.rDeltaT[c] + dd2.a) * V : F.rDeltaT[c] * V;
     const int ds = m.slotDiag[c];
+    const size_t sbd = sellOf ? (size_t)sellOf[ds] : 0;
     for (int q = 0; q < 25; ++q) {
         const double dd = (q % 6 == 0) ? diagS : 0.0;
         const double v = dd + fd[q];
         if (F32) A32[25 * (size_t)ds + q] = (float)v;
         else A64[25 * (size_t)ds + q] = v;
+        if (sellOf) {
+            if (F32) S32[sbd + 64 * q] = (float)v;
+            else S64[sbd + 64 * q] = v;
+        }
     }
     // source: (rDeltaT psi_old V - diag psi) + (-R V), psi_old = psi; dual2ndDST:
     // V (rDeltaT psi - b psi.oldTime() - c psi.oldTime().oldTime()) - diag psi
@@ -218,27 +232,40 @@ void jacobian_assemble(Case& c) {
         if (mixed) {
             if (c.P32.n != nn) c.P32.alloc(nn);
             hipLaunchKernelGGL((k_assemble<true>), dim3(blocks_for(m.nC, 128)), dim3(128), 0, c.stream, md, c.f, c.th,
-                               c.d_bc.p, c.st.beta, (double*)nullptr, c.P32.p, c.src.p, dd2);
+                               c.d_bc.p, c.st.beta, (double*)nullptr, c.P32.p, c.src.p, dd2, (const int64_t*)nullptr,
+                               (double*)nullptr, (float*)nullptr);
         } else {
             if (c.P64.n != nn) c.P64.alloc(nn);
             hipLaunchKernelGGL((k_assemble<false>), dim3(blocks_for(m.nC, 128)), dim3(128), 0, c.stream, md, c.f, c.th,
-                               c.d_bc.p, c.st.beta, c.P64.p, (float*)nullptr, c.src.p, dd2);
+                               c.d_bc.p, c.st.beta, c.P64.p, (float*)nullptr, c.src.p, dd2, (const int64_t*)nullptr,
+                               (double*)nullptr, (float*)nullptr);
         }
         newton_operator(c);
         RDST_HIP(hipGetLastError());
         c.hasMatrix = true;
         return;
     }
-    c.sellFresh = false;  // the sliced-ELL copy is refreshed from block-CSR in the solve
+    // block-CSR for the ILU(0) factor and the read-back, sliced-ELL for the SpMV
+    const size_t nn = 25 * (size_t)m.nnzb, nv = 25 * (size_t)m.sellBlocks;
     if (mixed) {
-        if (c.A32.n != 25 * (size_t)m.nnzb) c.A32.alloc(25 * (size_t)m.nnzb);
+        if (c.A32.n != nn) c.A32.alloc(nn);
+        if (c.S32.n != nv) {
+            c.S32.alloc(nv);
+            c.S32.zero(c.stream);  // padding blocks stay zero
+        }
         hipLaunchKernelGGL((k_assemble<true>), dim3(blocks_for(m.nC, 128)), dim3(128), 0, c.stream, md, c.f, c.th, c.d_bc.p,
-                           c.st.beta, (double*)nullptr, c.A32.p, c.src.p, dd2);
+                           c.st.beta, (double*)nullptr, c.A32.p, c.src.p, dd2, m.d_sellOfSlot.p, (double*)nullptr, c.S32.p);
     } else {
-        if (c.A64.n != 25 * (size_t)m.nnzb) c.A64.alloc(25 * (size_t)m.nnzb);
+        if (c.A64.n != nn) c.A64.alloc(nn);
+        if (c.S64.n != nv) {
+            c.S64.alloc(nv);
+            c.S64.zero(c.stream);
+        }
         hipLaunchKernelGGL((k_assemble<false>), dim3(blocks_for(m.nC, 128)), dim3(128), 0, c.stream, md, c.f, c.th, c.d_bc.p,
-                           c.st.beta, c.A64.p, (float*)nullptr, c.src.p, dd2);
+                           c.st.beta, c.A64.p, (float*)nullptr, c.src.p, dd2, m.d_sellOfSlot.p, c.S64.p, (float*)nullptr);
     }
+    c.sellFresh = true;
+    c.csrStale = false;
     RDST_HIP(hipGetLastError());
     c.hasMatrix = true;
 }

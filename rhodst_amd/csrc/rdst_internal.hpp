@@ -312,7 +312,8 @@ struct Case {
     DBuf<float> P32;
     DBuf<double> S64;               // sliced-ELL copies for the SpMV
     DBuf<float> S32;
-    bool sellFresh = false;  // S64/S32 were written by the Newton operator itself (no block-CSR copy)
+    bool sellFresh = false;  // S64/S32 hold the assembled operator (written by the assembly itself)
+    bool csrStale = false;   // A64 does not hold the operator (the Newton operator is written sliced-ELL only)
     bool hasState = false, hasMatrix = false;
     // solver scratch owned by the solver module
     struct SolverWork* work = nullptr;

@@ -1364,6 +1364,7 @@ void ldu_solve(Case& c, const double* diag, const double* upper, const double* l
     RDST_HIP(hipMemcpyAsync(c.lduX.p, x, sizeof(double) * n5, hipMemcpyHostToDevice, c.stream));
     RDST_HIP(hipMemcpyAsync(c.lduB.p, b, sizeof(double) * n5, hipMemcpyHostToDevice, c.stream));
     c.sellFresh = false;
+    c.csrStale = false;
     c.hasMatrix = true;
     if (mixed) {
         if (c.A32.n != nn) c.A32.alloc(nn);
