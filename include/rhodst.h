@@ -405,7 +405,11 @@ int rdst_case_destroy(rdst_case* c);
  * values, phi = interpolate(rho*U)&Sf, W, courant and rDeltaTDST */
 int rdst_case_set_state(rdst_case* c, const double* p, const double* T, const double* U);
 int rdst_case_get(rdst_case* c, int field, double* out);
-/* overwrite a cell field (RDST_F_RHO, _RHOU, _RHOE, _W, _RDELTAT) */
+/* overwrite a field: cells RDST_F_RHO, _RHOU, _RHOE, _W, _RDELTAT, _NU_TILDA,
+ * _MUT, _ALPHAT; faces RDST_F_PHI (the stored mass flux the boundary
+ * conditions read); boundary faces RDST_F_BMUT (alphat_b = mut_b / Prt with a
+ * turbulence model on) -- the last four restore a state or carry one, with
+ * the turbulence frozen, to another mesh */
 int rdst_case_put(rdst_case* c, int field, const double* in);
 
 /* computeFluxDST + calcFields: fills phi, phiUp, phiEp, Up and R_ */
@@ -490,12 +494,19 @@ int rdst_turbulence_correct(rdst_case* c, rdst_solver_perf* perf);
 
 /* ----------------------------------------------------------- discrete adjoint
  * New (no counterpart in the reference): exact linearisation of the
- * first-order inviscid residual R(W) = (1/V) sum_f s_f F_f (baseFluxDST.C:
- * 223-255 with the W -> primitive chain of updateFields.H:40-94 and the
- * boundary conditions at the same state), its transpose, and the adjoint
- * solve for force-coefficient sensitivities dJ/dalpha = dJ/dalpha|_W -
- * lambda^T dR/dalpha|_W with J^T lambda = dJ/dW.  Flux schemes AUSM+,
- * AUSM+up, HLL, HLLC, Kurganov (HLLCP's sensor couples second neighbours). */
+ * first-order residual R(W) = (1/V) sum_f s_f F_f (baseFluxDST.C:223-255 with
+ * the W -> primitive chain of updateFields.H:40-94 and the boundary
+ * conditions at the same state), its transpose, and the adjoint solve for
+ * force-coefficient sensitivities dJ/dalpha = dJ/dalpha|_W - lambda^T
+ * dR/dalpha|_W with J^T lambda = dJ/dW.  Flux schemes AUSM+, AUSM+up, HLL,
+ * HLLC, Kurganov (HLLCP's sensor couples second neighbours).  With
+ * settings.viscous the viscous terms of calcFields are included exactly
+ * (Gauss gradient of U and its corrected boundary value, the laplacian with
+ * its non-orthogonal correction, Sf & tauMC, the work on the interface
+ * velocity, laminar mu(T) and alpha(T)); J then couples second neighbours and
+ * is formed on that pattern, its face-neighbour part feeding the ILU(0).  A
+ * turbulence model's mut and alphat are frozen at their stored values, and
+ * the objective's force includes the viscous part. */
 typedef struct rdst_adjoint_settings {
     int32_t precision;      /* RDST_PREC_FP64, or RDST_PREC_MIXED: FP32 transposed ILU and Krylov cycles
                                inside an FP64 defect correction on the exact FP64 J^T */
@@ -512,7 +523,8 @@ int rdst_adjoint_linearise(rdst_case* c);
 int rdst_adjoint_apply(rdst_case* c, int transpose, const double* x, double* y);
 /* R(W) [n_cells][5] by the residual chain that rdst_adjoint_linearise differentiates */
 int rdst_adjoint_residual(rdst_case* c, double* R);
-/* dJ/dW [n_cells][5] of J = sum_{faces of patch} (p_b - p_ref) (Sf . dir) / q */
+/* dJ/dW [n_cells][5] of J = sum_{faces of patch} (p_b - p_ref) (Sf . dir) / q
+ * (+ Sf & devRhoReff_b . dir / q with settings.viscous, as rdst_forces) */
 int rdst_adjoint_objective_gradient(rdst_case* c, int patch, const double* dir, double q, double* g);
 /* linearises at the current state, then solves J^T lambda = dJ/dW; lambda
  * [n_cells][5] holds the initial guess (zeros for none) and receives the solution */

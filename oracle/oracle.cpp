@@ -2996,6 +2996,9 @@ int orc_case_put(void* h, int field, const double* in) {
             case RDST_F_RHOU: for (int i = 0; i < n; ++i) c.rhoU[i] = v3(in[3 * i], in[3 * i + 1], in[3 * i + 2]); break;
             case RDST_F_W: c.W.assign(in, in + 5 * (size_t)n); break;
             case RDST_F_RDELTAT: c.rDeltaT.assign(in, in + n); break;
+            // the stored mass flux, which the boundary conditions' masks read
+            // (a state restored before a differenced residual)
+            case RDST_F_PHI: c.phi.assign(in, in + c.phi.size()); break;
             // scripted inputs of setCourantDST (timeDST.C:316-414): the
             // residual R_, the time-control state and the previous rho, e
             case RDST_F_R: c.R.assign(in, in + 5 * (size_t)n); break;

@@ -1,7 +1,8 @@
 """Discrete-adjoint force-coefficient sensitivities behind torch.autograd.
 
 The device computes the exact linearisation J = dR/dW of the first-order
-inviscid residual (rdst_adjoint_linearise) and solves J^T lambda = dC/dW
+residual, inviscid or with the viscous terms (rdst_adjoint_linearise; with a
+turbulence model its eddy viscosity frozen) and solves J^T lambda = dC/dW
 (rdst_adjoint_solve, FP64 or FP32 transposed ILU inside an FP64 defect
 correction).  With the state converged (R(W*, x) = 0) the total derivative of
 a force coefficient C with respect to shape parameters a is
@@ -73,7 +74,7 @@ class BumpProblem:
     def coefficient(self, dc, pressure_only=True):
         patch = dc.spec.mesh.patch_index("bottom")
         out = dc.forces(patch, self.direction, (0.0, 1.0, 0.0), self.p_ref, 1.0, 1.0, 1.0)
-        F = out[2:5]  # the force vector (pressure part: the case is inviscid)
+        F = out[2:5]  # the force vector (pressure and, viscous, the viscous part)
         return float(F @ self.direction / self.q)
 
     def solve(self, amplitudes):
@@ -92,6 +93,11 @@ class BumpProblem:
         dc.update_fields()
         dc.flux_residual()
         dc.update_fields()
+        if self.kw.get("viscous"):
+            # the boundary gradient that the viscous force reads, formed from
+            # boundary values consistent with W
+            dc.flux_residual()
+            dc.update_fields()
         return dc
 
     def partials(self, W, amplitudes, direction_vec, node_disp=None):

@@ -284,12 +284,15 @@ def box_sa_case(model: str = "SAneg", scheme: str = "HLLC", nx: int = 24, ny: in
 
 
 def bump_case(amplitudes=(0.02, 0.01, -0.005, 0.004), nx: int = 48, ny: int = 24, mach: float = 0.5,
-              scheme: str = "HLLC", node_disp=None, **settings) -> CaseSpec:
-    """Inviscid subsonic flow through a channel over a Hicks-Henne bump on the
-    bottom wall (meshgen.bump_channel): characteristic far-field inlet and
-    outlet (subsonic inflow/outflow everywhere), slip bottom wall, symmetry
-    top.  Air at 300 K / 1e5 Pa.  The workload of the discrete-adjoint
-    checks (force coefficients of the bottom wall against wall shape)."""
+              scheme: str = "HLLC", node_disp=None, viscous: bool = False, reynolds: float = 500.0,
+              **settings) -> CaseSpec:
+    """Subsonic flow through a channel over a Hicks-Henne bump on the bottom
+    wall (meshgen.bump_channel): characteristic far-field inlet and outlet
+    (subsonic inflow/outflow everywhere), symmetry top.  Air at 300 K / 1e5 Pa.
+    Inviscid: slip bottom wall.  viscous: laminar, a no-slip adiabatic bottom
+    wall and a constant mu giving Re = reynolds on the channel length.  The
+    workload of the discrete-adjoint checks (force coefficients of the bottom
+    wall against wall shape)."""
     mesh = meshgen.bump_channel(nx, ny, amplitudes=amplitudes, node_disp=node_disp)
     gamma, R = 1.4, 287.0
     Cp = gamma * R / (gamma - 1)
@@ -297,13 +300,19 @@ def bump_case(amplitudes=(0.02, 0.01, -0.005, 0.004), nx: int = 48, ny: int = 24
     U = (mach * np.sqrt(gamma * R * Tinf), 0.0, 0.0)
     ff = {"p": ("farField", pinf), "T": ("farField", Tinf), "U": ("farField", U),
           "inf": {"p": pinf, "T": Tinf, "U": U}}
-    spec = {"inlet": ff, "outlet": ff, "bottom": {"p": "zeroGradient", "T": "zeroGradient", "U": "slip"},
+    bottom = {"p": "zeroGradient", "T": "zeroGradient",
+              "U": ("fixedValue", (0.0, 0.0, 0.0)) if viscous else "slip"}
+    spec = {"inlet": ff, "outlet": ff, "bottom": bottom,
             "top": {"p": "symmetryPlane", "T": "symmetryPlane", "U": "symmetryPlane"}}
+    mu = (pinf / (R * Tinf)) * U[0] * 2.0 / reynolds if viscous else 0.0
     n = mesh.n_cells
     kw = dict(flux_scheme=scheme, neighbouring=1, initial_co=5.0, min_co=1.0, max_co=1.0e5,
               solver="GMRES", precond="ILUC0", n_directions=30, min_iter=1, max_iter=2,
               tolerance=1e-14, rel_tol=1e-4, p_bounds=(100.0, 1e7), T_bounds=(50.0, 2000.0),
               rho_bounds=(1e-3, 100.0), ilu_order="partitioned", gram_schmidt="classical")
+    if viscous:
+        kw["viscous"] = 1
     kw.update(settings)
-    return CaseSpec("bump", mesh, make_thermo(Cp=Cp, R=R, mu=0.0, Pr=0.72), make_bcs(mesh, spec),
-                    make_settings(**kw), np.full(n, pinf), np.full(n, Tinf), np.tile(np.array(U), (n, 1)))
+    return CaseSpec("bumpViscous" if viscous else "bump", mesh, make_thermo(Cp=Cp, R=R, mu=mu, Pr=0.72),
+                    make_bcs(mesh, spec), make_settings(**kw), np.full(n, pinf), np.full(n, Tinf),
+                    np.tile(np.array(U), (n, 1)))

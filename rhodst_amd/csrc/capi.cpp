@@ -437,6 +437,21 @@ int rdst_case_put(rdst_case* cc, int field, const double* in) {
             case RDST_F_W: one(f.W, 5 * n); break;
             case RDST_F_RDELTAT: one(f.rDeltaT, n); break;
             case RDST_F_NU_TILDA: sa_put_nut(*c, in); break;
+            // the stored mass flux (read by the boundary conditions' masks) and
+            // the eddy viscosity and conductivity, for a state restored or
+            // carried to another mesh with the turbulence frozen
+            case RDST_F_PHI: one(f.phi, (size_t)c->mesh->nF); break;
+            case RDST_F_MUT: one(f.mut, n); break;
+            case RDST_F_ALPHAT: one(f.alphat, n); break;
+            case RDST_F_BMUT: {
+                const size_t nB = c->mesh->nB;
+                one(f.bmut, nB);
+                const double prt = c->sa.on ? c->sa.t.Prt : 1.0;  // alphat_b = mut_b / Prt as the model sets it
+                std::vector<double> t(in, in + nB);
+                for (double& v : t) v /= prt;
+                if (c->sa.on) RDST_HIP(hipMemcpy(f.balphat, t.data(), nB * sizeof(double), hipMemcpyHostToDevice));
+                break;
+            }
             case RDST_F_RHOU: {
                 std::vector<double> t(n);
                 double* d[3] = {f.rhoUx, f.rhoUy, f.rhoUz};

@@ -176,6 +176,15 @@ struct Mesh {
     // wall-patch faces fanned from the mean of their points: [nT][3 corners][3]
     std::vector<double> wallTri;
     bool uploaded = false;  // device copies are made by the first rdst_case_create
+    // second-neighbour block pattern of the exact viscous linearisation (built
+    // on first use): row r holds r, its face neighbours and theirs, columns
+    // ascending; wideT[e] is the position of the transposed block of entry e,
+    // wideOfSlot[s] the position of block-CSR slot s; wideColor colours the
+    // cells so that two cells of one colour are more than four faces apart,
+    // hence no row holds two columns of one colour
+    bool wideBuilt = false;
+    int wideColors = 0;
+    std::vector<int32_t> wideRowPtr, wideCols, wideT, wideOfSlot, wideColor;
 
     // ---- device copies
     DBuf<int32_t> d_owner, d_neigh, d_bcell, d_bpatch;
@@ -191,6 +200,7 @@ struct Mesh {
     DBuf<int32_t> d_sellWidth, d_sellCol;
     DBuf<int32_t> d_bWall;  // [nB] 1 where the patch is a wall
     DBuf<double> d_C;       // cell centres SoA [3][nC]
+    DBuf<int32_t> d_wideRowPtr, d_wideCols, d_wideT, d_wideOfSlot, d_wideColor;
 };
 
 // largest partition of the block-Jacobi ILU(0); its FP64 vector (40 B/cell)
@@ -331,6 +341,13 @@ struct Case {
     DBuf<double> adjJ, adjJT;
     DBuf<float> adjJT32;
     bool hasAdjJ = false;
+    // viscous residual: J and J^T on the second-neighbour pattern (FP64, and
+    // J^T in FP32 for the mixed solve); adjJT then holds the block-CSR part of
+    // J^T, whose ILU(0) preconditions the solve
+    DBuf<double> adjW, adjWT;
+    DBuf<float> adjWT32;
+    DBuf<double> avGrad, avR, avJv, avObj;  // scratch of the forward-mode passes
+    bool adjWide = false;                   // the last linearisation is the second-neighbour one
 
     MeshDev meshDev() const;
 };
@@ -377,6 +394,7 @@ void adjoint_jacobian(Case& c);   // J = dR/dW only
 void newton_operator(Case& c);
 void newton_operator_csr(Case& c);  // block-CSR copy of the FP64 Newton operator (read-back)    // V dR/dW + rDeltaTDST V I into the case's A64 / A32
 void adjoint_linearise(Case& c);  // J and its transpose
+void build_wide(Mesh& m);         // the second-neighbour pattern of the viscous linearisation
 void adjoint_residual(Case& c, double* R);
 void adjoint_objective_gradient(Case& c, int patch, const double* dir, double q, double* g);
 void adjoint_apply(Case& c, bool transpose, const double* x, double* y);

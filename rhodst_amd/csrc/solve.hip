@@ -939,6 +939,13 @@ struct Ctx {
 // y = A x: the sliced-ELL copy of A when one was built for this step
 template <class TA, class TX, class TY>
 void spmv(Ctx& k, const TA* A, const TX* x, TY* y) {
+    const void* Av = (const void*)A;
+    if (Av && (Av == (const void*)k.c.adjW.p || Av == (const void*)k.c.adjWT.p || Av == (const void*)k.c.adjWT32.p)) {
+        // the viscous adjoint's operators on the second-neighbour pattern
+        hipLaunchKernelGGL((k_bspmv<TA, TX, TY>), dim3(blocks_for(k.m.nC, 256)), dim3(256), 0, k.s, k.m.nC,
+                           k.m.d_wideRowPtr.p, k.m.d_wideCols.p, A, x, y);
+        return;
+    }
     const TA* S = nullptr;
     if ((const void*)A == (const void*)k.c.A64.p) S = (const TA*)k.c.S64.p;
     if ((const void*)A == (const void*)k.c.A32.p) S = (const TA*)k.c.S32.p;
@@ -1406,13 +1413,16 @@ void adjoint_apply(Case& c, bool transpose, const double* x, double* y) {
     if (!c.hasAdjJ) throw ArgError("rdst_adjoint_apply: no linearisation (call rdst_adjoint_linearise)", RDST_ERR_STATE);
     ensure_work(c);
     Ctx k{c, *c.mesh, *c.work, c.stream, c.work->G};
-    spmv(k, (const double*)(transpose ? c.adjJT.p : c.adjJ.p), x, y);
+    if (c.adjWide) spmv(k, (const double*)(transpose ? c.adjWT.p : c.adjW.p), x, y);
+    else spmv(k, (const double*)(transpose ? c.adjJT.p : c.adjJ.p), x, y);
     RDST_HIP(hipGetLastError());
 }
 
 // J^T lambda = g with the primal solver machinery on the transposed
 // matrix: block ILU(0) of J^T (natural or partitioned order as the case's
-// settings) and GMRES(m); mixed: FP32 factors and cycles, FP64 outer residual
+// settings) and GMRES(m); mixed: FP32 factors and cycles, FP64 outer residual.
+// Viscous: the operator is J^T on the second-neighbour pattern and the ILU(0)
+// is taken of its block-CSR part (the face-neighbour blocks)
 void adjoint_solve(Case& c, const double* g, double* lambda, const rdst_adjoint_settings& as, rdst_solver_perf* perf) {
     if (!c.hasAdjJ) throw ArgError("rdst_adjoint_solve: no linearisation", RDST_ERR_STATE);
     const rdst_settings saved = c.st;
@@ -1435,10 +1445,18 @@ void adjoint_solve(Case& c, const double* g, double* lambda, const rdst_adjoint_
             hipLaunchKernelGGL((k_cast<double, float>), dim3(std::min(4096, blocks_for((long)nn, 256))), dim3(256), 0,
                                c.stream, (long)nn, c.adjJT.p, c.adjJT32.p);
             ilu_factor<float>(k, c.adjJT32.p, k.w.LU32.p, k.w.Dinv32.p);
-            p = gmres_mixed<double>(k, c.adjJT.p, c.adjJT32.p, lambda, g);
+            if (c.adjWide) {
+                const size_t nw = c.adjWT.n;
+                if (c.adjWT32.n != nw) c.adjWT32.alloc(nw);
+                hipLaunchKernelGGL((k_cast<double, float>), dim3(std::min(4096, blocks_for((long)nw, 256))), dim3(256),
+                                   0, c.stream, (long)nw, c.adjWT.p, c.adjWT32.p);
+                p = gmres_mixed<double>(k, c.adjWT.p, c.adjWT32.p, lambda, g);
+            } else {
+                p = gmres_mixed<double>(k, c.adjJT.p, c.adjJT32.p, lambda, g);
+            }
         } else {
             ilu_factor<double>(k, c.adjJT.p, k.w.LU.p, k.w.Dinv.p);
-            p = gmres_fp64(k, c.adjJT.p, lambda, g);
+            p = gmres_fp64(k, c.adjWide ? c.adjWT.p : c.adjJT.p, lambda, g);
         }
         RDST_HIP(hipStreamSynchronize(c.stream));
         if (perf) *perf = p;
