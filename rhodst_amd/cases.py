@@ -284,7 +284,7 @@ def box_sa_case(model: str = "SAneg", scheme: str = "HLLC", nx: int = 24, ny: in
 
 
 def bump_case(amplitudes=(0.02, 0.01, -0.005, 0.004), nx: int = 48, ny: int = 24, mach: float = 0.5,
-              scheme: str = "HLLC", node_disp=None, viscous: bool = False, reynolds: float = 500.0,
+              scheme: str = "HLLC", node_disp=None, viscous: bool = False, reynolds: float = 5000.0,
               **settings) -> CaseSpec:
     """Subsonic flow through a channel over a Hicks-Henne bump on the bottom
     wall (meshgen.bump_channel): characteristic far-field inlet and outlet

@@ -203,7 +203,7 @@ def test_frozen_eddy_viscosity_sa():
 
 def test_viscous_shape_gradient():
     """dCd/da of the Hicks-Henne amplitudes of the laminar channel bump (Re
-    500): the adjoint against central differences of the re-converged Cd."""
+    5000): the adjoint against central differences of the re-converged Cd."""
     from rhodst_amd.adjoint import BumpProblem
     print("[viscous adjoint shape gradient]", flush=True)
     a0 = np.array([0.02, 0.01, -0.005, 0.004])
