@@ -234,6 +234,69 @@ def configs_mode(args) -> int:
     return 0
 
 
+def adjoint_mode(args) -> int:
+    """The discrete adjoint on the viscous hybrid airfoil (205 k cells, HLLC +
+    SAneg, M 0.73, Re 6.5e6) at a shape seeded with Hicks-Henne bumps of 1e-4
+    chord: the flow converged by the mixed path, the exact viscous J with the
+    eddy viscosity frozen, J^T lambda = dCd/dW solved with the FP32
+    transposed ILU inside the FP64 defect correction (and in FP64 for the
+    comparison), J v against central differences of the residual (h 1e-7),
+    and dCd/d(displacement) of the 400 wall nodes (batched central
+    differences of the frozen-state partials, 16 mesh builds), a few nodes
+    also displaced alone."""
+    from rhodst_amd.adjoint import AirfoilProblem
+    out = {"workload": "hybrid airfoil 205191 cells, HLLC + SAneg (frozen eddy viscosity in the adjoint), "
+                       "400 wall nodes, seeded Hicks-Henne bumps 1e-4 chord"}
+    prob = AirfoilProblem(precision="mixed", ilu_order="partitioned", gram_schmidt="classical")
+    t0 = time.perf_counter()
+    cd, steps, hist = prob.solve(max_steps=args.adjoint_steps, drop=1e-8, log=log)
+    out["primal"] = {"steps": steps, "seconds": round(time.perf_counter() - t0, 2), "drop": hist[-1][1], "Cd": cd}
+    dc = prob.dc
+    patch = dc.spec.mesh.patch_index("airfoil")
+    t = time.perf_counter()
+    dc.adjoint_linearise()
+    m = dc.mesh
+    out["linearise_s"] = round(time.perf_counter() - t, 3)
+    g = dc.adjoint_objective_gradient(patch, prob.drag, prob.q)
+    sol = {}
+    for prec in ("mixed", "fp64"):
+        t = time.perf_counter()
+        lam, perf = prob.adjoint(precision=prec)
+        sec = time.perf_counter() - t
+        r = np.linalg.norm(dc.adjoint_apply(lam, transpose=True) - g) / np.linalg.norm(g)
+        sol[prec] = lam
+        out[f"adjoint_{prec}"] = {"seconds": round(sec, 3), "cycles": int(perf.n_iterations),
+                                 "residual_rel": float(r)}
+        log(f"adjoint {prec}: {out[f'adjoint_{prec}']}")
+    out["lambda_mixed_vs_fp64"] = float(np.abs(sol["mixed"] - sol["fp64"]).max() / np.abs(sol["fp64"]).max())
+    # J v against central differences of the residual
+    W = dc.get(9).reshape(-1, 5)
+    rng = np.random.default_rng(5)
+    v = rng.uniform(-1, 1, size=W.shape) * np.abs(W).max(axis=0)
+    Jv = dc.adjoint_apply(v)
+    h = 1e-7
+    dc.put(9, W + h * v)
+    Rp = dc.adjoint_residual()
+    dc.put(9, W - h * v)
+    Rm = dc.adjoint_residual()
+    dc.put(9, W)
+    fd = (Rp - Rm) / (2 * h)
+    out["Jv_vs_fd_rel"] = (np.abs(Jv - fd).max(axis=0) / np.abs(fd).max(axis=0)).tolist()
+    log(f"J v vs central differences: {out['Jv_vs_fd_rel']}")
+    lam = sol["mixed"]
+    t = time.perf_counter()
+    gn = prob.node_gradient(lam)
+    out["node_gradient_s"] = round(time.perf_counter() - t, 2)
+    picks = [0, 97, 200, 301, 399]
+    gs = prob.node_gradient(lam, nodes=picks)
+    out["node_gradient"] = {"n_nodes": len(gn), "max_abs": float(np.abs(gn).max()),
+                            "batched_vs_alone_rel": float(np.abs(gn[picks] - gs).max() / np.abs(gn).max()),
+                            "sample": {int(k): float(gn[k]) for k in picks}}
+    log(f"node gradient: {out['node_gradient']} in {out['node_gradient_s']} s")
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def cpu_baseline(args, mesh_kw, case_kw, wall_dist, faces):
     """The oracle's face flux + residual on the host: all threads (OpenMP
     build, bit-identical to the serial one) and one thread, each in a child
@@ -285,6 +348,9 @@ def main(argv=None) -> int:
     ap.add_argument("--configs", action="store_true",
                     help="only the two smaller workloads (forward step to 1e-8, hybrid airfoil per step) on the GPU "
                          "and the CPU oracle (all threads and one): one JSON line")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="only the discrete adjoint on the viscous hybrid airfoil (400 wall nodes): one JSON line")
+    ap.add_argument("--adjoint-steps", type=int, default=4000, help="step limit of the airfoil flow before the adjoint")
     ap.add_argument("--flux-only", action="store_true",
                     help="only the face-flux kernel (n calls) and a 256 MiB device copy of known size: the "
                          "short program the FETCH_SIZE / WRITE_SIZE counter runs profile")
@@ -296,6 +362,8 @@ def main(argv=None) -> int:
 
     if args.configs:
         return configs_mode(args)
+    if args.adjoint:
+        return adjoint_mode(args)
     from rhodst_amd import cases, meshgen
     from rhodst_amd._capi import DeviceMesh
     mesh_kw = dict(n_wake=24, n_surf=64, n_normal=64, n_span_wing=24, n_span_out=8) if args.small else {}

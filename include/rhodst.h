@@ -526,6 +526,11 @@ int rdst_adjoint_residual(rdst_case* c, double* R);
 /* dJ/dW [n_cells][5] of J = sum_{faces of patch} (p_b - p_ref) (Sf . dir) / q
  * (+ Sf & devRhoReff_b . dir / q with settings.viscous, as rdst_forces) */
 int rdst_adjoint_objective_gradient(rdst_case* c, int patch, const double* dir, double q, double* g);
+/* each cell's share of J (values, [n_cells]): its faces on the patch,
+ * (p_b - p_ref)(Sf . dir)/q (+ the viscous force with settings.viscous); the
+ * shares of cells near a displaced wall node give that node's dJ/dx|_W by
+ * differences without summing the whole patch */
+int rdst_adjoint_objective_cells(rdst_case* c, int patch, const double* dir, double q, double p_ref, double* out);
 /* linearises at the current state, then solves J^T lambda = dJ/dW; lambda
  * [n_cells][5] holds the initial guess (zeros for none) and receives the solution */
 int rdst_adjoint_solve(rdst_case* c, int patch, const double* dir, double q, const rdst_adjoint_settings* s,

@@ -3010,6 +3010,22 @@ int orc_case_put(void* h, int field, const double* in) {
                 c.tb.nut.assign(in, in + n);
                 saNutBoundary(c);
                 break;
+            // an eddy viscosity held fixed (a differenced residual with the
+            // turbulence frozen); alphat_b = mut_b / Prt as saMutBoundary sets it
+            case RDST_F_MUT:
+                if (!c.tb.on) throw std::runtime_error("no turbulence model");
+                c.tb.mut.assign(in, in + n);
+                break;
+            case RDST_F_ALPHAT:
+                if (!c.tb.on) throw std::runtime_error("no turbulence model");
+                c.tb.alphat.assign(in, in + n);
+                break;
+            case RDST_F_BMUT:
+                if (!c.tb.on) throw std::runtime_error("no turbulence model");
+                c.tb.bmut.assign(in, in + c.m.nB);
+                c.tb.balphat.resize(c.m.nB);
+                for (int b = 0; b < c.m.nB; ++b) c.tb.balphat[b] = c.tb.bmut[b] / c.tb.t.Prt;
+                break;
             default: throw std::runtime_error("field not writable");
         }
     })

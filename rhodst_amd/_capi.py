@@ -27,7 +27,7 @@ EXPORTS = ["rdst_last_error", "rdst_version", "rdst_init", "rdst_mesh_create", "
            "rdst_case_destroy", "rdst_case_set_state", "rdst_case_get", "rdst_case_put",
            "rdst_flux_residual", "rdst_jacobian_assemble", "rdst_block_solve", "rdst_ldu_solve", "rdst_update_fields",
            "rdst_set_courant", "rdst_iterate", "rdst_dual_step", "rdst_time_info", "rdst_case_set_turbulence", "rdst_turbulence_correct",
-           "rdst_adjoint_linearise", "rdst_adjoint_apply", "rdst_adjoint_residual",
+           "rdst_adjoint_linearise", "rdst_adjoint_apply", "rdst_adjoint_residual", "rdst_adjoint_objective_cells",
            "rdst_adjoint_objective_gradient", "rdst_adjoint_solve",
            "rdst_forces", "rdst_forces_noalloc", "rdst_last_flux_ms", "rdst_time_flux"]
 
@@ -80,6 +80,8 @@ def lib():
     L.rdst_adjoint_linearise.argtypes = [C.c_void_p]
     L.rdst_adjoint_apply.argtypes = [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double)]
     L.rdst_adjoint_residual.argtypes = [C.c_void_p, P(C.c_double)]
+    L.rdst_adjoint_objective_cells.argtypes = [C.c_void_p, C.c_int, P(C.c_double), C.c_double, C.c_double,
+                                               P(C.c_double)]
     L.rdst_adjoint_objective_gradient.argtypes = [C.c_void_p, C.c_int, P(C.c_double), C.c_double, P(C.c_double)]
     L.rdst_adjoint_solve.argtypes = [C.c_void_p, C.c_int, P(C.c_double), C.c_double, P(rdst_adjoint_settings),
                                      P(C.c_double), P(rdst_solver_perf)]
@@ -207,6 +209,12 @@ class DeviceCase:
         R = np.zeros((self.n, 5))
         check(lib().rdst_adjoint_residual(self.h, dptr(R)))
         return R
+
+    def adjoint_objective_cells(self, patch, direction, q, p_ref):
+        d = np.ascontiguousarray(direction, dtype=np.float64)
+        o = np.zeros(self.n)
+        check(lib().rdst_adjoint_objective_cells(self.h, patch, dptr(d), float(q), float(p_ref), dptr(o)))
+        return o
 
     def adjoint_objective_gradient(self, patch, direction, q):
         d = np.ascontiguousarray(direction, dtype=np.float64)

@@ -151,6 +151,147 @@ class BumpProblem:
         return g
 
 
+class AirfoilProblem:
+    """Cd of the viscous hybrid airfoil (cases.airfoil_case: M 0.73, AoA 2.3
+    deg, Re 6.5e6 on the unit chord, Spalart-Allmaras) as a function of the
+    displacements of its wall nodes along their wall-layer lines (the
+    surface normal at the wall; meshgen.hybrid_airfoil wall_disp), at a shape
+    seeded with smooth Hicks-Henne bumps of ``seed_amplitude`` chords on each
+    surface.  The design nodes are the wall loop's surface points, lower and
+    upper, without the lower trailing-edge corner (n_surf of them).
+
+    The adjoint freezes the eddy viscosity of the converged flow (mut, alphat
+    and mut on the boundary): the gradient is exact for the residual with
+    that eddy viscosity held, and leaves out the turbulence model's response
+    to the shape.  The partial derivatives at the frozen state are central
+    differences of each cell's residual and objective share on displaced
+    meshes; nodes ``stride`` apart along the loop are displaced together
+    (their residual supports -- the cells of the two wall-layer columns
+    beside the node and two face neighbours out -- do not overlap at a
+    stride of 8) and each changed cell is credited to the nearest displaced
+    node, so the whole gradient costs 2 stride mesh builds."""
+
+    def __init__(self, mesh_kw=None, turbulence="SAneg", seed_amplitude=1e-4, n_seed_modes=4, seed=3,
+                 fd_step=1e-7, **case_kw):
+        from . import meshgen
+        self.meshgen = meshgen
+        self.mesh_kw = dict(mesh_kw or {})
+        n_surf = self.mesh_kw.get("n_surf", 400)
+        loop, kind = meshgen.airfoil_loop(n_surf, self.mesh_kw.get("thickness", 0.12))
+        self.loop, self.kind = loop, kind
+        lower = np.nonzero(kind == 0)[0]
+        te_lower = lower[np.argmax(loop[lower, 0] - 1e-9 * loop[lower, 1])]
+        self.design = np.array([i for i in np.nonzero(kind < 2)[0] if i != te_lower])
+        rng = np.random.default_rng(seed)
+        self.base_disp = np.zeros(len(loop))
+        for k in (0, 1):
+            pts = np.nonzero(kind == k)[0]
+            amp = seed_amplitude * rng.uniform(-1.0, 1.0, n_seed_modes)
+            self.base_disp[pts] = amp @ meshgen.hicks_henne(loop[pts, 0], n_seed_modes)
+        self.turbulence = turbulence
+        self.case_kw = case_kw
+        self.fd_step = fd_step
+        gamma, R, Tinf, pinf = 1.4, 287.0, 288.15, 101325.0
+        mach = case_kw.get("mach", 0.73)
+        a = np.deg2rad(case_kw.get("aoa_deg", 2.3))
+        self.U = mach * np.sqrt(gamma * R * Tinf)
+        self.rho = pinf / (R * Tinf)
+        self.p_ref = pinf
+        self.a_ref = 1.0 * self.mesh_kw.get("depth", 0.1)
+        self.q = 0.5 * self.rho * self.U ** 2 * self.a_ref
+        self.drag = np.array([np.cos(a), np.sin(a), 0.0])
+        self.lift = np.array([-np.sin(a), np.cos(a), 0.0])
+        self.dc = None
+
+    def mesh(self, d=None):
+        """The mesh at the seeded shape, displaced by d at the design nodes."""
+        if getattr(self, "_undisplaced", None) is None:
+            self._undisplaced = self.meshgen.hybrid_airfoil(**self.mesh_kw)
+        disp = self.base_disp.copy()
+        if d is not None:
+            disp[self.design] += d
+        return self.meshgen.displace_airfoil_wall(self._undisplaced, disp, self.mesh_kw.get("n_layers", 40))
+
+    def coefficient(self, dc):
+        patch = dc.spec.mesh.patch_index("airfoil")
+        return float(dc.forces(patch, self.drag, self.lift, self.p_ref, self.rho, self.U, self.a_ref)[0])
+
+    def solve(self, max_steps=3000, drop=1e-8, chunk=100, log=print):
+        """Converge the flow on the seeded shape; keep the state and its eddy viscosity."""
+        from ._capi import DeviceMesh
+        mesh = self.mesh()
+        dm = DeviceMesh(mesh)
+        wd = dm.wall_distance() if self.turbulence else None
+        spec = cases.airfoil_case(mesh=mesh, turbulence=self.turbulence, wall_dist=wd, **self.case_kw)
+        dc = DeviceCase(spec, dm)
+        r0, steps, hist = None, 0, []
+        while steps < max_steps:
+            res = dc.iterate(chunk)
+            steps += chunk
+            if r0 is None:
+                r0 = res[0]
+            if not np.all(np.isfinite(res)):
+                raise RuntimeError("airfoil case diverged")
+            hist.append((steps, res[-1] / r0))
+            if log:
+                log(f"[airfoil] step {steps} drop {res[-1] / r0:.3e}")
+            if res[-1] <= drop * r0:
+                break
+        self.dc, self.mesh0 = dc, mesh
+        self.W = dc.get(9).reshape(-1, 5)
+        self.frozen_turb = (dc.get(30), dc.get(31), dc.get(34)) if self.turbulence else None
+        return self.coefficient(dc), steps, hist
+
+    def adjoint(self, **solver):
+        dc = self.dc
+        opts = dict(precision="mixed", n_directions=30, max_cycles=400, tolerance=1e-12, rel_tol=0.0)
+        opts.update(solver)
+        return dc.adjoint_solve(dc.spec.mesh.patch_index("airfoil"), self.drag, self.q, **opts)
+
+    def frozen(self, d, W):
+        """A case on the mesh displaced by d carrying W and the frozen eddy
+        viscosity, its boundary values and boundary gradient formed from W."""
+        spec = cases.airfoil_case(mesh=self.mesh(d), turbulence=None, **self.case_kw)
+        dc = DeviceCase(spec)
+        dc.put(9, W)
+        if self.frozen_turb is not None:
+            mut, alphat, bmut = self.frozen_turb
+            dc.put(30, mut)
+            dc.put(31, alphat)
+            dc.put(34, bmut)
+        dc.update_fields()
+        dc.flux_residual()
+        dc.update_fields()
+        dc.flux_residual()
+        dc.update_fields()
+        return dc
+
+    def _shares(self, dc):
+        patch = dc.spec.mesh.patch_index("airfoil")
+        return dc.adjoint_residual(), dc.adjoint_objective_cells(patch, self.drag, self.q, self.p_ref)
+
+    def node_gradient(self, lam, stride=8, nodes=None):
+        """dCd/d(displacement) of the design nodes (all, or the indices
+        ``nodes`` into the design list, each then displaced alone)."""
+        h = self.fd_step
+        nd = len(self.design)
+        cen = self.meshgen.cell_centres_approx(self.mesh0)[:, :2]
+        wall = self.loop[self.design]
+        g = np.zeros(nd)
+        groups = [np.array([k]) for k in nodes] if nodes is not None else \
+            [np.arange(col, nd, stride) for col in range(stride)]
+        for sel in groups:
+            e = np.zeros(nd)
+            e[sel] = h
+            Rp, op = self._shares(self.frozen(e, self.W))
+            Rm, om = self._shares(self.frozen(-e, self.W))
+            contrib = ((op - om) - np.sum(lam * (Rp - Rm), axis=1)) / (2 * h)
+            hit = np.nonzero(np.any(Rp != Rm, axis=1) | (op != om))[0]
+            d2 = np.sum((cen[hit, None, :] - wall[None, sel, :]) ** 2, axis=2)
+            np.add.at(g, sel[np.argmin(d2, axis=1)], contrib[hit])
+        return g if nodes is None else g[np.asarray(nodes)]
+
+
 class ForceCoefficient(torch.autograd.Function):
     """a (torch float64 [n_modes]) -> C(a); backward by the discrete adjoint."""
 

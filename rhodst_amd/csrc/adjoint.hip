@@ -445,6 +445,31 @@ __global__ __launch_bounds__(256) void k_adj_objective(MeshDev m, Fields F, cons
     for (int kk = 0; kk < 5; ++kk) g[5 * (size_t)c + kk] = gc[kk];
 }
 
+// each cell's share of J = sum_{b in patch} (p_b - pRef)(Sf_b . dir)/q (values)
+__global__ __launch_bounds__(256) void k_adj_objective_cells(MeshDev m, Fields F, const BcPatchDev* bc, Thermo th,
+                                                             Bounds bd, int patch, double dx, double dy, double dz,
+                                                             double rq, double pRef, double* o) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.nC) return;
+    const size_t E = (size_t)m.ellW * m.nC;
+    const Prim<double> I = prim_from_w(F.W[5 * (size_t)c], F.W[5 * (size_t)c + 1], F.W[5 * (size_t)c + 2],
+                                       F.W[5 * (size_t)c + 3], F.W[5 * (size_t)c + 4], th, bd);
+    double s = 0.0;
+    for (int ke = 0; ke < m.ellW; ++ke) {
+        const size_t e = (size_t)ke * m.nC + c;
+        const uint32_t code = m.ellCode[e];
+        const uint32_t kind = code >> 30;
+        if (kind == ELL_NONE) break;
+        if (kind != ELL_BOUNDARY) continue;
+        const int idx = (int)(code & ELL_IDX_MASK);
+        if (m.bpatch[idx] != patch) continue;
+        const double sx = m.ellSf[e], sy = m.ellSf[E + e], sz = m.ellSf[2 * E + e], ms = m.ellMagSf[e];
+        const BState<double> bs = bc_state(bc[patch], F, idx, m.nI + idx, sx / ms, sy / ms, sz / ms, ms, I, th, bd);
+        s += (bs.p - pRef) * ((sx * dx + sy * dy + sz * dz) * rq);
+    }
+    o[c] = s;
+}
+
 // J^T on the same slots: (r, c) <- (c, r)^T
 __global__ void k_transpose_faces(int nI, const int32_t* slotUpper, const int32_t* slotLower, const double* J,
                                   double* JT) {
@@ -590,7 +615,7 @@ __global__ __launch_bounds__(256) void k_av_resid(MeshDev m, Fields F, const BcP
                                                   int lowMach, double Mainf, double lapPsi, double snPsi, int heat,
                                                   const int32_t* color, int col, int comp, const double* gd,
                                                   double* Rv, double* Jv, int patch, double dx, double dy, double dz,
-                                                  double rq, double* objd) {
+                                                  double rq, double pRef, double* objd, double* objv) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
     const int n = m.nC;
@@ -640,7 +665,7 @@ __global__ __launch_bounds__(256) void k_av_resid(MeshDev m, Fields F, const BcP
             if (heat) qh = (F.balphat[idx] + alpha_lam(th, bs.Tb)) * ms * (m.ellDc[e] * (bs.h - Pc.h));
             if (m.bpatch[idx] == patch) {
                 // forceCoeffs: (p_b - p_ref) Sf + Sf & devRhoReff_b, devRhoReff = -muEff dev(twoSymm(grad U))
-                ob += bs.p * ((sx * dx + sy * dy + sz * dz) * rq);
+                ob += (bs.p - pRef) * ((sx * dx + sy * dy + sz * dz) * rq);
                 const D1 tr = gb[0] + gb[4] + gb[8];
                 const double S3[3] = {sx, sy, sz}, dir[3] = {dx, dy, dz};
                 for (int j = 0; j < 3; ++j) {
@@ -722,6 +747,7 @@ __global__ __launch_bounds__(256) void k_av_resid(MeshDev m, Fields F, const BcP
         if (Jv) Jv[5 * (size_t)c + q] = r[q].d;
     }
     if (objd) objd[c] = ob.d;
+    if (objv) objv[c] = ob.v;
 }
 
 // column comp of the blocks of this pass's colour: row r's tangent is the
@@ -810,13 +836,14 @@ void launch_res(Case& c, double* R) {
 
 template <int S>
 void launch_av_resid(Case& c, int col, int comp, double* Rv, double* Jv, int patch, const double* dir, double rq,
-                     double* objd) {
+                     double* objd, double pRef = 0.0, double* objv = nullptr) {
     const Mesh& m = *c.mesh;
     const int nb = blocks_for(m.nC, 256);
     hipLaunchKernelGGL((k_av_resid<S>), dim3(nb), dim3(256), 0, c.stream, c.meshDev(), c.f, c.d_bc.p, c.th, bounds_of(c),
                        c.st.low_mach, c.st.Mainf, sngrad_psi(c.st.laplacian_sngrad, c.st.laplacian_limit),
                        sngrad_psi(c.st.sngrad, c.st.sngrad_limit), c.st.heat_conduction, m.d_wideColor.p, col, comp,
-                       c.avGrad.p, Rv, Jv, patch, dir ? dir[0] : 0.0, dir ? dir[1] : 0.0, dir ? dir[2] : 0.0, rq, objd);
+                       c.avGrad.p, Rv, Jv, patch, dir ? dir[0] : 0.0, dir ? dir[1] : 0.0, dir ? dir[2] : 0.0, rq, pRef,
+                       objd, objv);
 }
 
 // one forward-mode pass: gradient, then residual (and objective) tangents
@@ -1063,6 +1090,39 @@ void adjoint_objective_gradient(Case& c, int patch, const double* dir, double q,
     const Mesh& m = *c.mesh;
     hipLaunchKernelGGL(k_adj_objective, dim3(blocks_for(m.nC, 256)), dim3(256), 0, c.stream, c.meshDev(), c.f, c.d_bc.p,
                        c.th, bounds_of(c), patch, dir[0], dir[1], dir[2], 1.0 / q, g);
+    RDST_HIP(hipGetLastError());
+}
+
+// each cell's share of the force objective (values; sum = J): the pressure
+// part of its faces on the patch and, viscous, their viscous force
+void adjoint_objective_cells(Case& c, int patch, const double* dir, double q, double pRef, double* o) {
+    adjoint_check_scheme(c);
+    Mesh& m = *c.mesh;
+    if (c.st.viscous) {
+        build_wide(m);
+        av_buffers(c);
+        hipLaunchKernelGGL(k_av_grad, dim3(blocks_for(m.nC, 256)), dim3(256), 0, c.stream, c.meshDev(), c.f, c.d_bc.p,
+                           c.th, bounds_of(c), m.d_wideColor.p, -1, 0, c.avGrad.p);
+        switch (c.st.flux_scheme) {
+            case RDST_FLUX_AUSMPLUS:
+                launch_av_resid<RDST_FLUX_AUSMPLUS>(c, -1, 0, nullptr, nullptr, patch, dir, 1.0 / q, nullptr, pRef, o);
+                break;
+            case RDST_FLUX_AUSMPLUSUP:
+                launch_av_resid<RDST_FLUX_AUSMPLUSUP>(c, -1, 0, nullptr, nullptr, patch, dir, 1.0 / q, nullptr, pRef, o);
+                break;
+            case RDST_FLUX_HLL:
+                launch_av_resid<RDST_FLUX_HLL>(c, -1, 0, nullptr, nullptr, patch, dir, 1.0 / q, nullptr, pRef, o);
+                break;
+            case RDST_FLUX_HLLC:
+                launch_av_resid<RDST_FLUX_HLLC>(c, -1, 0, nullptr, nullptr, patch, dir, 1.0 / q, nullptr, pRef, o);
+                break;
+            default:
+                launch_av_resid<RDST_FLUX_KURGANOV>(c, -1, 0, nullptr, nullptr, patch, dir, 1.0 / q, nullptr, pRef, o);
+        }
+    } else {
+        hipLaunchKernelGGL(k_adj_objective_cells, dim3(blocks_for(m.nC, 256)), dim3(256), 0, c.stream, c.meshDev(), c.f,
+                           c.d_bc.p, c.th, bounds_of(c), patch, dir[0], dir[1], dir[2], 1.0 / q, pRef, o);
+    }
     RDST_HIP(hipGetLastError());
 }
 

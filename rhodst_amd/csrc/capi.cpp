@@ -791,6 +791,20 @@ int rdst_adjoint_objective_gradient(rdst_case* cc, int patch, const double* dir,
     })
 }
 
+int rdst_adjoint_objective_cells(rdst_case* cc, int patch, const double* dir, double q, double p_ref, double* out) {
+    RDST_API_BODY({
+        Case* c = adjoint_case(cc);
+        check_objective(*c, patch, dir, q);
+        if (!out) throw ArgError("rdst_adjoint_objective_cells: null array");
+        const size_t n = (size_t)c->mesh->nC;
+        DBuf<double> d;
+        d.alloc(n);
+        adjoint_objective_cells(*c, patch, dir, q, p_ref, d.p);
+        RDST_HIP(hipStreamSynchronize(c->stream));
+        RDST_HIP(hipMemcpy(out, d.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    })
+}
+
 int rdst_adjoint_solve(rdst_case* cc, int patch, const double* dir, double q, const rdst_adjoint_settings* s,
                        double* lambda, rdst_solver_perf* perf) {
     RDST_API_BODY({

@@ -397,6 +397,7 @@ void adjoint_linearise(Case& c);  // J and its transpose
 void build_wide(Mesh& m);         // the second-neighbour pattern of the viscous linearisation
 void adjoint_residual(Case& c, double* R);
 void adjoint_objective_gradient(Case& c, int patch, const double* dir, double q, double* g);
+void adjoint_objective_cells(Case& c, int patch, const double* dir, double q, double pRef, double* o);
 void adjoint_apply(Case& c, bool transpose, const double* x, double* y);
 void adjoint_solve(Case& c, const double* g, double* lambda, const rdst_adjoint_settings& as, rdst_solver_perf* perf);
 

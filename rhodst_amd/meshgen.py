@@ -19,7 +19,7 @@ from __future__ import annotations
 
 import os
 import re
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 
 import numpy as np
 
@@ -945,9 +945,31 @@ def _polygon_cell(poly2d: list, nb2: int, z_off: int) -> list:
     return faces
 
 
+def airfoil_loop(n_surf: int = 400, thickness: float = 0.12):
+    """The wall loop of hybrid_airfoil: points [ns, 2] in the loop's
+    (counter-clockwise) order and each point's kind (0 lower surface, 1 upper
+    surface, 2 blunt-base)."""
+    beta = np.linspace(0.0, np.pi, n_surf // 2 + 1)
+    xs = 0.5 * (1 - np.cos(beta))
+    yt = naca4_blunt(xs, thickness)
+    lower = np.stack([xs[::-1], -yt[::-1]], axis=1)          # TE -> LE (lower)
+    upper = np.stack([xs[1:], yt[1:]], axis=1)                # LE -> TE (upper)
+    h_te = yt[-1]
+    n_base = max(2, int(np.ceil(2 * h_te / (xs[-1] - xs[-2]))))
+    base = np.stack([np.ones(n_base - 1), np.linspace(h_te, -h_te, n_base + 1)[1:-1]], axis=1)
+    loop = np.concatenate([lower, upper, base], axis=0)
+    kind = np.concatenate([np.zeros(len(lower), int), np.ones(len(upper), int), np.full(len(base), 2)])
+    # counter-clockwise (signed area > 0)
+    area = 0.5 * np.sum(loop[:, 0] * np.roll(loop[:, 1], -1) - np.roll(loop[:, 0], -1) * loop[:, 1])
+    if area < 0:
+        loop, kind = loop[::-1].copy(), kind[::-1].copy()
+    return loop, kind
+
+
 def hybrid_airfoil(n_surf: int = 400, n_layers: int = 40, first_cell: float = 2.0e-6, ratio: float = 1.15,
                    far: float = 50.0, thickness: float = 0.12, outer_ratio: float = 1.15, layer_aspect: float = 1.8,
-                   jitter: float = 0.2, split_aspect: float = 2.5, seed: int = 1, depth: float = 0.1) -> PolyMesh:
+                   jitter: float = 0.2, split_aspect: float = 2.5, seed: int = 1, depth: float = 0.1,
+                   wall_disp=None) -> PolyMesh:
     """Hybrid mesh about a 12 % symmetric 4-digit section (blunt trailing edge):
     an O-grid marched off the wall along smoothed normals, n_layers quad
     layers at 'ratio' from 'first_cell', then layers growing at 'outer_ratio'
@@ -965,21 +987,16 @@ def hybrid_airfoil(n_surf: int = 400, n_layers: int = 40, first_cell: float = 2.
     non-orthogonality would follow the height variation along them)
     (seeded).  Extruded one cell thick: hexes and prisms.  Patches: airfoil
     (wall), farfield (patch), frontAndBack (empty).  The defaults give about
-    2e5 cells."""
+    2e5 cells.
+
+    wall_disp [ns] (airfoil_loop order): a displacement of each wall point
+    along its wall-layer line (the direction from the wall point to the next
+    ring's point, taken from the undisplaced mesh), carried into the wall
+    layers with a weight falling linearly in arc length from 1 at the wall to
+    0 at ring n_layers; the rest of the mesh and its topology are unchanged,
+    so the displaced mesh is a linear function of wall_disp."""
     rng = np.random.default_rng(seed)
-    beta = np.linspace(0.0, np.pi, n_surf // 2 + 1)
-    xs = 0.5 * (1 - np.cos(beta))
-    yt = naca4_blunt(xs, thickness)
-    lower = np.stack([xs[::-1], -yt[::-1]], axis=1)          # TE -> LE (lower), counter-clockwise start
-    upper = np.stack([xs[1:], yt[1:]], axis=1)                # LE -> TE (upper)
-    h_te = yt[-1]
-    n_base = max(2, int(np.ceil(2 * h_te / (xs[-1] - xs[-2]))))
-    base = np.stack([np.ones(n_base - 1), np.linspace(h_te, -h_te, n_base + 1)[1:-1]], axis=1)
-    loop = np.concatenate([lower, upper, base], axis=0)
-    # counter-clockwise check (signed area > 0)
-    area = 0.5 * np.sum(loop[:, 0] * np.roll(loop[:, 1], -1) - np.roll(loop[:, 0], -1) * loop[:, 1])
-    if area < 0:
-        loop = loop[::-1].copy()
+    loop, _ = airfoil_loop(n_surf, thickness)
     ns = len(loop)
     ctr = np.array([0.5, 0.0])
     rings = [loop]
@@ -1079,5 +1096,35 @@ def hybrid_airfoil(n_surf: int = 400, n_layers: int = 40, first_cell: float = 2.
         out[(~zf) & (rr < 2.0)] = 0
         return out
 
-    return cells_polymesh(points, cells, classify,
+    mesh = cells_polymesh(points, cells, classify,
                           [("airfoil", PATCH_WALL), ("farfield", PATCH_GENERIC), ("frontAndBack", PATCH_EMPTY)])
+    if wall_disp is not None:
+        mesh = displace_airfoil_wall(mesh, wall_disp, n_layers)
+    return mesh
+
+
+def displace_airfoil_wall(mesh: PolyMesh, wall_disp, n_layers: int) -> PolyMesh:
+    """hybrid_airfoil's mesh with its wall points displaced: point i of the
+    wall loop (airfoil_loop order, len(wall_disp) points) moves by
+    wall_disp[i] along its wall-layer line (the unit vector from the wall point
+    to the next ring's point of the given mesh), and the points of that line
+    up to ring n_layers by wall_disp[i] times a weight falling linearly in arc
+    length from 1 at the wall to 0 at ring n_layers.  The topology is kept, so
+    the points are a linear function of wall_disp."""
+    d = np.asarray(wall_disp, dtype=np.float64)
+    ns = len(d)
+    nP = len(mesh.points) // 2
+    if nP % ns:
+        raise ValueError("wall_disp does not match the mesh's wall loop")
+    P = mesh.points[:nP, :2].reshape(-1, ns, 2)
+    L = min(n_layers, P.shape[0] - 1)
+    seg = np.linalg.norm(np.diff(P[:L + 1], axis=0), axis=2)
+    arc = np.concatenate([np.zeros((1, ns)), np.cumsum(seg, axis=0)], axis=0)
+    fade = 1.0 - arc / arc[-1][None, :]
+    line = P[1] - P[0]
+    line = line / np.linalg.norm(line, axis=1)[:, None]
+    delta = ((d[None, :, None] * fade[:, :, None]) * line[None, :, :]).reshape(-1, 2)
+    pts = mesh.points.copy()
+    pts[:(L + 1) * ns, :2] += delta
+    pts[nP:nP + (L + 1) * ns, :2] += delta
+    return replace(mesh, points=pts)

@@ -223,3 +223,63 @@ def test_viscous_shape_gradient():
         print(f"mode {i}: adjoint {g_adj[i]:.10e} fd {fd[i]:.10e}", flush=True)
     rel = np.abs(g_adj - fd) / np.abs(fd).max()
     assert np.all(rel < 1e-5), rel
+
+
+def test_airfoil_node_gradient_batching():
+    """Wall-node gradient of a small Spalart-Allmaras hybrid airfoil (10 k
+    cells, frozen eddy viscosity): nodes 8 apart displaced together, each
+    changed cell credited to the nearest displaced node, against each node
+    displaced alone."""
+    from rhodst_amd.adjoint import AirfoilProblem
+    print("[viscous adjoint airfoil node gradient]", flush=True)
+    prob = AirfoilProblem(mesh_kw=dict(n_surf=80, n_layers=16, first_cell=2e-5, far=8.0))
+    cd, steps, _ = prob.solve(max_steps=200, log=None)
+    lam, perf = prob.adjoint(precision="fp64", max_cycles=40, tolerance=1e-8)
+    print("Cd", cd, "adjoint cycles", perf.n_iterations, flush=True)
+    g = prob.node_gradient(lam)
+    picks = [0, 17, 40, 63, 79]
+    gs = prob.node_gradient(lam, nodes=picks)
+    print("batched", g[picks], "alone", gs, flush=True)
+    assert np.all(np.isfinite(g)) and np.abs(g).max() > 0
+    # the two differ by the rounding of the differences (h = 1e-7)
+    assert np.abs(g[picks] - gs).max() <= 1e-7 * np.abs(g).max()
+
+
+def test_airfoil_full_size_jacobian_against_oracle(oracle):
+    """The adjoint benchmark's workload at full size (hybrid airfoil, 205 k
+    cells, HLLC + SAneg with the eddy viscosity frozen): J v against central
+    differences (h = 1e-7) of the oracle's residual, the oracle holding the
+    device's eddy viscosity and stored mass flux."""
+    from rhodst_amd.adjoint import AirfoilProblem
+    print("[viscous adjoint airfoil 205k vs oracle]", flush=True)
+    prob = AirfoilProblem(precision="mixed", ilu_order="partitioned", gram_schmidt="classical")
+    prob.solve(max_steps=20, chunk=20, log=None)
+    dc = prob.dc
+    dc.update_fields()
+    dc.update_fields()
+    dc.adjoint_linearise()
+    W = dc.get(9).reshape(-1, 5)
+    phi = dc.get(12)
+    mut, alphat, bmut = dc.get(30), dc.get(31), dc.get(34)
+    oc = oracle.OracleCase(dc.spec)
+    print("oracle case built", flush=True)
+
+    def R_or(Wx):
+        oc.put(9, Wx)
+        oc.put(12, phi)
+        oc.update_fields()
+        oc.update_fields()
+        oc.put(30, mut)
+        oc.put(31, alphat)
+        oc.put(34, bmut)
+        oc.flux_residual()
+        return oc.get(8).reshape(-1, 5)
+
+    rng = np.random.default_rng(9)
+    v = rng.uniform(-1, 1, size=W.shape) * np.abs(W).max(axis=0)
+    Jv = dc.adjoint_apply(v)
+    h = 1e-7
+    fd = (R_or(W + h * v) - R_or(W - h * v)) / (2 * h)
+    err = np.abs(Jv - fd).max(axis=0) / np.abs(fd).max(axis=0)
+    print("relative error per component", err, flush=True)
+    assert np.all(err < 1e-6), err

@@ -37,3 +37,32 @@ def test_hybrid_airfoil_quality(oracle):
     wall_d = wd[m.owner[start:start + size]]
     assert abs(np.median(wall_d) - 1e-5) < 1e-7
     assert wall_d.min() > 0.75e-5 and wall_d.max() < 1.01e-5
+
+
+def test_airfoil_wall_displacement(oracle):
+    """displace_airfoil_wall: each wall point moves by its displacement along
+    its wall-layer line, the line's points by the same times a weight falling
+    to 0 at ring n_layers; linear in the displacement; seeded bumps of 1e-4
+    chord on the full first-cell height keep every cell positive."""
+    kw = dict(n_surf=80, n_layers=16, first_cell=2e-6, far=8.0)
+    m0 = meshgen.hybrid_airfoil(**kw)
+    loop, kind = meshgen.airfoil_loop(80)
+    ns = len(loop)
+    rng = np.random.default_rng(1)
+    d1, d2 = 1e-4 * rng.uniform(-1, 1, ns), 1e-4 * rng.uniform(-1, 1, ns)
+    m1 = meshgen.displace_airfoil_wall(m0, d1, 16)
+    m2 = meshgen.displace_airfoil_wall(m0, d2, 16)
+    m12 = meshgen.displace_airfoil_wall(m0, d1 + d2, 16)
+    assert np.abs((m12.points - m0.points) - (m1.points - m0.points) - (m2.points - m0.points)).max() < 1e-15
+    nP = len(m0.points) // 2
+    P0 = m0.points[:nP, :2].reshape(-1, ns, 2)
+    P1 = m1.points[:nP, :2].reshape(-1, ns, 2)
+    line = (P0[1] - P0[0]) / np.linalg.norm(P0[1] - P0[0], axis=1)[:, None]
+    assert np.abs((P1[0] - P0[0]) - d1[:, None] * line).max() < 1e-15
+    assert np.array_equal(P1[16:], P0[16:])
+    assert np.array_equal(m1.points[:, 2], m0.points[:, 2])
+    assert np.array_equal(m1.face_points, m0.face_points) and np.array_equal(m1.owner, m0.owner)
+    # the same as generating with wall_disp
+    assert np.array_equal(meshgen.hybrid_airfoil(**kw, wall_disp=d1).points, m1.points)
+    g = oracle.geometry(m1)
+    assert (g["V"] > 0).all()
