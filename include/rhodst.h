@@ -514,6 +514,9 @@ typedef struct rdst_adjoint_settings {
     int32_t max_cycles;     /* restart cycles */
     int32_t pad;
     double tolerance, rel_tol;  /* on the normalised residual |r| as for the primal solve */
+    double precond_shift;       /* the ILU(0) is taken of J^T (viscous: of its face-neighbour part) plus
+                                   precond_shift x the case's pseudo-time diagonal rDeltaT (the primal's
+                                   implicit operator at its current CFL for 1); 0: of J^T itself */
 } rdst_adjoint_settings;
 
 /* J = dR/dW at the current state (and J^T), kept on the device */

@@ -172,8 +172,9 @@ class AirfoilProblem:
     node, so the whole gradient costs 2 stride mesh builds."""
 
     def __init__(self, mesh_kw=None, turbulence="SAneg", seed_amplitude=1e-4, n_seed_modes=4, seed=3,
-                 fd_step=1e-7, **case_kw):
+                 fd_step=1e-7, precond_shift=0.0, **case_kw):
         from . import meshgen
+        self.precond_shift = precond_shift
         self.meshgen = meshgen
         self.mesh_kw = dict(mesh_kw or {})
         n_surf = self.mesh_kw.get("n_surf", 400)
@@ -244,7 +245,8 @@ class AirfoilProblem:
 
     def adjoint(self, **solver):
         dc = self.dc
-        opts = dict(precision="mixed", n_directions=30, max_cycles=400, tolerance=1e-12, rel_tol=0.0)
+        opts = dict(precision="mixed", n_directions=30, max_cycles=400, tolerance=1e-12, rel_tol=0.0,
+                    precond_shift=self.precond_shift)
         opts.update(solver)
         return dc.adjoint_solve(dc.spec.mesh.patch_index("airfoil"), self.drag, self.q, **opts)
 

@@ -345,6 +345,7 @@ struct Case {
     // J^T in FP32 for the mixed solve); adjJT then holds the block-CSR part of
     // J^T, whose ILU(0) preconditions the solve
     DBuf<double> adjW, adjWT;
+    DBuf<double> adjPT;  // the preconditioner's matrix with a pseudo-time shift
     DBuf<float> adjWT32;
     DBuf<double> avGrad, avR, avJv, avObj;  // scratch of the forward-mode passes
     bool adjWide = false;                   // the last linearisation is the second-neighbour one

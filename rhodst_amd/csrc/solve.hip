@@ -1418,6 +1418,15 @@ void adjoint_apply(Case& c, bool transpose, const double* x, double* y) {
     RDST_HIP(hipGetLastError());
 }
 
+// P = A + shift rDeltaT on the diagonal of every block-CSR row
+__global__ void k_shift_diag(int n, const int32_t* slotDiag, const double* A, const double* rDeltaT, double shift,
+                             double* P) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const size_t d = 25 * (size_t)slotDiag[c];
+    for (int q = 0; q < 25; q += 6) P[d + q] = A[d + q] + shift * rDeltaT[c];
+}
+
 // J^T lambda = g with the primal solver machinery on the transposed
 // matrix: block ILU(0) of J^T (natural or partitioned order as the case's
 // settings) and GMRES(m); mixed: FP32 factors and cycles, FP64 outer residual.
@@ -1439,12 +1448,26 @@ void adjoint_solve(Case& c, const double* g, double* lambda, const rdst_adjoint_
         Mesh& m = *c.mesh;
         Ctx k{c, m, *c.work, c.stream, c.work->G};
         rdst_solver_perf p{};
+        // the preconditioner's matrix: J^T's block-CSR part, shifted on request
+        const size_t nn = 25 * (size_t)m.nnzb;
+        const double* PT = c.adjJT.p;
+        if (as.precond_shift > 0.0) {
+            if (c.adjPT.n != nn) c.adjPT.alloc(nn);
+            RDST_HIP(hipMemcpyAsync(c.adjPT.p, c.adjJT.p, nn * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+            hipLaunchKernelGGL(k_shift_diag, dim3(blocks_for(m.nC, 256)), dim3(256), 0, c.stream, m.nC, m.d_slotDiag.p,
+                               c.adjJT.p, c.f.rDeltaT, as.precond_shift, c.adjPT.p);
+            PT = c.adjPT.p;
+        }
         if (as.precision == RDST_PREC_MIXED) {
-            const size_t nn = 25 * (size_t)m.nnzb;
+            // FP32 factors of the preconditioner's matrix; the inner FP32
+            // operator is J^T itself (cast after the factor when shifted)
             if (c.adjJT32.n != nn) c.adjJT32.alloc(nn);
-            hipLaunchKernelGGL((k_cast<double, float>), dim3(std::min(4096, blocks_for((long)nn, 256))), dim3(256), 0,
-                               c.stream, (long)nn, c.adjJT.p, c.adjJT32.p);
+            const int gb = std::min(4096, blocks_for((long)nn, 256));
+            hipLaunchKernelGGL((k_cast<double, float>), dim3(gb), dim3(256), 0, c.stream, (long)nn, PT, c.adjJT32.p);
             ilu_factor<float>(k, c.adjJT32.p, k.w.LU32.p, k.w.Dinv32.p);
+            if (PT != c.adjJT.p)
+                hipLaunchKernelGGL((k_cast<double, float>), dim3(gb), dim3(256), 0, c.stream, (long)nn, c.adjJT.p,
+                                   c.adjJT32.p);
             if (c.adjWide) {
                 const size_t nw = c.adjWT.n;
                 if (c.adjWT32.n != nw) c.adjWT32.alloc(nw);
@@ -1455,7 +1478,7 @@ void adjoint_solve(Case& c, const double* g, double* lambda, const rdst_adjoint_
                 p = gmres_mixed<double>(k, c.adjJT.p, c.adjJT32.p, lambda, g);
             }
         } else {
-            ilu_factor<double>(k, c.adjJT.p, k.w.LU.p, k.w.Dinv.p);
+            ilu_factor<double>(k, PT, k.w.LU.p, k.w.Dinv.p);
             p = gmres_fp64(k, c.adjWide ? c.adjWT.p : c.adjJT.p, lambda, g);
         }
         RDST_HIP(hipStreamSynchronize(c.stream));
