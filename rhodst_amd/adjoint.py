@@ -172,7 +172,7 @@ class AirfoilProblem:
     node, so the whole gradient costs 2 stride mesh builds."""
 
     def __init__(self, mesh_kw=None, turbulence="SAneg", seed_amplitude=1e-4, n_seed_modes=4, seed=3,
-                 fd_step=1e-7, precond_shift=0.0, **case_kw):
+                 fd_step=1e-7, precond_shift=1.0, **case_kw):
         from . import meshgen
         self.precond_shift = precond_shift
         self.meshgen = meshgen

@@ -143,6 +143,10 @@ __global__ void k_copy(long n, TY* y, const TX* x) {
 __global__ void k_sub(long n, double* r, const double* b, const double* a) {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) r[i] = b[i] - a[i];
 }
+template <class T>
+__global__ void k_acc(long n, T* y, const T* a) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] += a[i];
+}
 __global__ void k_add(long n, double* y, const double* a) {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] += a[i];
 }
@@ -934,6 +938,7 @@ struct Ctx {
     SolverWork& w;
     hipStream_t s;
     int G;
+    bool right = false;  // GMRES preconditioned from the right (minimises the true residual)
 };
 
 // y = A x: the sliced-ELL copy of A when one was built for this step
@@ -1155,15 +1160,21 @@ void gmres_cycle(Ctx& k, const TM* A, const TM* LU, const TM* Dinv, TV* z, const
     double* sc = k.w.scal.p;
     double* H = sc + S_H;
     double* yh = H + m * m + 3 * m + 1;
-    // wA = M^-1 rA; beta = |wA|
-    precondition(k, LU, Dinv, rres, wv, ytmp);
+    // wA = M^-1 rA (right: rA); beta = |wA|
+    if (k.right) hipLaunchKernelGGL((k_copy<TV, TV>), dim3(G), dim3(RED_BS), 0, k.s, n5, wv, rres);
+    else precondition(k, LU, Dinv, rres, wv, ytmp);
     dot(k, n5, wv, wv, sc + S_BETA, 1);
     hipLaunchKernelGGL(k_gmres_start, dim3(1), dim3(1), 0, k.s, sc, m);
     for (int i = 0; i < m; ++i) {
         TV* Vi = V + (size_t)i * n5;
         hipLaunchKernelGGL((k_scale_div<TV>), dim3(G), dim3(RED_BS), 0, k.s, n5, Vi, wv, sc + S_BETA);
-        spmv(k, A, Vi, rv);
-        precondition(k, LU, Dinv, rv, wv, ytmp);
+        if (k.right) {
+            precondition(k, LU, Dinv, Vi, rv, ytmp);
+            spmv(k, A, rv, wv);
+        } else {
+            spmv(k, A, Vi, rv);
+            precondition(k, LU, Dinv, rv, wv, ytmp);
+        }
         double* pin = k.w.partA.p;
         double* pout = k.w.partB.p;
         if (k.c.st.gram_schmidt == RDST_GS_CLASSICAL) {
@@ -1189,7 +1200,15 @@ void gmres_cycle(Ctx& k, const TM* A, const TM* LU, const TM* Dinv, TV* z, const
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(RED_BS), 0, k.s, pin, G, sc + S_BETA, 1);
     }
     hipLaunchKernelGGL(k_gmres_backsub, dim3(1), dim3(1), 0, k.s, sc, m);
-    hipLaunchKernelGGL((k_update_x<TV, TV>), dim3(G), dim3(RED_BS), 0, k.s, n5, z, V, yh, m);
+    if (k.right) {
+        // z += M^-1 (V y)
+        RDST_HIP(hipMemsetAsync(wv, 0, sizeof(TV) * n5, k.s));
+        hipLaunchKernelGGL((k_update_x<TV, TV>), dim3(G), dim3(RED_BS), 0, k.s, n5, wv, V, yh, m);
+        precondition(k, LU, Dinv, wv, rv, ytmp);
+        hipLaunchKernelGGL((k_acc<TV>), dim3(G), dim3(RED_BS), 0, k.s, n5, z, rv);
+    } else {
+        hipLaunchKernelGGL((k_update_x<TV, TV>), dim3(G), dim3(RED_BS), 0, k.s, n5, z, V, yh, m);
+    }
 }
 
 rdst_solver_perf gmres_fp64(Ctx& k, const double* A, double* x, const double* b) {
@@ -1447,6 +1466,9 @@ void adjoint_solve(Case& c, const double* g, double* lambda, const rdst_adjoint_
         ensure_work(c);
         Mesh& m = *c.mesh;
         Ctx k{c, m, *c.work, c.stream, c.work->G};
+        // the viscous operator differs from the matrix the ILU factors: GMRES
+        // then minimises the true residual (right preconditioning)
+        k.right = c.adjWide;
         rdst_solver_perf p{};
         // the preconditioner's matrix: J^T's block-CSR part, shifted on request
         const size_t nn = 25 * (size_t)m.nnzb;
