@@ -326,6 +326,56 @@ __global__ __launch_bounds__(RED_BS) void k_multidot_partial(long n, const T* w,
         if (threadIdx.x == 0) part[(size_t)(j0 + q) * gridDim.x + blockIdx.x] = t;
     }
 }
+// the same for an FP32 basis whose length is a multiple of four: each thread
+// reads four consecutive entries per 16-byte load (twice the bytes in flight
+// of the scalar form); per thread the entries are summed in grid-stride order
+// of their float4 groups
+template <int NACC>
+__global__ __launch_bounds__(RED_BS) void k_multidot_partial4(long n, const float* w, const float* V, int j0, int cnt,
+                                                              double* part) {
+    __shared__ double sh[16];
+    double acc[NACC];
+    for (int q = 0; q < NACC; ++q) acc[q] = 0.0;
+    const long n4 = n / 4;
+    const long st = (long)gridDim.x * blockDim.x;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const float4* w4 = reinterpret_cast<const float4*>(w);
+    const float4* Vb = reinterpret_cast<const float4*>(V + (size_t)j0 * n);
+    for (; i + st < n4; i += 2 * st) {
+        float4 wi[2], v[2][NACC];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) wi[u] = w4[i + u * st];
+#pragma unroll
+        for (int q = 0; q < NACC; ++q)
+            if (q < cnt)
+#pragma unroll
+                for (int u = 0; u < 2; ++u) v[u][q] = Vb[(size_t)q * n4 + i + u * st];
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int q = 0; q < NACC; ++q)
+                if (q < cnt)
+                    acc[q] += (double)wi[u].x * (double)v[u][q].x + (double)wi[u].y * (double)v[u][q].y +
+                              (double)wi[u].z * (double)v[u][q].z + (double)wi[u].w * (double)v[u][q].w;
+    }
+    for (; i < n4; i += st) {
+        const float4 wi = w4[i];
+#pragma unroll
+        for (int q = 0; q < NACC; ++q)
+            if (q < cnt) {
+                const float4 v = Vb[(size_t)q * n4 + i];
+                acc[q] += (double)wi.x * (double)v.x + (double)wi.y * (double)v.y + (double)wi.z * (double)v.z +
+                          (double)wi.w * (double)v.w;
+            }
+    }
+#pragma unroll
+    for (int q = 0; q < NACC; ++q) {
+        if (q >= cnt) break;
+        const double t = block_sum(acc[q], sh);
+        if (threadIdx.x == 0) part[(size_t)(j0 + q) * gridDim.x + blockIdx.x] = t;
+    }
+}
+
 // H[j][col] = sum of the G partials of dot j; block j reduces dot j
 __global__ __launch_bounds__(RED_BS) void k_multidot_final(const double* part, int G, int cnt, double* H, int m, int col) {
     __shared__ double sh[16];
@@ -1179,9 +1229,17 @@ void gmres_cycle(Ctx& k, const TM* A, const TM* LU, const TM* Dinv, TV* z, const
         double* pout = k.w.partB.p;
         if (k.c.st.gram_schmidt == RDST_GS_CLASSICAL) {
             // all projections from one multi-dot, then one fused update (+ |w|^2 partials)
-            for (int j0 = 0; j0 <= i; j0 += 8)
+            for (int j0 = 0; j0 <= i; j0 += 8) {
+                if constexpr (sizeof(TV) == 4) {
+                    if (n5 % 4 == 0) {
+                        hipLaunchKernelGGL((k_multidot_partial4<8>), dim3(G), dim3(RED_BS), 0, k.s, n5, (const float*)wv,
+                                           (const float*)V, j0, std::min(8, i + 1 - j0), k.w.multi.p);
+                        continue;
+                    }
+                }
                 hipLaunchKernelGGL((k_multidot_partial<TV, 8>), dim3(G), dim3(RED_BS), 0, k.s, n5, wv, V, j0,
                                    std::min(8, i + 1 - j0), k.w.multi.p);
+            }
             hipLaunchKernelGGL(k_multidot_final, dim3(i + 1), dim3(RED_BS), 0, k.s, k.w.multi.p, G, i + 1, H, m, i);
             hipLaunchKernelGGL((k_cgs_update<TV>), dim3(G), dim3(RED_BS), 0, k.s, n5, wv, V, H, m, i, i + 1, pin);
         } else {
