@@ -427,6 +427,63 @@ __global__ __launch_bounds__(RED_BS) void k_cgs_update(long n, T* w, const T* V,
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
+// the same for an FP32 basis whose length is a multiple of four: 16-byte
+// loads, two float4 groups per thread and two basis vectors per pass; each
+// entry is updated in the same order (j ascending) as above, so w is
+// unchanged; the |w|^2 partials are summed per float4 group
+__global__ __launch_bounds__(RED_BS) void k_cgs_update4(long n, float* w, const float* V, const double* H, int m, int col,
+                                                        int cnt, double* part) {
+    constexpr int HC = RED_BS;
+    __shared__ double sh[16];
+    __shared__ double hc[HC];
+    for (int t = threadIdx.x; t < cnt && t < HC; t += blockDim.x) hc[t] = H[t * m + col];
+    __syncthreads();
+    double s = 0.0;
+    const long n4 = n / 4;
+    const long st = (long)gridDim.x * blockDim.x;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    float4* w4 = reinterpret_cast<float4*>(w);
+    const float4* V4 = reinterpret_cast<const float4*>(V);
+    auto upd = [](double* a, double h, const float4& v) {
+        a[0] -= h * (double)v.x; a[1] -= h * (double)v.y; a[2] -= h * (double)v.z; a[3] -= h * (double)v.w;
+    };
+    for (; i < n4; i += 2 * st) {
+        const bool two = i + st < n4;
+        double a[2][4];
+        {
+            const float4 w0 = w4[i];
+            a[0][0] = w0.x; a[0][1] = w0.y; a[0][2] = w0.z; a[0][3] = w0.w;
+            const float4 w1 = two ? w4[i + st] : make_float4(0.f, 0.f, 0.f, 0.f);
+            a[1][0] = w1.x; a[1][1] = w1.y; a[1][2] = w1.z; a[1][3] = w1.w;
+        }
+        int j = 0;
+        for (; j + 1 < cnt; j += 2) {
+            const double h0 = j < HC ? hc[j] : H[j * m + col];
+            const double h1 = j + 1 < HC ? hc[j + 1] : H[(j + 1) * m + col];
+            const float4* Vj = V4 + (size_t)j * n4;
+            const float4 v00 = Vj[i], v10 = Vj[n4 + i];
+            const float4 v01 = two ? Vj[i + st] : make_float4(0.f, 0.f, 0.f, 0.f);
+            const float4 v11 = two ? Vj[n4 + i + st] : make_float4(0.f, 0.f, 0.f, 0.f);
+            upd(a[0], h0, v00); upd(a[0], h1, v10);
+            upd(a[1], h0, v01); upd(a[1], h1, v11);
+        }
+        if (j < cnt) {
+            const double h0 = j < HC ? hc[j] : H[j * m + col];
+            const float4* Vj = V4 + (size_t)j * n4;
+            upd(a[0], h0, Vj[i]);
+            if (two) upd(a[1], h0, Vj[i + st]);
+        }
+        for (int u = 0; u < (two ? 2 : 1); ++u) {
+            const float4 o = make_float4((float)a[u][0], (float)a[u][1], (float)a[u][2], (float)a[u][3]);
+            w4[i + u * st] = o;
+            s += (double)o.x * (double)o.x + (double)o.y * (double)o.y + (double)o.z * (double)o.z +
+                 (double)o.w * (double)o.w;
+        }
+    }
+    const double t = block_sum(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
 // ---------------------------------------------------------------- block ILU(0)
 template <class T>
 __device__ inline void mm5(const T* A, const T* B, T* C) {
@@ -1241,7 +1298,16 @@ void gmres_cycle(Ctx& k, const TM* A, const TM* LU, const TM* Dinv, TV* z, const
                                    std::min(8, i + 1 - j0), k.w.multi.p);
             }
             hipLaunchKernelGGL(k_multidot_final, dim3(i + 1), dim3(RED_BS), 0, k.s, k.w.multi.p, G, i + 1, H, m, i);
-            hipLaunchKernelGGL((k_cgs_update<TV>), dim3(G), dim3(RED_BS), 0, k.s, n5, wv, V, H, m, i, i + 1, pin);
+            bool done = false;
+            if constexpr (sizeof(TV) == 4) {
+                if (n5 % 4 == 0) {
+                    hipLaunchKernelGGL(k_cgs_update4, dim3(G), dim3(RED_BS), 0, k.s, n5, (float*)wv, (const float*)V, H,
+                                       m, i, i + 1, pin);
+                    done = true;
+                }
+            }
+            if (!done)
+                hipLaunchKernelGGL((k_cgs_update<TV>), dim3(G), dim3(RED_BS), 0, k.s, n5, wv, V, H, m, i, i + 1, pin);
         } else {
             // partial dot(w, V0), then the fused MGS steps
             hipLaunchKernelGGL((k_dot_partial<TV, TV>), dim3(G), dim3(RED_BS), 0, k.s, n5, wv, V, pin);
