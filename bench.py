@@ -38,8 +38,16 @@ array resident in HBM.  The line reports:
   cpu_baseline    the CPU FP64 oracle's face flux + residual on the same
                   mesh and state, all host threads (OpenMP) and one.
 
+Other modes, one JSON line each: --configs (the forward step to 1e-8 and the
+hybrid airfoil per step, GPU and CPU oracle), --adjoint (the discrete adjoint
+on the viscous hybrid airfoil: exact viscous J, the adjoint solve, J v against
+central differences, the 400-wall-node gradient), --converge-only (the Newton
+time to 1e-8), --flux-only (the face-flux kernel and a copy of known size for
+the counter runs).
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--small]
        [--turbulence SARC2|SAneg|none] [--skip-converge] [--no-cpu]
+       [--configs | --adjoint | --converge-only | --flux-only]
 """
 from __future__ import annotations
 
