@@ -512,7 +512,10 @@ typedef struct rdst_adjoint_settings {
                                inside an FP64 defect correction on the exact FP64 J^T */
     int32_t n_directions;   /* GMRES restart length */
     int32_t max_cycles;     /* restart cycles */
-    int32_t pad;
+    int32_t inner_operator; /* mixed precision: 0 the inner FP32 cycles solve with J^T itself; 1 with the
+                               preconditioner's matrix (J^T's face-neighbour part plus the precond_shift
+                               diagonal): a defect correction by that pseudo-time operator, each outer
+                               FP64 residual against the exact J^T */
     double tolerance, rel_tol;  /* on the normalised residual |r| as for the primal solve */
     double precond_shift;       /* the ILU(0) is taken of J^T (viscous: of its face-neighbour part) plus
                                    precond_shift x the case's pseudo-time diagonal rDeltaT (the primal's

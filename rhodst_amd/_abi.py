@@ -126,13 +126,15 @@ def make_scalar_bcs(mesh, spec: dict):
 
 
 class rdst_adjoint_settings(C.Structure):
-    _fields_ = [("precision", C.c_int32), ("n_directions", C.c_int32), ("max_cycles", C.c_int32), ("pad", C.c_int32),
+    _fields_ = [("precision", C.c_int32), ("n_directions", C.c_int32), ("max_cycles", C.c_int32),
+                ("inner_operator", C.c_int32),
                 ("tolerance", C.c_double), ("rel_tol", C.c_double), ("precond_shift", C.c_double)]
 
 
 def make_adjoint_settings(precision="fp64", n_directions=30, max_cycles=200, tolerance=1e-12, rel_tol=0.0,
-                          precond_shift=0.0):
+                          precond_shift=0.0, inner_operator=0):
     s = rdst_adjoint_settings()
+    s.inner_operator = int(inner_operator)
     s.precision = PRECISION[precision] if isinstance(precision, str) else int(precision)
     s.n_directions, s.max_cycles = int(n_directions), int(max_cycles)
     s.tolerance, s.rel_tol = float(tolerance), float(rel_tol)

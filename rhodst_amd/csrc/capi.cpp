@@ -814,6 +814,8 @@ int rdst_adjoint_solve(rdst_case* cc, int patch, const double* dir, double q, co
         if (s->precision != RDST_PREC_FP64 && s->precision != RDST_PREC_MIXED)
             throw ArgError("rdst_adjoint_solve: unknown precision", RDST_ERR_UNKNOWN_NAME);
         if (s->n_directions < 1 || s->max_cycles < 1) throw ArgError("rdst_adjoint_solve: n_directions, max_cycles >= 1");
+        if (s->inner_operator != 0 && s->inner_operator != 1)
+            throw ArgError("rdst_adjoint_solve: inner_operator is 0 or 1", RDST_ERR_UNKNOWN_NAME);
         if (!(s->precond_shift >= 0.0) || !std::isfinite(s->precond_shift))
             throw ArgError("rdst_adjoint_solve: precond_shift must be finite and >= 0");
         adjoint_linearise(*c);  // at the current state

@@ -1606,22 +1606,27 @@ void adjoint_solve(Case& c, const double* g, double* lambda, const rdst_adjoint_
         }
         if (as.precision == RDST_PREC_MIXED) {
             // FP32 factors of the preconditioner's matrix; the inner FP32
-            // operator is J^T itself (cast after the factor when shifted)
+            // operator is J^T itself (cast after the factor when shifted) or,
+            // with inner_operator 1, that matrix (a defect correction by the
+            // pseudo-time operator)
             if (c.adjJT32.n != nn) c.adjJT32.alloc(nn);
             const int gb = std::min(4096, blocks_for((long)nn, 256));
             hipLaunchKernelGGL((k_cast<double, float>), dim3(gb), dim3(256), 0, c.stream, (long)nn, PT, c.adjJT32.p);
             ilu_factor<float>(k, c.adjJT32.p, k.w.LU32.p, k.w.Dinv32.p);
-            if (PT != c.adjJT.p)
-                hipLaunchKernelGGL((k_cast<double, float>), dim3(gb), dim3(256), 0, c.stream, (long)nn, c.adjJT.p,
-                                   c.adjJT32.p);
-            if (c.adjWide) {
+            const double* outer = c.adjWide ? c.adjWT.p : c.adjJT.p;
+            if (as.inner_operator == 1) {
+                p = gmres_mixed<double>(k, outer, c.adjJT32.p, lambda, g);
+            } else if (c.adjWide) {
                 const size_t nw = c.adjWT.n;
                 if (c.adjWT32.n != nw) c.adjWT32.alloc(nw);
                 hipLaunchKernelGGL((k_cast<double, float>), dim3(std::min(4096, blocks_for((long)nw, 256))), dim3(256),
                                    0, c.stream, (long)nw, c.adjWT.p, c.adjWT32.p);
-                p = gmres_mixed<double>(k, c.adjWT.p, c.adjWT32.p, lambda, g);
+                p = gmres_mixed<double>(k, outer, c.adjWT32.p, lambda, g);
             } else {
-                p = gmres_mixed<double>(k, c.adjJT.p, c.adjJT32.p, lambda, g);
+                if (PT != c.adjJT.p)
+                    hipLaunchKernelGGL((k_cast<double, float>), dim3(gb), dim3(256), 0, c.stream, (long)nn, c.adjJT.p,
+                                       c.adjJT32.p);
+                p = gmres_mixed<double>(k, outer, c.adjJT32.p, lambda, g);
             }
         } else {
             ilu_factor<double>(k, PT, k.w.LU.p, k.w.Dinv.p);

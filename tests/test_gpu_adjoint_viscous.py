@@ -283,3 +283,22 @@ def test_airfoil_full_size_jacobian_against_oracle(oracle):
     err = np.abs(Jv - fd).max(axis=0) / np.abs(fd).max(axis=0)
     print("relative error per component", err, flush=True)
     assert np.all(err < 1e-6), err
+
+
+def test_defect_correction_solve():
+    """inner_operator 1: FP64 residuals against the exact viscous J^T, FP32
+    inner cycles on the face-neighbour part of J^T shifted by 1/dtau; the
+    solution satisfies the exact system."""
+    print("[viscous adjoint defect correction]", flush=True)
+    spec, dc = _box("HLLC", steps=60)
+    patch = spec.mesh.patch_index("bottom")
+    d = np.array([1.0, 0.0, 0.0])
+    lam, perf = dc.adjoint_solve(patch, d, 0.5, precision="mixed", n_directions=30, max_cycles=2000,
+                                 tolerance=0.0, rel_tol=1e-12, precond_shift=0.3, inner_operator=1)
+    g = dc.adjoint_objective_gradient(patch, d, 0.5)
+    r = dc.adjoint_apply(lam, transpose=True) - g
+    print("cycles", perf.n_iterations, "|r|/|g|", np.linalg.norm(r) / np.linalg.norm(g), flush=True)
+    assert np.linalg.norm(r) <= 1e-9 * np.linalg.norm(g)
+    lam64, _ = dc.adjoint_solve(patch, d, 0.5, precision="fp64", n_directions=30, max_cycles=200,
+                                tolerance=1e-13, rel_tol=0.0)
+    assert np.abs(lam - lam64).max() <= 1e-8 * np.abs(lam64).max()
