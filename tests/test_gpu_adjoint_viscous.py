@@ -302,3 +302,36 @@ def test_defect_correction_solve():
     lam64, _ = dc.adjoint_solve(patch, d, 0.5, precision="fp64", n_directions=30, max_cycles=200,
                                 tolerance=1e-13, rel_tol=0.0)
     assert np.abs(lam - lam64).max() <= 1e-8 * np.abs(lam64).max()
+
+
+@pytest.mark.parametrize("viscous", [False, True])
+def test_objective_cells_sum_to_force(viscous):
+    """rdst_adjoint_objective_cells: the cells' shares sum to the force
+    coefficient of rdst_forces (pressure, and the viscous part when viscous)
+    at a state whose boundary values and boundary gradient are consistent."""
+    print(f"[objective cells] viscous={viscous}", flush=True)
+    if viscous:
+        spec, dc = _box("HLLC")
+        patch = spec.mesh.patch_index("bottom")
+    else:
+        spec = cases.bump_case()
+        dc = DeviceCase(spec)
+        dc.iterate(50)
+        dc.update_fields()
+        dc.update_fields()
+        patch = spec.mesh.patch_index("bottom")
+    dc.flux_residual()
+    phi = dc.get(12)
+    dc.update_fields()
+    dc.update_fields()
+    dc.flux_residual()
+    dc.put(12, phi)
+    d = np.array([1.0, 0.0, 0.0])
+    q, p_ref = 0.5, 1e5
+    o = dc.adjoint_objective_cells(patch, d, q, p_ref)
+    F = dc.forces(patch, d, (0, 1, 0), p_ref, 1.0, 1.0, 1.0)[2:5]
+    c = float(F @ d) / q
+    print("sum of shares", o.sum(), "forces", c, flush=True)
+    assert abs(o.sum() - c) <= 1e-11 * max(abs(c), 1e-30)
+    # only the cells on the patch hold a share
+    assert 0 < np.count_nonzero(o) <= spec.mesh.patches[patch][3]

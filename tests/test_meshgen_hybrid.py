@@ -66,3 +66,21 @@ def test_airfoil_wall_displacement(oracle):
     assert np.array_equal(meshgen.hybrid_airfoil(**kw, wall_disp=d1).points, m1.points)
     g = oracle.geometry(m1)
     assert (g["V"] > 0).all()
+
+
+def test_airfoil_problem_design_nodes():
+    """The wall-node design set of the adjoint's airfoil problem: the loop's
+    surface points without the lower trailing-edge corner (n_surf of them),
+    and seeded Hicks-Henne bumps of at most ~1e-4 chord that vanish at the
+    leading and trailing edges."""
+    from rhodst_amd.adjoint import AirfoilProblem
+    p = AirfoilProblem(mesh_kw=dict(n_surf=80))
+    loop, kind = p.loop, p.kind
+    assert len(p.design) == 80
+    assert np.all(kind[p.design] < 2)
+    lower_te = np.nonzero(kind == 0)[0][np.argmax(loop[kind == 0, 0])]
+    assert lower_te not in set(p.design.tolist())
+    b = p.base_disp
+    assert np.all(b[kind == 2] == 0.0)
+    assert 0.2e-4 < np.abs(b).max() < 2e-4
+    assert abs(b[np.argmin(loop[:, 0])]) < 1e-12   # the leading edge
