@@ -38,7 +38,7 @@ array resident in HBM.  The line reports:
   cpu_baseline    the CPU FP64 oracle's face flux + residual on the same
                   mesh and state, all host threads (OpenMP) and one.
 
-Other modes, one JSON line each: --configs (the forward step to 1e-8 and the
+Other modes, one JSON line each: --workloads (the forward step to 1e-8 and the
 hybrid airfoil per step, GPU and CPU oracle), --adjoint (the discrete adjoint
 on the viscous hybrid airfoil: exact viscous J, the adjoint solve, J v against
 central differences, the 400-wall-node gradient), --converge-only (the Newton
@@ -47,7 +47,7 @@ the counter runs).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--small]
        [--turbulence SARC2|SAneg|none] [--skip-converge] [--no-cpu]
-       [--configs | --adjoint | --converge-only | --flux-only]
+       [--workloads | --adjoint | --converge-only | --flux-only]
 """
 from __future__ import annotations
 
@@ -190,7 +190,7 @@ def converge_path(name, spec, mesh, max_steps, dmesh=None, chunk=20):
     return out
 
 
-def configs_mode(args) -> int:
+def workloads_mode(args) -> int:
     """The forward step (120 x 40, AUSM+, CFL 3, GMRES(20) + ILU(0)) run to a
     1e-8 drop, and the hybrid airfoil (HLLC + SAneg) per step, on the GPU
     (plain FP64 path and mixed precision) and on the CPU oracle."""
@@ -227,16 +227,16 @@ def configs_mode(args) -> int:
         log(f"hybrid airfoil {name}: {out['hybrid_airfoil'][name]}")
     cpu = {}
     all_threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-    for cfg, budget in ((1, 200.0), (2, 25.0)):
+    for wl, budget in (("forward_step", 200.0), ("hybrid_airfoil", 25.0)):
         for threads in (all_threads, 1):
             env = dict(os.environ, OMP_NUM_THREADS=str(threads), ORACLE_OPENMP="1")
-            log(f"cpu oracle config {cfg}, {threads} thread(s)")
-            r = subprocess.run([sys.executable, "-m", "oracle.cpu_configs", "--config", str(cfg), "--seconds",
+            log(f"cpu oracle {wl}, {threads} thread(s)")
+            r = subprocess.run([sys.executable, "-m", "oracle.cpu_workloads", "--workload", wl, "--seconds",
                                 str(budget)], cwd=ROOT, env=env, stdout=subprocess.PIPE, text=True, timeout=900)
             if r.returncode != 0:
-                log(f"cpu config {cfg} failed (exit {r.returncode})")
+                log(f"cpu {wl} failed (exit {r.returncode})")
                 continue
-            cpu[f"config{cfg}_{threads}t"] = json.loads(r.stdout.strip().splitlines()[-1])
+            cpu[f"{wl}_{threads}t"] = json.loads(r.stdout.strip().splitlines()[-1])
     out["cpu_oracle"] = cpu
     print(json.dumps(out), flush=True)
     return 0
@@ -372,7 +372,7 @@ def main(argv=None) -> int:
     ap.add_argument("--converge-steps", type=int, default=2500, help="step limit of the Newton time-to-1e-8 runs")
     ap.add_argument("--paths", default="mixed,fp64", help="time-to-1e-8 paths, in order")
     ap.add_argument("--converge-only", action="store_true", help="only the Newton time-to-1e-8 runs (one JSON line)")
-    ap.add_argument("--configs", action="store_true",
+    ap.add_argument("--workloads", action="store_true",
                     help="only the two smaller workloads (forward step to 1e-8, hybrid airfoil per step) on the GPU "
                          "and the CPU oracle (all threads and one): one JSON line")
     ap.add_argument("--adjoint", action="store_true",
@@ -389,8 +389,8 @@ def main(argv=None) -> int:
               "multi-GPU runs are not part of this build", file=sys.stderr)
         return 2
 
-    if args.configs:
-        return configs_mode(args)
+    if args.workloads:
+        return workloads_mode(args)
     if args.adjoint:
         return adjoint_mode(args)
     from rhodst_amd import cases, meshgen

@@ -7,7 +7,7 @@ TEST INFRASTRUCTURE ONLY (the oracle is the checker; here it is the CPU
 reference the GPU numbers are reported beside).  Run as a child process of
 bench.py so that the thread count is fixed before the OpenMP runtime starts:
 
-    OMP_NUM_THREADS=16 ORACLE_OPENMP=1 python -m oracle.cpu_configs --config 1 --seconds 240
+    OMP_NUM_THREADS=16 ORACLE_OPENMP=1 python -m oracle.cpu_workloads --workload forward_step --seconds 240
 
 Prints one JSON line.
 """
@@ -28,14 +28,14 @@ if ROOT not in sys.path:
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--config", type=int, choices=(1, 2), required=True)
+    ap.add_argument("--workload", choices=("forward_step", "hybrid_airfoil"), required=True)
     ap.add_argument("--seconds", type=float, default=120.0, help="time budget of the iteration")
     ap.add_argument("--max-steps", type=int, default=20000)
     args = ap.parse_args(argv)
     from rhodst_amd import cases, meshgen
 
     from oracle import binding
-    if args.config == 1:
+    if args.workload == "forward_step":
         spec = cases.forward_step_case(120, 40, initial_co=3.0, min_co=3.0, max_co=3.0)
         chunk = 100
     else:
@@ -53,7 +53,7 @@ def main(argv=None) -> int:
         hist.extend(r.tolist())
         h = np.array(hist) / hist[0]
         if spent >= next_log:
-            print(f"[cpu config {args.config}] step {len(hist)} drop {h[-1]:.3e} {spent:.0f}s", file=sys.stderr, flush=True)
+            print(f"[cpu {args.workload}] step {len(hist)} drop {h[-1]:.3e} {spent:.0f}s", file=sys.stderr, flush=True)
             next_log += 20.0
         if not np.isfinite(h[-1]):
             break
@@ -61,7 +61,7 @@ def main(argv=None) -> int:
             reached, at = True, int(np.argmax(h <= 1e-8)) + 1
             break
     h = np.array(hist) / hist[0]
-    print(json.dumps({"config": args.config, "cells": spec.mesh.n_cells, "steps": len(hist), "seconds": spent,
+    print(json.dumps({"workload": args.workload, "cells": spec.mesh.n_cells, "steps": len(hist), "seconds": spent,
                       "ms_per_step": spent / len(hist) * 1e3, "reached_1e-8": reached, "step_1e-8": at,
                       "final_drop": float(h[-1]), "threads": int(os.environ.get("OMP_NUM_THREADS", "1")),
                       "openmp": binding._OMP}), flush=True)
