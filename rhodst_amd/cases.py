@@ -283,6 +283,52 @@ def box_sa_case(model: str = "SAneg", scheme: str = "HLLC", nx: int = 24, ny: in
     return spec
 
 
+def tet_prism_case(model: str | None = "SAneg", scheme: str = "Kurganov", mesh=None, mach: float = 0.3,
+                   perturb: float = 0.05, seed: int = 2, **settings) -> CaseSpec:
+    """Three-dimensional flow over a no-slip wall on tetrahedra above prism
+    layers (meshgen.tet_prism_box, 600 cells by default): the wall at z = 0 (U
+    fixedValue 0, p and T zeroGradient), the characteristic far field on the
+    other five sides with the free stream along x, Sutherland air, a seeded
+    perturbed state, viscous terms on; model "SAneg"/"SARC2"/... adds the
+    Spalart-Allmaras model (nuTilda fixedValue 0 on the wall, inletOutlet 3 nu
+    elsewhere, the exact wall distance, a seeded nuTilda of about 3 nu),
+    None: laminar."""
+    mesh = mesh if mesh is not None else meshgen.tet_prism_box(jitter=0.15)
+    gamma, R = 1.4, 287.0
+    Cp = gamma * R / (gamma - 1)
+    Tinf, pinf = 300.0, 1.0e5
+    c = np.sqrt(gamma * R * Tinf)
+    Uinf = (mach * c, 0.0, 0.0)
+    ff = {"p": ("farField", pinf), "T": ("farField", Tinf), "U": ("farField", Uinf),
+          "inf": {"p": pinf, "T": Tinf, "U": Uinf}}
+    wall = {"p": "zeroGradient", "T": "zeroGradient", "U": ("fixedValue", (0.0, 0.0, 0.0))}
+    spec = {"bottom": wall, **{k: ff for k in ("top", "left", "right", "front", "back")}}
+    n = mesh.n_cells
+    rng = np.random.default_rng(seed)
+    p0 = pinf * (1 + perturb * rng.uniform(-1, 1, n))
+    T0 = Tinf * (1 + perturb * rng.uniform(-1, 1, n))
+    U0 = np.tile(np.array(Uinf), (n, 1)) * (1 + perturb * rng.uniform(-1, 1, (n, 1)))
+    U0[:, 1] = 0.1 * Uinf[0] * rng.uniform(-1, 1, n)
+    U0[:, 2] = 0.05 * Uinf[0] * rng.uniform(-1, 1, n)
+    kw = dict(flux_scheme=scheme, neighbouring=1, initial_co=2.0, min_co=0.5, max_co=20.0, viscous=1,
+              solver="GMRES", precond="ILUC0", n_directions=10, min_iter=1, max_iter=2,
+              tolerance=1e-15, rel_tol=1e-3, p_bounds=(1.0, 1e7), T_bounds=(1.0, 1e4), rho_bounds=(1e-4, 1e3))
+    kw.update(settings)
+    out = CaseSpec("tetPrism", mesh, make_thermo(Cp=Cp, R=R, Pr=0.72, sutherland=(1.458e-6, 110.4)),
+                   make_bcs(mesh, spec), make_settings(**kw), p0, T0, np.ascontiguousarray(U0))
+    if model:
+        from ._abi import make_scalar_bcs, make_turbulence
+        mu = 1.458e-6 * np.sqrt(Tinf) / (1.0 + 110.4 / Tinf)
+        nu = mu / (pinf / (R * Tinf))
+        nut0 = 3.0 * nu * (1.0 + 0.5 * rng.uniform(-1, 1, n))
+        sff = ("inletOutlet", 3.0 * nu)
+        sbc = make_scalar_bcs(mesh, {"bottom": ("fixedValue", 0.0),
+                                     **{k: sff for k in ("top", "left", "right", "front", "back")}})
+        out.turbulence = (make_turbulence(model), sbc, meshgen.wall_distance(mesh, ["bottom"]), nut0)
+        out.name = f"tetPrism_{model}"
+    return out
+
+
 def bump_case(amplitudes=(0.02, 0.01, -0.005, 0.004), nx: int = 48, ny: int = 24, mach: float = 0.5,
               scheme: str = "HLLC", node_disp=None, viscous: bool = False, reynolds: float = 5000.0,
               **settings) -> CaseSpec:

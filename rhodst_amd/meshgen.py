@@ -865,9 +865,14 @@ def cells_polymesh(points: np.ndarray, cells: list, classify, patch_defs) -> Pol
         for li, f in enumerate(faces):
             fc.append(ci); fl.append(li)
             fp.append(list(f) + [-1] * (4 - len(f)))
-    fc = np.array(fc, dtype=np.int64)
-    fl = np.array(fl, dtype=np.int64)
-    fp = np.array(fp, dtype=np.int64)
+    return faces_polymesh(points, np.array(fc, dtype=np.int64), np.array(fl, dtype=np.int64),
+                          np.array(fp, dtype=np.int64), len(cells), classify, patch_defs)
+
+
+def faces_polymesh(points: np.ndarray, fc: np.ndarray, fl: np.ndarray, fp: np.ndarray, n_cells: int, classify,
+                   patch_defs) -> PolyMesh:
+    """cells_polymesh on arrays: face k belongs to cell fc[k] as its local
+    face fl[k] with the outward-oriented points fp[k] (-1 padded to 4)."""
     key = np.sort(np.where(fp < 0, np.iinfo(np.int64).max, fp), axis=1)
     order = np.lexsort((key[:, 3], key[:, 2], key[:, 1], key[:, 0]))
     ks = key[order]
@@ -885,7 +890,12 @@ def cells_polymesh(points: np.ndarray, cells: list, classify, patch_defs) -> Pol
     paired[b] = True
     bs = np.nonzero(~paired)[0]
     npts = (fp >= 0).sum(axis=1)
-    cen = np.array([points[fp[s, :npts[s]]].mean(axis=0) for s in bs]) if len(bs) else np.zeros((0, 3))
+    if len(bs):
+        fpb = fp[bs]
+        pts = points[np.where(fpb < 0, fpb[:, :1], fpb)]          # padding repeats a real point
+        cen = (pts * (fpb >= 0)[:, :, None]).sum(axis=1) / npts[bs][:, None]
+    else:
+        cen = np.zeros((0, 3))
     bpatch = np.asarray(classify(fc[bs], fl[bs], cen), dtype=np.int64)
     if np.any(bpatch < 0):
         raise ValueError("unclassified boundary faces")
@@ -894,7 +904,8 @@ def cells_polymesh(points: np.ndarray, cells: list, classify, patch_defs) -> Pol
     slots = np.concatenate([own_slot, bs])
     sizes = npts[slots]
     offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
-    fpts = np.concatenate([fp[s, :npts[s]] for s in slots]).astype(np.int32)
+    fps = fp[slots]
+    fpts = fps[fps >= 0].astype(np.int32)                            # row-major: each face's points in order
     owner = np.concatenate([own_cell, fc[bs]]).astype(np.int32)
     patches = []
     start = len(own_cell)
@@ -903,7 +914,107 @@ def cells_polymesh(points: np.ndarray, cells: list, classify, patch_defs) -> Pol
         patches.append((name, ptype, start, size))
         start += size
     return PolyMesh(points=np.ascontiguousarray(points, dtype=np.float64), face_offsets=offsets, face_points=fpts,
-                    owner=owner, neighbour=nei_cell.astype(np.int32), n_cells=len(cells), patches=patches)
+                    owner=owner, neighbour=nei_cell.astype(np.int32), n_cells=int(n_cells), patches=patches)
+
+
+def _outward(points, faces, apex):
+    """Reverse the faces [k, npts] whose normal (first three points) points
+    towards the cell point apex[k] (a point strictly inside or on the other
+    side), so that every face points out of its cell."""
+    a, b, c = points[faces[:, 0]], points[faces[:, 1]], points[faces[:, 2]]
+    inward = np.einsum("ij,ij->i", np.cross(b - a, c - a), apex - a) > 0
+    out = faces.copy()
+    out[inward] = out[inward][:, ::-1]
+    return out
+
+
+def tet_prism_box(nx: int = 6, ny: int = 5, nz: int = 4, prism_layers: int = 1, lx: float = 0.01,
+                  ly: float = 0.008, lz: float = 0.006, first: float = 0.0, jitter: float = 0.0,
+                  seed: int = 1) -> PolyMesh:
+    """A 3-D box of tetrahedra over prism layers: the structured nx x ny x nz
+    hex grid (z layers graded from a first height 'first' when given), the
+    lowest prism_layers layers each hex cut into two triangular prisms along
+    the (i, j)-(i+1, j+1) diagonal of its horizontal faces, every hex above
+    into the six tetrahedra about its (0,0,0)-(1,1,1) diagonal (Kuhn's
+    subdivision: neighbouring hexes cut their shared faces alike, and its
+    horizontal cuts match the prisms' triangles).  Interior points jittered by
+    up to 'jitter' of the local spacing (seeded).  Patches: bottom (wall, z = 0),
+    top, left, right, front, back (generic)."""
+    rng = np.random.default_rng(seed)
+    x = np.linspace(0.0, lx, nx + 1)
+    y = np.linspace(0.0, ly, ny + 1)
+    if first > 0.0:
+        r = _stretch(nz, first, lz) if nz > 1 else np.array([0.0, lz])
+        z = np.asarray(r, dtype=np.float64)
+    else:
+        z = np.linspace(0.0, lz, nz + 1)
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    P = np.stack([X, Y, Z], axis=-1)
+    if jitter > 0.0:
+        dx, dy = lx / nx, ly / ny
+        dz = np.diff(z).min()
+        inner = np.zeros(P.shape[:3], dtype=bool)
+        inner[1:-1, 1:-1, 1:-1] = True
+        d = rng.uniform(-1.0, 1.0, P.shape) * jitter * np.array([dx, dy, dz])
+        P = P + d * inner[..., None]
+    points = P.reshape(-1, 3)
+    pid = lambda i, j, k: (i * (ny + 1) + j) * (nz + 1) + k
+    I, J, K = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    I, J, K = I.ravel(), J.ravel(), K.ravel()
+    corner = lambda a, b, c: pid(I + a, J + b, K + c)
+    prism = K < prism_layers
+    cells_f = []   # (faces [k, 4] padded, cell id per face, local id)
+    # prisms: (000, 100, 110) and (000, 110, 010) at the bottom, the same on top
+    ncell = 0
+    fc_l, fl_l, fp_l = [], [], []
+    pr = np.nonzero(prism)[0]
+    for tri in ((corner(0, 0, 0), corner(1, 0, 0), corner(1, 1, 0)), (corner(0, 0, 0), corner(1, 1, 0), corner(0, 1, 0))):
+        a, b, c = (t[pr] for t in tri)
+        up = pid(0, 0, 1) - pid(0, 0, 0)
+        A, B, Cc = a + up, b + up, c + up
+        cen = (points[a] + points[b] + points[c] + points[A] + points[B] + points[Cc]) / 6.0
+        faces = [np.stack([a, b, c, -np.ones_like(a)], 1), np.stack([A, B, Cc, -np.ones_like(a)], 1),
+                 np.stack([a, b, B, A], 1), np.stack([b, c, Cc, B], 1), np.stack([c, a, A, Cc], 1)]
+        ids = ncell + np.arange(len(a))
+        for li, f in enumerate(faces):
+            ff = _outward(points, f, cen)
+            fc_l.append(ids); fl_l.append(np.full(len(a), li)); fp_l.append(ff)
+        ncell += len(a)
+    # tetrahedra: 000 -> 111 through each permutation of the axes
+    te = np.nonzero(~prism)[0]
+    for perm in ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)):
+        steps = [np.zeros(3, int)]
+        for ax in perm:
+            v = steps[-1].copy()
+            v[ax] = 1
+            steps.append(v)
+        v = [corner(*s)[te] for s in steps]
+        cen = sum(points[q] for q in v) / 4.0
+        faces = [np.stack([v[0], v[1], v[2], -np.ones_like(v[0])], 1), np.stack([v[0], v[1], v[3], -np.ones_like(v[0])], 1),
+                 np.stack([v[0], v[2], v[3], -np.ones_like(v[0])], 1), np.stack([v[1], v[2], v[3], -np.ones_like(v[0])], 1)]
+        ids = ncell + np.arange(len(te))
+        for li, f in enumerate(faces):
+            ff = _outward(points, f, cen)
+            fc_l.append(ids); fl_l.append(np.full(len(te), li)); fp_l.append(ff)
+        ncell += len(te)
+    fc = np.concatenate(fc_l).astype(np.int64)
+    fl = np.concatenate(fl_l).astype(np.int64)
+    fp = np.concatenate(fp_l).astype(np.int64)
+    tol = 1e-12 * max(lx, ly, lz)
+
+    def classify(cells_, lf, cen):
+        out = np.full(len(cells_), -1)
+        out[np.abs(cen[:, 2]) < tol] = 0
+        out[np.abs(cen[:, 2] - lz) < tol] = 1
+        out[np.abs(cen[:, 0]) < tol] = 2
+        out[np.abs(cen[:, 0] - lx) < tol] = 3
+        out[np.abs(cen[:, 1]) < tol] = 4
+        out[np.abs(cen[:, 1] - ly) < tol] = 5
+        return out
+
+    return faces_polymesh(points, fc, fl, fp, ncell, classify,
+                          [("bottom", PATCH_WALL), ("top", PATCH_GENERIC), ("left", PATCH_GENERIC),
+                           ("right", PATCH_GENERIC), ("front", PATCH_GENERIC), ("back", PATCH_GENERIC)])
 
 
 def naca4_blunt(x: np.ndarray, t: float) -> np.ndarray:
