@@ -43,11 +43,12 @@ hybrid airfoil per step, GPU and CPU oracle), --adjoint (the discrete adjoint
 on the viscous hybrid airfoil: exact viscous J, the adjoint solve, J v against
 central differences, the 400-wall-node gradient), --converge-only (the Newton
 time to 1e-8), --flux-only (the face-flux kernel and a copy of known size for
-the counter runs).
+the counter runs), --tetprism (8.32 M tetrahedra and prisms, Kurganov + SAneg,
+mixed and FP64 per step).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--small]
        [--turbulence SARC2|SAneg|none] [--skip-converge] [--no-cpu]
-       [--workloads | --adjoint | --converge-only | --flux-only]
+       [--workloads | --adjoint | --tetprism | --converge-only | --flux-only]
 """
 from __future__ import annotations
 
@@ -324,6 +325,46 @@ def adjoint_mode(args) -> int:
     return 0
 
 
+def tetprism_mode(args) -> int:
+    """Tetrahedra over prism layers at 8.32 M cells (meshgen.tet_prism_box
+    160 x 100 x 100 hexes, 20 prism layers from a first height of 2e-5 over a
+    1 x 0.6 x 0.4 box): a no-slip flat wall under the characteristic far
+    field, M 0.3, Sutherland air, Kurganov first order, SAneg, viscous terms,
+    the Roe operator, GMRES(30) + partitioned block-ILU(0), classical
+    Gram-Schmidt: ms per step in mixed precision and in FP64, and the face-flux
+    kernel's rate."""
+    from rhodst_amd import DeviceCase, cases, meshgen
+    from rhodst_amd._capi import DeviceMesh
+    t0 = time.perf_counter()
+    mesh = meshgen.tet_prism_box(160, 100, 100, prism_layers=20, lx=1.0, ly=0.6, lz=0.4, first=2e-5, jitter=0.1)
+    log(f"mesh: {mesh.n_cells} cells, {mesh.n_faces} faces in {time.perf_counter() - t0:.1f} s")
+    dm = DeviceMesh(mesh)
+    wd = dm.wall_distance()
+    nF = mesh.n_internal + mesh.n_boundary_nonempty()
+    out = {"workload": f"tetrahedra over prism layers, {mesh.n_cells} cells ({nF} faces), flat no-slip wall, "
+                       f"M0.3, Kurganov first order, SAneg, viscous, Roe operator, GMRES(30) + partitioned ILU(0)",
+           "cells": mesh.n_cells, "faces": nF}
+    for name, prec in (("mixed", "mixed"), ("fp64_partitioned", "fp64")):
+        dc = DeviceCase(cases.tet_prism_case(mesh=mesh, wall_dist=wd, precision=prec, ilu_order="partitioned",
+                                             gram_schmidt="classical", n_directions=30, max_iter=1, rel_tol=1e-2),
+                        dm)
+        dc.iterate(args.warmup)
+        t = time.perf_counter()
+        r = dc.iterate(args.steps)
+        sec = time.perf_counter() - t
+        out[f"ms_per_step_{name}"] = round(sec / args.steps * 1e3, 3)
+        out[f"drop_{name}"] = float(r[-1] / r[0])
+        if name == "mixed":
+            ms = dc.time_flux(args.flux_calls)
+            out["flux_kernel_ms"] = round(ms, 5)
+            out["flux_gfaces_per_s"] = round(nF / (ms * 1e-3) / 1e9, 3)
+        log(f"{name}: {out[f'ms_per_step_{name}']} ms/step")
+        del dc
+    out["mixed_speedup_same_preconditioner"] = round(out["ms_per_step_fp64_partitioned"] / out["ms_per_step_mixed"], 3)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def cpu_baseline(args, mesh_kw, case_kw, wall_dist, faces):
     """The oracle's face flux + residual on the host: all threads (OpenMP
     build, bit-identical to the serial one) and one thread, each in a child
@@ -377,6 +418,8 @@ def main(argv=None) -> int:
                          "and the CPU oracle (all threads and one): one JSON line")
     ap.add_argument("--adjoint", action="store_true",
                     help="only the discrete adjoint on the viscous hybrid airfoil (400 wall nodes): one JSON line")
+    ap.add_argument("--tetprism", action="store_true",
+                    help="only the 8.32 M-cell tetrahedron/prism workload (Kurganov + SAneg), mixed and FP64: one JSON line")
     ap.add_argument("--adjoint-steps", type=int, default=4000, help="step limit of the airfoil flow before the adjoint")
     ap.add_argument("--adjoint-no-heat", action="store_true",
                     help="the airfoil flow of --adjoint without the heat-conduction extension")
@@ -393,6 +436,8 @@ def main(argv=None) -> int:
         return workloads_mode(args)
     if args.adjoint:
         return adjoint_mode(args)
+    if args.tetprism:
+        return tetprism_mode(args)
     from rhodst_amd import cases, meshgen
     from rhodst_amd._capi import DeviceMesh
     mesh_kw = dict(n_wake=24, n_surf=64, n_normal=64, n_span_wing=24, n_span_out=8) if args.small else {}

@@ -284,7 +284,7 @@ def box_sa_case(model: str = "SAneg", scheme: str = "HLLC", nx: int = 24, ny: in
 
 
 def tet_prism_case(model: str | None = "SAneg", scheme: str = "Kurganov", mesh=None, mach: float = 0.3,
-                   perturb: float = 0.05, seed: int = 2, **settings) -> CaseSpec:
+                   perturb: float = 0.05, seed: int = 2, wall_dist=None, **settings) -> CaseSpec:
     """Three-dimensional flow over a no-slip wall on tetrahedra above prism
     layers (meshgen.tet_prism_box, 600 cells by default): the wall at z = 0 (U
     fixedValue 0, p and T zeroGradient), the characteristic far field on the
@@ -324,7 +324,8 @@ def tet_prism_case(model: str | None = "SAneg", scheme: str = "Kurganov", mesh=N
         sff = ("inletOutlet", 3.0 * nu)
         sbc = make_scalar_bcs(mesh, {"bottom": ("fixedValue", 0.0),
                                      **{k: sff for k in ("top", "left", "right", "front", "back")}})
-        out.turbulence = (make_turbulence(model), sbc, meshgen.wall_distance(mesh, ["bottom"]), nut0)
+        d = wall_dist if wall_dist is not None else meshgen.wall_distance(mesh, ["bottom"])
+        out.turbulence = (make_turbulence(model), sbc, d, nut0)
         out.name = f"tetPrism_{model}"
     return out
 
