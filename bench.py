@@ -358,6 +358,24 @@ def tetprism_mode(args) -> int:
             ms = dc.time_flux(args.flux_calls)
             out["flux_kernel_ms"] = round(ms, 5)
             out["flux_gfaces_per_s"] = round(nF / (ms * 1e-3) / 1e9, 3)
+            if args.tetprism_adjoint:
+                # the viscous adjoint at this size (eddy viscosity frozen): the
+                # exact J on the second-neighbour pattern and the defect correction
+                dc.update_fields()
+                dc.update_fields()
+                patch = mesh.patch_index("bottom")
+                t = time.perf_counter()
+                dc.adjoint_linearise()
+                out["adjoint_linearise_s"] = round(time.perf_counter() - t, 2)
+                g = dc.adjoint_objective_gradient(patch, (1.0, 0.0, 0.0), 1.0)
+                t = time.perf_counter()
+                lam, perf = dc.adjoint_solve(patch, (1.0, 0.0, 0.0), 1.0, precision="mixed", n_directions=30,
+                                             max_cycles=args.adjoint_steps, tolerance=0.0, rel_tol=1e-8,
+                                             precond_shift=0.3, inner_operator=1)
+                res = np.linalg.norm(dc.adjoint_apply(lam, transpose=True) - g) / np.linalg.norm(g)
+                out["adjoint_solve"] = {"seconds": round(time.perf_counter() - t, 2), "cycles": int(perf.n_iterations),
+                                        "residual_rel": float(res)}
+                log(f"adjoint: {out['adjoint_solve']} after a {out['adjoint_linearise_s']} s linearisation")
         log(f"{name}: {out[f'ms_per_step_{name}']} ms/step")
         del dc
     out["mixed_speedup_same_preconditioner"] = round(out["ms_per_step_fp64_partitioned"] / out["ms_per_step_mixed"], 3)
@@ -420,7 +438,11 @@ def main(argv=None) -> int:
                     help="only the discrete adjoint on the viscous hybrid airfoil (400 wall nodes): one JSON line")
     ap.add_argument("--tetprism", action="store_true",
                     help="only the 8.32 M-cell tetrahedron/prism workload (Kurganov + SAneg), mixed and FP64: one JSON line")
-    ap.add_argument("--adjoint-steps", type=int, default=4000, help="step limit of the airfoil flow before the adjoint")
+    ap.add_argument("--tetprism-adjoint", action="store_true",
+                    help="with --tetprism: also the viscous adjoint (J and the defect-correction solve) at that size")
+    ap.add_argument("--adjoint-steps", type=int, default=4000,
+                    help="step limit of the airfoil flow before the adjoint (--adjoint); cycle limit of the "
+                         "tetrahedron/prism adjoint solve (--tetprism-adjoint)")
     ap.add_argument("--adjoint-no-heat", action="store_true",
                     help="the airfoil flow of --adjoint without the heat-conduction extension")
     ap.add_argument("--flux-only", action="store_true",
