@@ -538,7 +538,9 @@ int rdst_adjoint_objective_gradient(rdst_case* c, int patch, const double* dir, 
  * differences without summing the whole patch */
 int rdst_adjoint_objective_cells(rdst_case* c, int patch, const double* dir, double q, double p_ref, double* out);
 /* linearises at the current state, then solves J^T lambda = dJ/dW; lambda
- * [n_cells][5] holds the initial guess (zeros for none) and receives the solution */
+ * [n_cells][5] holds the initial guess (zeros for none) and receives the solution.
+ * With settings.viscous GMRES is preconditioned from the right (it minimises
+ * the true residual, the ILU's matrix being only the face-neighbour part). */
 int rdst_adjoint_solve(rdst_case* c, int patch, const double* dir, double q, const rdst_adjoint_settings* s,
                        double* lambda, rdst_solver_perf* perf);
 
