@@ -261,6 +261,10 @@ typedef struct rdst_turbulence {
     double relax;                 /* nuTilda relaxation factor (<= 0: none) */
     double tolerance, rel_tol;    /* smoothSolver tolerance / relTol */
     int32_t min_iter, max_iter;   /* symmetric Gauss-Seidel sweeps */
+    int32_t convection;           /* divSchemes div(phi,nuTilda): 0 Gauss upwind, 1 Gauss linearUpwind
+                                     grad(nuTilda) (upwind matrix, explicit (Cf - C_upwind).grad(nuTilda)
+                                     correction in the source, Gauss linear gradient) */
+    int32_t pad;
 } rdst_turbulence;
 
 /* 0/nuTilda boundary condition of one patch (the wall patches get mut = 0) */

@@ -2415,6 +2415,17 @@ static void saCorrect(Case& c, Perf* perfOut) {
         double corr = gms * (snGradLimiter(psiL, un * un, q * q) * q);
         src[o] += corr;
         src[nb] -= corr;
+        if (t.convection == 1) {
+            // div(phi,nuTilda) Gauss linearUpwind grad(nuTilda): the explicit
+            // correction phi (Cf - C_u).grad(nuTilda)_u of the upwind cell u
+            // (gaussConvectionScheme::fvmDiv adds fvc::surfaceIntegrate of it)
+            const double phi = c.phi[f];
+            const int u = phi > 0.0 ? o : nb;
+            const V3 d = m.Cf[f] - m.C[u];
+            const double lc = phi * (d.x * gn[u].x + d.y * gn[u].y + d.z * gn[u].z);
+            src[o] -= lc;
+            src[nb] += lc;
+        }
     }
     for (int b = 0; b < nB; ++b) {
         const rdst_scalar_bc& bc = tb.bc[m.bPatch[b]];

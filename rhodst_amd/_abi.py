@@ -78,7 +78,7 @@ class rdst_turbulence(C.Structure):
                 ("Ct4", C.c_double),
                 ("Cr1", C.c_double), ("Cr2", C.c_double), ("Cr3", C.c_double),
                 ("relax", C.c_double), ("tolerance", C.c_double), ("rel_tol", C.c_double),
-                ("min_iter", C.c_int32), ("max_iter", C.c_int32)]
+                ("min_iter", C.c_int32), ("max_iter", C.c_int32), ("convection", C.c_int32), ("pad", C.c_int32)]
 
 
 class rdst_scalar_bc(C.Structure):
@@ -92,7 +92,8 @@ def make_turbulence(model="SAneg", **kw) -> rdst_turbulence:
     """RASProperties <model>Coeffs with the reference's defaults
     (SAneg.C:168-307, SARC2.C:172-285, SARC1.C:196-330: Cr2 = 12) plus the nuTilda solver settings of
     tutorials/RAE2822/system/fvSolution (symGaussSeidel, tolerance 1e-6,
-    relTol 0.1, minIter 1, relaxation 0.5)."""
+    relTol 0.1, minIter 1, relaxation 0.5); convection "upwind" or "linearUpwind"
+    (divSchemes div(phi,nuTilda), the tutorials' Gauss linearUpwind grad(nuTilda))."""
     t = rdst_turbulence()
     t.model = TURB_MODELS[model] if isinstance(model, str) else int(model)
     t.negative_nu_tilda = int(kw.get("negative_nu_tilda", 1))
@@ -107,6 +108,8 @@ def make_turbulence(model="SAneg", **kw) -> rdst_turbulence:
         setattr(t, k, kw.get(k, v))
     t.min_iter = int(kw.get("min_iter", 1))
     t.max_iter = int(kw.get("max_iter", 1000))
+    conv = kw.get("convection", "upwind")
+    t.convection = {"upwind": 0, "linearUpwind": 1}[conv] if isinstance(conv, str) else int(conv)
     return t
 
 

@@ -201,8 +201,10 @@ def rae2822_case(scheme: str = "Kurganov", turbulence: str | None = "SpalartAllm
     reconstruction), fvSolution (BiCGStab + ILUC0, maxIter 100, tolerance
     and relTol 1e-10; fieldBounds rho 0.01 10, p 100 1e6, T 10 800) and the
     SpalartAllmaras model with 0/nuTilda (4.4e-5 inside and at the far field,
-    0 on the wall).  The mesh is read from the committed array copy of the
-    tutorial's polyMesh."""
+    0 on the wall) and its div(phi,nuTilda) Gauss linearUpwind grad(nuTilda).
+    The mesh is read from the committed array copy of the tutorial's
+    polyMesh.  (fvSchemes' grad(U) cellLimited Gauss linear 1 runs as Gauss
+    linear.)"""
     mesh = mesh if mesh is not None else meshgen.polymesh_from_npz(RAE2822_MESH)
     Uinf, pinf, Tinf = (233.44, 9.417, 0.0), 108987.773, 255.556
     R = 8314.47 / 28.966
@@ -221,7 +223,7 @@ def rae2822_case(scheme: str = "Kurganov", turbulence: str | None = "SpalartAllm
         from ._abi import make_scalar_bcs, make_turbulence
         sbc = make_scalar_bcs(mesh, {"airfoil": ("fixedValue", 0.0), "freestreamBC": ("inletOutlet", 4.4e-5)})
         d = wall_dist if wall_dist is not None else meshgen.wall_distance(mesh, ["airfoil"])
-        spec.turbulence = (make_turbulence(turbulence), sbc, d, np.full(n, 4.4e-5))
+        spec.turbulence = (make_turbulence(turbulence, convection="linearUpwind"), sbc, d, np.full(n, 4.4e-5))
         spec.name = f"RAE2822_{turbulence}"
     return spec
 
@@ -284,7 +286,8 @@ def box_sa_case(model: str = "SAneg", scheme: str = "HLLC", nx: int = 24, ny: in
 
 
 def tet_prism_case(model: str | None = "SAneg", scheme: str = "Kurganov", mesh=None, mach: float = 0.3,
-                   perturb: float = 0.05, seed: int = 2, wall_dist=None, **settings) -> CaseSpec:
+                   perturb: float = 0.05, seed: int = 2, wall_dist=None, turb_kw: dict | None = None,
+                   **settings) -> CaseSpec:
     """Three-dimensional flow over a no-slip wall on tetrahedra above prism
     layers (meshgen.tet_prism_box, 600 cells by default): the wall at z = 0 (U
     fixedValue 0, p and T zeroGradient), the characteristic far field on the
@@ -325,7 +328,7 @@ def tet_prism_case(model: str | None = "SAneg", scheme: str = "Kurganov", mesh=N
         sbc = make_scalar_bcs(mesh, {"bottom": ("fixedValue", 0.0),
                                      **{k: sff for k in ("top", "left", "right", "front", "back")}})
         d = wall_dist if wall_dist is not None else meshgen.wall_distance(mesh, ["bottom"])
-        out.turbulence = (make_turbulence(model), sbc, d, nut0)
+        out.turbulence = (make_turbulence(model, **(turb_kw or {})), sbc, d, nut0)
         out.name = f"tetPrism_{model}"
     return out
 

@@ -25,6 +25,7 @@ MeshDev Case::meshDev() const {
     d.ellCode = m.d_ellCode.p; d.ellFace = m.d_ellFace.p; d.ellSlot = m.d_ellSlot.p; d.slotDiag = m.d_slotDiag.p;
     d.slotUpper = m.d_slotUpper.p; d.slotLower = m.d_slotLower.p;
     d.C = m.d_C.p;
+    d.Cf = m.d_Cf.p;
     d.ellSf = m.d_ellSf.p; d.ellMagSf = m.d_ellMagSf.p; d.ellWt = m.d_ellWt.p; d.ellDc = m.d_ellDc.p;
     d.ellCorr = m.d_ellCorr.p;
     return d;
@@ -708,6 +709,9 @@ int rdst_case_set_turbulence(rdst_case* cc, const rdst_turbulence* turb, const r
                 throw ArgError("rdst_case_set_turbulence: sigmaNut, kappa, Prt and Cv2 must be > 0");
             if (turb->relax > 1.0) throw ArgError("rdst_case_set_turbulence: relaxation factor > 1");
             if (turb->max_iter < 1) throw ArgError("rdst_case_set_turbulence: max_iter < 1");
+            if (turb->convection != 0 && turb->convection != 1)
+                throw ArgError("rdst_case_set_turbulence: convection is 0 (upwind) or 1 (linearUpwind)",
+                               RDST_ERR_UNKNOWN_NAME);
         }
         std::vector<double> wd;
         if (turb->model != RDST_TURB_LAMINAR && !wall_dist) {

@@ -619,6 +619,9 @@ void mesh_upload(Mesh& m) {
     for (int k = 0; k < nF; ++k)
         for (int d = 0; d < 3; ++d) soa[(size_t)d * nF + k] = m.Sf[3 * (size_t)k + d];
     m.d_Sf.upload(soa);
+    for (int k = 0; k < nF; ++k)
+        for (int d = 0; d < 3; ++d) soa[(size_t)d * nF + k] = m.Cf[3 * (size_t)k + d];
+    m.d_Cf.upload(soa);
     m.d_magSf.upload(m.magSf);
     m.d_w.upload(m.w);
     m.d_dc.upload(m.dc);

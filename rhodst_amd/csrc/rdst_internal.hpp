@@ -200,6 +200,7 @@ struct Mesh {
     DBuf<int32_t> d_sellWidth, d_sellCol;
     DBuf<int32_t> d_bWall;  // [nB] 1 where the patch is a wall
     DBuf<double> d_C;       // cell centres SoA [3][nC]
+    DBuf<double> d_Cf;      // face centres SoA [3][nF]
     DBuf<int32_t> d_wideRowPtr, d_wideCols, d_wideT, d_wideOfSlot, d_wideColor;
 };
 
@@ -295,6 +296,7 @@ struct MeshDev {
     const int32_t* slotDiag;
     const int32_t *slotUpper, *slotLower;  // per internal face: (owner row, neighbour col), (neighbour row, owner col)
     const double* C;                       // cell centres, SoA [3][nC]
+    const double* Cf;                      // face centres, SoA [3][nF]
     const double *ellSf, *ellMagSf, *ellWt, *ellDc, *ellCorr;
 };
 

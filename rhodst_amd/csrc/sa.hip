@@ -309,6 +309,16 @@ __global__ __launch_bounds__(256) void k_sa_assemble(MeshDev m, Fields F, SaDev 
         }
         off -= coef;
         diag += coef;
+        if (t.convection == 1) {
+            // linearUpwind grad(nuTilda): phi (Cf - C_u).grad(nuTilda)_u of the
+            // upwind cell u (the owner when phi > 0) leaves the source
+            const int u = (phi > 0.0) == own ? c : idx;
+            const double gu0 = u == c ? gnc0 : go0, gu1 = u == c ? gnc1 : go1, gu2 = u == c ? gnc2 : go2;
+            const double lc = phi * ((m.Cf[fi] - m.C[u]) * gu0 + (m.Cf[m.nF + fi] - m.C[n + u]) * gu1 +
+                                     (m.Cf[2 * (size_t)m.nF + fi] - m.C[2 * (size_t)n + u]) * gu2);
+            if (own) src -= lc;
+            else src += lc;
+        }
         s.A[own ? m.slotUpper[fi] : m.slotLower[fi]] = off;  // this row's slot of the face
         sumOff += fabs(off);
     }

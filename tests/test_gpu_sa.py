@@ -16,7 +16,8 @@ def rel(a, b):
 
 
 VARIANTS = [("SAneg", {}), ("SAneg", {"stilda_modification": 0}), ("SAneg", {"negative_nu_tilda": 0}),
-            ("SARC2", {}), ("SARC2", {"rcm_correction": 0, "relax": 0.0}), ("SARC1", {})]
+            ("SARC2", {}), ("SARC2", {"rcm_correction": 0, "relax": 0.0}), ("SARC1", {}),
+            ("SAneg", {"convection": "linearUpwind"}), ("SARC2", {"convection": "linearUpwind"})]
 
 
 @pytest.mark.parametrize("model,kw", VARIANTS)

@@ -15,11 +15,12 @@ def rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
 
 
-@pytest.mark.parametrize("model,scheme", [("SAneg", "Kurganov"), ("SARC2", "HLLC"), (None, "AUSM+up")])
-def test_tet_prism_iterate_parity(oracle, model, scheme):
+@pytest.mark.parametrize("model,scheme,conv", [("SAneg", "Kurganov", "upwind"), ("SARC2", "HLLC", "upwind"),
+                                               (None, "AUSM+up", "upwind"), ("SAneg", "Kurganov", "linearUpwind")])
+def test_tet_prism_iterate_parity(oracle, model, scheme, conv):
     from rhodst_amd import _capi
-    print(f"[tet/prism] {model} {scheme}", flush=True)
-    spec = cases.tet_prism_case(model=model, scheme=scheme)
+    print(f"[tet/prism] {model} {scheme} {conv}", flush=True)
+    spec = cases.tet_prism_case(model=model, scheme=scheme, turb_kw={"convection": conv})
     dc = DeviceCase(spec)
     oc = oracle.OracleCase(spec)
     if model:

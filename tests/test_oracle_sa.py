@@ -381,3 +381,46 @@ def test_sarc1_rotation_correction_acts(oracle):
         res[on] = oc.get(29)
         assert np.all(np.isfinite(res[on]))
     assert _rel(res[1], res[0]) > 1e-8
+
+
+def test_linear_upwind_correction_on_a_linear_field(oracle):
+    """div(phi,nuTilda) Gauss linearUpwind grad(nuTilda): on a uniform grid a
+    linear nuTilda has the exact Gauss gradient in cells two layers from the
+    boundary, so the explicit correction each face leaves in the source is
+    phi_f (nuTilda(Cf) - nuTilda(C_u)) with u the upwind cell (the owner for
+    phi > 0), from the linear function itself; the matrix is the upwind one
+    either way, so the two sources differ by exactly that sum (minus on the
+    owner, plus on the neighbour)."""
+    from rhodst_amd import cases
+    nx, ny = 12, 10
+    base = dict(nx=nx, ny=ny, jitter=0.0, negative_fraction=0.0)
+    out = {}
+    for conv in ("upwind", "linearUpwind"):
+        spec = cases.box_sa_case(turb_kw={"convection": conv}, **base)
+        oc = oracle.OracleCase(spec)
+        oc.flux_residual()
+        g = oracle.geometry(spec.mesh)
+        a, gx, gy = 3e-5, 2e-3, -1.5e-3
+        nut = a + gx * g["C"][:, 0] + gy * g["C"][:, 1]
+        oc.put(29, nut)
+        oc.turbulence_correct()
+        out[conv] = (oc.get(38), oc.get(12), g)
+    dS = out["linearUpwind"][0] - out["upwind"][0]
+    phi, g = out["upwind"][1], out["upwind"][2]
+    m = spec.mesh
+    nI = m.n_internal
+    own, nei = m.owner[:nI], m.neighbour
+    lin = lambda p: a + gx * p[..., 0] + gy * p[..., 1]
+    up = np.where(phi[:nI] > 0.0, own, nei)
+    lc = phi[:nI] * (lin(g["Cf"][:nI]) - lin(g["C"][up]))
+    expect = np.zeros(m.n_cells)
+    np.add.at(expect, own, -lc)
+    np.add.at(expect, nei, lc)
+    # cells two layers from the boundary (box_2d numbers cells i + nx j)
+    i, j = np.arange(m.n_cells) % nx, np.arange(m.n_cells) // nx
+    inner = (i >= 2) & (i < nx - 2) & (j >= 2) & (j < ny - 2)
+    scale = np.abs(lc).max()
+    assert scale > 0
+    assert np.abs(dS[inner] - expect[inner]).max() <= 1e-10 * scale, np.abs(dS[inner] - expect[inner]).max() / scale
+    # and the correction is not negligible against the upwind source
+    assert np.abs(expect[inner]).max() > 1e-4 * np.abs(out["upwind"][0][inner]).max()
