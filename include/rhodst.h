@@ -195,7 +195,32 @@ typedef struct rdst_settings {
      * rhoE change by at most this fraction of their values before the update
      * (a per-cell damping for the exact-Jacobian operator at large CFL). */
     double update_limit;
+    /* fvSchemes gradSchemes grad(U) (rdst_grad_kind; tutorials/RAE2822/
+     * system/fvSchemes:26 selects "cellLimited Gauss linear 1"): the gradient
+     * of U that tauMC (baseFluxDST.C:233-236), the Spalart-Allmaras strain and
+     * vorticity (SAneg.C:524, SARC2.C:458) and the force coefficients read.
+     * grad_u_limit is the k of cellLimited, in [0, 1]; k below 1e-15 leaves
+     * the gradient unlimited.  The non-orthogonal correction of snGrad(U) is
+     * formed from the unlimited Gauss linear gradient (foam-extend 4.1
+     * correctedSnGrad reads the gradScheme of "snGradCorr(U)", the default). */
+    int32_t grad_u;
+    int32_t grad_u_pad;
+    double grad_u_limit;
 } rdst_settings;
+
+/* foam-extend 4.1 gradSchemes for grad(U).  GAUSS_LINEAR: sum_f Sf U_f / V,
+ * U_f linearly interpolated (the boundary value on boundary faces).
+ * CELL_LIMITED (cellLimitedGrad): the Gauss linear gradient with column j
+ * (dU_j/dx_i, i = 0..2) scaled by l_j in [0, 1], the largest factor for which
+ * U_j + (Cf - C) & grad stays within [min, max] of U_j over the cell, its
+ * face neighbours and its boundary-face values at every face centre Cf of
+ * the cell; for k < 1 that interval is widened on both sides by
+ * (1/k - 1)(max - min).  Per face and component: where the extrapolated
+ * change e exceeds max - U_j (by more than 1e-300), l_j = min(l_j,
+ * (max - U_j)/e); where it is below min - U_j, l_j = min(l_j, (min - U_j)/e).
+ * The boundary gradient is then corrected as for Gauss linear
+ * (gaussGrad::correctBoundaryConditions). */
+enum rdst_grad_kind { RDST_GRAD_GAUSS_LINEAR = 0, RDST_GRAD_CELL_LIMITED = 1 };
 
 /* Non-orthogonal correction of a surface-normal gradient (foam-extend 4.1
  * snGradSchemes).  CORRECTED: dc (x_N - x_P) + corr & interp(grad x), with
@@ -308,7 +333,9 @@ enum rdst_field {
     RDST_F_SA_DIAG = 35,          /* nuTilda matrix after relax(): diagonal incl. boundary coefficients [n_cells] */
     RDST_F_SA_UPPER = 36,         /* [n_internal] (owner row, neighbour column) */
     RDST_F_SA_LOWER = 37,         /* [n_internal] (neighbour row, owner column) */
-    RDST_F_SA_SOURCE = 38         /* [n_cells] incl. boundary coefficients and relaxation */
+    RDST_F_SA_SOURCE = 38,        /* [n_cells] incl. boundary coefficients and relaxation */
+    RDST_F_GRAD_U = 39            /* cells [n][9]: fvc::grad(U) of the last residual evaluation of a viscous
+                                     case (settings.grad_u applied), dU_j/dx_i at 3 i + j */
 };
 
 /* integer arrays of a mesh */

@@ -113,7 +113,7 @@ def euler_jacobian(U, gamma, rhoE, rho, n):
     return J.reshape(5, 5)
 
 
-_FIELD_SHAPES = {1: 3, 5: 3, 8: 5, 9: 5, 13: 3, 15: 3, 17: 3, 20: 3, 25: 5}
+_FIELD_SHAPES = {1: 3, 5: 3, 8: 5, 9: 5, 13: 3, 15: 3, 17: 3, 20: 3, 25: 5, 39: 9}
 
 
 class OracleCase:

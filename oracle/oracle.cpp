@@ -939,9 +939,69 @@ static V3 snGradU_b(const Case& c, int b) {
     }
 }
 
-// fvc::grad(U) Gauss linear (cells) and the corrected boundary gradient
-// (gaussGrad::correctBoundaryConditions: g_b = g_I + n (snGrad_b - n.g_I))
-static void gradU(const Case& c, std::vector<T9>& g, std::vector<T9>& gb) {
+// cellLimitedGrad (foam-extend 4.1 gradSchemes, not part of the reference
+// tree; tutorials/RAE2822/system/fvSchemes:26 selects "cellLimited Gauss
+// linear 1" for grad(U)), restated from its published algorithm: the max and
+// min of each U component over a cell, its face neighbours (a face loop,
+// owner and neighbour sides) and its boundary-face values, less the cell
+// value; for k < 1 both widened by (1/k - 1)(max - min); a limiter l = 1 per
+// component, lowered face by face (internal faces for both sides, then
+// boundary faces) by limitFace on the change (Cf - C) & grad towards the
+// face centre; grad_ij *= l_j.  limitFace: where the change exceeds
+// max + VSMALL, l = min(l, max/change); else where it is below min - VSMALL,
+// l = min(l, min/change); VSMALL = 1e-300.
+static void limitFaceCmp(double& l, double dmax, double dmin, double e) {
+    if (e > dmax + 1.0e-300) l = std::min(l, dmax / e);
+    else if (e < dmin - 1.0e-300) l = std::min(l, dmin / e);
+}
+static void limitFace(V3& l, const V3& dmax, const V3& dmin, const V3& e) {
+    limitFaceCmp(l.x, dmax.x, dmin.x, e.x);
+    limitFaceCmp(l.y, dmax.y, dmin.y, e.y);
+    limitFaceCmp(l.z, dmax.z, dmin.z, e.z);
+}
+static V3 maxV(const V3& a, const V3& b) { return v3(std::max(a.x, b.x), std::max(a.y, b.y), std::max(a.z, b.z)); }
+static V3 minV(const V3& a, const V3& b) { return v3(std::min(a.x, b.x), std::min(a.y, b.y), std::min(a.z, b.z)); }
+static inline V3 dotVT(V3 v, const T9& t);
+static void cellLimitGradU(const Case& c, double k, std::vector<T9>& g) {
+    const Mesh& m = c.m;
+    const int n = m.nCells;
+    std::vector<V3> mx(c.U.begin(), c.U.begin() + n), mn(c.U.begin(), c.U.begin() + n);
+    for (int f = 0; f < m.nInternal; ++f) {
+        const int o = m.owner[f], nb = m.neighbour[f];
+        mx[o] = maxV(mx[o], c.U[nb]); mn[o] = minV(mn[o], c.U[nb]);
+        mx[nb] = maxV(mx[nb], c.U[o]); mn[nb] = minV(mn[nb], c.U[o]);
+    }
+    for (int b = 0; b < m.nB; ++b) {
+        const int o = m.bCell[b];
+        mx[o] = maxV(mx[o], c.bU[b]); mn[o] = minV(mn[o], c.bU[b]);
+    }
+    for (int i = 0; i < n; ++i) { mx[i] = mx[i] - c.U[i]; mn[i] = mn[i] - c.U[i]; }
+    if (k < 1.0) {
+        for (int i = 0; i < n; ++i) {
+            const V3 d = (1.0 / k - 1.0) * (mx[i] - mn[i]);
+            mx[i] = mx[i] + d; mn[i] = mn[i] - d;
+        }
+    }
+    std::vector<V3> lim(n, v3(1.0, 1.0, 1.0));
+    for (int f = 0; f < m.nInternal; ++f) {
+        const int o = m.owner[f], nb = m.neighbour[f];
+        limitFace(lim[o], mx[o], mn[o], dotVT(m.Cf[f] - m.C[o], g[o]));
+        limitFace(lim[nb], mx[nb], mn[nb], dotVT(m.Cf[f] - m.C[nb], g[nb]));
+    }
+    for (int b = 0; b < m.nB; ++b) {
+        const int o = m.bCell[b];
+        limitFace(lim[o], mx[o], mn[o], dotVT(m.Cf[m.bFace[b]] - m.C[o], g[o]));
+    }
+    for (int i = 0; i < n; ++i)
+        for (int r = 0; r < 3; ++r)
+            for (int j = 0; j < 3; ++j) g[i].a[3 * r + j] = cmp(lim[i], j) * g[i].a[3 * r + j];
+}
+
+// fvc::grad(U) Gauss linear (cells), cellLimited when the settings select it,
+// and the corrected boundary gradient (gaussGrad::correctBoundaryConditions:
+// g_b = g_I + n (snGrad_b - n.g_I)).  graw, when given, receives the
+// unlimited Gauss linear cell gradient (the snGrad correction's).
+static void gradU(const Case& c, std::vector<T9>& g, std::vector<T9>& gb, std::vector<T9>* graw = nullptr) {
     const Mesh& m = c.m;
     int n = m.nCells;
     g.assign(n, T9{});
@@ -970,6 +1030,8 @@ static void gradU(const Case& c, std::vector<T9>& g, std::vector<T9>& gb) {
         }
         for (int k = 0; k < 9; ++k) g[ci].a[k] /= m.V[ci];
     }
+    if (graw) *graw = g;
+    if (c.st.grad_u == RDST_GRAD_CELL_LIMITED && c.st.grad_u_limit >= 1.0e-15) cellLimitGradU(c, c.st.grad_u_limit, g);
     gb.assign(m.nB, T9{});
     for (int b = 0; b < m.nB; ++b) {
         int f = m.bFace[b], ci = m.bCell[b];
@@ -1025,8 +1087,8 @@ static void viscousTerms(Case& c) {
     const double psiL = snGradPsi(c.st.laplacian_sngrad, c.st.laplacian_limit);
     const double psiS = snGradPsi(c.st.sngrad, c.st.sngrad_limit);
     int n = m.nCells;
-    std::vector<T9> g, gb;
-    gradU(c, g, gb);
+    std::vector<T9> g, gb, gr;
+    gradU(c, g, gb, &gr);
     std::vector<double> muEff, bmuEff;
     muAlphaEff(c, muEff, bmuEff, nullptr, nullptr);
     std::vector<T9> tau(n), btau(m.nB);
@@ -1045,8 +1107,10 @@ static void viscousTerms(Case& c) {
         double w = m.w[f];
         double muf = w * (muEff[o] - muEff[nb]) + muEff[nb];
         T9 gf, tf;
+        // the correction interpolates the unlimited Gauss linear gradient
+        // (foam-extend 4.1 correctedSnGrad: gradScheme "snGradCorr(U)")
         for (int k = 0; k < 9; ++k) {
-            gf.a[k] = w * (g[o].a[k] - g[nb].a[k]) + g[nb].a[k];
+            gf.a[k] = w * (gr[o].a[k] - gr[nb].a[k]) + gr[nb].a[k];
             tf.a[k] = w * (tau[o].a[k] - tau[nb].a[k]) + tau[nb].a[k];
         }
         // orthogonal part and non-orthogonal correction of snGrad(U): the
@@ -2991,6 +3055,13 @@ int orc_case_get(void* h, int field, double* out) {
             case RDST_F_BNU_TILDA: putS(c.tb.bnut); break; case RDST_F_BMUT: putS(c.tb.bmut); break;
             case RDST_F_SA_DIAG: putS(c.tb.Ad); break; case RDST_F_SA_UPPER: putS(c.tb.Au); break;
             case RDST_F_SA_LOWER: putS(c.tb.Al); break; case RDST_F_SA_SOURCE: putS(c.tb.As); break;
+            case RDST_F_GRAD_U: {  // grad(U) of the current U and boundary values
+                std::vector<T9> g, gb;
+                gradU(c, g, gb);
+                for (size_t i = 0; i < g.size(); ++i)
+                    for (int q = 0; q < 9; ++q) out[9 * i + q] = g[i].a[q];
+                break;
+            }
             default: throw std::runtime_error("bad field id");
         }
         (void)m;

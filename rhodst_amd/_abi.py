@@ -60,7 +60,8 @@ class rdst_settings(C.Structure):
                 ("iteration_start", C.c_int32), ("mag_converged_slope", C.c_double), ("min_residual", C.c_double),
                 ("delta_t", C.c_double), ("laplacian_sngrad", C.c_int32), ("sngrad", C.c_int32),
                 ("laplacian_limit", C.c_double), ("sngrad_limit", C.c_double), ("heat_conduction", C.c_int32),
-                ("update_limit", C.c_double)]
+                ("update_limit", C.c_double), ("grad_u", C.c_int32), ("grad_u_pad", C.c_int32),
+                ("grad_u_limit", C.c_double)]
 
 
 class rdst_solver_perf(C.Structure):
@@ -265,7 +266,20 @@ def make_settings(**kw) -> rdst_settings:
     s.sngrad, s.sngrad_limit = _sngrad(kw.get("sngrad", "corrected"))
     s.heat_conduction = int(kw.get("heat_conduction", 0))
     s.update_limit = float(kw.get("update_limit", 0.0))
+    # gradSchemes grad(U): "Gauss linear" or ("cellLimited", k)
+    s.grad_u, s.grad_u_limit = _grad_scheme(kw.get("grad_u", "Gauss linear"))
     return s
+
+
+def _grad_scheme(spec):
+    if isinstance(spec, str):
+        if spec != "Gauss linear":
+            raise ValueError("grad(U) scheme: 'Gauss linear' or ('cellLimited', k)")
+        return 0, 0.0
+    kind, k = spec
+    if kind != "cellLimited":
+        raise ValueError("grad(U) scheme: 'Gauss linear' or ('cellLimited', k)")
+    return 1, float(k)
 
 
 SNGRAD = {"corrected": 0, "uncorrected": 1, "limited": 2}

@@ -164,7 +164,7 @@ class DeviceMesh:
         return out
 
 
-_FIELD_SHAPES = {1: 3, 5: 3, 8: 5, 9: 5, 13: 3, 15: 3, 17: 3, 20: 3, 24: 25, 25: 5}
+_FIELD_SHAPES = {1: 3, 5: 3, 8: 5, 9: 5, 13: 3, 15: 3, 17: 3, 20: 3, 24: 25, 25: 5, 39: 9}
 
 
 class DeviceCase:

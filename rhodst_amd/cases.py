@@ -201,10 +201,9 @@ def rae2822_case(scheme: str = "Kurganov", turbulence: str | None = "SpalartAllm
     reconstruction), fvSolution (BiCGStab + ILUC0, maxIter 100, tolerance
     and relTol 1e-10; fieldBounds rho 0.01 10, p 100 1e6, T 10 800) and the
     SpalartAllmaras model with 0/nuTilda (4.4e-5 inside and at the far field,
-    0 on the wall) and its div(phi,nuTilda) Gauss linearUpwind grad(nuTilda).
-    The mesh is read from the committed array copy of the tutorial's
-    polyMesh.  (fvSchemes' grad(U) cellLimited Gauss linear 1 runs as Gauss
-    linear.)"""
+    0 on the wall), its div(phi,nuTilda) Gauss linearUpwind grad(nuTilda)
+    and grad(U) cellLimited Gauss linear 1 (fvSchemes:26).  The mesh is read
+    from the committed array copy of the tutorial's polyMesh."""
     mesh = mesh if mesh is not None else meshgen.polymesh_from_npz(RAE2822_MESH)
     Uinf, pinf, Tinf = (233.44, 9.417, 0.0), 108987.773, 255.556
     R = 8314.47 / 28.966
@@ -213,7 +212,8 @@ def rae2822_case(scheme: str = "Kurganov", turbulence: str | None = "SpalartAllm
     wall = {"p": "zeroGradient", "T": "zeroGradient", "U": ("fixedValue", (0.0, 0.0, 0.0))}
     kw = dict(flux_scheme=scheme, neighbouring=1, initial_co=1.0, min_co=1.0, max_co=500.0, viscous=1,
               solver="BiCGStab", precond="ILUC0", min_iter=1, max_iter=100, tolerance=1e-10, rel_tol=1e-10,
-              reconstruction="OSPRE", p_bounds=(100.0, 1e6), T_bounds=(10.0, 800.0), rho_bounds=(0.01, 10.0))
+              reconstruction="OSPRE", p_bounds=(100.0, 1e6), T_bounds=(10.0, 800.0), rho_bounds=(0.01, 10.0),
+              grad_u=("cellLimited", 1.0))
     kw.update(settings)
     n = mesh.n_cells
     spec = CaseSpec("RAE2822", mesh, make_thermo(Cp=1005.0, R=R, Pr=0.72, sutherland=(1.4584e-6, 110.33)),

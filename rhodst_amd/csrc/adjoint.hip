@@ -969,6 +969,8 @@ void adjoint_check_scheme(const Case& c) {
         throw ArgError("adjoint: the exact linearisation covers first-order (upwind) reconstruction");
     if (c.sa.on && !c.st.viscous)
         throw ArgError("adjoint: a turbulence model needs the viscous residual (settings.viscous = 1)");
+    if (c.st.viscous && grad_u_limited(c.st))
+        throw ArgError("adjoint: cellLimited grad(U) (its min/max limiter) is not linearised; use Gauss linear");
 }
 
 void adjoint_jacobian(Case& c) {

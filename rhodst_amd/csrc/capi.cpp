@@ -36,7 +36,8 @@ static void alloc_fields(Case& c) {
     const size_t n = m.nC, nB = std::max(m.nB, 1), nF = m.nF;
     c.cellBuf.alloc(14 * n);
     c.bndBuf.alloc(21 * nB);
-    c.gradBuf.alloc(9 * n + 9 * nB);
+    const bool lim = grad_u_limited(c.st);
+    c.gradBuf.alloc(9 * n + 9 * nB + (lim ? 9 * n : 0));
     c.faceBuf.alloc(8 * nF);
     c.RBuf.alloc(5 * n);
     c.WBuf.alloc(5 * n);
@@ -65,6 +66,7 @@ static void alloc_fields(Case& c) {
     for (int i = 0; i < 21; ++i) *bnd[i] = p + i * nB;
     f.gradU = c.gradBuf.p;
     f.bgradU = c.gradBuf.p + 9 * n;
+    f.gradUc = lim ? c.gradBuf.p + 9 * n + 9 * nB : f.gradU;
     p = c.faceBuf.p;
     double** face[8] = {&f.phi, &f.phiUx, &f.phiUy, &f.phiUz, &f.phiE, &f.Upx, &f.Upy, &f.Upz};
     for (int i = 0; i < 8; ++i) *face[i] = p + i * nF;
@@ -283,6 +285,11 @@ int rdst_case_create(rdst_mesh* mesh, const rdst_thermo* thermo, const rdst_bc* 
         if (s.heat_conduction != 0 && s.heat_conduction != 1)
             throw ArgError("rdst_case_create: heat_conduction is 0 or 1");
         if (!(s.update_limit >= 0.0)) throw ArgError("rdst_case_create: update_limit must be >= 0");
+        if (s.grad_u != RDST_GRAD_GAUSS_LINEAR && s.grad_u != RDST_GRAD_CELL_LIMITED)
+            throw ArgError("Unknown gradScheme for grad(U); valid: Gauss linear, cellLimited Gauss linear",
+                           RDST_ERR_UNKNOWN_NAME);
+        if (s.grad_u == RDST_GRAD_CELL_LIMITED && !(s.grad_u_limit >= 0.0 && s.grad_u_limit <= 1.0))
+            throw ArgError("rdst_case_create: cellLimited coefficient k must be >= 0 and <= 1");
         if (!(thermo->Cp > thermo->R) || !(thermo->R > 0)) throw ArgError("rdst_case_create: need Cp > R > 0");
         if (!m->uploaded) {
             mesh_upload(*m);
@@ -398,6 +405,13 @@ int rdst_case_get(rdst_case* cc, int field, double* out) {
             case RDST_F_MUT: one(f.mut, n); break;
             case RDST_F_ALPHAT: one(f.alphat, n); break;
             case RDST_F_BMUT: one(f.bmut, nB); break;
+            case RDST_F_GRAD_U: {
+                if (!c->st.viscous) throw ArgError("rdst_case_get: grad(U) is formed by viscous cases only", RDST_ERR_STATE);
+                const double* v[9];
+                for (int q = 0; q < 9; ++q) v[q] = f.gradU + (size_t)q * n;
+                download_soa(v, 9, n, out);
+                break;
+            }
             case RDST_F_NU_TILDA: case RDST_F_WALL_DIST: case RDST_F_BNU_TILDA: case RDST_F_SA_DIAG:
             case RDST_F_SA_UPPER: case RDST_F_SA_LOWER: case RDST_F_SA_SOURCE: {
                 const SaState& s = c->sa;

@@ -65,16 +65,31 @@ __device__ inline void tau_mc(double mu, const double* g, double* t) {
         for (int j = 0; j < 3; ++j) t[3 * i + j] = mu * (g[3 * j + i] - (i == j ? (2.0 / 3.0) * tr : 0.0));
 }
 
-// K2: fvc::grad(U), Gauss linear, one thread per cell
-__global__ __launch_bounds__(256) void k_grad_u(MeshDev m, Fields F) {
+// cellLimitedGrad's limitFace for one component: e is the change from the
+// cell value to a face centre along the cell's gradient, [dmin, dmax] the
+// admissible change; 1e-300 is VSMALL
+__device__ inline void limit_face(double& l, double dmax, double dmin, double e) {
+    if (e > dmax + 1.0e-300) l = fmin(l, dmax / e);
+    else if (e < dmin - 1.0e-300) l = fmin(l, dmin / e);
+}
+
+// K2: fvc::grad(U), Gauss linear, one thread per cell.  LIM (cellLimited
+// Gauss linear k): the Gauss linear gradient is kept in gradUc and, in the
+// same pass over the cell's faces, the min/max of U over the face neighbours
+// and boundary values is formed; a second pass over the faces gives the
+// per-component limiter from the face centres, and gradU receives the
+// limited gradient (column j scaled by l_j).
+template <bool LIM>
+__global__ __launch_bounds__(256) void k_grad_u(MeshDev m, Fields F, double k) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= m.nC) return;
     const size_t E = (size_t)m.ellW * m.nC;
     const double ux = F.Ux[c], uy = F.Uy[c], uz = F.Uz[c];
     double g[9];
     for (int q = 0; q < 9; ++q) g[q] = 0.0;
-    for (int k = 0; k < m.ellW; ++k) {
-        const size_t e = (size_t)k * m.nC + c;
+    double mx[3] = {ux, uy, uz}, mn[3] = {ux, uy, uz};
+    for (int kk = 0; kk < m.ellW; ++kk) {
+        const size_t e = (size_t)kk * m.nC + c;
         const uint32_t code = m.ellCode[e];
         const uint32_t kind = code >> 30;
         if (kind == ELL_NONE) break;
@@ -84,6 +99,8 @@ __global__ __launch_bounds__(256) void k_grad_u(MeshDev m, Fields F) {
         double sgn = 1.0;
         if (kind == ELL_BOUNDARY) {
             f[0] = F.bUx[idx]; f[1] = F.bUy[idx]; f[2] = F.bUz[idx];
+            if (LIM)
+                for (int j = 0; j < 3; ++j) { mx[j] = fmax(mx[j], f[j]); mn[j] = fmin(mn[j], f[j]); }
         } else {
             const double w = m.ellWt[e];
             const double ox = F.Ux[idx], oy = F.Uy[idx], oz = F.Uz[idx];
@@ -92,6 +109,10 @@ __global__ __launch_bounds__(256) void k_grad_u(MeshDev m, Fields F) {
             const double Nx = own ? ox : ux, Ny = own ? oy : uy, Nz = own ? oz : uz;
             f[0] = w * (Px - Nx) + Nx; f[1] = w * (Py - Ny) + Ny; f[2] = w * (Pz - Nz) + Nz;
             if (!own) sgn = -1.0;
+            if (LIM) {
+                const double o[3] = {ox, oy, oz};
+                for (int j = 0; j < 3; ++j) { mx[j] = fmax(mx[j], o[j]); mn[j] = fmin(mn[j], o[j]); }
+            }
         }
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) {
@@ -100,7 +121,34 @@ __global__ __launch_bounds__(256) void k_grad_u(MeshDev m, Fields F) {
             }
     }
     const double V = m.V[c];
-    for (int q = 0; q < 9; ++q) F.gradU[(size_t)q * m.nC + c] = g[q] / V;
+    for (int q = 0; q < 9; ++q) g[q] = g[q] / V;
+    if (!LIM) {
+        for (int q = 0; q < 9; ++q) F.gradU[(size_t)q * m.nC + c] = g[q];
+        return;
+    }
+    for (int q = 0; q < 9; ++q) F.gradUc[(size_t)q * m.nC + c] = g[q];
+    const double u[3] = {ux, uy, uz};
+    for (int j = 0; j < 3; ++j) { mx[j] = mx[j] - u[j]; mn[j] = mn[j] - u[j]; }
+    if (k < 1.0) {
+        const double wd = 1.0 / k - 1.0;
+        for (int j = 0; j < 3; ++j) {
+            const double d = wd * (mx[j] - mn[j]);
+            mx[j] = mx[j] + d; mn[j] = mn[j] - d;
+        }
+    }
+    double l[3] = {1.0, 1.0, 1.0};
+    const size_t nF = m.nF;
+    const double cx = m.C[c], cy = m.C[m.nC + c], cz = m.C[2 * (size_t)m.nC + c];
+    for (int kk = 0; kk < m.ellW; ++kk) {
+        const size_t e = (size_t)kk * m.nC + c;
+        const uint32_t kind = m.ellCode[e] >> 30;
+        if (kind == ELL_NONE) break;
+        const int fi = m.ellFace[e];
+        const double dx = m.Cf[fi] - cx, dy = m.Cf[nF + fi] - cy, dz = m.Cf[2 * nF + fi] - cz;
+        for (int j = 0; j < 3; ++j) limit_face(l[j], mx[j], mn[j], dx * g[j] + dy * g[3 + j] + dz * g[6 + j]);
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) F.gradU[(size_t)(3 * i + j) * m.nC + c] = l[j] * g[3 * i + j];
 }
 
 // gaussGrad::correctBoundaryConditions: g_b = g_I + n (snGrad_b - n.g_I)
@@ -277,7 +325,9 @@ __device__ inline void recon_sides(const MeshDev& m, const Fields& F, int P, int
 }
 
 // K1: face flux + residual, one thread per cell.
-template <int S, int STORE, bool VISC, bool RECON>
+// VISC: 0 inviscid, 1 viscous, 2 viscous with a limited grad(U) (tauMC from
+// gradU, the snGrad correction from the unlimited gradUc)
+template <int S, int STORE, int VISC, bool RECON>
 __global__ __launch_bounds__(256) void k_flux_residual(MeshDev m, Fields F, const BcPatchDev* bc, double gamma, int lowMach,
                                                        double Mainf, double lapPsi, double snPsi, int heat) {
     const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -367,9 +417,18 @@ __global__ __launch_bounds__(256) void k_flux_residual(MeshDev m, Fields F, cons
                 const double UNx = own ? oUx : uCx, UNy = own ? oUy : uCy, UNz = own ? oUz : uCz;
                 const double muf = w * (muP - muN) + muN;
                 double gf[9], tf[9];
-                for (int q = 0; q < 9; ++q) {
-                    gf[q] = w * (gP[q] - gN[q]) + gN[q];
-                    tf[q] = w * (tP[q] - tN[q]) + tN[q];
+                if constexpr (VISC == 2) {
+                    const size_t iP = own ? c : idx, iN = own ? idx : c;
+                    for (int q = 0; q < 9; ++q) {
+                        const double a = F.gradUc[(size_t)q * m.nC + iP], b = F.gradUc[(size_t)q * m.nC + iN];
+                        gf[q] = w * (a - b) + b;
+                        tf[q] = w * (tP[q] - tN[q]) + tN[q];
+                    }
+                } else {
+                    for (int q = 0; q < 9; ++q) {
+                        gf[q] = w * (gP[q] - gN[q]) + gN[q];
+                        tf[q] = w * (tP[q] - tN[q]) + tN[q];
+                    }
                 }
                 const double cx = m.ellCorr[e], cy = m.ellCorr[E + e], cz = m.ellCorr[2 * E + e];
                 const double dcf = m.ellDc[e];
@@ -511,11 +570,14 @@ void launch_flux_mode(Case& c) {
     const BcPatchDev* bc = c.d_bc.p;
     const double lapPsi = sngrad_psi(c.st.laplacian_sngrad, c.st.laplacian_limit);
     const double snPsi = sngrad_psi(c.st.sngrad, c.st.sngrad_limit);
-    if (c.st.viscous)
-        hipLaunchKernelGGL((k_flux_residual<S, STORE, true, RECON>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, bc,
+    if (c.st.viscous && grad_u_limited(c.st))
+        hipLaunchKernelGGL((k_flux_residual<S, STORE, 2, RECON>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, bc,
+                           c.th.gamma, c.st.low_mach, c.st.Mainf, lapPsi, snPsi, c.st.heat_conduction);
+    else if (c.st.viscous)
+        hipLaunchKernelGGL((k_flux_residual<S, STORE, 1, RECON>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, bc,
                            c.th.gamma, c.st.low_mach, c.st.Mainf, lapPsi, snPsi, c.st.heat_conduction);
     else
-        hipLaunchKernelGGL((k_flux_residual<S, STORE, false, RECON>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, bc,
+        hipLaunchKernelGGL((k_flux_residual<S, STORE, 0, RECON>), dim3(nb), dim3(bs), 0, c.stream, md, c.f, bc,
                            c.th.gamma, c.st.low_mach, c.st.Mainf, lapPsi, snPsi, 0);
 }
 
@@ -547,7 +609,11 @@ void launch_flux(Case& c, int store) {
 void grad_u_launch(Case& c) {
     MeshDev md = c.meshDev();
     const int bs = 256;
-    hipLaunchKernelGGL(k_grad_u, dim3(blocks_for(md.nC, bs)), dim3(bs), 0, c.stream, md, c.f);
+    if (grad_u_limited(c.st))
+        hipLaunchKernelGGL(k_grad_u<true>, dim3(blocks_for(md.nC, bs)), dim3(bs), 0, c.stream, md, c.f,
+                           c.st.grad_u_limit);
+    else
+        hipLaunchKernelGGL(k_grad_u<false>, dim3(blocks_for(md.nC, bs)), dim3(bs), 0, c.stream, md, c.f, 1.0);
     if (md.nB) hipLaunchKernelGGL(k_grad_u_bnd, dim3(blocks_for(md.nB, bs)), dim3(bs), 0, c.stream, md, c.f, c.d_bc.p);
 }
 

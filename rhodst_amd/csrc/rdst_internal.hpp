@@ -237,6 +237,10 @@ struct Fields {
     double *refUx, *refUy, *refUz;  // mixed U BCs: refValue
     // viscous terms: grad(U) [9][nC] (T[3i+j] = dU_j/dx_i), corrected boundary gradient [9][nB]
     double *gradU, *bgradU;
+    // the unlimited Gauss linear grad(U) [9][nC] that the non-orthogonal
+    // correction of snGrad(U) interpolates: gradU itself unless grad(U) is
+    // cellLimited
+    double* gradUc;
     // surface fields (internal then boundary)
     double *phi, *phiUx, *phiUy, *phiUz, *phiE, *Upx, *Upy, *Upz;
     // residual R_ [5][nC]
@@ -286,6 +290,12 @@ struct SaState {
     bool rhoPrevValid = false;
     int nCorrect = 0;
 };
+
+// cellLimited grad(U) in effect (k below 1e-15 leaves the gradient unlimited,
+// as cellLimitedGrad::calcGrad returns the Gauss linear one then)
+inline bool grad_u_limited(const rdst_settings& s) {
+    return s.grad_u == RDST_GRAD_CELL_LIMITED && s.grad_u_limit >= 1.0e-15;
+}
 
 struct MeshDev {
     int nC, nI, nB, nF, ellW;
