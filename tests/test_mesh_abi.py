@@ -145,6 +145,21 @@ def test_unknown_flux_scheme_lists_names():
         assert name in msg
 
 
+@pytest.mark.parametrize("grad_u,limit,rc,text", [(7, 1.0, "RDST_ERR_UNKNOWN_NAME", "cellLimited Gauss linear"),
+                                                    (1, 1.5, "RDST_ERR_ARG", "k must be >= 0 and <= 1"),
+                                                    (1, -0.1, "RDST_ERR_ARG", "k must be >= 0 and <= 1")])
+def test_grad_scheme_validated(grad_u, limit, rc, text):
+    """gradSchemes grad(U): an unknown scheme or a cellLimited k outside
+    [0, 1] is refused before anything reaches the device."""
+    spec = cases.forward_step_case(nx=15, ny=5)
+    spec.settings.grad_u, spec.settings.grad_u_limit = grad_u, limit
+    dm = _capi.DeviceMesh(spec.mesh)
+    h = C.c_void_p()
+    got = _capi.lib().rdst_case_create(dm.h, C.byref(spec.thermo), spec.bcs, C.byref(spec.settings), C.byref(h))
+    assert got == getattr(_capi, rc)
+    assert text in _capi.lib().rdst_last_error().decode()
+
+
 def test_bad_mesh_rejected():
     m = meshgen.box_2d(3, 3)
     md = MeshDesc(m)
