@@ -451,7 +451,7 @@ def main(argv=None) -> int:
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print("bench.py measures one GPU: every path here runs on a single MI355X; "
-              "multi-GPU runs are not part of this build", file=sys.stderr)
+              "the sharded multi-GPU path is not implemented", file=sys.stderr)
         return 2
 
     if args.workloads:

@@ -1,8 +1,8 @@
 """Writes tests/golden/rae2822_polymesh.npz: the RAE2822 tutorial's polyMesh
 (tutorials/RAE2822/constant/polyMesh of the reference, a data file of its
-tutorial) as arrays, so the GPU box, which has no reference checkout, can run
-the case.  Read with the package's own ASCII polyMesh reader; run in the
-build container:
+tutorial) as arrays, so that the tests run the case where no reference
+checkout is present.  Read with the package's own ASCII polyMesh reader; run
+where the reference tree is:
 
     python tests/golden/make_rae2822_mesh.py /root/reference/tutorials/RAE2822
 """
