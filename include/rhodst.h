@@ -196,7 +196,7 @@ typedef struct rdst_settings {
      * (a per-cell damping for the exact-Jacobian operator at large CFL). */
     double update_limit;
     /* fvSchemes gradSchemes grad(U) (rdst_grad_kind; tutorials/RAE2822/
-     * system/fvSchemes:26 selects "cellLimited Gauss linear 1"): the gradient
+     * system/fvSchemes:27 selects "cellLimited Gauss linear 1"): the gradient
      * of U that tauMC (baseFluxDST.C:233-236), the Spalart-Allmaras strain and
      * vorticity (SAneg.C:524, SARC2.C:458) and the force coefficients read.
      * grad_u_limit is the k of cellLimited, in [0, 1]; k below 1e-15 leaves

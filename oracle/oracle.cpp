@@ -940,7 +940,7 @@ static V3 snGradU_b(const Case& c, int b) {
 }
 
 // cellLimitedGrad (foam-extend 4.1 gradSchemes, not part of the reference
-// tree; tutorials/RAE2822/system/fvSchemes:26 selects "cellLimited Gauss
+// tree; tutorials/RAE2822/system/fvSchemes:27 selects "cellLimited Gauss
 // linear 1" for grad(U)), restated from its published algorithm: the max and
 // min of each U component over a cell, its face neighbours (a face loop,
 // owner and neighbour sides) and its boundary-face values, less the cell

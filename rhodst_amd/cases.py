@@ -202,7 +202,7 @@ def rae2822_case(scheme: str = "Kurganov", turbulence: str | None = "SpalartAllm
     and relTol 1e-10; fieldBounds rho 0.01 10, p 100 1e6, T 10 800) and the
     SpalartAllmaras model with 0/nuTilda (4.4e-5 inside and at the far field,
     0 on the wall), its div(phi,nuTilda) Gauss linearUpwind grad(nuTilda)
-    and grad(U) cellLimited Gauss linear 1 (fvSchemes:26).  The mesh is read
+    and grad(U) cellLimited Gauss linear 1 (fvSchemes:27).  The mesh is read
     from the committed array copy of the tutorial's polyMesh."""
     mesh = mesh if mesh is not None else meshgen.polymesh_from_npz(RAE2822_MESH)
     Uinf, pinf, Tinf = (233.44, 9.417, 0.0), 108987.773, 255.556

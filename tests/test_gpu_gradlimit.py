@@ -1,5 +1,5 @@
 """GPU parity of grad(U) cellLimited Gauss linear (foam-extend 4.1
-cellLimitedGrad; tutorials/RAE2822/system/fvSchemes:26) against the oracle:
+cellLimitedGrad; tutorials/RAE2822/system/fvSchemes:27) against the oracle:
 the limited gradient itself and the residual that reads it (tauMC from the
 limited gradient, the snGrad correction from the Gauss linear one) on
 tetrahedra over prism layers, twelve Spalart-Allmaras steps (strain and

@@ -1,5 +1,5 @@
 """Pins of the oracle's cellLimited grad(U) (foam-extend 4.1 cellLimitedGrad,
-selected by tutorials/RAE2822/system/fvSchemes:26 "cellLimited Gauss linear
+selected by tutorials/RAE2822/system/fvSchemes:27 "cellLimited Gauss linear
 1").
 
 * Closed forms on a uniform 8 x 4 grid with zeroGradient boundaries and U
