@@ -334,8 +334,9 @@ enum rdst_field {
     RDST_F_SA_UPPER = 36,         /* [n_internal] (owner row, neighbour column) */
     RDST_F_SA_LOWER = 37,         /* [n_internal] (neighbour row, owner column) */
     RDST_F_SA_SOURCE = 38,        /* [n_cells] incl. boundary coefficients and relaxation */
-    RDST_F_GRAD_U = 39            /* cells [n][9]: fvc::grad(U) of the last residual evaluation of a viscous
-                                     case (settings.grad_u applied), dU_j/dx_i at 3 i + j */
+    RDST_F_GRAD_U = 39            /* cells [n][9]: fvc::grad(U) of a viscous case as last formed (by the
+                                     residual evaluation or the turbulence model's correct step), with
+                                     settings.grad_u applied; dU_j/dx_i at 3 i + j */
 };
 
 /* integer arrays of a mesh */
